@@ -1,0 +1,95 @@
+/*
+ * irlprj.h -- projector-level C entry points of libirl_amd.
+ *
+ * These are the reference's internal projector/backprojector functions,
+ * exported with host pointers so that a caller (or a parity test) can drive
+ * one view at a time exactly as vOsemRecon does.  Each call copies its inputs
+ * to the GPU, runs the gfx950 kernels and copies the result back.
+ *
+ *   Drf_t, DrfTab_t        replace drftabio.h:7-28 (same layout)
+ *   Srf_t                  replaces irlprivate.h:116-120 (same layout)
+ *   InitPrj                replaces irlprivate.h:461-466 (prjbck.c:42-66)
+ *   Project                replaces irlprivate.h:468-489 (prjbck.c:68-252)
+ *   BackProject            replaces irlprivate.h:491-511 (prjbck.c:254-427)
+ *   FreePrj                replaces irlprivate.h:517     (prjbck.c:432-453)
+ *   rot3dpar               replaces irlprivate.h:521-529 (rot3dpar.c:19-198)
+ *   ConvolveDrfYpad        replaces irlprivate.h:233-242 (convolve_ypad.c:22-96)
+ *   sReadDrfTab            replaces drftabio.h:37-41     (drftabio.c:150-275)
+ *   vWriteDrfTab           replaces drftabio.h:30-35     (drftabio.c:57-148)
+ *
+ * Workspace semantics follow prjbck.c: after Project the rotated estimate
+ * and rotated attenuation map stay in the library's workspace, so a later
+ * Project(..., bActImgRotated=1) or BackProject(..., bAtnMapRotated=1) for
+ * the same angle reuses them, and BackProject leaves its rotated planes in
+ * the rotated-estimate workspace.  A DrfTab_t passed to Project/BackProject
+ * is copied to the GPU on first use and cached until FreePrj/InitPrj; it
+ * must not be modified in between.  bFastRotate is accepted and ignored:
+ * rotation is always the bilinear rot3dpar rotation.
+ */
+#ifndef IRLPRJ_H
+#define IRLPRJ_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int MaxOffsX;  /* PSF is (2*MaxOffsX-1) x (2*MaxOffsY-1), x fastest */
+    int MaxOffsY;
+    int bFFTdone;
+    float *Psf;
+} Drf_t;
+
+typedef struct DrfTab_s {
+    int iNumDistances;
+    float fStartDistance;
+    float fDrfSpacing;
+    float fPixSize;
+    int bIncKrnl;
+    int bDrfFromFile;
+    struct DrfTab_s *psBckDrfTab;
+    float fGap, fHoleDiam, fColThick, fIntrinsicFWHM, fMaxFracErr;
+    Drf_t *psDrfs;
+    int bFFTconvolve;
+} DrfTab_t;
+
+typedef struct {
+    int iSrfCollapseFac;
+    int iSrfParmLineNum;
+    void *pvScatterData;
+} Srf_t;
+
+void InitPrj(int iNumSlices, int iNumPixels, int iNumRotPixs, int iModel);
+
+void Project(int iNumSlices, int iNumPixels, int iNumRotPixs, int iPrjModel,
+             float fPixelWidth, int iAngle, float fAngle, float fCFCR,
+             DrfTab_t *psDrfTab, Srf_t *psSrf, float *atnmatrix, float *reconmatrix,
+             float *calcprj, float fPrimaryFac, float fAtnFac, int iAxialPadLength,
+             int iAxialAvgLength, int bAtnMapRotated, int bActImgRotated,
+             int bFastRotate);
+
+void BackProject(int iNumSlices, int iNumPixels, int iNumRotPixs, int iBckModel,
+                 float fPixelWidth, int iAngle, float fAngle, float fCFCR,
+                 DrfTab_t *psDrfTab, Srf_t *psSrf, float *atnmatrix, float *ratioprj,
+                 float *imagematrix, float fPrimaryFac, float fAtnFac,
+                 int iAxialPadLength, int iAxialAvgLength, int bAtnMapRotated,
+                 int bFastRotate);
+
+void FreePrj(void);
+
+void rot3dpar(float fAngle, float *OldMatrix, float *NewMatrix, int PB_FLAG,
+              int NumSlices, int NumPixels, int NumRotPixs);
+
+void ConvolveDrfYpad(float *pfIn, int inX, int inY, Drf_t *psDrf, int bFFTConvolve,
+                     int iYpad, int iYavg, float *pfOut);
+
+DrfTab_t sReadDrfTab(char *pchInFname, char **ppchInTitle, char **ppchInColName);
+
+void vWriteDrfTab(DrfTab_t *psOutDrfTab, char *pchOutFname, char *pchTitle,
+                  char *pchColName);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IRLPRJ_H */

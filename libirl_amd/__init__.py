@@ -1,0 +1,257 @@
+"""libirl_amd -- MI355X-native OSEM projector/backprojector with libirl's C API.
+
+This module is the Python mirror of the C interface in include/: it loads the
+in-tree libirl_amd.so (built for gfx950 by libirl_amd/build.py) and exposes
+IrlOsem, the hook/option/detector calls and the device-resident
+reconstruction API with numpy arrays.  Arrays are passed by pointer, so the
+in-place side effects of IrlOsem on the caller's arrays (reordered and scaled
+attenuation map, clamped projections, result in pfReconImage) are visible
+exactly as for a C caller.
+
+There is no fallback: if the shared library is missing, importing the
+bindings raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libirl_amd.so")
+
+MODEL_ATN = 1
+MODEL_DRF = 2
+MODEL_SRF = 4
+
+
+class IrlParms(ctypes.Structure):
+    """IrlParms_t (irl.h:39-79)."""
+    _fields_ = [
+        ("NumPixels", ctypes.c_int),
+        ("NumSlices", ctypes.c_int),
+        ("NumViews", ctypes.c_int),
+        ("BinWidth", ctypes.c_float),
+        ("fAtnScaleFac", ctypes.c_float),
+        ("iNumSrfIterations", ctypes.c_int),
+        ("fScatEstFac", ctypes.c_float),
+        ("NumAngPerSubset", ctypes.c_int),
+        ("NumIterations", ctypes.c_int),
+        ("pchNormImageBase", ctypes.c_char_p),
+        ("SrfCollapseFac", ctypes.c_int),
+        ("fHoleDiam", ctypes.c_float),
+        ("fHoleLen", ctypes.c_float),
+        ("fBackToDet", ctypes.c_float),
+        ("fIntrinsicFWHM", ctypes.c_float),
+    ]
+
+
+class Options(ctypes.Structure):
+    """Options_t (irl.h:81-124)."""
+    _fields_ = [
+        ("bModelDrf", ctypes.c_int),
+        ("bReconIsInitEst", ctypes.c_int),
+        ("bUseGrfInBck", ctypes.c_int),
+        ("bFFTConvolve", ctypes.c_int),
+        ("bUseContourSupport", ctypes.c_int),
+        ("fAtnMapThresh", ctypes.c_float),
+        ("iMsgLevel", ctypes.c_int),
+        ("iAxialPadLength", ctypes.c_int),
+        ("iAxialAvgLength", ctypes.c_int),
+    ]
+
+
+class PrjView(ctypes.Structure):
+    """PrjView_t (irl.h:127-143)."""
+    _fields_ = [
+        ("Angle", ctypes.c_float),
+        ("Left", ctypes.c_float),
+        ("Right", ctypes.c_float),
+        ("CFCR", ctypes.c_float),
+    ]
+
+
+class OsemData(ctypes.Structure):
+    """sOsemData (osemhooks.h:18-27)."""
+    _fields_ = [
+        ("iIterationIdx", ctypes.c_int),
+        ("iSubsetIdx", ctypes.c_int),
+        ("iAngleIdx", ctypes.c_int),
+        ("pfEst", ctypes.POINTER(ctypes.c_float)),
+        ("pfSct", ctypes.POINTER(ctypes.c_float)),
+        ("pfNorm", ctypes.POINTER(ctypes.c_float)),
+        ("pfUpdate", ctypes.POINTER(ctypes.c_float)),
+    ]
+
+
+class AdvOptions(ctypes.Structure):
+    """sAdvOptions (osemhooks.h:29-33)."""
+    _fields_ = [("bFastRotate", ctypes.c_int), ("iStartIteration", ctypes.c_int)]
+
+
+ITER_CB = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.POINTER(ctypes.c_float))
+HOOK_CB = ctypes.CFUNCTYPE(None, ctypes.POINTER(OsemData), ctypes.c_void_p)
+_FP = ctypes.POINTER(ctypes.c_float)
+
+_lib = None
+
+
+def load():
+    """Loads libirl_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libirl_amd.so is not built (run libirl_amd/build.py): " + LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    lib.IrlOsem.restype = ctypes.c_int
+    lib.IrlOsem.argtypes = [ctypes.POINTER(IrlParms), ctypes.POINTER(Options),
+                            ctypes.POINTER(PrjView), ctypes.c_char_p, ctypes.c_char_p,
+                            ITER_CB, _FP, _FP, _FP, _FP, ctypes.c_char_p, ctypes.c_char_p]
+    lib.pchIrlErrorString.restype = ctypes.c_char_p
+    lib.OsemSetAdvOptions.argtypes = [ctypes.POINTER(AdvOptions)]
+    for name in ("PrevIter", "PostIter", "PrevSubset", "PostSubset", "PrevAngle", "PostAngle"):
+        f = getattr(lib, "OsemSet%sHook" % name)
+        f.argtypes = [HOOK_CB, ctypes.c_void_p]
+        f.restype = ctypes.c_int
+    lib.iSetupExtraPrjViewData.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    lib.iSetupDetectors.argtypes = [ctypes.c_int, ctypes.c_void_p]
+    lib.pvStoreDetectorData.restype = ctypes.c_void_p
+    lib.pvStoreDetectorData.argtypes = [ctypes.c_float] * 5 + [ctypes.c_int, ctypes.c_int,
+                                                               ctypes.c_void_p]
+    lib.irlamd_recon_create.restype = ctypes.c_void_p
+    lib.irlamd_recon_create.argtypes = [ctypes.POINTER(IrlParms), ctypes.POINTER(Options),
+                                        ctypes.POINTER(PrjView), ctypes.c_char_p,
+                                        ctypes.c_char_p, _FP, _FP, _FP, _FP]
+    lib.irlamd_recon_subsets.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
+                                         ctypes.POINTER(ctypes.c_int)]
+    lib.irlamd_recon_subiteration.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.irlamd_recon_synchronize.argtypes = [ctypes.c_void_p]
+    lib.irlamd_recon_get_estimate.argtypes = [ctypes.c_void_p, _FP]
+    lib.irlamd_recon_get_norm.argtypes = [ctypes.c_void_p, ctypes.c_int, _FP]
+    lib.irlamd_recon_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.irlamd_recon_reset_timing.argtypes = [ctypes.c_void_p]
+    lib.irlamd_recon_stage_count.argtypes = [ctypes.c_void_p]
+    lib.irlamd_recon_stage.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p,
+                                       ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(ctypes.c_int)]
+    lib.irlamd_recon_destroy.argtypes = [ctypes.c_void_p]
+    _lib = lib
+    return lib
+
+
+def fptr(a):
+    """float* of a C-contiguous float32 array, or NULL for None."""
+    if a is None:
+        return None
+    if not (isinstance(a, np.ndarray) and a.dtype == np.float32 and a.flags.c_contiguous):
+        raise TypeError("expected a C-contiguous float32 array")
+    return a.ctypes.data_as(_FP)
+
+
+def error_string():
+    return load().pchIrlErrorString().decode(errors="replace")
+
+
+def _bytes(s):
+    return None if s is None else os.fsencode(s)
+
+
+def irl_osem(parms, options, views, drf_file, krnl_file, scat, atn, prj, recon,
+             iter_cb=None, log_file=None, msg_file=None):
+    """IrlOsem (irl.h:145-213).  numpy arrays are modified in place exactly as
+    the C function modifies the caller's arrays.  iter_cb(it, est) receives a
+    copy of the estimate in external order.  Returns the C return code."""
+    lib = load()
+    keep = []
+
+    def _cb(it, p):
+        if iter_cb is not None:
+            n = parms.NumPixels * parms.NumPixels * parms.NumSlices
+            iter_cb(it, np.ctypeslib.as_array(p, shape=(n,)).copy())
+    cb = ITER_CB(_cb)
+    keep.append(cb)
+    vv = (PrjView * len(views))(*views)
+    return lib.IrlOsem(ctypes.byref(parms), ctypes.byref(options), vv, _bytes(drf_file),
+                       _bytes(krnl_file), cb, fptr(scat), fptr(atn), fptr(prj), fptr(recon),
+                       _bytes(log_file), _bytes(msg_file))
+
+
+def set_adv_options(fast_rotate=1, start_iteration=0):
+    o = AdvOptions(fast_rotate, start_iteration)
+    return load().OsemSetAdvOptions(ctypes.byref(o))
+
+
+class Recon:
+    """Device-resident reconstruction (include/irl_amd.h)."""
+
+    def __init__(self, parms, options, views, drf_file=None, krnl_file=None, scat=None,
+                 atn=None, prj=None, recon=None):
+        lib = load()
+        vv = (PrjView * len(views))(*views)
+        self._lib = lib
+        self.parms = parms
+        h = lib.irlamd_recon_create(ctypes.byref(parms), ctypes.byref(options), vv,
+                                    _bytes(drf_file), _bytes(krnl_file), fptr(scat), fptr(atn),
+                                    fptr(prj), fptr(recon))
+        if not h:
+            raise RuntimeError("irlamd_recon_create failed: " + error_string())
+        self._h = h
+        s, vs = ctypes.c_int(), ctypes.c_int()
+        lib.irlamd_recon_subsets(h, ctypes.byref(s), ctypes.byref(vs))
+        self.num_subsets, self.views_per_subset = s.value, vs.value
+
+    def _check(self, rc):
+        if rc != 0:
+            raise RuntimeError(error_string())
+
+    def subiteration(self, s):
+        self._check(self._lib.irlamd_recon_subiteration(self._h, s))
+
+    def iteration(self):
+        for s in range(self.num_subsets):
+            self.subiteration(s)
+
+    def synchronize(self):
+        self._check(self._lib.irlamd_recon_synchronize(self._h))
+
+    def estimate(self):
+        p = self.parms
+        out = np.empty((p.NumSlices, p.NumPixels, p.NumPixels), np.float32)
+        self._check(self._lib.irlamd_recon_get_estimate(self._h, fptr(out)))
+        return out
+
+    def norm(self, s):
+        p = self.parms
+        out = np.empty((p.NumPixels, p.NumSlices, p.NumPixels), np.float32)
+        self._check(self._lib.irlamd_recon_get_norm(self._h, s, fptr(out)))
+        return out
+
+    def set_timing(self, on=True):
+        self._lib.irlamd_recon_set_timing(self._h, 1 if on else 0)
+
+    def reset_timing(self):
+        self._check(self._lib.irlamd_recon_reset_timing(self._h))
+
+    def stage_times(self):
+        """{kernel: (total_ms, launches)} from HIP events on the recon's stream."""
+        n = self._lib.irlamd_recon_stage_count(self._h)
+        out = {}
+        for i in range(n):
+            name = ctypes.create_string_buffer(64)
+            ms = ctypes.c_double()
+            cnt = ctypes.c_int()
+            self._check(self._lib.irlamd_recon_stage(self._h, i, name, 64, ctypes.byref(ms),
+                                                     ctypes.byref(cnt)))
+            out[name.value.decode()] = (ms.value, cnt.value)
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.irlamd_recon_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

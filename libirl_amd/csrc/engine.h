@@ -1,0 +1,129 @@
+// engine.h -- device engine of libirl_amd (host-side interface, no HIP types).
+//
+// The engine owns every device buffer of one reconstruction and launches the
+// hand-written gfx950 kernels of engine.hip.  Host code (api.cpp) builds the
+// per-view tables in the reference's float arithmetic and hands them over;
+// the engine never computes geometry decisions itself.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace irlamd {
+
+// Error raised by the engine and the host layers; mapped to the C API's
+// nonzero return code plus pchIrlErrorString() at the boundary.
+struct IrlError {
+    int code;
+    std::string msg;
+};
+
+enum ErrType {
+    ETYPE_ILLEGAL_VALUE = 2,
+    ETYPE_IO = 3,
+    ETYPE_MALLOC = 4,
+    ETYPE_SYNTAX = 5,
+    ETYPE_UNKNOWN = 6,
+    ETYPE_DEVICE = 7,
+};
+
+[[noreturn]] void fail(int code, const char *fmt, ...);
+
+// DRF table padded to one common (KY x KX) support, centred.
+struct PaddedDrf {
+    int nDist = 0;
+    int KX = 1, KY = 1;           // padded full sizes (odd)
+    std::vector<float> psf;       // nDist * KY * KX, x fastest
+    std::vector<int> cy;          // true half-size in y per distance
+};
+
+// Per-view constants, all computed on the host exactly as the reference.
+struct ViewTab {
+    float fsin, fcos;             // forward rotation (0.5*pi + Angle)
+    float bsin, bcos;             // back rotation (-Angle - 0.5*pi)
+    float viewtime, sensitivity;
+};
+
+struct Geometry {
+    int N = 0;      // NumPixels
+    int Nz = 0;     // NumSlices
+    int NRot = 0;   // NumRotPixs (== N through IrlOsem)
+    int V = 0;      // NumViews
+};
+
+// Precision of the DRF convolutions.
+enum ConvPrecision { CONV_FP32 = 0 };
+
+class Device;
+
+// One device-resident OSEM reconstruction (IrlOsem's state between setup and
+// teardown).
+class Recon {
+public:
+    struct Config {
+        Geometry g;
+        int S = 1, Vs = 1;
+        int prjModel = 0, bckModel = 0;
+        std::vector<int> subsets;          // S*Vs view indices
+        std::vector<ViewTab> views;        // V
+        PaddedDrf drf;                     // empty when DRF not modelled
+        std::vector<int> rowKernel;        // V*NRot: DRF index per row, -1 = none
+        std::vector<int> rowPad;           // V*NRot: axial pad rows per row
+        int axialAvg = 0;
+        std::vector<uint8_t> colMeasOne;   // V*NRot (norm mask)
+        std::vector<uint8_t> colKeep;      // V*NRot (ratio mask)
+        bool hasScatterEstimate = false;
+        bool hasAtn = false;
+    };
+
+    explicit Recon(const Config &cfg);
+    ~Recon();
+
+    // Host -> device uploads (internal [y][z][x] order for volumes,
+    // [view][slice][bin] for projections).
+    void uploadEstimate(const float *x);
+    void uploadAtn(const float *mu);
+    void uploadProjections(const float *prj);
+    void uploadScatterEstimate(const float *scat);
+
+    // Attenuation factors of every view (reused by all sub-iterations) and
+    // the FP64 sensitivity images of every subset.
+    void precompute();
+
+    // One OSEM sub-iteration over subset s (osemrecon.c:687-863 body).
+    void subIteration(int s, bool keepUpdate);
+
+    // Device -> host downloads.
+    void downloadEstimate(float *x) const;
+    void downloadUpdate(float *u) const;
+    void downloadNorm(int s, float *ninv) const;
+    void uploadEstimateFromHost(const float *x) { uploadEstimate(x); }
+    void uploadUpdate(const float *u);
+    void uploadNorm(int s, const float *ninv);
+
+    // Per-kernel timing of the last sub-iterations (HIP events on the
+    // engine's stream).  Enabled by setTiming(true).
+    void setTiming(bool on);
+    struct StageTime {
+        std::string name;
+        double ms_total = 0;
+        int launches = 0;
+    };
+    std::vector<StageTime> stageTimes() const;
+    void resetTimes();
+    void synchronize() const;
+
+    const Config &config() const { return cfg_; }
+
+private:
+    struct Impl;
+    Impl *impl_;
+    Config cfg_;
+};
+
+// Stateless single-call helpers behind the prjbck-level C entry points.
+void dev_rot3dpar(float angle, const float *in_host, float *out_host, int pbFlag,
+                  int Nz, int N, int NRot);
+
+}  // namespace irlamd

@@ -1,0 +1,974 @@
+// engine.hip -- gfx950 kernels and the device-resident OSEM engine.
+//
+// Data layout in HBM (one reconstruction, N = NumPixels = NumRotPixs,
+// Nz = NumSlices, V views, S subsets of Vs views):
+//   x, mu, U, Ninv[s]   [y][z][x] fp32 volumes (internal order of the
+//                       reference, irlosem.c:171-195), M = N*Nz*N floats each
+//   prj, scat           [view][slice][bin] fp32
+//   afac                [V][row][slice][bin] fp32 attenuation factors
+//                       exp(-sum of half-pixel mu) of every view, computed
+//                       once per reconstruction (mu never changes)
+//   att                 [Vs][row][slice][bin] rotated estimate x afac
+//   ppart               [Vs][chunk][slice][bin] partial projections
+//   ratio               [Vs][slice][bin]
+//   bplanes             [Vs][row][slice][bin] convolved backprojection planes
+//
+// Kernels (reference loops they replace):
+//   k_rot3dpar   rot3dpar.c:92-192 (generic, prjbck-level entry point)
+//   k_afac       prjbck.c:146-163, 212-225, 247-250 (rotated mu, running
+//                half-pixel line integral, exp) for every view at once
+//   k_att        prjbck.c:129-140, 219-225 (rotate x, attenuate)
+//   k_drf<PRJ>   prjbck.c:195-251 + conv2d_ypad.c:92-103: per-plane DRF
+//                convolution accumulated into the projection
+//   k_ratio      osemrecon.c:727-820
+//   k_drf<BCK>   prjbck.c:346-393 + conv2d_ypad.c:92-103
+//   k_bck_update prjbck.c:408-426 + osemrecon.c:850-852: gather-form back
+//                rotation, per-view EPS clamp, multiplicative update
+//   k_norm       osemrecon.c:282-320 with FP64 accumulation
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "engine.h"
+
+namespace irlamd {
+
+void fail(int code, const char *fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    throw IrlError{code, buf};
+}
+
+#define HIP_CHECK(expr)                                                                  \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            ::irlamd::fail(ETYPE_DEVICE, "HIP error '%s' in %s (%s:%d)",                 \
+                           hipGetErrorString(e_), #expr, __FILE__, __LINE__);            \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Host geometry helpers (reference float arithmetic)
+// ---------------------------------------------------------------------------
+
+// Inscribed-circle row extents of rot3dpar.c:98-101.
+static void circle_rows(int NewDim, std::vector<int> &head, std::vector<int> &npix)
+{
+    head.resize(NewDim);
+    npix.resize(NewDim);
+    volatile float Radius = 0.5 * NewDim;
+    for (int NewY = 0; NewY < NewDim; ++NewY) {
+        float rr = Radius * Radius;
+        int NumPix = (int)(sqrt(rr - (Radius - NewY - .5) * (Radius - NewY - .5)) + .5);
+        NumPix *= 2;
+        head[NewY] = (int)(0.5 * (NewDim - NumPix));
+        npix[NewY] = NumPix;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Device helpers
+// ---------------------------------------------------------------------------
+
+// Bilinear sample of rot3dpar.c:119-189 at output pixel (NewX, NewY) of one
+// slice.  'plane' points at element [0][slice][0] of a [OldDim][Nz][OldDim]
+// volume, ystride = Nz*OldDim.  Every multiply and add is rounded separately
+// (the library is compiled with -ffp-contract=off), so sample positions and
+// taps are those of the reference bit for bit.
+__device__ __forceinline__ float rot_sample(const float *__restrict__ plane, long ystride,
+                                            int OldDim, float X, float Y, float sn,
+                                            float cs, float OldAxis)
+{
+    const float FloatX = X * cs - Y * sn + OldAxis;
+    const float FloatY = X * sn + Y * cs + OldAxis;
+    const int OldX = (int)FloatX;
+    const int OldY = (int)FloatY;
+    const bool x0 = OldX >= 0 && OldX < OldDim;
+    const bool y0 = OldY >= 0 && OldY < OldDim;
+    const bool x1 = OldX + 1 >= 0 && OldX + 1 < OldDim;
+    const bool y1 = OldY + 1 >= 0 && OldY + 1 < OldDim;
+    const float z0 = (x0 && y0) ? plane[OldY * ystride + OldX] : 0.0f;
+    const float z1 = (x0 && y1) ? plane[(OldY + 1) * ystride + OldX] : 0.0f;
+    const float z2 = (x1 && y0) ? plane[OldY * ystride + OldX + 1] : 0.0f;
+    const float z3 = (x1 && y1) ? plane[(OldY + 1) * ystride + OldX + 1] : 0.0f;
+    const float wy = FloatY - (float)OldY;
+    float z01 = z0 + wy * (z1 - z0);
+    float z23 = z2 + wy * (z3 - z2);
+    if (FloatY < 0) {
+        z01 = z0;
+        z23 = z2;
+    }
+    const float wx = FloatX - (float)OldX;
+    float z0123 = z01 + wx * (z23 - z01);
+    if (FloatX < 0) z0123 = z01;
+    return z0123;
+}
+
+__device__ __forceinline__ bool in_circle(const int *__restrict__ head,
+                                          const int *__restrict__ npix, int NewY, int NewX)
+{
+    const int h = head[NewY];
+    return NewX >= h && NewX <= h + npix[NewY] - 1;
+}
+
+// ---------------------------------------------------------------------------
+// k_rot3dpar: generic rotation (rot3dpar.c:19-198), one thread per output.
+// ---------------------------------------------------------------------------
+__global__ void k_rot3dpar(const float *__restrict__ in, float *__restrict__ out, int Nz,
+                           int OldDim, int NewDim, float sn, float cs, float NewAxis,
+                           float OldAxis, const int *__restrict__ head,
+                           const int *__restrict__ npix)
+{
+    const long total = (long)NewDim * Nz * NewDim;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total;
+         i += (long)gridDim.x * blockDim.x) {
+        const int NewX = (int)(i % NewDim);
+        const long r = i / NewDim;
+        const int slice = (int)(r % Nz);
+        const int NewY = (int)(r / Nz);
+        float v = 0.0f;
+        if (in_circle(head, npix, NewY, NewX)) {
+            const float X = (float)NewX - NewAxis;
+            const float Y = (float)NewY - NewAxis;
+            v = rot_sample(in + (long)slice * OldDim, (long)Nz * OldDim, OldDim, X, Y, sn, cs,
+                           OldAxis);
+        }
+        out[i] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_afac: attenuation factors of every view.  One thread per (view, slice,
+// bin) column walks the rows nearest-detector first: S += mu/2, a = exp(-S),
+// S += mu/2 (prjbck.c:212-225, 247-250), with S rounded to float after each
+// double-precision add exactly as the reference's float table.
+// ---------------------------------------------------------------------------
+__global__ void k_afac(const float *__restrict__ mu, float *__restrict__ afac, int V, int Nz,
+                       int N, int NRot, const float *__restrict__ fsin,
+                       const float *__restrict__ fcos, float atnFac,
+                       const int *__restrict__ head, const int *__restrict__ npix)
+{
+    const long cols = (long)Nz * NRot;
+    const long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (gid >= cols * V) return;
+    const int v = (int)(gid / cols);
+    const int c = (int)(gid % cols);
+    const int z = c / NRot, X = c % NRot;
+    const float NewAxis = 0.5 * (NRot - 1);
+    const float OldAxis = 0.5 * (N - 1);
+    const float sn = fsin[v], cs = fcos[v];
+    const float *plane = mu + (long)z * N;
+    float *o = afac + (long)v * NRot * cols + c;
+    const float Xf = (float)X - NewAxis;
+    float S = 0.0f;
+    for (int Y = 0; Y < NRot; ++Y) {
+        float m = 0.0f;
+        if (in_circle(head, npix, Y, X))
+            m = rot_sample(plane, (long)Nz * N, N, Xf, (float)Y - NewAxis, sn, cs, OldAxis);
+        S = (float)((double)S + 0.5 * (double)m);
+        o[(long)Y * cols] = (float)exp((double)(-S * atnFac));
+        S = (float)((double)S + 0.5 * (double)m);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_att: rotated estimate times attenuation factor for the batch's views.
+// ---------------------------------------------------------------------------
+__global__ void k_att(const float *__restrict__ x, const float *__restrict__ afac,
+                      float *__restrict__ att, int B, int Nz, int N, int NRot,
+                      const int *__restrict__ bview, const float *__restrict__ fsin,
+                      const float *__restrict__ fcos, const int *__restrict__ head,
+                      const int *__restrict__ npix)
+{
+    const long vol = (long)NRot * Nz * NRot;
+    const long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (gid >= vol * B) return;
+    const int b = (int)(gid / vol);
+    const long i = gid % vol;
+    const int X = (int)(i % NRot);
+    const long r = i / NRot;
+    const int z = (int)(r % Nz);
+    const int Y = (int)(r / Nz);
+    const int v = bview[b];
+    float val = 0.0f;
+    if (in_circle(head, npix, Y, X)) {
+        const float NewAxis = 0.5 * (NRot - 1);
+        const float OldAxis = 0.5 * (N - 1);
+        val = rot_sample(x + (long)z * N, (long)Nz * N, N, (float)X - NewAxis,
+                         (float)Y - NewAxis, fsin[v], fcos[v], OldAxis);
+    }
+    if (afac) val = val * afac[(long)v * vol + i];
+    att[gid] = val;
+}
+
+// ---------------------------------------------------------------------------
+// k_drf: per-plane DRF convolution (conv2d_ypad.c:39-108).
+//   PRJ: partial projection of a chunk of rows, sum over rows of
+//        DRF_row (*) att[row]                       (prjbck.c:195-251)
+//   BCK: bplanes[row] = DRF_row (*) (ratio x afac[row]) (prjbck.c:346-393)
+// Block = 256 threads = 64 bins x 4 slice groups; each thread owns ZT
+// consecutive slices of one bin.  The input tile with its halo and the padded
+// PSF are staged in LDS; for each PSF column the thread keeps the column in
+// registers and slides over the input rows, so every LDS value feeds up to
+// ZT FMAs.  Rows whose plane index is -1 (distance <= 0) are added without
+// convolution, as the reference skips ConvolveDrfYpad there.
+// ---------------------------------------------------------------------------
+constexpr int DRF_TX = 64;
+constexpr int DRF_ZG = 4;
+constexpr int DRF_ZT = 16;
+constexpr int DRF_TZ = DRF_ZG * DRF_ZT;
+
+struct DrfArgs {
+    int NRot, Nz;
+    int KX;                       // padded PSF width (odd)
+    const float *psf;             // nDist * KY * KX
+    const int *rowKernel;         // [V][NRot]
+    const int *rowPad;            // [V][NRot]
+    int avg;                      // axial average rows (clamped to [0, Nz])
+    const int *bview;             // batch slot -> view
+    const float *in;              // PRJ: att [B][NRot][Nz][NRot]; BCK: ratio [B][Nz][NRot]
+    const float *afac;            // BCK: [V][NRot][Nz][NRot] or null
+    float *out;                   // PRJ: ppart [B][nchunk][Nz][NRot]; BCK: [B][NRot][Nz][NRot]
+    int rchunk, nchunk;           // PRJ row chunking
+    int xtiles, ztiles;
+};
+
+template <bool BCK>
+__device__ __forceinline__ float drf_in(const DrfArgs &a, int b, int v, int row, int g,
+                                        int gx)
+{
+    const long pl = (long)a.Nz * a.NRot;
+    if (BCK) {
+        float r = a.in[(long)b * pl + (long)g * a.NRot + gx];
+        if (a.afac) r = r * a.afac[((long)v * a.NRot + row) * pl + (long)g * a.NRot + gx];
+        return r;
+    } else {
+        return a.in[((long)b * a.NRot + row) * pl + (long)g * a.NRot + gx];
+    }
+}
+
+// Value of padded-plane row g (global slice index, may be outside [0,Nz)) at
+// bin gx, following pfPad2d/pfAvgRows/pfCopyRows (pad2d.c:21-141).
+template <bool BCK>
+__device__ float drf_stage_val(const DrfArgs &a, int b, int v, int row, int g, int gx,
+                               int pad)
+{
+    if (gx < 0 || gx >= a.NRot) return 0.0f;
+    if (g >= 0 && g < a.Nz) return drf_in<BCK>(a, b, v, row, g, gx);
+    if (pad <= 0 || a.avg <= 0) return 0.0f;
+    if (g < -pad || g >= a.Nz + pad) return 0.0f;
+    const int base = g < 0 ? 0 : a.Nz - a.avg;
+    float s = 0.0f;
+    for (int k = 0; k < a.avg; ++k) s += drf_in<BCK>(a, b, v, row, base + k, gx);
+    if (a.avg > 1) s /= (float)a.avg;
+    return s;
+}
+
+template <int KY, bool BCK>
+__global__ __launch_bounds__(256) void k_drf(DrfArgs a)
+{
+    extern __shared__ float lds[];
+    constexpr int CY = (KY - 1) / 2;
+    const int KX = a.KX;
+    const int CX = (KX - 1) / 2;
+    const int TW = DRF_TX + KX - 1;
+    constexpr int TH = DRF_TZ + KY - 1;
+    float *tile = lds;
+    float *psf = lds + TH * TW;
+
+    int bid = blockIdx.x;
+    const int xt = bid % a.xtiles;
+    bid /= a.xtiles;
+    const int zt = bid % a.ztiles;
+    bid /= a.ztiles;
+    int b, r0, r1;
+    if (BCK) {
+        b = bid / a.NRot;
+        r0 = bid % a.NRot;
+        r1 = r0 + 1;
+    } else {
+        b = bid / a.nchunk;
+        const int ch = bid % a.nchunk;
+        r0 = ch * a.rchunk;
+        r1 = min(r0 + a.rchunk, a.NRot);
+    }
+    const int v = a.bview[b];
+    const int x0 = xt * DRF_TX, z0 = zt * DRF_TZ;
+    const int tx = threadIdx.x % DRF_TX;
+    const int zl0 = (threadIdx.x / DRF_TX) * DRF_ZT;
+    const int x = x0 + tx;
+    const long pl = (long)a.Nz * a.NRot;
+
+    float acc[DRF_ZT];
+#pragma unroll
+    for (int t = 0; t < DRF_ZT; ++t) acc[t] = 0.0f;
+
+    for (int row = r0; row < r1; ++row) {
+        const int kidx = a.rowKernel[(long)v * a.NRot + row];
+        if (BCK) {
+#pragma unroll
+            for (int t = 0; t < DRF_ZT; ++t) acc[t] = 0.0f;
+        }
+        if (kidx < 0) {
+            if (x < a.NRot) {
+#pragma unroll
+                for (int t = 0; t < DRF_ZT; ++t) {
+                    const int z = z0 + zl0 + t;
+                    if (z < a.Nz) acc[t] += drf_in<BCK>(a, b, v, row, z, x);
+                }
+            }
+        } else {
+            const int pad = a.rowPad[(long)v * a.NRot + row];
+            for (int e = threadIdx.x; e < TH * TW; e += blockDim.x) {
+                const int q = e / TW, c = e % TW;
+                tile[e] = drf_stage_val<BCK>(a, b, v, row, z0 - CY + q, x0 - CX + c, pad);
+            }
+            const float *kp = a.psf + (long)kidx * KY * KX;
+            for (int e = threadIdx.x; e < KY * KX; e += blockDim.x) psf[e] = kp[e];
+            __syncthreads();
+            for (int i = 0; i < KX; ++i) {
+                float pc[KY];
+#pragma unroll
+                for (int j = 0; j < KY; ++j) pc[j] = psf[j * KX + i];
+                const float *tcol = tile + zl0 * TW + tx + KX - 1 - i;
+#pragma unroll
+                for (int q = 0; q < DRF_ZT + KY - 1; ++q) {
+                    const float val = tcol[q * TW];
+#pragma unroll
+                    for (int t = 0; t < DRF_ZT; ++t) {
+                        const int j = t + KY - 1 - q;
+                        if (j >= 0 && j < KY) acc[t] = fmaf(pc[j], val, acc[t]);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        if (BCK && x < a.NRot) {
+            float *o = a.out + ((long)b * a.NRot + row) * pl;
+#pragma unroll
+            for (int t = 0; t < DRF_ZT; ++t) {
+                const int z = z0 + zl0 + t;
+                if (z < a.Nz) o[(long)z * a.NRot + x] = acc[t];
+            }
+        }
+    }
+    if (!BCK && x < a.NRot) {
+        const int ch = r0 / a.rchunk;
+        float *o = a.out + ((long)b * a.nchunk + ch) * pl;
+#pragma unroll
+        for (int t = 0; t < DRF_ZT; ++t) {
+            const int z = z0 + zl0 + t;
+            if (z < a.Nz) o[(long)z * a.NRot + x] = acc[t];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_ratio: projection -> measured/calculated ratio (osemrecon.c:727-820).
+// ---------------------------------------------------------------------------
+struct RatioArgs {
+    int B, Nz, NRot, nchunk;
+    const int *bview;
+    const float *ppart;           // [B][nchunk][Nz][NRot]
+    const float *prj;             // [V][Nz][NRot] measured
+    const float *scat;            // [V][Nz][NRot] or null
+    const float *vtsens;          // [V] ViewTime*Sensitivity (float product)
+    const float *vt;              // [V]
+    const uint8_t *colKeep;       // [V][NRot]
+    float scale;                  // fScaleFac
+    float *ratio;                 // [B][Nz][NRot]
+};
+
+__global__ void k_ratio(RatioArgs a)
+{
+    const long pl = (long)a.Nz * a.NRot;
+    const long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (gid >= pl * a.B) return;
+    const int b = (int)(gid / pl);
+    const long i = gid % pl;
+    const int xb = (int)(i % a.NRot);
+    const int v = a.bview[b];
+    float p = 0.0f;
+    for (int c = 0; c < a.nchunk; ++c) p += a.ppart[((long)b * a.nchunk + c) * pl + i];
+    float calc = p * a.scale;
+    calc = calc * a.vtsens[v];
+    if (a.scat) calc = calc + a.scat[(long)v * pl + i];
+    // osemrecon.c:802-803 hands view 0's measured data to the mask
+    if (!a.colKeep[(long)v * a.NRot + xb] || a.prj[i] < 0.0f) calc = 0.0f;
+    float r = ((double)calc > 1e-20) ? a.prj[(long)v * pl + i] / calc : 0.0f;
+    a.ratio[gid] = r * a.vt[v];
+}
+
+// ---------------------------------------------------------------------------
+// k_bck_update: gather-form back rotation of the batch's convolved planes,
+// in subset order, with the EPS clamp after every view (prjbck.c:408-426),
+// then x *= U * Ninv (osemrecon.c:850-852).  One thread per voxel, so the
+// order-dependent clamp is reproduced exactly and no atomics are needed.
+// ---------------------------------------------------------------------------
+struct BckArgs {
+    int B, N, Nz, NRot;
+    const int *bview;
+    const float *bsin, *bcos;
+    const float *planes;          // [B][NRot][Nz][NRot]
+    const int *head, *npix;
+    float *x;                     // estimate (updated in place) or null
+    const float *ninv;            // sensitivity inverse or null
+    float *u;                     // accumulator; read when accumulate != 0
+    int accumulate;               // start from u instead of 0
+    int writeU;
+};
+
+__global__ void k_bck_update(BckArgs a)
+{
+    const long M = (long)a.N * a.Nz * a.N;
+    const long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (gid >= M) return;
+    const int xc = (int)(gid % a.N);
+    const long r = gid / a.N;
+    const int z = (int)(r % a.Nz);
+    const int y = (int)(r / a.Nz);
+    const int skip = (a.NRot - a.N) / 2;
+    const int Y = y + skip, X = xc + skip;
+    const float Axis = 0.5 * (a.NRot - 1);
+    const long vol = (long)a.NRot * a.Nz * a.NRot;
+    const bool inside = in_circle(a.head, a.npix, Y, X);
+    const float Xf = (float)X - Axis, Yf = (float)Y - Axis;
+    float U = a.accumulate ? a.u[gid] : 0.0f;
+    for (int b = 0; b < a.B; ++b) {
+        const int v = a.bview[b];
+        float val = 0.0f;
+        if (inside)
+            val = rot_sample(a.planes + (long)b * vol + (long)z * a.NRot,
+                             (long)a.Nz * a.NRot, a.NRot, Xf, Yf, a.bsin[v], a.bcos[v], Axis);
+        U = U + val;
+        if ((double)U < 1.0e-5) U = 0.0f;
+    }
+    if (a.writeU) a.u[gid] = U;
+    if (a.x) a.x[gid] = a.x[gid] * (U * a.ninv[gid]);
+}
+
+// ---------------------------------------------------------------------------
+// k_norm: FP64 sensitivity accumulation of one subset and inversion with the
+// 1e-6 threshold (osemrecon.c:282-320).
+// ---------------------------------------------------------------------------
+__global__ void k_norm(BckArgs a, float *__restrict__ ninv)
+{
+    const long M = (long)a.N * a.Nz * a.N;
+    const long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (gid >= M) return;
+    const int xc = (int)(gid % a.N);
+    const long r = gid / a.N;
+    const int z = (int)(r % a.Nz);
+    const int y = (int)(r / a.Nz);
+    const int skip = (a.NRot - a.N) / 2;
+    const int Y = y + skip, X = xc + skip;
+    const float Axis = 0.5 * (a.NRot - 1);
+    const long vol = (long)a.NRot * a.Nz * a.NRot;
+    const bool inside = in_circle(a.head, a.npix, Y, X);
+    const float Xf = (float)X - Axis, Yf = (float)Y - Axis;
+    double U = 0.0;
+    for (int b = 0; b < a.B; ++b) {
+        const int v = a.bview[b];
+        double val = 0.0;
+        if (inside)
+            val = rot_sample(a.planes + (long)b * vol + (long)z * a.NRot,
+                             (long)a.Nz * a.NRot, a.NRot, Xf, Yf, a.bsin[v], a.bcos[v], Axis);
+        U += val;
+        if (U < 1.0e-5) U = 0.0;
+    }
+    ninv[gid] = U < 1e-6 ? 0.0f : (float)(1.0 / U);
+}
+
+// Sensitivity input: measured-bin mask times view time (osemrecon.c:290-295).
+__global__ void k_norm_input(int B, int Nz, int NRot, const int *__restrict__ bview,
+                             const float *__restrict__ prj, const uint8_t *__restrict__ colMeasOne,
+                             const float *__restrict__ vt, float *__restrict__ out)
+{
+    const long pl = (long)Nz * NRot;
+    const long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (gid >= pl * B) return;
+    const int b = (int)(gid / pl);
+    const long i = gid % pl;
+    const int v = bview[b];
+    const int xb = (int)(i % NRot);
+    float m = (colMeasOne[(long)v * NRot + xb] && prj[(long)v * pl + i] >= 0.0f) ? 1.0f : 0.0f;
+    out[gid] = m * vt[v];
+}
+
+// ---------------------------------------------------------------------------
+// Engine implementation
+// ---------------------------------------------------------------------------
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count)
+    {
+        free();
+        if (count == 0) return;
+        HIP_CHECK(hipMalloc((void **)&p, count * sizeof(T)));
+        n = count;
+    }
+    void free()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void upload(const T *h, size_t count, hipStream_t s)
+    {
+        HIP_CHECK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void download(T *h, size_t count, hipStream_t s) const
+    {
+        HIP_CHECK(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+    }
+    ~DBuf() { free(); }
+};
+
+struct Recon::Impl {
+    Config c;
+    hipStream_t stream = nullptr;
+    long M = 0, P = 0, vol = 0;
+    int nchunk = 1, rchunk = 1;
+    DBuf<float> x, mu, prj, scat, u, ninv, afac, att, ppart, ratio, planes;
+    DBuf<float> fsin, fcos, bsin, bcos, vt, vtsens, psf;
+    DBuf<int> rowKernel, rowPad, subsets, head, npix, rowNone;
+    DBuf<uint8_t> colMeasOne, colKeep;
+    std::vector<int> hhead, hnpix;
+    const int *rowKernelNone() const { return rowNone.p; }
+
+    bool timing = false;
+    struct Stage {
+        std::vector<hipEvent_t> ev;  // start/stop pairs awaiting readout
+        double ms = 0;
+        int launches = 0;
+    };
+    std::map<std::string, Stage> stages;
+    std::vector<std::string> order;
+
+    template <typename F>
+    void launch(const char *name, F &&f)
+    {
+        if (!timing) {
+            f();
+            HIP_CHECK(hipGetLastError());
+            return;
+        }
+        hipEvent_t e0, e1;
+        HIP_CHECK(hipEventCreate(&e0));
+        HIP_CHECK(hipEventCreate(&e1));
+        HIP_CHECK(hipEventRecord(e0, stream));
+        f();
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipEventRecord(e1, stream));
+        auto it = stages.find(name);
+        if (it == stages.end()) {
+            order.push_back(name);
+            it = stages.emplace(name, Stage{}).first;
+        }
+        it->second.ev.push_back(e0);
+        it->second.ev.push_back(e1);
+    }
+
+    void collect()
+    {
+        HIP_CHECK(hipStreamSynchronize(stream));
+        for (auto &kv : stages) {
+            auto &st = kv.second;
+            for (size_t k = 0; k + 1 < st.ev.size(); k += 2) {
+                float ms = 0;
+                HIP_CHECK(hipEventElapsedTime(&ms, st.ev[k], st.ev[k + 1]));
+                st.ms += ms;
+                st.launches++;
+                (void)hipEventDestroy(st.ev[k]);
+                (void)hipEventDestroy(st.ev[k + 1]);
+            }
+            st.ev.clear();
+        }
+    }
+
+    int ky_bucket() const
+    {
+        static const int sizes[] = {7, 15, 23, 31, 47, 63};
+        for (int s : sizes)
+            if (c.drf.KY <= s) return s;
+        fail(ETYPE_ILLEGAL_VALUE, "DRF axial size %d exceeds the supported 63", c.drf.KY);
+    }
+
+    void run_drf(bool bck, const DrfArgs &args, int nblocks, int KYb)
+    {
+        const int TW = DRF_TX + args.KX - 1;
+        const int TH = DRF_TZ + KYb - 1;
+        const size_t lds = sizeof(float) * ((size_t)TH * TW + (size_t)KYb * args.KX);
+        if (lds > 160 * 1024) fail(ETYPE_ILLEGAL_VALUE, "DRF PSF too wide (KX=%d)", args.KX);
+        dim3 grid(nblocks), block(256);
+#define IRL_DRF_CASE(K)                                                                     \
+    case K:                                                                                 \
+        if (bck)                                                                            \
+            hipLaunchKernelGGL((k_drf<K, true>), grid, block, lds, stream, args);           \
+        else                                                                                \
+            hipLaunchKernelGGL((k_drf<K, false>), grid, block, lds, stream, args);          \
+        break;
+        switch (KYb) {
+            IRL_DRF_CASE(7)
+            IRL_DRF_CASE(15)
+            IRL_DRF_CASE(23)
+            IRL_DRF_CASE(31)
+            IRL_DRF_CASE(47)
+            IRL_DRF_CASE(63)
+        default:
+            fail(ETYPE_ILLEGAL_VALUE, "unsupported DRF bucket %d", KYb);
+        }
+#undef IRL_DRF_CASE
+    }
+};
+
+static int div_up(long a, long b) { return (int)((a + b - 1) / b); }
+
+Recon::Recon(const Config &cfg) : impl_(new Impl), cfg_(cfg)
+{
+    Impl &I = *impl_;
+    I.c = cfg;
+    const Geometry &g = cfg.g;
+    if (g.N <= 0 || g.Nz <= 0 || g.NRot < g.N || g.V <= 0)
+        fail(ETYPE_ILLEGAL_VALUE, "bad geometry N=%d Nz=%d NRot=%d V=%d", g.N, g.Nz, g.NRot, g.V);
+    HIP_CHECK(hipStreamCreate(&I.stream));
+    I.M = (long)g.N * g.Nz * g.N;
+    I.P = (long)g.NRot * g.Nz;
+    I.vol = (long)g.NRot * g.Nz * g.NRot;
+    I.rchunk = 16;
+    I.nchunk = div_up(g.NRot, I.rchunk);
+    const int Vs = cfg.Vs;
+    I.x.alloc(I.M);
+    I.u.alloc(I.M);
+    I.ninv.alloc(I.M * cfg.S);
+    I.prj.alloc(I.P * g.V);
+    if (cfg.hasScatterEstimate) I.scat.alloc(I.P * g.V);
+    if (cfg.hasAtn) {
+        I.mu.alloc(I.M);
+        I.afac.alloc(I.vol * g.V);
+    }
+    I.att.alloc(I.vol * Vs);
+    I.ppart.alloc(I.P * I.nchunk * Vs);
+    I.ratio.alloc(I.P * Vs);
+    I.planes.alloc(I.vol * Vs);
+
+    std::vector<float> fs(g.V), fc(g.V), bs(g.V), bc(g.V), vt(g.V), vts(g.V);
+    for (int v = 0; v < g.V; ++v) {
+        fs[v] = cfg.views[v].fsin;
+        fc[v] = cfg.views[v].fcos;
+        bs[v] = cfg.views[v].bsin;
+        bc[v] = cfg.views[v].bcos;
+        vt[v] = cfg.views[v].viewtime;
+        vts[v] = cfg.views[v].viewtime * cfg.views[v].sensitivity;
+    }
+    auto up = [&](DBuf<float> &d, const std::vector<float> &h) {
+        d.alloc(h.size());
+        d.upload(h.data(), h.size(), I.stream);
+    };
+    up(I.fsin, fs);
+    up(I.fcos, fc);
+    up(I.bsin, bs);
+    up(I.bcos, bc);
+    up(I.vt, vt);
+    up(I.vtsens, vts);
+    if (cfg.drf.nDist > 0) {
+        // Device copy padded (centred) to the kernel's axial bucket size.
+        const int KY = cfg.drf.KY, KX = cfg.drf.KX, KYb = I.ky_bucket(), oy = (KYb - KY) / 2;
+        std::vector<float> padded((size_t)cfg.drf.nDist * KYb * KX, 0.0f);
+        for (int d = 0; d < cfg.drf.nDist; ++d)
+            for (int j = 0; j < KY; ++j)
+                memcpy(&padded[((size_t)d * KYb + j + oy) * KX],
+                       &cfg.drf.psf[((size_t)d * KY + j) * KX], sizeof(float) * KX);
+        up(I.psf, padded);
+    }
+    std::vector<int> rk = cfg.rowKernel, rp = cfg.rowPad;
+    if (rk.empty()) rk.assign((size_t)g.V * g.NRot, -1);
+    if (rp.empty()) rp.assign((size_t)g.V * g.NRot, 0);
+    I.rowKernel.alloc(rk.size());
+    I.rowKernel.upload(rk.data(), rk.size(), I.stream);
+    I.rowPad.alloc(rp.size());
+    I.rowPad.upload(rp.data(), rp.size(), I.stream);
+    std::vector<int> none((size_t)g.V * g.NRot, -1);
+    I.rowNone.alloc(none.size());
+    I.rowNone.upload(none.data(), none.size(), I.stream);
+    I.subsets.alloc(cfg.subsets.size());
+    I.subsets.upload(cfg.subsets.data(), cfg.subsets.size(), I.stream);
+    circle_rows(g.NRot, I.hhead, I.hnpix);
+    I.head.alloc(g.NRot);
+    I.head.upload(I.hhead.data(), g.NRot, I.stream);
+    I.npix.alloc(g.NRot);
+    I.npix.upload(I.hnpix.data(), g.NRot, I.stream);
+    I.colMeasOne.alloc(cfg.colMeasOne.size());
+    I.colMeasOne.upload(cfg.colMeasOne.data(), cfg.colMeasOne.size(), I.stream);
+    I.colKeep.alloc(cfg.colKeep.size());
+    I.colKeep.upload(cfg.colKeep.data(), cfg.colKeep.size(), I.stream);
+    HIP_CHECK(hipStreamSynchronize(I.stream));
+}
+
+Recon::~Recon()
+{
+    if (impl_) {
+        if (impl_->stream) {
+            (void)hipStreamSynchronize(impl_->stream);
+            for (auto &kv : impl_->stages)
+                for (auto e : kv.second.ev) (void)hipEventDestroy(e);
+            (void)hipStreamDestroy(impl_->stream);
+        }
+        delete impl_;
+    }
+}
+
+void Recon::uploadEstimate(const float *h)
+{
+    impl_->x.upload(h, impl_->M, impl_->stream);
+    synchronize();
+}
+void Recon::uploadAtn(const float *h)
+{
+    if (!impl_->mu.p) return;
+    impl_->mu.upload(h, impl_->M, impl_->stream);
+    synchronize();
+}
+void Recon::uploadProjections(const float *h)
+{
+    impl_->prj.upload(h, impl_->P * cfg_.g.V, impl_->stream);
+    synchronize();
+}
+void Recon::uploadScatterEstimate(const float *h)
+{
+    if (!impl_->scat.p) return;
+    impl_->scat.upload(h, impl_->P * cfg_.g.V, impl_->stream);
+    synchronize();
+}
+void Recon::uploadUpdate(const float *h)
+{
+    impl_->u.upload(h, impl_->M, impl_->stream);
+    synchronize();
+}
+void Recon::uploadNorm(int s, const float *h)
+{
+    HIP_CHECK(hipMemcpyAsync(impl_->ninv.p + impl_->M * s, h, impl_->M * sizeof(float),
+                             hipMemcpyHostToDevice, impl_->stream));
+    synchronize();
+}
+void Recon::downloadEstimate(float *h) const
+{
+    impl_->x.download(h, impl_->M, impl_->stream);
+    synchronize();
+}
+void Recon::downloadUpdate(float *h) const
+{
+    impl_->u.download(h, impl_->M, impl_->stream);
+    synchronize();
+}
+void Recon::downloadNorm(int s, float *h) const
+{
+    HIP_CHECK(hipMemcpyAsync(h, impl_->ninv.p + impl_->M * s, impl_->M * sizeof(float),
+                             hipMemcpyDeviceToHost, impl_->stream));
+    synchronize();
+}
+void Recon::synchronize() const { HIP_CHECK(hipStreamSynchronize(impl_->stream)); }
+void Recon::setTiming(bool on) { impl_->timing = on; }
+void Recon::resetTimes()
+{
+    impl_->collect();
+    impl_->stages.clear();
+    impl_->order.clear();
+}
+std::vector<Recon::StageTime> Recon::stageTimes() const
+{
+    impl_->collect();
+    std::vector<StageTime> out;
+    for (auto &n : impl_->order) {
+        auto &st = impl_->stages[n];
+        out.push_back(StageTime{n, st.ms, st.launches});
+    }
+    return out;
+}
+
+void Recon::precompute()
+{
+    Impl &I = *impl_;
+    const Geometry &g = cfg_.g;
+    const int threads = 256;
+    if (cfg_.hasAtn) {
+        const long n = (long)g.V * g.Nz * g.NRot;
+        I.launch("afac", [&] {
+            hipLaunchKernelGGL(k_afac, dim3(div_up(n, threads)), dim3(threads), 0, I.stream,
+                               I.mu.p, I.afac.p, g.V, g.Nz, g.N, g.NRot, I.fsin.p, I.fcos.p,
+                               1.0f, I.head.p, I.npix.p);
+        });
+    }
+    // Sensitivity images: backproject mask*ViewTime for each subset.
+    const int Vs = cfg_.Vs;
+    const int KYb = cfg_.drf.nDist > 0 ? I.ky_bucket() : 7;
+    for (int s = 0; s < cfg_.S; ++s) {
+        const int *bview = I.subsets.p + (long)s * Vs;
+        I.launch("norm_input", [&] {
+            hipLaunchKernelGGL(k_norm_input, dim3(div_up(I.P * Vs, threads)), dim3(threads), 0,
+                               I.stream, Vs, g.Nz, g.NRot, bview, I.prj.p, I.colMeasOne.p,
+                               I.vt.p, I.ratio.p);
+        });
+        DrfArgs da{};
+        da.NRot = g.NRot;
+        da.Nz = g.Nz;
+        da.KX = cfg_.drf.nDist > 0 ? cfg_.drf.KX : 1;
+        da.psf = I.psf.p;
+        da.rowKernel = (cfg_.bckModel & 2) ? I.rowKernel.p : nullptr;
+        da.rowPad = I.rowPad.p;
+        da.avg = cfg_.axialAvg;
+        da.bview = bview;
+        da.in = I.ratio.p;
+        da.afac = (cfg_.bckModel & 1) ? I.afac.p : nullptr;
+        da.out = I.planes.p;
+        da.xtiles = div_up(g.NRot, DRF_TX);
+        da.ztiles = div_up(g.Nz, DRF_TZ);
+        if (!(cfg_.bckModel & 2)) da.rowKernel = I.rowKernelNone();
+        I.launch("drf_bck_norm", [&] {
+            I.run_drf(true, da, da.xtiles * da.ztiles * Vs * g.NRot, KYb);
+        });
+        BckArgs ba{};
+        ba.B = Vs;
+        ba.N = g.N;
+        ba.Nz = g.Nz;
+        ba.NRot = g.NRot;
+        ba.bview = bview;
+        ba.bsin = I.bsin.p;
+        ba.bcos = I.bcos.p;
+        ba.planes = I.planes.p;
+        ba.head = I.head.p;
+        ba.npix = I.npix.p;
+        float *dst = I.ninv.p + I.M * s;
+        I.launch("norm", [&] {
+            hipLaunchKernelGGL(k_norm, dim3(div_up(I.M, threads)), dim3(threads), 0, I.stream,
+                               ba, dst);
+        });
+    }
+    synchronize();
+}
+
+void Recon::subIteration(int s, bool keepUpdate)
+{
+    Impl &I = *impl_;
+    const Geometry &g = cfg_.g;
+    const int threads = 256;
+    const int Vs = cfg_.Vs;
+    const int *bview = I.subsets.p + (long)s * Vs;
+    const int KYb = cfg_.drf.nDist > 0 ? I.ky_bucket() : 7;
+    const float *afac = (cfg_.prjModel & 1) ? I.afac.p : nullptr;
+
+    I.launch("att", [&] {
+        hipLaunchKernelGGL(k_att, dim3(div_up(I.vol * Vs, threads)), dim3(threads), 0,
+                           I.stream, I.x.p, afac, I.att.p, Vs, g.Nz, g.N, g.NRot, bview,
+                           I.fsin.p, I.fcos.p, I.head.p, I.npix.p);
+    });
+    DrfArgs da{};
+    da.NRot = g.NRot;
+    da.Nz = g.Nz;
+    da.KX = cfg_.drf.nDist > 0 ? cfg_.drf.KX : 1;
+    da.psf = I.psf.p;
+    da.rowKernel = (cfg_.prjModel & 2) ? I.rowKernel.p : I.rowKernelNone();
+    da.rowPad = I.rowPad.p;
+    da.avg = cfg_.axialAvg;
+    da.bview = bview;
+    da.in = I.att.p;
+    da.afac = nullptr;
+    da.out = I.ppart.p;
+    da.rchunk = I.rchunk;
+    da.nchunk = I.nchunk;
+    da.xtiles = div_up(g.NRot, DRF_TX);
+    da.ztiles = div_up(g.Nz, DRF_TZ);
+    I.launch("drf_prj", [&] {
+        I.run_drf(false, da, da.xtiles * da.ztiles * Vs * I.nchunk, KYb);
+    });
+    RatioArgs ra{};
+    ra.B = Vs;
+    ra.Nz = g.Nz;
+    ra.NRot = g.NRot;
+    ra.nchunk = I.nchunk;
+    ra.bview = bview;
+    ra.ppart = I.ppart.p;
+    ra.prj = I.prj.p;
+    ra.scat = I.scat.p;
+    ra.vtsens = I.vtsens.p;
+    ra.vt = I.vt.p;
+    ra.colKeep = I.colKeep.p;
+    ra.scale = 1.0f;
+    ra.ratio = I.ratio.p;
+    I.launch("ratio", [&] {
+        hipLaunchKernelGGL(k_ratio, dim3(div_up(I.P * Vs, threads)), dim3(threads), 0,
+                           I.stream, ra);
+    });
+    DrfArgs db = da;
+    db.rowKernel = (cfg_.bckModel & 2) ? I.rowKernel.p : I.rowKernelNone();
+    db.in = I.ratio.p;
+    db.afac = (cfg_.bckModel & 1) ? I.afac.p : nullptr;
+    db.out = I.planes.p;
+    I.launch("drf_bck", [&] {
+        I.run_drf(true, db, db.xtiles * db.ztiles * Vs * g.NRot, KYb);
+    });
+    BckArgs ba{};
+    ba.B = Vs;
+    ba.N = g.N;
+    ba.Nz = g.Nz;
+    ba.NRot = g.NRot;
+    ba.bview = bview;
+    ba.bsin = I.bsin.p;
+    ba.bcos = I.bcos.p;
+    ba.planes = I.planes.p;
+    ba.head = I.head.p;
+    ba.npix = I.npix.p;
+    ba.x = I.x.p;
+    ba.ninv = I.ninv.p + I.M * s;
+    ba.u = I.u.p;
+    ba.accumulate = 0;
+    ba.writeU = keepUpdate ? 1 : 0;
+    I.launch("bck_update", [&] {
+        hipLaunchKernelGGL(k_bck_update, dim3(div_up(I.M, threads)), dim3(threads), 0,
+                           I.stream, ba);
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Stateless helper for the rot3dpar entry point.
+// ---------------------------------------------------------------------------
+void dev_rot3dpar(float angle, const float *in_host, float *out_host, int pbFlag, int Nz,
+                  int N, int NRot)
+{
+    const int OldDim = pbFlag ? NRot : N, NewDim = NRot;
+    const float sn = (float)sin(angle), cs = (float)cos(angle);
+    const float NewAxis = 0.5 * (NewDim - 1), OldAxis = 0.5 * (OldDim - 1);
+    std::vector<int> head, npix;
+    circle_rows(NewDim, head, npix);
+    hipStream_t st;
+    HIP_CHECK(hipStreamCreate(&st));
+    DBuf<float> din, dout;
+    DBuf<int> dh, dn;
+    const long nin = (long)OldDim * Nz * OldDim, nout = (long)NewDim * Nz * NewDim;
+    din.alloc(nin);
+    dout.alloc(nout);
+    dh.alloc(NewDim);
+    dn.alloc(NewDim);
+    din.upload(in_host, nin, st);
+    dh.upload(head.data(), NewDim, st);
+    dn.upload(npix.data(), NewDim, st);
+    hipLaunchKernelGGL(k_rot3dpar, dim3(div_up(nout, 256)), dim3(256), 0, st, din.p, dout.p, Nz,
+                       OldDim, NewDim, sn, cs, NewAxis, OldAxis, dh.p, dn.p);
+    HIP_CHECK(hipGetLastError());
+    dout.download(out_host, nout, st);
+    HIP_CHECK(hipStreamSynchronize(st));
+    HIP_CHECK(hipStreamDestroy(st));
+}
+
+}  // namespace irlamd

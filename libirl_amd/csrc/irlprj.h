@@ -1,0 +1,36 @@
+// irlprj.h -- internal types shared by the host layers of libirl_amd.
+// The C declarations exported to callers live in include/irlprj.h; this file
+// only adds the C++ pieces the host layers share.
+#pragma once
+
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/irl.h"
+#include "../../include/irlprj.h"
+#include "../../include/osemhooks.h"
+#include "engine.h"
+
+namespace irlamd {
+
+// Message and log plumbing (the reference's vPrintMsg / vErrorHandler
+// behaviour: level-filtered status messages, warnings to the log file).
+void msg(int level, const char *fmt, ...);
+void warn(const char *fmt, ...);
+void set_msg_level(int level);
+void set_msg_file(FILE *fp);
+void set_log_file(FILE *fp);
+void set_error_string(const std::string &s);
+
+// Reference-arithmetic host geometry.
+int drf_row_index(const DrfTab_t *t, float fCFCR, float fPixelWidth, int iNumRotPixs, int iRow,
+                  bool *skip);
+PaddedDrf pad_drf_table(const DrfTab_t *t);
+void view_trig(float angle, ViewTab &vt);
+
+// DRFT / KRNL file readers (drftabio.c:150-275, krnlio.c:196-331).
+DrfTab_t read_drf_tab(const char *fname, std::string *title, std::string *col);
+void free_drf_tab_contents(DrfTab_t *t);
+
+}  // namespace irlamd

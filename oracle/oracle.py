@@ -1,0 +1,198 @@
+"""ctypes bindings of the CPU oracle (oracle/irl_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker or the timed CPU baseline.  The
+product (libirl_amd) never imports it.
+
+Two builds: 'f32' (reference semantics, float values) and 'f64' (FP64
+oracle).  Volumes are the reference's internal [y][z][x] order unless noted.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import build as _build
+
+_libs = {}
+
+
+class OrcView(ctypes.Structure):
+    _fields_ = [("angle", ctypes.c_float), ("left", ctypes.c_float),
+                ("right", ctypes.c_float), ("cfcr", ctypes.c_float),
+                ("viewtime", ctypes.c_float), ("sensitivity", ctypes.c_float)]
+
+
+def lib(real="f64"):
+    if real in _libs:
+        return _libs[real]
+    path = _build.lib_path(real)
+    if not os.path.exists(path):
+        _build.build(force=False)
+    L = ctypes.CDLL(path)
+    vp = ctypes.c_void_p
+    L.orc_drftab_new.restype = vp
+    L.orc_drftab_new.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_float, vp, vp]
+    L.orc_drftab_free.argtypes = [vp]
+    L.orc_rot3dpar.argtypes = [ctypes.c_float, vp, vp, ctypes.c_int, ctypes.c_int,
+                               ctypes.c_int, ctypes.c_int]
+    L.orc_conv2d_ypad.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int,
+                                  ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
+    L.orc_init_prj.argtypes = [ctypes.c_int] * 4
+    L.orc_drf_index.argtypes = [vp, ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_int]
+    L.orc_project.argtypes = ([ctypes.c_int] * 4 + [ctypes.c_float] * 3 +
+                              [vp, ctypes.c_int, vp, vp, vp, vp] + [ctypes.c_float] * 2 +
+                              [ctypes.c_int] * 4)
+    L.orc_backproject.argtypes = ([ctypes.c_int] * 4 + [ctypes.c_float] * 3 +
+                                  [vp, ctypes.c_int, vp, vp, vp, vp] + [ctypes.c_float] * 2 +
+                                  [ctypes.c_int] * 3)
+    L.orc_subsets.argtypes = [ctypes.c_int, ctypes.c_int, vp]
+    L.orc_osem.argtypes = ([ctypes.c_int] * 8 + [ctypes.c_float] + [vp, vp, ctypes.c_int, vp] +
+                           [ctypes.c_int] * 3 + [vp] * 4 + [vp, vp])
+    L.orc_esse_init.restype = vp
+    L.orc_esse_init.argtypes = ([vp, vp, ctypes.c_int, ctypes.c_int] + [ctypes.c_float] * 5 +
+                                [ctypes.c_int] * 4 + [ctypes.c_float] * 2)
+    L.orc_esse_free.argtypes = [vp]
+    L.orc_esse_effsrc.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
+    L.orc_reorder.argtypes = [ctypes.c_int] * 3 + [vp, vp]
+    L.orc_real_size.restype = ctypes.c_int
+    _libs[real] = L
+    return L
+
+
+def _dt(real):
+    return np.float32 if real == "f32" else np.float64
+
+
+def _arr(a, real):
+    return np.ascontiguousarray(a, dtype=_dt(real))
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+class DrfTable:
+    """DRF table in the reference layout: offs[(MaxOffsX, MaxOffsY)], psfs
+    of shape (2*MaxOffsY-1, 2*MaxOffsX-1)."""
+
+    def __init__(self, start, spacing, offs, psfs, fft=False):
+        self.start, self.spacing, self.fft = float(start), float(spacing), bool(fft)
+        self.offs = np.ascontiguousarray(np.array(offs, dtype=np.int32).reshape(-1))
+        self.psf = np.ascontiguousarray(np.concatenate([np.asarray(p, np.float32).reshape(-1)
+                                                        for p in psfs]))
+        self.n = len(psfs)
+        self._h = {}
+
+    def handle(self, real):
+        if real not in self._h:
+            self._h[real] = lib(real).orc_drftab_new(self.n, self.start, self.spacing,
+                                                     self.offs.ctypes.data, self.psf.ctypes.data)
+        return self._h[real]
+
+    def __del__(self):
+        for real, h in self._h.items():
+            try:
+                lib(real).orc_drftab_free(h)
+            except Exception:
+                pass
+
+
+def rot3dpar(angle, vol, pb_flag, nz, n, nrot, real="f64"):
+    vol = _arr(vol, real)
+    out = np.zeros((nrot, nz, nrot), _dt(real))
+    lib(real).orc_rot3dpar(angle, vol.ctypes.data, out.ctypes.data, pb_flag, nz, n, nrot)
+    return out
+
+
+def conv2d_ypad(plane, psf, pad_y=0, avg_y=0, real="f64"):
+    plane = _arr(plane, real)
+    psf = np.ascontiguousarray(psf, np.float32)
+    ny, nx = plane.shape
+    out = np.zeros_like(plane)
+    lib(real).orc_conv2d_ypad(plane.ctypes.data, nx, ny, psf.ctypes.data,
+                              (psf.shape[1] - 1) // 2, (psf.shape[0] - 1) // 2, pad_y, avg_y,
+                              out.ctypes.data)
+    return out
+
+
+def init_prj(nz, n, nrot, model=7, real="f64"):
+    lib(real).orc_init_prj(nz, n, nrot, model)
+
+
+def project(nz, n, nrot, model, pw, angle, cfcr, drf, atn, recon, primary=1.0, atnfac=1.0,
+            pad=0, avg=0, atn_rotated=0, act_rotated=0, real="f64", esse=None):
+    L = lib(real)
+    atn = None if atn is None else _arr(atn, real)
+    recon = _arr(recon, real)
+    out = np.zeros((nz, nrot), _dt(real))
+    L.orc_project(nz, n, nrot, model, pw, angle, cfcr, drf.handle(real) if drf else None,
+                  int(drf.fft) if drf else 0, esse, _p(atn), recon.ctypes.data, out.ctypes.data,
+                  primary, atnfac, pad, avg, atn_rotated, act_rotated)
+    return out
+
+
+def backproject(nz, n, nrot, model, pw, angle, cfcr, drf, atn, ratio, image, primary=1.0,
+                atnfac=1.0, pad=0, avg=0, atn_rotated=0, real="f64", esse=None):
+    """Adds the backprojection of 'ratio' into 'image' (in place, with the EPS
+    clamp); 'image' must be a contiguous array of the oracle's dtype."""
+    L = lib(real)
+    atn = None if atn is None else _arr(atn, real)
+    ratio = _arr(ratio, real)
+    assert image.dtype == _dt(real) and image.flags.c_contiguous
+    L.orc_backproject(nz, n, nrot, model, pw, angle, cfcr, drf.handle(real) if drf else None,
+                      int(drf.fft) if drf else 0, esse, _p(atn), ratio.ctypes.data,
+                      image.ctypes.data, primary, atnfac, pad, avg, atn_rotated)
+    return image
+
+
+def subsets(nviews, per):
+    L = lib("f64")
+    s = L.orc_subsets(nviews, per, None)
+    if s < 0:
+        return s
+    out = np.zeros(s * per, np.int32)
+    L.orc_subsets(nviews, per, out.ctypes.data)
+    return out.reshape(s, per)
+
+
+def reorder(d0, d1, d2, a, real="f64"):
+    a = _arr(a, real)
+    out = np.empty((d1, d0, d2), _dt(real))
+    lib(real).orc_reorder(d0, d1, d2, a.ctypes.data, out.ctypes.data)
+    return out
+
+
+ITER_CB = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p)
+
+
+def osem(nz, n, views, per_subset, iterations, prj_model, bck_model, pw, drf, prj, atn, x0,
+         scat=None, start_iteration=0, pad=0, avg=0, real="f64", iter_cb=None, esse=None,
+         srf_iterations=0):
+    """vOsemRecon on internal-order inputs (atn already scaled to 1/pixel and
+    clamped, prj clamped, x0 internal order).  views: list of dicts with
+    angle, left, right, cfcr, viewtime, sensitivity.  Returns the estimate."""
+    L = lib(real)
+    nv = len(views)
+    vv = (OrcView * nv)(*[OrcView(v["angle"], v["left"], v["right"], v["cfcr"],
+                                  v.get("viewtime", 1.0), v.get("sensitivity", 1.0))
+                          for v in views])
+    x = _arr(x0, real).copy()
+    prj = _arr(prj, real)
+    atn = None if atn is None else _arr(atn, real)
+    scat = None if scat is None else _arr(scat, real)
+    M = x.size
+
+    def _cb(it, ptr, ctx):
+        if iter_cb is not None:
+            buf = (ctypes.c_char * (M * x.itemsize)).from_address(ptr)
+            iter_cb(it, np.frombuffer(buf, dtype=_dt(real)).copy())
+    cb = ITER_CB(_cb)
+    rc = L.orc_osem(nz, n, nv, per_subset, iterations, start_iteration, prj_model, bck_model,
+                    pw, ctypes.addressof(vv), drf.handle(real) if drf else None,
+                    int(drf.fft) if drf else 0, esse, srf_iterations, pad, avg,
+                    prj.ctypes.data, _p(scat), _p(atn), x.ctypes.data,
+                    ctypes.cast(cb, ctypes.c_void_p), None)
+    if rc != 0:
+        raise ValueError("orc_osem failed with %d" % rc)
+    return x
