@@ -1,0 +1,175 @@
+"""OSEM sub-iteration throughput of libirl_amd on one MI355X.
+
+Workload (the 128^3 x 120-view configuration): 128^3 fp32 voxels of 0.33 cm,
+120 views over 360 deg on a seeded non-circular (ellipse) orbit with radius
+20-28 cm, seeded ellipsoid phantom with 6-10 inserts and a lung region,
+attenuation modelled, Gaussian distance-dependent DRF tabulated per plane with
+31x31 kernels, data forward-projected on the GPU (IrlGenprj) and Poisson
+sampled at 2M counts, 8 subsets x 15 views.
+
+A step is one OSEM sub-iteration (15 views of projection, ratio,
+backprojection, then the multiplicative update) with every input resident in
+HBM.  The timed region cycles through the subsets in order.
+
+Prints one JSON line (see the keys below).
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "OSEM subiters/s, 128³×120 views: ESSE vs CNN-scatter at ≤1% NRMSE"
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def build_workload(tmpdir, seed=2):
+    import libirl_amd as L
+    from libirl_amd import synth
+    cfg = synth.CONFIGS["c2"]
+    n, nz, pix, V = cfg["n"], cfg["nz"], cfg["pix"], cfg["views"]
+    act, mu = synth.phantom(n, nz, seed, n_inserts=cfg["inserts"])
+    views = synth.views_ellipse(V, n, seed)
+    drf_file = os.path.join(tmpdir, "c2.drft")
+    offs, psfs, start = synth.drf_for_orbit(drf_file, pix, n, views, cfg["half"])
+    parms = L.IrlParms()
+    parms.NumPixels, parms.NumSlices, parms.NumViews = n, nz, V
+    parms.BinWidth = pix
+    parms.fAtnScaleFac = pix
+    parms.fScatEstFac = 1.0
+    parms.NumAngPerSubset = cfg["vs"]
+    parms.NumIterations = cfg["iters"]
+    parms.SrfCollapseFac = 1
+    opts = L.Options()
+    opts.bModelDrf = 1
+    opts.bReconIsInitEst = 0
+    opts.iMsgLevel = 1
+    prj = np.zeros((V, nz, n), np.float32)
+    rc = L.irl_genprj(parms, opts, views, drf_file, None, None, act.copy(), mu.copy(), prj)
+    if rc != 0:
+        raise RuntimeError("IrlGenprj failed: " + L.error_string())
+    data = synth.poisson(prj, cfg["counts"], seed)
+    return dict(parms=parms, opts=opts, views=views, drf_file=drf_file, mu=mu, prj=data,
+                act=act, offs=offs, psfs=psfs, start=start, cfg=cfg)
+
+
+def drf_flops_per_subset(w, recon):
+    """Algorithmic flops of one subset's projector (or backprojector) DRF
+    stage: 2 * Nz * N * taps for every (view, row) whose plane is convolved
+    (distance > 0, prjbck.c:233)."""
+    p = w["parms"]
+    n, nz, pw = p.NumPixels, p.NumSlices, p.BinWidth
+    taps = [(2 * ox - 1) * (2 * oy - 1) for ox, oy in w["offs"]]
+    per_view = []
+    for v in w["views"]:
+        f = 0
+        for r in range(n):
+            dist = v.CFCR - pw * n / 2 + 0.5 * pw + r * pw
+            if dist > 0:
+                k = int(np.floor((dist - w["start"]) / pw + 0.5))
+                k = min(max(k, 0), len(taps) - 1)
+                f += 2 * nz * n * taps[k]
+        per_view.append(f)
+    S, Vs = recon.num_subsets, recon.views_per_subset
+    sub = [[s // 2 + (s % 2) * S // 2 + m * S for m in range(Vs)] for s in range(S)]
+    return [sum(per_view[a] for a in subset) for subset in sub]
+
+
+def cpu_baseline(w, nviews=2):
+    """Reference-semantics CPU path (oracle, FP32, 1 thread) on a bounded
+    sample: Project + BackProject of 'nviews' views of the same workload;
+    scaled to one 15-view sub-iteration."""
+    from oracle import oracle as O
+    p = w["parms"]
+    n, nz, pw = p.NumPixels, p.NumSlices, p.BinWidth
+    drf = O.DrfTable(w["start"], pw, w["offs"], w["psfs"])
+    atn = O.reorder(nz, n, n, w["mu"] * pw, real="f32")
+    x = np.full((n, nz, n), 1.0, np.float32)
+    img = np.zeros((n, nz, n), np.float32)
+    O.init_prj(nz, n, n, 3, real="f32")
+    t0 = time.perf_counter()
+    for v in w["views"][:nviews]:
+        prj = O.project(nz, n, n, 3, pw, v.Angle, v.CFCR, drf, atn, x, real="f32")
+        O.backproject(nz, n, n, 3, pw, v.Angle, v.CFCR, drf, atn, prj, img, atn_rotated=1,
+                      real="f32")
+    dt = time.perf_counter() - t0
+    per_sub = dt / nviews * p.NumAngPerSubset
+    return dict(value=1.0 / per_sub, unit="subiters/s", cores=1, kind="port",
+                sample="Project+BackProject of %d of the 15 views of one sub-iteration "
+                       "(%.1f s), scaled x%.1f" % (nviews, dt, p.NumAngPerSubset / nviews))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=24)
+    ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only; multi-GPU runs are not part of this build "
+              "(requested --gpus %d)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import libirl_amd as L
+    tmpdir = tempfile.mkdtemp(prefix="irlbench")
+    w = build_workload(tmpdir)
+    L.set_adv_options(0, 0)
+    rec = L.Recon(w["parms"], w["opts"], w["views"], w["drf_file"], None, None,
+                  w["mu"], w["prj"], None)
+    S = rec.num_subsets
+    for k in range(args.warmup):
+        rec.subiteration(k % S)
+    rec.synchronize()
+    rec.reset_timing()
+    rec.set_timing(True)
+    rec.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        rec.subiteration(k % S)
+    rec.synchronize()
+    dt = time.perf_counter() - t0
+    stages = rec.stage_times()
+    rec.set_timing(False)
+    ms_step = dt * 1e3 / args.steps
+    flops = drf_flops_per_subset(w, rec)
+    avg_flops = float(np.mean([flops[k % S] for k in range(args.steps)]))
+    dom = max(("drf_prj", "drf_bck"), key=lambda k: stages.get(k, (0, 1))[0])
+    dom_ms, dom_n = stages[dom]
+    achieved = avg_flops / (dom_ms / dom_n * 1e-3) / 1e12
+    out = {
+        "metric": METRIC,
+        "value": args.steps / dt,
+        "unit": "subiters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded phantom, GPU forward projection, Poisson 2M counts)",
+        "config": {"workload": "osem-128^3x120v-atn-drf31-8subsets (plain FP32, no scatter)",
+                   "N": 128, "Nz": 128, "views": 120, "subsets": S,
+                   "views_per_subset": rec.views_per_subset, "drf": "31x31 Gaussian",
+                   "parallelism": "single GPU"},
+        "roofline": {"kernel": dom, "bound": "mfma", "achieved": achieved,
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None},
+        "stages_ms_per_launch": {k: v[0] / max(v[1], 1) for k, v in stages.items()},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(w)
+    rec.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

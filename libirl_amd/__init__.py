@@ -107,6 +107,11 @@ def load():
     lib.IrlOsem.argtypes = [ctypes.POINTER(IrlParms), ctypes.POINTER(Options),
                             ctypes.POINTER(PrjView), ctypes.c_char_p, ctypes.c_char_p,
                             ITER_CB, _FP, _FP, _FP, _FP, ctypes.c_char_p, ctypes.c_char_p]
+    lib.IrlGenprj.restype = ctypes.c_int
+    lib.IrlGenprj.argtypes = [ctypes.POINTER(IrlParms), ctypes.POINTER(Options),
+                              ctypes.POINTER(PrjView), ctypes.c_char_p, ctypes.c_char_p,
+                              ITER_CB, _FP, _FP, _FP, _FP, ctypes.c_float, ctypes.c_char_p,
+                              ctypes.c_char_p]
     lib.pchIrlErrorString.restype = ctypes.c_char_p
     lib.OsemSetAdvOptions.argtypes = [ctypes.POINTER(AdvOptions)]
     for name in ("PrevIter", "PostIter", "PrevSubset", "PostSubset", "PrevAngle", "PostAngle"):
@@ -174,6 +179,24 @@ def irl_osem(parms, options, views, drf_file, krnl_file, scat, atn, prj, recon,
     return lib.IrlOsem(ctypes.byref(parms), ctypes.byref(options), vv, _bytes(drf_file),
                        _bytes(krnl_file), cb, fptr(scat), fptr(atn), fptr(prj), fptr(recon),
                        _bytes(log_file), _bytes(msg_file))
+
+
+def irl_genprj(parms, options, views, drf_file, krnl_file, scat, act, atn, prj,
+               primary=1.0, view_cb=None, log_file=None, msg_file=None):
+    """IrlGenprj (irl.h:215-288).  Writes the projections into 'prj'
+    ([view][slice][bin] float32); act/atn/scat are modified in place as the C
+    function modifies them."""
+    lib = load()
+
+    def _cb(v, p):
+        if view_cb is not None:
+            n = parms.NumPixels * parms.NumSlices
+            view_cb(v, np.ctypeslib.as_array(p, shape=(n,)).copy())
+    cb = ITER_CB(_cb)
+    vv = (PrjView * len(views))(*views)
+    return lib.IrlGenprj(ctypes.byref(parms), ctypes.byref(options), vv, _bytes(drf_file),
+                         _bytes(krnl_file), cb, fptr(scat), fptr(act), fptr(atn), fptr(prj),
+                         primary, _bytes(log_file), _bytes(msg_file))
 
 
 def set_adv_options(fast_rotate=1, start_iteration=0):

@@ -87,9 +87,13 @@ public:
     void uploadProjections(const float *prj);
     void uploadScatterEstimate(const float *scat);
 
-    // Attenuation factors of every view (reused by all sub-iterations) and
-    // the FP64 sensitivity images of every subset.
-    void precompute();
+    // Attenuation factors of every view (reused by all sub-iterations) and,
+    // when 'norms' is set, the FP64 sensitivity images of every subset.
+    void precompute(bool norms = true);
+
+    // Projection of the current estimate for every view (IrlGenprj's
+    // per-view output before the 1/NumViews scaling), [view][slice][bin].
+    void forwardProject(float primary, float *out_host);
 
     // One OSEM sub-iteration over subset s (osemrecon.c:687-863 body).
     void subIteration(int s, bool keepUpdate);

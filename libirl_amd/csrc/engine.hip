@@ -31,6 +31,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <map>
 #include <vector>
 
@@ -408,6 +409,24 @@ __global__ void k_ratio(RatioArgs a)
     a.ratio[gid] = r * a.vt[v];
 }
 
+// k_prj_finish: IrlGenprj's per-view output (irlgenprj.c:229-258): projection
+// scaled by the primary factor, plus the scatter estimate, times
+// ViewTime*Sensitivity.
+__global__ void k_prj_finish(RatioArgs a, float *__restrict__ out)
+{
+    const long pl = (long)a.Nz * a.NRot;
+    const long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (gid >= pl * a.B) return;
+    const int b = (int)(gid / pl);
+    const long i = gid % pl;
+    const int v = a.bview[b];
+    float p = 0.0f;
+    for (int c = 0; c < a.nchunk; ++c) p += a.ppart[((long)b * a.nchunk + c) * pl + i];
+    float calc = p * a.scale;
+    if (a.scat) calc = calc + a.scat[(long)v * pl + i];
+    out[(long)v * pl + i] = calc * a.vtsens[v];
+}
+
 // ---------------------------------------------------------------------------
 // k_bck_update: gather-form back rotation of the batch's convolved planes,
 // in subset order, with the EPS clamp after every view (prjbck.c:408-426),
@@ -526,10 +545,12 @@ struct DBuf {
     }
     void upload(const T *h, size_t count, hipStream_t s)
     {
+        if (count == 0) return;
         HIP_CHECK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
     }
     void download(T *h, size_t count, hipStream_t s) const
     {
+        if (count == 0) return;
         HIP_CHECK(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
     }
     ~DBuf() { free(); }
@@ -797,7 +818,7 @@ std::vector<Recon::StageTime> Recon::stageTimes() const
     return out;
 }
 
-void Recon::precompute()
+void Recon::precompute(bool norms)
 {
     Impl &I = *impl_;
     const Geometry &g = cfg_.g;
@@ -813,7 +834,7 @@ void Recon::precompute()
     // Sensitivity images: backproject mask*ViewTime for each subset.
     const int Vs = cfg_.Vs;
     const int KYb = cfg_.drf.nDist > 0 ? I.ky_bucket() : 7;
-    for (int s = 0; s < cfg_.S; ++s) {
+    for (int s = 0; norms && s < cfg_.S; ++s) {
         const int *bview = I.subsets.p + (long)s * Vs;
         I.launch("norm_input", [&] {
             hipLaunchKernelGGL(k_norm_input, dim3(div_up(I.P * Vs, threads)), dim3(threads), 0,
@@ -938,6 +959,67 @@ void Recon::subIteration(int s, bool keepUpdate)
         hipLaunchKernelGGL(k_bck_update, dim3(div_up(I.M, threads)), dim3(threads), 0,
                            I.stream, ba);
     });
+}
+
+// Forward projection of the current estimate for every view (IrlGenprj).
+void Recon::forwardProject(float primary, float *out_host)
+{
+    Impl &I = *impl_;
+    const Geometry &g = cfg_.g;
+    const int threads = 256;
+    const int Vs = cfg_.Vs;
+    const int KYb = cfg_.drf.nDist > 0 ? I.ky_bucket() : 7;
+    const float *afac = (cfg_.prjModel & 1) ? I.afac.p : nullptr;
+    DBuf<int> order;
+    std::vector<int> all(g.V);
+    for (int v = 0; v < g.V; ++v) all[v] = v;
+    order.alloc(g.V);
+    order.upload(all.data(), g.V, I.stream);
+    DBuf<float> out;
+    out.alloc(I.P * g.V);
+    for (int v0 = 0; v0 < g.V; v0 += Vs) {
+        const int B = std::min(Vs, g.V - v0);
+        const int *bview = order.p + v0;
+        I.launch("att", [&] {
+            hipLaunchKernelGGL(k_att, dim3(div_up(I.vol * B, threads)), dim3(threads), 0,
+                               I.stream, I.x.p, afac, I.att.p, B, g.Nz, g.N, g.NRot, bview,
+                               I.fsin.p, I.fcos.p, I.head.p, I.npix.p);
+        });
+        DrfArgs da{};
+        da.NRot = g.NRot;
+        da.Nz = g.Nz;
+        da.KX = cfg_.drf.nDist > 0 ? cfg_.drf.KX : 1;
+        da.psf = I.psf.p;
+        da.rowKernel = (cfg_.prjModel & 2) ? I.rowKernel.p : I.rowKernelNone();
+        da.rowPad = I.rowPad.p;
+        da.avg = cfg_.axialAvg;
+        da.bview = bview;
+        da.in = I.att.p;
+        da.out = I.ppart.p;
+        da.rchunk = I.rchunk;
+        da.nchunk = I.nchunk;
+        da.xtiles = div_up(g.NRot, DRF_TX);
+        da.ztiles = div_up(g.Nz, DRF_TZ);
+        I.launch("drf_prj", [&] {
+            I.run_drf(false, da, da.xtiles * da.ztiles * B * I.nchunk, KYb);
+        });
+        RatioArgs ra{};
+        ra.B = B;
+        ra.Nz = g.Nz;
+        ra.NRot = g.NRot;
+        ra.nchunk = I.nchunk;
+        ra.bview = bview;
+        ra.ppart = I.ppart.p;
+        ra.scat = I.scat.p;
+        ra.vtsens = I.vtsens.p;
+        ra.scale = primary;
+        I.launch("prj_finish", [&] {
+            hipLaunchKernelGGL(k_prj_finish, dim3(div_up(I.P * B, threads)), dim3(threads), 0,
+                               I.stream, ra, out.p);
+        });
+    }
+    out.download(out_host, I.P * g.V, I.stream);
+    synchronize();
 }
 
 // ---------------------------------------------------------------------------
