@@ -667,7 +667,9 @@ Recon::Recon(const Config &cfg) : impl_(new Impl), cfg_(cfg)
     I.M = (long)g.N * g.Nz * g.N;
     I.P = (long)g.NRot * g.Nz;
     I.vol = (long)g.NRot * g.Nz * g.NRot;
-    I.rchunk = 16;
+    // Rows per projection chunk: enough chunks that a subset's projector
+    // launch fills the 256 CUs several times over.
+    I.rchunk = std::max(1, g.NRot / 32);
     I.nchunk = div_up(g.NRot, I.rchunk);
     const int Vs = cfg.Vs;
     I.x.alloc(I.M);
