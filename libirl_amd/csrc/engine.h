@@ -80,8 +80,8 @@ public:
     explicit Recon(const Config &cfg);
     ~Recon();
 
-    // Host -> device uploads (internal [y][z][x] order for volumes,
-    // [view][slice][bin] for projections).
+    // Host -> device uploads (volumes in [z][y][x] order, projections
+    // [view][slice][bin]).
     void uploadEstimate(const float *x);
     void uploadAtn(const float *mu);
     void uploadProjections(const float *prj);

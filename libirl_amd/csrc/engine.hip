@@ -2,8 +2,11 @@
 //
 // Data layout in HBM (one reconstruction, N = NumPixels = NumRotPixs,
 // Nz = NumSlices, V views, S subsets of Vs views):
-//   x, mu, U, Ninv[s]   [y][z][x] fp32 volumes (internal order of the
-//                       reference, irlosem.c:171-195), M = N*Nz*N floats each
+//   x, mu, U, Ninv[s]   [z][y][x] fp32 volumes, M = N*N*Nz floats each: one
+//                       slice's (y,x) plane is a contiguous N*N block, which is
+//                       the unit the rotations stage in LDS (the reference's
+//                       internal [y][z][x] order, irlosem.c:171-195, exists
+//                       only on the host side of the hooks)
 //   prj, scat           [view][slice][bin] fp32
 //   afac                [V][row][slice][bin] fp32 attenuation factors
 //                       exp(-sum of half-pixel mu) of every view, computed
@@ -11,7 +14,9 @@
 //   att                 [Vs][row][slice][bin] rotated estimate x afac
 //   ppart               [Vs][chunk][slice][bin] partial projections
 //   ratio               [Vs][slice][bin]
-//   bplanes             [Vs][row][slice][bin] convolved backprojection planes
+//   bplanes             [Vs][slice][row][bin] convolved backprojection planes
+//                       (slice-major, so back rotation stages one contiguous
+//                       NRot*NRot plane per slice)
 //
 // Kernels (reference loops they replace):
 //   k_rot3dpar   rot3dpar.c:92-192 (generic, prjbck-level entry point)
@@ -22,9 +27,9 @@
 //                convolution accumulated into the projection
 //   k_ratio      osemrecon.c:727-820
 //   k_drf<BCK>   prjbck.c:346-393 + conv2d_ypad.c:92-103
-//   k_bck_update prjbck.c:408-426 + osemrecon.c:850-852: gather-form back
+//   k_bck<false> prjbck.c:408-426 + osemrecon.c:850-852: gather-form back
 //                rotation, per-view EPS clamp, multiplicative update
-//   k_norm       osemrecon.c:282-320 with FP64 accumulation
+//   k_bck<true>  osemrecon.c:282-320 with FP64 accumulation
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -32,6 +37,7 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <type_traits>
 #include <map>
 #include <vector>
 
@@ -167,14 +173,14 @@ __global__ void k_afac(const float *__restrict__ mu, float *__restrict__ afac, i
     const float NewAxis = 0.5 * (NRot - 1);
     const float OldAxis = 0.5 * (N - 1);
     const float sn = fsin[v], cs = fcos[v];
-    const float *plane = mu + (long)z * N;
+    const float *plane = mu + (long)z * N * N;
     float *o = afac + (long)v * NRot * cols + c;
     const float Xf = (float)X - NewAxis;
     float S = 0.0f;
     for (int Y = 0; Y < NRot; ++Y) {
         float m = 0.0f;
         if (in_circle(head, npix, Y, X))
-            m = rot_sample(plane, (long)Nz * N, N, Xf, (float)Y - NewAxis, sn, cs, OldAxis);
+            m = rot_sample(plane, N, N, Xf, (float)Y - NewAxis, sn, cs, OldAxis);
         S = (float)((double)S + 0.5 * (double)m);
         o[(long)Y * cols] = (float)exp((double)(-S * atnFac));
         S = (float)((double)S + 0.5 * (double)m);
@@ -183,18 +189,24 @@ __global__ void k_afac(const float *__restrict__ mu, float *__restrict__ afac, i
 
 // ---------------------------------------------------------------------------
 // k_att: rotated estimate times attenuation factor for the batch's views.
+// One thread per output (view, row, slice, bin); a wave covers consecutive
+// bins of one row and slice, whose bilinear taps lie on a line inside one
+// contiguous N*N slice of x, so the gathers stay within a few cache lines.
 // ---------------------------------------------------------------------------
-__global__ void k_att(const float *__restrict__ x, const float *__restrict__ afac,
-                      float *__restrict__ att, int B, int Nz, int N, int NRot,
-                      const int *__restrict__ bview, const float *__restrict__ fsin,
-                      const float *__restrict__ fcos, const int *__restrict__ head,
-                      const int *__restrict__ npix)
+__global__ __launch_bounds__(256) void k_att(const float *__restrict__ x,
+                                             const float *__restrict__ afac,
+                                             float *__restrict__ att, int B, int Nz, int N,
+                                             int NRot, const int *__restrict__ bview,
+                                             const float *__restrict__ fsin,
+                                             const float *__restrict__ fcos,
+                                             const int *__restrict__ head,
+                                             const int *__restrict__ npix)
 {
-    const long vol = (long)NRot * Nz * NRot;
+    const long pl = (long)Nz * NRot, vol = pl * NRot;
     const long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (gid >= vol * B) return;
     const int b = (int)(gid / vol);
-    const long i = gid % vol;
+    const long i = gid - (long)b * vol;
     const int X = (int)(i % NRot);
     const long r = i / NRot;
     const int z = (int)(r % Nz);
@@ -204,8 +216,8 @@ __global__ void k_att(const float *__restrict__ x, const float *__restrict__ afa
     if (in_circle(head, npix, Y, X)) {
         const float NewAxis = 0.5 * (NRot - 1);
         const float OldAxis = 0.5 * (N - 1);
-        val = rot_sample(x + (long)z * N, (long)Nz * N, N, (float)X - NewAxis,
-                         (float)Y - NewAxis, fsin[v], fcos[v], OldAxis);
+        val = rot_sample(x + (long)z * N * N, N, N, (float)X - NewAxis, (float)Y - NewAxis,
+                         fsin[v], fcos[v], OldAxis);
     }
     if (afac) val = val * afac[(long)v * vol + i];
     att[gid] = val;
@@ -217,28 +229,29 @@ __global__ void k_att(const float *__restrict__ x, const float *__restrict__ afa
 //        DRF_row (*) att[row]                       (prjbck.c:195-251)
 //   BCK: bplanes[row] = DRF_row (*) (ratio x afac[row]) (prjbck.c:346-393)
 // Block = 256 threads = 64 bins x 4 slice groups; each thread owns ZT
-// consecutive slices of one bin.  The input tile with its halo and the padded
-// PSF are staged in LDS; for each PSF column the thread keeps the column in
-// registers and slides over the input rows, so every LDS value feeds up to
-// ZT FMAs.  Rows whose plane index is -1 (distance <= 0) are added without
-// convolution, as the reference skips ConvolveDrfYpad there.
+// consecutive slices of one bin.  The input tile with its halo is staged in
+// LDS column-major (slices contiguous per bin, column stride = 4 mod 32 dwords
+// so 128-bit reads of neighbouring lanes hit disjoint banks) and the padded
+// PSF transposed (its columns contiguous).  For each PSF column a thread reads
+// the column and its input window with 128-bit LDS loads and performs ZT x KY
+// FMAs from registers, so LDS traffic is a few percent of the FMA issue.
+// Rows whose plane index is -1 (distance <= 0) are added without convolution,
+// as the reference skips ConvolveDrfYpad there.
 // ---------------------------------------------------------------------------
 constexpr int DRF_TX = 64;
 constexpr int DRF_ZG = 4;
-constexpr int DRF_ZT = 16;
-constexpr int DRF_TZ = DRF_ZG * DRF_ZT;
 
 struct DrfArgs {
     int NRot, Nz;
     int KX;                       // padded PSF width (odd)
-    const float *psf;             // nDist * KY * KX
+    const float *psf;             // [nDist][KX][KYP] transposed, KYP = bucket padded to 4
     const int *rowKernel;         // [V][NRot]
     const int *rowPad;            // [V][NRot]
     int avg;                      // axial average rows (clamped to [0, Nz])
     const int *bview;             // batch slot -> view
     const float *in;              // PRJ: att [B][NRot][Nz][NRot]; BCK: ratio [B][Nz][NRot]
     const float *afac;            // BCK: [V][NRot][Nz][NRot] or null
-    float *out;                   // PRJ: ppart [B][nchunk][Nz][NRot]; BCK: [B][NRot][Nz][NRot]
+    float *out;                   // PRJ: ppart [B][nchunk][Nz][NRot]; BCK: [B][Nz][NRot][NRot]
     int rchunk, nchunk;           // PRJ row chunking
     int xtiles, ztiles;
 };
@@ -274,17 +287,28 @@ __device__ float drf_stage_val(const DrfArgs &a, int b, int v, int row, int g, i
     return s;
 }
 
-template <int KY, bool BCK>
+__host__ __device__ constexpr int drf_col_stride(int TH)
+{
+    // >= TH, multiple of 4 (aligned float4), and 4 mod 32 dwords
+    return ((TH + 27) / 32) * 32 + 4;
+}
+
+template <int KY, int ZT, bool BCK>
 __global__ __launch_bounds__(256) void k_drf(DrfArgs a)
 {
     extern __shared__ float lds[];
     constexpr int CY = (KY - 1) / 2;
+    constexpr int TZ = DRF_ZG * ZT;
+    constexpr int TH = TZ + KY - 1;           // tile rows (slices)
+    constexpr int CS = drf_col_stride(TH);    // LDS column stride
+    constexpr int KYP = (KY + 3) & ~3;        // padded PSF column length
+    constexpr int WIN = ZT + KY - 1;          // input window per thread
+    constexpr int WIN4 = (WIN + 3) / 4;
     const int KX = a.KX;
     const int CX = (KX - 1) / 2;
-    const int TW = DRF_TX + KX - 1;
-    constexpr int TH = DRF_TZ + KY - 1;
-    float *tile = lds;
-    float *psf = lds + TH * TW;
+    const int TW = DRF_TX + KX - 1;           // tile columns (bins)
+    float *tile = lds;                        // [TW][CS]
+    float *psfT = lds + TW * CS;              // [KX][KYP]
 
     int bid = blockIdx.x;
     const int xt = bid % a.xtiles;
@@ -303,70 +327,84 @@ __global__ __launch_bounds__(256) void k_drf(DrfArgs a)
         r1 = min(r0 + a.rchunk, a.NRot);
     }
     const int v = a.bview[b];
-    const int x0 = xt * DRF_TX, z0 = zt * DRF_TZ;
+    const int x0 = xt * DRF_TX, z0 = zt * TZ;
     const int tx = threadIdx.x % DRF_TX;
-    const int zl0 = (threadIdx.x / DRF_TX) * DRF_ZT;
+    const int zl0 = (threadIdx.x / DRF_TX) * ZT;
     const int x = x0 + tx;
-    const long pl = (long)a.Nz * a.NRot;
 
-    float acc[DRF_ZT];
+    float acc[ZT];
 #pragma unroll
-    for (int t = 0; t < DRF_ZT; ++t) acc[t] = 0.0f;
+    for (int t = 0; t < ZT; ++t) acc[t] = 0.0f;
 
     for (int row = r0; row < r1; ++row) {
         const int kidx = a.rowKernel[(long)v * a.NRot + row];
         if (BCK) {
 #pragma unroll
-            for (int t = 0; t < DRF_ZT; ++t) acc[t] = 0.0f;
+            for (int t = 0; t < ZT; ++t) acc[t] = 0.0f;
         }
         if (kidx < 0) {
             if (x < a.NRot) {
 #pragma unroll
-                for (int t = 0; t < DRF_ZT; ++t) {
+                for (int t = 0; t < ZT; ++t) {
                     const int z = z0 + zl0 + t;
                     if (z < a.Nz) acc[t] += drf_in<BCK>(a, b, v, row, z, x);
                 }
             }
         } else {
             const int pad = a.rowPad[(long)v * a.NRot + row];
+            // stage: consecutive threads take consecutive bins (coalesced)
             for (int e = threadIdx.x; e < TH * TW; e += blockDim.x) {
-                const int q = e / TW, c = e % TW;
-                tile[e] = drf_stage_val<BCK>(a, b, v, row, z0 - CY + q, x0 - CX + c, pad);
+                const int q = e / TW, c = e - q * TW;
+                tile[c * CS + q] = drf_stage_val<BCK>(a, b, v, row, z0 - CY + q, x0 - CX + c, pad);
             }
-            const float *kp = a.psf + (long)kidx * KY * KX;
-            for (int e = threadIdx.x; e < KY * KX; e += blockDim.x) psf[e] = kp[e];
+            const float *kp = a.psf + (long)kidx * KX * KYP;
+            for (int e = threadIdx.x; e < KX * KYP; e += blockDim.x) psfT[e] = kp[e];
             __syncthreads();
             for (int i = 0; i < KX; ++i) {
-                float pc[KY];
+                float pc[KYP];
+                const float4 *pp = (const float4 *)(psfT + i * KYP);
 #pragma unroll
-                for (int j = 0; j < KY; ++j) pc[j] = psf[j * KX + i];
-                const float *tcol = tile + zl0 * TW + tx + KX - 1 - i;
+                for (int j4 = 0; j4 < KYP / 4; ++j4) {
+                    const float4 w = pp[j4];
+                    pc[4 * j4] = w.x;
+                    pc[4 * j4 + 1] = w.y;
+                    pc[4 * j4 + 2] = w.z;
+                    pc[4 * j4 + 3] = w.w;
+                }
+                const float4 *tc = (const float4 *)(tile + (tx + KX - 1 - i) * CS + zl0);
 #pragma unroll
-                for (int q = 0; q < DRF_ZT + KY - 1; ++q) {
-                    const float val = tcol[q * TW];
+                for (int q4 = 0; q4 < WIN4; ++q4) {
+                    const float4 w = tc[q4];
+                    const float wv[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
-                    for (int t = 0; t < DRF_ZT; ++t) {
-                        const int j = t + KY - 1 - q;
-                        if (j >= 0 && j < KY) acc[t] = fmaf(pc[j], val, acc[t]);
+                    for (int e = 0; e < 4; ++e) {
+                        const int q = 4 * q4 + e;
+                        if (q >= WIN) continue;
+#pragma unroll
+                        for (int t = 0; t < ZT; ++t) {
+                            const int j = t + KY - 1 - q;
+                            if (j >= 0 && j < KY) acc[t] = fmaf(pc[j], wv[e], acc[t]);
+                        }
                     }
                 }
             }
             __syncthreads();
         }
         if (BCK && x < a.NRot) {
-            float *o = a.out + ((long)b * a.NRot + row) * pl;
 #pragma unroll
-            for (int t = 0; t < DRF_ZT; ++t) {
+            for (int t = 0; t < ZT; ++t) {
                 const int z = z0 + zl0 + t;
-                if (z < a.Nz) o[(long)z * a.NRot + x] = acc[t];
+                if (z < a.Nz)
+                    a.out[(((long)b * a.Nz + z) * a.NRot + row) * a.NRot + x] = acc[t];
             }
         }
     }
     if (!BCK && x < a.NRot) {
+        const long pl = (long)a.Nz * a.NRot;
         const int ch = r0 / a.rchunk;
         float *o = a.out + ((long)b * a.nchunk + ch) * pl;
 #pragma unroll
-        for (int t = 0; t < DRF_ZT; ++t) {
+        for (int t = 0; t < ZT; ++t) {
             const int z = z0 + zl0 + t;
             if (z < a.Nz) o[(long)z * a.NRot + x] = acc[t];
         }
@@ -428,83 +466,61 @@ __global__ void k_prj_finish(RatioArgs a, float *__restrict__ out)
 }
 
 // ---------------------------------------------------------------------------
-// k_bck_update: gather-form back rotation of the batch's convolved planes,
-// in subset order, with the EPS clamp after every view (prjbck.c:408-426),
-// then x *= U * Ninv (osemrecon.c:850-852).  One thread per voxel, so the
+// k_bck: gather-form back rotation of the batch's convolved planes, in
+// subset order, with the EPS clamp after every view (prjbck.c:408-426).
+// One thread per voxel keeps the running sum in a register, so the
 // order-dependent clamp is reproduced exactly and no atomics are needed.
+// The planes are slice-major, so a wave's taps fall inside one contiguous
+// NRot*NRot plane per view.
+//   NORM = false: x *= U * Ninv (osemrecon.c:850-852), U optionally stored.
+//   NORM = true:  FP64 sums, Ninv = U < 1e-6 ? 0 : 1/U (osemrecon.c:317-320).
 // ---------------------------------------------------------------------------
 struct BckArgs {
     int B, N, Nz, NRot;
     const int *bview;
     const float *bsin, *bcos;
-    const float *planes;          // [B][NRot][Nz][NRot]
+    const float *planes;          // [B][Nz][NRot][NRot]
     const int *head, *npix;
     float *x;                     // estimate (updated in place) or null
-    const float *ninv;            // sensitivity inverse or null
+    const float *ninv;            // sensitivity inverse
     float *u;                     // accumulator; read when accumulate != 0
     int accumulate;               // start from u instead of 0
     int writeU;
+    float *ninvOut;               // NORM: output
 };
 
-__global__ void k_bck_update(BckArgs a)
+template <bool NORM>
+__global__ __launch_bounds__(256) void k_bck(BckArgs a)
 {
-    const long M = (long)a.N * a.Nz * a.N;
+    typedef typename std::conditional<NORM, double, float>::type acc_t;
+    const long NN = (long)a.N * a.N;
     const long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (gid >= M) return;
-    const int xc = (int)(gid % a.N);
-    const long r = gid / a.N;
-    const int z = (int)(r % a.Nz);
-    const int y = (int)(r / a.Nz);
+    if (gid >= NN * a.Nz) return;
+    const int z = (int)(gid / NN);
+    const int o = (int)(gid - (long)z * NN);
     const int skip = (a.NRot - a.N) / 2;
-    const int Y = y + skip, X = xc + skip;
+    const int Y = o / a.N + skip, X = o % a.N + skip;
     const float Axis = 0.5 * (a.NRot - 1);
-    const long vol = (long)a.NRot * a.Nz * a.NRot;
     const bool inside = in_circle(a.head, a.npix, Y, X);
     const float Xf = (float)X - Axis, Yf = (float)Y - Axis;
-    float U = a.accumulate ? a.u[gid] : 0.0f;
+    const long PP = (long)a.NRot * a.NRot;
+    acc_t U = (!NORM && a.accumulate) ? (acc_t)a.u[gid] : (acc_t)0;
     for (int b = 0; b < a.B; ++b) {
         const int v = a.bview[b];
         float val = 0.0f;
         if (inside)
-            val = rot_sample(a.planes + (long)b * vol + (long)z * a.NRot,
-                             (long)a.Nz * a.NRot, a.NRot, Xf, Yf, a.bsin[v], a.bcos[v], Axis);
-        U = U + val;
-        if ((double)U < 1.0e-5) U = 0.0f;
+            val = rot_sample(a.planes + ((long)b * a.Nz + z) * PP, a.NRot, a.NRot, Xf, Yf,
+                             a.bsin[v], a.bcos[v], Axis);
+        U = U + (acc_t)val;
+        if ((double)U < 1.0e-5) U = 0;
     }
-    if (a.writeU) a.u[gid] = U;
-    if (a.x) a.x[gid] = a.x[gid] * (U * a.ninv[gid]);
-}
-
-// ---------------------------------------------------------------------------
-// k_norm: FP64 sensitivity accumulation of one subset and inversion with the
-// 1e-6 threshold (osemrecon.c:282-320).
-// ---------------------------------------------------------------------------
-__global__ void k_norm(BckArgs a, float *__restrict__ ninv)
-{
-    const long M = (long)a.N * a.Nz * a.N;
-    const long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (gid >= M) return;
-    const int xc = (int)(gid % a.N);
-    const long r = gid / a.N;
-    const int z = (int)(r % a.Nz);
-    const int y = (int)(r / a.Nz);
-    const int skip = (a.NRot - a.N) / 2;
-    const int Y = y + skip, X = xc + skip;
-    const float Axis = 0.5 * (a.NRot - 1);
-    const long vol = (long)a.NRot * a.Nz * a.NRot;
-    const bool inside = in_circle(a.head, a.npix, Y, X);
-    const float Xf = (float)X - Axis, Yf = (float)Y - Axis;
-    double U = 0.0;
-    for (int b = 0; b < a.B; ++b) {
-        const int v = a.bview[b];
-        double val = 0.0;
-        if (inside)
-            val = rot_sample(a.planes + (long)b * vol + (long)z * a.NRot,
-                             (long)a.Nz * a.NRot, a.NRot, Xf, Yf, a.bsin[v], a.bcos[v], Axis);
-        U += val;
-        if (U < 1.0e-5) U = 0.0;
+    if (NORM) {
+        a.ninvOut[gid] = U < 1e-6 ? 0.0f : (float)(1.0 / (double)U);
+    } else {
+        const float Uf = (float)U;
+        if (a.writeU) a.u[gid] = Uf;
+        if (a.x) a.x[gid] = a.x[gid] * (Uf * a.ninv[gid]);
     }
-    ninv[gid] = U < 1e-6 ? 0.0f : (float)(1.0 / U);
 }
 
 // Sensitivity input: measured-bin mask times view time (osemrecon.c:290-295).
@@ -526,6 +542,8 @@ __global__ void k_norm_input(int B, int Nz, int NRot, const int *__restrict__ bv
 // ---------------------------------------------------------------------------
 // Engine implementation
 // ---------------------------------------------------------------------------
+static int div_up(long a, long b) { return (int)((a + b - 1) / b); }
+
 template <typename T>
 struct DBuf {
     T *p = nullptr;
@@ -626,19 +644,66 @@ struct Recon::Impl {
         fail(ETYPE_ILLEGAL_VALUE, "DRF axial size %d exceeds the supported 63", c.drf.KY);
     }
 
+    void run_att(const float *afac, int B, const int *bview)
+    {
+        const Geometry &g = c.g;
+        hipLaunchKernelGGL(k_att, dim3(div_up(vol * B, 256)), dim3(256), 0, stream, x.p, afac,
+                           att.p, B, g.Nz, g.N, g.NRot, bview, fsin.p, fcos.p, head.p, npix.p);
+    }
+
+    BckArgs bck_args(const int *bview)
+    {
+        const Geometry &g = c.g;
+        BckArgs ba{};
+        ba.B = c.Vs;
+        ba.N = g.N;
+        ba.Nz = g.Nz;
+        ba.NRot = g.NRot;
+        ba.bview = bview;
+        ba.bsin = bsin.p;
+        ba.bcos = bcos.p;
+        ba.planes = planes.p;
+        ba.head = head.p;
+        ba.npix = npix.p;
+        return ba;
+    }
+
+    void run_bck(bool norm, const BckArgs &ba)
+    {
+        dim3 grid(div_up(M, 256)), block(256);
+        if (norm)
+            hipLaunchKernelGGL(k_bck<true>, grid, block, 0, stream, ba);
+        else
+            hipLaunchKernelGGL(k_bck<false>, grid, block, 0, stream, ba);
+    }
+
+    // Slices per thread: 32 when the volume has at least 128 slices, else 16.
+    int drf_zt() const { return c.g.Nz >= 4 * DRF_ZG * 8 ? 32 : 16; }
+
+    int drf_ztiles() const { return div_up(c.g.Nz, DRF_ZG * drf_zt()); }
+
     void run_drf(bool bck, const DrfArgs &args, int nblocks, int KYb)
     {
+        const int ZT = drf_zt();
         const int TW = DRF_TX + args.KX - 1;
-        const int TH = DRF_TZ + KYb - 1;
-        const size_t lds = sizeof(float) * ((size_t)TH * TW + (size_t)KYb * args.KX);
+        const int TH = DRF_ZG * ZT + KYb - 1;
+        const int KYP = (KYb + 3) & ~3;
+        const size_t lds = sizeof(float) * ((size_t)TW * drf_col_stride(TH) + (size_t)args.KX * KYP);
         if (lds > 160 * 1024) fail(ETYPE_ILLEGAL_VALUE, "DRF PSF too wide (KX=%d)", args.KX);
         dim3 grid(nblocks), block(256);
 #define IRL_DRF_CASE(K)                                                                     \
     case K:                                                                                 \
-        if (bck)                                                                            \
-            hipLaunchKernelGGL((k_drf<K, true>), grid, block, lds, stream, args);           \
-        else                                                                                \
-            hipLaunchKernelGGL((k_drf<K, false>), grid, block, lds, stream, args);          \
+        if (ZT == 32) {                                                                     \
+            if (bck)                                                                        \
+                hipLaunchKernelGGL((k_drf<K, 32, true>), grid, block, lds, stream, args);   \
+            else                                                                            \
+                hipLaunchKernelGGL((k_drf<K, 32, false>), grid, block, lds, stream, args);  \
+        } else {                                                                            \
+            if (bck)                                                                        \
+                hipLaunchKernelGGL((k_drf<K, 16, true>), grid, block, lds, stream, args);   \
+            else                                                                            \
+                hipLaunchKernelGGL((k_drf<K, 16, false>), grid, block, lds, stream, args);  \
+        }                                                                                   \
         break;
         switch (KYb) {
             IRL_DRF_CASE(7)
@@ -654,7 +719,6 @@ struct Recon::Impl {
     }
 };
 
-static int div_up(long a, long b) { return (int)((a + b - 1) / b); }
 
 Recon::Recon(const Config &cfg) : impl_(new Impl), cfg_(cfg)
 {
@@ -663,6 +727,14 @@ Recon::Recon(const Config &cfg) : impl_(new Impl), cfg_(cfg)
     const Geometry &g = cfg.g;
     if (g.N <= 0 || g.Nz <= 0 || g.NRot < g.N || g.V <= 0)
         fail(ETYPE_ILLEGAL_VALUE, "bad geometry N=%d Nz=%d NRot=%d V=%d", g.N, g.Nz, g.NRot, g.V);
+    if ((long)g.NRot * g.NRot * sizeof(float) > 160 * 1024)
+        fail(ETYPE_ILLEGAL_VALUE, "NumPixels %d too large: one slice must fit in 160 KiB of LDS",
+             g.NRot);
+    if (g.N % 2)
+        fail(ETYPE_ILLEGAL_VALUE,
+             "odd NumPixels (%d) is not supported: the reference rotation writes past the "
+             "row end for odd sizes",
+             g.N);
     HIP_CHECK(hipStreamCreate(&I.stream));
     I.M = (long)g.N * g.Nz * g.N;
     I.P = (long)g.NRot * g.Nz;
@@ -706,13 +778,16 @@ Recon::Recon(const Config &cfg) : impl_(new Impl), cfg_(cfg)
     up(I.vt, vt);
     up(I.vtsens, vts);
     if (cfg.drf.nDist > 0) {
-        // Device copy padded (centred) to the kernel's axial bucket size.
+        // Device copy centred in the kernel's axial bucket size and stored
+        // column-major ([dist][x][y], y padded to a multiple of 4).
         const int KY = cfg.drf.KY, KX = cfg.drf.KX, KYb = I.ky_bucket(), oy = (KYb - KY) / 2;
-        std::vector<float> padded((size_t)cfg.drf.nDist * KYb * KX, 0.0f);
+        const int KYP = (KYb + 3) & ~3;
+        std::vector<float> padded((size_t)cfg.drf.nDist * KX * KYP, 0.0f);
         for (int d = 0; d < cfg.drf.nDist; ++d)
             for (int j = 0; j < KY; ++j)
-                memcpy(&padded[((size_t)d * KYb + j + oy) * KX],
-                       &cfg.drf.psf[((size_t)d * KY + j) * KX], sizeof(float) * KX);
+                for (int i = 0; i < KX; ++i)
+                    padded[((size_t)d * KX + i) * KYP + j + oy] =
+                        cfg.drf.psf[((size_t)d * KY + j) * KX + i];
         up(I.psf, padded);
     }
     std::vector<int> rk = cfg.rowKernel, rp = cfg.rowPad;
@@ -856,27 +931,14 @@ void Recon::precompute(bool norms)
         da.afac = (cfg_.bckModel & 1) ? I.afac.p : nullptr;
         da.out = I.planes.p;
         da.xtiles = div_up(g.NRot, DRF_TX);
-        da.ztiles = div_up(g.Nz, DRF_TZ);
+        da.ztiles = I.drf_ztiles();
         if (!(cfg_.bckModel & 2)) da.rowKernel = I.rowKernelNone();
         I.launch("drf_bck_norm", [&] {
             I.run_drf(true, da, da.xtiles * da.ztiles * Vs * g.NRot, KYb);
         });
-        BckArgs ba{};
-        ba.B = Vs;
-        ba.N = g.N;
-        ba.Nz = g.Nz;
-        ba.NRot = g.NRot;
-        ba.bview = bview;
-        ba.bsin = I.bsin.p;
-        ba.bcos = I.bcos.p;
-        ba.planes = I.planes.p;
-        ba.head = I.head.p;
-        ba.npix = I.npix.p;
-        float *dst = I.ninv.p + I.M * s;
-        I.launch("norm", [&] {
-            hipLaunchKernelGGL(k_norm, dim3(div_up(I.M, threads)), dim3(threads), 0, I.stream,
-                               ba, dst);
-        });
+        BckArgs ba = I.bck_args(bview);
+        ba.ninvOut = I.ninv.p + I.M * s;
+        I.launch("norm", [&] { I.run_bck(true, ba); });
     }
     synchronize();
 }
@@ -891,11 +953,7 @@ void Recon::subIteration(int s, bool keepUpdate)
     const int KYb = cfg_.drf.nDist > 0 ? I.ky_bucket() : 7;
     const float *afac = (cfg_.prjModel & 1) ? I.afac.p : nullptr;
 
-    I.launch("att", [&] {
-        hipLaunchKernelGGL(k_att, dim3(div_up(I.vol * Vs, threads)), dim3(threads), 0,
-                           I.stream, I.x.p, afac, I.att.p, Vs, g.Nz, g.N, g.NRot, bview,
-                           I.fsin.p, I.fcos.p, I.head.p, I.npix.p);
-    });
+    I.launch("att", [&] { I.run_att(afac, Vs, bview); });
     DrfArgs da{};
     da.NRot = g.NRot;
     da.Nz = g.Nz;
@@ -911,7 +969,7 @@ void Recon::subIteration(int s, bool keepUpdate)
     da.rchunk = I.rchunk;
     da.nchunk = I.nchunk;
     da.xtiles = div_up(g.NRot, DRF_TX);
-    da.ztiles = div_up(g.Nz, DRF_TZ);
+    da.ztiles = I.drf_ztiles();
     I.launch("drf_prj", [&] {
         I.run_drf(false, da, da.xtiles * da.ztiles * Vs * I.nchunk, KYb);
     });
@@ -941,26 +999,13 @@ void Recon::subIteration(int s, bool keepUpdate)
     I.launch("drf_bck", [&] {
         I.run_drf(true, db, db.xtiles * db.ztiles * Vs * g.NRot, KYb);
     });
-    BckArgs ba{};
-    ba.B = Vs;
-    ba.N = g.N;
-    ba.Nz = g.Nz;
-    ba.NRot = g.NRot;
-    ba.bview = bview;
-    ba.bsin = I.bsin.p;
-    ba.bcos = I.bcos.p;
-    ba.planes = I.planes.p;
-    ba.head = I.head.p;
-    ba.npix = I.npix.p;
+    BckArgs ba = I.bck_args(bview);
     ba.x = I.x.p;
     ba.ninv = I.ninv.p + I.M * s;
     ba.u = I.u.p;
     ba.accumulate = 0;
     ba.writeU = keepUpdate ? 1 : 0;
-    I.launch("bck_update", [&] {
-        hipLaunchKernelGGL(k_bck_update, dim3(div_up(I.M, threads)), dim3(threads), 0,
-                           I.stream, ba);
-    });
+    I.launch("bck_update", [&] { I.run_bck(false, ba); });
 }
 
 // Forward projection of the current estimate for every view (IrlGenprj).
@@ -982,11 +1027,7 @@ void Recon::forwardProject(float primary, float *out_host)
     for (int v0 = 0; v0 < g.V; v0 += Vs) {
         const int B = std::min(Vs, g.V - v0);
         const int *bview = order.p + v0;
-        I.launch("att", [&] {
-            hipLaunchKernelGGL(k_att, dim3(div_up(I.vol * B, threads)), dim3(threads), 0,
-                               I.stream, I.x.p, afac, I.att.p, B, g.Nz, g.N, g.NRot, bview,
-                               I.fsin.p, I.fcos.p, I.head.p, I.npix.p);
-        });
+        I.launch("att", [&] { I.run_att(afac, B, bview); });
         DrfArgs da{};
         da.NRot = g.NRot;
         da.Nz = g.Nz;
@@ -1001,7 +1042,7 @@ void Recon::forwardProject(float primary, float *out_host)
         da.rchunk = I.rchunk;
         da.nchunk = I.nchunk;
         da.xtiles = div_up(g.NRot, DRF_TX);
-        da.ztiles = div_up(g.Nz, DRF_TZ);
+        da.ztiles = I.drf_ztiles();
         I.launch("drf_prj", [&] {
             I.run_drf(false, da, da.xtiles * da.ztiles * B * I.nchunk, KYb);
         });

@@ -29,3 +29,14 @@ def test_irlosem_small_matches_reference_semantics(atn, drf):
     e32, e64 = rel_l2(got, ref32), rel_l2(got, ref64)
     print("rel L2 vs f32 %.3g vs f64 %.3g (f32 vs f64 %.3g)" % (e32, e64, rel_l2(ref32, ref64)))
     assert e32 <= 1e-4 and e64 <= 1e-4
+
+
+def test_irlosem_tall_volume_uses_wide_tiles():
+    # Nz >= 128 selects 32 slices per thread in the DRF kernels
+    print("case: N=64 Nz=128 V=8 atn+drf", flush=True)
+    pb = Problem(n=64, nz=128, nviews=8, per=2, iters=1, half=7, pix=0.4)
+    ref32 = pb.oracle_osem("f32")
+    got = _gpu_osem(pb)
+    e = rel_l2(got, ref32)
+    print("rel L2 vs f32 %.3g" % e)
+    assert e <= 1e-4
