@@ -88,6 +88,44 @@ class AdvOptions(ctypes.Structure):
     _fields_ = [("bFastRotate", ctypes.c_int), ("iStartIteration", ctypes.c_int)]
 
 
+class Drf(ctypes.Structure):
+    """Drf_t (drftabio.h:7-12)."""
+    _fields_ = [("MaxOffsX", ctypes.c_int), ("MaxOffsY", ctypes.c_int),
+                ("bFFTdone", ctypes.c_int), ("Psf", ctypes.POINTER(ctypes.c_float))]
+
+
+class DrfTab(ctypes.Structure):
+    """DrfTab_t (drftabio.h:17-28)."""
+
+
+DrfTab._fields_ = [
+    ("iNumDistances", ctypes.c_int), ("fStartDistance", ctypes.c_float),
+    ("fDrfSpacing", ctypes.c_float), ("fPixSize", ctypes.c_float), ("bIncKrnl", ctypes.c_int),
+    ("bDrfFromFile", ctypes.c_int), ("psBckDrfTab", ctypes.POINTER(DrfTab)),
+    ("fGap", ctypes.c_float), ("fHoleDiam", ctypes.c_float), ("fColThick", ctypes.c_float),
+    ("fIntrinsicFWHM", ctypes.c_float), ("fMaxFracErr", ctypes.c_float),
+    ("psDrfs", ctypes.POINTER(Drf)), ("bFFTconvolve", ctypes.c_int)]
+
+
+def make_drftab(start, spacing, offs, psfs, fft=False, pixsize=None):
+    """DrfTab_t holding the given PSFs (kept alive by the returned object)."""
+    t = DrfTab()
+    keep = [np.ascontiguousarray(p, np.float32) for p in psfs]
+    drfs = (Drf * len(psfs))()
+    for i, ((ox, oy), p) in enumerate(zip(offs, keep)):
+        assert p.shape == (2 * oy - 1, 2 * ox - 1)
+        drfs[i].MaxOffsX, drfs[i].MaxOffsY, drfs[i].bFFTdone = ox, oy, 0
+        drfs[i].Psf = p.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    t.iNumDistances = len(psfs)
+    t.fStartDistance = start
+    t.fDrfSpacing = spacing
+    t.fPixSize = spacing if pixsize is None else pixsize
+    t.psDrfs = drfs
+    t.bFFTconvolve = 1 if fft else 0
+    t._keep = (keep, drfs)
+    return t
+
+
 ITER_CB = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.POINTER(ctypes.c_float))
 HOOK_CB = ctypes.CFUNCTYPE(None, ctypes.POINTER(OsemData), ctypes.c_void_p)
 _FP = ctypes.POINTER(ctypes.c_float)
@@ -140,6 +178,20 @@ def load():
                                        ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(ctypes.c_int)]
     lib.irlamd_recon_destroy.argtypes = [ctypes.c_void_p]
+    lib.InitPrj.argtypes = [ctypes.c_int] * 4
+    vp = ctypes.c_void_p
+    lib.Project.argtypes = ([ctypes.c_int] * 4 + [ctypes.c_float, ctypes.c_int, ctypes.c_float,
+                                                 ctypes.c_float, vp, vp, _FP, _FP, _FP,
+                                                 ctypes.c_float, ctypes.c_float] +
+                            [ctypes.c_int] * 5)
+    lib.BackProject.argtypes = ([ctypes.c_int] * 4 + [ctypes.c_float, ctypes.c_int,
+                                                     ctypes.c_float, ctypes.c_float, vp, vp, _FP,
+                                                     _FP, _FP, ctypes.c_float, ctypes.c_float] +
+                                [ctypes.c_int] * 4)
+    lib.rot3dpar.argtypes = [ctypes.c_float, _FP, _FP] + [ctypes.c_int] * 4
+    lib.ConvolveDrfYpad.argtypes = [_FP, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Drf)] + \
+        [ctypes.c_int] * 3 + [_FP]
+    lib.irlamd_last_error.restype = ctypes.c_int
     _lib = lib
     return lib
 
@@ -197,6 +249,59 @@ def irl_genprj(parms, options, views, drf_file, krnl_file, scat, act, atn, prj,
     return lib.IrlGenprj(ctypes.byref(parms), ctypes.byref(options), vv, _bytes(drf_file),
                          _bytes(krnl_file), cb, fptr(scat), fptr(act), fptr(atn), fptr(prj),
                          primary, _bytes(log_file), _bytes(msg_file))
+
+
+def _tab(t):
+    return None if t is None else ctypes.cast(ctypes.pointer(t), ctypes.c_void_p)
+
+
+def init_prj(nz, n, nrot, model):
+    load().InitPrj(nz, n, nrot, model)
+
+
+def free_prj():
+    load().FreePrj()
+
+
+def project(nz, n, nrot, model, pw, angle, cfcr, drftab, atn, recon, primary=1.0, atnfac=1.0,
+            pad=0, avg=0, atn_rotated=0, act_rotated=0):
+    """Project (prjbck.c:68-252) of internal-order [y][z][x] float32 volumes;
+    returns the [slice][bin] projection."""
+    out = np.zeros((nz, nrot), np.float32)
+    load().Project(nz, n, nrot, model, pw, 0, angle, cfcr, _tab(drftab), None, fptr(atn),
+                   fptr(recon), fptr(out), primary, atnfac, pad, avg, atn_rotated,
+                   act_rotated, 0)
+    return out
+
+
+def backproject(nz, n, nrot, model, pw, angle, cfcr, drftab, atn, ratio, image, primary=1.0,
+                atnfac=1.0, pad=0, avg=0, atn_rotated=0):
+    """BackProject (prjbck.c:254-427): adds into 'image' in place."""
+    load().BackProject(nz, n, nrot, model, pw, 0, angle, cfcr, _tab(drftab), None, fptr(atn),
+                       fptr(ratio), fptr(image), primary, atnfac, pad, avg, atn_rotated, 0)
+    return image
+
+
+def rot3dpar(angle, vol, pb_flag, nz, n, nrot):
+    out = np.zeros((nrot, nz, nrot), np.float32)
+    load().rot3dpar(angle, fptr(np.ascontiguousarray(vol, np.float32)), fptr(out), pb_flag, nz,
+                    n, nrot)
+    return out
+
+
+def convolve_drf_ypad(plane, psf, fft=0, pad=0, avg=0):
+    psf = np.ascontiguousarray(psf, np.float32)
+    d = Drf((psf.shape[1] + 1) // 2, (psf.shape[0] + 1) // 2, 0,
+            psf.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    plane = np.ascontiguousarray(plane, np.float32)
+    out = np.zeros_like(plane)
+    load().ConvolveDrfYpad(fptr(plane), plane.shape[1], plane.shape[0], ctypes.byref(d), fft,
+                           pad, avg, fptr(out))
+    return out
+
+
+def last_error():
+    return load().irlamd_last_error()
 
 
 def set_adv_options(fast_rotate=1, start_iteration=0):

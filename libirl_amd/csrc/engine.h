@@ -126,6 +126,27 @@ private:
     Config cfg_;
 };
 
+// One view of the projector-level entry points.  Tables come from the host
+// layer, computed with the reference's float arithmetic.
+struct PrjCall {
+    int Nz, N, NRot, model;
+    float fsin, fcos, bsin, bcos;  // rotation constants of this view
+    float primary, atnFac;
+    int avg;
+    int atnRotated, actRotated;
+    const int *rowKernel;          // NRot entries: DRF index or -1
+    const int *rowPad;             // NRot entries
+    const void *drfKey;            // identity of the DRF table (cache key)
+    const PaddedDrf *drf;          // null when DRF is not modelled
+};
+
+void prjws_init(int Nz, int N, int NRot, int model);
+void prjws_free();
+void prjws_project(const PrjCall &c, const float *atn, const float *recon, float *calcprj);
+void prjws_backproject(const PrjCall &c, const float *atn, const float *ratio, float *image);
+void dev_convolve_plane(const float *in, int inX, int inY, const PaddedDrf &d, int pad, int avg,
+                        float *out);
+
 // Stateless single-call helpers behind the prjbck-level C entry points.
 void dev_rot3dpar(float angle, const float *in_host, float *out_host, int pbFlag,
                   int Nz, int N, int NRot);

@@ -242,7 +242,9 @@ constexpr int DRF_TX = 64;
 constexpr int DRF_ZG = 4;
 
 struct DrfArgs {
-    int NRot, Nz;
+    int NRot, Nz;                 // plane width (bins) and height (slices)
+    int nrows;                    // planes (rows) per batch item
+    int bckRowMajor;              // BCK output [B][rows][Nz][NRot] instead of slice-major
     int KX;                       // padded PSF width (odd)
     const float *psf;             // [nDist][KX][KYP] transposed, KYP = bucket padded to 4
     const int *rowKernel;         // [V][NRot]
@@ -250,7 +252,8 @@ struct DrfArgs {
     int avg;                      // axial average rows (clamped to [0, Nz])
     const int *bview;             // batch slot -> view
     const float *in;              // PRJ: att [B][NRot][Nz][NRot]; BCK: ratio [B][Nz][NRot]
-    const float *afac;            // BCK: [V][NRot][Nz][NRot] or null
+    const float *afac;            // [V][rows][Nz][NRot] factors multiplied into the input, or null
+    float scale;                  // BCK: output scale (fScaleFac)
     float *out;                   // PRJ: ppart [B][nchunk][Nz][NRot]; BCK: [B][Nz][NRot][NRot]
     int rchunk, nchunk;           // PRJ row chunking
     int xtiles, ztiles;
@@ -263,10 +266,12 @@ __device__ __forceinline__ float drf_in(const DrfArgs &a, int b, int v, int row,
     const long pl = (long)a.Nz * a.NRot;
     if (BCK) {
         float r = a.in[(long)b * pl + (long)g * a.NRot + gx];
-        if (a.afac) r = r * a.afac[((long)v * a.NRot + row) * pl + (long)g * a.NRot + gx];
+        if (a.afac) r = r * a.afac[((long)v * a.nrows + row) * pl + (long)g * a.NRot + gx];
         return r;
     } else {
-        return a.in[((long)b * a.NRot + row) * pl + (long)g * a.NRot + gx];
+        float r = a.in[((long)b * a.nrows + row) * pl + (long)g * a.NRot + gx];
+        if (a.afac) r = r * a.afac[((long)v * a.nrows + row) * pl + (long)g * a.NRot + gx];
+        return r;
     }
 }
 
@@ -317,14 +322,14 @@ __global__ __launch_bounds__(256) void k_drf(DrfArgs a)
     bid /= a.ztiles;
     int b, r0, r1;
     if (BCK) {
-        b = bid / a.NRot;
-        r0 = bid % a.NRot;
+        b = bid / a.nrows;
+        r0 = bid % a.nrows;
         r1 = r0 + 1;
     } else {
         b = bid / a.nchunk;
         const int ch = bid % a.nchunk;
         r0 = ch * a.rchunk;
-        r1 = min(r0 + a.rchunk, a.NRot);
+        r1 = min(r0 + a.rchunk, a.nrows);
     }
     const int v = a.bview[b];
     const int x0 = xt * DRF_TX, z0 = zt * TZ;
@@ -337,7 +342,7 @@ __global__ __launch_bounds__(256) void k_drf(DrfArgs a)
     for (int t = 0; t < ZT; ++t) acc[t] = 0.0f;
 
     for (int row = r0; row < r1; ++row) {
-        const int kidx = a.rowKernel[(long)v * a.NRot + row];
+        const int kidx = a.rowKernel[(long)v * a.nrows + row];
         if (BCK) {
 #pragma unroll
             for (int t = 0; t < ZT; ++t) acc[t] = 0.0f;
@@ -351,7 +356,7 @@ __global__ __launch_bounds__(256) void k_drf(DrfArgs a)
                 }
             }
         } else {
-            const int pad = a.rowPad[(long)v * a.NRot + row];
+            const int pad = a.rowPad[(long)v * a.nrows + row];
             // stage: consecutive threads take consecutive bins (coalesced)
             for (int e = threadIdx.x; e < TH * TW; e += blockDim.x) {
                 const int q = e / TW, c = e - q * TW;
@@ -395,7 +400,9 @@ __global__ __launch_bounds__(256) void k_drf(DrfArgs a)
             for (int t = 0; t < ZT; ++t) {
                 const int z = z0 + zl0 + t;
                 if (z < a.Nz)
-                    a.out[(((long)b * a.Nz + z) * a.NRot + row) * a.NRot + x] = acc[t];
+                    a.out[a.bckRowMajor ? (((long)b * a.nrows + row) * a.Nz + z) * a.NRot + x
+                                        : (((long)b * a.Nz + z) * a.nrows + row) * a.NRot + x] =
+                        acc[t] * a.scale;
             }
         }
     }
@@ -544,6 +551,9 @@ __global__ void k_norm_input(int B, int Nz, int NRot, const int *__restrict__ bv
 // ---------------------------------------------------------------------------
 static int div_up(long a, long b) { return (int)((a + b - 1) / b); }
 
+static void launch_drf(hipStream_t stream, bool bck, const DrfArgs &args, int nblocks, int KYb,
+                       int ZT);
+
 template <typename T>
 struct DBuf {
     T *p = nullptr;
@@ -684,38 +694,7 @@ struct Recon::Impl {
 
     void run_drf(bool bck, const DrfArgs &args, int nblocks, int KYb)
     {
-        const int ZT = drf_zt();
-        const int TW = DRF_TX + args.KX - 1;
-        const int TH = DRF_ZG * ZT + KYb - 1;
-        const int KYP = (KYb + 3) & ~3;
-        const size_t lds = sizeof(float) * ((size_t)TW * drf_col_stride(TH) + (size_t)args.KX * KYP);
-        if (lds > 160 * 1024) fail(ETYPE_ILLEGAL_VALUE, "DRF PSF too wide (KX=%d)", args.KX);
-        dim3 grid(nblocks), block(256);
-#define IRL_DRF_CASE(K)                                                                     \
-    case K:                                                                                 \
-        if (ZT == 32) {                                                                     \
-            if (bck)                                                                        \
-                hipLaunchKernelGGL((k_drf<K, 32, true>), grid, block, lds, stream, args);   \
-            else                                                                            \
-                hipLaunchKernelGGL((k_drf<K, 32, false>), grid, block, lds, stream, args);  \
-        } else {                                                                            \
-            if (bck)                                                                        \
-                hipLaunchKernelGGL((k_drf<K, 16, true>), grid, block, lds, stream, args);   \
-            else                                                                            \
-                hipLaunchKernelGGL((k_drf<K, 16, false>), grid, block, lds, stream, args);  \
-        }                                                                                   \
-        break;
-        switch (KYb) {
-            IRL_DRF_CASE(7)
-            IRL_DRF_CASE(15)
-            IRL_DRF_CASE(23)
-            IRL_DRF_CASE(31)
-            IRL_DRF_CASE(47)
-            IRL_DRF_CASE(63)
-        default:
-            fail(ETYPE_ILLEGAL_VALUE, "unsupported DRF bucket %d", KYb);
-        }
-#undef IRL_DRF_CASE
+        launch_drf(stream, bck, args, nblocks, KYb, drf_zt());
     }
 };
 
@@ -920,6 +899,8 @@ void Recon::precompute(bool norms)
         });
         DrfArgs da{};
         da.NRot = g.NRot;
+        da.nrows = g.NRot;
+        da.scale = 1.0f;
         da.Nz = g.Nz;
         da.KX = cfg_.drf.nDist > 0 ? cfg_.drf.KX : 1;
         da.psf = I.psf.p;
@@ -956,6 +937,8 @@ void Recon::subIteration(int s, bool keepUpdate)
     I.launch("att", [&] { I.run_att(afac, Vs, bview); });
     DrfArgs da{};
     da.NRot = g.NRot;
+    da.nrows = g.NRot;
+    da.scale = 1.0f;
     da.Nz = g.Nz;
     da.KX = cfg_.drf.nDist > 0 ? cfg_.drf.KX : 1;
     da.psf = I.psf.p;
@@ -1030,6 +1013,8 @@ void Recon::forwardProject(float primary, float *out_host)
         I.launch("att", [&] { I.run_att(afac, B, bview); });
         DrfArgs da{};
         da.NRot = g.NRot;
+        da.nrows = g.NRot;
+        da.scale = 1.0f;
         da.Nz = g.Nz;
         da.KX = cfg_.drf.nDist > 0 ? cfg_.drf.KX : 1;
         da.psf = I.psf.p;
@@ -1063,6 +1048,338 @@ void Recon::forwardProject(float primary, float *out_host)
     }
     out.download(out_host, I.P * g.V, I.stream);
     synchronize();
+}
+
+
+// ---------------------------------------------------------------------------
+// Projector-level entry points (prjbck.c semantics, one view per call).
+// Volumes are the reference's internal [y][z][x] order here, because the
+// callers hand over host arrays in that order and the workspace contents
+// (rotated estimate, rotated attenuation map) are part of the contract.
+// ---------------------------------------------------------------------------
+
+// Attenuation factors of one view from an already rotated mu:
+// afac[row][z][x] = exp(-(sum_{r<row} mu + mu_row/2) * atnFac).
+__global__ void k_afac_rotated(const float *__restrict__ rmu, float *__restrict__ afac, int Nz,
+                               int NRot, float atnFac)
+{
+    const long cols = (long)Nz * NRot;
+    const long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (c >= cols) return;
+    float S = 0.0f;
+    for (int Y = 0; Y < NRot; ++Y) {
+        const float m = rmu[(long)Y * cols + c];
+        S = (float)((double)S + 0.5 * (double)m);
+        afac[(long)Y * cols + c] = (float)exp((double)(-S * atnFac));
+        S = (float)((double)S + 0.5 * (double)m);
+    }
+}
+
+__global__ void k_sum_chunks(const float *__restrict__ ppart, float *__restrict__ out,
+                             long pl, int nchunk, float scale)
+{
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= pl) return;
+    float p = 0.0f;
+    for (int c = 0; c < nchunk; ++c) p += ppart[(long)c * pl + i];
+    out[i] = p * scale;
+}
+
+// imagematrix += crop(unrot), then the EPS clamp over the whole image
+// (prjbck.c:418-426).
+__global__ void k_acc_clamp(float *__restrict__ img, const float *__restrict__ unrot, int N,
+                            int Nz, int NRot)
+{
+    const long M = (long)N * Nz * N;
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    const int c = (int)(i % N);
+    const long r = i / N;
+    const int sl = (int)(r % Nz);
+    const int y = (int)(r / Nz);
+    const int skip = (NRot - N) / 2;
+    float v = img[i] + unrot[((long)(y + skip) * Nz + sl) * NRot + c + skip];
+    if ((double)v < 1.0e-5) v = 0.0f;
+    img[i] = v;
+}
+
+struct PrjWs {
+    int Nz = 0, N = 0, NRot = 0, model = 0;
+    hipStream_t stream = nullptr;
+    DBuf<float> rotRecon, rotAtn, unrot, vol, afac, ppart, plane, img, psf;
+    DBuf<int> rowKernel, rowPad, zero, head, npix;
+    const void *drfKey = nullptr;
+    int drfKX = 1, drfKYb = 7;
+    std::vector<int> hhead, hnpix;
+
+    PrjWs(int nz, int n, int nrot, int m) : Nz(nz), N(n), NRot(nrot), model(m)
+    {
+        HIP_CHECK(hipStreamCreate(&stream));
+        const long v = (long)NRot * Nz * NRot;
+        rotRecon.alloc(v);
+        rotAtn.alloc(v);
+        unrot.alloc(v);
+        vol.alloc(v);
+        afac.alloc(v);
+        ppart.alloc((long)Nz * NRot * NRot);
+        plane.alloc((long)Nz * NRot);
+        img.alloc((long)N * Nz * N);
+        rowKernel.alloc(NRot);
+        rowPad.alloc(NRot);
+        int z0 = 0;
+        zero.alloc(1);
+        zero.upload(&z0, 1, stream);
+        circle_rows(NRot, hhead, hnpix);
+        head.alloc(NRot);
+        head.upload(hhead.data(), NRot, stream);
+        npix.alloc(NRot);
+        npix.upload(hnpix.data(), NRot, stream);
+        HIP_CHECK(hipMemsetAsync(rotRecon.p, 0, v * sizeof(float), stream));
+        HIP_CHECK(hipMemsetAsync(rotAtn.p, 0, v * sizeof(float), stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+    }
+    ~PrjWs()
+    {
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
+    }
+
+    static int bucket(int KY)
+    {
+        static const int sizes[] = {7, 15, 23, 31, 47, 63};
+        for (int s : sizes)
+            if (KY <= s) return s;
+        fail(ETYPE_ILLEGAL_VALUE, "DRF axial size %d exceeds the supported 63", KY);
+    }
+
+    void set_drf(const void *key, const PaddedDrf &d)
+    {
+        if (key == drfKey && key != nullptr) return;
+        const int KYb = bucket(d.KY), KYP = (KYb + 3) & ~3, oy = (KYb - d.KY) / 2;
+        std::vector<float> h((size_t)d.nDist * d.KX * KYP, 0.0f);
+        for (int k = 0; k < d.nDist; ++k)
+            for (int j = 0; j < d.KY; ++j)
+                for (int i = 0; i < d.KX; ++i)
+                    h[((size_t)k * d.KX + i) * KYP + j + oy] = d.psf[((size_t)k * d.KY + j) * d.KX + i];
+        psf.alloc(h.size());
+        psf.upload(h.data(), h.size(), stream);
+        drfKey = key;
+        drfKX = d.KX;
+        drfKYb = KYb;
+    }
+
+    void rotate(const float *src, float *dst, float sn, float cs, int oldDim)
+    {
+        const long nout = (long)NRot * Nz * NRot;
+        const float NewAxis = 0.5 * (NRot - 1), OldAxis = 0.5 * (oldDim - 1);
+        hipLaunchKernelGGL(k_rot3dpar, dim3(div_up(nout, 256)), dim3(256), 0, stream, src, dst,
+                           Nz, oldDim, NRot, sn, cs, NewAxis, OldAxis, head.p, npix.p);
+        HIP_CHECK(hipGetLastError());
+    }
+
+    DrfArgs drf_args(const int *rk, const int *rp, int avg)
+    {
+        DrfArgs da{};
+        da.NRot = NRot;
+        da.nrows = NRot;
+        da.Nz = Nz;
+        da.KX = drfKX;
+        da.psf = psf.p;
+        da.rowKernel = rk;
+        da.rowPad = rp;
+        da.avg = std::max(0, std::min(avg, Nz));
+        da.bview = zero.p;
+        da.scale = 1.0f;
+        da.xtiles = div_up(NRot, DRF_TX);
+        return da;
+    }
+};
+
+static PrjWs *g_prj = nullptr;
+
+void prjws_init(int Nz, int N, int NRot, int model)
+{
+    delete g_prj;
+    g_prj = nullptr;
+    if (N <= 0 || Nz <= 0 || NRot < N)
+        fail(ETYPE_ILLEGAL_VALUE, "InitPrj: bad sizes %d %d %d", Nz, N, NRot);
+    g_prj = new PrjWs(Nz, N, NRot, model);
+}
+
+void prjws_free()
+{
+    delete g_prj;
+    g_prj = nullptr;
+}
+
+static PrjWs &ws(int Nz, int N, int NRot)
+{
+    if (!g_prj) fail(ETYPE_ILLEGAL_VALUE, "InitPrj must be called before Project/BackProject");
+    if (g_prj->Nz != Nz || g_prj->N != N || g_prj->NRot != NRot)
+        fail(ETYPE_ILLEGAL_VALUE, "sizes differ from those passed to InitPrj");
+    return *g_prj;
+}
+
+static int zt_for(int Nz) { return Nz >= 4 * DRF_ZG * 8 ? 32 : 16; }
+
+void prjws_project(const PrjCall &c, const float *atn, const float *recon, float *calcprj)
+{
+    PrjWs &W = ws(c.Nz, c.N, c.NRot);
+    const long nin = (long)c.N * c.Nz * c.N;
+    const bool useAtn = (c.model & 1) || (c.model & 4);
+    if (c.model & 4) fail(ETYPE_ILLEGAL_VALUE, "ESSE scatter modelling is not available in this build");
+    if (!c.actRotated) {
+        W.vol.upload(recon, nin, W.stream);
+        W.rotate(W.vol.p, W.rotRecon.p, c.fsin, c.fcos, c.N);
+    }
+    if (useAtn && !c.atnRotated) {
+        W.vol.upload(atn, nin, W.stream);
+        W.rotate(W.vol.p, W.rotAtn.p, c.fsin, c.fcos, c.N);
+    }
+    const long cols = (long)c.Nz * c.NRot;
+    if (c.model & 1)
+        hipLaunchKernelGGL(k_afac_rotated, dim3(div_up(cols, 256)), dim3(256), 0, W.stream,
+                           W.rotAtn.p, W.afac.p, c.Nz, c.NRot, c.atnFac);
+    if (c.drf) W.set_drf(c.drfKey, *c.drf);
+    W.rowKernel.upload(c.rowKernel, c.NRot, W.stream);
+    W.rowPad.upload(c.rowPad, c.NRot, W.stream);
+    DrfArgs da = W.drf_args(W.rowKernel.p, W.rowPad.p, c.avg);
+    const int ZT = zt_for(c.Nz);
+    da.ztiles = div_up(c.Nz, DRF_ZG * ZT);
+    da.in = W.rotRecon.p;
+    da.afac = (c.model & 1) ? W.afac.p : nullptr;
+    da.out = W.ppart.p;
+    da.rchunk = 1;
+    da.nchunk = c.NRot;
+    launch_drf(W.stream, false, da, da.xtiles * da.ztiles * c.NRot, W.drfKYb, ZT);
+    hipLaunchKernelGGL(k_sum_chunks, dim3(div_up(cols, 256)), dim3(256), 0, W.stream, W.ppart.p,
+                       W.plane.p, cols, c.NRot, c.primary);
+    HIP_CHECK(hipGetLastError());
+    W.plane.download(calcprj, cols, W.stream);
+    HIP_CHECK(hipStreamSynchronize(W.stream));
+}
+
+void prjws_backproject(const PrjCall &c, const float *atn, const float *ratio, float *image)
+{
+    PrjWs &W = ws(c.Nz, c.N, c.NRot);
+    const long nin = (long)c.N * c.Nz * c.N;
+    const bool useAtn = (c.model & 1) || (c.model & 4);
+    if (c.model & 4) fail(ETYPE_ILLEGAL_VALUE, "ESSE scatter modelling is not available in this build");
+    if (useAtn && !c.atnRotated) {
+        W.vol.upload(atn, nin, W.stream);
+        W.rotate(W.vol.p, W.rotAtn.p, c.fsin, c.fcos, c.N);
+    }
+    const long cols = (long)c.Nz * c.NRot;
+    if (c.model & 1)
+        hipLaunchKernelGGL(k_afac_rotated, dim3(div_up(cols, 256)), dim3(256), 0, W.stream,
+                           W.rotAtn.p, W.afac.p, c.Nz, c.NRot, c.atnFac);
+    if (c.drf) W.set_drf(c.drfKey, *c.drf);
+    W.rowKernel.upload(c.rowKernel, c.NRot, W.stream);
+    W.rowPad.upload(c.rowPad, c.NRot, W.stream);
+    W.plane.upload(ratio, cols, W.stream);
+    DrfArgs da = W.drf_args(W.rowKernel.p, W.rowPad.p, c.avg);
+    const int ZT = zt_for(c.Nz);
+    da.ztiles = div_up(c.Nz, DRF_ZG * ZT);
+    da.in = W.plane.p;
+    da.afac = (c.model & 1) ? W.afac.p : nullptr;
+    da.out = W.rotRecon.p;   // the reference backprojects into RotReconImage
+    da.bckRowMajor = 1;
+    da.scale = c.primary;
+    launch_drf(W.stream, true, da, da.xtiles * da.ztiles * c.NRot, W.drfKYb, ZT);
+    W.rotate(W.rotRecon.p, W.unrot.p, c.bsin, c.bcos, c.NRot);
+    W.img.upload(image, nin, W.stream);
+    hipLaunchKernelGGL(k_acc_clamp, dim3(div_up(nin, 256)), dim3(256), 0, W.stream, W.img.p,
+                       W.unrot.p, c.N, c.Nz, c.NRot);
+    HIP_CHECK(hipGetLastError());
+    W.img.download(image, nin, W.stream);
+    HIP_CHECK(hipStreamSynchronize(W.stream));
+}
+
+// ConvolveDrfYpad on one plane (convolve_ypad.c:22-96).
+void dev_convolve_plane(const float *in, int inX, int inY, const PaddedDrf &d, int pad, int avg,
+                        float *out)
+{
+    hipStream_t st;
+    HIP_CHECK(hipStreamCreate(&st));
+    const int KYb = PrjWs::bucket(d.KY), KYP = (KYb + 3) & ~3, oy = (KYb - d.KY) / 2;
+    std::vector<float> h((size_t)d.KX * KYP, 0.0f);
+    for (int j = 0; j < d.KY; ++j)
+        for (int i = 0; i < d.KX; ++i) h[(size_t)i * KYP + j + oy] = d.psf[(size_t)j * d.KX + i];
+    DBuf<float> dpsf, din, dout;
+    DBuf<int> rk, rp, zero;
+    dpsf.alloc(h.size());
+    dpsf.upload(h.data(), h.size(), st);
+    const long pl = (long)inX * inY;
+    din.alloc(pl);
+    din.upload(in, pl, st);
+    dout.alloc(pl);
+    int k0 = 0, p0 = pad;
+    rk.alloc(1);
+    rk.upload(&k0, 1, st);
+    rp.alloc(1);
+    rp.upload(&p0, 1, st);
+    zero.alloc(1);
+    zero.upload(&k0, 1, st);
+    DrfArgs da{};
+    da.NRot = inX;
+    da.nrows = 1;
+    da.Nz = inY;
+    da.KX = d.KX;
+    da.psf = dpsf.p;
+    da.rowKernel = rk.p;
+    da.rowPad = rp.p;
+    da.avg = std::max(0, std::min(avg, inY));
+    da.bview = zero.p;
+    da.in = din.p;
+    da.out = dout.p;
+    da.scale = 1.0f;
+    da.bckRowMajor = 1;
+    const int ZT = zt_for(inY);
+    da.xtiles = div_up(inX, DRF_TX);
+    da.ztiles = div_up(inY, DRF_ZG * ZT);
+    launch_drf(st, true, da, da.xtiles * da.ztiles, KYb, ZT);
+    dout.download(out, pl, st);
+    HIP_CHECK(hipStreamSynchronize(st));
+    HIP_CHECK(hipStreamDestroy(st));
+}
+
+static void launch_drf(hipStream_t stream, bool bck, const DrfArgs &args, int nblocks, int KYb,
+                       int ZT)
+{
+    const int TW = DRF_TX + args.KX - 1;
+    const int TH = DRF_ZG * ZT + KYb - 1;
+    const int KYP = (KYb + 3) & ~3;
+    const size_t lds = sizeof(float) * ((size_t)TW * drf_col_stride(TH) + (size_t)args.KX * KYP);
+    if (lds > 160 * 1024) fail(ETYPE_ILLEGAL_VALUE, "DRF PSF too wide (KX=%d)", args.KX);
+    dim3 grid(nblocks), block(256);
+#define IRL_DRF_CASE(K)                                                                     \
+    case K:                                                                                 \
+        if (ZT == 32) {                                                                     \
+            if (bck)                                                                        \
+                hipLaunchKernelGGL((k_drf<K, 32, true>), grid, block, lds, stream, args);   \
+            else                                                                            \
+                hipLaunchKernelGGL((k_drf<K, 32, false>), grid, block, lds, stream, args);  \
+        } else {                                                                            \
+            if (bck)                                                                        \
+                hipLaunchKernelGGL((k_drf<K, 16, true>), grid, block, lds, stream, args);   \
+            else                                                                            \
+                hipLaunchKernelGGL((k_drf<K, 16, false>), grid, block, lds, stream, args);  \
+        }                                                                                   \
+        break;
+    switch (KYb) {
+        IRL_DRF_CASE(7)
+        IRL_DRF_CASE(15)
+        IRL_DRF_CASE(23)
+        IRL_DRF_CASE(31)
+        IRL_DRF_CASE(47)
+        IRL_DRF_CASE(63)
+    default:
+        fail(ETYPE_ILLEGAL_VALUE, "unsupported DRF bucket %d", KYb);
+    }
+#undef IRL_DRF_CASE
+    HIP_CHECK(hipGetLastError());
 }
 
 // ---------------------------------------------------------------------------

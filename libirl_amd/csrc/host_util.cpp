@@ -15,11 +15,14 @@ static int g_msgLevel = 4;
 static FILE *g_msgFile = nullptr;
 static FILE *g_logFile = nullptr;
 static std::string g_errString;
+static int g_lastErr = 0;
 
 void set_msg_level(int level) { g_msgLevel = level; }
 void set_msg_file(FILE *fp) { g_msgFile = fp; }
 void set_log_file(FILE *fp) { g_logFile = fp; }
 void set_error_string(const std::string &s) { g_errString = s; }
+void set_last_error(int code) { g_lastErr = code; }
+int last_error() { return g_lastErr; }
 const std::string &error_string() { return g_errString; }
 
 void msg(int level, const char *fmt, ...)

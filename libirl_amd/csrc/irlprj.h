@@ -22,6 +22,8 @@ void set_msg_level(int level);
 void set_msg_file(FILE *fp);
 void set_log_file(FILE *fp);
 void set_error_string(const std::string &s);
+void set_last_error(int code);
+int last_error();
 
 // Reference-arithmetic host geometry.
 int drf_row_index(const DrfTab_t *t, float fCFCR, float fPixelWidth, int iNumRotPixs, int iRow,

@@ -1,0 +1,197 @@
+// prjbck_api.cpp -- projector-level C entry points (include/irlprj.h).
+//
+// InitPrj / Project / BackProject / FreePrj keep the workspace semantics of
+// prjbck.c:29-453; rot3dpar and ConvolveDrfYpad run one rotation or one plane
+// convolution on the GPU.  The per-row DRF plane indices are computed here
+// with the reference's float arithmetic (prjbck.c:186-204, 343-360).
+#include <algorithm>
+#include <cstring>
+#include <map>
+
+#include "../../include/irl_amd.h"
+#include "irlprj.h"
+
+using namespace irlamd;
+
+namespace {
+
+struct CachedDrf {
+    PaddedDrf pad;
+    int nDist;
+    Drf_t *drfs;
+};
+
+std::map<const DrfTab_t *, CachedDrf> g_drfCache;
+
+const PaddedDrf &cached_drf(const DrfTab_t *t)
+{
+    auto it = g_drfCache.find(t);
+    if (it != g_drfCache.end() && it->second.nDist == t->iNumDistances &&
+        it->second.drfs == t->psDrfs)
+        return it->second.pad;
+    CachedDrf c{pad_drf_table(t), t->iNumDistances, t->psDrfs};
+    g_drfCache[t] = std::move(c);
+    return g_drfCache[t].pad;
+}
+
+void report(const IrlError &e, const char *fn)
+{
+    set_error_string(e.msg);
+    set_last_error(e.code);
+    fprintf(stderr, "%s: %s\n", fn, e.msg.c_str());
+}
+
+// Builds the per-row tables and rotation constants of one call.
+PrjCall make_call(int Nz, int N, int NRot, int model, float pw, float fAngle, float fCFCR,
+                  const DrfTab_t *t, float primary, float atnFac, int pad, int avg,
+                  int atnRot, int actRot, std::vector<int> &rk, std::vector<int> &rp)
+{
+    PrjCall c{};
+    c.Nz = Nz;
+    c.N = N;
+    c.NRot = NRot;
+    c.model = model;
+    ViewTab vt;
+    view_trig(fAngle, vt);
+    c.fsin = vt.fsin;
+    c.fcos = vt.fcos;
+    c.bsin = vt.bsin;
+    c.bcos = vt.bcos;
+    // prjbck.c:127, 317: with scatter modelling the primary factor is applied
+    // by the scatter source instead.
+    c.primary = (model & MODEL_SRF) ? 1.0f : primary;
+    c.atnFac = atnFac;
+    c.avg = avg;
+    c.atnRotated = atnRot;
+    c.actRotated = actRot;
+    rk.assign(NRot, -1);
+    rp.assign(NRot, 0);
+    c.drf = nullptr;
+    c.drfKey = nullptr;
+    if ((model & MODEL_DRF) && t) {
+        c.drf = &cached_drf(t);
+        c.drfKey = t;
+        for (int r = 0; r < NRot; ++r) {
+            bool skip;
+            const int d = drf_row_index(t, fCFCR, pw, NRot, r, &skip);
+            if (skip) continue;
+            rk[r] = d;
+            int p = std::max(pad, 0);
+            p = std::min(p, t->psDrfs[d].MaxOffsY - 1);
+            if (t->bFFTconvolve) p = 0;
+            rp[r] = p;
+        }
+    } else if (model & MODEL_DRF) {
+        fail(ETYPE_ILLEGAL_VALUE, "DRF modelling requested without a DRF table");
+    }
+    c.rowKernel = rk.data();
+    c.rowPad = rp.data();
+    return c;
+}
+
+}  // namespace
+
+extern "C" {
+
+// prjbck.c:42-66
+void InitPrj(int iNumSlices, int iNumPixels, int iNumRotPixs, int iModel)
+{
+    set_last_error(0);
+    try {
+        g_drfCache.clear();
+        prjws_init(iNumSlices, iNumPixels, iNumRotPixs, iModel);
+    } catch (const IrlError &e) {
+        report(e, "InitPrj");
+    }
+}
+
+// prjbck.c:432-453
+void FreePrj(void)
+{
+    try {
+        g_drfCache.clear();
+        prjws_free();
+    } catch (const IrlError &e) {
+        report(e, "FreePrj");
+    }
+}
+
+// prjbck.c:68-252
+void Project(int iNumSlices, int iNumPixels, int iNumRotPixs, int iPrjModel, float fPixelWidth,
+             int iAngle, float fAngle, float fCFCR, DrfTab_t *psDrfTab, Srf_t *psSrf,
+             float *atnmatrix, float *reconmatrix, float *calcprj, float fPrimaryFac,
+             float fAtnFac, int iAxialPadLength, int iAxialAvgLength, int bAtnMapRotated,
+             int bActImgRotated, int bFastRotate)
+{
+    (void)iAngle;
+    (void)psSrf;
+    (void)bFastRotate;
+    try {
+        std::vector<int> rk, rp;
+        PrjCall c = make_call(iNumSlices, iNumPixels, iNumRotPixs, iPrjModel, fPixelWidth,
+                              fAngle, fCFCR, psDrfTab, fPrimaryFac, fAtnFac, iAxialPadLength,
+                              iAxialAvgLength, bAtnMapRotated, bActImgRotated, rk, rp);
+        prjws_project(c, atnmatrix, reconmatrix, calcprj);
+    } catch (const IrlError &e) {
+        report(e, "Project");
+    }
+}
+
+// prjbck.c:254-427
+void BackProject(int iNumSlices, int iNumPixels, int iNumRotPixs, int iBckModel,
+                 float fPixelWidth, int iAngle, float fAngle, float fCFCR, DrfTab_t *psDrfTab,
+                 Srf_t *psSrf, float *atnmatrix, float *ratioprj, float *imagematrix,
+                 float fPrimaryFac, float fAtnFac, int iAxialPadLength, int iAxialAvgLength,
+                 int bAtnMapRotated, int bFastRotate)
+{
+    (void)iAngle;
+    (void)psSrf;
+    (void)bFastRotate;
+    try {
+        // prjbck.c:308-310: a separate backprojection table wins when present
+        if (psDrfTab && psDrfTab->psBckDrfTab) psDrfTab = psDrfTab->psBckDrfTab;
+        std::vector<int> rk, rp;
+        PrjCall c = make_call(iNumSlices, iNumPixels, iNumRotPixs, iBckModel, fPixelWidth,
+                              fAngle, fCFCR, psDrfTab, fPrimaryFac, fAtnFac, iAxialPadLength,
+                              iAxialAvgLength, bAtnMapRotated, 0, rk, rp);
+        prjws_backproject(c, atnmatrix, ratioprj, imagematrix);
+    } catch (const IrlError &e) {
+        report(e, "BackProject");
+    }
+}
+
+// rot3dpar.c:19-198
+void rot3dpar(float fAngle, float *OldMatrix, float *NewMatrix, int PB_FLAG, int NumSlices,
+              int NumPixels, int NumRotPixs)
+{
+    try {
+        dev_rot3dpar(fAngle, OldMatrix, NewMatrix, PB_FLAG, NumSlices, NumPixels, NumRotPixs);
+    } catch (const IrlError &e) {
+        report(e, "rot3dpar");
+    }
+}
+
+// convolve_ypad.c:22-96
+void ConvolveDrfYpad(float *pfIn, int inX, int inY, Drf_t *psDrf, int bFFTConvolve, int iYpad,
+                     int iYavg, float *pfOut)
+{
+    try {
+        if (!psDrf) fail(ETYPE_ILLEGAL_VALUE, "can't convolve using NULL DRF");
+        DrfTab_t one;
+        memset(&one, 0, sizeof one);
+        one.iNumDistances = 1;
+        one.psDrfs = psDrf;
+        PaddedDrf d = pad_drf_table(&one);
+        int pad = std::min(std::max(iYpad, 0), psDrf->MaxOffsY - 1);
+        if (bFFTConvolve) {
+            // the FFT branch is the same linear convolution without axial pad
+            pad = 0;
+            iYavg = 0;
+        }
+        dev_convolve_plane(pfIn, inX, inY, d, pad, iYavg, pfOut);
+    } catch (const IrlError &e) {
+        report(e, "ConvolveDrfYpad");
+    }
+}
+
+}  // extern "C"
