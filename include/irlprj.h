@@ -16,6 +16,9 @@
  *   ConvolveDrfYpad        replaces irlprivate.h:233-242 (convolve_ypad.c:22-96)
  *   sReadDrfTab            replaces drftabio.h:37-41     (drftabio.c:150-275)
  *   vWriteDrfTab           replaces drftabio.h:30-35     (drftabio.c:57-148)
+ *   pvInitESSEScatterModel replaces krnlprj.h:9-10       (krnlprj3d.c:956-1009)
+ *   CalcEffScatterSource   replaces krnlprj.h:12-13      (krnlprj3d.c:676-954)
+ *   pvDeinitScatterPrj     replaces krnlprj.h:15         (krnlprj3d.c:377-401)
  *
  * Workspace semantics follow prjbck.c: after Project the rotated estimate
  * and rotated attenuation map stay in the library's workspace, so a later
@@ -87,6 +90,17 @@ DrfTab_t sReadDrfTab(char *pchInFname, char **ppchInTitle, char **ppchInColName)
 
 void vWriteDrfTab(DrfTab_t *psOutDrfTab, char *pchOutFname, char *pchTitle,
                   char *pchColName);
+
+/* ESSE scatter model.  pvInitESSEScatterModel reads a KRNL file and prepares
+ * the model for nPix x nPix x nSlices volumes; the parameter string is
+ * "xypadfac zpadfac numexpterms geometry" ("2 2 2 3" in IrlOsem).  The
+ * returned handle goes into Srf_t.pvScatterData for Project/BackProject with
+ * MODEL_SRF.  Images are [ny][nz][nx]: nx bins, ny depth rows, nz slices. */
+void *pvInitESSEScatterModel(char *pchKrnlFile, char *pchParString, float fPixSize, int nPix,
+                             int nSlices, int iCollapseFac, int bDebugFlag);
+void CalcEffScatterSource(void *pvScatterData, float *pfAct, float *pfMap, int nx, int ny,
+                          int nz, float *pfEffSrc);
+void *pvDeinitScatterPrj(void *pvScatterData);
 
 #ifdef __cplusplus
 }

@@ -14,8 +14,8 @@ LIB = os.path.join(HERE, "libirl_amd.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
-HIP_SOURCES = ["engine.hip"]
-CPP_SOURCES = ["host_util.cpp", "irlosem.cpp", "prjbck_api.cpp", "abi_check.cpp"]
+HIP_SOURCES = ["engine.hip", "esse.hip"]
+CPP_SOURCES = ["host_util.cpp", "irlosem.cpp", "prjbck_api.cpp", "abi_check.cpp", "esse_host.cpp"]
 
 # -ffp-contract=off: every float multiply/add in the geometry and
 # interpolation code is rounded separately, as in the reference; the

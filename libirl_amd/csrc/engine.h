@@ -7,6 +7,7 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -52,10 +53,15 @@ struct Geometry {
     int V = 0;      // NumViews
 };
 
-// Precision of the DRF convolutions.
-enum ConvPrecision { CONV_FP32 = 0 };
+struct EsseModel;
 
-class Device;
+// Scatter-model handling of one sub-iteration (osemrecon.c:652-676, 734-788).
+enum SrfMode {
+    SRF_NONE = 0,        // no scatter model in the projector
+    SRF_MODEL = 1,       // ESSE in the projector
+    SRF_MODEL_STORE = 2, // ESSE, and store the scatter projection estimate
+    SRF_ADD_STORED = 3,  // no ESSE; add the stored estimate
+};
 
 // One device-resident OSEM reconstruction (IrlOsem's state between setup and
 // teardown).
@@ -75,6 +81,7 @@ public:
         std::vector<uint8_t> colKeep;      // V*NRot (ratio mask)
         bool hasScatterEstimate = false;
         bool hasAtn = false;
+        std::shared_ptr<const EsseModel> esse;  // scatter model, or null
     };
 
     explicit Recon(const Config &cfg);
@@ -93,10 +100,13 @@ public:
 
     // Projection of the current estimate for every view (IrlGenprj's
     // per-view output before the 1/NumViews scaling), [view][slice][bin].
-    void forwardProject(float primary, float *out_host);
+    void forwardProject(float primary, float *out_host, bool srf = false);
 
     // One OSEM sub-iteration over subset s (osemrecon.c:687-863 body).
-    void subIteration(int s, bool keepUpdate);
+    void subIteration(int s, bool keepUpdate, SrfMode srf = SRF_NONE);
+
+    // Stored scatter projection estimate (pfPrjScatEst), [view][slice][bin].
+    void downloadScatterProjections(float *h) const;
 
     // Device -> host downloads.
     void downloadEstimate(float *x) const;
@@ -138,6 +148,8 @@ struct PrjCall {
     const int *rowPad;             // NRot entries
     const void *drfKey;            // identity of the DRF table (cache key)
     const PaddedDrf *drf;          // null when DRF is not modelled
+    const void *esse;              // pvInitESSEScatterModel handle (MODEL_SRF), may be null
+    float primaryFac;              // fPrimaryFac as passed (scatter source scaling)
 };
 
 void prjws_init(int Nz, int N, int NRot, int model);

@@ -42,6 +42,7 @@
 #include <vector>
 
 #include "engine.h"
+#include "esse_dev.h"
 
 namespace irlamd {
 
@@ -432,6 +433,10 @@ struct RatioArgs {
     const uint8_t *colKeep;       // [V][NRot]
     float scale;                  // fScaleFac
     float *ratio;                 // [B][Nz][NRot]
+    // Scatter-model bookkeeping (osemrecon.c:734-788)
+    const float *ppart2;          // partials of the projection without scatter model
+    float *prjscat;               // [V][Nz][NRot] stored scatter projection estimate
+    int srfMode;                  // 0 none, 2 store estimate, 3 add stored estimate
 };
 
 __global__ void k_ratio(RatioArgs a)
@@ -447,6 +452,14 @@ __global__ void k_ratio(RatioArgs a)
     for (int c = 0; c < a.nchunk; ++c) p += a.ppart[((long)b * a.nchunk + c) * pl + i];
     float calc = p * a.scale;
     calc = calc * a.vtsens[v];
+    if (a.srfMode == 3) {
+        calc = calc + a.prjscat[(long)v * pl + i];
+    } else if (a.srfMode == 2) {
+        float p2 = 0.0f;
+        for (int c = 0; c < a.nchunk; ++c) p2 += a.ppart2[((long)b * a.nchunk + c) * pl + i];
+        p2 = (p2 * a.scale) * a.vt[v];   // scaled by ViewTime only (osemrecon.c:772-773)
+        a.prjscat[(long)v * pl + i] = calc - p2;
+    }
     if (a.scat) calc = calc + a.scat[(long)v * pl + i];
     // osemrecon.c:802-803 hands view 0's measured data to the mask
     if (!a.colKeep[(long)v * a.NRot + xb] || a.prj[i] < 0.0f) calc = 0.0f;
@@ -590,6 +603,8 @@ struct Recon::Impl {
     long M = 0, P = 0, vol = 0;
     int nchunk = 1, rchunk = 1;
     DBuf<float> x, mu, prj, scat, u, ninv, afac, att, ppart, ratio, planes;
+    DBuf<float> rotx, rotmu, terms, ppart2, prjscat;  // scatter model
+    std::unique_ptr<EsseDev> esse;
     DBuf<float> fsin, fcos, bsin, bcos, vt, vtsens, psf;
     DBuf<int> rowKernel, rowPad, subsets, head, npix, rowNone;
     DBuf<uint8_t> colMeasOne, colKeep;
@@ -654,11 +669,25 @@ struct Recon::Impl {
         fail(ETYPE_ILLEGAL_VALUE, "DRF axial size %d exceeds the supported 63", c.drf.KY);
     }
 
-    void run_att(const float *afac, int B, const int *bview)
+    void run_att(const float *afac, int B, const int *bview, const float *src = nullptr,
+                 float *dst = nullptr)
     {
         const Geometry &g = c.g;
-        hipLaunchKernelGGL(k_att, dim3(div_up(vol * B, 256)), dim3(256), 0, stream, x.p, afac,
-                           att.p, B, g.Nz, g.N, g.NRot, bview, fsin.p, fcos.p, head.p, npix.p);
+        hipLaunchKernelGGL(k_att, dim3(div_up(vol * B, 256)), dim3(256), 0, stream,
+                           src ? src : x.p, afac, dst ? dst : att.p, B, g.Nz, g.N, g.NRot, bview,
+                           fsin.p, fcos.p, head.p, npix.p);
+    }
+
+    // Attenuated scatter-plus-primary source of the batch's views into att
+    // (prjbck.c:129-176, 219-222): rotate x and mu, ESSE, combine.
+    void run_srf_att(const float *afacp, int B, const int *bview)
+    {
+        launch("rot_x", [&] { run_att(nullptr, B, bview, x.p, rotx.p); });
+        launch("rot_mu", [&] { run_att(nullptr, B, bview, mu.p, rotmu.p); });
+        launch("esse_conv", [&] { esse->convolve(rotx.p, B, terms.p); });
+        launch("esse_source", [&] {
+            esse->source(rotx.p, rotmu.p, terms.p, B, 1.0f, afacp, bview, att.p);
+        });
     }
 
     BckArgs bck_args(const int *bview)
@@ -733,6 +762,21 @@ Recon::Recon(const Config &cfg) : impl_(new Impl), cfg_(cfg)
         I.afac.alloc(I.vol * g.V);
     }
     I.att.alloc(I.vol * Vs);
+    if (cfg.prjModel & 4) {
+        if (!cfg.hasAtn)
+            fail(ETYPE_ILLEGAL_VALUE, "scatter modelling needs an attenuation map");
+        I.rotx.alloc(I.vol * Vs);
+        I.rotmu.alloc(I.vol * Vs);
+        if (cfg.esse && cfg.esse->enabled) {
+            if (cfg.esse->nx != g.NRot || cfg.esse->ny != g.NRot || cfg.esse->nz != g.Nz)
+                fail(ETYPE_ILLEGAL_VALUE, "scatter model prepared for another volume size");
+            I.esse.reset(new EsseDev(*cfg.esse, Vs, I.stream));
+            I.terms.alloc(I.vol * Vs * cfg.esse->nterms);
+        }
+        I.ppart2.alloc(I.P * I.nchunk * Vs);
+        I.prjscat.alloc(I.P * g.V);
+        HIP_CHECK(hipMemsetAsync(I.prjscat.p, 0, I.P * g.V * sizeof(float), I.stream));
+    }
     I.ppart.alloc(I.P * I.nchunk * Vs);
     I.ratio.alloc(I.P * Vs);
     I.planes.alloc(I.vol * Vs);
@@ -855,6 +899,12 @@ void Recon::downloadNorm(int s, float *h) const
                              hipMemcpyDeviceToHost, impl_->stream));
     synchronize();
 }
+void Recon::downloadScatterProjections(float *h) const
+{
+    if (!impl_->prjscat.p) return;
+    impl_->prjscat.download(h, impl_->P * cfg_.g.V, impl_->stream);
+    synchronize();
+}
 void Recon::synchronize() const { HIP_CHECK(hipStreamSynchronize(impl_->stream)); }
 void Recon::setTiming(bool on) { impl_->timing = on; }
 void Recon::resetTimes()
@@ -924,7 +974,7 @@ void Recon::precompute(bool norms)
     synchronize();
 }
 
-void Recon::subIteration(int s, bool keepUpdate)
+void Recon::subIteration(int s, bool keepUpdate, SrfMode srf)
 {
     Impl &I = *impl_;
     const Geometry &g = cfg_.g;
@@ -933,8 +983,16 @@ void Recon::subIteration(int s, bool keepUpdate)
     const int *bview = I.subsets.p + (long)s * Vs;
     const int KYb = cfg_.drf.nDist > 0 ? I.ky_bucket() : 7;
     const float *afac = (cfg_.prjModel & 1) ? I.afac.p : nullptr;
+    const bool model = srf == SRF_MODEL || srf == SRF_MODEL_STORE;
+    if (srf != SRF_NONE && !I.prjscat.p)
+        fail(ETYPE_ILLEGAL_VALUE, "scatter model requested without a scatter kernel");
 
-    I.launch("att", [&] { I.run_att(afac, Vs, bview); });
+    if (model && I.esse)
+        I.run_srf_att(afac, Vs, bview);
+    else if (model)  // scatFac == 0: no effective source, the projected image is x
+        I.launch("att", [&] { I.run_att(afac, Vs, bview); });
+    else
+        I.launch("att", [&] { I.run_att(afac, Vs, bview); });
     DrfArgs da{};
     da.NRot = g.NRot;
     da.nrows = g.NRot;
@@ -956,7 +1014,20 @@ void Recon::subIteration(int s, bool keepUpdate)
     I.launch("drf_prj", [&] {
         I.run_drf(false, da, da.xtiles * da.ztiles * Vs * I.nchunk, KYb);
     });
+    if (srf == SRF_MODEL_STORE) {
+        // projection without the scatter model for the stored estimate
+        // (osemrecon.c:757-786), reusing the rotated estimate
+        I.launch("att", [&] { I.run_att(afac, Vs, bview); });
+        DrfArgs d2 = da;
+        d2.out = I.ppart2.p;
+        I.launch("drf_prj", [&] {
+            I.run_drf(false, d2, d2.xtiles * d2.ztiles * Vs * I.nchunk, KYb);
+        });
+    }
     RatioArgs ra{};
+    ra.srfMode = srf == SRF_MODEL_STORE ? 2 : srf == SRF_ADD_STORED ? 3 : 0;
+    ra.ppart2 = I.ppart2.p;
+    ra.prjscat = I.prjscat.p;
     ra.B = Vs;
     ra.Nz = g.Nz;
     ra.NRot = g.NRot;
@@ -992,7 +1063,7 @@ void Recon::subIteration(int s, bool keepUpdate)
 }
 
 // Forward projection of the current estimate for every view (IrlGenprj).
-void Recon::forwardProject(float primary, float *out_host)
+void Recon::forwardProject(float primary, float *out_host, bool srf)
 {
     Impl &I = *impl_;
     const Geometry &g = cfg_.g;
@@ -1010,7 +1081,16 @@ void Recon::forwardProject(float primary, float *out_host)
     for (int v0 = 0; v0 < g.V; v0 += Vs) {
         const int B = std::min(Vs, g.V - v0);
         const int *bview = order.p + v0;
-        I.launch("att", [&] { I.run_att(afac, B, bview); });
+        if (srf && I.esse) {
+            I.launch("rot_x", [&] { I.run_att(nullptr, B, bview, I.x.p, I.rotx.p); });
+            I.launch("rot_mu", [&] { I.run_att(nullptr, B, bview, I.mu.p, I.rotmu.p); });
+            I.launch("esse_conv", [&] { I.esse->convolve(I.rotx.p, B, I.terms.p); });
+            I.launch("esse_source", [&] {
+                I.esse->source(I.rotx.p, I.rotmu.p, I.terms.p, B, primary, afac, bview, I.att.p);
+            });
+        } else {
+            I.launch("att", [&] { I.run_att(afac, B, bview); });
+        }
         DrfArgs da{};
         da.NRot = g.NRot;
         da.nrows = g.NRot;
@@ -1040,7 +1120,7 @@ void Recon::forwardProject(float primary, float *out_host)
         ra.ppart = I.ppart.p;
         ra.scat = I.scat.p;
         ra.vtsens = I.vtsens.p;
-        ra.scale = primary;
+        ra.scale = (srf && I.esse) ? 1.0f : primary;
         I.launch("prj_finish", [&] {
             hipLaunchKernelGGL(k_prj_finish, dim3(div_up(I.P * B, threads)), dim3(threads), 0,
                                I.stream, ra, out.p);
@@ -1073,6 +1153,12 @@ __global__ void k_afac_rotated(const float *__restrict__ rmu, float *__restrict_
         afac[(long)Y * cols + c] = (float)exp((double)(-S * atnFac));
         S = (float)((double)S + 0.5 * (double)m);
     }
+}
+
+__global__ void k_scale(const float *__restrict__ in, float *__restrict__ out, long n, float f)
+{
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[i] * f;
 }
 
 __global__ void k_sum_chunks(const float *__restrict__ ppart, float *__restrict__ out,
@@ -1111,6 +1197,40 @@ struct PrjWs {
     const void *drfKey = nullptr;
     int drfKX = 1, drfKYb = 7;
     std::vector<int> hhead, hnpix;
+    const void *esseKey = nullptr;
+    std::unique_ptr<EsseDev> esse;
+    DBuf<float> scatter, terms;
+
+    // ESSE device instance of the handle (kept until another handle comes).
+    EsseDev *esse_for(const void *h)
+    {
+        const EsseModel *m = esse_handle_model(h);
+        if (!m || !m->enabled) return nullptr;
+        if (h != esseKey) {
+            if (m->nx != NRot || m->ny != NRot || m->nz != Nz)
+                fail(ETYPE_ILLEGAL_VALUE, "scatter model prepared for another volume size");
+            esse.reset(new EsseDev(*m, 1, stream));
+            terms.alloc((long)NRot * Nz * NRot * m->nterms);
+            esseKey = h;
+        }
+        return esse.get();
+    }
+
+    // q = act * primaryFac + effective scatter source (prjbck.c:164-176,
+    // 395-406), written to dst (may alias act).
+    void scatter_source(const void *h, const float *act, float primaryFac, float *dst)
+    {
+        EsseDev *e = esse_for(h);
+        const long n = (long)NRot * Nz * NRot;
+        if (e) {
+            e->convolve(act, 1, terms.p);
+            e->source(act, rotAtn.p, terms.p, 1, primaryFac, nullptr, nullptr, dst);
+        } else {
+            hipLaunchKernelGGL(k_scale, dim3(div_up(n, 256)), dim3(256), 0, stream, act, dst, n,
+                               primaryFac);
+            HIP_CHECK(hipGetLastError());
+        }
+    }
 
     PrjWs(int nz, int n, int nrot, int m) : Nz(nz), N(n), NRot(nrot), model(m)
     {
@@ -1124,6 +1244,7 @@ struct PrjWs {
         ppart.alloc((long)Nz * NRot * NRot);
         plane.alloc((long)Nz * NRot);
         img.alloc((long)N * Nz * N);
+        if (m & 4) scatter.alloc(v);
         rowKernel.alloc(NRot);
         rowPad.alloc(NRot);
         int z0 = 0;
@@ -1229,7 +1350,6 @@ void prjws_project(const PrjCall &c, const float *atn, const float *recon, float
     PrjWs &W = ws(c.Nz, c.N, c.NRot);
     const long nin = (long)c.N * c.Nz * c.N;
     const bool useAtn = (c.model & 1) || (c.model & 4);
-    if (c.model & 4) fail(ETYPE_ILLEGAL_VALUE, "ESSE scatter modelling is not available in this build");
     if (!c.actRotated) {
         W.vol.upload(recon, nin, W.stream);
         W.rotate(W.vol.p, W.rotRecon.p, c.fsin, c.fcos, c.N);
@@ -1249,6 +1369,11 @@ void prjws_project(const PrjCall &c, const float *atn, const float *recon, float
     const int ZT = zt_for(c.Nz);
     da.ztiles = div_up(c.Nz, DRF_ZG * ZT);
     da.in = W.rotRecon.p;
+    if (c.model & 4) {
+        if (!W.scatter.p) fail(ETYPE_ILLEGAL_VALUE, "InitPrj was called without MODEL_SRF");
+        W.scatter_source(c.esse, W.rotRecon.p, c.primaryFac, W.scatter.p);
+        da.in = W.scatter.p;
+    }
     da.afac = (c.model & 1) ? W.afac.p : nullptr;
     da.out = W.ppart.p;
     da.rchunk = 1;
@@ -1266,7 +1391,6 @@ void prjws_backproject(const PrjCall &c, const float *atn, const float *ratio, f
     PrjWs &W = ws(c.Nz, c.N, c.NRot);
     const long nin = (long)c.N * c.Nz * c.N;
     const bool useAtn = (c.model & 1) || (c.model & 4);
-    if (c.model & 4) fail(ETYPE_ILLEGAL_VALUE, "ESSE scatter modelling is not available in this build");
     if (useAtn && !c.atnRotated) {
         W.vol.upload(atn, nin, W.stream);
         W.rotate(W.vol.p, W.rotAtn.p, c.fsin, c.fcos, c.N);
@@ -1288,6 +1412,7 @@ void prjws_backproject(const PrjCall &c, const float *atn, const float *ratio, f
     da.bckRowMajor = 1;
     da.scale = c.primary;
     launch_drf(W.stream, true, da, da.xtiles * da.ztiles * c.NRot, W.drfKYb, ZT);
+    if (c.model & 4) W.scatter_source(c.esse, W.rotRecon.p, c.primaryFac, W.rotRecon.p);
     W.rotate(W.rotRecon.p, W.unrot.p, c.bsin, c.bcos, c.NRot);
     W.img.upload(image, nin, W.stream);
     hipLaunchKernelGGL(k_acc_clamp, dim3(div_up(nin, 256)), dim3(256), 0, W.stream, W.img.p,

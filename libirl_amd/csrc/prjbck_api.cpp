@@ -9,6 +9,7 @@
 #include <map>
 
 #include "../../include/irl_amd.h"
+#include "esse.h"
 #include "irlprj.h"
 
 using namespace irlamd;
@@ -60,6 +61,8 @@ PrjCall make_call(int Nz, int N, int NRot, int model, float pw, float fAngle, fl
     // prjbck.c:127, 317: with scatter modelling the primary factor is applied
     // by the scatter source instead.
     c.primary = (model & MODEL_SRF) ? 1.0f : primary;
+    c.primaryFac = primary;
+    c.esse = nullptr;
     c.atnFac = atnFac;
     c.avg = avg;
     c.atnRotated = atnRot;
@@ -124,13 +127,13 @@ void Project(int iNumSlices, int iNumPixels, int iNumRotPixs, int iPrjModel, flo
              int bActImgRotated, int bFastRotate)
 {
     (void)iAngle;
-    (void)psSrf;
     (void)bFastRotate;
     try {
         std::vector<int> rk, rp;
         PrjCall c = make_call(iNumSlices, iNumPixels, iNumRotPixs, iPrjModel, fPixelWidth,
                               fAngle, fCFCR, psDrfTab, fPrimaryFac, fAtnFac, iAxialPadLength,
                               iAxialAvgLength, bAtnMapRotated, bActImgRotated, rk, rp);
+        if (iPrjModel & MODEL_SRF) c.esse = psSrf ? psSrf->pvScatterData : nullptr;
         prjws_project(c, atnmatrix, reconmatrix, calcprj);
     } catch (const IrlError &e) {
         report(e, "Project");
@@ -145,7 +148,6 @@ void BackProject(int iNumSlices, int iNumPixels, int iNumRotPixs, int iBckModel,
                  int bAtnMapRotated, int bFastRotate)
 {
     (void)iAngle;
-    (void)psSrf;
     (void)bFastRotate;
     try {
         // prjbck.c:308-310: a separate backprojection table wins when present
@@ -154,6 +156,7 @@ void BackProject(int iNumSlices, int iNumPixels, int iNumRotPixs, int iBckModel,
         PrjCall c = make_call(iNumSlices, iNumPixels, iNumRotPixs, iBckModel, fPixelWidth,
                               fAngle, fCFCR, psDrfTab, fPrimaryFac, fAtnFac, iAxialPadLength,
                               iAxialAvgLength, bAtnMapRotated, 0, rk, rp);
+        if (iBckModel & MODEL_SRF) c.esse = psSrf ? psSrf->pvScatterData : nullptr;
         prjws_backproject(c, atnmatrix, ratioprj, imagematrix);
     } catch (const IrlError &e) {
         report(e, "BackProject");
@@ -192,6 +195,49 @@ void ConvolveDrfYpad(float *pfIn, int inX, int inY, Drf_t *psDrf, int bFFTConvol
     } catch (const IrlError &e) {
         report(e, "ConvolveDrfYpad");
     }
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// krnlprj3d.c:956-1009.  Returns NULL when the kernel's scatter factor is 0
+// (no scatter data, krnlprj3d.c:428), as the reference does.
+void *pvInitESSEScatterModel(char *pchKrnlFile, char *pchParString, float fPixSize, int nPix,
+                             int nSlices, int iCollapseFac, int bDebugFlag)
+{
+    (void)bDebugFlag;
+    set_last_error(0);
+    try {
+        if (iCollapseFac < 1 || iCollapseFac > 4)
+            fail(ETYPE_ILLEGAL_VALUE, "collapse factor %d: must be 1 to 4", iCollapseFac);
+        KrnlData k = read_krnl_file(pchKrnlFile);
+        EsseModel m = make_esse_model(k, pchParString, fPixSize, nPix, nSlices, iCollapseFac);
+        if (!m.enabled) return nullptr;
+        return esse_handle_create(m);
+    } catch (const IrlError &e) {
+        report(e, "pvInitESSEScatterModel");
+        return nullptr;
+    }
+}
+
+// krnlprj3d.c:676-954 on host arrays [ny][nz][nx]: nx bins, ny depth rows,
+// nz slices.
+void CalcEffScatterSource(void *pvScatterData, float *pfAct, float *pfMap, int nx, int ny,
+                          int nz, float *pfEffSrc)
+{
+    try {
+        esse_handle_effsrc(pvScatterData, pfAct, pfMap, nx, ny, nz, pfEffSrc);
+    } catch (const IrlError &e) {
+        report(e, "CalcEffScatterSource");
+    }
+}
+
+// krnlprj3d.c:377-401
+void *pvDeinitScatterPrj(void *pvScatterData)
+{
+    esse_handle_free(pvScatterData);
+    return nullptr;
 }
 
 }  // extern "C"

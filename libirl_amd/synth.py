@@ -119,10 +119,47 @@ def poisson(prj, total_counts, seed):
     return rng.poisson(np.clip(lam, 0, None)).astype(np.float32)
 
 
+def write_krnl(fname, krnl, krnl_mu, mu0, pixsize, hi_e=154.0, lo_e=126.0, pri_fac=1.0,
+               scat_fac=1.0, cntr_pix_fac=0.0, title="esse kernel", isotope="Tc99m"):
+    """Writes a KRNL file (krnlio.c:30-60 layout): krnl and krnl_mu are
+    [nDists][nXYoffs][nXYoffs] one-quadrant kernels."""
+    krnl = np.ascontiguousarray(krnl, dtype="<f4")
+    krnl_mu = np.ascontiguousarray(krnl_mu, dtype="<f4")
+    nd, n, n2 = krnl.shape
+    assert n == n2 and krnl_mu.shape == krnl.shape
+    with open(fname, "wb") as f:
+        f.write(b"KRNL")
+        f.write(struct.pack("<i", 1))
+        f.write(title.encode()[:79].ljust(80, b"\0"))
+        f.write(isotope.encode()[:19].ljust(20, b"\0"))
+        f.write(struct.pack("<7f", mu0, hi_e, lo_e, pixsize, pri_fac, scat_fac, cntr_pix_fac))
+        f.write(struct.pack("<ii", n, nd))
+        f.write(krnl.tobytes())
+        f.write(krnl_mu.tobytes())
+
+
+def gaussian_mixture_krnl(noffs, ndepths, pixsize, amps=(0.02, 0.006, 0.0015),
+                          sigmas=(0.8, 2.0, 4.5), mu0=0.15, mu_spread=0.02):
+    """Seeded-free Gaussian-mixture ESSE kernel on one quadrant of offsets
+    (depth index d has offset ndepths//2 - 1 - d, krnlprj3d.c:92-97) and a
+    smoothly varying average-attenuation kernel around mu0 (1/cm)."""
+    off = np.arange(noffs) * pixsize
+    dep = (ndepths // 2 - 1 - np.arange(ndepths)) * pixsize
+    r2 = off[None, :, None] ** 2 + off[None, None, :] ** 2 + dep[:, None, None] ** 2
+    k = np.zeros((ndepths, noffs, noffs))
+    for a, s in zip(amps, sigmas):
+        k += a * np.exp(-r2 / (2 * s * s))
+    kmu = mu0 * (1.0 - mu_spread * np.sqrt(r2) / (np.sqrt(r2).max() + 1e-9))
+    return k.astype(np.float32), kmu.astype(np.float32)
+
+
 # Workload definitions of the benchmark configurations.
 CONFIGS = {
     "c1": dict(n=64, nz=64, pix=0.4, views=60, vs=15, iters=2, orbit="circular", cfcr=25.0,
                inserts=(3, 5), half=7, counts=5e5),
     "c2": dict(n=128, nz=128, pix=0.33, views=120, vs=15, iters=4, orbit="ellipse",
                inserts=(6, 10), half=15, counts=2e6),
+    "c3": dict(n=128, nz=128, pix=0.33, views=120, vs=15, iters=4, orbit="ellipse",
+               inserts=(6, 10), half=15, counts=2e6, krnl_offs=17, krnl_depths=33,
+               srf_iters=4),
 }

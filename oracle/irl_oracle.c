@@ -342,6 +342,9 @@ void *orc_esse_init(const float *krnl, const float *krnlmu, int noffs,
     e->cntrpixfac = cntrpixfac;
     e->nterms = nterms;
     e->collapse = collapse;
+    /* with one term the reference leaves the average mu unset
+       (krnlprj3d.c:499-524); mu0 makes the depth correction exp(0) = 1 */
+    e->muavg = mu0;
     nx = nPix / collapse;
     ny = nPix / collapse;
     nz = nSlices / collapse;
@@ -844,7 +847,10 @@ int orc_osem(int Nz, int N, int V, int NumAngPerSubset, int NumIterations,
     tmp = (REAL *)calloc(M, sizeof(REAL));
     normImg = (REAL *)calloc(M, sizeof(REAL));
     calc = (REAL *)calloc(P, sizeof(REAL));
-    if (bPrjModelSrf && iNumSrfIterations > 0) prjscat = (REAL *)calloc(P * V, sizeof(REAL));
+    /* osemrecon.c:610-615 allocates the stored estimate only for
+       iNumSrfIterations > 0 and would then read a NULL pointer; zeros are
+       used instead */
+    if (bPrjModelSrf) prjscat = (REAL *)calloc(P * V, sizeof(REAL));
     orc_init_prj(Nz, N, NRot, iPrjModel | iBckModel);
 
     /* psComputeNormImages, osemrecon.c:282-331 */
@@ -862,7 +868,9 @@ int orc_osem(int Nz, int N, int V, int NumAngPerSubset, int NumIterations,
         for (i = 0; i < M; ++i) tmp[i] = tmp[i] < 1e-6 ? 0.0 : 1.0 / tmp[i];
         memcpy(norm + M * iSubset, tmp, sizeof(REAL) * M);
     }
-    if (bPrjModelSrf) bScatNext = StartIteration + 1 <= iNumSrfIterations;
+    /* osemrecon.c:628 reads StartIteration before it is set from the
+       advanced options (:636), i.e. as 0 */
+    if (bPrjModelSrf) bScatNext = 0 + 1 <= iNumSrfIterations;
 
     for (iIter = StartIteration + 1; iIter <= NumIterations; ++iIter) {
         iModel = iPrjModel;

@@ -163,6 +163,35 @@ def reorder(d0, d1, d2, a, real="f64"):
     return out
 
 
+class Esse:
+    """ESSE model of the oracle (orc_esse_init); None handle when scatFac is 0."""
+
+    def __init__(self, krnl, krnl_mu, mu0, pixsize, n, nz, collapse=1, nterms=2, scatfac=1.0,
+                 cntrpixfac=0.0, real="f64", xypad=2.0, zpad=2.0):
+        self.real = real
+        self.krnl = np.ascontiguousarray(krnl, np.float32)
+        self.krnl_mu = np.ascontiguousarray(krnl_mu, np.float32)
+        nd, no, _ = self.krnl.shape
+        self.h = lib(real).orc_esse_init(self.krnl.ctypes.data, self.krnl_mu.ctypes.data, no, nd,
+                                         mu0, scatfac, 1.0, cntrpixfac, pixsize, n, nz, collapse,
+                                         nterms, xypad, zpad)
+
+    def effsrc(self, act, mp):
+        act = _arr(act, self.real)
+        mp = _arr(mp, self.real)
+        ny, nz, nx = act.shape
+        out = np.zeros_like(act)
+        lib(self.real).orc_esse_effsrc(self.h, act.ctypes.data, mp.ctypes.data, nx, ny, nz,
+                                       out.ctypes.data)
+        return out
+
+    def __del__(self):
+        try:
+            lib(self.real).orc_esse_free(self.h)
+        except Exception:
+            pass
+
+
 ITER_CB = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p)
 
 
