@@ -30,10 +30,10 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
 
-def build_workload(tmpdir, seed=2):
+def build_workload(tmpdir, seed=2, name="c2"):
     import libirl_amd as L
     from libirl_amd import synth
-    cfg = synth.CONFIGS["c2"]
+    cfg = synth.CONFIGS[name]
     n, nz, pix, V = cfg["n"], cfg["nz"], cfg["pix"], cfg["views"]
     act, mu = synth.phantom(n, nz, seed, n_inserts=cfg["inserts"])
     views = synth.views_ellipse(V, n, seed)
@@ -51,13 +51,44 @@ def build_workload(tmpdir, seed=2):
     opts.bModelDrf = 1
     opts.bReconIsInitEst = 0
     opts.iMsgLevel = 1
+    krnl_file = None
+    if "krnl_offs" in cfg:
+        # ESSE kernel (33^3 support): Gaussian mixture, mu0 = 0.15/cm
+        k, km = synth.gaussian_mixture_krnl(cfg["krnl_offs"], cfg["krnl_depths"], pix)
+        krnl_file = os.path.join(tmpdir, name + ".krnl")
+        synth.write_krnl(krnl_file, k, km, 0.15, pix)
+        parms.iNumSrfIterations = cfg["srf_iters"]
     prj = np.zeros((V, nz, n), np.float32)
-    rc = L.irl_genprj(parms, opts, views, drf_file, None, None, act.copy(), mu.copy(), prj)
+    rc = L.irl_genprj(parms, opts, views, drf_file, krnl_file, None, act.copy(), mu.copy(), prj)
     if rc != 0:
         raise RuntimeError("IrlGenprj failed: " + L.error_string())
     data = synth.poisson(prj, cfg["counts"], seed)
     return dict(parms=parms, opts=opts, views=views, drf_file=drf_file, mu=mu, prj=data,
-                act=act, offs=offs, psfs=psfs, start=start, cfg=cfg)
+                act=act, offs=offs, psfs=psfs, start=start, cfg=cfg, krnl_file=krnl_file)
+
+
+def time_subiterations(w, steps, warmup):
+    """Device-resident reconstruction of the workload; returns (recon, seconds
+    for 'steps' sub-iterations, per-kernel {name: (ms_total, launches)})."""
+    import libirl_amd as L
+    L.set_adv_options(0, 0)
+    rec = L.Recon(w["parms"], w["opts"], w["views"], w["drf_file"], w["krnl_file"], None,
+                  w["mu"], w["prj"], None)
+    S = rec.num_subsets
+    for k in range(warmup):
+        rec.subiteration(k % S)
+    rec.synchronize()
+    rec.reset_timing()
+    rec.set_timing(True)
+    rec.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        rec.subiteration(k % S)
+    rec.synchronize()
+    dt = time.perf_counter() - t0
+    stages = rec.stage_times()
+    rec.set_timing(False)
+    return rec, dt, stages
 
 
 def drf_flops_per_subset(w, recon):
@@ -82,10 +113,10 @@ def drf_flops_per_subset(w, recon):
     return [sum(per_view[a] for a in subset) for subset in sub]
 
 
-def cpu_baseline(w, nviews=2):
-    """Reference-semantics CPU path (oracle, FP32, 1 thread) on a bounded
-    sample: Project + BackProject of 'nviews' views of the same workload;
-    scaled to one 15-view sub-iteration."""
+def cpu_baseline(w, nviews=15):
+    """Reference-semantics CPU path (oracle, FP32, 1 thread): Project +
+    BackProject of the 15 views of one sub-iteration of the same workload
+    (the update itself is negligible)."""
     from oracle import oracle as O
     p = w["parms"]
     n, nz, pw = p.NumPixels, p.NumSlices, p.BinWidth
@@ -103,7 +134,7 @@ def cpu_baseline(w, nviews=2):
     per_sub = dt / nviews * p.NumAngPerSubset
     return dict(value=1.0 / per_sub, unit="subiters/s", cores=1, kind="port",
                 sample="Project+BackProject of %d of the 15 views of one sub-iteration "
-                       "(%.1f s), scaled x%.1f" % (nviews, dt, p.NumAngPerSubset / nviews))
+                       "(%.1f s), scaled x%.2f" % (nviews, dt, p.NumAngPerSubset / nviews))
 
 
 def main():
@@ -112,6 +143,7 @@ def main():
     ap.add_argument("--steps", type=int, default=24)
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-esse", action="store_true", help="skip the ESSE workload line")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -129,26 +161,10 @@ def main():
         print("bench.py measures one GPU only; multi-GPU runs are not part of this build "
               "(requested --gpus %d)" % args.gpus, file=sys.stderr)
         sys.exit(2)
-    import libirl_amd as L
     tmpdir = tempfile.mkdtemp(prefix="irlbench")
     w = build_workload(tmpdir)
-    L.set_adv_options(0, 0)
-    rec = L.Recon(w["parms"], w["opts"], w["views"], w["drf_file"], None, None,
-                  w["mu"], w["prj"], None)
+    rec, dt, stages = time_subiterations(w, args.steps, args.warmup)
     S = rec.num_subsets
-    for k in range(args.warmup):
-        rec.subiteration(k % S)
-    rec.synchronize()
-    rec.reset_timing()
-    rec.set_timing(True)
-    rec.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        rec.subiteration(k % S)
-    rec.synchronize()
-    dt = time.perf_counter() - t0
-    stages = rec.stage_times()
-    rec.set_timing(False)
     ms_step = dt * 1e3 / args.steps
     flops = drf_flops_per_subset(w, rec)
     avg_flops = float(np.mean([flops[k % S] for k in range(args.steps)]))
@@ -177,9 +193,19 @@ def main():
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None},
         "stages_ms_per_launch": {k: v[0] / max(v[1], 1) for k, v in stages.items()},
     }
+    rec.close()
+    if not args.no_esse:
+        # Same geometry with ESSE scatter modelled in the projector (the
+        # plain-method scatter configuration); sub-iterations of the first
+        # scatter-modelling iterations.
+        we = build_workload(tmpdir, name="c3")
+        rec_e, dte, st_e = time_subiterations(we, 16, 8)
+        rec_e.close()
+        out["esse"] = {"workload": "osem-128^3x120v-atn-drf31-esse33-8subsets (FP32)",
+                       "value": 16 / dte, "unit": "subiters/s", "ms_per_step": dte * 1e3 / 16,
+                       "stages_ms_per_launch": {k: v[0] / max(v[1], 1) for k, v in st_e.items()}}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w)
-    rec.close()
     print(json.dumps(out))
 
 
