@@ -105,6 +105,13 @@ public:
     // One OSEM sub-iteration over subset s (osemrecon.c:687-863 body).
     void subIteration(int s, bool keepUpdate, SrfMode srf = SRF_NONE);
 
+    // One view at a time (for per-angle hooks): zeroUpdate at the start of
+    // the subset, then member m of subset s is projected and backprojected
+    // into U; the update of x follows with finishSubset.
+    void zeroUpdate();
+    void angleStep(int s, int m, SrfMode srf = SRF_NONE);
+    void finishSubset(int s);
+
     // Stored scatter projection estimate (pfPrjScatEst), [view][slice][bin].
     void downloadScatterProjections(float *h) const;
 
@@ -131,6 +138,8 @@ public:
     const Config &config() const { return cfg_; }
 
 private:
+    void runViews(int s, int first, int B, SrfMode srf, bool accumulate, bool update,
+                  bool keepUpdate);
     struct Impl;
     Impl *impl_;
     Config cfg_;

@@ -543,6 +543,14 @@ __global__ __launch_bounds__(256) void k_bck(BckArgs a)
     }
 }
 
+// x *= U * Ninv (osemrecon.c:850-852), for the per-angle path.
+__global__ void k_update(float *__restrict__ x, const float *__restrict__ u,
+                         const float *__restrict__ ninv, long M)
+{
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < M) x[i] = x[i] * (u[i] * ninv[i]);
+}
+
 // Sensitivity input: measured-bin mask times view time (osemrecon.c:290-295).
 __global__ void k_norm_input(int B, int Nz, int NRot, const int *__restrict__ bview,
                              const float *__restrict__ prj, const uint8_t *__restrict__ colMeasOne,
@@ -976,11 +984,37 @@ void Recon::precompute(bool norms)
 
 void Recon::subIteration(int s, bool keepUpdate, SrfMode srf)
 {
+    runViews(s, 0, cfg_.Vs, srf, false, true, keepUpdate);
+}
+
+void Recon::angleStep(int s, int m, SrfMode srf)
+{
+    runViews(s, m, 1, srf, true, false, true);
+}
+
+void Recon::zeroUpdate()
+{
+    HIP_CHECK(hipMemsetAsync(impl_->u.p, 0, impl_->M * sizeof(float), impl_->stream));
+}
+
+void Recon::finishSubset(int s)
+{
+    Impl &I = *impl_;
+    hipLaunchKernelGGL(k_update, dim3(div_up(I.M, 256)), dim3(256), 0, I.stream, I.x.p, I.u.p,
+                       I.ninv.p + I.M * s, I.M);
+    HIP_CHECK(hipGetLastError());
+}
+
+// Views [first, first+Vs) of subset s: projection, ratio, backprojection into
+// U (starting from the stored U when 'accumulate'), and the multiplicative
+// update of x when 'update'.
+void Recon::runViews(int s, int first, int Vs, SrfMode srf, bool accumulate, bool update,
+                     bool keepUpdate)
+{
     Impl &I = *impl_;
     const Geometry &g = cfg_.g;
     const int threads = 256;
-    const int Vs = cfg_.Vs;
-    const int *bview = I.subsets.p + (long)s * Vs;
+    const int *bview = I.subsets.p + (long)s * cfg_.Vs + first;
     const int KYb = cfg_.drf.nDist > 0 ? I.ky_bucket() : 7;
     const float *afac = (cfg_.prjModel & 1) ? I.afac.p : nullptr;
     const bool model = srf == SRF_MODEL || srf == SRF_MODEL_STORE;
@@ -1054,10 +1088,11 @@ void Recon::subIteration(int s, bool keepUpdate, SrfMode srf)
         I.run_drf(true, db, db.xtiles * db.ztiles * Vs * g.NRot, KYb);
     });
     BckArgs ba = I.bck_args(bview);
-    ba.x = I.x.p;
+    ba.B = Vs;
+    ba.x = update ? I.x.p : nullptr;
     ba.ninv = I.ninv.p + I.M * s;
     ba.u = I.u.p;
-    ba.accumulate = 0;
+    ba.accumulate = accumulate ? 1 : 0;
     ba.writeU = keepUpdate ? 1 : 0;
     I.launch("bck_update", [&] { I.run_bck(false, ba); });
 }
