@@ -32,6 +32,8 @@
 //   k_bck<true>  osemrecon.c:282-320 with FP64 accumulation
 #include <hip/hip_runtime.h>
 
+#include <climits>
+
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -88,13 +90,13 @@ static void circle_rows(int NewDim, std::vector<int> &head, std::vector<int> &np
 // ---------------------------------------------------------------------------
 
 // Bilinear sample of rot3dpar.c:119-189 at output pixel (NewX, NewY) of one
-// slice.  'plane' points at element [0][slice][0] of a [OldDim][Nz][OldDim]
-// volume, ystride = Nz*OldDim.  Every multiply and add is rounded separately
-// (the library is compiled with -ffp-contract=off), so sample positions and
-// taps are those of the reference bit for bit.
+// slice.  'plane' points at the slice's element (0, 0); source pixel
+// (OldY, OldX) is plane[OldY*ystride + OldX*xstride].  Every multiply and
+// add is rounded separately (the library is compiled with -ffp-contract=off),
+// so sample positions and taps are those of the reference bit for bit.
 __device__ __forceinline__ float rot_sample(const float *__restrict__ plane, long ystride,
-                                            int OldDim, float X, float Y, float sn,
-                                            float cs, float OldAxis)
+                                            long xstride, int OldDim, float X, float Y,
+                                            float sn, float cs, float OldAxis)
 {
     const float FloatX = X * cs - Y * sn + OldAxis;
     const float FloatY = X * sn + Y * cs + OldAxis;
@@ -104,10 +106,11 @@ __device__ __forceinline__ float rot_sample(const float *__restrict__ plane, lon
     const bool y0 = OldY >= 0 && OldY < OldDim;
     const bool x1 = OldX + 1 >= 0 && OldX + 1 < OldDim;
     const bool y1 = OldY + 1 >= 0 && OldY + 1 < OldDim;
-    const float z0 = (x0 && y0) ? plane[OldY * ystride + OldX] : 0.0f;
-    const float z1 = (x0 && y1) ? plane[(OldY + 1) * ystride + OldX] : 0.0f;
-    const float z2 = (x1 && y0) ? plane[OldY * ystride + OldX + 1] : 0.0f;
-    const float z3 = (x1 && y1) ? plane[(OldY + 1) * ystride + OldX + 1] : 0.0f;
+    const float *p0 = plane + OldY * ystride + OldX * xstride;
+    const float z0 = (x0 && y0) ? p0[0] : 0.0f;
+    const float z1 = (x0 && y1) ? p0[ystride] : 0.0f;
+    const float z2 = (x1 && y0) ? p0[xstride] : 0.0f;
+    const float z3 = (x1 && y1) ? p0[ystride + xstride] : 0.0f;
     const float wy = FloatY - (float)OldY;
     float z01 = z0 + wy * (z1 - z0);
     float z23 = z2 + wy * (z3 - z2);
@@ -147,18 +150,28 @@ __global__ void k_rot3dpar(const float *__restrict__ in, float *__restrict__ out
         if (in_circle(head, npix, NewY, NewX)) {
             const float X = (float)NewX - NewAxis;
             const float Y = (float)NewY - NewAxis;
-            v = rot_sample(in + (long)slice * OldDim, (long)Nz * OldDim, OldDim, X, Y, sn, cs,
-                           OldAxis);
+            v = rot_sample(in + (long)slice * OldDim, (long)Nz * OldDim, 1, OldDim, X, Y, sn,
+                           cs, OldAxis);
         }
         out[i] = v;
     }
 }
 
 // ---------------------------------------------------------------------------
-// k_afac: attenuation factors of every view.  One thread per (view, slice,
-// bin) column walks the rows nearest-detector first: S += mu/2, a = exp(-S),
-// S += mu/2 (prjbck.c:212-225, 247-250), with S rounded to float after each
-// double-precision add exactly as the reference's float table.
+// Device layout of the reconstruction engine: every volume is stored with
+// the slice index fastest -- estimate, mu, update and sensitivity as
+// [y][x][z], rotated volumes as [row][bin][z], projections as [bin][z].  The
+// rotation is about the slice axis, so the 64 lanes of a wave (64
+// consecutive slices of one pixel) share one sample position and every
+// bilinear tap is one contiguous 256-byte run.
+// ---------------------------------------------------------------------------
+
+// ---------------------------------------------------------------------------
+// k_afac: attenuation factors of every view.  One thread per (view, bin,
+// slice) column walks the rows nearest-detector first: S += mu/2,
+// a = exp(-S), S += mu/2 (prjbck.c:212-225, 247-250), with S rounded to
+// float after each double-precision add exactly as the reference's float
+// table.  mu is [y][x][z]; afac is [V][row][bin][z].
 // ---------------------------------------------------------------------------
 __global__ void k_afac(const float *__restrict__ mu, float *__restrict__ afac, int V, int Nz,
                        int N, int NRot, const float *__restrict__ fsin,
@@ -170,125 +183,147 @@ __global__ void k_afac(const float *__restrict__ mu, float *__restrict__ afac, i
     if (gid >= cols * V) return;
     const int v = (int)(gid / cols);
     const int c = (int)(gid % cols);
-    const int z = c / NRot, X = c % NRot;
+    const int X = c / Nz, z = c % Nz;
     const float NewAxis = 0.5 * (NRot - 1);
     const float OldAxis = 0.5 * (N - 1);
     const float sn = fsin[v], cs = fcos[v];
-    const float *plane = mu + (long)z * N * N;
+    const float *plane = mu + z;
     float *o = afac + (long)v * NRot * cols + c;
     const float Xf = (float)X - NewAxis;
     float S = 0.0f;
     for (int Y = 0; Y < NRot; ++Y) {
         float m = 0.0f;
         if (in_circle(head, npix, Y, X))
-            m = rot_sample(plane, N, N, Xf, (float)Y - NewAxis, sn, cs, OldAxis);
+            m = rot_sample(plane, (long)N * Nz, Nz, N, Xf, (float)Y - NewAxis, sn, cs, OldAxis);
         S = (float)((double)S + 0.5 * (double)m);
         o[(long)Y * cols] = (float)exp((double)(-S * atnFac));
         S = (float)((double)S + 0.5 * (double)m);
     }
 }
 
+// Launch shape of the per-pixel kernels: a block is 64 consecutive slices x
+// 4 pixels of one row; grid = (slice chunks x pixel quads, rows, batch).  All
+// index arithmetic stays in 32 bits and per-lane divisions are avoided.
+constexpr int PIX_Q = 4;
+
+struct PixIdx {
+    int z, X, Y, b;
+    bool ok;
+};
+
+__device__ __forceinline__ PixIdx pix_index(int Nz, int width)
+{
+    const int nzb = (Nz + 63) >> 6;
+    const int zb = blockIdx.x % nzb, xq = blockIdx.x / nzb;
+    PixIdx p;
+    p.z = zb * 64 + (threadIdx.x & 63);
+    p.X = xq * PIX_Q + (threadIdx.x >> 6);
+    p.Y = blockIdx.y;
+    p.b = blockIdx.z;
+    p.ok = p.z < Nz && p.X < width;
+    return p;
+}
+
+static dim3 pix_grid(int Nz, int width, int rows, int B)
+{
+    return dim3(((Nz + 63) / 64) * ((width + PIX_Q - 1) / PIX_Q), rows, B);
+}
+
 // ---------------------------------------------------------------------------
 // k_att: rotated estimate times attenuation factor for the batch's views.
-// One thread per output (view, row, slice, bin); a wave covers consecutive
-// bins of one row and slice, whose bilinear taps lie on a line inside one
-// contiguous N*N slice of x, so the gathers stay within a few cache lines.
+// One thread per output (view, row, bin, slice), slices fastest.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_att(const float *__restrict__ x,
                                              const float *__restrict__ afac,
-                                             float *__restrict__ att, int B, int Nz, int N,
-                                             int NRot, const int *__restrict__ bview,
+                                             float *__restrict__ att, int Nz, int N, int NRot,
+                                             const int *__restrict__ bview,
                                              const float *__restrict__ fsin,
                                              const float *__restrict__ fcos,
                                              const int *__restrict__ head,
                                              const int *__restrict__ npix)
 {
-    const long pl = (long)Nz * NRot, vol = pl * NRot;
-    const long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (gid >= vol * B) return;
-    const int b = (int)(gid / vol);
-    const long i = gid - (long)b * vol;
-    const int X = (int)(i % NRot);
-    const long r = i / NRot;
-    const int z = (int)(r % Nz);
-    const int Y = (int)(r / Nz);
-    const int v = bview[b];
+    const PixIdx p = pix_index(Nz, NRot);
+    if (!p.ok) return;
+    const long vol = (long)Nz * NRot * NRot;
+    const int i = (p.Y * NRot + p.X) * Nz + p.z;
+    const int v = bview[p.b];
     float val = 0.0f;
-    if (in_circle(head, npix, Y, X)) {
+    if (in_circle(head, npix, p.Y, p.X)) {
         const float NewAxis = 0.5 * (NRot - 1);
         const float OldAxis = 0.5 * (N - 1);
-        val = rot_sample(x + (long)z * N * N, N, N, (float)X - NewAxis, (float)Y - NewAxis,
-                         fsin[v], fcos[v], OldAxis);
+        val = rot_sample(x + p.z, (long)N * Nz, Nz, N, (float)p.X - NewAxis,
+                         (float)p.Y - NewAxis, fsin[v], fcos[v], OldAxis);
     }
     if (afac) val = val * afac[(long)v * vol + i];
-    att[gid] = val;
+    att[(long)p.b * vol + i] = val;
 }
 
 // ---------------------------------------------------------------------------
 // k_drf: per-plane DRF convolution (conv2d_ypad.c:39-108).
 //   PRJ: partial projection of a chunk of rows, sum over rows of
 //        DRF_row (*) att[row]                       (prjbck.c:195-251)
-//   BCK: bplanes[row] = DRF_row (*) (ratio x afac[row]) (prjbck.c:346-393)
-// Block = 256 threads = 64 bins x 4 slice groups; each thread owns ZT
-// consecutive slices of one bin.  The input tile with its halo is staged in
-// LDS column-major (slices contiguous per bin, column stride = 4 mod 32 dwords
-// so 128-bit reads of neighbouring lanes hit disjoint banks) and the padded
-// PSF transposed (its columns contiguous).  For each PSF column a thread reads
-// the column and its input window with 128-bit LDS loads and performs ZT x KY
-// FMAs from registers, so LDS traffic is a few percent of the FMA issue.
-// Rows whose plane index is -1 (distance <= 0) are added without convolution,
-// as the reference skips ConvolveDrfYpad there.
+//   BCK: planes[row] = DRF_row (*) (ratio x afac[row]) (prjbck.c:346-393)
+// Block = 64 bins x ZG slice groups (ZG = 4 or 8 waves); each thread owns
+// DRF_ZT consecutive slices of one bin.  The input tile with its halo is
+// staged in LDS column-major (slices contiguous per bin, column stride = 4
+// mod 32 dwords so 128-bit reads of neighbouring lanes hit disjoint banks)
+// and the padded PSF transposed (its columns contiguous).  For each PSF
+// column a thread reads the column and its input window with 128-bit LDS
+// loads and performs DRF_ZT x KY FMAs from registers.  Two blocks of eight
+// waves share a CU, so one block's staging overlaps the other's FMAs.
+// Rows whose plane index is -1 (distance <= 0) are added without
+// convolution, as the reference skips ConvolveDrfYpad there.
+// Element addressing is by strides, so the same kernel serves the engine's
+// slice-fastest planes and the projector-level entry points' bin-fastest
+// planes:  in (b,row,z,x) = in[b*inB + row*inRow + z*zs + x*xs],
+// afac (v,row,z,x) = afac[v*afB + row*afRow + z*zs + x*xs],
+// out (b,row|chunk,z,x) = out[b*outB + row*outRow + z*outZS + x*outXS].
 // ---------------------------------------------------------------------------
 constexpr int DRF_TX = 64;
-constexpr int DRF_ZG = 4;
+constexpr int DRF_ZT = 16;
 
 struct DrfArgs {
     int NRot, Nz;                 // plane width (bins) and height (slices)
     int nrows;                    // planes (rows) per batch item
-    int bckRowMajor;              // BCK output [B][rows][Nz][NRot] instead of slice-major
     int KX;                       // padded PSF width (odd)
     const float *psf;             // [nDist][KX][KYP] transposed, KYP = bucket padded to 4
     const int *rowKernel;         // [V][NRot]
     const int *rowPad;            // [V][NRot]
     int avg;                      // axial average rows (clamped to [0, Nz])
     const int *bview;             // batch slot -> view
-    const float *in;              // PRJ: att [B][NRot][Nz][NRot]; BCK: ratio [B][Nz][NRot]
-    const float *afac;            // [V][rows][Nz][NRot] factors multiplied into the input, or null
+    const float *in;              // PRJ: rotated, attenuated planes; BCK: ratio plane
+    long inB, inRow;              // BCK: inRow = 0 (one ratio plane per view)
+    const float *afac;            // factors multiplied into the input, or null
+    long afB, afRow;
+    int zs, xs;                   // in/afac slice and bin strides
     float scale;                  // BCK: output scale (fScaleFac)
-    float *out;                   // PRJ: ppart [B][nchunk][Nz][NRot]; BCK: [B][Nz][NRot][NRot]
+    float *out;                   // PRJ: partial sums per chunk; BCK: planes
+    long outB, outRow;
+    int outZS, outXS;
+    int vec4;                     // outZS == 1 and 16-byte aligned rows: float4 stores
     int rchunk, nchunk;           // PRJ row chunking
     int xtiles, ztiles;
 };
 
-template <bool BCK>
-__device__ __forceinline__ float drf_in(const DrfArgs &a, int b, int v, int row, int g,
-                                        int gx)
+__device__ __forceinline__ float drf_in(const DrfArgs &a, int b, int v, int row, int g, int gx)
 {
-    const long pl = (long)a.Nz * a.NRot;
-    if (BCK) {
-        float r = a.in[(long)b * pl + (long)g * a.NRot + gx];
-        if (a.afac) r = r * a.afac[((long)v * a.nrows + row) * pl + (long)g * a.NRot + gx];
-        return r;
-    } else {
-        float r = a.in[((long)b * a.nrows + row) * pl + (long)g * a.NRot + gx];
-        if (a.afac) r = r * a.afac[((long)v * a.nrows + row) * pl + (long)g * a.NRot + gx];
-        return r;
-    }
+    const long off = (long)g * a.zs + (long)gx * a.xs;
+    float r = a.in[(long)b * a.inB + (long)row * a.inRow + off];
+    if (a.afac) r = r * a.afac[(long)v * a.afB + (long)row * a.afRow + off];
+    return r;
 }
 
 // Value of padded-plane row g (global slice index, may be outside [0,Nz)) at
 // bin gx, following pfPad2d/pfAvgRows/pfCopyRows (pad2d.c:21-141).
-template <bool BCK>
-__device__ float drf_stage_val(const DrfArgs &a, int b, int v, int row, int g, int gx,
-                               int pad)
+__device__ float drf_stage_val(const DrfArgs &a, int b, int v, int row, int g, int gx, int pad)
 {
     if (gx < 0 || gx >= a.NRot) return 0.0f;
-    if (g >= 0 && g < a.Nz) return drf_in<BCK>(a, b, v, row, g, gx);
+    if (g >= 0 && g < a.Nz) return drf_in(a, b, v, row, g, gx);
     if (pad <= 0 || a.avg <= 0) return 0.0f;
     if (g < -pad || g >= a.Nz + pad) return 0.0f;
     const int base = g < 0 ? 0 : a.Nz - a.avg;
     float s = 0.0f;
-    for (int k = 0; k < a.avg; ++k) s += drf_in<BCK>(a, b, v, row, base + k, gx);
+    for (int k = 0; k < a.avg; ++k) s += drf_in(a, b, v, row, base + k, gx);
     if (a.avg > 1) s /= (float)a.avg;
     return s;
 }
@@ -299,12 +334,31 @@ __host__ __device__ constexpr int drf_col_stride(int TH)
     return ((TH + 27) / 32) * 32 + 4;
 }
 
-template <int KY, int ZT, bool BCK>
-__global__ __launch_bounds__(256) void k_drf(DrfArgs a)
+template <bool V4>
+__device__ __forceinline__ void drf_store(const DrfArgs &a, float *o, int z, const float *acc,
+                                          float scale)
+{
+    if (V4) {
+#pragma unroll
+        for (int t = 0; t < DRF_ZT; t += 4)
+            if (z + t < a.Nz)
+                *(float4 *)(o + z + t) = make_float4(acc[t] * scale, acc[t + 1] * scale,
+                                                     acc[t + 2] * scale, acc[t + 3] * scale);
+    } else {
+#pragma unroll
+        for (int t = 0; t < DRF_ZT; ++t)
+            if (z + t < a.Nz) o[(long)(z + t) * a.outZS] = acc[t] * scale;
+    }
+}
+
+template <int KY, int ZG, bool BCK>
+__global__ __launch_bounds__(64 * ZG) void k_drf(DrfArgs a)
 {
     extern __shared__ float lds[];
+    constexpr int NT = DRF_TX * ZG;
+    constexpr int ZT = DRF_ZT;
     constexpr int CY = (KY - 1) / 2;
-    constexpr int TZ = DRF_ZG * ZT;
+    constexpr int TZ = ZG * ZT;
     constexpr int TH = TZ + KY - 1;           // tile rows (slices)
     constexpr int CS = drf_col_stride(TH);    // LDS column stride
     constexpr int KYP = (KY + 3) & ~3;        // padded PSF column length
@@ -315,6 +369,7 @@ __global__ __launch_bounds__(256) void k_drf(DrfArgs a)
     const int TW = DRF_TX + KX - 1;           // tile columns (bins)
     float *tile = lds;                        // [TW][CS]
     float *psfT = lds + TW * CS;              // [KX][KYP]
+    const bool zfast = a.zs == 1;
 
     int bid = blockIdx.x;
     const int xt = bid % a.xtiles;
@@ -337,6 +392,7 @@ __global__ __launch_bounds__(256) void k_drf(DrfArgs a)
     const int tx = threadIdx.x % DRF_TX;
     const int zl0 = (threadIdx.x / DRF_TX) * ZT;
     const int x = x0 + tx;
+    const int zb = z0 + zl0;
 
     float acc[ZT];
 #pragma unroll
@@ -352,19 +408,27 @@ __global__ __launch_bounds__(256) void k_drf(DrfArgs a)
             if (x < a.NRot) {
 #pragma unroll
                 for (int t = 0; t < ZT; ++t) {
-                    const int z = z0 + zl0 + t;
-                    if (z < a.Nz) acc[t] += drf_in<BCK>(a, b, v, row, z, x);
+                    const int z = zb + t;
+                    if (z < a.Nz) acc[t] += drf_in(a, b, v, row, z, x);
                 }
             }
         } else {
             const int pad = a.rowPad[(long)v * a.nrows + row];
-            // stage: consecutive threads take consecutive bins (coalesced)
-            for (int e = threadIdx.x; e < TH * TW; e += blockDim.x) {
-                const int q = e / TW, c = e - q * TW;
-                tile[c * CS + q] = drf_stage_val<BCK>(a, b, v, row, z0 - CY + q, x0 - CX + c, pad);
+            // stage: consecutive threads take consecutive elements of the
+            // input's fastest axis (coalesced loads)
+            if (zfast) {
+                for (int e = threadIdx.x; e < TH * TW; e += NT) {
+                    const int c = e / TH, q = e - c * TH;
+                    tile[c * CS + q] = drf_stage_val(a, b, v, row, z0 - CY + q, x0 - CX + c, pad);
+                }
+            } else {
+                for (int e = threadIdx.x; e < TH * TW; e += NT) {
+                    const int q = e / TW, c = e - q * TW;
+                    tile[c * CS + q] = drf_stage_val(a, b, v, row, z0 - CY + q, x0 - CX + c, pad);
+                }
             }
             const float *kp = a.psf + (long)kidx * KX * KYP;
-            for (int e = threadIdx.x; e < KX * KYP; e += blockDim.x) psfT[e] = kp[e];
+            for (int e = threadIdx.x; e < KX * KYP; e += NT) psfT[e] = kp[e];
             __syncthreads();
             for (int i = 0; i < KX; ++i) {
                 float pc[KYP];
@@ -397,25 +461,20 @@ __global__ __launch_bounds__(256) void k_drf(DrfArgs a)
             __syncthreads();
         }
         if (BCK && x < a.NRot) {
-#pragma unroll
-            for (int t = 0; t < ZT; ++t) {
-                const int z = z0 + zl0 + t;
-                if (z < a.Nz)
-                    a.out[a.bckRowMajor ? (((long)b * a.nrows + row) * a.Nz + z) * a.NRot + x
-                                        : (((long)b * a.Nz + z) * a.nrows + row) * a.NRot + x] =
-                        acc[t] * a.scale;
-            }
+            float *o = a.out + (long)b * a.outB + (long)row * a.outRow + (long)x * a.outXS;
+            if (a.vec4)
+                drf_store<true>(a, o, zb, acc, a.scale);
+            else
+                drf_store<false>(a, o, zb, acc, a.scale);
         }
     }
     if (!BCK && x < a.NRot) {
-        const long pl = (long)a.Nz * a.NRot;
         const int ch = r0 / a.rchunk;
-        float *o = a.out + ((long)b * a.nchunk + ch) * pl;
-#pragma unroll
-        for (int t = 0; t < ZT; ++t) {
-            const int z = z0 + zl0 + t;
-            if (z < a.Nz) o[(long)z * a.NRot + x] = acc[t];
-        }
+        float *o = a.out + (long)b * a.outB + (long)ch * a.outRow + (long)x * a.outXS;
+        if (a.vec4)
+            drf_store<true>(a, o, zb, acc, 1.0f);
+        else
+            drf_store<false>(a, o, zb, acc, 1.0f);
     }
 }
 
@@ -425,17 +484,17 @@ __global__ __launch_bounds__(256) void k_drf(DrfArgs a)
 struct RatioArgs {
     int B, Nz, NRot, nchunk;
     const int *bview;
-    const float *ppart;           // [B][nchunk][Nz][NRot]
-    const float *prj;             // [V][Nz][NRot] measured
-    const float *scat;            // [V][Nz][NRot] or null
+    const float *ppart;           // [B][nchunk][NRot][Nz]
+    const float *prj;             // [V][NRot][Nz] measured
+    const float *scat;            // [V][NRot][Nz] or null
     const float *vtsens;          // [V] ViewTime*Sensitivity (float product)
     const float *vt;              // [V]
     const uint8_t *colKeep;       // [V][NRot]
     float scale;                  // fScaleFac
-    float *ratio;                 // [B][Nz][NRot]
+    float *ratio;                 // [B][NRot][Nz]
     // Scatter-model bookkeeping (osemrecon.c:734-788)
     const float *ppart2;          // partials of the projection without scatter model
-    float *prjscat;               // [V][Nz][NRot] stored scatter projection estimate
+    float *prjscat;               // [V][NRot][Nz] stored scatter projection estimate
     int srfMode;                  // 0 none, 2 store estimate, 3 add stored estimate
 };
 
@@ -446,7 +505,7 @@ __global__ void k_ratio(RatioArgs a)
     if (gid >= pl * a.B) return;
     const int b = (int)(gid / pl);
     const long i = gid % pl;
-    const int xb = (int)(i % a.NRot);
+    const int xb = (int)(i / a.Nz);   // projections are [bin][slice]
     const int v = a.bview[b];
     float p = 0.0f;
     for (int c = 0; c < a.nchunk; ++c) p += a.ppart[((long)b * a.nchunk + c) * pl + i];
@@ -490,8 +549,8 @@ __global__ void k_prj_finish(RatioArgs a, float *__restrict__ out)
 // subset order, with the EPS clamp after every view (prjbck.c:408-426).
 // One thread per voxel keeps the running sum in a register, so the
 // order-dependent clamp is reproduced exactly and no atomics are needed.
-// The planes are slice-major, so a wave's taps fall inside one contiguous
-// NRot*NRot plane per view.
+// Voxels are [y][x][z] and planes [row][bin][z]: a wave's 64 slices share
+// one sample position and each tap is a contiguous run.
 //   NORM = false: x *= U * Ninv (osemrecon.c:850-852), U optionally stored.
 //   NORM = true:  FP64 sums, Ninv = U < 1e-6 ? 0 : 1/U (osemrecon.c:317-320).
 // ---------------------------------------------------------------------------
@@ -499,7 +558,7 @@ struct BckArgs {
     int B, N, Nz, NRot;
     const int *bview;
     const float *bsin, *bcos;
-    const float *planes;          // [B][Nz][NRot][NRot]
+    const float *planes;          // [B][NRot][NRot][Nz]
     const int *head, *npix;
     float *x;                     // estimate (updated in place) or null
     const float *ninv;            // sensitivity inverse
@@ -513,24 +572,23 @@ template <bool NORM>
 __global__ __launch_bounds__(256) void k_bck(BckArgs a)
 {
     typedef typename std::conditional<NORM, double, float>::type acc_t;
-    const long NN = (long)a.N * a.N;
-    const long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (gid >= NN * a.Nz) return;
-    const int z = (int)(gid / NN);
-    const int o = (int)(gid - (long)z * NN);
+    const PixIdx p = pix_index(a.Nz, a.N);
+    if (!p.ok) return;
+    const int z = p.z;
+    const long gid = ((long)p.Y * a.N + p.X) * a.Nz + z;
     const int skip = (a.NRot - a.N) / 2;
-    const int Y = o / a.N + skip, X = o % a.N + skip;
+    const int Y = p.Y + skip, X = p.X + skip;
     const float Axis = 0.5 * (a.NRot - 1);
     const bool inside = in_circle(a.head, a.npix, Y, X);
     const float Xf = (float)X - Axis, Yf = (float)Y - Axis;
-    const long PP = (long)a.NRot * a.NRot;
+    const long vol = (long)a.NRot * a.NRot * a.Nz;
     acc_t U = (!NORM && a.accumulate) ? (acc_t)a.u[gid] : (acc_t)0;
     for (int b = 0; b < a.B; ++b) {
         const int v = a.bview[b];
         float val = 0.0f;
         if (inside)
-            val = rot_sample(a.planes + ((long)b * a.Nz + z) * PP, a.NRot, a.NRot, Xf, Yf,
-                             a.bsin[v], a.bcos[v], Axis);
+            val = rot_sample(a.planes + (long)b * vol + z, (long)a.NRot * a.Nz, a.Nz, a.NRot,
+                             Xf, Yf, a.bsin[v], a.bcos[v], Axis);
         U = U + (acc_t)val;
         if ((double)U < 1.0e-5) U = 0;
     }
@@ -562,7 +620,7 @@ __global__ void k_norm_input(int B, int Nz, int NRot, const int *__restrict__ bv
     const int b = (int)(gid / pl);
     const long i = gid % pl;
     const int v = bview[b];
-    const int xb = (int)(i % NRot);
+    const int xb = (int)(i / Nz);
     float m = (colMeasOne[(long)v * NRot + xb] && prj[(long)v * pl + i] >= 0.0f) ? 1.0f : 0.0f;
     out[gid] = m * vt[v];
 }
@@ -572,8 +630,7 @@ __global__ void k_norm_input(int B, int Nz, int NRot, const int *__restrict__ bv
 // ---------------------------------------------------------------------------
 static int div_up(long a, long b) { return (int)((a + b - 1) / b); }
 
-static void launch_drf(hipStream_t stream, bool bck, const DrfArgs &args, int nblocks, int KYb,
-                       int ZT);
+static void launch_drf(hipStream_t stream, bool bck, DrfArgs args, int B, int KYb);
 
 template <typename T>
 struct DBuf {
@@ -681,8 +738,8 @@ struct Recon::Impl {
                  float *dst = nullptr)
     {
         const Geometry &g = c.g;
-        hipLaunchKernelGGL(k_att, dim3(div_up(vol * B, 256)), dim3(256), 0, stream,
-                           src ? src : x.p, afac, dst ? dst : att.p, B, g.Nz, g.N, g.NRot, bview,
+        hipLaunchKernelGGL(k_att, pix_grid(g.Nz, g.NRot, g.NRot, B), dim3(256), 0, stream,
+                           src ? src : x.p, afac, dst ? dst : att.p, g.Nz, g.N, g.NRot, bview,
                            fsin.p, fcos.p, head.p, npix.p);
     }
 
@@ -717,21 +774,63 @@ struct Recon::Impl {
 
     void run_bck(bool norm, const BckArgs &ba)
     {
-        dim3 grid(div_up(M, 256)), block(256);
+        const dim3 grid = pix_grid(c.g.Nz, c.g.N, c.g.N, 1), block(256);
         if (norm)
             hipLaunchKernelGGL(k_bck<true>, grid, block, 0, stream, ba);
         else
             hipLaunchKernelGGL(k_bck<false>, grid, block, 0, stream, ba);
     }
 
-    // Slices per thread: 32 when the volume has at least 128 slices, else 16.
-    int drf_zt() const { return c.g.Nz >= 4 * DRF_ZG * 8 ? 32 : 16; }
-
-    int drf_ztiles() const { return div_up(c.g.Nz, DRF_ZG * drf_zt()); }
-
-    void run_drf(bool bck, const DrfArgs &args, int nblocks, int KYb)
+    // DRF arguments of the engine's slice-fastest planes.  PRJ: input att
+    // [B][row][bin][z], output ppart [B][chunk][bin][z]; BCK: input ratio
+    // [B][bin][z] times afac [V][row][bin][z], output planes [B][row][bin][z].
+    DrfArgs drf_args(bool bck, const int *bview, int model) const
     {
-        launch_drf(stream, bck, args, nblocks, KYb, drf_zt());
+        const Geometry &g = c.g;
+        const long pl = (long)g.Nz * g.NRot;
+        DrfArgs d{};
+        d.NRot = g.NRot;
+        d.Nz = g.Nz;
+        d.nrows = g.NRot;
+        d.KX = c.drf.nDist > 0 ? c.drf.KX : 1;
+        d.psf = psf.p;
+        d.rowKernel = (model & 2) ? rowKernel.p : rowNone.p;
+        d.rowPad = rowPad.p;
+        d.avg = c.axialAvg;
+        d.bview = bview;
+        d.zs = 1;
+        d.xs = g.Nz;
+        d.scale = 1.0f;
+        d.outZS = 1;
+        d.outXS = g.Nz;
+        d.vec4 = g.Nz % 4 == 0;
+        d.xtiles = div_up(g.NRot, DRF_TX);
+        if (bck) {
+            d.in = ratio.p;
+            d.inB = pl;
+            d.inRow = 0;
+            d.afac = (model & 1) ? afac.p : nullptr;
+            d.afB = vol;
+            d.afRow = pl;
+            d.out = planes.p;
+            d.outB = vol;
+            d.outRow = pl;
+        } else {
+            d.in = att.p;
+            d.inB = vol;
+            d.inRow = pl;
+            d.out = ppart.p;
+            d.outB = pl * nchunk;
+            d.outRow = pl;
+            d.rchunk = rchunk;
+            d.nchunk = nchunk;
+        }
+        return d;
+    }
+
+    void run_drf(bool bck, const DrfArgs &args, int B, int KYb)
+    {
+        launch_drf(stream, bck, args, B, KYb);
     }
 };
 
@@ -743,14 +842,14 @@ Recon::Recon(const Config &cfg) : impl_(new Impl), cfg_(cfg)
     const Geometry &g = cfg.g;
     if (g.N <= 0 || g.Nz <= 0 || g.NRot < g.N || g.V <= 0)
         fail(ETYPE_ILLEGAL_VALUE, "bad geometry N=%d Nz=%d NRot=%d V=%d", g.N, g.Nz, g.NRot, g.V);
-    if ((long)g.NRot * g.NRot * sizeof(float) > 160 * 1024)
-        fail(ETYPE_ILLEGAL_VALUE, "NumPixels %d too large: one slice must fit in 160 KiB of LDS",
-             g.NRot);
     if (g.N % 2)
         fail(ETYPE_ILLEGAL_VALUE,
              "odd NumPixels (%d) is not supported: the reference rotation writes past the "
              "row end for odd sizes",
              g.N);
+    if ((long)g.NRot * g.NRot * g.Nz >= INT_MAX)
+        fail(ETYPE_ILLEGAL_VALUE, "volume %d x %d x %d too large (2^31 elements per view)",
+             g.NRot, g.NRot, g.Nz);
     HIP_CHECK(hipStreamCreate(&I.stream));
     I.M = (long)g.N * g.Nz * g.N;
     I.P = (long)g.NRot * g.Nz;
@@ -778,7 +877,7 @@ Recon::Recon(const Config &cfg) : impl_(new Impl), cfg_(cfg)
         if (cfg.esse && cfg.esse->enabled) {
             if (cfg.esse->nx != g.NRot || cfg.esse->ny != g.NRot || cfg.esse->nz != g.Nz)
                 fail(ETYPE_ILLEGAL_VALUE, "scatter model prepared for another volume size");
-            I.esse.reset(new EsseDev(*cfg.esse, Vs, I.stream));
+            I.esse.reset(new EsseDev(*cfg.esse, Vs, I.stream, true));
             I.terms.alloc(I.vol * Vs * cfg.esse->nterms);
         }
         I.ppart2.alloc(I.P * I.nchunk * Vs);
@@ -858,60 +957,115 @@ Recon::~Recon()
     }
 }
 
+// Host-side layout conversions of the engine boundary: volumes [z][y][x] <->
+// device [y][x][z], projections [view][slice][bin] <-> device [view][bin][slice].
+static std::vector<float> vol_to_dev(const float *h, int Nz, int N)
+{
+    std::vector<float> d((size_t)N * N * Nz);
+    for (int z = 0; z < Nz; ++z)
+        for (int y = 0; y < N; ++y) {
+            const float *src = h + ((size_t)z * N + y) * N;
+            float *dst = d.data() + (size_t)y * N * Nz + z;
+            for (int x = 0; x < N; ++x) dst[(size_t)x * Nz] = src[x];
+        }
+    return d;
+}
+
+static void vol_from_dev(const float *d, float *h, int Nz, int N)
+{
+    for (int y = 0; y < N; ++y)
+        for (int x = 0; x < N; ++x) {
+            const float *src = d + ((size_t)y * N + x) * Nz;
+            for (int z = 0; z < Nz; ++z) h[((size_t)z * N + y) * N + x] = src[z];
+        }
+}
+
+static std::vector<float> prj_to_dev(const float *h, int V, int Nz, int NRot)
+{
+    std::vector<float> d((size_t)V * Nz * NRot);
+    for (int v = 0; v < V; ++v)
+        for (int z = 0; z < Nz; ++z)
+            for (int x = 0; x < NRot; ++x)
+                d[((size_t)v * NRot + x) * Nz + z] = h[((size_t)v * Nz + z) * NRot + x];
+    return d;
+}
+
+static void prj_from_dev(const float *d, float *h, int V, int Nz, int NRot)
+{
+    for (int v = 0; v < V; ++v)
+        for (int x = 0; x < NRot; ++x)
+            for (int z = 0; z < Nz; ++z)
+                h[((size_t)v * Nz + z) * NRot + x] = d[((size_t)v * NRot + x) * Nz + z];
+}
+
 void Recon::uploadEstimate(const float *h)
 {
-    impl_->x.upload(h, impl_->M, impl_->stream);
+    const auto d = vol_to_dev(h, cfg_.g.Nz, cfg_.g.N);
+    impl_->x.upload(d.data(), impl_->M, impl_->stream);
     synchronize();
 }
 void Recon::uploadAtn(const float *h)
 {
     if (!impl_->mu.p) return;
-    impl_->mu.upload(h, impl_->M, impl_->stream);
+    const auto d = vol_to_dev(h, cfg_.g.Nz, cfg_.g.N);
+    impl_->mu.upload(d.data(), impl_->M, impl_->stream);
     synchronize();
 }
 void Recon::uploadProjections(const float *h)
 {
-    impl_->prj.upload(h, impl_->P * cfg_.g.V, impl_->stream);
+    const auto d = prj_to_dev(h, cfg_.g.V, cfg_.g.Nz, cfg_.g.NRot);
+    impl_->prj.upload(d.data(), impl_->P * cfg_.g.V, impl_->stream);
     synchronize();
 }
 void Recon::uploadScatterEstimate(const float *h)
 {
     if (!impl_->scat.p) return;
-    impl_->scat.upload(h, impl_->P * cfg_.g.V, impl_->stream);
+    const auto d = prj_to_dev(h, cfg_.g.V, cfg_.g.Nz, cfg_.g.NRot);
+    impl_->scat.upload(d.data(), impl_->P * cfg_.g.V, impl_->stream);
     synchronize();
 }
 void Recon::uploadUpdate(const float *h)
 {
-    impl_->u.upload(h, impl_->M, impl_->stream);
+    const auto d = vol_to_dev(h, cfg_.g.Nz, cfg_.g.N);
+    impl_->u.upload(d.data(), impl_->M, impl_->stream);
     synchronize();
 }
 void Recon::uploadNorm(int s, const float *h)
 {
-    HIP_CHECK(hipMemcpyAsync(impl_->ninv.p + impl_->M * s, h, impl_->M * sizeof(float),
+    const auto d = vol_to_dev(h, cfg_.g.Nz, cfg_.g.N);
+    HIP_CHECK(hipMemcpyAsync(impl_->ninv.p + impl_->M * s, d.data(), impl_->M * sizeof(float),
                              hipMemcpyHostToDevice, impl_->stream));
     synchronize();
 }
 void Recon::downloadEstimate(float *h) const
 {
-    impl_->x.download(h, impl_->M, impl_->stream);
+    std::vector<float> d(impl_->M);
+    impl_->x.download(d.data(), impl_->M, impl_->stream);
     synchronize();
+    vol_from_dev(d.data(), h, cfg_.g.Nz, cfg_.g.N);
 }
 void Recon::downloadUpdate(float *h) const
 {
-    impl_->u.download(h, impl_->M, impl_->stream);
+    std::vector<float> d(impl_->M);
+    impl_->u.download(d.data(), impl_->M, impl_->stream);
     synchronize();
+    vol_from_dev(d.data(), h, cfg_.g.Nz, cfg_.g.N);
 }
 void Recon::downloadNorm(int s, float *h) const
 {
-    HIP_CHECK(hipMemcpyAsync(h, impl_->ninv.p + impl_->M * s, impl_->M * sizeof(float),
+    std::vector<float> d(impl_->M);
+    HIP_CHECK(hipMemcpyAsync(d.data(), impl_->ninv.p + impl_->M * s, impl_->M * sizeof(float),
                              hipMemcpyDeviceToHost, impl_->stream));
     synchronize();
+    vol_from_dev(d.data(), h, cfg_.g.Nz, cfg_.g.N);
 }
 void Recon::downloadScatterProjections(float *h) const
 {
     if (!impl_->prjscat.p) return;
-    impl_->prjscat.download(h, impl_->P * cfg_.g.V, impl_->stream);
+    std::vector<float> d(impl_->P * cfg_.g.V);
+    impl_->prjscat.download(d.data(), impl_->P * cfg_.g.V, impl_->stream);
     synchronize();
+    prj_from_dev(d.data(), h, cfg_.g.V, cfg_.g.Nz, cfg_.g.NRot);
 }
 void Recon::synchronize() const { HIP_CHECK(hipStreamSynchronize(impl_->stream)); }
 void Recon::setTiming(bool on) { impl_->timing = on; }
@@ -955,26 +1109,8 @@ void Recon::precompute(bool norms)
                                I.stream, Vs, g.Nz, g.NRot, bview, I.prj.p, I.colMeasOne.p,
                                I.vt.p, I.ratio.p);
         });
-        DrfArgs da{};
-        da.NRot = g.NRot;
-        da.nrows = g.NRot;
-        da.scale = 1.0f;
-        da.Nz = g.Nz;
-        da.KX = cfg_.drf.nDist > 0 ? cfg_.drf.KX : 1;
-        da.psf = I.psf.p;
-        da.rowKernel = (cfg_.bckModel & 2) ? I.rowKernel.p : nullptr;
-        da.rowPad = I.rowPad.p;
-        da.avg = cfg_.axialAvg;
-        da.bview = bview;
-        da.in = I.ratio.p;
-        da.afac = (cfg_.bckModel & 1) ? I.afac.p : nullptr;
-        da.out = I.planes.p;
-        da.xtiles = div_up(g.NRot, DRF_TX);
-        da.ztiles = I.drf_ztiles();
-        if (!(cfg_.bckModel & 2)) da.rowKernel = I.rowKernelNone();
-        I.launch("drf_bck_norm", [&] {
-            I.run_drf(true, da, da.xtiles * da.ztiles * Vs * g.NRot, KYb);
-        });
+        const DrfArgs da = I.drf_args(true, bview, cfg_.bckModel);
+        I.launch("drf_bck_norm", [&] { I.run_drf(true, da, Vs, KYb); });
         BckArgs ba = I.bck_args(bview);
         ba.ninvOut = I.ninv.p + I.M * s;
         I.launch("norm", [&] { I.run_bck(true, ba); });
@@ -1027,36 +1163,15 @@ void Recon::runViews(int s, int first, int Vs, SrfMode srf, bool accumulate, boo
         I.launch("att", [&] { I.run_att(afac, Vs, bview); });
     else
         I.launch("att", [&] { I.run_att(afac, Vs, bview); });
-    DrfArgs da{};
-    da.NRot = g.NRot;
-    da.nrows = g.NRot;
-    da.scale = 1.0f;
-    da.Nz = g.Nz;
-    da.KX = cfg_.drf.nDist > 0 ? cfg_.drf.KX : 1;
-    da.psf = I.psf.p;
-    da.rowKernel = (cfg_.prjModel & 2) ? I.rowKernel.p : I.rowKernelNone();
-    da.rowPad = I.rowPad.p;
-    da.avg = cfg_.axialAvg;
-    da.bview = bview;
-    da.in = I.att.p;
-    da.afac = nullptr;
-    da.out = I.ppart.p;
-    da.rchunk = I.rchunk;
-    da.nchunk = I.nchunk;
-    da.xtiles = div_up(g.NRot, DRF_TX);
-    da.ztiles = I.drf_ztiles();
-    I.launch("drf_prj", [&] {
-        I.run_drf(false, da, da.xtiles * da.ztiles * Vs * I.nchunk, KYb);
-    });
+    const DrfArgs da = I.drf_args(false, bview, cfg_.prjModel);
+    I.launch("drf_prj", [&] { I.run_drf(false, da, Vs, KYb); });
     if (srf == SRF_MODEL_STORE) {
         // projection without the scatter model for the stored estimate
         // (osemrecon.c:757-786), reusing the rotated estimate
         I.launch("att", [&] { I.run_att(afac, Vs, bview); });
         DrfArgs d2 = da;
         d2.out = I.ppart2.p;
-        I.launch("drf_prj", [&] {
-            I.run_drf(false, d2, d2.xtiles * d2.ztiles * Vs * I.nchunk, KYb);
-        });
+        I.launch("drf_prj", [&] { I.run_drf(false, d2, Vs, KYb); });
     }
     RatioArgs ra{};
     ra.srfMode = srf == SRF_MODEL_STORE ? 2 : srf == SRF_ADD_STORED ? 3 : 0;
@@ -1079,14 +1194,8 @@ void Recon::runViews(int s, int first, int Vs, SrfMode srf, bool accumulate, boo
         hipLaunchKernelGGL(k_ratio, dim3(div_up(I.P * Vs, threads)), dim3(threads), 0,
                            I.stream, ra);
     });
-    DrfArgs db = da;
-    db.rowKernel = (cfg_.bckModel & 2) ? I.rowKernel.p : I.rowKernelNone();
-    db.in = I.ratio.p;
-    db.afac = (cfg_.bckModel & 1) ? I.afac.p : nullptr;
-    db.out = I.planes.p;
-    I.launch("drf_bck", [&] {
-        I.run_drf(true, db, db.xtiles * db.ztiles * Vs * g.NRot, KYb);
-    });
+    const DrfArgs db = I.drf_args(true, bview, cfg_.bckModel);
+    I.launch("drf_bck", [&] { I.run_drf(true, db, Vs, KYb); });
     BckArgs ba = I.bck_args(bview);
     ba.B = Vs;
     ba.x = update ? I.x.p : nullptr;
@@ -1126,26 +1235,8 @@ void Recon::forwardProject(float primary, float *out_host, bool srf)
         } else {
             I.launch("att", [&] { I.run_att(afac, B, bview); });
         }
-        DrfArgs da{};
-        da.NRot = g.NRot;
-        da.nrows = g.NRot;
-        da.scale = 1.0f;
-        da.Nz = g.Nz;
-        da.KX = cfg_.drf.nDist > 0 ? cfg_.drf.KX : 1;
-        da.psf = I.psf.p;
-        da.rowKernel = (cfg_.prjModel & 2) ? I.rowKernel.p : I.rowKernelNone();
-        da.rowPad = I.rowPad.p;
-        da.avg = cfg_.axialAvg;
-        da.bview = bview;
-        da.in = I.att.p;
-        da.out = I.ppart.p;
-        da.rchunk = I.rchunk;
-        da.nchunk = I.nchunk;
-        da.xtiles = div_up(g.NRot, DRF_TX);
-        da.ztiles = I.drf_ztiles();
-        I.launch("drf_prj", [&] {
-            I.run_drf(false, da, da.xtiles * da.ztiles * B * I.nchunk, KYb);
-        });
+        const DrfArgs da = I.drf_args(false, bview, cfg_.prjModel);
+        I.launch("drf_prj", [&] { I.run_drf(false, da, B, KYb); });
         RatioArgs ra{};
         ra.B = B;
         ra.Nz = g.Nz;
@@ -1161,8 +1252,10 @@ void Recon::forwardProject(float primary, float *out_host, bool srf)
                                I.stream, ra, out.p);
         });
     }
-    out.download(out_host, I.P * g.V, I.stream);
+    std::vector<float> hd(I.P * g.V);
+    out.download(hd.data(), I.P * g.V, I.stream);
     synchronize();
+    prj_from_dev(hd.data(), out_host, g.V, g.Nz, g.NRot);
 }
 
 
@@ -1244,7 +1337,7 @@ struct PrjWs {
         if (h != esseKey) {
             if (m->nx != NRot || m->ny != NRot || m->nz != Nz)
                 fail(ETYPE_ILLEGAL_VALUE, "scatter model prepared for another volume size");
-            esse.reset(new EsseDev(*m, 1, stream));
+            esse.reset(new EsseDev(*m, 1, stream, false));
             terms.alloc((long)NRot * Nz * NRot * m->nterms);
             esseKey = h;
         }
@@ -1335,8 +1428,11 @@ struct PrjWs {
         HIP_CHECK(hipGetLastError());
     }
 
+    // DRF arguments of the workspace's bin-fastest planes ([row][z][x], as
+    // the reference's RotReconImage), one view per call.
     DrfArgs drf_args(const int *rk, const int *rp, int avg)
     {
+        const long pl = (long)Nz * NRot;
         DrfArgs da{};
         da.NRot = NRot;
         da.nrows = NRot;
@@ -1348,6 +1444,13 @@ struct PrjWs {
         da.avg = std::max(0, std::min(avg, Nz));
         da.bview = zero.p;
         da.scale = 1.0f;
+        da.zs = NRot;
+        da.xs = 1;
+        da.inRow = pl;
+        da.afRow = pl;
+        da.outRow = pl;
+        da.outZS = NRot;
+        da.outXS = 1;
         da.xtiles = div_up(NRot, DRF_TX);
         return da;
     }
@@ -1378,8 +1481,6 @@ static PrjWs &ws(int Nz, int N, int NRot)
     return *g_prj;
 }
 
-static int zt_for(int Nz) { return Nz >= 4 * DRF_ZG * 8 ? 32 : 16; }
-
 void prjws_project(const PrjCall &c, const float *atn, const float *recon, float *calcprj)
 {
     PrjWs &W = ws(c.Nz, c.N, c.NRot);
@@ -1401,8 +1502,6 @@ void prjws_project(const PrjCall &c, const float *atn, const float *recon, float
     W.rowKernel.upload(c.rowKernel, c.NRot, W.stream);
     W.rowPad.upload(c.rowPad, c.NRot, W.stream);
     DrfArgs da = W.drf_args(W.rowKernel.p, W.rowPad.p, c.avg);
-    const int ZT = zt_for(c.Nz);
-    da.ztiles = div_up(c.Nz, DRF_ZG * ZT);
     da.in = W.rotRecon.p;
     if (c.model & 4) {
         if (!W.scatter.p) fail(ETYPE_ILLEGAL_VALUE, "InitPrj was called without MODEL_SRF");
@@ -1413,7 +1512,7 @@ void prjws_project(const PrjCall &c, const float *atn, const float *recon, float
     da.out = W.ppart.p;
     da.rchunk = 1;
     da.nchunk = c.NRot;
-    launch_drf(W.stream, false, da, da.xtiles * da.ztiles * c.NRot, W.drfKYb, ZT);
+    launch_drf(W.stream, false, da, 1, W.drfKYb);
     hipLaunchKernelGGL(k_sum_chunks, dim3(div_up(cols, 256)), dim3(256), 0, W.stream, W.ppart.p,
                        W.plane.p, cols, c.NRot, c.primary);
     HIP_CHECK(hipGetLastError());
@@ -1439,14 +1538,12 @@ void prjws_backproject(const PrjCall &c, const float *atn, const float *ratio, f
     W.rowPad.upload(c.rowPad, c.NRot, W.stream);
     W.plane.upload(ratio, cols, W.stream);
     DrfArgs da = W.drf_args(W.rowKernel.p, W.rowPad.p, c.avg);
-    const int ZT = zt_for(c.Nz);
-    da.ztiles = div_up(c.Nz, DRF_ZG * ZT);
     da.in = W.plane.p;
+    da.inRow = 0;
     da.afac = (c.model & 1) ? W.afac.p : nullptr;
     da.out = W.rotRecon.p;   // the reference backprojects into RotReconImage
-    da.bckRowMajor = 1;
     da.scale = c.primary;
-    launch_drf(W.stream, true, da, da.xtiles * da.ztiles * c.NRot, W.drfKYb, ZT);
+    launch_drf(W.stream, true, da, 1, W.drfKYb);
     if (c.model & 4) W.scatter_source(c.esse, W.rotRecon.p, c.primaryFac, W.rotRecon.p);
     W.rotate(W.rotRecon.p, W.unrot.p, c.bsin, c.bcos, c.NRot);
     W.img.upload(image, nin, W.stream);
@@ -1493,40 +1590,53 @@ void dev_convolve_plane(const float *in, int inX, int inY, const PaddedDrf &d, i
     da.avg = std::max(0, std::min(avg, inY));
     da.bview = zero.p;
     da.in = din.p;
+    da.zs = inX;
+    da.xs = 1;
     da.out = dout.p;
+    da.outZS = inX;
+    da.outXS = 1;
     da.scale = 1.0f;
-    da.bckRowMajor = 1;
-    const int ZT = zt_for(inY);
     da.xtiles = div_up(inX, DRF_TX);
-    da.ztiles = div_up(inY, DRF_ZG * ZT);
-    launch_drf(st, true, da, da.xtiles * da.ztiles, KYb, ZT);
+    launch_drf(st, true, da, 1, KYb);
     dout.download(out, pl, st);
     HIP_CHECK(hipStreamSynchronize(st));
     HIP_CHECK(hipStreamDestroy(st));
 }
 
-static void launch_drf(hipStream_t stream, bool bck, const DrfArgs &args, int nblocks, int KYb,
-                       int ZT)
+static size_t drf_lds_bytes(int ZG, int KYb, int KX)
 {
-    const int TW = DRF_TX + args.KX - 1;
-    const int TH = DRF_ZG * ZT + KYb - 1;
+    const int TW = DRF_TX + KX - 1;
+    const int TH = ZG * DRF_ZT + KYb - 1;
     const int KYP = (KYb + 3) & ~3;
-    const size_t lds = sizeof(float) * ((size_t)TW * drf_col_stride(TH) + (size_t)args.KX * KYP);
-    if (lds > 160 * 1024) fail(ETYPE_ILLEGAL_VALUE, "DRF PSF too wide (KX=%d)", args.KX);
-    dim3 grid(nblocks), block(256);
-#define IRL_DRF_CASE(K)                                                                     \
-    case K:                                                                                 \
-        if (ZT == 32) {                                                                     \
-            if (bck)                                                                        \
-                hipLaunchKernelGGL((k_drf<K, 32, true>), grid, block, lds, stream, args);   \
-            else                                                                            \
-                hipLaunchKernelGGL((k_drf<K, 32, false>), grid, block, lds, stream, args);  \
-        } else {                                                                            \
-            if (bck)                                                                        \
-                hipLaunchKernelGGL((k_drf<K, 16, true>), grid, block, lds, stream, args);   \
-            else                                                                            \
-                hipLaunchKernelGGL((k_drf<K, 16, false>), grid, block, lds, stream, args);  \
-        }                                                                                   \
+    return sizeof(float) * ((size_t)TW * drf_col_stride(TH) + (size_t)KX * KYP);
+}
+
+// Launches k_drf over B batch items.  Eight slice groups (128 slices per
+// block) when the volume has more than 64 slices and two such blocks fit in
+// a CU's 160 KiB of LDS, else four.
+static void launch_drf(hipStream_t stream, bool bck, DrfArgs args, int B, int KYb)
+{
+    const size_t ldsMax = 160 * 1024;
+    int ZG = (args.Nz > 64 && 2 * drf_lds_bytes(8, KYb, args.KX) <= ldsMax) ? 8 : 4;
+    const size_t lds = drf_lds_bytes(ZG, KYb, args.KX);
+    if (lds > ldsMax) fail(ETYPE_ILLEGAL_VALUE, "DRF PSF too wide (KX=%d)", args.KX);
+    args.ztiles = div_up(args.Nz, ZG * DRF_ZT);
+    const long nblocks = (long)args.xtiles * args.ztiles * B * (bck ? args.nrows : args.nchunk);
+    if (nblocks <= 0 || nblocks > INT_MAX) fail(ETYPE_ILLEGAL_VALUE, "bad DRF launch size");
+    dim3 grid((unsigned)nblocks), block(64 * ZG);
+#define IRL_DRF_CASE(K)                                                                   \
+    case K:                                                                               \
+        if (ZG == 8) {                                                                    \
+            if (bck)                                                                      \
+                hipLaunchKernelGGL((k_drf<K, 8, true>), grid, block, lds, stream, args);  \
+            else                                                                          \
+                hipLaunchKernelGGL((k_drf<K, 8, false>), grid, block, lds, stream, args); \
+        } else {                                                                          \
+            if (bck)                                                                      \
+                hipLaunchKernelGGL((k_drf<K, 4, true>), grid, block, lds, stream, args);  \
+            else                                                                          \
+                hipLaunchKernelGGL((k_drf<K, 4, false>), grid, block, lds, stream, args); \
+        }                                                                                 \
         break;
     switch (KYb) {
         IRL_DRF_CASE(7)

@@ -253,71 +253,102 @@ __global__ __launch_bounds__(256) void k_fft_y_conv(const float2 *__restrict__ b
     }
 }
 
+// Volume layouts: zfast = [b][ny][nx][nz] (the engine's slice-fastest
+// order), else [b][ny][nz][nx] (the reference's image order).
+struct VolStrides {
+    long sy;
+    int sz, sx;
+};
+
+__device__ __forceinline__ VolStrides vol_strides(int nz, int nx, bool zfast)
+{
+    return VolStrides{(long)nz * nx, zfast ? 1 : nx, zfast ? nz : 1};
+}
+
+// Decodes the linear index of an element of a [b][ny](nz,nx) volume.
+__device__ __forceinline__ void vol_coords(long i, int ny, int nz, int nx, bool zfast, long &b,
+                                           int &y, int &z, int &x)
+{
+    long r;
+    if (zfast) {
+        z = (int)(i % nz);
+        r = i / nz;
+        x = (int)(r % nx);
+        r /= nx;
+    } else {
+        x = (int)(i % nx);
+        r = i / nx;
+        z = (int)(r % nz);
+        r /= nz;
+    }
+    y = (int)(r % ny);
+    b = r / ny;
+}
+
 // CollapseByFac (krnlprj3d.c:549-593): sum of c^3 blocks, summed with the
 // depth index slowest and the bin index fastest.
 __global__ void k_collapse(const float *__restrict__ in, float *__restrict__ out, long nOut,
-                           int ny, int nz, int nx, int c)
+                           int ny, int nz, int nx, int c, bool zfast)
 {
     const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (i >= nOut) return;
-    const int nxc = nx / c, nzc = nz / c, nyc = ny / c;
-    const int xc = (int)(i % nxc);
-    long r = i / nxc;
-    const int zc = (int)(r % nzc);
-    r /= nzc;
-    const int yc = (int)(r % nyc);
-    const long b = r / nyc;
+    long b;
+    int yc, zc, xc;
+    vol_coords(i, ny / c, nz / c, nx / c, zfast, b, yc, zc, xc);
+    const VolStrides st = vol_strides(nz, nx, zfast);
     const float *src = in + b * (long)ny * nz * nx;
     float s = 0.0f;
     for (int k = 0; k < c; ++k)
         for (int j = 0; j < c; ++j)
             for (int ii = 0; ii < c; ++ii)
-                s += src[((long)(yc * c + k) * nz + zc * c + j) * nx + xc * c + ii];
+                s += src[(long)(yc * c + k) * st.sy + (long)(zc * c + j) * st.sz +
+                         (long)(xc * c + ii) * st.sx];
     out[i] = s;
 }
 
 // ExpandByFac replication (krnlprj3d.c:610-630): value / c / c / c.
 __global__ void k_expand_rep(const float *__restrict__ in, float *__restrict__ out, long nOut,
-                             int ny, int nz, int nx, int c)
+                             int ny, int nz, int nx, int c, bool zfast)
 {
     const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (i >= nOut) return;
-    const int x = (int)(i % nx);
-    long r = i / nx;
-    const int z = (int)(r % nz);
-    r /= nz;
-    const int y = (int)(r % ny);
-    const long b = r / ny;
+    long b;
+    int y, z, x;
+    vol_coords(i, ny, nz, nx, zfast, b, y, z, x);
     const int nxc = nx / c, nzc = nz / c, nyc = ny / c;
-    out[i] = in[((b * nyc + y / c) * nzc + z / c) * nxc + x / c] / c / c / c;
+    const VolStrides st = vol_strides(nzc, nxc, zfast);
+    out[i] = in[b * (long)nyc * nzc * nxc + (long)(y / c) * st.sy + (long)(z / c) * st.sz +
+                (long)(x / c) * st.sx] / c / c / c;
 }
 
 // ExpandByFac in-place box smoothing along one axis (krnlprj3d.c:632-670):
 // v[k] = (sum of the window k-jj..k+jj, whose earlier entries are already
 // smoothed) / (2jj+1), k walking upward.  One thread per line; axis 0 =
-// depth rows, 1 = slices, 2 = bins of [bt][ny][nz][nx] volumes.
+// depth rows, 1 = slices, 2 = bins of [bt][ny](nz,nx) volumes.
 __global__ void k_smooth(float *__restrict__ img, long BT, int ny, int nz, int nx, int axis,
-                         int jj)
+                         int jj, bool zfast)
 {
-    const long vol = (long)ny * nz * nx;
+    const long pl = (long)nz * nx, vol = pl * ny;
+    const VolStrides st = vol_strides(nz, nx, zfast);
     long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
     long base, stride;
     int len;
     if (axis == 0) {
-        if (t >= BT * nz * nx) return;
-        base = (t / ((long)nz * nx)) * vol + t % ((long)nz * nx);
-        stride = (long)nz * nx;
+        if (t >= BT * pl) return;
+        base = (t / pl) * vol + t % pl;
+        stride = pl;
         len = ny;
     } else if (axis == 1) {
         if (t >= BT * ny * nx) return;
         const long bt = t / ((long)ny * nx), r = t % ((long)ny * nx);
-        base = bt * vol + (r / nx) * (long)nz * nx + r % nx;
-        stride = nx;
+        base = bt * vol + (r / nx) * pl + (r % nx) * st.sx;
+        stride = st.sz;
         len = nz;
     } else {
         if (t >= BT * ny * nz) return;
-        base = t * nx;
-        stride = 1;
+        const long bt = t / ((long)ny * nz), r = t % ((long)ny * nz);
+        base = bt * vol + (r / nz) * pl + (r % nz) * st.sz;
+        stride = st.sx;
         len = nx;
     }
     float *p = img + base;
@@ -331,6 +362,7 @@ __global__ void k_smooth(float *__restrict__ img, long BT, int ny, int nz, int n
 // Effective source, primaries and attenuation of one column (krnlprj3d.c:
 // 826-948; prjbck.c:170-173, 219-222).  Walks the depth rows with the
 // half-pixel running sum of mu, as the reference's three depth loops do.
+// Every operand has the same layout; only the depth stride nz*nx matters.
 __global__ void k_esse_source(const float *__restrict__ act, const float *__restrict__ map,
                               const float *__restrict__ terms, int B, int T, int ny, int nz,
                               int nx, float fAtnCnvFac, float fMu, float cntr, float fDensFac,
@@ -399,19 +431,23 @@ static int lines_for(int P) { return P <= 256 ? 16 : 8; }
         ESSE_CHECK(hipGetLastError());                                                       \
     } while (0)
 
-EsseDev::EsseDev(const EsseModel &m, int maxBatch, hipStream_t stream)
-    : m_(m), maxB_(maxBatch), stream_(stream)
+EsseDev::EsseDev(const EsseModel &m, int maxBatch, hipStream_t stream, bool zfast)
+    : m_(m), maxB_(maxBatch), stream_(stream), zfast_(zfast)
 {
     if (!m_.enabled) return;
     const int Px = m_.Px, Py = m_.Py, Pz = m_.Pz, T = m_.nterms;
+    // The FFT passes run along the layout's fastest axis first ("f"), then
+    // the other in-plane axis ("m"), then depth.
+    Pf_ = zfast ? Pz : Px;
+    Pm_ = zfast ? Px : Pz;
     const long grid = (long)Px * Py * Pz;
-    auto tx = twiddles(Px), ty = twiddles(Py), tz = twiddles(Pz);
-    twx_ = dalloc<float2>(tx.size());
+    auto tf = twiddles(Pf_), ty = twiddles(Py), tm = twiddles(Pm_);
+    twf_ = dalloc<float2>(tf.size());
     twy_ = dalloc<float2>(ty.size());
-    twz_ = dalloc<float2>(tz.size());
-    ESSE_CHECK(hipMemcpyAsync(twx_, tx.data(), tx.size() * sizeof(float2), hipMemcpyHostToDevice, stream_));
+    twm_ = dalloc<float2>(tm.size());
+    ESSE_CHECK(hipMemcpyAsync(twf_, tf.data(), tf.size() * sizeof(float2), hipMemcpyHostToDevice, stream_));
     ESSE_CHECK(hipMemcpyAsync(twy_, ty.data(), ty.size() * sizeof(float2), hipMemcpyHostToDevice, stream_));
-    ESSE_CHECK(hipMemcpyAsync(twz_, tz.data(), tz.size() * sizeof(float2), hipMemcpyHostToDevice, stream_));
+    ESSE_CHECK(hipMemcpyAsync(twm_, tm.data(), tm.size() * sizeof(float2), hipMemcpyHostToDevice, stream_));
     khat_ = dalloc<float2>(grid * T);
     buf_ = dalloc<float2>((size_t)maxB_ * m_.nyc * Pz * Px);
     outk_ = dalloc<float2>((size_t)maxB_ * T * m_.nyc * Pz * Px);
@@ -419,20 +455,31 @@ EsseDev::EsseDev(const EsseModel &m, int maxBatch, hipStream_t stream)
         coll_ = dalloc<float>((size_t)maxB_ * m_.nyc * m_.nzc * m_.nxc);
         res_ = dalloc<float>((size_t)maxB_ * T * m_.nyc * m_.nzc * m_.nxc);
     }
-    // Kernel spectra: full 3-D forward FFT of each folded kernel grid.
+    // Kernel spectra [T][Py][Pm][Pf]: full 3-D forward FFT of each folded
+    // kernel grid (held [Py][Pz][Px], transposed for the slice-fastest layout).
     float *dgrid = dalloc<float>(grid);
     float2 *tmp = dalloc<float2>(grid);
-    const int lx = ilog2(Px), ly = ilog2(Py), lz = ilog2(Pz);
+    const int lf = ilog2(Pf_), ly = ilog2(Py), lm = ilog2(Pm_);
+    std::vector<float> h(grid);
     for (int t = 0; t < T; ++t) {
-        ESSE_CHECK(hipMemcpyAsync(dgrid, m_.grid[t].data(), grid * sizeof(float),
-                                  hipMemcpyHostToDevice, stream_));
-        const long rows = (long)Py * Pz;
-        ESSE_L_DISPATCH(Px, k_fft_x_fwd, cdiv(rows, lines_for(Px)), 2 * (Px + 1) * sizeof(float2),
-                        dgrid, tmp, rows, Pz, Px, Pz, Px, lx, twx_);
-        ESSE_L_DISPATCH(Pz, k_fft_z_fwd, (long)Py * (Px / lines_for(Pz)),
-                        2 * (Pz + 1) * sizeof(float2), tmp, Pz, lz, Px, Pz, Pz, twz_);
-        ESSE_L_DISPATCH(Py, k_fft_y_fwd, (long)Pz * (Px / lines_for(Py)),
-                        2 * (Py + 1) * sizeof(float2), tmp, khat_ + grid * t, Py, ly, Pz, Px, twy_);
+        const float *g = m_.grid[t].data();
+        if (zfast) {
+            for (int y = 0; y < Py; ++y)
+                for (int z = 0; z < Pz; ++z)
+                    for (int x = 0; x < Px; ++x)
+                        h[((size_t)y * Px + x) * Pz + z] = g[((size_t)y * Pz + z) * Px + x];
+            g = h.data();
+        }
+        ESSE_CHECK(hipMemcpyAsync(dgrid, g, grid * sizeof(float), hipMemcpyHostToDevice, stream_));
+        const long rows = (long)Py * Pm_;
+        ESSE_L_DISPATCH(Pf_, k_fft_x_fwd, cdiv(rows, lines_for(Pf_)), 2 * (Pf_ + 1) * sizeof(float2),
+                        dgrid, tmp, rows, Pm_, Pf_, Pm_, Pf_, lf, twf_);
+        ESSE_L_DISPATCH(Pm_, k_fft_z_fwd, (long)Py * (Pf_ / lines_for(Pm_)),
+                        2 * (Pm_ + 1) * sizeof(float2), tmp, Pm_, lm, Pf_, Pm_, Pm_, twm_);
+        ESSE_L_DISPATCH(Py, k_fft_y_fwd, (long)Pm_ * (Pf_ / lines_for(Py)),
+                        2 * (Py + 1) * sizeof(float2), tmp, khat_ + grid * t, Py, ly, Pm_, Pf_, twy_);
+        // the host grid is reused by the next term: wait for the copy
+        ESSE_CHECK(hipStreamSynchronize(stream_));
     }
     ESSE_CHECK(hipStreamSynchronize(stream_));
     (void)hipFree(dgrid);
@@ -443,7 +490,7 @@ EsseDev::~EsseDev()
 {
     if (stream_) (void)hipStreamSynchronize(stream_);
     for (void *p : {(void *)khat_, (void *)buf_, (void *)outk_, (void *)coll_, (void *)res_,
-                    (void *)twx_, (void *)twy_, (void *)twz_})
+                    (void *)twf_, (void *)twy_, (void *)twm_})
         if (p) (void)hipFree(p);
 }
 
@@ -451,41 +498,42 @@ void EsseDev::convolve(const float *act, int B, float *terms)
 {
     if (!m_.enabled) return;
     if (B > maxB_) fail(ETYPE_ILLEGAL_VALUE, "ESSE batch %d exceeds %d", B, maxB_);
-    const int Px = m_.Px, Py = m_.Py, Pz = m_.Pz, T = m_.nterms;
-    const int nx = m_.nxc, ny = m_.nyc, nz = m_.nzc, c = m_.c;
-    const int lx = ilog2(Px), ly = ilog2(Py), lz = ilog2(Pz);
+    const int Py = m_.Py, Pf = Pf_, Pm = Pm_, T = m_.nterms;
+    const int ny = m_.nyc, c = m_.c;
+    const int nf = zfast_ ? m_.nzc : m_.nxc, nm = zfast_ ? m_.nxc : m_.nzc;
+    const int lf = ilog2(Pf), ly = ilog2(Py), lm = ilog2(Pm);
     const float *src = act;
     if (c > 1) {
-        const long n = (long)B * ny * nz * nx;
+        const long n = (long)B * ny * nf * nm;
         hipLaunchKernelGGL(k_collapse, dim3(cdiv(n, 256)), dim3(256), 0, stream_, act, coll_, n,
-                           m_.ny, m_.nz, m_.nx, c);
+                           m_.ny, m_.nz, m_.nx, c, zfast_);
         ESSE_CHECK(hipGetLastError());
         src = coll_;
     }
-    const long rows = (long)B * ny * nz;
-    ESSE_L_DISPATCH(Px, k_fft_x_fwd, cdiv(rows, lines_for(Px)), 2 * (Px + 1) * sizeof(float2), src,
-                    buf_, rows, nz, nx, Pz, Px, lx, twx_);
-    ESSE_L_DISPATCH(Pz, k_fft_z_fwd, (long)B * ny * (Px / lines_for(Pz)),
-                    2 * (Pz + 1) * sizeof(float2), buf_, Pz, lz, Px, nz, Pz, twz_);
-    ESSE_L_DISPATCH(Py, k_fft_y_conv, (long)B * Pz * (Px / lines_for(Py)),
-                    3 * (Py + 1) * sizeof(float2), buf_, khat_, outk_, T, ny, Py, ly, Pz, Px, twy_);
-    ESSE_L_DISPATCH(Pz, k_fft_z_inv, (long)B * T * ny * (Px / lines_for(Pz)),
-                    2 * (Pz + 1) * sizeof(float2), outk_, Pz, lz, Px, Pz, nz, twz_);
-    const float scale = (float)(1.0 / ((double)Px * Py * Pz));
-    const long orows = (long)B * T * ny * nz;
+    const long rows = (long)B * ny * nm;
+    ESSE_L_DISPATCH(Pf, k_fft_x_fwd, cdiv(rows, lines_for(Pf)), 2 * (Pf + 1) * sizeof(float2), src,
+                    buf_, rows, nm, nf, Pm, Pf, lf, twf_);
+    ESSE_L_DISPATCH(Pm, k_fft_z_fwd, (long)B * ny * (Pf / lines_for(Pm)),
+                    2 * (Pm + 1) * sizeof(float2), buf_, Pm, lm, Pf, nm, Pm, twm_);
+    ESSE_L_DISPATCH(Py, k_fft_y_conv, (long)B * Pm * (Pf / lines_for(Py)),
+                    3 * (Py + 1) * sizeof(float2), buf_, khat_, outk_, T, ny, Py, ly, Pm, Pf, twy_);
+    ESSE_L_DISPATCH(Pm, k_fft_z_inv, (long)B * T * ny * (Pf / lines_for(Pm)),
+                    2 * (Pm + 1) * sizeof(float2), outk_, Pm, lm, Pf, Pm, nm, twm_);
+    const float scale = (float)(1.0 / ((double)m_.Px * Py * m_.Pz));
+    const long orows = (long)B * T * ny * nm;
     float *dst = c > 1 ? res_ : terms;
-    ESSE_L_DISPATCH(Px, k_fft_x_inv, cdiv(orows, lines_for(Px)), 2 * (Px + 1) * sizeof(float2),
-                    outk_, dst, orows, nz, nx, Pz, Px, lx, twx_, scale);
+    ESSE_L_DISPATCH(Pf, k_fft_x_inv, cdiv(orows, lines_for(Pf)), 2 * (Pf + 1) * sizeof(float2),
+                    outk_, dst, orows, nm, nf, Pm, Pf, lf, twf_, scale);
     if (c > 1) {
         const int NY = m_.ny, NZ = m_.nz, NX = m_.nx, jj = c / 2;
         const long n = (long)B * T * NY * NZ * NX;
         hipLaunchKernelGGL(k_expand_rep, dim3(cdiv(n, 256)), dim3(256), 0, stream_, res_, terms, n,
-                           NY, NZ, NX, c);
+                           NY, NZ, NX, c, zfast_);
         const long BT = (long)B * T;
         for (int axis = 0; axis < 3; ++axis) {
             const long lines = axis == 0 ? BT * NZ * NX : axis == 1 ? BT * NY * NX : BT * NY * NZ;
             hipLaunchKernelGGL(k_smooth, dim3(cdiv(lines, 256)), dim3(256), 0, stream_, terms, BT,
-                               NY, NZ, NX, axis, jj);
+                               NY, NZ, NX, axis, jj, zfast_);
         }
         ESSE_CHECK(hipGetLastError());
     }
@@ -551,7 +599,7 @@ void esse_handle_effsrc(void *p, const float *act, const float *map, int nx, int
         fail(ETYPE_ILLEGAL_VALUE, "scatter model prepared for %d x %d x %d, called with %d x %d x %d",
              m.ny, m.nz, m.nx, ny, nz, nx);
     if (!h->stream) ESSE_CHECK(hipStreamCreate(&h->stream));
-    if (!h->dev) h->dev.reset(new EsseDev(m, 1, h->stream));
+    if (!h->dev) h->dev.reset(new EsseDev(m, 1, h->stream, false));
     float *da = dalloc<float>(n), *dm = dalloc<float>(n), *dt = dalloc<float>(n * m.nterms),
           *de = dalloc<float>(n);
     ESSE_CHECK(hipMemcpyAsync(da, act, n * sizeof(float), hipMemcpyHostToDevice, h->stream));

@@ -7,23 +7,26 @@
 
 namespace irlamd {
 
-// Batched ESSE: for B rotated activity volumes [B][ny][nz][nx] computes the
-// FFT convolutions with the kernel grids (K, K(mu-muavg), K(mu-muavg)^2),
-// collapsed and expanded as the model asks, into terms [B][T][ny][nz][nx].
+// Batched ESSE: for B rotated activity volumes computes the FFT
+// convolutions with the kernel grids (K, K(mu-muavg), K(mu-muavg)^2),
+// collapsed and expanded as the model asks.  Volumes are [B][ny][nz][nx]
+// (the reference's image order) or, with zfast, [B][ny][nx][nz] (slices
+// fastest, the reconstruction engine's order); terms are [B][T] volumes of
+// the same layout.
 class EsseDev {
 public:
-    EsseDev(const EsseModel &m, int maxBatch, hipStream_t stream);
+    EsseDev(const EsseModel &m, int maxBatch, hipStream_t stream, bool zfast);
     ~EsseDev();
     EsseDev(const EsseDev &) = delete;
     EsseDev &operator=(const EsseDev &) = delete;
 
-    // act: [B][ny][nz][nx] device; terms: [B][T][ny][nz][nx] device.
+    // act: [B] volumes on the device; terms: [B][T] volumes.
     void convolve(const float *act, int B, float *terms);
 
     // Effective scatter source plus primaries (krnlprj3d.c:826-948 and
     // prjbck.c:164-176): q = act*primary + eff(act, map), optionally times
-    // afac (the attenuation factors of the view, [V][ny][nz][nx]).
-    // Writes q * afac into out [B][ny][nz][nx].
+    // afac (the attenuation factors of the view, [V] volumes).
+    // Writes q * afac into out ([B] volumes); all in the same layout.
     void source(const float *act, const float *map, const float *terms, int B, float primary,
                 const float *afac, const int *bview, float *out);
 
@@ -34,12 +37,14 @@ private:
     EsseModel m_;
     int maxB_;
     hipStream_t stream_;
-    float2 *khat_ = nullptr;   // [T][Py][Pz][Px]
-    float2 *buf_ = nullptr;    // [B][nyc][Pz][Px]
-    float2 *outk_ = nullptr;   // [B][T][nyc][Pz][Px]
+    bool zfast_;
+    int Pf_ = 0, Pm_ = 0;      // FFT lengths along the fastest and the other in-plane axis
+    float2 *khat_ = nullptr;   // [T][Py][Pm][Pf]
+    float2 *buf_ = nullptr;    // [B][nyc][Pm][Pf]
+    float2 *outk_ = nullptr;   // [B][T][nyc][Pm][Pf]
     float *coll_ = nullptr;    // collapsed activity [B][nyc][nzc][nxc]
     float *res_ = nullptr;     // collapsed terms [B][T][nyc][nzc][nxc] (c > 1)
-    float2 *twx_ = nullptr, *twy_ = nullptr, *twz_ = nullptr;
+    float2 *twf_ = nullptr, *twy_ = nullptr, *twm_ = nullptr;
 };
 
 }  // namespace irlamd
