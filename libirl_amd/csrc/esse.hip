@@ -1,19 +1,23 @@
 // esse.hip -- ESSE effective scatter source on gfx950 (krnlprj3d.c:549-954).
 //
 // The reference convolves the (optionally collapsed) rotated activity with
-// three kernels by real FFTs on a 2x padded grid.  Here the same circular
-// convolutions are evaluated as linear convolutions with the folded kernels
-// on power-of-two grids (esse_host.cpp), by a batched hand-written Stockham
-// FFT over all views of a subset:
-//   X forward  : rows of the activity -> complex rows (zero rows skipped)
-//   Z forward  : columns along the slices
-//   Y conv     : forward along depth, multiply by every kernel spectrum,
-//                inverse along depth, keep the cropped depth range
-//   Z inverse, X inverse: back to real, cropped to the volume
-// Each pass loads 8 or 16 lines into LDS (line stride padded by one complex
-// element against bank conflicts), runs log2(P) radix-2 stages there and
-// writes the lines back; lines of the strided axes are grouped so that every
-// load and store moves 8-16 consecutive complex values.
+// two or three kernels by real FFTs on a 2x padded grid.  Here the same
+// circular convolutions are evaluated as linear convolutions with the folded
+// kernels on power-of-two grids (esse_host.cpp), by a batched hand-written
+// FFT over all views of a subset.  Axes: "f" = the volume layout's fastest
+// axis, "m" = the other in-plane axis, y = depth.
+//   F forward : real lines along f -> half spectrum (kf <= Pf/2, the input
+//               is real), lines beyond the volume skipped
+//   M forward : lines along m
+//   Y conv    : forward along depth, multiply by every kernel spectrum,
+//               inverse along depth, keep the cropped depth range
+//   M inverse, F inverse (complex-to-real): back to the cropped volume
+// Each line FFT is a radix-16 Stockham transform held in registers: 16
+// values per thread, P/16 threads per line, one LDS exchange per radix-16
+// pass (two passes and one exchange for P = 256).  Lines along strided axes
+// are grouped so that consecutive lanes take consecutive columns
+// (coalesced); lines along the fastest axis put consecutive lanes on
+// consecutive elements.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -34,222 +38,330 @@ namespace irlamd {
                            hipGetErrorString(e_), #expr, __FILE__, __LINE__);            \
     } while (0)
 
-static int ilog2(int p)
-{
-    int l = 0;
-    while ((1 << l) < p) ++l;
-    return l;
-}
-
 static long cdiv(long a, long b) { return (a + b - 1) / b; }
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 {
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
 }
 
-// Stockham radix-2 FFT of L lines of length P (line stride LS) held in LDS.
-// Stage st: n = P >> st, m = n/2, s = 2^st; for p < m, q < s:
-//   y[q + s*2p] = a + b,  y[q + s*(2p+1)] = (a - b) * exp(-+2 pi i p s / P)
-// with a = x[q + s*p], b = x[q + s*(p+m)].  Output is in natural order.
-template <int L>
-__device__ float2 *lds_fft(float2 *a, float2 *b, int P, int logP, int LS,
-                           const float2 *__restrict__ tw, bool inv)
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+// cos and sin of 2*pi*k/16, k = 0..7.
+__device__ constexpr float kCos16[8] = {1.0f, 0.923879532511286756f, 0.707106781186547524f,
+                                        0.382683432365089772f, 0.0f, -0.382683432365089772f,
+                                        -0.707106781186547524f, -0.923879532511286756f};
+__device__ constexpr float kSin16[8] = {0.0f, 0.382683432365089772f, 0.707106781186547524f,
+                                        0.923879532511286756f, 1.0f, 0.923879532511286756f,
+                                        0.707106781186547524f, 0.382683432365089772f};
+
+// z * exp(-+2 pi i K/16) for a compile-time K in [0, 8).
+template <int K, bool INV>
+__device__ __forceinline__ float2 rot16(float2 z)
 {
-    const int half = P >> 1;
-    for (int st = 0; st < logP; ++st) {
-        const int s = 1 << st;
-        const int m = half >> st;
-        for (int idx = threadIdx.x; idx < L * half; idx += blockDim.x) {
-            const int line = idx >> (logP - 1);
-            const int i = idx & (half - 1);
-            const int p = i >> st, q = i & (s - 1);
-            const float2 *x = a + line * LS;
-            float2 *y = b + line * LS;
-            const float2 A = x[q + s * p], B = x[q + s * (p + m)];
-            float2 w = tw[p * s];
-            if (inv) w.y = -w.y;
-            y[q + s * 2 * p] = make_float2(A.x + B.x, A.y + B.y);
-            y[q + s * (2 * p + 1)] = cmul(make_float2(A.x - B.x, A.y - B.y), w);
-        }
-        __syncthreads();
-        float2 *t = a;
-        a = b;
-        b = t;
-    }
-    return a;
+    if (K == 0) return z;
+    if (K == 4) return INV ? make_float2(-z.y, z.x) : make_float2(z.y, -z.x);
+    const float c = kCos16[K], sn = INV ? kSin16[K] : -kSin16[K];
+    return make_float2(fmaf(z.x, c, -z.y * sn), fmaf(z.x, sn, z.y * c));
 }
 
-// X forward: real rows (r = plane*nz + z, plane = b*ny + y) -> buf rows
-// [plane][z][Px]; x >= nx is zero.
-template <int L>
-__global__ __launch_bounds__(256) void k_fft_x_fwd(const float *__restrict__ in,
-                                                   float2 *__restrict__ buf, long rows, int nz,
-                                                   int nx, int Pz, int Px, int logPx,
+// One radix-2 Stockham stage of an r-point DFT held in registers:
+// y[q + 2sp] = a + b, y[q + s(2p+1)] = (a - b) W_r^{ps}, a = x[q + sp],
+// b = x[q + s(p + m)], m = r/(2s).
+template <int r, int s, bool INV>
+__device__ __forceinline__ void dft_stage(float2 *x)
+{
+    constexpr int m = r / (2 * s);
+    float2 y[r];
+#pragma unroll
+    for (int p = 0; p < m; ++p)
+#pragma unroll
+        for (int q = 0; q < s; ++q) {
+            const float2 a = x[q + s * p], b = x[q + s * (p + m)];
+            y[q + 2 * s * p] = cadd(a, b);
+            y[q + s * (2 * p + 1)] = csub(a, b);
+        }
+#pragma unroll
+    for (int p = 0; p < m; ++p)
+#pragma unroll
+        for (int q = 0; q < s; ++q) {
+            // twiddle index p*s in units of 1/r turns = p*s*(16/r) sixteenths
+            constexpr int dummy = 0;
+            (void)dummy;
+            const int k = p * s * (16 / r);
+            float2 &z = y[q + s * (2 * p + 1)];
+            switch (k) {
+            case 0: break;
+            case 1: z = rot16<1, INV>(z); break;
+            case 2: z = rot16<2, INV>(z); break;
+            case 3: z = rot16<3, INV>(z); break;
+            case 4: z = rot16<4, INV>(z); break;
+            case 5: z = rot16<5, INV>(z); break;
+            case 6: z = rot16<6, INV>(z); break;
+            case 7: z = rot16<7, INV>(z); break;
+            }
+        }
+#pragma unroll
+    for (int i = 0; i < r; ++i) x[i] = y[i];
+}
+
+// In-place r-point DFT (r = 2, 4, 8, 16), natural order in and out:
+// x_v <- sum_u x_u exp(-+2 pi i uv/r).
+template <int r, bool INV>
+__device__ __forceinline__ void dft(float2 *x)
+{
+    if constexpr (r >= 2) dft_stage<r, 1, INV>(x);
+    if constexpr (r >= 4) dft_stage<r, 2, INV>(x);
+    if constexpr (r >= 8) dft_stage<r, 4, INV>(x);
+    if constexpr (r >= 16) dft_stage<r, 8, INV>(x);
+}
+
+// Line FFT of length P (power of two, 16..512): 16 values per thread, T =
+// P/16 threads per line.  Thread tt holds x[tt + T*w] in a[w] on entry and
+// X[tt + T*w] on exit.  Stockham passes of radix 16 (the last may be 2, 4
+// or 8) with stride S: unit (p, q), p < m = P/(S r), q < S, reads
+// x[q + S(p + m u)], applies the r-point DFT and the twiddle W_P^{v p S},
+// and writes y[q + S(r p + v)].  Between passes the line goes through LDS
+// with a skew of one element per 16 against bank conflicts.
+template <int P>
+struct FftLine {
+    static constexpr int R = 16;
+    static constexpr int T = P / R;
+    static constexpr int LS = P + P / 16 + 1;  // LDS line stride (float2)
+};
+
+__device__ __forceinline__ int fft_skew(int i) { return i + (i >> 4); }
+
+template <int P, int S, bool INV>
+struct FftPass {
+    static constexpr int r = (P / S) >= 16 ? 16 : P / S;
+    static constexpr int m = P / (S * r);
+    static constexpr int U = 16 / r;          // units per thread
+    static constexpr int T = P / 16;
+    __device__ static int in_pos(int tt, int k, int u)
+    {
+        const int j = tt + T * k, p = j / S, q = j % S;
+        return q + S * (p + m * u);
+    }
+    __device__ static int out_pos(int tt, int k, int v)
+    {
+        const int j = tt + T * k, p = j / S, q = j % S;
+        return q + S * (r * p + v);
+    }
+    __device__ static void compute(float2 *a, int tt, const float2 *__restrict__ tw)
+    {
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            dft<r, INV>(a + k * r);
+            if (m > 1) {
+                const int p = (tt + T * k) / S;
+#pragma unroll
+                for (int v = 1; v < r; ++v) {
+                    float2 w = tw[(v * p * S) & (P - 1)];
+                    if (INV) w.y = -w.y;
+                    a[k * r + v] = cmul(a[k * r + v], w);
+                }
+            }
+        }
+    }
+};
+
+template <int P, int S, bool INV>
+__device__ __forceinline__ void fft_passes(float2 *a, float2 *ldsl, int tt,
+                                           const float2 *__restrict__ tw)
+{
+    using PS = FftPass<P, S, INV>;
+    if constexpr (S > 1) {
+#pragma unroll
+        for (int k = 0; k < PS::U; ++k)
+#pragma unroll
+            for (int u = 0; u < PS::r; ++u) a[k * PS::r + u] = ldsl[fft_skew(PS::in_pos(tt, k, u))];
+    }
+    PS::compute(a, tt, tw);
+    if constexpr (PS::m == 1) {
+        // final pass: a[k r + v] holds X[tt + T k + S v]; reorder to a[w] = X[tt + T w]
+        float2 b[16];
+#pragma unroll
+        for (int k = 0; k < PS::U; ++k)
+#pragma unroll
+            for (int v = 0; v < PS::r; ++v) b[k + PS::U * v] = a[k * PS::r + v];
+#pragma unroll
+        for (int w = 0; w < 16; ++w) a[w] = b[w];
+    } else {
+        __syncthreads();   // earlier readers of the LDS line are done
+#pragma unroll
+        for (int k = 0; k < PS::U; ++k)
+#pragma unroll
+            for (int v = 0; v < PS::r; ++v) ldsl[fft_skew(PS::out_pos(tt, k, v))] = a[k * PS::r + v];
+        __syncthreads();
+        fft_passes<P, S * PS::r, INV>(a, ldsl, tt, tw);
+    }
+}
+
+template <int P, bool INV>
+__device__ __forceinline__ void fft_line(float2 *a, float2 *ldsl, int tt,
+                                         const float2 *__restrict__ tw)
+{
+    fft_passes<P, 1, INV>(a, ldsl, tt, tw);
+}
+
+// F forward: real lines r = (plane, m), m < nm, of nf values (zero to Pf)
+// -> half spectrum buf[(plane*Pm + m)*Wc + k], k < Wc.
+template <int P>
+__global__ __launch_bounds__(256) void k_fft_f_fwd(const float *__restrict__ in,
+                                                   float2 *__restrict__ buf, long rows, int nm,
+                                                   int nf, int Pm, int Wc,
                                                    const float2 *__restrict__ tw)
 {
+    using F = FftLine<P>;
+    constexpr int LB = 256 / F::T;
     extern __shared__ float2 sm[];
-    const int LS = Px + 1;
-    float2 *a = sm, *b = sm + L * LS;
-    const long r0 = (long)blockIdx.x * L;
-    for (int e = threadIdx.x; e < L * Px; e += blockDim.x) {
-        const int l = e >> logPx, x = e & (Px - 1);
-        const long r = r0 + l;
-        a[l * LS + x] = (r < rows && x < nx) ? make_float2(in[r * nx + x], 0.0f)
-                                             : make_float2(0.0f, 0.0f);
+    const int li = threadIdx.x / F::T, tt = threadIdx.x % F::T;
+    const long r = (long)blockIdx.x * LB + li;
+    const bool act = r < rows;
+    float2 a[16];
+#pragma unroll
+    for (int w = 0; w < 16; ++w) {
+        const int n = tt + F::T * w;
+        a[w] = (act && n < nf) ? make_float2(in[r * nf + n], 0.0f) : make_float2(0.0f, 0.0f);
     }
-    __syncthreads();
-    const float2 *res = lds_fft<L>(a, b, Px, logPx, LS, tw, false);
-    for (int e = threadIdx.x; e < L * Px; e += blockDim.x) {
-        const int l = e >> logPx, x = e & (Px - 1);
-        const long r = r0 + l;
-        if (r < rows) {
-            const long plane = r / nz, z = r - plane * nz;
-            buf[(plane * Pz + z) * Px + x] = res[l * LS + x];
-        }
+    fft_line<P, false>(a, sm + li * F::LS, tt, tw);
+    if (!act) return;
+    const long plane = r / nm, m = r - plane * nm;
+    float2 *dst = buf + (plane * Pm + m) * Wc;
+#pragma unroll
+    for (int w = 0; w < 16; ++w) {
+        const int n = tt + F::T * w;
+        if (n < Wc) dst[n] = a[w];
     }
 }
 
-// X inverse: buf rows [plane][z][Px] for z < nz -> real [plane][nz][nx].
-template <int L>
-__global__ __launch_bounds__(256) void k_fft_x_inv(const float2 *__restrict__ buf,
-                                                   float *__restrict__ out, long rows, int nz,
-                                                   int nx, int Pz, int Px, int logPx,
+// F inverse (complex to real): half spectra of the lines (plane, m < nm)
+// -> real lines out[r*nf + n], n < nf, times scale.  X[k] for k > P/2 is
+// conj(X[P-k]).
+template <int P>
+__global__ __launch_bounds__(256) void k_fft_f_inv(const float2 *__restrict__ buf,
+                                                   float *__restrict__ out, long rows, int nm,
+                                                   int nf, int Pm, int Wc,
                                                    const float2 *__restrict__ tw, float scale)
 {
+    using F = FftLine<P>;
+    constexpr int LB = 256 / F::T;
     extern __shared__ float2 sm[];
-    const int LS = Px + 1;
-    float2 *a = sm, *b = sm + L * LS;
-    const long r0 = (long)blockIdx.x * L;
-    for (int e = threadIdx.x; e < L * Px; e += blockDim.x) {
-        const int l = e >> logPx, x = e & (Px - 1);
-        const long r = r0 + l;
+    const int li = threadIdx.x / F::T, tt = threadIdx.x % F::T;
+    const long r = (long)blockIdx.x * LB + li;
+    const bool act = r < rows;
+    const long plane = act ? r / nm : 0, m = act ? r - plane * nm : 0;
+    const float2 *src = buf + (plane * Pm + m) * Wc;
+    float2 a[16];
+#pragma unroll
+    for (int w = 0; w < 16; ++w) {
+        const int n = tt + F::T * w;
         float2 v = make_float2(0.0f, 0.0f);
-        if (r < rows) {
-            const long plane = r / nz, z = r - plane * nz;
-            v = buf[(plane * Pz + z) * Px + x];
+        if (act) {
+            if (n <= P / 2) {
+                v = src[n];
+            } else {
+                v = src[P - n];
+                v.y = -v.y;
+            }
         }
-        a[l * LS + x] = v;
+        a[w] = v;
     }
-    __syncthreads();
-    const float2 *res = lds_fft<L>(a, b, Px, logPx, LS, tw, true);
-    for (int e = threadIdx.x; e < L * Px; e += blockDim.x) {
-        const int l = e >> logPx, x = e & (Px - 1);
-        const long r = r0 + l;
-        if (r < rows && x < nx) out[r * nx + x] = res[l * LS + x].x * scale;
+    fft_line<P, true>(a, sm + li * F::LS, tt, tw);
+    if (!act) return;
+#pragma unroll
+    for (int w = 0; w < 16; ++w) {
+        const int n = tt + F::T * w;
+        if (n < nf) out[r * nf + n] = a[w].x * scale;
     }
 }
 
-// Z pass on planes [plane][Pz][Px]: lines along z at L consecutive kx.
-// Input z >= nzIn is zero; only z < nzOut is written back.
-template <int L, bool INV>
-__device__ void fft_z_body(float2 *__restrict__ data, int Pz, int logPz, int Px, int nzIn,
-                           int nzOut, const float2 *__restrict__ tw)
+// Strided lines: element n of line (bi, c), c < W, is at bi*bstride +
+// n*nstride + c.  Input n >= nIn is zero; n < nOut is written to dst (may
+// alias src).  Blocks take LB consecutive columns of one base.
+template <int P, bool INV>
+__global__ __launch_bounds__(256) void k_fft_strided(const float2 *src, float2 *dst, long bstride,
+                                                     long nstride, int W, int nIn, int nOut,
+                                                     const float2 *__restrict__ tw)
 {
+    using F = FftLine<P>;
+    constexpr int LB = 256 / F::T;
     extern __shared__ float2 sm[];
-    const int LS = Pz + 1;
-    float2 *a = sm, *b = sm + L * LS;
-    const int groups = Px / L;
-    const long plane = blockIdx.x / groups;
-    const int kx0 = (blockIdx.x % groups) * L;
-    float2 *base = data + plane * Pz * Px + kx0;
-    for (int e = threadIdx.x; e < L * Pz; e += blockDim.x) {
-        const int l = e % L, z = e / L;
-        a[l * LS + z] = z < nzIn ? base[(long)z * Px + l] : make_float2(0.0f, 0.0f);
+    const int li = threadIdx.x % LB, tt = threadIdx.x / LB;
+    const int ngroups = (W + LB - 1) / LB;
+    const long bi = blockIdx.x / ngroups;
+    const int c = (int)(blockIdx.x % ngroups) * LB + li;
+    const bool act = c < W;
+    const long off = bi * bstride + c;
+    float2 a[16];
+#pragma unroll
+    for (int w = 0; w < 16; ++w) {
+        const int n = tt + F::T * w;
+        a[w] = (act && n < nIn) ? src[off + n * nstride] : make_float2(0.0f, 0.0f);
     }
-    __syncthreads();
-    const float2 *res = lds_fft<L>(a, b, Pz, logPz, LS, tw, INV);
-    for (int e = threadIdx.x; e < L * Pz; e += blockDim.x) {
-        const int l = e % L, z = e / L;
-        if (z < nzOut) base[(long)z * Px + l] = res[l * LS + z];
-    }
-}
-
-template <int L>
-__global__ __launch_bounds__(256) void k_fft_z_fwd(float2 *__restrict__ data, int Pz, int logPz,
-                                                   int Px, int nzIn, int nzOut,
-                                                   const float2 *__restrict__ tw)
-{
-    fft_z_body<L, false>(data, Pz, logPz, Px, nzIn, nzOut, tw);
-}
-
-template <int L>
-__global__ __launch_bounds__(256) void k_fft_z_inv(float2 *__restrict__ data, int Pz, int logPz,
-                                                   int Px, int nzIn, int nzOut,
-                                                   const float2 *__restrict__ tw)
-{
-    fft_z_body<L, true>(data, Pz, logPz, Px, nzIn, nzOut, tw);
-}
-
-// Y forward only (kernel spectra): src [Py][Pz][Px] -> dst same layout.
-template <int L>
-__global__ __launch_bounds__(256) void k_fft_y_fwd(const float2 *__restrict__ src,
-                                                   float2 *__restrict__ dst, int Py, int logPy,
-                                                   int Pz, int Px,
-                                                   const float2 *__restrict__ tw)
-{
-    extern __shared__ float2 sm[];
-    const int LS = Py + 1;
-    float2 *a = sm, *b = sm + L * LS;
-    const int groups = Px / L;
-    const int z = blockIdx.x / groups;
-    const int kx0 = (blockIdx.x % groups) * L;
-    const long ystride = (long)Pz * Px;
-    for (int e = threadIdx.x; e < L * Py; e += blockDim.x) {
-        const int l = e % L, y = e / L;
-        a[l * LS + y] = src[y * ystride + (long)z * Px + kx0 + l];
-    }
-    __syncthreads();
-    const float2 *res = lds_fft<L>(a, b, Py, logPy, LS, tw, false);
-    for (int e = threadIdx.x; e < L * Py; e += blockDim.x) {
-        const int l = e % L, y = e / L;
-        dst[y * ystride + (long)z * Px + kx0 + l] = res[l * LS + y];
+    fft_line<P, INV>(a, sm + li * F::LS, tt, tw);
+    if (!act) return;
+#pragma unroll
+    for (int w = 0; w < 16; ++w) {
+        const int n = tt + F::T * w;
+        if (n < nOut) dst[off + n * nstride] = a[w];
     }
 }
 
-// Y conv: per batch item b, lines along depth at (z, L consecutive kx):
-// forward FFT (rows y >= ny are zero), then for every kernel t multiply by
-// its spectrum, inverse FFT and keep y < ny.
-template <int L>
+// Y conv: lines along depth at (b, m, column c).  Forward FFT (rows y >= ny
+// are zero), then for every kernel t multiply by its spectrum, inverse FFT
+// and keep y < ny.  The batch index runs fastest over the blocks, so the
+// B blocks that read the same kernel columns run together and share them
+// through L2.
+template <int P>
 __global__ __launch_bounds__(256) void k_fft_y_conv(const float2 *__restrict__ buf,
                                                     const float2 *__restrict__ khat,
-                                                    float2 *__restrict__ outk, int T, int ny,
-                                                    int Py, int logPy, int Pz, int Px,
+                                                    float2 *__restrict__ outk, int B, int Tt,
+                                                    int ny, int Pm, int Wc,
                                                     const float2 *__restrict__ tw)
 {
+    using F = FftLine<P>;
+    constexpr int LB = 256 / F::T;
     extern __shared__ float2 sm[];
-    const int LS = Py + 1;
-    float2 *spec = sm, *a = sm + L * LS, *b = sm + 2 * L * LS;
-    const int groups = Px / L;
+    const int li = threadIdx.x % LB, tt = threadIdx.x / LB;
+    const int ngroups = (Wc + LB - 1) / LB;
     long bid = blockIdx.x;
-    const int kx0 = (int)(bid % groups) * L;
-    bid /= groups;
-    const int z = (int)(bid % Pz);
-    const long bb = bid / Pz;
-    const long ystride = (long)Pz * Px;
-    const float2 *src = buf + bb * ny * ystride + (long)z * Px + kx0;
-    for (int e = threadIdx.x; e < L * Py; e += blockDim.x) {
-        const int l = e % L, y = e / L;
-        a[l * LS + y] = y < ny ? src[y * ystride + l] : make_float2(0.0f, 0.0f);
+    const int b = (int)(bid % B);
+    bid /= B;
+    const int g = (int)(bid % ngroups);
+    const int m = (int)(bid / ngroups);
+    const int c = g * LB + li;
+    const bool act = c < Wc;
+    const long ystride = (long)Pm * Wc;
+    const long col = (long)m * Wc + c;
+    const float2 *src = buf + (long)b * ny * ystride + col;
+    float2 a[16], spec[16];
+#pragma unroll
+    for (int w = 0; w < 16; ++w) {
+        const int n = tt + F::T * w;
+        a[w] = (act && n < ny) ? src[n * ystride] : make_float2(0.0f, 0.0f);
     }
-    __syncthreads();
-    const float2 *f = lds_fft<L>(a, b, Py, logPy, LS, tw, false);
-    for (int e = threadIdx.x; e < L * LS; e += blockDim.x) spec[e] = f[e];
-    __syncthreads();
-    for (int t = 0; t < T; ++t) {
-        const float2 *kt = khat + (long)t * Py * ystride + (long)z * Px + kx0;
-        for (int e = threadIdx.x; e < L * Py; e += blockDim.x) {
-            const int l = e % L, y = e / L;
-            a[l * LS + y] = cmul(spec[l * LS + y], kt[y * ystride + l]);
+    fft_line<P, false>(a, sm + li * F::LS, tt, tw);
+#pragma unroll
+    for (int w = 0; w < 16; ++w) spec[w] = a[w];
+    for (int t = 0; t < Tt; ++t) {
+        const float2 *kt = khat + (long)t * P * ystride + col;
+#pragma unroll
+        for (int w = 0; w < 16; ++w) {
+            const int n = tt + F::T * w;
+            a[w] = act ? cmul(spec[w], kt[n * ystride]) : make_float2(0.0f, 0.0f);
         }
-        __syncthreads();
-        const float2 *g = lds_fft<L>(a, b, Py, logPy, LS, tw, true);
-        float2 *dst = outk + (bb * T + t) * ny * ystride + (long)z * Px + kx0;
-        for (int e = threadIdx.x; e < L * Py; e += blockDim.x) {
-            const int l = e % L, y = e / L;
-            if (y < ny) dst[y * ystride + l] = g[l * LS + y];
+        fft_line<P, true>(a, sm + li * F::LS, tt, tw);
+        if (act) {
+            float2 *dst = outk + ((long)b * Tt + t) * ny * ystride + col;
+#pragma unroll
+            for (int w = 0; w < 16; ++w) {
+                const int n = tt + F::T * w;
+                if (n < ny) dst[n * ystride] = a[w];
+            }
         }
-        __syncthreads();
     }
 }
 
@@ -400,10 +512,11 @@ __global__ void k_esse_source(const float *__restrict__ act, const float *__rest
     }
 }
 
+// W_P^k = exp(-2 pi i k/P), k < P, in double rounded to float.
 static std::vector<float2> twiddles(int P)
 {
-    std::vector<float2> t(P / 2 > 0 ? P / 2 : 1);
-    for (int k = 0; k < P / 2; ++k) {
+    std::vector<float2> t(P);
+    for (int k = 0; k < P; ++k) {
         const double a = -2.0 * M_PI * k / P;
         t[k] = make_float2((float)cos(a), (float)sin(a));
     }
@@ -418,18 +531,60 @@ static T *dalloc(size_t n)
     return p;
 }
 
-static int lines_for(int P) { return P <= 256 ? 16 : 8; }
+static int lines_per_block(int P) { return 256 / (P / 16); }
+static size_t fft_lds(int P) { return (size_t)lines_per_block(P) * (P + P / 16 + 1) * sizeof(float2); }
 
-#define ESSE_L_DISPATCH(P, KERNEL, GRID, LDS1, ...)                                         \
-    do {                                                                                     \
-        if (lines_for(P) == 16)                                                              \
-            hipLaunchKernelGGL((KERNEL<16>), dim3(GRID), dim3(256), (LDS1) * 16, stream_,    \
-                               __VA_ARGS__);                                                 \
-        else                                                                                 \
-            hipLaunchKernelGGL((KERNEL<8>), dim3(GRID), dim3(256), (LDS1) * 8, stream_,      \
-                               __VA_ARGS__);                                                 \
-        ESSE_CHECK(hipGetLastError());                                                       \
+// Half-spectrum width along the fastest axis: kf <= P/2, padded to 16.
+static int half_width(int P) { return std::min(P, ((P / 2 + 1 + 15) / 16) * 16); }
+
+#define ESSE_P_DISPATCH(P, LAUNCH)                                                           \
+    do {                                                                                      \
+        switch (P) {                                                                          \
+        case 16: LAUNCH(16); break;                                                           \
+        case 32: LAUNCH(32); break;                                                           \
+        case 64: LAUNCH(64); break;                                                           \
+        case 128: LAUNCH(128); break;                                                         \
+        case 256: LAUNCH(256); break;                                                         \
+        case 512: LAUNCH(512); break;                                                         \
+        default: fail(ETYPE_ILLEGAL_VALUE, "unsupported ESSE FFT length %d", P);               \
+        }                                                                                     \
+        ESSE_CHECK(hipGetLastError());                                                        \
     } while (0)
+
+void EsseDev::fft_f_fwd(const float *in, float2 *out, long rows, int nm, int nf)
+{
+    const dim3 grid((unsigned)cdiv(rows, lines_per_block(Pf_)));
+#define L_(PP) hipLaunchKernelGGL(k_fft_f_fwd<PP>, grid, dim3(256), fft_lds(PP), stream_, in, out, \
+                                  rows, nm, nf, Pm_, Wc_, twf_)
+    ESSE_P_DISPATCH(Pf_, L_);
+#undef L_
+}
+
+void EsseDev::fft_f_inv(const float2 *in, float *out, long rows, int nm, int nf, float scale)
+{
+    const dim3 grid((unsigned)cdiv(rows, lines_per_block(Pf_)));
+#define L_(PP) hipLaunchKernelGGL(k_fft_f_inv<PP>, grid, dim3(256), fft_lds(PP), stream_, in, out, \
+                                  rows, nm, nf, Pm_, Wc_, twf_, scale)
+    ESSE_P_DISPATCH(Pf_, L_);
+#undef L_
+}
+
+void EsseDev::fft_strided(int P, bool inv, const float2 *src, float2 *dst, long nbases,
+                          long bstride, long nstride, int W, int nIn, int nOut, const float2 *tw)
+{
+    const dim3 grid((unsigned)(nbases * cdiv(W, lines_per_block(P))));
+#define L_(PP)                                                                                  \
+    do {                                                                                        \
+        if (inv)                                                                                \
+            hipLaunchKernelGGL((k_fft_strided<PP, true>), grid, dim3(256), fft_lds(PP), stream_, \
+                               src, dst, bstride, nstride, W, nIn, nOut, tw);                   \
+        else                                                                                    \
+            hipLaunchKernelGGL((k_fft_strided<PP, false>), grid, dim3(256), fft_lds(PP),        \
+                               stream_, src, dst, bstride, nstride, W, nIn, nOut, tw);          \
+    } while (0)
+    ESSE_P_DISPATCH(P, L_);
+#undef L_
+}
 
 EsseDev::EsseDev(const EsseModel &m, int maxBatch, hipStream_t stream, bool zfast)
     : m_(m), maxB_(maxBatch), stream_(stream), zfast_(zfast)
@@ -440,7 +595,7 @@ EsseDev::EsseDev(const EsseModel &m, int maxBatch, hipStream_t stream, bool zfas
     // the other in-plane axis ("m"), then depth.
     Pf_ = zfast ? Pz : Px;
     Pm_ = zfast ? Px : Pz;
-    const long grid = (long)Px * Py * Pz;
+    Wc_ = half_width(Pf_);
     auto tf = twiddles(Pf_), ty = twiddles(Py), tm = twiddles(Pm_);
     twf_ = dalloc<float2>(tf.size());
     twy_ = dalloc<float2>(ty.size());
@@ -448,18 +603,19 @@ EsseDev::EsseDev(const EsseModel &m, int maxBatch, hipStream_t stream, bool zfas
     ESSE_CHECK(hipMemcpyAsync(twf_, tf.data(), tf.size() * sizeof(float2), hipMemcpyHostToDevice, stream_));
     ESSE_CHECK(hipMemcpyAsync(twy_, ty.data(), ty.size() * sizeof(float2), hipMemcpyHostToDevice, stream_));
     ESSE_CHECK(hipMemcpyAsync(twm_, tm.data(), tm.size() * sizeof(float2), hipMemcpyHostToDevice, stream_));
-    khat_ = dalloc<float2>(grid * T);
-    buf_ = dalloc<float2>((size_t)maxB_ * m_.nyc * Pz * Px);
-    outk_ = dalloc<float2>((size_t)maxB_ * T * m_.nyc * Pz * Px);
+    const long spec = (long)Py * Pm_ * Wc_;
+    khat_ = dalloc<float2>(spec * T);
+    buf_ = dalloc<float2>((size_t)maxB_ * m_.nyc * Pm_ * Wc_);
+    outk_ = dalloc<float2>((size_t)maxB_ * T * m_.nyc * Pm_ * Wc_);
     if (m_.c > 1) {
         coll_ = dalloc<float>((size_t)maxB_ * m_.nyc * m_.nzc * m_.nxc);
         res_ = dalloc<float>((size_t)maxB_ * T * m_.nyc * m_.nzc * m_.nxc);
     }
-    // Kernel spectra [T][Py][Pm][Pf]: full 3-D forward FFT of each folded
-    // kernel grid (held [Py][Pz][Px], transposed for the slice-fastest layout).
+    // Kernel spectra [T][Py][Pm][Wc]: 3-D forward FFT of each folded kernel
+    // grid (held [Py][Pz][Px], transposed for the slice-fastest layout).
+    const long grid = (long)Px * Py * Pz;
     float *dgrid = dalloc<float>(grid);
-    float2 *tmp = dalloc<float2>(grid);
-    const int lf = ilog2(Pf_), ly = ilog2(Py), lm = ilog2(Pm_);
+    float2 *tmp = dalloc<float2>(spec);
     std::vector<float> h(grid);
     for (int t = 0; t < T; ++t) {
         const float *g = m_.grid[t].data();
@@ -471,13 +627,10 @@ EsseDev::EsseDev(const EsseModel &m, int maxBatch, hipStream_t stream, bool zfas
             g = h.data();
         }
         ESSE_CHECK(hipMemcpyAsync(dgrid, g, grid * sizeof(float), hipMemcpyHostToDevice, stream_));
-        const long rows = (long)Py * Pm_;
-        ESSE_L_DISPATCH(Pf_, k_fft_x_fwd, cdiv(rows, lines_for(Pf_)), 2 * (Pf_ + 1) * sizeof(float2),
-                        dgrid, tmp, rows, Pm_, Pf_, Pm_, Pf_, lf, twf_);
-        ESSE_L_DISPATCH(Pm_, k_fft_z_fwd, (long)Py * (Pf_ / lines_for(Pm_)),
-                        2 * (Pm_ + 1) * sizeof(float2), tmp, Pm_, lm, Pf_, Pm_, Pm_, twm_);
-        ESSE_L_DISPATCH(Py, k_fft_y_fwd, (long)Pm_ * (Pf_ / lines_for(Py)),
-                        2 * (Py + 1) * sizeof(float2), tmp, khat_ + grid * t, Py, ly, Pm_, Pf_, twy_);
+        fft_f_fwd(dgrid, tmp, (long)Py * Pm_, Pm_, Pf_);
+        fft_strided(Pm_, false, tmp, tmp, Py, (long)Pm_ * Wc_, Wc_, Wc_, Pm_, Pm_, twm_);
+        fft_strided(Py, false, tmp, khat_ + spec * t, 1, 0, (long)Pm_ * Wc_, Pm_ * Wc_, Py, Py,
+                    twy_);
         // the host grid is reused by the next term: wait for the copy
         ESSE_CHECK(hipStreamSynchronize(stream_));
     }
@@ -498,10 +651,9 @@ void EsseDev::convolve(const float *act, int B, float *terms)
 {
     if (!m_.enabled) return;
     if (B > maxB_) fail(ETYPE_ILLEGAL_VALUE, "ESSE batch %d exceeds %d", B, maxB_);
-    const int Py = m_.Py, Pf = Pf_, Pm = Pm_, T = m_.nterms;
+    const int Py = m_.Py, Pm = Pm_, Wc = Wc_, T = m_.nterms;
     const int ny = m_.nyc, c = m_.c;
     const int nf = zfast_ ? m_.nzc : m_.nxc, nm = zfast_ ? m_.nxc : m_.nzc;
-    const int lf = ilog2(Pf), ly = ilog2(Py), lm = ilog2(Pm);
     const float *src = act;
     if (c > 1) {
         const long n = (long)B * ny * nf * nm;
@@ -510,20 +662,20 @@ void EsseDev::convolve(const float *act, int B, float *terms)
         ESSE_CHECK(hipGetLastError());
         src = coll_;
     }
-    const long rows = (long)B * ny * nm;
-    ESSE_L_DISPATCH(Pf, k_fft_x_fwd, cdiv(rows, lines_for(Pf)), 2 * (Pf + 1) * sizeof(float2), src,
-                    buf_, rows, nm, nf, Pm, Pf, lf, twf_);
-    ESSE_L_DISPATCH(Pm, k_fft_z_fwd, (long)B * ny * (Pf / lines_for(Pm)),
-                    2 * (Pm + 1) * sizeof(float2), buf_, Pm, lm, Pf, nm, Pm, twm_);
-    ESSE_L_DISPATCH(Py, k_fft_y_conv, (long)B * Pm * (Pf / lines_for(Py)),
-                    3 * (Py + 1) * sizeof(float2), buf_, khat_, outk_, T, ny, Py, ly, Pm, Pf, twy_);
-    ESSE_L_DISPATCH(Pm, k_fft_z_inv, (long)B * T * ny * (Pf / lines_for(Pm)),
-                    2 * (Pm + 1) * sizeof(float2), outk_, Pm, lm, Pf, Pm, nm, twm_);
+    const long plane = (long)Pm * Wc;
+    fft_f_fwd(src, buf_, (long)B * ny * nm, nm, nf);
+    fft_strided(Pm, false, buf_, buf_, (long)B * ny, plane, Wc, Wc, nm, Pm, twm_);
+    {
+        const dim3 grid((unsigned)((long)B * cdiv(Wc, lines_per_block(Py)) * Pm));
+#define L_(PP) hipLaunchKernelGGL(k_fft_y_conv<PP>, grid, dim3(256), fft_lds(PP), stream_, buf_, \
+                                  khat_, outk_, B, T, ny, Pm, Wc, twy_)
+        ESSE_P_DISPATCH(Py, L_);
+#undef L_
+    }
+    fft_strided(Pm, true, outk_, outk_, (long)B * T * ny, plane, Wc, Wc, Pm, nm, twm_);
     const float scale = (float)(1.0 / ((double)m_.Px * Py * m_.Pz));
-    const long orows = (long)B * T * ny * nm;
     float *dst = c > 1 ? res_ : terms;
-    ESSE_L_DISPATCH(Pf, k_fft_x_inv, cdiv(orows, lines_for(Pf)), 2 * (Pf + 1) * sizeof(float2),
-                    outk_, dst, orows, nm, nf, Pm, Pf, lf, twf_, scale);
+    fft_f_inv(outk_, dst, (long)B * T * ny * nm, nm, nf, scale);
     if (c > 1) {
         const int NY = m_.ny, NZ = m_.nz, NX = m_.nx, jj = c / 2;
         const long n = (long)B * T * NY * NZ * NX;

@@ -34,14 +34,20 @@ public:
     int terms() const { return m_.nterms; }
 
 private:
+    void fft_f_fwd(const float *in, float2 *out, long rows, int nm, int nf);
+    void fft_f_inv(const float2 *in, float *out, long rows, int nm, int nf, float scale);
+    void fft_strided(int P, bool inv, const float2 *src, float2 *dst, long nbases, long bstride,
+                     long nstride, int W, int nIn, int nOut, const float2 *tw);
+
     EsseModel m_;
     int maxB_;
     hipStream_t stream_;
     bool zfast_;
     int Pf_ = 0, Pm_ = 0;      // FFT lengths along the fastest and the other in-plane axis
-    float2 *khat_ = nullptr;   // [T][Py][Pm][Pf]
-    float2 *buf_ = nullptr;    // [B][nyc][Pm][Pf]
-    float2 *outk_ = nullptr;   // [B][T][nyc][Pm][Pf]
+    int Wc_ = 0;               // stored half-spectrum width along the fastest axis
+    float2 *khat_ = nullptr;   // [T][Py][Pm][Wc]
+    float2 *buf_ = nullptr;    // [B][nyc][Pm][Wc]
+    float2 *outk_ = nullptr;   // [B][T][nyc][Pm][Wc]
     float *coll_ = nullptr;    // collapsed activity [B][nyc][nzc][nxc]
     float *res_ = nullptr;     // collapsed terms [B][T][nyc][nzc][nxc] (c > 1)
     float2 *twf_ = nullptr, *twy_ = nullptr, *twm_ = nullptr;
