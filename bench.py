@@ -137,6 +137,20 @@ def cpu_baseline(w, nviews=15):
                        "(%.1f s), scaled x%.2f" % (nviews, dt, p.NumAngPerSubset / nviews))
 
 
+def measured_traffic(stage):
+    """HBM bytes per launch of a stage's kernel, from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this workload (counters
+    scaled by the copy-kernel calibration), or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "traffic_c2.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return t["kernels"][t["stages"][stage]]["hbm_bytes_per_dispatch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,6 +158,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-esse", action="store_true", help="skip the ESSE workload line")
+    ap.add_argument("--only-esse", action="store_true",
+                    help="time only the ESSE workload (for profiling runs)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -162,6 +178,15 @@ def main():
               "(requested --gpus %d)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     tmpdir = tempfile.mkdtemp(prefix="irlbench")
+    if args.only_esse:
+        we = build_workload(tmpdir, name="c3")
+        rec_e, dte, st_e = time_subiterations(we, args.steps, args.warmup)
+        rec_e.close()
+        print(json.dumps({"workload": "osem-128^3x120v-atn-drf31-esse33-8subsets (FP32)",
+                          "value": args.steps / dte, "unit": "subiters/s",
+                          "stages_ms_per_launch": {k: v[0] / max(v[1], 1)
+                                                   for k, v in st_e.items()}}))
+        return
     w = build_workload(tmpdir)
     rec, dt, stages = time_subiterations(w, args.steps, args.warmup)
     S = rec.num_subsets
@@ -171,6 +196,7 @@ def main():
     dom = max(("drf_prj", "drf_bck"), key=lambda k: stages.get(k, (0, 1))[0])
     dom_ms, dom_n = stages[dom]
     achieved = avg_flops / (dom_ms / dom_n * 1e-3) / 1e12
+    traffic = measured_traffic(dom)
     out = {
         "metric": METRIC,
         "value": args.steps / dt,
@@ -190,7 +216,7 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"kernel": dom, "bound": "mfma", "achieved": achieved,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None},
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic},
         "stages_ms_per_launch": {k: v[0] / max(v[1], 1) for k, v in stages.items()},
     }
     rec.close()

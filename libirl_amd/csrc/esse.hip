@@ -312,9 +312,21 @@ __global__ __launch_bounds__(256) void k_fft_strided(const float2 *src, float2 *
 
 // Y conv: lines along depth at (b, m, column c).  Forward FFT (rows y >= ny
 // are zero), then for every kernel t multiply by its spectrum, inverse FFT
-// and keep y < ny.  The batch index runs fastest over the blocks, so the
-// B blocks that read the same kernel columns run together and share them
-// through L2.
+// and keep y < ny.  The batch index runs fastest over the logical blocks,
+// so the B blocks that read the same kernel columns run together on one XCD
+// and share them through its L2.
+// Logical block index of a launch whose blocks are dispatched round-robin
+// over the chip's 8 XCDs (each with its own L2): logically consecutive
+// blocks land on the same XCD, so blocks that share inputs share its L2.
+__device__ __forceinline__ long xcd_block()
+{
+    constexpr int NXCD = 8;
+    const long n = gridDim.x, p = blockIdx.x;
+    const long per = n / NXCD, rem = n % NXCD;
+    const long xcd = p % NXCD, k = p / NXCD;
+    return (xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per) + k;
+}
+
 template <int P>
 __global__ __launch_bounds__(256) void k_fft_y_conv(const float2 *__restrict__ buf,
                                                     const float2 *__restrict__ khat,
@@ -327,7 +339,7 @@ __global__ __launch_bounds__(256) void k_fft_y_conv(const float2 *__restrict__ b
     extern __shared__ float2 sm[];
     const int li = threadIdx.x % LB, tt = threadIdx.x / LB;
     const int ngroups = (Wc + LB - 1) / LB;
-    long bid = blockIdx.x;
+    long bid = xcd_block();
     const int b = (int)(bid % B);
     bid /= B;
     const int g = (int)(bid % ngroups);
