@@ -13,6 +13,7 @@
  *   BackProject            replaces irlprivate.h:491-511 (prjbck.c:254-427)
  *   FreePrj                replaces irlprivate.h:517     (prjbck.c:432-453)
  *   rot3dpar               replaces irlprivate.h:521-529 (rot3dpar.c:19-198)
+ *   vShearsRotate          replaces irlprivate.h:255-262 (fastrotate.c:330-563)
  *   ConvolveDrfYpad        replaces irlprivate.h:233-242 (convolve_ypad.c:22-96)
  *   sReadDrfTab            replaces drftabio.h:37-41     (drftabio.c:150-275)
  *   vWriteDrfTab           replaces drftabio.h:30-35     (drftabio.c:57-148)
@@ -26,8 +27,9 @@
  * the same angle reuses them, and BackProject leaves its rotated planes in
  * the rotated-estimate workspace.  A DrfTab_t passed to Project/BackProject
  * is copied to the GPU on first use and cached until FreePrj/InitPrj; it
- * must not be modified in between.  bFastRotate is accepted and ignored:
- * rotation is always the bilinear rot3dpar rotation.
+ * must not be modified in between.  bFastRotate selects the three-shear
+ * rotation (vShearsRotate, which needs iNumRotPixs == iNumPixels) instead of
+ * rot3dpar, as in the reference.
  */
 #ifndef IRLPRJ_H
 #define IRLPRJ_H
@@ -82,6 +84,12 @@ void FreePrj(void);
 
 void rot3dpar(float fAngle, float *OldMatrix, float *NewMatrix, int PB_FLAG,
               int NumSlices, int NumPixels, int NumRotPixs);
+
+/* Three-shear rotation of a [y][z][x] volume of NumPixels x NumSlices x
+ * NumPixels (PB_FLAG is unused, as in the reference).  The circle extents
+ * that the reference sets up in vInitFastRotate are computed per call. */
+void vShearsRotate(float fAngle, float *pfOldMatrix, float *pfRotMatrix, int PB_FLAG,
+                   int NumSlices, int NumPixels);
 
 void ConvolveDrfYpad(float *pfIn, int inX, int inY, Drf_t *psDrf, int bFFTConvolve,
                      int iYpad, int iYavg, float *pfOut);

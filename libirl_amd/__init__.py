@@ -189,6 +189,7 @@ def load():
                                                      _FP, _FP, ctypes.c_float, ctypes.c_float] +
                                 [ctypes.c_int] * 4)
     lib.rot3dpar.argtypes = [ctypes.c_float, _FP, _FP] + [ctypes.c_int] * 4
+    lib.vShearsRotate.argtypes = [ctypes.c_float, _FP, _FP] + [ctypes.c_int] * 3
     lib.ConvolveDrfYpad.argtypes = [_FP, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Drf)] + \
         [ctypes.c_int] * 3 + [_FP]
     lib.irlamd_last_error.restype = ctypes.c_int
@@ -264,22 +265,31 @@ def free_prj():
 
 
 def project(nz, n, nrot, model, pw, angle, cfcr, drftab, atn, recon, primary=1.0, atnfac=1.0,
-            pad=0, avg=0, atn_rotated=0, act_rotated=0):
+            pad=0, avg=0, atn_rotated=0, act_rotated=0, fast=0):
     """Project (prjbck.c:68-252) of internal-order [y][z][x] float32 volumes;
-    returns the [slice][bin] projection."""
+    returns the [slice][bin] projection.  fast selects the shear rotation."""
     out = np.zeros((nz, nrot), np.float32)
     load().Project(nz, n, nrot, model, pw, 0, angle, cfcr, _tab(drftab), None, fptr(atn),
                    fptr(recon), fptr(out), primary, atnfac, pad, avg, atn_rotated,
-                   act_rotated, 0)
+                   act_rotated, int(fast))
     return out
 
 
 def backproject(nz, n, nrot, model, pw, angle, cfcr, drftab, atn, ratio, image, primary=1.0,
-                atnfac=1.0, pad=0, avg=0, atn_rotated=0):
+                atnfac=1.0, pad=0, avg=0, atn_rotated=0, fast=0):
     """BackProject (prjbck.c:254-427): adds into 'image' in place."""
     load().BackProject(nz, n, nrot, model, pw, 0, angle, cfcr, _tab(drftab), None, fptr(atn),
-                       fptr(ratio), fptr(image), primary, atnfac, pad, avg, atn_rotated, 0)
+                       fptr(ratio), fptr(image), primary, atnfac, pad, avg, atn_rotated,
+                       int(fast))
     return image
+
+
+def shears_rotate(angle, vol, nz, n):
+    """vShearsRotate (fastrotate.c:330-563) of a [y][z][x] float32 volume."""
+    out = np.zeros((n, nz, n), np.float32)
+    load().vShearsRotate(angle, fptr(np.ascontiguousarray(vol, np.float32)), fptr(out), 0, nz,
+                         n)
+    return out
 
 
 def rot3dpar(angle, vol, pb_flag, nz, n, nrot):

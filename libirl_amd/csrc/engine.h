@@ -44,7 +44,26 @@ struct ViewTab {
     float fsin, fcos;             // forward rotation (0.5*pi + Angle)
     float bsin, bcos;             // back rotation (-Angle - 0.5*pi)
     float viewtime, sensitivity;
+    float fang = 0, bang = 0;     // the two rotation angles as passed (float)
 };
+
+// Geometry of the three-shear rotation of an N x N slice by one angle
+// (fastrotate.c:330-563), computed on the host in the reference's float
+// arithmetic.  The angle is folded into (-pi/4, pi/4] with the 90/180
+// degree index flips b90/b180; the intermediate grid is mDim = N + 2*iAdd.
+// Row j of the x shifts uses shift fl[j] with weights f1[j], f2[j] (the
+// first and the last pass use the same values); column i of the y shift
+// uses jT[i], g1[i], g2[i] and writes rows [jS[i], jE[i]).
+struct ShearGeom {
+    int iAdd = 0, mDim = 0, b90 = 0, b180 = 0;
+    std::vector<int> fl;
+    std::vector<float> f1, f2;
+    std::vector<int> jT, jS, jE;
+    std::vector<float> g1, g2;
+};
+ShearGeom shear_geometry(float fAngle, int N);
+// Row extents [start, end) of the shear rotation's circle (fastrotate.c:573-589).
+void shear_extents(int N, std::vector<int> &start, std::vector<int> &end);
 
 struct Geometry {
     int N = 0;      // NumPixels
@@ -81,6 +100,7 @@ public:
         std::vector<uint8_t> colKeep;      // V*NRot (ratio mask)
         bool hasScatterEstimate = false;
         bool hasAtn = false;
+        bool shears = false;               // three-shear rotation (bFastRotate) instead of rot3dpar
         std::shared_ptr<const EsseModel> esse;  // scatter model, or null
     };
 
@@ -159,6 +179,8 @@ struct PrjCall {
     const PaddedDrf *drf;          // null when DRF is not modelled
     const void *esse;              // pvInitESSEScatterModel handle (MODEL_SRF), may be null
     float primaryFac;              // fPrimaryFac as passed (scatter source scaling)
+    int fast;                      // bFastRotate: three-shear rotation
+    float fang, bang;              // forward and back rotation angles (float)
 };
 
 void prjws_init(int Nz, int N, int NRot, int model);
@@ -171,5 +193,6 @@ void dev_convolve_plane(const float *in, int inX, int inY, const PaddedDrf &d, i
 // Stateless single-call helpers behind the prjbck-level C entry points.
 void dev_rot3dpar(float angle, const float *in_host, float *out_host, int pbFlag,
                   int Nz, int N, int NRot);
+void dev_shears_rotate(float angle, const float *in_host, float *out_host, int Nz, int N);
 
 }  // namespace irlamd

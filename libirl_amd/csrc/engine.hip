@@ -132,6 +132,90 @@ __device__ __forceinline__ bool in_circle(const int *__restrict__ head,
 }
 
 // ---------------------------------------------------------------------------
+// Three-shear rotation (fastrotate.c:330-563) as a gather.  The reference
+// shifts whole rows and columns of a slice through two intermediate grids;
+// here each output pixel recomputes the four first-pass values and the two
+// second-pass values it depends on, with the same float operations, so the
+// result is bit-identical.  Tables per angle slot a (host-computed in the
+// reference's float arithmetic): par[a] = (iAdd, mDim, b90, b180), per row
+// fl/f1/f2 [a][N], per column of the grown grid jT/jS/jE/g1/g2 [a][MD].
+// ---------------------------------------------------------------------------
+struct ShearDev {
+    int N, MD;
+    const int4 *par;
+    const int *fl;
+    const float *f1, *f2;
+    const int *jT, *jS, *jE;
+    const float *g1, *g2;
+    const int *pS, *pE;           // circle row extents [start, end)
+};
+
+// First pass (x shift with the 90/180 degree flips), value at row jk,
+// column ic of the grown grid.  'src' is the slice's element (0, 0); source
+// pixel (row y, column x) is src[y*ys + x*xs].
+__device__ __forceinline__ float shear_x1(const ShearDev &t, int a, int4 par,
+                                          const float *__restrict__ src, long ys, long xs,
+                                          int jk, int ic)
+{
+    const int N = t.N, iAdd = par.x;
+    const int j = jk - iAdd;
+    if (j < 0 || j >= N) return 0.0f;
+    const int fl = t.fl[a * N + j];
+    if (ic < t.pS[j] - fl + iAdd || ic >= t.pE[j] - fl - 1 + iAdd) return 0.0f;
+    const float f1 = t.f1[a * N + j], f2 = t.f2[a * N + j];
+    float v0, v1;
+    if (!par.w) {
+        const int c = ic + fl - iAdd;
+        if (!par.z) {   // rows as they are
+            v0 = src[j * ys + c * xs];
+            v1 = src[j * ys + (c + 1) * xs];
+        } else {        // 90 degrees: column N-j-1 read along rows
+            const int jj = N - j - 1;
+            v0 = src[c * ys + jj * xs];
+            v1 = src[(c + 1) * ys + jj * xs];
+        }
+    } else {
+        const int c = N - ic - (fl - iAdd + 1);
+        if (!par.z) {   // 180 degrees: row N-j-1 read backwards
+            const int jj = N - j - 1;
+            v0 = src[jj * ys + c * xs];
+            v1 = src[jj * ys + (c - 1) * xs];
+        } else {        // 270 degrees: column j read backwards along rows
+            v0 = src[c * ys + j * xs];
+            v1 = src[(c - 1) * ys + j * xs];
+        }
+    }
+    return f1 * v0 + f2 * v1;
+}
+
+// Second pass (y shift) at row jr, column ic of the grown grid.
+__device__ __forceinline__ float shear_y(const ShearDev &t, int a, int4 par,
+                                         const float *__restrict__ src, long ys, long xs, int jr,
+                                         int ic)
+{
+    if (ic < 0 || ic >= par.y) return 0.0f;
+    const int k = a * t.MD + ic;
+    if (jr < t.jS[k] || jr >= t.jE[k]) return 0.0f;
+    const int r = jr + t.jT[k];
+    return t.g1[k] * shear_x1(t, a, par, src, ys, xs, r, ic) +
+           t.g2[k] * shear_x1(t, a, par, src, ys, xs, r + 1, ic);
+}
+
+// Rotated value at output pixel (row j, column i): third pass (x shift) into
+// the circle trimmed by one pixel per side.
+__device__ __forceinline__ float shear_sample(const ShearDev &t, int a,
+                                              const float *__restrict__ src, long ys, long xs,
+                                              int i, int j)
+{
+    if (i < t.pS[j] + 1 || i >= t.pE[j] - 1) return 0.0f;
+    const int4 par = t.par[a];
+    const int fl = t.fl[a * t.N + j];
+    const int jj = j + par.x, c = i + fl + par.x;
+    return t.f1[a * t.N + j] * shear_y(t, a, par, src, ys, xs, jj, c) +
+           t.f2[a * t.N + j] * shear_y(t, a, par, src, ys, xs, jj, c + 1);
+}
+
+// ---------------------------------------------------------------------------
 // k_rot3dpar: generic rotation (rot3dpar.c:19-198), one thread per output.
 // ---------------------------------------------------------------------------
 __global__ void k_rot3dpar(const float *__restrict__ in, float *__restrict__ out, int Nz,
@@ -166,6 +250,21 @@ __global__ void k_rot3dpar(const float *__restrict__ in, float *__restrict__ out
 // bilinear tap is one contiguous 256-byte run.
 // ---------------------------------------------------------------------------
 
+// Three-shear rotation of an N x Nz x N volume in the reference's [y][z][x]
+// order (vShearsRotate), angle slot a of the tables.
+__global__ void k_shear_rot(const float *__restrict__ in, float *__restrict__ out, int Nz, int N,
+                            ShearDev sd, int a)
+{
+    const long total = (long)N * Nz * N;
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int x = (int)(i % N);
+    const long r = i / N;
+    const int z = (int)(r % Nz);
+    const int y = (int)(r / Nz);
+    out[i] = shear_sample(sd, a, in + (long)z * N, (long)Nz * N, 1, x, y);
+}
+
 // ---------------------------------------------------------------------------
 // k_afac: attenuation factors of every view.  One thread per (view, bin,
 // slice) column walks the rows nearest-detector first: S += mu/2,
@@ -173,10 +272,11 @@ __global__ void k_rot3dpar(const float *__restrict__ in, float *__restrict__ out
 // float after each double-precision add exactly as the reference's float
 // table.  mu is [y][x][z]; afac is [V][row][bin][z].
 // ---------------------------------------------------------------------------
+template <bool SHEAR>
 __global__ void k_afac(const float *__restrict__ mu, float *__restrict__ afac, int V, int Nz,
                        int N, int NRot, const float *__restrict__ fsin,
                        const float *__restrict__ fcos, float atnFac,
-                       const int *__restrict__ head, const int *__restrict__ npix)
+                       const int *__restrict__ head, const int *__restrict__ npix, ShearDev sd)
 {
     const long cols = (long)Nz * NRot;
     const long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -193,7 +293,9 @@ __global__ void k_afac(const float *__restrict__ mu, float *__restrict__ afac, i
     float S = 0.0f;
     for (int Y = 0; Y < NRot; ++Y) {
         float m = 0.0f;
-        if (in_circle(head, npix, Y, X))
+        if (SHEAR)
+            m = shear_sample(sd, 2 * v, plane, (long)N * Nz, Nz, X, Y);
+        else if (in_circle(head, npix, Y, X))
             m = rot_sample(plane, (long)N * Nz, Nz, N, Xf, (float)Y - NewAxis, sn, cs, OldAxis);
         S = (float)((double)S + 0.5 * (double)m);
         o[(long)Y * cols] = (float)exp((double)(-S * atnFac));
@@ -233,6 +335,7 @@ static dim3 pix_grid(int Nz, int width, int rows, int B)
 // k_att: rotated estimate times attenuation factor for the batch's views.
 // One thread per output (view, row, bin, slice), slices fastest.
 // ---------------------------------------------------------------------------
+template <bool SHEAR>
 __global__ __launch_bounds__(256) void k_att(const float *__restrict__ x,
                                              const float *__restrict__ afac,
                                              float *__restrict__ att, int Nz, int N, int NRot,
@@ -240,7 +343,7 @@ __global__ __launch_bounds__(256) void k_att(const float *__restrict__ x,
                                              const float *__restrict__ fsin,
                                              const float *__restrict__ fcos,
                                              const int *__restrict__ head,
-                                             const int *__restrict__ npix)
+                                             const int *__restrict__ npix, ShearDev sd)
 {
     const PixIdx p = pix_index(Nz, NRot);
     if (!p.ok) return;
@@ -248,7 +351,9 @@ __global__ __launch_bounds__(256) void k_att(const float *__restrict__ x,
     const int i = (p.Y * NRot + p.X) * Nz + p.z;
     const int v = bview[p.b];
     float val = 0.0f;
-    if (in_circle(head, npix, p.Y, p.X)) {
+    if (SHEAR) {
+        val = shear_sample(sd, 2 * v, x + p.z, (long)N * Nz, Nz, p.X, p.Y);
+    } else if (in_circle(head, npix, p.Y, p.X)) {
         const float NewAxis = 0.5 * (NRot - 1);
         const float OldAxis = 0.5 * (N - 1);
         val = rot_sample(x + p.z, (long)N * Nz, Nz, N, (float)p.X - NewAxis,
@@ -566,9 +671,10 @@ struct BckArgs {
     int accumulate;               // start from u instead of 0
     int writeU;
     float *ninvOut;               // NORM: output
+    ShearDev sd;                  // shear tables (back rotation in slot 2v+1)
 };
 
-template <bool NORM>
+template <bool NORM, bool SHEAR>
 __global__ __launch_bounds__(256) void k_bck(BckArgs a)
 {
     typedef typename std::conditional<NORM, double, float>::type acc_t;
@@ -586,7 +692,10 @@ __global__ __launch_bounds__(256) void k_bck(BckArgs a)
     for (int b = 0; b < a.B; ++b) {
         const int v = a.bview[b];
         float val = 0.0f;
-        if (inside)
+        if (SHEAR)
+            val = shear_sample(a.sd, 2 * v + 1, a.planes + (long)b * vol + z,
+                               (long)a.NRot * a.Nz, a.Nz, X, Y);
+        else if (inside)
             val = rot_sample(a.planes + (long)b * vol + z, (long)a.NRot * a.Nz, a.Nz, a.NRot,
                              Xf, Yf, a.bsin[v], a.bcos[v], Axis);
         U = U + (acc_t)val;
@@ -662,6 +771,82 @@ struct DBuf {
     ~DBuf() { free(); }
 };
 
+// Device copy of the shear tables of a set of angle slots.
+struct ShearSet {
+    DBuf<int> par, fl, jT, jS, jE, pS, pE;
+    DBuf<float> f1, f2, g1, g2;
+    int N = 0, MD = 0;
+
+    void set(const std::vector<ShearGeom> &sg, int n, hipStream_t st)
+    {
+        N = n;
+        const int A = (int)sg.size();
+        MD = 0;
+        for (auto &q : sg) MD = std::max(MD, q.mDim);
+        std::vector<int> hpar(4 * A), hfl((size_t)A * N), hjT((size_t)A * MD, 0),
+            hjS((size_t)A * MD, 0), hjE((size_t)A * MD, 0);
+        std::vector<float> hf1((size_t)A * N), hf2((size_t)A * N), hg1((size_t)A * MD, 0.0f),
+            hg2((size_t)A * MD, 0.0f);
+        for (int a = 0; a < A; ++a) {
+            const ShearGeom &q = sg[a];
+            hpar[4 * a] = q.iAdd;
+            hpar[4 * a + 1] = q.mDim;
+            hpar[4 * a + 2] = q.b90;
+            hpar[4 * a + 3] = q.b180;
+            std::copy(q.fl.begin(), q.fl.end(), hfl.begin() + (size_t)a * N);
+            std::copy(q.f1.begin(), q.f1.end(), hf1.begin() + (size_t)a * N);
+            std::copy(q.f2.begin(), q.f2.end(), hf2.begin() + (size_t)a * N);
+            std::copy(q.jT.begin(), q.jT.end(), hjT.begin() + (size_t)a * MD);
+            std::copy(q.jS.begin(), q.jS.end(), hjS.begin() + (size_t)a * MD);
+            std::copy(q.jE.begin(), q.jE.end(), hjE.begin() + (size_t)a * MD);
+            std::copy(q.g1.begin(), q.g1.end(), hg1.begin() + (size_t)a * MD);
+            std::copy(q.g2.begin(), q.g2.end(), hg2.begin() + (size_t)a * MD);
+        }
+        std::vector<int> hpS, hpE;
+        shear_extents(N, hpS, hpE);
+        auto upi = [&](DBuf<int> &d, const std::vector<int> &h) {
+            if (d.n < h.size()) d.alloc(h.size());
+            d.upload(h.data(), h.size(), st);
+        };
+        auto upf = [&](DBuf<float> &d, const std::vector<float> &h) {
+            if (d.n < h.size()) d.alloc(h.size());
+            d.upload(h.data(), h.size(), st);
+        };
+        upi(par, hpar);
+        upi(fl, hfl);
+        upi(jT, hjT);
+        upi(jS, hjS);
+        upi(jE, hjE);
+        upi(pS, hpS);
+        upi(pE, hpE);
+        upf(f1, hf1);
+        upf(f2, hf2);
+        upf(g1, hg1);
+        upf(g2, hg2);
+        // the host vectors die with this call
+        HIP_CHECK(hipStreamSynchronize(st));
+    }
+
+    ShearDev dev() const
+    {
+        ShearDev d{};
+        d.N = N;
+        d.MD = MD;
+        d.par = (const int4 *)par.p;
+        d.fl = fl.p;
+        d.f1 = f1.p;
+        d.f2 = f2.p;
+        d.jT = jT.p;
+        d.jS = jS.p;
+        d.jE = jE.p;
+        d.g1 = g1.p;
+        d.g2 = g2.p;
+        d.pS = pS.p;
+        d.pE = pE.p;
+        return d;
+    }
+};
+
 struct Recon::Impl {
     Config c;
     hipStream_t stream = nullptr;
@@ -672,6 +857,9 @@ struct Recon::Impl {
     std::unique_ptr<EsseDev> esse;
     DBuf<float> fsin, fcos, bsin, bcos, vt, vtsens, psf;
     DBuf<int> rowKernel, rowPad, subsets, head, npix, rowNone;
+    // three-shear rotation tables, slot 2v (forward) and 2v+1 (back)
+    ShearSet shset;
+    ShearDev shear() const { return shset.dev(); }
     DBuf<uint8_t> colMeasOne, colKeep;
     std::vector<int> hhead, hnpix;
     const int *rowKernelNone() const { return rowNone.p; }
@@ -738,9 +926,14 @@ struct Recon::Impl {
                  float *dst = nullptr)
     {
         const Geometry &g = c.g;
-        hipLaunchKernelGGL(k_att, pix_grid(g.Nz, g.NRot, g.NRot, B), dim3(256), 0, stream,
-                           src ? src : x.p, afac, dst ? dst : att.p, g.Nz, g.N, g.NRot, bview,
-                           fsin.p, fcos.p, head.p, npix.p);
+        if (c.shears)
+            hipLaunchKernelGGL(k_att<true>, pix_grid(g.Nz, g.NRot, g.NRot, B), dim3(256), 0,
+                               stream, src ? src : x.p, afac, dst ? dst : att.p, g.Nz, g.N,
+                               g.NRot, bview, fsin.p, fcos.p, head.p, npix.p, shear());
+        else
+            hipLaunchKernelGGL(k_att<false>, pix_grid(g.Nz, g.NRot, g.NRot, B), dim3(256), 0,
+                               stream, src ? src : x.p, afac, dst ? dst : att.p, g.Nz, g.N,
+                               g.NRot, bview, fsin.p, fcos.p, head.p, npix.p, shear());
     }
 
     // Attenuated scatter-plus-primary source of the batch's views into att
@@ -775,10 +968,16 @@ struct Recon::Impl {
     void run_bck(bool norm, const BckArgs &ba)
     {
         const dim3 grid = pix_grid(c.g.Nz, c.g.N, c.g.N, 1), block(256);
-        if (norm)
-            hipLaunchKernelGGL(k_bck<true>, grid, block, 0, stream, ba);
+        BckArgs b = ba;
+        b.sd = shear();
+        if (norm && c.shears)
+            hipLaunchKernelGGL((k_bck<true, true>), grid, block, 0, stream, b);
+        else if (norm)
+            hipLaunchKernelGGL((k_bck<true, false>), grid, block, 0, stream, b);
+        else if (c.shears)
+            hipLaunchKernelGGL((k_bck<false, true>), grid, block, 0, stream, b);
         else
-            hipLaunchKernelGGL(k_bck<false>, grid, block, 0, stream, ba);
+            hipLaunchKernelGGL((k_bck<false, false>), grid, block, 0, stream, b);
     }
 
     // DRF arguments of the engine's slice-fastest planes.  PRJ: input att
@@ -932,6 +1131,16 @@ Recon::Recon(const Config &cfg) : impl_(new Impl), cfg_(cfg)
     I.rowNone.upload(none.data(), none.size(), I.stream);
     I.subsets.alloc(cfg.subsets.size());
     I.subsets.upload(cfg.subsets.data(), cfg.subsets.size(), I.stream);
+    if (cfg.shears) {
+        if (g.N != g.NRot)
+            fail(ETYPE_ILLEGAL_VALUE, "the shear rotation needs NumRotPixs == NumPixels");
+        std::vector<ShearGeom> sg;
+        for (int v = 0; v < g.V; ++v) {
+            sg.push_back(shear_geometry(cfg.views[v].fang, g.N));
+            sg.push_back(shear_geometry(cfg.views[v].bang, g.N));
+        }
+        I.shset.set(sg, g.N, I.stream);
+    }
     circle_rows(g.NRot, I.hhead, I.hnpix);
     I.head.alloc(g.NRot);
     I.head.upload(I.hhead.data(), g.NRot, I.stream);
@@ -1094,9 +1303,14 @@ void Recon::precompute(bool norms)
     if (cfg_.hasAtn) {
         const long n = (long)g.V * g.Nz * g.NRot;
         I.launch("afac", [&] {
-            hipLaunchKernelGGL(k_afac, dim3(div_up(n, threads)), dim3(threads), 0, I.stream,
-                               I.mu.p, I.afac.p, g.V, g.Nz, g.N, g.NRot, I.fsin.p, I.fcos.p,
-                               1.0f, I.head.p, I.npix.p);
+            if (cfg_.shears)
+                hipLaunchKernelGGL(k_afac<true>, dim3(div_up(n, threads)), dim3(threads), 0,
+                                   I.stream, I.mu.p, I.afac.p, g.V, g.Nz, g.N, g.NRot, I.fsin.p,
+                                   I.fcos.p, 1.0f, I.head.p, I.npix.p, I.shear());
+            else
+                hipLaunchKernelGGL(k_afac<false>, dim3(div_up(n, threads)), dim3(threads), 0,
+                                   I.stream, I.mu.p, I.afac.p, g.V, g.Nz, g.N, g.NRot, I.fsin.p,
+                                   I.fcos.p, 1.0f, I.head.p, I.npix.p, I.shear());
         });
     }
     // Sensitivity images: backproject mask*ViewTime for each subset.
@@ -1328,6 +1542,33 @@ struct PrjWs {
     const void *esseKey = nullptr;
     std::unique_ptr<EsseDev> esse;
     DBuf<float> scatter, terms;
+    ShearSet shears;              // slot 0: forward angle, slot 1: back angle of the call
+
+    // Rotation of a [y][z][x] volume: the three-shear rotation (slot a of
+    // the call's tables) when 'fast', else rot3dpar.
+    void rotate_any(bool fast, int a, const float *src, float *dst, float sn, float cs,
+                    int oldDim)
+    {
+        if (!fast) {
+            rotate(src, dst, sn, cs, oldDim);
+            return;
+        }
+        const long n = (long)NRot * Nz * NRot;
+        hipLaunchKernelGGL(k_shear_rot, dim3(div_up(n, 256)), dim3(256), 0, stream, src, dst, Nz,
+                           NRot, shears.dev(), a);
+        HIP_CHECK(hipGetLastError());
+    }
+
+    void set_shears(const PrjCall &c)
+    {
+        if (!c.fast) return;
+        if (c.N != c.NRot)
+            fail(ETYPE_ILLEGAL_VALUE, "the shear rotation needs NumRotPixs == NumPixels");
+        if (c.N % 2)
+            fail(ETYPE_ILLEGAL_VALUE, "odd NumPixels (%d) is not supported: the reference "
+                 "shear rotation reads past the row end for odd sizes", c.N);
+        shears.set({shear_geometry(c.fang, NRot), shear_geometry(c.bang, NRot)}, NRot, stream);
+    }
 
     // ESSE device instance of the handle (kept until another handle comes).
     EsseDev *esse_for(const void *h)
@@ -1486,13 +1727,14 @@ void prjws_project(const PrjCall &c, const float *atn, const float *recon, float
     PrjWs &W = ws(c.Nz, c.N, c.NRot);
     const long nin = (long)c.N * c.Nz * c.N;
     const bool useAtn = (c.model & 1) || (c.model & 4);
+    W.set_shears(c);
     if (!c.actRotated) {
         W.vol.upload(recon, nin, W.stream);
-        W.rotate(W.vol.p, W.rotRecon.p, c.fsin, c.fcos, c.N);
+        W.rotate_any(c.fast, 0, W.vol.p, W.rotRecon.p, c.fsin, c.fcos, c.N);
     }
     if (useAtn && !c.atnRotated) {
         W.vol.upload(atn, nin, W.stream);
-        W.rotate(W.vol.p, W.rotAtn.p, c.fsin, c.fcos, c.N);
+        W.rotate_any(c.fast, 0, W.vol.p, W.rotAtn.p, c.fsin, c.fcos, c.N);
     }
     const long cols = (long)c.Nz * c.NRot;
     if (c.model & 1)
@@ -1525,9 +1767,10 @@ void prjws_backproject(const PrjCall &c, const float *atn, const float *ratio, f
     PrjWs &W = ws(c.Nz, c.N, c.NRot);
     const long nin = (long)c.N * c.Nz * c.N;
     const bool useAtn = (c.model & 1) || (c.model & 4);
+    W.set_shears(c);
     if (useAtn && !c.atnRotated) {
         W.vol.upload(atn, nin, W.stream);
-        W.rotate(W.vol.p, W.rotAtn.p, c.fsin, c.fcos, c.N);
+        W.rotate_any(c.fast, 0, W.vol.p, W.rotAtn.p, c.fsin, c.fcos, c.N);
     }
     const long cols = (long)c.Nz * c.NRot;
     if (c.model & 1)
@@ -1545,7 +1788,7 @@ void prjws_backproject(const PrjCall &c, const float *atn, const float *ratio, f
     da.scale = c.primary;
     launch_drf(W.stream, true, da, 1, W.drfKYb);
     if (c.model & 4) W.scatter_source(c.esse, W.rotRecon.p, c.primaryFac, W.rotRecon.p);
-    W.rotate(W.rotRecon.p, W.unrot.p, c.bsin, c.bcos, c.NRot);
+    W.rotate_any(c.fast, 1, W.rotRecon.p, W.unrot.p, c.bsin, c.bcos, c.NRot);
     W.img.upload(image, nin, W.stream);
     hipLaunchKernelGGL(k_acc_clamp, dim3(div_up(nin, 256)), dim3(256), 0, W.stream, W.img.p,
                        W.unrot.p, c.N, c.Nz, c.NRot);
@@ -1680,6 +1923,32 @@ void dev_rot3dpar(float angle, const float *in_host, float *out_host, int pbFlag
     HIP_CHECK(hipGetLastError());
     dout.download(out_host, nout, st);
     HIP_CHECK(hipStreamSynchronize(st));
+    HIP_CHECK(hipStreamDestroy(st));
+}
+
+// vShearsRotate on host arrays (fastrotate.c:330-563), [y][z][x] volumes of
+// N x Nz x N.
+void dev_shears_rotate(float angle, const float *in_host, float *out_host, int Nz, int N)
+{
+    // odd N gives the centre row N+1 pixels (fastrotate.c:584-586), past the row end
+    if (N % 2)
+        fail(ETYPE_ILLEGAL_VALUE, "odd NumPixels (%d) is not supported by the shear rotation", N);
+    hipStream_t st;
+    HIP_CHECK(hipStreamCreate(&st));
+    {
+        ShearSet ss;
+        ss.set({shear_geometry(angle, N)}, N, st);
+        DBuf<float> din, dout;
+        const long n = (long)N * Nz * N;
+        din.alloc(n);
+        dout.alloc(n);
+        din.upload(in_host, n, st);
+        hipLaunchKernelGGL(k_shear_rot, dim3(div_up(n, 256)), dim3(256), 0, st, din.p, dout.p, Nz,
+                           N, ss.dev(), 0);
+        HIP_CHECK(hipGetLastError());
+        dout.download(out_host, n, st);
+        HIP_CHECK(hipStreamSynchronize(st));
+    }
     HIP_CHECK(hipStreamDestroy(st));
 }
 

@@ -107,6 +107,83 @@ void view_trig(float angle, ViewTab &vt)
     vt.fcos = (float)cos(fa);
     vt.bsin = (float)sin(ba);
     vt.bcos = (float)cos(ba);
+    vt.fang = fa;
+    vt.bang = ba;
+}
+
+void shear_extents(int N, std::vector<int> &start, std::vector<int> &end)
+{
+    start.resize(N);
+    end.resize(N);
+    const float c = (N - 1) * 0.5;
+    for (int j = 0; j < N; ++j) {
+        const float d = (float)j - c;
+        const int len = 2 * floor(sqrt(c * c - d * d)) + 2;
+        start[j] = (N - len) / 2;
+        end[j] = start[j] + len;
+    }
+}
+
+// fastrotate.c:358-388 (folding, grid size) and the per-row / per-column
+// shift decisions of its three passes (:396-404, :489-500, :520-530).
+ShearGeom shear_geometry(float fAngle, int N)
+{
+    ShearGeom g;
+    while (fAngle > 0.75 * M_PI) {
+        fAngle -= M_PI;
+        g.b180 ^= 1;
+    }
+    while (fAngle <= -M_PI / 4.0) {
+        fAngle += M_PI;
+        g.b180 ^= 1;
+    }
+    if (fAngle > M_PI / 4.0) {
+        g.b90 = 1;
+        fAngle -= M_PI / 2.0;
+    }
+    const float fCntr = ((float)N - 1.0) * 0.5;
+    int iAdd = 0;
+    for (int j = N / 4; j < N / 2; ++j) {
+        const float yDist = (float)j - fCntr;
+        float fDelta = fabs(yDist * tan((double)(fAngle * 0.5)));
+        fDelta -= fCntr - sqrt(fCntr * fCntr - yDist * yDist);
+        const int t = ceil(fDelta);
+        if (t > iAdd) iAdd = t;
+    }
+    g.iAdd = iAdd;
+    g.mDim = N + 2 * iAdd;
+    const float tCntr = ((float)g.mDim - 1.0) * 0.5;
+    g.fl.resize(N);
+    g.f1.resize(N);
+    g.f2.resize(N);
+    for (int j = 0; j < N; ++j) {
+        const float yDist = (float)j - fCntr;
+        const float fDelta = -1.0 * yDist * tan((double)(fAngle * 0.5));
+        const int fl = floorf(fDelta);
+        const float f2 = fDelta - (float)fl;
+        const float f1 = 1.0 - f2;
+        g.fl[j] = fl;
+        g.f1[j] = f1;
+        g.f2[j] = f2;
+    }
+    g.jT.resize(g.mDim);
+    g.jS.resize(g.mDim);
+    g.jE.resize(g.mDim);
+    g.g1.resize(g.mDim);
+    g.g2.resize(g.mDim);
+    for (int i = 0; i < g.mDim; ++i) {
+        const float xDist = (float)i - tCntr;
+        const float fDelta = xDist * sin((double)fAngle);
+        const int jT = floorf(fDelta);
+        const float f2 = fDelta - (float)jT;
+        const float f1 = 1.0 - f2;
+        g.jT[i] = jT;
+        g.g1[i] = f1;
+        g.g2[i] = f2;
+        g.jS[i] = std::max(0, -jT);
+        g.jE[i] = std::min(g.mDim, g.mDim - jT - 1);
+    }
+    return g;
 }
 
 // ---------------------------------------------------------------------------

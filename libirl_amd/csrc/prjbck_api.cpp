@@ -58,6 +58,8 @@ PrjCall make_call(int Nz, int N, int NRot, int model, float pw, float fAngle, fl
     c.fcos = vt.fcos;
     c.bsin = vt.bsin;
     c.bcos = vt.bcos;
+    c.fang = vt.fang;
+    c.bang = vt.bang;
     // prjbck.c:127, 317: with scatter modelling the primary factor is applied
     // by the scatter source instead.
     c.primary = (model & MODEL_SRF) ? 1.0f : primary;
@@ -127,12 +129,12 @@ void Project(int iNumSlices, int iNumPixels, int iNumRotPixs, int iPrjModel, flo
              int bActImgRotated, int bFastRotate)
 {
     (void)iAngle;
-    (void)bFastRotate;
     try {
         std::vector<int> rk, rp;
         PrjCall c = make_call(iNumSlices, iNumPixels, iNumRotPixs, iPrjModel, fPixelWidth,
                               fAngle, fCFCR, psDrfTab, fPrimaryFac, fAtnFac, iAxialPadLength,
                               iAxialAvgLength, bAtnMapRotated, bActImgRotated, rk, rp);
+        c.fast = bFastRotate != 0;   // prjbck.c:130-138, 154-161
         if (iPrjModel & MODEL_SRF) c.esse = psSrf ? psSrf->pvScatterData : nullptr;
         prjws_project(c, atnmatrix, reconmatrix, calcprj);
     } catch (const IrlError &e) {
@@ -148,7 +150,6 @@ void BackProject(int iNumSlices, int iNumPixels, int iNumRotPixs, int iBckModel,
                  int bAtnMapRotated, int bFastRotate)
 {
     (void)iAngle;
-    (void)bFastRotate;
     try {
         // prjbck.c:308-310: a separate backprojection table wins when present
         if (psDrfTab && psDrfTab->psBckDrfTab) psDrfTab = psDrfTab->psBckDrfTab;
@@ -156,6 +157,7 @@ void BackProject(int iNumSlices, int iNumPixels, int iNumRotPixs, int iBckModel,
         PrjCall c = make_call(iNumSlices, iNumPixels, iNumRotPixs, iBckModel, fPixelWidth,
                               fAngle, fCFCR, psDrfTab, fPrimaryFac, fAtnFac, iAxialPadLength,
                               iAxialAvgLength, bAtnMapRotated, 0, rk, rp);
+        c.fast = bFastRotate != 0;   // prjbck.c:327-334, 408-416
         if (iBckModel & MODEL_SRF) c.esse = psSrf ? psSrf->pvScatterData : nullptr;
         prjws_backproject(c, atnmatrix, ratioprj, imagematrix);
     } catch (const IrlError &e) {
@@ -171,6 +173,21 @@ void rot3dpar(float fAngle, float *OldMatrix, float *NewMatrix, int PB_FLAG, int
         dev_rot3dpar(fAngle, OldMatrix, NewMatrix, PB_FLAG, NumSlices, NumPixels, NumRotPixs);
     } catch (const IrlError &e) {
         report(e, "rot3dpar");
+    }
+}
+
+// fastrotate.c:330-563 (the circle extents of vInitFastRotate,
+// fastrotate.c:573-589, are computed per call)
+void vShearsRotate(float fAngle, float *pfOldMatrix, float *pfRotMatrix, int PB_FLAG,
+                   int NumSlices, int NumPixels)
+{
+    (void)PB_FLAG;
+    try {
+        if (NumSlices <= 0 || NumPixels <= 0)
+            fail(ETYPE_ILLEGAL_VALUE, "vShearsRotate: bad sizes %d %d", NumSlices, NumPixels);
+        dev_shears_rotate(fAngle, pfOldMatrix, pfRotMatrix, NumSlices, NumPixels);
+    } catch (const IrlError &e) {
+        report(e, "vShearsRotate");
     }
 }
 

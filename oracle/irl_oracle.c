@@ -21,6 +21,8 @@
  *
  * Reference files restated (paths relative to the reference tree):
  *   rot3dpar.c:19-198            orc_rot3dpar
+ *   fastrotate.c:330-563, 573-589
+ *                                orc_shears_rotate (3-pass shear rotation)
  *   conv2d_ypad.c:39-108,
  *   pad2d.c:21-141               orc_conv2d_ypad
  *   convolve_ypad.c:22-96        orc_convolve_drf (spatial path; the FFT path
@@ -163,6 +165,130 @@ void orc_rot3dpar(float fAngle, const REAL *OldMatrix, REAL *NewMatrix,
     }
 #undef O_PTR
 #undef N_PTR
+}
+
+/* ------------------------------------------------------------------ */
+/* fastrotate.c:573-589 -- per-row extents of the in-plane circle used */
+/* by the shear rotation: row j keeps columns [start, end).            */
+/* ------------------------------------------------------------------ */
+void orc_shears_extents(int NumPixels, int *start, int *end)
+{
+    int j, len;
+    float d, c = (NumPixels - 1) * 0.5;
+    for (j = 0; j < NumPixels; j++) {
+        d = (float)j - c;
+        len = 2 * floor(sqrt(c * c - d * d)) + 2;
+        start[j] = (NumPixels - len) / 2;
+        end[j] = start[j] + len;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* fastrotate.c:330-563 -- rotation by three shears of a [y][z][x]     */
+/* volume (N x N per slice): the angle is folded into (-pi/4, pi/4]    */
+/* with 90/180 degree index flips, then x-shift by -r tan(a/2), y-shift */
+/* by r sin(a) on a grid grown by iAdd pixels per side, and x-shift by  */
+/* -r tan(a/2) again, each a two-tap linear interpolation.  The final  */
+/* rows are trimmed by one pixel at each end of the circle extent.     */
+/* ------------------------------------------------------------------ */
+void orc_shears_rotate(float fAngle, const REAL *Old, REAL *Rot, int NumSlices, int N)
+{
+    int i, j, k, jj, jk, fl, iAdd, mDim, b90 = 0, b180 = 0, iS, iE, jT, jS, jE, t;
+    float fCntr, tCntr, fDelta, yDist, xDist, f1, f2;
+    int *pS = (int *)malloc(sizeof(int) * N), *pE = (int *)malloc(sizeof(int) * N);
+    REAL *X, *Y;
+    long vol = (long)N * NumSlices * N;
+#define S_PTR(x, y, s) ((long)(y) * NumSlices * N + (long)(s) * N + (x))
+    orc_shears_extents(N, pS, pE);
+    for (i = 0; i < vol; ++i) Rot[i] = 0;
+    while (fAngle > 0.75 * M_PI) {
+        fAngle -= M_PI;
+        b180 ^= 1;
+    }
+    while (fAngle <= -M_PI / 4.0) {
+        fAngle += M_PI;
+        b180 ^= 1;
+    }
+    if (fAngle > M_PI / 4.0) {
+        b90 = 1;
+        fAngle -= M_PI / 2.0;
+    }
+    fCntr = ((float)N - 1.0) * 0.5;
+    iAdd = 0;
+    for (j = N / 4; j < N / 2; j++) {
+        yDist = (float)j - fCntr;
+        fDelta = fabs(yDist * tan((double)(fAngle * 0.5)));
+        fDelta -= fCntr - sqrt(fCntr * fCntr - yDist * yDist);
+        t = ceil(fDelta);
+        if (t > iAdd) iAdd = t;
+    }
+    mDim = N + 2 * iAdd;
+    tCntr = ((float)mDim - 1.0) * 0.5;
+    X = (REAL *)calloc((long)mDim * mDim, sizeof(REAL));
+    Y = (REAL *)calloc((long)mDim * mDim, sizeof(REAL));
+    for (k = 0; k < NumSlices; k++) {
+        /* first x shift, reading the source through the 90/180 flips */
+        for (j = 0; j < N; j++) {
+            yDist = (float)j - fCntr;
+            fDelta = -1.0 * yDist * tan((double)(fAngle * 0.5));
+            fl = floorf(fDelta);
+            f2 = fDelta - (float)fl;
+            f1 = 1.0 - f2;
+            iS = pS[j] - fl + iAdd;
+            iE = pE[j] - fl - 1 + iAdd;
+            jk = j + iAdd;
+            jj = N - j - 1;
+            for (i = iS; i < iE; i++) {
+                int a = i + fl - iAdd;          /* source index along the sheared axis */
+                int b = N - i - (fl - iAdd + 1); /* mirrored index for the 180 flip */
+                REAL v0, v1;
+                if (!b180 && !b90) {
+                    v0 = Old[S_PTR(a, j, k)];
+                    v1 = Old[S_PTR(a + 1, j, k)];
+                } else if (!b180) {
+                    v0 = Old[S_PTR(jj, a, k)];
+                    v1 = Old[S_PTR(jj, a + 1, k)];
+                } else if (!b90) {
+                    v0 = Old[S_PTR(b, jj, k)];
+                    v1 = Old[S_PTR(b - 1, jj, k)];
+                } else {
+                    v0 = Old[S_PTR(j, b, k)];
+                    v1 = Old[S_PTR(j, b - 1, k)];
+                }
+                X[i + (long)mDim * jk] = f1 * v0 + f2 * v1;
+            }
+        }
+        /* y shift of every column of the grown grid */
+        for (i = 0; i < mDim; i++) {
+            xDist = (float)i - tCntr;
+            fDelta = xDist * sin((double)fAngle);
+            jT = floorf(fDelta);
+            f2 = fDelta - (float)jT;
+            f1 = 1.0 - f2;
+            jS = ORC_MAX(0, -jT);
+            jE = ORC_MIN(mDim, mDim - jT - 1);
+            for (j = jS; j < jE; j++)
+                Y[i + (long)mDim * j] = f1 * X[i + (long)mDim * (j + jT)] +
+                                        f2 * X[i + (long)mDim * (j + jT + 1)];
+        }
+        /* second x shift into the trimmed circle */
+        for (j = 0; j < N; j++) {
+            yDist = (float)j - fCntr;
+            fDelta = -1.0 * yDist * tan((double)(fAngle * 0.5));
+            fl = floorf(fDelta);
+            f2 = fDelta - (float)fl;
+            f1 = 1.0 - f2;
+            jj = j + iAdd;
+            for (i = pS[j] + 1; i < pE[j] - 1; i++)
+                Rot[S_PTR(i, j, k)] = f1 * Y[i + fl + iAdd + (long)mDim * jj] +
+                                      f2 * Y[i + fl + iAdd + 1 + (long)mDim * jj];
+        }
+    }
+#undef S_PTR
+    free(X);
+    free(Y);
+    free(pS);
+    free(pE);
 }
 
 /* ------------------------------------------------------------------ */
@@ -615,6 +741,19 @@ void orc_init_prj(int Nz, int N, int NRot, int iModel)
     W.NRot = NRot;
 }
 
+/* prjbck.c:129-138, 153-161, 408-416: shear rotation when bFastRotate
+ * (volume size NumPixels, which the shear path assumes equals
+ * NumRotPixs), else rot3dpar.  Note the reference passes -0.5*pi-fAngle to
+ * the shear rotation and -fAngle-0.5*pi to rot3dpar. */
+static void orc_rotate(int bFast, float fAngle, const REAL *in, REAL *out, int PB_FLAG,
+                       int Nz, int N, int NRot)
+{
+    if (bFast)
+        orc_shears_rotate(fAngle, in, out, Nz, N);
+    else
+        orc_rot3dpar(fAngle, in, out, PB_FLAG, Nz, N, NRot);
+}
+
 /* prjbck.c:186-204 -- DRF plane index in the reference's float
  * arithmetic; returns -1 when fDist <= 0 (no convolution). */
 int orc_drf_index(const orc_drftab *t, float fCFCR, float fPixelWidth,
@@ -636,7 +775,8 @@ void orc_project(int iNumSlices, int iNumPixels, int iNumRotPixs, int iPrjModel,
                  const orc_drftab *drf, int bFFT, const void *srf,
                  const REAL *atnmatrix, const REAL *reconmatrix, REAL *calcprj,
                  float fPrimaryFac, float fAtnFac, int iAxialPadLength,
-                 int iAxialAvgLength, int bAtnMapRotated, int bActImgRotated)
+                 int iAxialAvgLength, int bAtnMapRotated, int bActImgRotated,
+                 int bFastRotate)
 {
     int iRow, iDist = 0;
     long i, pl = (long)iNumSlices * iNumRotPixs, vol = pl * iNumRotPixs;
@@ -647,13 +787,13 @@ void orc_project(int iNumSlices, int iNumPixels, int iNumRotPixs, int iPrjModel,
     float fScaleFac = bModelSrf ? 1.0 : fPrimaryFac;
 
     if (!bActImgRotated)
-        orc_rot3dpar(0.5 * M_PI + fAngle, reconmatrix, W.RotRecon, 0, iNumSlices,
-                     iNumPixels, iNumRotPixs);
+        orc_rotate(bFastRotate, 0.5 * M_PI + fAngle, reconmatrix, W.RotRecon, 0, iNumSlices,
+                   iNumPixels, iNumRotPixs);
     if (bModelAtn || bModelSrf) {
         for (i = 0; i < pl; ++i) W.SumAtn[i] = 0;
         if (!bAtnMapRotated)
-            orc_rot3dpar(0.5 * M_PI + fAngle, atnmatrix, W.RotAtn, 0, iNumSlices,
-                         iNumPixels, iNumRotPixs);
+            orc_rotate(bFastRotate, 0.5 * M_PI + fAngle, atnmatrix, W.RotAtn, 0, iNumSlices,
+                       iNumPixels, iNumRotPixs);
     }
     if (bModelSrf) {
         orc_esse_effsrc(srf, W.RotRecon, W.RotAtn, iNumRotPixs, iNumRotPixs,
@@ -694,7 +834,8 @@ void orc_backproject(int iNumSlices, int iNumPixels, int iNumRotPixs,
                      const orc_drftab *drf, int bFFT, const void *srf,
                      const REAL *atnmatrix, const REAL *ratioprj,
                      REAL *imagematrix, float fPrimaryFac, float fAtnFac,
-                     int iAxialPadLength, int iAxialAvgLength, int bAtnMapRotated)
+                     int iAxialPadLength, int iAxialAvgLength, int bAtnMapRotated,
+                     int bFastRotate)
 {
     int iRow, iCol, iSlice, nRowSkip, iDist = 0;
     long i, pl = (long)iNumSlices * iNumRotPixs, vol = pl * iNumRotPixs;
@@ -706,8 +847,8 @@ void orc_backproject(int iNumSlices, int iNumPixels, int iNumRotPixs,
 
     if (bModelAtn || bModelSrf) {
         if (!bAtnMapRotated)
-            orc_rot3dpar(0.5 * M_PI + fAngle, atnmatrix, W.RotAtn, 0, iNumSlices,
-                         iNumPixels, iNumRotPixs);
+            orc_rotate(bFastRotate, 0.5 * M_PI + fAngle, atnmatrix, W.RotAtn, 0, iNumSlices,
+                       iNumPixels, iNumRotPixs);
         for (i = 0; i < pl; ++i) W.SumAtn[i] = 0;
     }
     for (iRow = 0; iRow < iNumRotPixs; iRow++) {
@@ -737,8 +878,8 @@ void orc_backproject(int iNumSlices, int iNumPixels, int iNumRotPixs,
         for (i = 0; i < vol; ++i)
             W.RotRecon[i] = fPrimaryFac * W.RotRecon[i] + W.Scatter[i];
     }
-    orc_rot3dpar(-fAngle - 0.5 * M_PI, W.RotRecon, W.UnRotRecon, 1, iNumSlices,
-                 iNumPixels, iNumRotPixs);
+    orc_rotate(bFastRotate, -0.5 * M_PI - fAngle, W.RotRecon, W.UnRotRecon, 1, iNumSlices,
+               iNumPixels, iNumRotPixs);
     nRowSkip = (iNumRotPixs - iNumPixels) / 2;
     for (iRow = 0; iRow < iNumPixels; iRow++)
         for (iSlice = 0; iSlice < iNumSlices; iSlice++)
@@ -829,7 +970,7 @@ int orc_osem(int Nz, int N, int V, int NumAngPerSubset, int NumIterations,
              const void *srf, int iNumSrfIterations, int iAxialPadLength,
              int iAxialAvgLength, const REAL *pfPrjImage,
              const REAL *pfScatterEstimate, const REAL *pfAtnMap,
-             REAL *pfReconImage, orc_iter_cb cb, void *ctx)
+             REAL *pfReconImage, orc_iter_cb cb, void *ctx, int bFastRotate)
 {
     int NRot = N, S, iSubset, iMember, iAngle, iIter;
     long M = (long)N * N * Nz, P = (long)NRot * Nz, i;
@@ -863,7 +1004,7 @@ int orc_osem(int Nz, int N, int V, int NumAngPerSubset, int NumIterations,
             for (i = 0; i < P; ++i) calc[i] *= views[iAngle].viewtime;
             orc_backproject(Nz, N, NRot, iBckModel, fPixelWidth, views[iAngle].angle,
                             views[iAngle].cfcr, drf, bFFT, srf, pfAtnMap, calc, tmp,
-                            1.0f, 1.0f, iAxialPadLength, iAxialAvgLength, 0);
+                            1.0f, 1.0f, iAxialPadLength, iAxialAvgLength, 0, bFastRotate);
         }
         for (i = 0; i < M; ++i) tmp[i] = tmp[i] < 1e-6 ? 0.0 : 1.0 / tmp[i];
         memcpy(norm + M * iSubset, tmp, sizeof(REAL) * M);
@@ -888,7 +1029,7 @@ int orc_osem(int Nz, int N, int V, int NumAngPerSubset, int NumIterations,
                 vw = views + iAngle;
                 orc_project(Nz, N, NRot, iModel, fPixelWidth, vw->angle, vw->cfcr, drf,
                             bFFT, srf, pfAtnMap, pfReconImage, calc, 1.0f, 1.0f,
-                            iAxialPadLength, iAxialAvgLength, 0, 0);
+                            iAxialPadLength, iAxialAvgLength, 0, 0, bFastRotate);
                 for (i = 0; i < P; ++i) calc[i] *= (float)(vw->viewtime * vw->sensitivity);
                 if (bPrjModelSrf) {
                     if (!bScatThis) {
@@ -898,7 +1039,7 @@ int orc_osem(int Nz, int N, int V, int NumAngPerSubset, int NumIterations,
                         orc_project(Nz, N, NRot, iPrjModel & ~ORC_MODEL_SRF, fPixelWidth,
                                     vw->angle, vw->cfcr, drf, bFFT, srf, pfAtnMap,
                                     pfReconImage, ps, 1.0f, 1.0f, iAxialPadLength,
-                                    iAxialAvgLength, 1, 1);
+                                    iAxialAvgLength, 1, 1, bFastRotate);
                         for (i = 0; i < P; ++i) ps[i] *= vw->viewtime;
                         for (i = 0; i < P; ++i) ps[i] = calc[i] - ps[i];
                     }
@@ -912,7 +1053,7 @@ int orc_osem(int Nz, int N, int V, int NumAngPerSubset, int NumIterations,
                 for (i = 0; i < P; ++i) calc[i] *= vw->viewtime;
                 orc_backproject(Nz, N, NRot, iBckModel, fPixelWidth, vw->angle, vw->cfcr,
                                 drf, bFFT, srf, pfAtnMap, calc, tmp, 1.0f, 1.0f,
-                                iAxialPadLength, iAxialAvgLength, 1);
+                                iAxialPadLength, iAxialAvgLength, 1, bFastRotate);
             }
             memcpy(normImg, norm + M * iSubset, sizeof(REAL) * M);
             for (i = 0; i < M; ++i) pfReconImage[i] *= tmp[i] * normImg[i];

@@ -42,13 +42,15 @@ def lib(real="f64"):
     L.orc_drf_index.argtypes = [vp, ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_int]
     L.orc_project.argtypes = ([ctypes.c_int] * 4 + [ctypes.c_float] * 3 +
                               [vp, ctypes.c_int, vp, vp, vp, vp] + [ctypes.c_float] * 2 +
-                              [ctypes.c_int] * 4)
+                              [ctypes.c_int] * 5)
     L.orc_backproject.argtypes = ([ctypes.c_int] * 4 + [ctypes.c_float] * 3 +
                                   [vp, ctypes.c_int, vp, vp, vp, vp] + [ctypes.c_float] * 2 +
-                                  [ctypes.c_int] * 3)
+                                  [ctypes.c_int] * 4)
+    L.orc_shears_rotate.argtypes = [ctypes.c_float, vp, vp, ctypes.c_int, ctypes.c_int]
+    L.orc_shears_extents.argtypes = [ctypes.c_int, vp, vp]
     L.orc_subsets.argtypes = [ctypes.c_int, ctypes.c_int, vp]
     L.orc_osem.argtypes = ([ctypes.c_int] * 8 + [ctypes.c_float] + [vp, vp, ctypes.c_int, vp] +
-                           [ctypes.c_int] * 3 + [vp] * 4 + [vp, vp])
+                           [ctypes.c_int] * 3 + [vp] * 4 + [vp, vp, ctypes.c_int])
     L.orc_esse_init.restype = vp
     L.orc_esse_init.argtypes = ([vp, vp, ctypes.c_int, ctypes.c_int] + [ctypes.c_float] * 5 +
                                 [ctypes.c_int] * 4 + [ctypes.c_float] * 2)
@@ -105,6 +107,21 @@ def rot3dpar(angle, vol, pb_flag, nz, n, nrot, real="f64"):
     return out
 
 
+def shears_rotate(angle, vol, nz, n, real="f64"):
+    """vShearsRotate of a [y][z][x] volume (n x nz x n)."""
+    vol = _arr(vol, real)
+    out = np.zeros((n, nz, n), _dt(real))
+    lib(real).orc_shears_rotate(angle, vol.ctypes.data, out.ctypes.data, nz, n)
+    return out
+
+
+def shears_extents(n):
+    s = np.zeros(n, np.int32)
+    e = np.zeros(n, np.int32)
+    lib("f64").orc_shears_extents(n, s.ctypes.data, e.ctypes.data)
+    return s, e
+
+
 def conv2d_ypad(plane, psf, pad_y=0, avg_y=0, real="f64"):
     plane = _arr(plane, real)
     psf = np.ascontiguousarray(psf, np.float32)
@@ -121,19 +138,19 @@ def init_prj(nz, n, nrot, model=7, real="f64"):
 
 
 def project(nz, n, nrot, model, pw, angle, cfcr, drf, atn, recon, primary=1.0, atnfac=1.0,
-            pad=0, avg=0, atn_rotated=0, act_rotated=0, real="f64", esse=None):
+            pad=0, avg=0, atn_rotated=0, act_rotated=0, real="f64", esse=None, fast=0):
     L = lib(real)
     atn = None if atn is None else _arr(atn, real)
     recon = _arr(recon, real)
     out = np.zeros((nz, nrot), _dt(real))
     L.orc_project(nz, n, nrot, model, pw, angle, cfcr, drf.handle(real) if drf else None,
                   int(drf.fft) if drf else 0, esse, _p(atn), recon.ctypes.data, out.ctypes.data,
-                  primary, atnfac, pad, avg, atn_rotated, act_rotated)
+                  primary, atnfac, pad, avg, atn_rotated, act_rotated, int(fast))
     return out
 
 
 def backproject(nz, n, nrot, model, pw, angle, cfcr, drf, atn, ratio, image, primary=1.0,
-                atnfac=1.0, pad=0, avg=0, atn_rotated=0, real="f64", esse=None):
+                atnfac=1.0, pad=0, avg=0, atn_rotated=0, real="f64", esse=None, fast=0):
     """Adds the backprojection of 'ratio' into 'image' (in place, with the EPS
     clamp); 'image' must be a contiguous array of the oracle's dtype."""
     L = lib(real)
@@ -142,7 +159,7 @@ def backproject(nz, n, nrot, model, pw, angle, cfcr, drf, atn, ratio, image, pri
     assert image.dtype == _dt(real) and image.flags.c_contiguous
     L.orc_backproject(nz, n, nrot, model, pw, angle, cfcr, drf.handle(real) if drf else None,
                       int(drf.fft) if drf else 0, esse, _p(atn), ratio.ctypes.data,
-                      image.ctypes.data, primary, atnfac, pad, avg, atn_rotated)
+                      image.ctypes.data, primary, atnfac, pad, avg, atn_rotated, int(fast))
     return image
 
 
@@ -197,7 +214,7 @@ ITER_CB = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p)
 
 def osem(nz, n, views, per_subset, iterations, prj_model, bck_model, pw, drf, prj, atn, x0,
          scat=None, start_iteration=0, pad=0, avg=0, real="f64", iter_cb=None, esse=None,
-         srf_iterations=0):
+         srf_iterations=0, fast=0):
     """vOsemRecon on internal-order inputs (atn already scaled to 1/pixel and
     clamped, prj clamped, x0 internal order).  views: list of dicts with
     angle, left, right, cfcr, viewtime, sensitivity.  Returns the estimate."""
@@ -221,7 +238,7 @@ def osem(nz, n, views, per_subset, iterations, prj_model, bck_model, pw, drf, pr
                     pw, ctypes.addressof(vv), drf.handle(real) if drf else None,
                     int(drf.fft) if drf else 0, esse, srf_iterations, pad, avg,
                     prj.ctypes.data, _p(scat), _p(atn), x.ctypes.data,
-                    ctypes.cast(cb, ctypes.c_void_p), None)
+                    ctypes.cast(cb, ctypes.c_void_p), None, int(fast))
     if rc != 0:
         raise ValueError("orc_osem failed with %d" % rc)
     return x

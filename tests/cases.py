@@ -71,13 +71,13 @@ class Problem:
     def oracle_views(self):
         return [dict(angle=v.Angle, left=v.Left, right=v.Right, cfcr=v.CFCR) for v in self.views]
 
-    def oracle_osem(self, real="f64", iters=None, iter_cb=None):
+    def oracle_osem(self, real="f64", iters=None, iter_cb=None, fast=0):
         x0 = O.reorder(self.nz, self.n, self.n, self.x0)
         prj = np.clip(self.prj, 0, None)
         x = O.osem(self.nz, self.n, self.oracle_views(), self.per,
                    self.iters if iters is None else iters, self.prj_model, self.prj_model,
                    self.pix, self.drf, prj, self.atn_int, x0, pad=self.pad, avg=self.avg,
-                   real=real, iter_cb=iter_cb)
+                   real=real, iter_cb=iter_cb, fast=fast)
         return O.reorder(self.n, self.nz, self.n, x.reshape(self.n, self.nz, self.n))
 
 
