@@ -57,6 +57,14 @@ void irlamd_recon_destroy(irlamd_recon *r);
 /* Nonzero when the last call into the library failed. */
 int irlamd_last_error(void);
 
+/* Arithmetic of the DRF convolution for reconstructions created afterwards
+ * (irlamd_recon_create, IrlOsem, IrlGenprj): 0 = FP32 (the default, the
+ * reference's arithmetic), 1 = BF16 MFMA with FP32 accumulation, using hi+lo
+ * split BF16 operands when the estimated per-plane relative error exceeds
+ * 1e-3, 2 = BF16 hi+lo always, 3 = plain BF16 always.  Modes 1-3 need
+ * NumSlices % 4 == 0.  Returns the previous mode, or -1 for an invalid one. */
+int irlamd_set_precision(int iMode);
+
 #ifdef __cplusplus
 }
 #endif

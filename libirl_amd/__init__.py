@@ -193,6 +193,8 @@ def load():
     lib.ConvolveDrfYpad.argtypes = [_FP, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Drf)] + \
         [ctypes.c_int] * 3 + [_FP]
     lib.irlamd_last_error.restype = ctypes.c_int
+    lib.irlamd_set_precision.argtypes = [ctypes.c_int]
+    lib.irlamd_set_precision.restype = ctypes.c_int
     _lib = lib
     return lib
 
@@ -312,6 +314,20 @@ def convolve_drf_ypad(plane, psf, fft=0, pad=0, avg=0):
 
 def last_error():
     return load().irlamd_last_error()
+
+
+PRECISION = {"fp32": 0, "bf16": 1, "bf16x3": 2, "bf16-plain": 3}
+
+
+def set_precision(mode):
+    """DRF convolution arithmetic of reconstructions created afterwards
+    (irlamd_set_precision): 'fp32', 'bf16' (hi+lo split where needed),
+    'bf16x3' (split always) or 'bf16-plain'.  Returns the previous mode."""
+    m = PRECISION[mode] if isinstance(mode, str) else int(mode)
+    old = load().irlamd_set_precision(m)
+    if old < 0:
+        raise ValueError("invalid precision mode %r" % (mode,))
+    return old
 
 
 def set_adv_options(fast_rotate=1, start_iteration=0):

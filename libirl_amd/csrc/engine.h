@@ -101,6 +101,9 @@ public:
         bool hasScatterEstimate = false;
         bool hasAtn = false;
         bool shears = false;               // three-shear rotation (bFastRotate) instead of rot3dpar
+        int precision = 0;                 // DRF convolution: 0 FP32, 1 BF16 (hi+lo split where
+                                           // the estimated per-plane error exceeds 1e-3),
+                                           // 2 BF16 hi+lo always, 3 BF16 plain always
         std::shared_ptr<const EsseModel> esse;  // scatter model, or null
     };
 

@@ -584,6 +584,156 @@ __global__ __launch_bounds__(64 * ZG) void k_drf(DrfArgs a)
 }
 
 // ---------------------------------------------------------------------------
+// k_drf_bf16: the DRF convolution of k_drf as BF16 MFMA contractions with
+// FP32 accumulation (the precision variant).  For PSF column i the
+// convolution along the slices is a banded Toeplitz product:
+//   out[z0+m][x] += sum_k A_i[m][k] * tile[16t + k][x + CX - i],
+//   A_i[m][k] = psf[m + KY - 1 - k][i]  (zero outside the band),
+// one v_mfma_f32_16x16x32_bf16 per 32 rows of the window (M = 16 output
+// slices, N = 16 bins, K = window slices).  The A fragments of every table
+// entry are built on the host; B fragments are 128-bit reads of the bf16
+// tile (slices contiguous per bin).  SPLIT keeps hi and lo bf16 parts of
+// both operands and adds the two cross products (3 MFMAs per step).
+// Block = 4 waves, wave w owns bins [16w, 16w+16) of the 64-bin tile and
+// NZT slice tiles of 16.  Engine layout only (slices fastest).
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float floatx4_t __attribute__((ext_vector_type(4)));
+
+__host__ __device__ constexpr int drf16_nks(int KY) { return (16 + KY - 1 + 31) / 32; }
+__host__ __device__ constexpr int drf16_rows(int KY, int NZT) { return 16 * (NZT - 1) + 32 * drf16_nks(KY); }
+__host__ __device__ constexpr int drf16_cs(int rows) { return ((rows + 55) / 64) * 64 + 8; }
+
+__device__ __forceinline__ uint32_t pack_bf16(__bf16 lo, __bf16 hi)
+{
+    return (uint32_t)__builtin_bit_cast(uint16_t, lo) |
+           ((uint32_t)__builtin_bit_cast(uint16_t, hi) << 16);
+}
+
+template <int KY, int NZT, bool BCK, bool SPLIT>
+__global__ __launch_bounds__(256) void k_drf_bf16(DrfArgs a, const uint4 *__restrict__ afr,
+                                                  const uint4 *__restrict__ afrLo)
+{
+    extern __shared__ uint32_t ldsw[];
+    constexpr int CY = (KY - 1) / 2;
+    constexpr int NKS = drf16_nks(KY);
+    constexpr int TZ = 16 * NZT;
+    constexpr int THP = drf16_rows(KY, NZT);   // staged rows (window of the last tile)
+    constexpr int CSB = drf16_cs(THP);          // bf16 column stride: 4 mod 32 dwords
+    const int KX = a.KX;
+    const int CX = (KX - 1) / 2;
+    const int TW = DRF_TX + KX - 1;
+    uint32_t *tileH = ldsw;                     // [TW][CSB] bf16, as dword pairs
+    uint32_t *tileL = ldsw + TW * CSB / 2;
+
+    int bid = blockIdx.x;
+    const int xt = bid % a.xtiles;
+    bid /= a.xtiles;
+    const int zt = bid % a.ztiles;
+    bid /= a.ztiles;
+    int b, r0, r1;
+    if (BCK) {
+        b = bid / a.nrows;
+        r0 = bid % a.nrows;
+        r1 = r0 + 1;
+    } else {
+        b = bid / a.nchunk;
+        const int ch = bid % a.nchunk;
+        r0 = ch * a.rchunk;
+        r1 = min(r0 + a.rchunk, a.nrows);
+    }
+    const int v = a.bview[b];
+    const int x0 = xt * DRF_TX, z0 = zt * TZ;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int n = lane & 15, g = lane >> 4;
+    const int x = x0 + 16 * w + n;
+
+    floatx4_t acc[NZT];
+#pragma unroll
+    for (int t = 0; t < NZT; ++t) acc[t] = floatx4_t{0.0f, 0.0f, 0.0f, 0.0f};
+
+    for (int row = r0; row < r1; ++row) {
+        const int kidx = a.rowKernel[(long)v * a.nrows + row];
+        if (BCK) {
+#pragma unroll
+            for (int t = 0; t < NZT; ++t) acc[t] = floatx4_t{0.0f, 0.0f, 0.0f, 0.0f};
+        }
+        if (kidx < 0) {
+            if (x < a.NRot) {
+#pragma unroll
+                for (int t = 0; t < NZT; ++t)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int z = z0 + 16 * t + 4 * g + r;
+                        if (z < a.Nz) acc[t][r] += drf_in(a, b, v, row, z, x);
+                    }
+            }
+        } else {
+            const int pad = a.rowPad[(long)v * a.nrows + row];
+            // stage pairs of consecutive slices as packed bf16 (hi, and lo)
+            for (int e = threadIdx.x; e < (THP / 2) * TW; e += 256) {
+                const int c = e / (THP / 2), q = 2 * (e - c * (THP / 2));
+                const float f0 = drf_stage_val(a, b, v, row, z0 - CY + q, x0 - CX + c, pad);
+                const float f1 = drf_stage_val(a, b, v, row, z0 - CY + q + 1, x0 - CX + c, pad);
+                const __bf16 h0 = (__bf16)f0, h1 = (__bf16)f1;
+                tileH[(c * CSB + q) / 2] = pack_bf16(h0, h1);
+                if (SPLIT)
+                    tileL[(c * CSB + q) / 2] =
+                        pack_bf16((__bf16)(f0 - (float)h0), (__bf16)(f1 - (float)h1));
+            }
+            __syncthreads();
+            const uint4 *ap = afr + (long)kidx * KX * NKS * 64 + lane;
+            const uint4 *apl = SPLIT ? afrLo + (long)kidx * KX * NKS * 64 + lane : nullptr;
+            for (int i = 0; i < KX; ++i) {
+                bf16x8_t A[NKS], AL[NKS];
+#pragma unroll
+                for (int s = 0; s < NKS; ++s) {
+                    A[s] = __builtin_bit_cast(bf16x8_t, ap[(i * NKS + s) * 64]);
+                    if (SPLIT) AL[s] = __builtin_bit_cast(bf16x8_t, apl[(i * NKS + s) * 64]);
+                }
+                const int col = 16 * w + n + KX - 1 - i;
+                const uint32_t *ch = tileH + (col * CSB + 8 * g) / 2;
+                const uint32_t *cl = tileL + (col * CSB + 8 * g) / 2;
+#pragma unroll
+                for (int t = 0; t < NZT; ++t)
+#pragma unroll
+                    for (int s = 0; s < NKS; ++s) {
+                        const bf16x8_t B =
+                            __builtin_bit_cast(bf16x8_t, *(const uint4 *)(ch + (16 * t + 32 * s) / 2));
+                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[s], B, acc[t], 0, 0, 0);
+                        if (SPLIT) {
+                            const bf16x8_t BL = __builtin_bit_cast(
+                                bf16x8_t, *(const uint4 *)(cl + (16 * t + 32 * s) / 2));
+                            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[s], BL, acc[t], 0, 0, 0);
+                            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(AL[s], B, acc[t], 0, 0, 0);
+                        }
+                    }
+            }
+            __syncthreads();
+        }
+        if (BCK && x < a.NRot) {
+            float *o = a.out + (long)b * a.outB + (long)row * a.outRow + (long)x * a.outXS;
+#pragma unroll
+            for (int t = 0; t < NZT; ++t) {
+                const int z = z0 + 16 * t + 4 * g;
+                if (z < a.Nz)   // Nz % 4 == 0 on this path
+                    *(float4 *)(o + z) = make_float4(acc[t][0] * a.scale, acc[t][1] * a.scale,
+                                                     acc[t][2] * a.scale, acc[t][3] * a.scale);
+            }
+        }
+    }
+    if (!BCK && x < a.NRot) {
+        const int ch = r0 / a.rchunk;
+        float *o = a.out + (long)b * a.outB + (long)ch * a.outRow + (long)x * a.outXS;
+#pragma unroll
+        for (int t = 0; t < NZT; ++t) {
+            const int z = z0 + 16 * t + 4 * g;
+            if (z < a.Nz) *(float4 *)(o + z) = make_float4(acc[t][0], acc[t][1], acc[t][2], acc[t][3]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // k_ratio: projection -> measured/calculated ratio (osemrecon.c:727-820).
 // ---------------------------------------------------------------------------
 struct RatioArgs {
@@ -740,6 +890,8 @@ __global__ void k_norm_input(int B, int Nz, int NRot, const int *__restrict__ bv
 static int div_up(long a, long b) { return (int)((a + b - 1) / b); }
 
 static void launch_drf(hipStream_t stream, bool bck, DrfArgs args, int B, int KYb);
+static void launch_drf_bf16(hipStream_t stream, bool bck, DrfArgs args, int B, int KYb,
+                            bool split, const uint4 *afr, const uint4 *afrLo);
 
 template <typename T>
 struct DBuf {
@@ -857,6 +1009,9 @@ struct Recon::Impl {
     std::unique_ptr<EsseDev> esse;
     DBuf<float> fsin, fcos, bsin, bcos, vt, vtsens, psf;
     DBuf<int> rowKernel, rowPad, subsets, head, npix, rowNone;
+    // BF16 precision variant: Toeplitz A fragments of every table entry
+    DBuf<uint4> afrHi, afrLo;
+    bool bf16 = false, bf16Split = false;
     // three-shear rotation tables, slot 2v (forward) and 2v+1 (back)
     ShearSet shset;
     ShearDev shear() const { return shset.dev(); }
@@ -1029,10 +1184,78 @@ struct Recon::Impl {
 
     void run_drf(bool bck, const DrfArgs &args, int B, int KYb)
     {
-        launch_drf(stream, bck, args, B, KYb);
+        if (bf16)
+            launch_drf_bf16(stream, bck, args, B, KYb, bf16Split, afrHi.p, afrLo.p);
+        else
+            launch_drf(stream, bck, args, B, KYb);
     }
 };
 
+
+// Round-to-nearest-even float -> bf16 bits, and back.
+static uint16_t f2bf(float f)
+{
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    u += 0x7FFF + ((u >> 16) & 1);
+    return (uint16_t)(u >> 16);
+}
+
+static float bf2f(uint16_t h)
+{
+    const uint32_t u = (uint32_t)h << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+// A fragments of k_drf_bf16 for every table entry, column i and K step s:
+// lane l holds A_i[m = l%16][k = 32s + 8(l/16) + e], e < 8, with A_i[m][k] =
+// psf[m + KYb - 1 - k][i] inside the band (PSF centred in the KYb bucket),
+// as bf16 hi parts and lo = bf16(value - hi).  Returns the largest
+// estimated per-plane relative error of the plain BF16 product over the
+// entries the rows use: 1.6e-3 (the std of the relative rounding of a product
+// of two bf16-rounded operands) times |psf|_2 / |psf|_1.
+static double build_bf16_fragments(const PaddedDrf &d, int KYb, const std::vector<int> &rowKernel,
+                                   std::vector<uint4> &hi, std::vector<uint4> &lo)
+{
+    const int NKS = drf16_nks(KYb), oy = (KYb - d.KY) / 2;
+    hi.assign((size_t)d.nDist * d.KX * NKS * 64, uint4{0, 0, 0, 0});
+    lo = hi;
+    for (int k = 0; k < d.nDist; ++k)
+        for (int i = 0; i < d.KX; ++i)
+            for (int st = 0; st < NKS; ++st)
+                for (int l = 0; l < 64; ++l) {
+                    uint16_t h[8], w[8];
+                    for (int e = 0; e < 8; ++e) {
+                        const int m = l % 16, kk = 32 * st + 8 * (l / 16) + e;
+                        const int j = m + KYb - 1 - kk - oy;
+                        const float val = (j >= 0 && j < d.KY)
+                                              ? d.psf[((size_t)k * d.KY + j) * d.KX + i]
+                                              : 0.0f;
+                        h[e] = f2bf(val);
+                        w[e] = f2bf(val - bf2f(h[e]));
+                    }
+                    const size_t o = (((size_t)k * d.KX + i) * NKS + st) * 64 + l;
+                    memcpy(&hi[o], h, 16);
+                    memcpy(&lo[o], w, 16);
+                }
+    std::vector<char> used(d.nDist, 0);
+    for (int r : rowKernel)
+        if (r >= 0 && r < d.nDist) used[r] = 1;
+    double worst = 0;
+    for (int k = 0; k < d.nDist; ++k) {
+        if (!used[k]) continue;
+        double l1 = 0, l2 = 0;
+        for (int t = 0; t < d.KX * d.KY; ++t) {
+            const double p = d.psf[(size_t)k * d.KX * d.KY + t];
+            l1 += fabs(p);
+            l2 += p * p;
+        }
+        if (l1 > 0) worst = std::max(worst, 1.6e-3 * sqrt(l2) / l1);
+    }
+    return worst;
+}
 
 Recon::Recon(const Config &cfg) : impl_(new Impl), cfg_(cfg)
 {
@@ -1118,6 +1341,19 @@ Recon::Recon(const Config &cfg) : impl_(new Impl), cfg_(cfg)
                     padded[((size_t)d * KX + i) * KYP + j + oy] =
                         cfg.drf.psf[((size_t)d * KY + j) * KX + i];
         up(I.psf, padded);
+        if (cfg.precision > 0) {
+            if (g.Nz % 4)
+                fail(ETYPE_ILLEGAL_VALUE, "the BF16 DRF path needs NumSlices %% 4 == 0 (%d)", g.Nz);
+            std::vector<uint4> hi, lo;
+            const double est = build_bf16_fragments(cfg.drf, KYb, cfg.rowKernel, hi, lo);
+            I.bf16 = true;
+            I.bf16Split = cfg.precision == 2 || (cfg.precision == 1 && est > 1e-3);
+            I.afrHi.alloc(hi.size());
+            I.afrHi.upload(hi.data(), hi.size(), I.stream);
+            I.afrLo.alloc(lo.size());
+            I.afrLo.upload(lo.data(), lo.size(), I.stream);
+            HIP_CHECK(hipStreamSynchronize(I.stream));   // host vectors die here
+        }
     }
     std::vector<int> rk = cfg.rowKernel, rp = cfg.rowPad;
     if (rk.empty()) rk.assign((size_t)g.V * g.NRot, -1);
@@ -1892,6 +2128,47 @@ static void launch_drf(hipStream_t stream, bool bck, DrfArgs args, int B, int KY
         fail(ETYPE_ILLEGAL_VALUE, "unsupported DRF bucket %d", KYb);
     }
 #undef IRL_DRF_CASE
+    HIP_CHECK(hipGetLastError());
+}
+
+static void launch_drf_bf16(hipStream_t stream, bool bck, DrfArgs args, int B, int KYb,
+                            bool split, const uint4 *afr, const uint4 *afrLo)
+{
+    if (args.zs != 1 || args.outZS != 1 || args.Nz % 4)
+        fail(ETYPE_ILLEGAL_VALUE, "the BF16 DRF path needs slice-fastest planes, Nz %% 4 == 0");
+    const int NZT = args.Nz > 64 ? 8 : 4;
+    const int TW = DRF_TX + args.KX - 1;
+    const int THP = 16 * (NZT - 1) + 32 * drf16_nks(KYb);
+    const size_t lds = (size_t)TW * drf16_cs(THP) * 2 * (split ? 2 : 1);
+    if (lds > 160 * 1024) fail(ETYPE_ILLEGAL_VALUE, "DRF PSF too wide for the BF16 path (KX=%d)", args.KX);
+    args.ztiles = div_up(args.Nz, 16 * NZT);
+    const long nblocks = (long)args.xtiles * args.ztiles * B * (bck ? args.nrows : args.nchunk);
+    if (nblocks <= 0 || nblocks > INT_MAX) fail(ETYPE_ILLEGAL_VALUE, "bad DRF launch size");
+    dim3 grid((unsigned)nblocks), block(256);
+#define IRL_DRF16_LAUNCH(K, Z, BK, SP) \
+    hipLaunchKernelGGL((k_drf_bf16<K, Z, BK, SP>), grid, block, lds, stream, args, afr, afrLo)
+#define IRL_DRF16_CASE(K)                                                            \
+    case K:                                                                          \
+        if (NZT == 8) {                                                              \
+            if (bck) { if (split) IRL_DRF16_LAUNCH(K, 8, true, true); else IRL_DRF16_LAUNCH(K, 8, true, false); } \
+            else { if (split) IRL_DRF16_LAUNCH(K, 8, false, true); else IRL_DRF16_LAUNCH(K, 8, false, false); } \
+        } else {                                                                     \
+            if (bck) { if (split) IRL_DRF16_LAUNCH(K, 4, true, true); else IRL_DRF16_LAUNCH(K, 4, true, false); } \
+            else { if (split) IRL_DRF16_LAUNCH(K, 4, false, true); else IRL_DRF16_LAUNCH(K, 4, false, false); } \
+        }                                                                            \
+        break;
+    switch (KYb) {
+        IRL_DRF16_CASE(7)
+        IRL_DRF16_CASE(15)
+        IRL_DRF16_CASE(23)
+        IRL_DRF16_CASE(31)
+        IRL_DRF16_CASE(47)
+        IRL_DRF16_CASE(63)
+    default:
+        fail(ETYPE_ILLEGAL_VALUE, "unsupported DRF bucket %d", KYb);
+    }
+#undef IRL_DRF16_CASE
+#undef IRL_DRF16_LAUNCH
     HIP_CHECK(hipGetLastError());
 }
 
