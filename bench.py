@@ -28,6 +28,7 @@ sys.path.insert(0, ROOT)
 METRIC = "OSEM subiters/s, 128³×120 views: ESSE vs CNN-scatter at ≤1% NRMSE"
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
+PEAK_BF16_TFLOPS = 2500.0
 
 
 def build_workload(tmpdir, seed=2, name="c2"):
@@ -67,13 +68,19 @@ def build_workload(tmpdir, seed=2, name="c2"):
                 act=act, offs=offs, psfs=psfs, start=start, cfg=cfg, krnl_file=krnl_file)
 
 
-def time_subiterations(w, steps, warmup):
-    """Device-resident reconstruction of the workload; returns (recon, seconds
-    for 'steps' sub-iterations, per-kernel {name: (ms_total, launches)})."""
+def time_subiterations(w, steps, warmup, precision="fp32"):
+    """Device-resident reconstruction of the workload: 'warmup' untimed
+    sub-iterations, then exactly 'steps' timed ones with per-kernel HIP events
+    on the engine's stream.  Returns (recon, seconds of the timed steps,
+    {kernel: (ms_total, launches)}, estimate after the timed steps)."""
     import libirl_amd as L
     L.set_adv_options(0, 0)
-    rec = L.Recon(w["parms"], w["opts"], w["views"], w["drf_file"], w["krnl_file"], None,
-                  w["mu"], w["prj"], None)
+    prev = L.set_precision(precision)
+    try:
+        rec = L.Recon(w["parms"], w["opts"], w["views"], w["drf_file"], w["krnl_file"], None,
+                      w["mu"], w["prj"], None)
+    finally:
+        L.set_precision(prev)
     S = rec.num_subsets
     for k in range(warmup):
         rec.subiteration(k % S)
@@ -83,12 +90,12 @@ def time_subiterations(w, steps, warmup):
     rec.synchronize()
     t0 = time.perf_counter()
     for k in range(steps):
-        rec.subiteration(k % S)
+        rec.subiteration((warmup + k) % S)
     rec.synchronize()
     dt = time.perf_counter() - t0
     stages = rec.stage_times()
     rec.set_timing(False)
-    return rec, dt, stages
+    return rec, dt, stages, rec.estimate()
 
 
 def drf_flops_per_subset(w, recon):
@@ -137,6 +144,81 @@ def cpu_baseline(w, nviews=15):
                        "(%.1f s), scaled x%.2f" % (nviews, dt, p.NumAngPerSubset / nviews))
 
 
+def drf_roofline(w, rec, stages, steps, warmup):
+    """Dominant DRF stage and its achieved algorithmic TFLOP/s (flops of the
+    timed subsets / average launch time from the HIP events)."""
+    S = rec.num_subsets
+    flops = drf_flops_per_subset(w, rec)
+    avg_flops = float(np.mean([flops[(warmup + k) % S] for k in range(steps)]))
+    cands = [k for k in stages if k.startswith("drf_prj") or k.startswith("drf_bck")]
+    dom = max(cands, key=lambda k: stages[k][0])
+    dom_ms, dom_n = stages[dom]
+    return dom, avg_flops / (dom_ms / dom_n * 1e-3) / 1e12
+
+
+def workload_line(w, steps, warmup, precision, ref_estimate=None):
+    """Sub-object of the JSON line for one workload and DRF arithmetic."""
+    rec, dt, stages, est = time_subiterations(w, steps, warmup, precision)
+    dom, achieved = drf_roofline(w, rec, stages, steps, warmup)
+    esse = w["krnl_file"] is not None
+    name = "osem-128^3x120v-atn-drf31-%s8subsets" % ("esse33-" if esse else "")
+    bf16 = precision != "fp32"
+    peak = PEAK_BF16_TFLOPS if bf16 else PEAK_FP32_TFLOPS
+    line = {"workload": name + (" (BF16 MFMA DRF, FP32 accumulation)" if bf16 else " (FP32)"),
+            "value": steps / dt, "unit": "subiters/s", "ms_per_step": dt * 1e3 / steps,
+            "drf_arithmetic": dom.split("_", 2)[2] if dom.count("_") >= 2 else "fp32",
+            "roofline": {"kernel": dom, "bound": "mfma", "achieved": achieved, "peak": peak,
+                         "unit": "TFLOP/s", "frac": achieved / peak},
+            "stages_ms_per_launch": {k: v[0] / max(v[1], 1) for k, v in stages.items()}}
+    if ref_estimate is not None:
+        line["errors"] = {"osem_rel_l2_vs_fp32_gpu_after_%d_subiterations" % (warmup + steps):
+                          float(np.linalg.norm(est - ref_estimate) /
+                                np.linalg.norm(ref_estimate))}
+    else:
+        line["_estimate"] = est
+    rec.close()
+    return line
+
+
+def bf16_projection_error(w, tmpdir, nviews=2):
+    """Forward projection (IrlGenprj) of two views of the workload with the
+    BF16 DRF against the FP64 oracle: the largest per-view relative L2."""
+    import libirl_amd as L
+    from oracle import oracle as O
+    p = w["parms"]
+    n, nz, pw = p.NumPixels, p.NumSlices, p.BinWidth
+    views = [w["views"][k] for k in (0, len(w["views"]) // 3)][:nviews]
+    q = L.IrlParms()
+    ctypes_copy(q, p)
+    q.NumViews = len(views)
+    q.NumAngPerSubset = len(views)
+    q.iNumSrfIterations = 0
+    prev = L.set_precision("bf16")
+    try:
+        prj = np.zeros((len(views), nz, n), np.float32)
+        rc = L.irl_genprj(q, w["opts"], views, w["drf_file"], None, None, w["act"].copy(),
+                          w["mu"].copy(), prj)
+        if rc != 0:
+            raise RuntimeError("IrlGenprj failed: " + L.error_string())
+    finally:
+        L.set_precision(prev)
+    drf = O.DrfTable(w["start"], pw, w["offs"], w["psfs"])
+    atn = O.reorder(nz, n, n, w["mu"] * pw)
+    x = O.reorder(nz, n, n, w["act"])
+    O.init_prj(nz, n, n, 3)
+    errs = []
+    for k, v in enumerate(views):
+        ref = O.project(nz, n, n, 3, pw, v.Angle, v.CFCR, drf, atn, x, fast=1)
+        ref = ref * np.float32(1.0 / np.float32(len(views)))
+        errs.append(float(np.linalg.norm(prj[k] - ref) / np.linalg.norm(ref)))
+    return max(errs)
+
+
+def ctypes_copy(dst, src):
+    for f, _ in src._fields_:
+        setattr(dst, f, getattr(src, f))
+
+
 def measured_traffic(stage):
     """HBM bytes per launch of a stage's kernel, from the committed
     rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this workload (counters
@@ -160,6 +242,9 @@ def main():
     ap.add_argument("--no-esse", action="store_true", help="skip the ESSE workload line")
     ap.add_argument("--only-esse", action="store_true",
                     help="time only the ESSE workload (for profiling runs)")
+    ap.add_argument("--no-bf16", action="store_true", help="skip the BF16 precision variant")
+    ap.add_argument("--no-errors", action="store_true",
+                    help="skip the FP64-oracle projection error of the BF16 variant")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -180,22 +265,14 @@ def main():
     tmpdir = tempfile.mkdtemp(prefix="irlbench")
     if args.only_esse:
         we = build_workload(tmpdir, name="c3")
-        rec_e, dte, st_e = time_subiterations(we, args.steps, args.warmup)
-        rec_e.close()
-        print(json.dumps({"workload": "osem-128^3x120v-atn-drf31-esse33-8subsets (FP32)",
-                          "value": args.steps / dte, "unit": "subiters/s",
-                          "stages_ms_per_launch": {k: v[0] / max(v[1], 1)
-                                                   for k, v in st_e.items()}}))
+        line = workload_line(we, args.steps, args.warmup, "fp32")
+        print(json.dumps(line))
         return
     w = build_workload(tmpdir)
-    rec, dt, stages = time_subiterations(w, args.steps, args.warmup)
+    rec, dt, stages, est32 = time_subiterations(w, args.steps, args.warmup)
     S = rec.num_subsets
     ms_step = dt * 1e3 / args.steps
-    flops = drf_flops_per_subset(w, rec)
-    avg_flops = float(np.mean([flops[k % S] for k in range(args.steps)]))
-    dom = max(("drf_prj", "drf_bck"), key=lambda k: stages.get(k, (0, 1))[0])
-    dom_ms, dom_n = stages[dom]
-    achieved = avg_flops / (dom_ms / dom_n * 1e-3) / 1e12
+    dom, achieved = drf_roofline(w, rec, stages, args.steps, args.warmup)
     traffic = measured_traffic(dom)
     out = {
         "metric": METRIC,
@@ -220,16 +297,21 @@ def main():
         "stages_ms_per_launch": {k: v[0] / max(v[1], 1) for k, v in stages.items()},
     }
     rec.close()
+    if not args.no_bf16:
+        # Precision variant of the same workload: BF16 MFMA DRF convolution.
+        line = workload_line(w, args.steps, args.warmup, "bf16", ref_estimate=est32)
+        if not args.no_errors:
+            line["errors"]["projection_rel_l2_vs_fp64_oracle"] = bf16_projection_error(w, tmpdir)
+        out["bf16"] = line
     if not args.no_esse:
         # Same geometry with ESSE scatter modelled in the projector (the
-        # plain-method scatter configuration); sub-iterations of the first
-        # scatter-modelling iterations.
+        # plain-method scatter configuration), FP32 and with the BF16 DRF.
         we = build_workload(tmpdir, name="c3")
-        rec_e, dte, st_e = time_subiterations(we, 16, 8)
-        rec_e.close()
-        out["esse"] = {"workload": "osem-128^3x120v-atn-drf31-esse33-8subsets (FP32)",
-                       "value": 16 / dte, "unit": "subiters/s", "ms_per_step": dte * 1e3 / 16,
-                       "stages_ms_per_launch": {k: v[0] / max(v[1], 1) for k, v in st_e.items()}}
+        out["esse"] = workload_line(we, 16, 8, "fp32")
+        if not args.no_bf16:
+            out["esse_bf16"] = workload_line(we, 16, 8, "bf16",
+                                             ref_estimate=out["esse"].pop("_estimate"))
+        out["esse"].pop("_estimate", None)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w)
     print(json.dumps(out))

@@ -1029,7 +1029,7 @@ struct Recon::Impl {
     std::vector<std::string> order;
 
     template <typename F>
-    void launch(const char *name, F &&f)
+    void launch(const std::string &name, F &&f)
     {
         if (!timing) {
             f();
@@ -1180,6 +1180,14 @@ struct Recon::Impl {
             d.nchunk = nchunk;
         }
         return d;
+    }
+
+    // Stage name of a DRF launch: the precision variant is suffixed _bf16
+    // (plain BF16 operands) or _bf16x3 (hi+lo split).
+    std::string drf_name(const char *base) const
+    {
+        if (!bf16) return base;
+        return std::string(base) + (bf16Split ? "_bf16x3" : "_bf16");
     }
 
     void run_drf(bool bck, const DrfArgs &args, int B, int KYb)
@@ -1560,7 +1568,7 @@ void Recon::precompute(bool norms)
                                I.vt.p, I.ratio.p);
         });
         const DrfArgs da = I.drf_args(true, bview, cfg_.bckModel);
-        I.launch("drf_bck_norm", [&] { I.run_drf(true, da, Vs, KYb); });
+        I.launch(I.drf_name("drf_bck_norm"), [&] { I.run_drf(true, da, Vs, KYb); });
         BckArgs ba = I.bck_args(bview);
         ba.ninvOut = I.ninv.p + I.M * s;
         I.launch("norm", [&] { I.run_bck(true, ba); });
@@ -1614,14 +1622,14 @@ void Recon::runViews(int s, int first, int Vs, SrfMode srf, bool accumulate, boo
     else
         I.launch("att", [&] { I.run_att(afac, Vs, bview); });
     const DrfArgs da = I.drf_args(false, bview, cfg_.prjModel);
-    I.launch("drf_prj", [&] { I.run_drf(false, da, Vs, KYb); });
+    I.launch(I.drf_name("drf_prj"), [&] { I.run_drf(false, da, Vs, KYb); });
     if (srf == SRF_MODEL_STORE) {
         // projection without the scatter model for the stored estimate
         // (osemrecon.c:757-786), reusing the rotated estimate
         I.launch("att", [&] { I.run_att(afac, Vs, bview); });
         DrfArgs d2 = da;
         d2.out = I.ppart2.p;
-        I.launch("drf_prj", [&] { I.run_drf(false, d2, Vs, KYb); });
+        I.launch(I.drf_name("drf_prj"), [&] { I.run_drf(false, d2, Vs, KYb); });
     }
     RatioArgs ra{};
     ra.srfMode = srf == SRF_MODEL_STORE ? 2 : srf == SRF_ADD_STORED ? 3 : 0;
@@ -1645,7 +1653,7 @@ void Recon::runViews(int s, int first, int Vs, SrfMode srf, bool accumulate, boo
                            I.stream, ra);
     });
     const DrfArgs db = I.drf_args(true, bview, cfg_.bckModel);
-    I.launch("drf_bck", [&] { I.run_drf(true, db, Vs, KYb); });
+    I.launch(I.drf_name("drf_bck"), [&] { I.run_drf(true, db, Vs, KYb); });
     BckArgs ba = I.bck_args(bview);
     ba.B = Vs;
     ba.x = update ? I.x.p : nullptr;
@@ -1686,7 +1694,7 @@ void Recon::forwardProject(float primary, float *out_host, bool srf)
             I.launch("att", [&] { I.run_att(afac, B, bview); });
         }
         const DrfArgs da = I.drf_args(false, bview, cfg_.prjModel);
-        I.launch("drf_prj", [&] { I.run_drf(false, da, B, KYb); });
+        I.launch(I.drf_name("drf_prj"), [&] { I.run_drf(false, da, B, KYb); });
         RatioArgs ra{};
         ra.B = B;
         ra.Nz = g.Nz;
