@@ -302,6 +302,13 @@ def main():
         line = workload_line(w, args.steps, args.warmup, "bf16", ref_estimate=est32)
         if not args.no_errors:
             line["errors"]["projection_rel_l2_vs_fp64_oracle"] = bf16_projection_error(w, tmpdir)
+        # the two fixed BF16 modes beside the per-plane choice
+        line["modes"] = {}
+        for mode in ("bf16-plain", "bf16x3"):
+            alt = workload_line(w, args.steps, args.warmup, mode, ref_estimate=est32)
+            line["modes"][mode] = {"value": alt["value"], "drf_ms_per_launch": {
+                k: v for k, v in alt["stages_ms_per_launch"].items() if k.startswith("drf")},
+                "errors": alt["errors"]}
         out["bf16"] = line
     if not args.no_esse:
         # Same geometry with ESSE scatter modelled in the projector (the

@@ -592,8 +592,11 @@ __global__ __launch_bounds__(64 * ZG) void k_drf(DrfArgs a)
 // one v_mfma_f32_16x16x32_bf16 per 32 rows of the window (M = 16 output
 // slices, N = 16 bins, K = window slices).  The A fragments of every table
 // entry are built on the host; B fragments are 128-bit reads of the bf16
-// tile (slices contiguous per bin).  SPLIT keeps hi and lo bf16 parts of
-// both operands and adds the two cross products (3 MFMAs per step).
+// tile (slices contiguous per bin).  For the table entries flagged in
+// distSplit (SPLIT instantiation only) hi and lo bf16 parts of both operands
+// are kept and the two cross products added (3 MFMAs per step).  Rows that
+// the reference adds unconvolved (distance <= 0) use an extra table entry,
+// a centred delta, always split: hi*1 + lo*1.
 // Block = 4 waves, wave w owns bins [16w, 16w+16) of the 64-bin tile and
 // NZT slice tiles of 16.  Engine layout only (slices fastest).
 // ---------------------------------------------------------------------------
@@ -612,7 +615,8 @@ __device__ __forceinline__ uint32_t pack_bf16(__bf16 lo, __bf16 hi)
 
 template <int KY, int NZT, bool BCK, bool SPLIT>
 __global__ __launch_bounds__(256) void k_drf_bf16(DrfArgs a, const uint4 *__restrict__ afr,
-                                                  const uint4 *__restrict__ afrLo)
+                                                  const uint4 *__restrict__ afrLo,
+                                                  const int *__restrict__ distSplit)
 {
     extern __shared__ uint32_t ldsw[];
     constexpr int CY = (KY - 1) / 2;
@@ -658,18 +662,10 @@ __global__ __launch_bounds__(256) void k_drf_bf16(DrfArgs a, const uint4 *__rest
 #pragma unroll
             for (int t = 0; t < NZT; ++t) acc[t] = floatx4_t{0.0f, 0.0f, 0.0f, 0.0f};
         }
-        if (kidx < 0) {
-            if (x < a.NRot) {
-#pragma unroll
-                for (int t = 0; t < NZT; ++t)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int z = z0 + 16 * t + 4 * g + r;
-                        if (z < a.Nz) acc[t][r] += drf_in(a, b, v, row, z, x);
-                    }
-            }
-        } else {
+        {
             const int pad = a.rowPad[(long)v * a.nrows + row];
+            // hi+lo split for the planes whose estimated BF16 error is too large
+            const bool sp = SPLIT && distSplit[kidx];
             // stage pairs of consecutive slices as packed bf16 (hi, and lo)
             for (int e = threadIdx.x; e < (THP / 2) * TW; e += 256) {
                 const int c = e / (THP / 2), q = 2 * (e - c * (THP / 2));
@@ -677,37 +673,65 @@ __global__ __launch_bounds__(256) void k_drf_bf16(DrfArgs a, const uint4 *__rest
                 const float f1 = drf_stage_val(a, b, v, row, z0 - CY + q + 1, x0 - CX + c, pad);
                 const __bf16 h0 = (__bf16)f0, h1 = (__bf16)f1;
                 tileH[(c * CSB + q) / 2] = pack_bf16(h0, h1);
-                if (SPLIT)
+                if (sp)
                     tileL[(c * CSB + q) / 2] =
                         pack_bf16((__bf16)(f0 - (float)h0), (__bf16)(f1 - (float)h1));
             }
             __syncthreads();
             const uint4 *ap = afr + (long)kidx * KX * NKS * 64 + lane;
-            const uint4 *apl = SPLIT ? afrLo + (long)kidx * KX * NKS * 64 + lane : nullptr;
-            for (int i = 0; i < KX; ++i) {
-                bf16x8_t A[NKS], AL[NKS];
-#pragma unroll
-                for (int s = 0; s < NKS; ++s) {
-                    A[s] = __builtin_bit_cast(bf16x8_t, ap[(i * NKS + s) * 64]);
-                    if (SPLIT) AL[s] = __builtin_bit_cast(bf16x8_t, apl[(i * NKS + s) * 64]);
-                }
-                const int col = 16 * w + n + KX - 1 - i;
-                const uint32_t *ch = tileH + (col * CSB + 8 * g) / 2;
-                const uint32_t *cl = tileL + (col * CSB + 8 * g) / 2;
-#pragma unroll
-                for (int t = 0; t < NZT; ++t)
+            const uint4 *apl = afrLo + (long)kidx * KX * NKS * 64 + lane;
+            if (sp) {
+                for (int i = 0; i < KX; ++i) {
+                    bf16x8_t A[NKS], AL[NKS];
 #pragma unroll
                     for (int s = 0; s < NKS; ++s) {
-                        const bf16x8_t B =
-                            __builtin_bit_cast(bf16x8_t, *(const uint4 *)(ch + (16 * t + 32 * s) / 2));
-                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[s], B, acc[t], 0, 0, 0);
-                        if (SPLIT) {
+                        A[s] = __builtin_bit_cast(bf16x8_t, ap[(i * NKS + s) * 64]);
+                        AL[s] = __builtin_bit_cast(bf16x8_t, apl[(i * NKS + s) * 64]);
+                    }
+                    const int col = 16 * w + n + KX - 1 - i;
+                    const uint32_t *ch = tileH + (col * CSB + 8 * g) / 2;
+                    const uint32_t *cl = tileL + (col * CSB + 8 * g) / 2;
+#pragma unroll
+                    for (int t = 0; t < NZT; ++t)
+#pragma unroll
+                        for (int s = 0; s < NKS; ++s) {
+                            const bf16x8_t B = __builtin_bit_cast(
+                                bf16x8_t, *(const uint4 *)(ch + (16 * t + 32 * s) / 2));
                             const bf16x8_t BL = __builtin_bit_cast(
                                 bf16x8_t, *(const uint4 *)(cl + (16 * t + 32 * s) / 2));
+                            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[s], B, acc[t], 0, 0, 0);
                             acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[s], BL, acc[t], 0, 0, 0);
                             acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(AL[s], B, acc[t], 0, 0, 0);
                         }
+                }
+            } else {
+                // the window of tile t, step s starts at row 16(t + 2s): the
+                // NZT + 2(NKS-1) distinct B fragments of a column are read once
+                constexpr int NB = NZT + 2 * (NKS - 1);
+                bf16x8_t A[NKS], An[NKS];
+#pragma unroll
+                for (int s = 0; s < NKS; ++s) A[s] = __builtin_bit_cast(bf16x8_t, ap[s * 64]);
+                for (int i = 0; i < KX; ++i) {
+                    if (i + 1 < KX) {
+#pragma unroll
+                        for (int s = 0; s < NKS; ++s)
+                            An[s] = __builtin_bit_cast(bf16x8_t, ap[((i + 1) * NKS + s) * 64]);
                     }
+                    const int col = 16 * w + n + KX - 1 - i;
+                    const uint32_t *ch = tileH + (col * CSB + 8 * g) / 2;
+                    bf16x8_t B[NB];
+#pragma unroll
+                    for (int u = 0; u < NB; ++u)
+                        B[u] = __builtin_bit_cast(bf16x8_t, *(const uint4 *)(ch + 8 * u));
+#pragma unroll
+                    for (int t = 0; t < NZT; ++t)
+#pragma unroll
+                        for (int s = 0; s < NKS; ++s)
+                            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[s], B[t + 2 * s],
+                                                                             acc[t], 0, 0, 0);
+#pragma unroll
+                    for (int s = 0; s < NKS; ++s) A[s] = An[s];
+                }
             }
             __syncthreads();
         }
@@ -891,7 +915,8 @@ static int div_up(long a, long b) { return (int)((a + b - 1) / b); }
 
 static void launch_drf(hipStream_t stream, bool bck, DrfArgs args, int B, int KYb);
 static void launch_drf_bf16(hipStream_t stream, bool bck, DrfArgs args, int B, int KYb,
-                            bool split, const uint4 *afr, const uint4 *afrLo);
+                            bool split, const uint4 *afr, const uint4 *afrLo,
+                            const int *distSplit);
 
 template <typename T>
 struct DBuf {
@@ -1011,7 +1036,10 @@ struct Recon::Impl {
     DBuf<int> rowKernel, rowPad, subsets, head, npix, rowNone;
     // BF16 precision variant: Toeplitz A fragments of every table entry
     DBuf<uint4> afrHi, afrLo;
-    bool bf16 = false, bf16Split = false;
+    DBuf<int> distSplit;          // per table entry: hi+lo split
+    DBuf<int> rowKernel16;        // rowKernel with -1 replaced by the delta entry nDist
+    bool bf16 = false, bf16Split = false;   // bf16Split: some entry of the table is split
+    int nSplit = 0, nUsed = 0;              // split / used table entries
     // three-shear rotation tables, slot 2v (forward) and 2v+1 (back)
     ShearSet shset;
     ShearDev shear() const { return shset.dev(); }
@@ -1148,7 +1176,7 @@ struct Recon::Impl {
         d.nrows = g.NRot;
         d.KX = c.drf.nDist > 0 ? c.drf.KX : 1;
         d.psf = psf.p;
-        d.rowKernel = (model & 2) ? rowKernel.p : rowNone.p;
+        d.rowKernel = (model & 2) ? (bf16 ? rowKernel16.p : rowKernel.p) : rowNone.p;
         d.rowPad = rowPad.p;
         d.avg = c.axialAvg;
         d.bview = bview;
@@ -1187,13 +1215,15 @@ struct Recon::Impl {
     std::string drf_name(const char *base) const
     {
         if (!bf16) return base;
-        return std::string(base) + (bf16Split ? "_bf16x3" : "_bf16");
+        const char *suf = nSplit == 0 ? "_bf16" : nSplit < nUsed ? "_bf16mix" : "_bf16x3";
+        return std::string(base) + suf;
     }
 
     void run_drf(bool bck, const DrfArgs &args, int B, int KYb)
     {
         if (bf16)
-            launch_drf_bf16(stream, bck, args, B, KYb, bf16Split, afrHi.p, afrLo.p);
+            launch_drf_bf16(stream, bck, args, B, KYb, bf16Split, afrHi.p, afrLo.p,
+                            distSplit.p);
         else
             launch_drf(stream, bck, args, B, KYb);
     }
@@ -1220,12 +1250,12 @@ static float bf2f(uint16_t h)
 // A fragments of k_drf_bf16 for every table entry, column i and K step s:
 // lane l holds A_i[m = l%16][k = 32s + 8(l/16) + e], e < 8, with A_i[m][k] =
 // psf[m + KYb - 1 - k][i] inside the band (PSF centred in the KYb bucket),
-// as bf16 hi parts and lo = bf16(value - hi).  Returns the largest
-// estimated per-plane relative error of the plain BF16 product over the
-// entries the rows use: 1.6e-3 (the std of the relative rounding of a product
-// of two bf16-rounded operands) times |psf|_2 / |psf|_1.
-static double build_bf16_fragments(const PaddedDrf &d, int KYb, const std::vector<int> &rowKernel,
-                                   std::vector<uint4> &hi, std::vector<uint4> &lo)
+// as bf16 hi parts and lo = bf16(value - hi).  est[k] is the estimated
+// relative error of a plane convolved with entry k in plain BF16: 1.6e-3
+// (the std of the relative rounding of a product of two bf16-rounded
+// operands) times |psf|_2 / |psf|_1.
+static void build_bf16_fragments(const PaddedDrf &d, int KYb, std::vector<uint4> &hi,
+                                 std::vector<uint4> &lo, std::vector<double> &est)
 {
     const int NKS = drf16_nks(KYb), oy = (KYb - d.KY) / 2;
     hi.assign((size_t)d.nDist * d.KX * NKS * 64, uint4{0, 0, 0, 0});
@@ -1248,21 +1278,16 @@ static double build_bf16_fragments(const PaddedDrf &d, int KYb, const std::vecto
                     memcpy(&hi[o], h, 16);
                     memcpy(&lo[o], w, 16);
                 }
-    std::vector<char> used(d.nDist, 0);
-    for (int r : rowKernel)
-        if (r >= 0 && r < d.nDist) used[r] = 1;
-    double worst = 0;
+    est.assign(d.nDist, 0.0);
     for (int k = 0; k < d.nDist; ++k) {
-        if (!used[k]) continue;
         double l1 = 0, l2 = 0;
         for (int t = 0; t < d.KX * d.KY; ++t) {
             const double p = d.psf[(size_t)k * d.KX * d.KY + t];
             l1 += fabs(p);
             l2 += p * p;
         }
-        if (l1 > 0) worst = std::max(worst, 1.6e-3 * sqrt(l2) / l1);
+        est[k] = l1 > 0 ? 1.6e-3 * sqrt(l2) / l1 : 0.0;
     }
-    return worst;
 }
 
 Recon::Recon(const Config &cfg) : impl_(new Impl), cfg_(cfg)
@@ -1352,10 +1377,32 @@ Recon::Recon(const Config &cfg) : impl_(new Impl), cfg_(cfg)
         if (cfg.precision > 0) {
             if (g.Nz % 4)
                 fail(ETYPE_ILLEGAL_VALUE, "the BF16 DRF path needs NumSlices %% 4 == 0 (%d)", g.Nz);
+            // the table plus a centred delta (entry nDist) for unconvolved rows
+            PaddedDrf d = cfg.drf;
+            const int nd = d.nDist;
+            d.psf.resize((size_t)(nd + 1) * d.KX * d.KY, 0.0f);
+            d.psf[((size_t)nd * d.KY + (d.KY - 1) / 2) * d.KX + (d.KX - 1) / 2] = 1.0f;
+            d.nDist = nd + 1;
             std::vector<uint4> hi, lo;
-            const double est = build_bf16_fragments(cfg.drf, KYb, cfg.rowKernel, hi, lo);
+            std::vector<double> est;
+            build_bf16_fragments(d, KYb, hi, lo, est);
+            std::vector<int> split(nd + 1, 0), used(nd + 1, 0), rk16 = cfg.rowKernel;
+            if (rk16.empty()) rk16.assign((size_t)g.V * g.NRot, -1);
+            for (auto &r : rk16)
+                if (r < 0) r = nd;
+            for (int r : rk16) used[r] = 1;
+            I.nSplit = I.nUsed = 0;
+            for (int k = 0; k <= nd; ++k) {
+                split[k] = k == nd || cfg.precision == 2 || (cfg.precision == 1 && est[k] > 1e-3);
+                I.nUsed += used[k];
+                I.nSplit += used[k] && split[k];
+            }
             I.bf16 = true;
-            I.bf16Split = cfg.precision == 2 || (cfg.precision == 1 && est > 1e-3);
+            I.bf16Split = I.nSplit > 0;
+            I.rowKernel16.alloc(rk16.size());
+            I.rowKernel16.upload(rk16.data(), rk16.size(), I.stream);
+            I.distSplit.alloc(split.size());
+            I.distSplit.upload(split.data(), split.size(), I.stream);
             I.afrHi.alloc(hi.size());
             I.afrHi.upload(hi.data(), hi.size(), I.stream);
             I.afrLo.alloc(lo.size());
@@ -2140,7 +2187,8 @@ static void launch_drf(hipStream_t stream, bool bck, DrfArgs args, int B, int KY
 }
 
 static void launch_drf_bf16(hipStream_t stream, bool bck, DrfArgs args, int B, int KYb,
-                            bool split, const uint4 *afr, const uint4 *afrLo)
+                            bool split, const uint4 *afr, const uint4 *afrLo,
+                            const int *distSplit)
 {
     if (args.zs != 1 || args.outZS != 1 || args.Nz % 4)
         fail(ETYPE_ILLEGAL_VALUE, "the BF16 DRF path needs slice-fastest planes, Nz %% 4 == 0");
@@ -2154,7 +2202,8 @@ static void launch_drf_bf16(hipStream_t stream, bool bck, DrfArgs args, int B, i
     if (nblocks <= 0 || nblocks > INT_MAX) fail(ETYPE_ILLEGAL_VALUE, "bad DRF launch size");
     dim3 grid((unsigned)nblocks), block(256);
 #define IRL_DRF16_LAUNCH(K, Z, BK, SP) \
-    hipLaunchKernelGGL((k_drf_bf16<K, Z, BK, SP>), grid, block, lds, stream, args, afr, afrLo)
+    hipLaunchKernelGGL((k_drf_bf16<K, Z, BK, SP>), grid, block, lds, stream, args, afr, afrLo, \
+                       distSplit)
 #define IRL_DRF16_CASE(K)                                                            \
     case K:                                                                          \
         if (NZT == 8) {                                                              \
