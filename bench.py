@@ -266,6 +266,7 @@ def main():
     if args.only_esse:
         we = build_workload(tmpdir, name="c3")
         line = workload_line(we, args.steps, args.warmup, "fp32")
+        line.pop("_estimate", None)
         print(json.dumps(line))
         return
     w = build_workload(tmpdir)

@@ -310,11 +310,6 @@ __global__ __launch_bounds__(256) void k_fft_strided(const float2 *src, float2 *
     }
 }
 
-// Y conv: lines along depth at (b, m, column c).  Forward FFT (rows y >= ny
-// are zero), then for every kernel t multiply by its spectrum, inverse FFT
-// and keep y < ny.  The batch index runs fastest over the logical blocks,
-// so the B blocks that read the same kernel columns run together on one XCD
-// and share them through its L2.
 // Logical block index of a launch whose blocks are dispatched round-robin
 // over the chip's 8 XCDs (each with its own L2): logically consecutive
 // blocks land on the same XCD, so blocks that share inputs share its L2.
@@ -327,11 +322,18 @@ __device__ __forceinline__ long xcd_block()
     return (xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per) + k;
 }
 
+// Y conv of one kernel term: lines along depth at (b, m, column c).
+// Forward FFT (rows y >= ny are zero), multiply by the term's spectrum,
+// inverse FFT, keep y < ny.  One launch per term keeps the kernel at one
+// line's worth of registers (the forward transform is recomputed per term
+// from the buffer instead of being held).  The batch index runs fastest over
+// the logical blocks, so the B blocks that read the same spectrum columns
+// run together on one XCD and share them through its L2.
 template <int P>
 __global__ __launch_bounds__(256) void k_fft_y_conv(const float2 *__restrict__ buf,
                                                     const float2 *__restrict__ khat,
-                                                    float2 *__restrict__ outk, int B, int Tt,
-                                                    int ny, int Pm, int Wc,
+                                                    float2 *__restrict__ outk, long outBStride,
+                                                    int B, int ny, int Pm, int Wc,
                                                     const float2 *__restrict__ tw)
 {
     using F = FftLine<P>;
@@ -349,31 +351,26 @@ __global__ __launch_bounds__(256) void k_fft_y_conv(const float2 *__restrict__ b
     const long ystride = (long)Pm * Wc;
     const long col = (long)m * Wc + c;
     const float2 *src = buf + (long)b * ny * ystride + col;
-    float2 a[16], spec[16];
+    float2 a[16];
 #pragma unroll
     for (int w = 0; w < 16; ++w) {
         const int n = tt + F::T * w;
         a[w] = (act && n < ny) ? src[n * ystride] : make_float2(0.0f, 0.0f);
     }
     fft_line<P, false>(a, sm + li * F::LS, tt, tw);
+    const float2 *kt = khat + col;
 #pragma unroll
-    for (int w = 0; w < 16; ++w) spec[w] = a[w];
-    for (int t = 0; t < Tt; ++t) {
-        const float2 *kt = khat + (long)t * P * ystride + col;
+    for (int w = 0; w < 16; ++w) {
+        const int n = tt + F::T * w;
+        a[w] = act ? cmul(a[w], kt[n * ystride]) : make_float2(0.0f, 0.0f);
+    }
+    fft_line<P, true>(a, sm + li * F::LS, tt, tw);
+    if (!act) return;
+    float2 *dst = outk + (long)b * outBStride + col;
 #pragma unroll
-        for (int w = 0; w < 16; ++w) {
-            const int n = tt + F::T * w;
-            a[w] = act ? cmul(spec[w], kt[n * ystride]) : make_float2(0.0f, 0.0f);
-        }
-        fft_line<P, true>(a, sm + li * F::LS, tt, tw);
-        if (act) {
-            float2 *dst = outk + ((long)b * Tt + t) * ny * ystride + col;
-#pragma unroll
-            for (int w = 0; w < 16; ++w) {
-                const int n = tt + F::T * w;
-                if (n < ny) dst[n * ystride] = a[w];
-            }
-        }
+    for (int w = 0; w < 16; ++w) {
+        const int n = tt + F::T * w;
+        if (n < ny) dst[n * ystride] = a[w];
     }
 }
 
@@ -677,10 +674,13 @@ void EsseDev::convolve(const float *act, int B, float *terms)
     const long plane = (long)Pm * Wc;
     fft_f_fwd(src, buf_, (long)B * ny * nm, nm, nf);
     fft_strided(Pm, false, buf_, buf_, (long)B * ny, plane, Wc, Wc, nm, Pm, twm_);
-    {
+    for (int t = 0; t < T; ++t) {
         const dim3 grid((unsigned)((long)B * cdiv(Wc, lines_per_block(Py)) * Pm));
+        const float2 *kh = khat_ + (long)Py * plane * t;
+        float2 *ok = outk_ + (long)t * ny * plane;
+        const long obs = (long)T * ny * plane;
 #define L_(PP) hipLaunchKernelGGL(k_fft_y_conv<PP>, grid, dim3(256), fft_lds(PP), stream_, buf_, \
-                                  khat_, outk_, B, T, ny, Pm, Wc, twy_)
+                                  kh, ok, obs, B, ny, Pm, Wc, twy_)
         ESSE_P_DISPATCH(Py, L_);
 #undef L_
     }
