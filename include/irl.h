@@ -13,6 +13,7 @@
  *   PrjView_t             replaces irl.h:127-143
  *   IrlOsem               replaces irl.h:145-213 (irlosem.c:22-285)
  *   IrlGenprj             replaces irl.h:215-288 (irlgenprj.c:23-309)
+ *   ComputeOrbit          replaces irl.h:291-316 (computeorbit.c:21-150)
  *   pchIrlErrorString     replaces irl.h:318     (irlosem.c:287-290)
  *
  * Volumes are x-fastest, then y, then z (slice) ("external" order); projection
@@ -87,6 +88,17 @@ int IrlGenprj(IrlParms_t *psIrlParms, Options_t *psOptions, PrjView_t *psPrjView
               float *pfScatterEstimate, float *pfActImage, float *pfAtnMap,
               float *pfPrjImage, float fPrimaryFac, char *pchLogFname,
               char *pchMessageFname);
+
+/* Camera-to-centre distances of an orbit from a 2-D image (irl.h:291-316,
+ * computeorbit.c:21-150).  OrbitType 0 sets every CFCR to fMargin; 1 moves
+ * each view to the thresholded object's edge plus the margin (pfMargins[v]
+ * when given); 2 additionally ties views v and v+NumViews/2 to the larger
+ * radius (L-mode).  fThresh > 0 is a fraction of the image maximum, <= 0 an
+ * absolute threshold of -fThresh; pixels at or below it are zeroed in
+ * pfPixels, which is modified.  Returns 0, 1 (allocation error) or 2
+ * (orbit type not 0, 1 or 2). */
+int ComputeOrbit(float *pfPixels, IrlParms_t *psParms, float fMargin, float *pfMargins,
+                 float fThresh, int OrbitType, PrjView_t *psViews);
 
 char *pchIrlErrorString(void);
 

@@ -14,6 +14,7 @@
  *   FreePrj                replaces irlprivate.h:517     (prjbck.c:432-453)
  *   rot3dpar               replaces irlprivate.h:521-529 (rot3dpar.c:19-198)
  *   vShearsRotate          replaces irlprivate.h:255-262 (fastrotate.c:330-563)
+ *   gendrf                 replaces irlprivate.h:272-285 (gendrf.c:221-427)
  *   ConvolveDrfYpad        replaces irlprivate.h:233-242 (convolve_ypad.c:22-96)
  *   sReadDrfTab            replaces drftabio.h:37-41     (drftabio.c:150-275)
  *   vWriteDrfTab           replaces drftabio.h:30-35     (drftabio.c:57-148)
@@ -90,6 +91,16 @@ void rot3dpar(float fAngle, float *OldMatrix, float *NewMatrix, int PB_FLAG,
  * that the reference sets up in vInitFastRotate are computed per call. */
 void vShearsRotate(float fAngle, float *pfOldMatrix, float *pfRotMatrix, int PB_FLAG,
                    int NumSlices, int NumPixels);
+
+/* Table of NumDists computed parallel-hole GRFs (round holes of diameter
+ * fHoleDiam, length fColThick, gap fGap to the image plane, Gaussian
+ * intrinsic blur fIntrinsic FWHM, sampled on fBinSize pixels) for distances
+ * minDist + k*fDrfSpacing; each truncated to the smallest centred square
+ * holding 1 - fMaxErr of its area (fMaxErr <= 0: NOffs) and renormalised.
+ * DrfTab[k].Psf is malloc'ed.  Only Type 0 (PARALLEL) is provided. */
+void gendrf(int Type, float Rfan, float fColThick, float fHoleDiam, float fGap, float fIntrinsic,
+            float fMaxErr, float fBinSize, float fDrfSpacing, float minDist, int NumDists,
+            int NOffs, Drf_t *DrfTab);
 
 void ConvolveDrfYpad(float *pfIn, int inX, int inY, Drf_t *psDrf, int bFFTConvolve,
                      int iYpad, int iYavg, float *pfOut);

@@ -193,6 +193,11 @@ def load():
     lib.ConvolveDrfYpad.argtypes = [_FP, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Drf)] + \
         [ctypes.c_int] * 3 + [_FP]
     lib.irlamd_last_error.restype = ctypes.c_int
+    lib.gendrf.argtypes = ([ctypes.c_int] + [ctypes.c_float] * 9 + [ctypes.c_int] * 2 +
+                           [ctypes.c_void_p])
+    lib.ComputeOrbit.argtypes = [_FP, ctypes.POINTER(IrlParms), ctypes.c_float, ctypes.c_void_p,
+                                 ctypes.c_float, ctypes.c_int, ctypes.c_void_p]
+    lib.ComputeOrbit.restype = ctypes.c_int
     lib.irlamd_set_precision.argtypes = [ctypes.c_int]
     lib.irlamd_set_precision.restype = ctypes.c_int
     _lib = lib
@@ -328,6 +333,39 @@ def set_precision(mode):
     if old < 0:
         raise ValueError("invalid precision mode %r" % (mode,))
     return old
+
+
+def gendrf(col_thick, hole_diam, gap, intrinsic, max_err, bin_size, spacing, min_dist, ndist,
+           noffs):
+    """Computed parallel-hole GRF table (gendrf, gendrf.c:221-427): a list of
+    (2*MaxOffsY-1, 2*MaxOffsX-1) float32 PSFs, one per distance."""
+    lib = load()
+    tab = (Drf * ndist)()
+    lib.gendrf(0, 0.0, col_thick, hole_diam, gap, intrinsic, max_err, bin_size, spacing,
+               min_dist, ndist, noffs, ctypes.addressof(tab))
+    if ndist and not tab[0].Psf:
+        raise RuntimeError("gendrf failed: " + error_string())
+    libc = ctypes.CDLL(None)
+    libc.free.argtypes = [ctypes.c_void_p]
+    out = []
+    for d in tab:
+        h, w = 2 * d.MaxOffsY - 1, 2 * d.MaxOffsX - 1
+        out.append(np.ctypeslib.as_array(d.Psf, shape=(h * w,)).reshape(h, w).copy())
+        libc.free(ctypes.cast(d.Psf, ctypes.c_void_p))
+    return out
+
+
+def compute_orbit(img, parms, margin, views, thresh, orbit_type, margins=None):
+    """ComputeOrbit (irl.h:291-316): fills views[v].CFCR; img (N x N float32,
+    contiguous) is thresholded in place.  Returns the C return code."""
+    vv = (PrjView * len(views))(*views)
+    m = None if margins is None else np.ascontiguousarray(margins, np.float32)
+    rc = load().ComputeOrbit(fptr(img), ctypes.byref(parms), margin,
+                             None if m is None else m.ctypes.data, thresh, orbit_type,
+                             ctypes.addressof(vv))
+    for k in range(len(views)):
+        views[k].CFCR = vv[k].CFCR
+    return rc
 
 
 def set_adv_options(fast_rotate=1, start_iteration=0):

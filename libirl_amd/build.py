@@ -15,7 +15,8 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
 HIP_SOURCES = ["engine.hip", "esse.hip"]
-CPP_SOURCES = ["host_util.cpp", "irlosem.cpp", "prjbck_api.cpp", "abi_check.cpp", "esse_host.cpp"]
+CPP_SOURCES = ["host_util.cpp", "irlosem.cpp", "prjbck_api.cpp", "abi_check.cpp", "esse_host.cpp",
+               "orbit.cpp", "grf.cpp"]
 
 # -ffp-contract=off: every float multiply/add in the geometry and
 # interpolation code is rounded separately, as in the reference; the

@@ -95,6 +95,8 @@ public:
         PaddedDrf drf;                     // empty when DRF not modelled
         std::vector<int> rowKernel;        // V*NRot: DRF index per row, -1 = none
         std::vector<int> rowPad;           // V*NRot: axial pad rows per row
+        std::vector<int> rowKernelBck;     // backprojector's, when it has its own table
+        std::vector<int> rowPadBck;        //   (empty: the projector's)
         int axialAvg = 0;
         std::vector<uint8_t> colMeasOne;   // V*NRot (norm mask)
         std::vector<uint8_t> colKeep;      // V*NRot (ratio mask)
@@ -197,5 +199,7 @@ void dev_convolve_plane(const float *in, int inX, int inY, const PaddedDrf &d, i
 void dev_rot3dpar(float angle, const float *in_host, float *out_host, int pbFlag,
                   int Nz, int N, int NRot);
 void dev_shears_rotate(float angle, const float *in_host, float *out_host, int Nz, int N);
+void dev_rot2d_views(const float *img_host, int N, int NRot, const float *angles, int V,
+                     float *out_host);
 
 }  // namespace irlamd

@@ -1038,6 +1038,8 @@ struct Recon::Impl {
     DBuf<uint4> afrHi, afrLo;
     DBuf<int> distSplit;          // per table entry: hi+lo split
     DBuf<int> rowKernel16;        // rowKernel with -1 replaced by the delta entry nDist
+    DBuf<int> rowKernel16B;       // the same for the backprojector's rows
+    DBuf<int> rowKernelB, rowPadB;   // backprojector's table rows (its own table, or copies)
     bool bf16 = false, bf16Split = false;   // bf16Split: some entry of the table is split
     int nSplit = 0, nUsed = 0;              // split / used table entries
     // three-shear rotation tables, slot 2v (forward) and 2v+1 (back)
@@ -1176,8 +1178,11 @@ struct Recon::Impl {
         d.nrows = g.NRot;
         d.KX = c.drf.nDist > 0 ? c.drf.KX : 1;
         d.psf = psf.p;
-        d.rowKernel = (model & 2) ? (bf16 ? rowKernel16.p : rowKernel.p) : rowNone.p;
-        d.rowPad = rowPad.p;
+        if (bck)
+            d.rowKernel = (model & 2) ? (bf16 ? rowKernel16B.p : rowKernelB.p) : rowNone.p;
+        else
+            d.rowKernel = (model & 2) ? (bf16 ? rowKernel16.p : rowKernel.p) : rowNone.p;
+        d.rowPad = bck ? rowPadB.p : rowPad.p;
         d.avg = c.axialAvg;
         d.bview = bview;
         d.zs = 1;
@@ -1386,11 +1391,15 @@ Recon::Recon(const Config &cfg) : impl_(new Impl), cfg_(cfg)
             std::vector<uint4> hi, lo;
             std::vector<double> est;
             build_bf16_fragments(d, KYb, hi, lo, est);
-            std::vector<int> split(nd + 1, 0), used(nd + 1, 0), rk16 = cfg.rowKernel;
+            std::vector<int> split(nd + 1, 0), used(nd + 1, 0), rk16 = cfg.rowKernel,
+                                               rk16B = cfg.rowKernelBck;
             if (rk16.empty()) rk16.assign((size_t)g.V * g.NRot, -1);
-            for (auto &r : rk16)
-                if (r < 0) r = nd;
-            for (int r : rk16) used[r] = 1;
+            if (rk16B.empty()) rk16B = rk16;
+            for (auto *v : {&rk16, &rk16B})
+                for (auto &r : *v) {
+                    if (r < 0) r = nd;
+                    used[r] = 1;
+                }
             I.nSplit = I.nUsed = 0;
             for (int k = 0; k <= nd; ++k) {
                 split[k] = k == nd || cfg.precision == 2 || (cfg.precision == 1 && est[k] > 1e-3);
@@ -1401,6 +1410,8 @@ Recon::Recon(const Config &cfg) : impl_(new Impl), cfg_(cfg)
             I.bf16Split = I.nSplit > 0;
             I.rowKernel16.alloc(rk16.size());
             I.rowKernel16.upload(rk16.data(), rk16.size(), I.stream);
+            I.rowKernel16B.alloc(rk16B.size());
+            I.rowKernel16B.upload(rk16B.data(), rk16B.size(), I.stream);
             I.distSplit.alloc(split.size());
             I.distSplit.upload(split.data(), split.size(), I.stream);
             I.afrHi.alloc(hi.size());
@@ -1417,6 +1428,14 @@ Recon::Recon(const Config &cfg) : impl_(new Impl), cfg_(cfg)
     I.rowKernel.upload(rk.data(), rk.size(), I.stream);
     I.rowPad.alloc(rp.size());
     I.rowPad.upload(rp.data(), rp.size(), I.stream);
+    {
+        const std::vector<int> &rkB = cfg.rowKernelBck.empty() ? rk : cfg.rowKernelBck;
+        const std::vector<int> &rpB = cfg.rowPadBck.empty() ? rp : cfg.rowPadBck;
+        I.rowKernelB.alloc(rkB.size());
+        I.rowKernelB.upload(rkB.data(), rkB.size(), I.stream);
+        I.rowPadB.alloc(rpB.size());
+        I.rowPadB.upload(rpB.data(), rpB.size(), I.stream);
+    }
     std::vector<int> none((size_t)g.V * g.NRot, -1);
     I.rowNone.alloc(none.size());
     I.rowNone.upload(none.data(), none.size(), I.stream);
@@ -2257,6 +2276,40 @@ void dev_rot3dpar(float angle, const float *in_host, float *out_host, int pbFlag
     HIP_CHECK(hipGetLastError());
     dout.download(out_host, nout, st);
     HIP_CHECK(hipStreamSynchronize(st));
+    HIP_CHECK(hipStreamDestroy(st));
+}
+
+// rot3dpar of one N x N image (PB_FLAG 0, one slice) into NRot x NRot for
+// each of V angles (ComputeOrbit, computeorbit.c:97-110); out is [V][NRot][NRot].
+void dev_rot2d_views(const float *img_host, int N, int NRot, const float *angles, int V,
+                     float *out_host)
+{
+    std::vector<int> head, npix;
+    circle_rows(NRot, head, npix);
+    hipStream_t st;
+    HIP_CHECK(hipStreamCreate(&st));
+    {
+        DBuf<float> din, dout;
+        DBuf<int> dh, dn;
+        const long nin = (long)N * N, nout = (long)NRot * NRot;
+        din.alloc(nin);
+        dout.alloc(nout * V);
+        dh.alloc(NRot);
+        dn.alloc(NRot);
+        din.upload(img_host, nin, st);
+        dh.upload(head.data(), NRot, st);
+        dn.upload(npix.data(), NRot, st);
+        const float NewAxis = 0.5 * (NRot - 1), OldAxis = 0.5 * (N - 1);
+        for (int v = 0; v < V; ++v) {
+            const float sn = (float)sin(angles[v]), cs = (float)cos(angles[v]);
+            hipLaunchKernelGGL(k_rot3dpar, dim3(div_up(nout, 256)), dim3(256), 0, st, din.p,
+                               dout.p + nout * v, 1, N, NRot, sn, cs, NewAxis, OldAxis, dh.p,
+                               dn.p);
+            HIP_CHECK(hipGetLastError());
+        }
+        dout.download(out_host, nout * V, st);
+        HIP_CHECK(hipStreamSynchronize(st));
+    }
     HIP_CHECK(hipStreamDestroy(st));
 }
 

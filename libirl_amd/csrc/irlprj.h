@@ -31,6 +31,13 @@ int drf_row_index(const DrfTab_t *t, float fCFCR, float fPixelWidth, int iNumRot
 PaddedDrf pad_drf_table(const DrfTab_t *t);
 void view_trig(float angle, ViewTab &vt);
 
+// Computed parallel-hole GRF tables (gendrf.c:221-427, IrlSetup.c:243-334).
+void gen_grf_table(float fColThick, float fHoleDiam, float fGap, float fIntrinsic, float fMaxErr,
+                   float fBinSize, float fDrfSpacing, float minDist, int NumDists, int NOffs,
+                   Drf_t *tab);
+void compute_grf_tab(DrfTab_t *t, float fPixelWidth, float fBinWidth, int NRot, float fMinDist,
+                     float fMaxDist);
+
 // DRFT / KRNL file readers (drftabio.c:150-275, krnlio.c:196-331).
 DrfTab_t read_drf_tab(const char *fname, std::string *title, std::string *col);
 void free_drf_tab_contents(DrfTab_t *t);

@@ -23,6 +23,7 @@
  *   rot3dpar.c:19-198            orc_rot3dpar
  *   fastrotate.c:330-563, 573-589
  *                                orc_shears_rotate (3-pass shear rotation)
+ *   computeorbit.c:21-150        orc_compute_orbit
  *   conv2d_ypad.c:39-108,
  *   pad2d.c:21-141               orc_conv2d_ypad
  *   convolve_ypad.c:22-96        orc_convolve_drf (spatial path; the FFT path
@@ -959,7 +960,9 @@ typedef struct {
 
 typedef void (*orc_iter_cb)(int iter, const REAL *recon_internal, void *ctx);
 
-/* osemrecon.c:557-897 with the in-memory norm images.  Arrays are in the
+/* osemrecon.c:557-897 with the in-memory norm images.  bckdrf, when not
+ * NULL, is the backprojector's own table (prjbck.c:308-310; a computed GRF
+ * with bFFTconvolve off, IrlSetup.c:559-569).  Arrays are in the
  * internal [y][z][x] order; prj/scat are [V][Nz][NRot].  pfAtnMap may be
  * NULL (no attenuation) and pfScatterEstimate may be NULL.  When cb is
  * non-NULL it is called after every iteration with the internal-order
@@ -970,7 +973,8 @@ int orc_osem(int Nz, int N, int V, int NumAngPerSubset, int NumIterations,
              const void *srf, int iNumSrfIterations, int iAxialPadLength,
              int iAxialAvgLength, const REAL *pfPrjImage,
              const REAL *pfScatterEstimate, const REAL *pfAtnMap,
-             REAL *pfReconImage, orc_iter_cb cb, void *ctx, int bFastRotate)
+             REAL *pfReconImage, orc_iter_cb cb, void *ctx, int bFastRotate,
+             const orc_drftab *bckdrf)
 {
     int NRot = N, S, iSubset, iMember, iAngle, iIter;
     long M = (long)N * N * Nz, P = (long)NRot * Nz, i;
@@ -1003,7 +1007,8 @@ int orc_osem(int Nz, int N, int V, int NumAngPerSubset, int NumIterations,
                                 views[iAngle].left, views[iAngle].right);
             for (i = 0; i < P; ++i) calc[i] *= views[iAngle].viewtime;
             orc_backproject(Nz, N, NRot, iBckModel, fPixelWidth, views[iAngle].angle,
-                            views[iAngle].cfcr, drf, bFFT, srf, pfAtnMap, calc, tmp,
+                            views[iAngle].cfcr, bckdrf ? bckdrf : drf, bckdrf ? 0 : bFFT, srf,
+                            pfAtnMap, calc, tmp,
                             1.0f, 1.0f, iAxialPadLength, iAxialAvgLength, 0, bFastRotate);
         }
         for (i = 0; i < M; ++i) tmp[i] = tmp[i] < 1e-6 ? 0.0 : 1.0 / tmp[i];
@@ -1052,7 +1057,8 @@ int orc_osem(int Nz, int N, int V, int NumAngPerSubset, int NumIterations,
                     calc[i] = calc[i] > 1e-20 ? pfPrjImage[P * iAngle + i] / calc[i] : 0.0;
                 for (i = 0; i < P; ++i) calc[i] *= vw->viewtime;
                 orc_backproject(Nz, N, NRot, iBckModel, fPixelWidth, vw->angle, vw->cfcr,
-                                drf, bFFT, srf, pfAtnMap, calc, tmp, 1.0f, 1.0f,
+                                bckdrf ? bckdrf : drf, bckdrf ? 0 : bFFT, srf, pfAtnMap, calc,
+                                tmp, 1.0f, 1.0f,
                                 iAxialPadLength, iAxialAvgLength, 1, bFastRotate);
             }
             memcpy(normImg, norm + M * iSubset, sizeof(REAL) * M);
@@ -1062,6 +1068,54 @@ int orc_osem(int Nz, int N, int V, int NumAngPerSubset, int NumIterations,
     }
     orc_free_prj();
     free(sub); free(norm); free(tmp); free(normImg); free(calc); free(prjscat);
+    return 0;
+}
+
+/* computeorbit.c:21-150 on a float N x N image (modified in place by the
+ * threshold); cfcr[v] receives the radius of view v, angles[v] its angle. */
+int orc_compute_orbit(float *img, int N, int V, float pix, float fMargin, const float *margins,
+                      float fThresh, int type, const float *angles, float *cfcr)
+{
+    int v, i, row, NRot;
+    float thr = 0.0f, mx, sum, r;
+    REAL *in, *out;
+    if (type == 0) {
+        for (v = 0; v < V; v++) cfcr[v] = fMargin;
+        return 0;
+    }
+    if (type > 2 || type < 0) return 2;
+    if (fThresh <= 0.0) {
+        thr = -fThresh;
+    } else {
+        mx = img[0];
+        for (i = 1; i < N * N; i++) mx = img[i] > mx ? img[i] : mx;
+        thr = mx * fThresh;
+    }
+    for (i = 0; i < N * N; i++)
+        if (img[i] <= thr) img[i] = 0.0f;
+    NRot = (int)ceil(sqrt(2.0) * (double)N);
+    if ((NRot % 2) != (N % 2)) NRot++;
+    in = (REAL *)malloc(sizeof(REAL) * N * N);
+    out = (REAL *)malloc(sizeof(REAL) * NRot * NRot);
+    for (i = 0; i < N * N; i++) in[i] = img[i];
+    for (v = 0; v < V; v++) {
+        orc_rot3dpar((float)(0.5 * M_PI + angles[v]), in, out, 0, 1, N, NRot);
+        sum = 0.0f;
+        for (row = 0; row < NRot / 2 && sum <= 0.0f; row++) {
+            sum = 0.0f;
+            for (i = 0; i < NRot; i++) sum += out[(long)row * NRot + i];
+        }
+        r = ((float)NRot / 2.0 - (float)row) * pix;
+        r += margins ? margins[v] : fMargin;
+        cfcr[v] = r;
+    }
+    if (type == 2)
+        for (v = 0; v < V / 2; v++) {
+            if (cfcr[v] > cfcr[v + V / 2]) cfcr[v + V / 2] = cfcr[v];
+            else cfcr[v] = cfcr[v + V / 2];
+        }
+    free(in);
+    free(out);
     return 0;
 }
 

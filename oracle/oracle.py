@@ -50,13 +50,16 @@ def lib(real="f64"):
     L.orc_shears_extents.argtypes = [ctypes.c_int, vp, vp]
     L.orc_subsets.argtypes = [ctypes.c_int, ctypes.c_int, vp]
     L.orc_osem.argtypes = ([ctypes.c_int] * 8 + [ctypes.c_float] + [vp, vp, ctypes.c_int, vp] +
-                           [ctypes.c_int] * 3 + [vp] * 4 + [vp, vp, ctypes.c_int])
+                           [ctypes.c_int] * 3 + [vp] * 4 + [vp, vp, ctypes.c_int, vp])
     L.orc_esse_init.restype = vp
     L.orc_esse_init.argtypes = ([vp, vp, ctypes.c_int, ctypes.c_int] + [ctypes.c_float] * 5 +
                                 [ctypes.c_int] * 4 + [ctypes.c_float] * 2)
     L.orc_esse_free.argtypes = [vp]
     L.orc_esse_effsrc.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
     L.orc_reorder.argtypes = [ctypes.c_int] * 3 + [vp, vp]
+    L.orc_compute_orbit.argtypes = ([vp, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                     ctypes.c_float, vp, ctypes.c_float, ctypes.c_int, vp, vp])
+    L.orc_compute_orbit.restype = ctypes.c_int
     L.orc_real_size.restype = ctypes.c_int
     _libs[real] = L
     return L
@@ -173,6 +176,18 @@ def subsets(nviews, per):
     return out.reshape(s, per)
 
 
+def compute_orbit(img, pix, margin, angles, thresh, otype, margins=None, real="f32"):
+    """ComputeOrbit (computeorbit.c:21-150); img (N x N float32) is
+    thresholded in place.  Returns (rc, cfcr array)."""
+    assert img.dtype == np.float32 and img.flags.c_contiguous
+    angles = np.ascontiguousarray(angles, np.float32)
+    out = np.zeros(len(angles), np.float32)
+    m = None if margins is None else np.ascontiguousarray(margins, np.float32)
+    rc = lib(real).orc_compute_orbit(img.ctypes.data, img.shape[0], len(angles), pix, margin,
+                                     _p(m), thresh, otype, angles.ctypes.data, out.ctypes.data)
+    return rc, out
+
+
 def reorder(d0, d1, d2, a, real="f64"):
     a = _arr(a, real)
     out = np.empty((d1, d0, d2), _dt(real))
@@ -214,7 +229,7 @@ ITER_CB = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p)
 
 def osem(nz, n, views, per_subset, iterations, prj_model, bck_model, pw, drf, prj, atn, x0,
          scat=None, start_iteration=0, pad=0, avg=0, real="f64", iter_cb=None, esse=None,
-         srf_iterations=0, fast=0):
+         srf_iterations=0, fast=0, bck_drf=None):
     """vOsemRecon on internal-order inputs (atn already scaled to 1/pixel and
     clamped, prj clamped, x0 internal order).  views: list of dicts with
     angle, left, right, cfcr, viewtime, sensitivity.  Returns the estimate."""
@@ -238,7 +253,8 @@ def osem(nz, n, views, per_subset, iterations, prj_model, bck_model, pw, drf, pr
                     pw, ctypes.addressof(vv), drf.handle(real) if drf else None,
                     int(drf.fft) if drf else 0, esse, srf_iterations, pad, avg,
                     prj.ctypes.data, _p(scat), _p(atn), x.ctypes.data,
-                    ctypes.cast(cb, ctypes.c_void_p), None, int(fast))
+                    ctypes.cast(cb, ctypes.c_void_p), None, int(fast),
+                    bck_drf.handle(real) if bck_drf else None)
     if rc != 0:
         raise ValueError("orc_osem failed with %d" % rc)
     return x

@@ -78,3 +78,61 @@ def test_osem_bf16_nrmse(mode, tol):
     e = rel_l2(recon, ref)
     print("OSEM rel L2 (NRMSE) vs FP64 oracle: %.3g" % e)
     assert e <= tol
+
+
+@pytest.mark.parametrize("otype", [1, 2])
+def test_compute_orbit_matches_oracle(otype):
+    print("ComputeOrbit type %d" % otype, flush=True)
+    import libirl_amd as L
+    from libirl_amd import IrlParms, synth
+    n = 64
+    act, _ = synth.phantom(n, 8, 3)
+    img = np.ascontiguousarray(act[4])
+    views = synth.views_circular(30, 20.0, n)
+    p = IrlParms()
+    p.NumPixels, p.NumSlices, p.NumViews, p.BinWidth = n, 8, 30, 0.4
+    g_img = img.copy()
+    rc = L.compute_orbit(g_img, p, 2.5, views, 0.2, otype)
+    assert rc == 0, L.error_string()
+    o_img = img.copy()
+    orc, ref = O.compute_orbit(o_img, 0.4, 2.5, [v.Angle for v in views], 0.2, otype)
+    assert orc == 0
+    assert np.array_equal(np.array([v.CFCR for v in views], np.float32), ref)
+    assert np.array_equal(g_img, o_img)
+
+
+@pytest.mark.parametrize("grf_in_bck", [0, 1])
+def test_irlosem_with_computed_grf(grf_in_bck):
+    """IrlOsem without a DRF table file (computed GRF, IrlSetup.c:243-334),
+    or with a file plus the GRF in the backprojector (bUseGrfInBck), against
+    the oracle run with the same tables."""
+    print("IrlOsem computed GRF, GRF in backprojector=%d" % grf_in_bck, flush=True)
+    import libirl_amd as L
+    pb = Problem(n=32, nz=16, nviews=16, per=4, iters=2, pix=0.4)
+    parms = pb.parms()
+    parms.fHoleDiam, parms.fHoleLen, parms.fBackToDet, parms.fIntrinsicFWHM = 0.15, 2.4, 0.5, 0.4
+    # the table vComputeDrfs builds: distances of the orbit, half size NRot/2
+    cf = [v.CFCR for v in pb.views]
+    mn = min(cf) - pb.n // 2 * pb.pix + pb.pix / 2
+    mn = pb.pix / 2 if mn < 0 else mn
+    mx = max(cf) + pb.n // 2 * pb.pix - pb.pix / 2
+    nd = int((np.float32(mx) - np.float32(mn)) / np.float32(pb.pix) + 1 + 0.5)
+    psfs = L.gendrf(2.4, 0.15, 0.5, 0.4, 0.05, pb.pix, pb.pix, mn, nd, pb.n // 2)
+    grf = O.DrfTable(mn, pb.pix, [((p.shape[1] + 1) // 2, (p.shape[0] + 1) // 2) for p in psfs],
+                     psfs)
+    x0 = O.reorder(pb.nz, pb.n, pb.n, pb.x0)
+    prj_tab = pb.drf if grf_in_bck else grf
+    ref = O.osem(pb.nz, pb.n, pb.oracle_views(), pb.per, pb.iters, pb.prj_model, pb.prj_model,
+                 pb.pix, prj_tab, np.clip(pb.prj, 0, None), pb.atn_int, x0, real="f32",
+                 bck_drf=grf if grf_in_bck else None)
+    ref = O.reorder(pb.n, pb.nz, pb.n, ref.reshape(pb.n, pb.nz, pb.n))
+    opts = pb.options()
+    opts.bUseGrfInBck = grf_in_bck
+    atn, prj, recon = pb.mu.copy(), pb.prj.copy(), pb.x0.copy()
+    L.set_adv_options(0, 0)
+    rc = L.irl_osem(parms, opts, pb.views, pb.drf_file if grf_in_bck else None, None, None, atn,
+                    prj, recon)
+    assert rc == 0, L.error_string()
+    e = rel_l2(recon, ref)
+    print("rel L2 vs f32 oracle with the same tables: %.3g" % e)
+    assert e <= 1e-4
