@@ -1030,7 +1030,8 @@ struct Recon::Impl {
     long M = 0, P = 0, vol = 0;
     int nchunk = 1, rchunk = 1;
     DBuf<float> x, mu, prj, scat, u, ninv, afac, att, ppart, ratio, planes;
-    DBuf<float> rotx, rotmu, terms, ppart2, prjscat;  // scatter model
+    DBuf<float> rotx, rotmu, terms, ppart2, prjscat;  // scatter model (rotmu: every view)
+    DBuf<int> allViews;                                // 0..V-1
     std::unique_ptr<EsseDev> esse;
     DBuf<float> fsin, fcos, bsin, bcos, vt, vtsens, psf;
     DBuf<int> rowKernel, rowPad, subsets, head, npix, rowNone;
@@ -1126,10 +1127,9 @@ struct Recon::Impl {
     void run_srf_att(const float *afacp, int B, const int *bview)
     {
         launch("rot_x", [&] { run_att(nullptr, B, bview, x.p, rotx.p); });
-        launch("rot_mu", [&] { run_att(nullptr, B, bview, mu.p, rotmu.p); });
         launch("esse_conv", [&] { esse->convolve(rotx.p, B, terms.p); });
         launch("esse_source", [&] {
-            esse->source(rotx.p, rotmu.p, terms.p, B, 1.0f, afacp, bview, att.p);
+            esse->source(rotx.p, rotmu.p, terms.p, B, 1.0f, afacp, bview, att.p, true);
         });
     }
 
@@ -1333,7 +1333,7 @@ Recon::Recon(const Config &cfg) : impl_(new Impl), cfg_(cfg)
         if (!cfg.hasAtn)
             fail(ETYPE_ILLEGAL_VALUE, "scatter modelling needs an attenuation map");
         I.rotx.alloc(I.vol * Vs);
-        I.rotmu.alloc(I.vol * Vs);
+        I.rotmu.alloc(I.vol * g.V);   // rotated mu of every view, computed once
         if (cfg.esse && cfg.esse->enabled) {
             if (cfg.esse->nx != g.NRot || cfg.esse->ny != g.NRot || cfg.esse->nz != g.Nz)
                 fail(ETYPE_ILLEGAL_VALUE, "scatter model prepared for another volume size");
@@ -1441,6 +1441,13 @@ Recon::Recon(const Config &cfg) : impl_(new Impl), cfg_(cfg)
     I.rowNone.upload(none.data(), none.size(), I.stream);
     I.subsets.alloc(cfg.subsets.size());
     I.subsets.upload(cfg.subsets.data(), cfg.subsets.size(), I.stream);
+    {
+        std::vector<int> iota(g.V);
+        for (int v = 0; v < g.V; ++v) iota[v] = v;
+        I.allViews.alloc(g.V);
+        I.allViews.upload(iota.data(), g.V, I.stream);
+        HIP_CHECK(hipStreamSynchronize(I.stream));
+    }
     if (cfg.shears) {
         if (g.N != g.NRot)
             fail(ETYPE_ILLEGAL_VALUE, "the shear rotation needs NumRotPixs == NumPixels");
@@ -1623,6 +1630,8 @@ void Recon::precompute(bool norms)
                                    I.fcos.p, 1.0f, I.head.p, I.npix.p, I.shear());
         });
     }
+    if (I.rotmu.p)   // the scatter model's rotated mu, fixed for a view like afac
+        I.launch("rot_mu", [&] { I.run_att(nullptr, g.V, I.allViews.p, I.mu.p, I.rotmu.p); });
     // Sensitivity images: backproject mask*ViewTime for each subset.
     const int Vs = cfg_.Vs;
     const int KYb = cfg_.drf.nDist > 0 ? I.ky_bucket() : 7;
@@ -1751,10 +1760,10 @@ void Recon::forwardProject(float primary, float *out_host, bool srf)
         const int *bview = order.p + v0;
         if (srf && I.esse) {
             I.launch("rot_x", [&] { I.run_att(nullptr, B, bview, I.x.p, I.rotx.p); });
-            I.launch("rot_mu", [&] { I.run_att(nullptr, B, bview, I.mu.p, I.rotmu.p); });
             I.launch("esse_conv", [&] { I.esse->convolve(I.rotx.p, B, I.terms.p); });
             I.launch("esse_source", [&] {
-                I.esse->source(I.rotx.p, I.rotmu.p, I.terms.p, B, primary, afac, bview, I.att.p);
+                I.esse->source(I.rotx.p, I.rotmu.p, I.terms.p, B, primary, afac, bview, I.att.p,
+                               true);
             });
         } else {
             I.launch("att", [&] { I.run_att(afac, B, bview); });

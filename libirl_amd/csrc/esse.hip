@@ -484,11 +484,12 @@ __global__ void k_smooth(float *__restrict__ img, long BT, int ny, int nz, int n
 // 826-948; prjbck.c:170-173, 219-222).  Walks the depth rows with the
 // half-pixel running sum of mu, as the reference's three depth loops do.
 // Every operand has the same layout; only the depth stride nz*nx matters.
-__global__ void k_esse_source(const float *__restrict__ act, const float *__restrict__ map,
-                              const float *__restrict__ terms, int B, int T, int ny, int nz,
-                              int nx, float fAtnCnvFac, float fMu, float cntr, float fDensFac,
-                              float primary, const float *__restrict__ afac,
-                              const int *__restrict__ bview, float *__restrict__ out)
+template <int T>
+__global__ __launch_bounds__(256) void k_esse_source(
+    const float *__restrict__ act, const float *__restrict__ map, const float *__restrict__ terms,
+    int B, int ny, int nz, int nx, float fAtnCnvFac, float fMu, float cntr, float fDensFac,
+    float primary, const float *__restrict__ afac, const int *__restrict__ bview, int mapByView,
+    float *__restrict__ out)
 {
     const long cols = (long)nz * nx;
     const long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -497,27 +498,42 @@ __global__ void k_esse_source(const float *__restrict__ act, const float *__rest
     const long c = gid % cols;
     const long vol = cols * ny;
     const float *A = act + b * vol + c;
-    const float *Mp = map + b * vol + c;
+    const float *Mp = map + (mapByView ? (long)bview[b] : (long)b) * vol + c;
     const float *E = terms + (long)b * T * vol + c;
     const float *F = afac ? afac + (long)bview[b] * vol + c : nullptr;
     float *O = out + b * vol + c;
     float sum = 0.0f;
-    for (int y = 0; y < ny; ++y) {
-        const long o = (long)y * cols;
-        const float m = Mp[o];
-        sum = (float)((double)sum + 0.5 * (double)m);
-        const float d = sum * fAtnCnvFac;
-        float eff = E[o];
-        if (T > 1) eff = eff - d * E[vol + o];
-        if (T > 2) eff = (float)((double)eff + 0.5 * (double)d * (double)d * (double)E[2 * vol + o]);
-        eff = (float)((double)eff * exp((double)(d * fMu)));
-        sum = (float)((double)sum + 0.5 * (double)m);
-        const float a = A[o];
-        if (cntr > 0) eff = eff + a * cntr;
-        eff = eff * (m * fDensFac);
-        if (eff < 0) eff = 0.0f;
-        const float q = a * primary + eff;
-        O[o] = F ? q * F[o] : q;
+    // four depth rows per step: their loads are issued before the running
+    // sum, which is the only serial dependence, walks through them
+    constexpr int U = 4;
+    for (int y0 = 0; y0 < ny; y0 += U) {
+        float m[U], e0[U], e1[U], e2[U], a[U], f[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            const long o = (long)min(y0 + k, ny - 1) * cols;
+            m[k] = Mp[o];
+            e0[k] = E[o];
+            e1[k] = T > 1 ? E[vol + o] : 0.0f;
+            e2[k] = T > 2 ? E[2 * vol + o] : 0.0f;
+            a[k] = A[o];
+            f[k] = F ? F[o] : 1.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            if (y0 + k >= ny) break;
+            sum = (float)((double)sum + 0.5 * (double)m[k]);
+            const float d = sum * fAtnCnvFac;
+            float eff = e0[k];
+            if (T > 1) eff = eff - d * e1[k];
+            if (T > 2) eff = (float)((double)eff + 0.5 * (double)d * (double)d * (double)e2[k]);
+            eff = (float)((double)eff * exp((double)(d * fMu)));
+            sum = (float)((double)sum + 0.5 * (double)m[k]);
+            if (cntr > 0) eff = eff + a[k] * cntr;
+            eff = eff * (m[k] * fDensFac);
+            if (eff < 0) eff = 0.0f;
+            const float q = a[k] * primary + eff;
+            O[(long)(y0 + k) * cols] = F ? q * f[k] : q;
+        }
     }
 }
 
@@ -704,15 +720,25 @@ void EsseDev::convolve(const float *act, int B, float *terms)
 }
 
 void EsseDev::source(const float *act, const float *map, const float *terms, int B,
-                     float primary, const float *afac, const int *bview, float *out)
+                     float primary, const float *afac, const int *bview, float *out,
+                     bool mapByView)
 {
     const long cols = (long)m_.nz * m_.nx;
     const float fAtnCnvFac = 1.0 / (m_.mu0);
     const float fMu = m_.mu0 - m_.muavg;
     const float fDens = fAtnCnvFac / m_.pixsize;
-    hipLaunchKernelGGL(k_esse_source, dim3(cdiv(cols * B, 256)), dim3(256), 0, stream_, act, map,
-                       terms, B, m_.nterms, m_.ny, m_.nz, m_.nx, fAtnCnvFac, fMu, m_.cntrPixFac,
-                       fDens, primary, afac, bview, out);
+    const dim3 grid((unsigned)cdiv(cols * B, 256));
+#define L_(TT)                                                                                \
+    hipLaunchKernelGGL(k_esse_source<TT>, grid, dim3(256), 0, stream_, act, map, terms, B,    \
+                       m_.ny, m_.nz, m_.nx, fAtnCnvFac, fMu, m_.cntrPixFac, fDens, primary,   \
+                       afac, bview, mapByView ? 1 : 0, out)
+    if (m_.nterms == 1)
+        L_(1);
+    else if (m_.nterms == 2)
+        L_(2);
+    else
+        L_(3);
+#undef L_
     ESSE_CHECK(hipGetLastError());
 }
 

@@ -27,8 +27,9 @@ public:
     // prjbck.c:164-176): q = act*primary + eff(act, map), optionally times
     // afac (the attenuation factors of the view, [V] volumes).
     // Writes q * afac into out ([B] volumes); all in the same layout.
+    // mapByView: map holds the rotated maps of every view, indexed by bview.
     void source(const float *act, const float *map, const float *terms, int B, float primary,
-                const float *afac, const int *bview, float *out);
+                const float *afac, const int *bview, float *out, bool mapByView = false);
 
     const EsseModel &model() const { return m_; }
     int terms() const { return m_.nterms; }
