@@ -32,7 +32,7 @@ struct EsseModel {
     int nx = 0, ny = 0, nz = 0;        // bins, depth rows, slices (full resolution)
     int c = 1;                         // collapse factor
     int nxc = 0, nyc = 0, nzc = 0;     // collapsed sizes
-    int Px = 0, Py = 0, Pz = 0;        // FFT grid (powers of two)
+    int Px = 0, Py = 0, Pz = 0;        // FFT grid (line lengths the device FFT supports)
     int nterms = 2;
     float mu0 = 0, muavg = 0, pixsize = 0, cntrPixFac = 0;
     std::vector<float> grid[3];        // folded kernels on the FFT grid, [Py][Pz][Px]

@@ -12,9 +12,10 @@
 //   Y conv    : forward along depth, multiply by every kernel spectrum,
 //               inverse along depth, keep the cropped depth range
 //   M inverse, F inverse (complex-to-real): back to the cropped volume
-// Each line FFT is a radix-16 Stockham transform held in registers: 16
-// values per thread, P/16 threads per line, one LDS exchange per radix-16
-// pass (two passes and one exchange for P = 256).  Lines along strided axes
+// Each line FFT is a Stockham transform held in registers (FftPlan: R
+// values per thread, P/R threads per line, one LDS exchange between passes;
+// radix-16 passes for powers of two, radices 4, 4, 10 for P = 160 and 4, 4,
+// 12 for P = 192, the 10- and 12-point DFTs done directly).  Lines along strided axes
 // are grouped so that consecutive lanes take consecutive columns
 // (coalesced); lines along the fastest axis put consecutive lanes on
 // consecutive elements.
@@ -117,28 +118,102 @@ __device__ __forceinline__ void dft(float2 *x)
     if constexpr (r >= 16) dft_stage<r, 8, INV>(x);
 }
 
-// Line FFT of length P (power of two, 16..512): 16 values per thread, T =
-// P/16 threads per line.  Thread tt holds x[tt + T*w] in a[w] on entry and
-// X[tt + T*w] on exit.  Stockham passes of radix 16 (the last may be 2, 4
-// or 8) with stride S: unit (p, q), p < m = P/(S r), q < S, reads
-// x[q + S(p + m u)], applies the r-point DFT and the twiddle W_P^{v p S},
-// and writes y[q + S(r p + v)].  Between passes the line goes through LDS
-// with a skew of one element per 16 against bank conflicts.
+// cos and sin of 2*pi*n/60 (the radix-10 and radix-12 DFTs use n = 6k, 5k).
+__device__ constexpr float kCos60[60] = {1.000000000f, 0.994521895f, 0.978147601f, 0.951056516f, 0.913545458f, 0.866025404f, 0.809016994f, 0.743144825f, 0.669130606f, 0.587785252f, 0.500000000f, 0.406736643f, 0.309016994f, 0.207911691f, 0.104528463f, 0.000000000f, -0.104528463f, -0.207911691f, -0.309016994f, -0.406736643f, -0.500000000f, -0.587785252f, -0.669130606f, -0.743144825f, -0.809016994f, -0.866025404f, -0.913545458f, -0.951056516f, -0.978147601f, -0.994521895f, -1.000000000f, -0.994521895f, -0.978147601f, -0.951056516f, -0.913545458f, -0.866025404f, -0.809016994f, -0.743144825f, -0.669130606f, -0.587785252f, -0.500000000f, -0.406736643f, -0.309016994f, -0.207911691f, -0.104528463f, -0.000000000f, 0.104528463f, 0.207911691f, 0.309016994f, 0.406736643f, 0.500000000f, 0.587785252f, 0.669130606f, 0.743144825f, 0.809016994f, 0.866025404f, 0.913545458f, 0.951056516f, 0.978147601f, 0.994521895f};
+__device__ constexpr float kSin60[60] = {0.000000000f, 0.104528463f, 0.207911691f, 0.309016994f, 0.406736643f, 0.500000000f, 0.587785252f, 0.669130606f, 0.743144825f, 0.809016994f, 0.866025404f, 0.913545458f, 0.951056516f, 0.978147601f, 0.994521895f, 1.000000000f, 0.994521895f, 0.978147601f, 0.951056516f, 0.913545458f, 0.866025404f, 0.809016994f, 0.743144825f, 0.669130606f, 0.587785252f, 0.500000000f, 0.406736643f, 0.309016994f, 0.207911691f, 0.104528463f, 0.000000000f, -0.104528463f, -0.207911691f, -0.309016994f, -0.406736643f, -0.500000000f, -0.587785252f, -0.669130606f, -0.743144825f, -0.809016994f, -0.866025404f, -0.913545458f, -0.951056516f, -0.978147601f, -0.994521895f, -1.000000000f, -0.994521895f, -0.978147601f, -0.951056516f, -0.913545458f, -0.866025404f, -0.809016994f, -0.743144825f, -0.669130606f, -0.587785252f, -0.500000000f, -0.406736643f, -0.309016994f, -0.207911691f, -0.104528463f};
+
+// z * exp(-+2 pi i n/60) for a compile-time n.
+template <int N60, bool INV>
+__device__ __forceinline__ float2 rot60(float2 z)
+{
+    constexpr int n = N60 % 60;
+    if (n == 0) return z;
+    if (n == 30) return make_float2(-z.x, -z.y);
+    if (n == 15) return INV ? make_float2(-z.y, z.x) : make_float2(z.y, -z.x);
+    if (n == 45) return INV ? make_float2(z.y, -z.x) : make_float2(-z.y, z.x);
+    const float c = kCos60[n], sn = INV ? kSin60[n] : -kSin60[n];
+    return make_float2(fmaf(z.x, c, -z.y * sn), fmaf(z.x, sn, z.y * c));
+}
+
+// Direct r-point DFT (r dividing 60), natural order: y_v = sum_u x_u W_r^{uv}.
+template <int r, bool INV, int V, int U>
+__device__ __forceinline__ void dft_direct_term(const float2 *x, float2 &acc)
+{
+    if constexpr (U < r) {
+        const float2 t = rot60<(U * V % r) * (60 / r), INV>(x[U]);
+        acc = cadd(acc, t);
+        dft_direct_term<r, INV, V, U + 1>(x, acc);
+    }
+}
+
+template <int r, bool INV, int V>
+__device__ __forceinline__ void dft_direct_out(const float2 *x, float2 *y)
+{
+    if constexpr (V < r) {
+        float2 acc = x[0];
+        dft_direct_term<r, INV, V, 1>(x, acc);
+        y[V] = acc;
+        dft_direct_out<r, INV, V + 1>(x, y);
+    }
+}
+
+template <int r, bool INV>
+__device__ __forceinline__ void dft_any(float2 *x)
+{
+    if constexpr ((r & (r - 1)) == 0) {
+        dft<r, INV>(x);
+    } else {
+        float2 y[r];
+        dft_direct_out<r, INV, 0>(x, y);
+#pragma unroll
+        for (int i = 0; i < r; ++i) x[i] = y[i];
+    }
+}
+
+// Line FFT plans: P = T threads x R values per thread, Stockham passes of
+// the listed radices, each radix r dividing R and P/r divisible by T, so
+// every thread owns R/r whole units in every pass.
+template <int P>
+struct FftPlan;
+template <> struct FftPlan<16> { static constexpr int T = 1, R = 16, r0 = 16, r1 = 0, r2 = 0; };
+template <> struct FftPlan<32> { static constexpr int T = 2, R = 16, r0 = 16, r1 = 2, r2 = 0; };
+template <> struct FftPlan<64> { static constexpr int T = 4, R = 16, r0 = 16, r1 = 4, r2 = 0; };
+template <> struct FftPlan<128> { static constexpr int T = 8, R = 16, r0 = 16, r1 = 8, r2 = 0; };
+template <> struct FftPlan<160> { static constexpr int T = 8, R = 20, r0 = 4, r1 = 4, r2 = 10; };
+template <> struct FftPlan<192> { static constexpr int T = 16, R = 12, r0 = 4, r1 = 4, r2 = 12; };
+template <> struct FftPlan<256> { static constexpr int T = 16, R = 16, r0 = 16, r1 = 16, r2 = 0; };
+template <> struct FftPlan<512> { static constexpr int T = 32, R = 16, r0 = 16, r1 = 16, r2 = 2; };
+
+// Line FFT of length P: thread tt holds x[tt + T*w] in a[w] (w < R) on
+// entry and X[tt + T*w] on exit.  Pass with radix r and stride S: unit
+// (p, q), p < m = P/(S r), q < S, reads x[q + S(p + m u)], applies the
+// r-point DFT and the twiddle W_P^{v p S}, and writes y[q + S(r p + v)].
+// Between passes the line goes through LDS with a skew of one element per
+// 16 against bank conflicts.
 template <int P>
 struct FftLine {
-    static constexpr int R = 16;
-    static constexpr int T = P / R;
+    static constexpr int R = FftPlan<P>::R;
+    static constexpr int T = FftPlan<P>::T;
     static constexpr int LS = P + P / 16 + 1;  // LDS line stride (float2)
+    static_assert(T * R == P, "plan");
 };
 
 __device__ __forceinline__ int fft_skew(int i) { return i + (i >> 4); }
 
-template <int P, int S, bool INV>
+template <int P, int IDX>
+__host__ __device__ constexpr int fft_radix()
+{
+    return IDX == 0 ? FftPlan<P>::r0 : IDX == 1 ? FftPlan<P>::r1 : IDX == 2 ? FftPlan<P>::r2 : 0;
+}
+
+template <int P, int IDX, int S, bool INV>
 struct FftPass {
-    static constexpr int r = (P / S) >= 16 ? 16 : P / S;
+    static constexpr int r = fft_radix<P, IDX>();
     static constexpr int m = P / (S * r);
-    static constexpr int U = 16 / r;          // units per thread
-    static constexpr int T = P / 16;
+    static constexpr int R = FftPlan<P>::R;
+    static constexpr int U = R / r;           // units per thread
+    static constexpr int T = FftPlan<P>::T;
+    static_assert(U * r == R && (P / r) % T == 0, "plan");
     __device__ static int in_pos(int tt, int k, int u)
     {
         const int j = tt + T * k, p = j / S, q = j % S;
@@ -153,12 +228,12 @@ struct FftPass {
     {
 #pragma unroll
         for (int k = 0; k < U; ++k) {
-            dft<r, INV>(a + k * r);
+            dft_any<r, INV>(a + k * r);
             if (m > 1) {
                 const int p = (tt + T * k) / S;
 #pragma unroll
                 for (int v = 1; v < r; ++v) {
-                    float2 w = tw[(v * p * S) & (P - 1)];
+                    float2 w = tw[(v * p * S) % P];
                     if (INV) w.y = -w.y;
                     a[k * r + v] = cmul(a[k * r + v], w);
                 }
@@ -167,12 +242,24 @@ struct FftPass {
     }
 };
 
-template <int P, int S, bool INV>
+template <int P, int IDX, int S, bool INV>
 __device__ __forceinline__ void fft_passes(float2 *a, float2 *ldsl, int tt,
                                            const float2 *__restrict__ tw)
 {
-    using PS = FftPass<P, S, INV>;
-    if constexpr (S > 1) {
+    using PS = FftPass<P, IDX, S, INV>;
+    constexpr int R = PS::R;
+    if constexpr (IDX == 0) {
+        if constexpr (PS::U > 1) {
+            // unit order: a_u[k r + u] = x[tt + T(k + U u)] = a[k + U u]
+            float2 b[R];
+#pragma unroll
+            for (int k = 0; k < PS::U; ++k)
+#pragma unroll
+                for (int u = 0; u < PS::r; ++u) b[k * PS::r + u] = a[k + PS::U * u];
+#pragma unroll
+            for (int w = 0; w < R; ++w) a[w] = b[w];
+        }
+    } else {
 #pragma unroll
         for (int k = 0; k < PS::U; ++k)
 #pragma unroll
@@ -181,13 +268,13 @@ __device__ __forceinline__ void fft_passes(float2 *a, float2 *ldsl, int tt,
     PS::compute(a, tt, tw);
     if constexpr (PS::m == 1) {
         // final pass: a[k r + v] holds X[tt + T k + S v]; reorder to a[w] = X[tt + T w]
-        float2 b[16];
+        float2 b[R];
 #pragma unroll
         for (int k = 0; k < PS::U; ++k)
 #pragma unroll
             for (int v = 0; v < PS::r; ++v) b[k + PS::U * v] = a[k * PS::r + v];
 #pragma unroll
-        for (int w = 0; w < 16; ++w) a[w] = b[w];
+        for (int w = 0; w < R; ++w) a[w] = b[w];
     } else {
         __syncthreads();   // earlier readers of the LDS line are done
 #pragma unroll
@@ -195,7 +282,7 @@ __device__ __forceinline__ void fft_passes(float2 *a, float2 *ldsl, int tt,
 #pragma unroll
             for (int v = 0; v < PS::r; ++v) ldsl[fft_skew(PS::out_pos(tt, k, v))] = a[k * PS::r + v];
         __syncthreads();
-        fft_passes<P, S * PS::r, INV>(a, ldsl, tt, tw);
+        fft_passes<P, IDX + 1, S * PS::r, INV>(a, ldsl, tt, tw);
     }
 }
 
@@ -203,7 +290,7 @@ template <int P, bool INV>
 __device__ __forceinline__ void fft_line(float2 *a, float2 *ldsl, int tt,
                                          const float2 *__restrict__ tw)
 {
-    fft_passes<P, 1, INV>(a, ldsl, tt, tw);
+    fft_passes<P, 0, 1, INV>(a, ldsl, tt, tw);
 }
 
 // F forward: real lines r = (plane, m), m < nm, of nf values (zero to Pf)
@@ -220,9 +307,9 @@ __global__ __launch_bounds__(256) void k_fft_f_fwd(const float *__restrict__ in,
     const int li = threadIdx.x / F::T, tt = threadIdx.x % F::T;
     const long r = (long)blockIdx.x * LB + li;
     const bool act = r < rows;
-    float2 a[16];
+    float2 a[F::R];
 #pragma unroll
-    for (int w = 0; w < 16; ++w) {
+    for (int w = 0; w < F::R; ++w) {
         const int n = tt + F::T * w;
         a[w] = (act && n < nf) ? make_float2(in[r * nf + n], 0.0f) : make_float2(0.0f, 0.0f);
     }
@@ -231,7 +318,7 @@ __global__ __launch_bounds__(256) void k_fft_f_fwd(const float *__restrict__ in,
     const long plane = r / nm, m = r - plane * nm;
     float2 *dst = buf + (plane * Pm + m) * Wc;
 #pragma unroll
-    for (int w = 0; w < 16; ++w) {
+    for (int w = 0; w < F::R; ++w) {
         const int n = tt + F::T * w;
         if (n < Wc) dst[n] = a[w];
     }
@@ -254,9 +341,9 @@ __global__ __launch_bounds__(256) void k_fft_f_inv(const float2 *__restrict__ bu
     const bool act = r < rows;
     const long plane = act ? r / nm : 0, m = act ? r - plane * nm : 0;
     const float2 *src = buf + (plane * Pm + m) * Wc;
-    float2 a[16];
+    float2 a[F::R];
 #pragma unroll
-    for (int w = 0; w < 16; ++w) {
+    for (int w = 0; w < F::R; ++w) {
         const int n = tt + F::T * w;
         float2 v = make_float2(0.0f, 0.0f);
         if (act) {
@@ -272,7 +359,7 @@ __global__ __launch_bounds__(256) void k_fft_f_inv(const float2 *__restrict__ bu
     fft_line<P, true>(a, sm + li * F::LS, tt, tw);
     if (!act) return;
 #pragma unroll
-    for (int w = 0; w < 16; ++w) {
+    for (int w = 0; w < F::R; ++w) {
         const int n = tt + F::T * w;
         if (n < nf) out[r * nf + n] = a[w].x * scale;
     }
@@ -295,16 +382,16 @@ __global__ __launch_bounds__(256) void k_fft_strided(const float2 *src, float2 *
     const int c = (int)(blockIdx.x % ngroups) * LB + li;
     const bool act = c < W;
     const long off = bi * bstride + c;
-    float2 a[16];
+    float2 a[F::R];
 #pragma unroll
-    for (int w = 0; w < 16; ++w) {
+    for (int w = 0; w < F::R; ++w) {
         const int n = tt + F::T * w;
         a[w] = (act && n < nIn) ? src[off + n * nstride] : make_float2(0.0f, 0.0f);
     }
     fft_line<P, INV>(a, sm + li * F::LS, tt, tw);
     if (!act) return;
 #pragma unroll
-    for (int w = 0; w < 16; ++w) {
+    for (int w = 0; w < F::R; ++w) {
         const int n = tt + F::T * w;
         if (n < nOut) dst[off + n * nstride] = a[w];
     }
@@ -351,16 +438,16 @@ __global__ __launch_bounds__(256) void k_fft_y_conv(const float2 *__restrict__ b
     const long ystride = (long)Pm * Wc;
     const long col = (long)m * Wc + c;
     const float2 *src = buf + (long)b * ny * ystride + col;
-    float2 a[16];
+    float2 a[F::R];
 #pragma unroll
-    for (int w = 0; w < 16; ++w) {
+    for (int w = 0; w < F::R; ++w) {
         const int n = tt + F::T * w;
         a[w] = (act && n < ny) ? src[n * ystride] : make_float2(0.0f, 0.0f);
     }
     fft_line<P, false>(a, sm + li * F::LS, tt, tw);
     const float2 *kt = khat + col;
 #pragma unroll
-    for (int w = 0; w < 16; ++w) {
+    for (int w = 0; w < F::R; ++w) {
         const int n = tt + F::T * w;
         a[w] = act ? cmul(a[w], kt[n * ystride]) : make_float2(0.0f, 0.0f);
     }
@@ -368,7 +455,7 @@ __global__ __launch_bounds__(256) void k_fft_y_conv(const float2 *__restrict__ b
     if (!act) return;
     float2 *dst = outk + (long)b * outBStride + col;
 #pragma unroll
-    for (int w = 0; w < 16; ++w) {
+    for (int w = 0; w < F::R; ++w) {
         const int n = tt + F::T * w;
         if (n < ny) dst[n * ystride] = a[w];
     }
@@ -556,7 +643,23 @@ static T *dalloc(size_t n)
     return p;
 }
 
-static int lines_per_block(int P) { return 256 / (P / 16); }
+// threads per line of each supported length (FftPlan<P>::T)
+static int fft_threads(int P)
+{
+    switch (P) {
+    case 16: return 1;
+    case 32: return 2;
+    case 64: return 4;
+    case 128: return 8;
+    case 160: return 8;
+    case 192: return 16;
+    case 256: return 16;
+    case 512: return 32;
+    }
+    fail(ETYPE_ILLEGAL_VALUE, "unsupported ESSE FFT length %d", P);
+}
+
+static int lines_per_block(int P) { return 256 / fft_threads(P); }
 static size_t fft_lds(int P) { return (size_t)lines_per_block(P) * (P + P / 16 + 1) * sizeof(float2); }
 
 // Half-spectrum width along the fastest axis: kf <= P/2, padded to 16.
@@ -569,6 +672,8 @@ static int half_width(int P) { return std::min(P, ((P / 2 + 1 + 15) / 16) * 16);
         case 32: LAUNCH(32); break;                                                           \
         case 64: LAUNCH(64); break;                                                           \
         case 128: LAUNCH(128); break;                                                         \
+        case 160: LAUNCH(160); break;                                                         \
+        case 192: LAUNCH(192); break;                                                         \
         case 256: LAUNCH(256); break;                                                         \
         case 512: LAUNCH(512); break;                                                         \
         default: fail(ETYPE_ILLEGAL_VALUE, "unsupported ESSE FFT length %d", P);               \

@@ -115,11 +115,12 @@ static std::vector<float> reorder_krnl(const float *in, int noffs, int ndepths, 
     return o;
 }
 
-static int next_pow2(int v)
+// Smallest line-FFT length of the device FFT (esse.hip) that is >= v, or 0.
+static int fft_length(int v)
 {
-    int p = 1;
-    while (p < v) p *= 2;
-    return p;
+    for (int p : {16, 32, 64, 128, 160, 192, 256, 512})
+        if (p >= v) return p;
+    return 0;
 }
 
 // krnlprj3d.c:956-1009 and 403-547.
@@ -201,15 +202,20 @@ EsseModel make_esse_model(const KrnlData &k, const char *parString, float pixsiz
                 }
     for (int a = 0; a < 3; ++a)
         if (lo[a] > hi[a]) lo[a] = hi[a] = 0;
+    // Circular convolution of length P of a line zero-padded from n equals
+    // the linear one on the cropped outputs i < n when no input index i - d,
+    // d in [lo, hi], wraps into [0, n): P >= n + hi and P >= n - lo (then
+    // also P > hi - lo, so the kernel's offsets land on distinct cells).
     int P3[3];
-    for (int a = 0; a < 3; ++a)
-        P3[a] = std::max(16, next_pow2(n3[a] + std::max(hi[a], 0) + std::max(-lo[a], 0)));
+    for (int a = 0; a < 3; ++a) {
+        const int need = n3[a] + std::max(std::max(hi[a], -lo[a]), 0);
+        P3[a] = fft_length(need);
+        if (P3[a] == 0)
+            fail(ETYPE_ILLEGAL_VALUE, "ESSE FFT length %d along axis %d exceeds 512", need, a);
+    }
     m.Py = P3[0];
     m.Pz = P3[1];
     m.Px = P3[2];
-    if (m.Px > 512 || m.Py > 512 || m.Pz > 512)
-        fail(ETYPE_ILLEGAL_VALUE, "ESSE FFT grid %d x %d x %d exceeds 512 per axis", m.Py, m.Pz,
-             m.Px);
     for (int t = 0; t < nterms; ++t) {
         m.grid[t].assign((size_t)m.Px * m.Py * m.Pz, 0.0f);
         for (int dy = -(m.nyc - 1); dy <= m.nyc - 1; ++dy) {

@@ -43,8 +43,11 @@ def _gpu_effsrc(fname, pix, n, nz, collapse, act, mp):
     return out
 
 
+# The last case needs FFT lines of 192 along x (128 + 48) and 160 along
+# depth (128 + 9): the mixed-radix line lengths.
 @pytest.mark.parametrize("n,nz,noffs,ndepths,collapse", [
-    (32, 16, 5, 9, 1), (32, 16, 5, 9, 2), (16, 8, 9, 33, 1), (64, 32, 9, 17, 4)])
+    (32, 16, 5, 9, 1), (32, 16, 5, 9, 2), (16, 8, 9, 33, 1), (64, 32, 9, 17, 4),
+    (128, 16, 49, 17, 1)])
 def test_effsrc_matches_oracle(n, nz, noffs, ndepths, collapse):
     print("ESSE n=%d nz=%d krnl %dx%d c=%d" % (n, nz, noffs, ndepths, collapse), flush=True)
     pix = 0.5
@@ -74,6 +77,32 @@ def test_irlosem_with_esse_matches_oracle():
     ref = O.reorder(pb.n, pb.nz, pb.n, ref.reshape(pb.n, pb.nz, pb.n))
     parms = pb.parms()
     parms.iNumSrfIterations = 2
+    atn, prj, recon = pb.mu.copy(), pb.prj.copy(), pb.x0.copy()
+    L.set_adv_options(0, 0)
+    rc = L.irl_osem(parms, pb.options(), pb.views, pb.drf_file, f, None, atn, prj, recon)
+    assert rc == 0, L.error_string()
+    e = rel_l2(recon, ref)
+    print("rel L2 vs f64 %.3g" % e)
+    assert e < 1e-4
+
+
+@pytest.mark.parametrize("noffs", [17, 49])
+def test_irlosem_with_esse_mixed_radix_slices(noffs):
+    # slice-fastest engine layout: FFT lines of 128 + (noffs - 1) along the
+    # slices, 160 and 192 points
+    nz = 128
+    print("IrlOsem + ESSE N=32 Nz=%d krnl offsets %d" % (nz, noffs), flush=True)
+    import libirl_amd as L
+    pb = Problem(n=32, nz=nz, nviews=2, per=2, iters=1, half=3, noise=False)
+    f, k, km = _krnl_file(noffs, 9, pb.pix)
+    esse = O.Esse(k, km, MU0, pb.pix, pb.n, pb.nz, real="f64")
+    x0 = O.reorder(pb.nz, pb.n, pb.n, pb.x0)
+    ref = O.osem(pb.nz, pb.n, pb.oracle_views(), pb.per, pb.iters, pb.prj_model | 4,
+                 pb.prj_model, pb.pix, pb.drf, np.clip(pb.prj, 0, None), pb.atn_int, x0,
+                 esse=esse.h, srf_iterations=1)
+    ref = O.reorder(pb.n, pb.nz, pb.n, ref.reshape(pb.n, pb.nz, pb.n))
+    parms = pb.parms()
+    parms.iNumSrfIterations = 1
     atn, prj, recon = pb.mu.copy(), pb.prj.copy(), pb.x0.copy()
     L.set_adv_options(0, 0)
     rc = L.irl_osem(parms, pb.options(), pb.views, pb.drf_file, f, None, atn, prj, recon)
