@@ -54,8 +54,10 @@ def build_workload(tmpdir, seed=2, name="c2"):
     opts.iMsgLevel = 1
     krnl_file = None
     if "krnl_offs" in cfg:
-        # ESSE kernel (33^3 support): Gaussian mixture, mu0 = 0.15/cm
+        # ESSE kernel (33^3 support): Gaussian mixture, mu0 = 0.15/cm, scaled
+        # to a 30% scatter fraction
         k, km = synth.gaussian_mixture_krnl(cfg["krnl_offs"], cfg["krnl_depths"], pix)
+        k = synth.scale_krnl_to_fraction(k, 0.3)
         krnl_file = os.path.join(tmpdir, name + ".krnl")
         synth.write_krnl(krnl_file, k, km, 0.15, pix)
         parms.iNumSrfIterations = cfg["srf_iters"]

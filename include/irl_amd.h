@@ -65,6 +65,69 @@ int irlamd_last_error(void);
  * NumSlices % 4 == 0.  Returns the previous mode, or -1 for an invalid one. */
 int irlamd_set_precision(int iMode);
 
+/* ---------------------------------------------------------------------------
+ * Learned scatter (no counterpart in the reference).  A small 3-D CNN on the
+ * half-resolution grid predicts, once per sub-iteration, the two FFT
+ * convolution terms of the ESSE effective source (CalcEffScatterSource,
+ * krnlprj3d.c:826-948) from the current estimate and the attenuation map;
+ * each view's source is then their first-order combination with the view's
+ * attenuation depth.  The per-view ESSE stays the fallback and the check:
+ * the first scatter-modelled subset (and every iCheckEvery-th one after it)
+ * runs both, and a relative L2 difference of their projected sources
+ * (primary + scatter, summed over depth before the collimator blur) above
+ * fMaxNrmse switches the reconstruction back to ESSE.
+ * Handles are independent of reconstructions; the network is shared (kept
+ * alive) by every reconstruction created while it is set.
+ * ------------------------------------------------------------------------- */
+typedef struct irlamd_cnn irlamd_cnn;
+
+/* Trains a network on iNumPairs (x, mu) -> ESSE-term pairs generated on the
+ * GPU from seeded phantoms with the scatter kernel file's ESSE model at
+ * iNumPixels^2 x iNumSlices voxels of fPixSize cm (the last iNumVal pairs are
+ * held out).  pfStats (5 floats, may be NULL): final training loss, held-out
+ * relative L2 of the two outputs, seconds, output scale. */
+irlamd_cnn *irlamd_cnn_train(const char *pchScatKrnlFname, int iNumPixels, int iNumSlices,
+                             float fPixSize, int iNumPairs, int iNumVal, int iEpochs,
+                             int iBatch, float fLearnRate, unsigned uSeed, float *pfStats);
+/* Untrained network (He-normal weights from uSeed) with output scale fKappa. */
+irlamd_cnn *irlamd_cnn_create(float fKappa, unsigned uSeed);
+int irlamd_cnn_save(irlamd_cnn *h, const char *pchFname);
+irlamd_cnn *irlamd_cnn_load(const char *pchFname);
+void irlamd_cnn_free(irlamd_cnn *h);
+
+/* Parameters in packed order: per layer W[27][Cout][Cin] (tap = 9(dy+1) +
+ * 3(dx+1) + dz+1) then b[Cout]; layers 2->32, 32->32, 32->32, 32->2. */
+int irlamd_cnn_num_params(void);
+int irlamd_cnn_get_params(irlamd_cnn *h, float *pfParams, float *pfKappa);
+int irlamd_cnn_set_params(irlamd_cnn *h, const float *pfParams);
+
+/* Network on a host input [Y][X][Z][2] -> output [Y][X][Z][2]; bBf16 selects
+ * the inference arithmetic (BF16 MFMA layers), else FP32 throughout. */
+int irlamd_cnn_forward(irlamd_cnn *h, const float *pfIn, int iY, int iX, int iZ, int bBf16,
+                       float *pfOut);
+/* FP32 loss mean((out - target)^2) of one sample and its parameter
+ * gradients (packed order). */
+int irlamd_cnn_loss_grad(irlamd_cnn *h, const float *pfIn, const float *pfTarget, int iY,
+                         int iX, int iZ, double *pdLoss, float *pfGrads);
+/* Source volumes g0, g1 ([y][x][z], slices fastest) of an estimate x and an
+ * attenuation map mu (per pixel) for a scatter model with fDens =
+ * 1/(mu0 pixsize) and fCntrPixFac. */
+int irlamd_cnn_sources(irlamd_cnn *h, const float *pfX, const float *pfMu, int iNumPixels,
+                       int iNumSlices, float fDens, float fCntrPixFac, float fPrimary,
+                       float *pfG0, float *pfG1);
+
+/* Network used by reconstructions created afterwards (NULL: ESSE only). */
+int irlamd_set_scatter_cnn(irlamd_cnn *h, int iCheckEvery, float fMaxNrmse);
+/* Diagnostic: the same first-order source with the terms from one FFT
+ * convolution of the unrotated estimate instead of the network. */
+int irlamd_set_scatter_cnn_exact(int bExact);
+/* State of the learned scatter of a reconstruction (or, with r = NULL, of the
+ * last IrlOsem call): whether it is still in use, checks run, subsets run on
+ * the network, last and worst checked relative L2 of the projected source,
+ * and the last one of its scatter part alone. */
+int irlamd_recon_cnn_status(irlamd_recon *r, int *piActive, int *piChecks, int *piUsed,
+                            float *pfLastNrmse, float *pfWorstNrmse, float *pfLastScatterNrmse);
+
 #ifdef __cplusplus
 }
 #endif

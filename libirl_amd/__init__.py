@@ -200,6 +200,29 @@ def load():
     lib.ComputeOrbit.restype = ctypes.c_int
     lib.irlamd_set_precision.argtypes = [ctypes.c_int]
     lib.irlamd_set_precision.restype = ctypes.c_int
+    vp = ctypes.c_void_p
+    lib.irlamd_cnn_train.restype = vp
+    lib.irlamd_cnn_train.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_float, ctypes.c_uint, _FP]
+    lib.irlamd_cnn_create.restype = vp
+    lib.irlamd_cnn_create.argtypes = [ctypes.c_float, ctypes.c_uint]
+    lib.irlamd_cnn_save.argtypes = [vp, ctypes.c_char_p]
+    lib.irlamd_cnn_load.restype = vp
+    lib.irlamd_cnn_load.argtypes = [ctypes.c_char_p]
+    lib.irlamd_cnn_free.argtypes = [vp]
+    lib.irlamd_cnn_free.restype = None
+    lib.irlamd_cnn_get_params.argtypes = [vp, _FP, _FP]
+    lib.irlamd_cnn_set_params.argtypes = [vp, _FP]
+    lib.irlamd_cnn_forward.argtypes = [vp, _FP] + [ctypes.c_int] * 4 + [_FP]
+    lib.irlamd_cnn_loss_grad.argtypes = [vp, _FP, _FP] + [ctypes.c_int] * 3 + \
+        [ctypes.POINTER(ctypes.c_double), _FP]
+    lib.irlamd_cnn_sources.argtypes = [vp, _FP, _FP, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                       ctypes.c_float, ctypes.c_float, _FP, _FP]
+    lib.irlamd_set_scatter_cnn.argtypes = [vp, ctypes.c_int, ctypes.c_float]
+    lib.irlamd_set_scatter_cnn_exact.argtypes = [ctypes.c_int]
+    lib.irlamd_recon_cnn_status.argtypes = [vp] + [ctypes.POINTER(ctypes.c_int)] * 3 + \
+        [ctypes.POINTER(ctypes.c_float)] * 3
     _lib = lib
     return lib
 
@@ -447,3 +470,132 @@ class Recon:
             self.close()
         except Exception:
             pass
+
+
+class ScatterCnn:
+    """Handle of the scatter CNN (irlamd_cnn_*): create(), train(), load()."""
+
+    def __init__(self, h):
+        if not h:
+            raise RuntimeError(error_string())
+        self._lib = load()
+        self._h = h
+
+    @classmethod
+    def create(cls, kappa=1.0, seed=0):
+        return cls(load().irlamd_cnn_create(kappa, seed))
+
+    @classmethod
+    def train(cls, krnl_file, n, nz, pixsize, pairs, val, epochs, batch=4, lr=2e-3, seed=1):
+        """Returns (network, stats) with stats = {loss, val_nrmse0, val_nrmse1, seconds, kappa}."""
+        st = np.zeros(5, np.float32)
+        h = load().irlamd_cnn_train(_bytes(krnl_file), n, nz, pixsize, pairs, val, epochs, batch,
+                                    lr, seed, fptr(st))
+        keys = ("loss", "val_nrmse0", "val_nrmse1", "seconds", "kappa")
+        return cls(h), dict(zip(keys, map(float, st)))
+
+    @classmethod
+    def load_file(cls, fname):
+        return cls(load().irlamd_cnn_load(_bytes(fname)))
+
+    def save(self, fname):
+        if self._lib.irlamd_cnn_save(self._h, _bytes(fname)):
+            raise RuntimeError(error_string())
+
+    @staticmethod
+    def num_params():
+        return load().irlamd_cnn_num_params()
+
+    def params(self):
+        p = np.zeros(self.num_params(), np.float32)
+        k = np.zeros(1, np.float32)
+        if self._lib.irlamd_cnn_get_params(self._h, fptr(p), fptr(k)):
+            raise RuntimeError(error_string())
+        return p, float(k[0])
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        if self._lib.irlamd_cnn_set_params(self._h, fptr(p)):
+            raise RuntimeError(error_string())
+
+    def forward(self, inp, bf16=False):
+        inp = np.ascontiguousarray(inp, np.float32)
+        Y, X, Z, _ = inp.shape
+        out = np.zeros((Y, X, Z, 2), np.float32)
+        if self._lib.irlamd_cnn_forward(self._h, fptr(inp), Y, X, Z, 1 if bf16 else 0, fptr(out)):
+            raise RuntimeError(error_string())
+        return out
+
+    def loss_grad(self, inp, tgt):
+        inp = np.ascontiguousarray(inp, np.float32)
+        tgt = np.ascontiguousarray(tgt, np.float32)
+        Y, X, Z, _ = inp.shape
+        g = np.zeros(self.num_params(), np.float32)
+        loss = ctypes.c_double()
+        if self._lib.irlamd_cnn_loss_grad(self._h, fptr(inp), fptr(tgt), Y, X, Z,
+                                          ctypes.byref(loss), fptr(g)):
+            raise RuntimeError(error_string())
+        return loss.value, g
+
+    def sources(self, x, mu, dens_fac, cntr=0.0, primary=1.0):
+        """g0, g1 of a volume pair in [y][x][z] (slices fastest) order."""
+        x = np.ascontiguousarray(x, np.float32)
+        mu = np.ascontiguousarray(mu, np.float32)
+        N, _, Nz = x.shape
+        g0 = np.zeros_like(x)
+        g1 = np.zeros_like(x)
+        if self._lib.irlamd_cnn_sources(self._h, fptr(x), fptr(mu), N, Nz, dens_fac, cntr, primary,
+                                        fptr(g0), fptr(g1)):
+            raise RuntimeError(error_string())
+        return g0, g1
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.irlamd_cnn_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def unpack_cnn_params(p):
+    """Packed parameters -> [(W[27][Cout][Cin], b[Cout])] per layer."""
+    layers = ((2, 32), (32, 32), (32, 32), (32, 2))
+    out, o = [], 0
+    for ci, co in layers:
+        W = np.asarray(p[o:o + 27 * co * ci]).reshape(27, co, ci)
+        o += 27 * co * ci
+        out.append((W, np.asarray(p[o:o + co])))
+        o += co
+    return out
+
+
+def pack_cnn_params(params):
+    return np.concatenate([np.concatenate([np.ravel(W), np.ravel(b)]) for W, b in params]).astype(
+        np.float32)
+
+
+def set_scatter_cnn(net, check_every=0, max_nrmse=0.01):
+    """Network used by reconstructions created afterwards (None: ESSE only)."""
+    h = net._h if net is not None else None
+    if load().irlamd_set_scatter_cnn(h, check_every, max_nrmse):
+        raise ValueError("invalid learned-scatter settings")
+
+
+def set_scatter_cnn_exact(on):
+    load().irlamd_set_scatter_cnn_exact(1 if on else 0)
+
+
+def cnn_status(recon=None):
+    """{active, checks, used, last_nrmse, worst_nrmse} of a Recon, or of the last IrlOsem."""
+    a, c, u = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    last, worst, lsc = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+    load().irlamd_recon_cnn_status(recon._h if recon is not None else None, ctypes.byref(a),
+                                   ctypes.byref(c), ctypes.byref(u), ctypes.byref(last),
+                                   ctypes.byref(worst), ctypes.byref(lsc))
+    return {"active": bool(a.value), "checks": c.value, "used": u.value,
+            "last_nrmse": last.value, "worst_nrmse": worst.value,
+            "last_scatter_nrmse": lsc.value}

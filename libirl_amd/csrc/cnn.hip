@@ -1,0 +1,1322 @@
+// cnn.hip -- scatter CNN on gfx950 (no counterpart in the reference; it
+// stands in for the FFT convolutions of CalcEffScatterSource,
+// krnlprj3d.c:826-948, which remain the fallback and the check).
+//
+// Network (half-resolution grid, tensors [Y][X][Z][C], channels fastest):
+//   input  2 ch: 2x2x2 mean of x / mean(x), 2x2x2 mean of mu * fDens
+//   L1 2->32, L2 32->32, L3 32->32 (ReLU), L4 32->2 (linear);
+//   3x3x3 taps at dilations 1, 2, 4, 8 (receptive field 31 coarse voxels)
+//   output 2 ch: E0 / mean(x), kappa (fMu E0 - E1) / mean(x), 2x2x2 means.
+// Inference: L1 and L4 on the VALU (FP32 accumulation, BF16-rounded weights),
+// L2 and L3 as implicit GEMMs on v_mfma_f32_16x16x32_bf16 (BF16 weights and
+// activations, FP32 accumulation).  Training: every layer in FP32 on the
+// VALU, the input gradient as the same convolution with flipped and
+// transposed weights, weight gradients reduced per tap in LDS and added to
+// global memory with atomics, Adam on FP32 master weights.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <random>
+
+#include "cnn_dev.h"
+#include "esse_dev.h"
+
+namespace irlamd {
+
+#define CNN_CHECK(expr)                                                                  \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            ::irlamd::fail(::irlamd::ETYPE_DEVICE, "HIP error '%s' in %s (%s:%d)",                 \
+                           hipGetErrorString(e_), #expr, __FILE__, __LINE__);            \
+    } while (0)
+
+typedef __bf16 cbf16x8_t __attribute__((ext_vector_type(8)));
+typedef float cfloatx4_t __attribute__((ext_vector_type(4)));
+
+long cnn_num_params()
+{
+    long n = 0;
+    for (int l = 0; l < CNN_LAYERS; ++l) n += cnn_layer_params(l);
+    return n;
+}
+
+static unsigned cdiv_u(long a, long b) { return (unsigned)((a + b - 1) / b); }
+
+__device__ __forceinline__ float to_f(float v) { return v; }
+__device__ __forceinline__ float to_f(uint16_t v)
+{
+    return __uint_as_float((uint32_t)v << 16);
+}
+__device__ __forceinline__ void from_f(float &d, float v) { d = v; }
+__device__ __forceinline__ void from_f(uint16_t &d, float v)
+{
+    d = __builtin_bit_cast(uint16_t, (__bf16)v);
+}
+
+__device__ __forceinline__ void tap_off(int t, int dil, int &dy, int &dx, int &dz)
+{
+    dy = dil * (t / 9 - 1);
+    dx = dil * ((t / 3) % 3 - 1);
+    dz = dil * (t % 3 - 1);
+}
+
+// ---------------------------------------------------------------------------
+// Mean of a volume (FP64 accumulation): partial sums per block, then 1/mean.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_sum_partial(const float *__restrict__ v, long n,
+                                                     double *__restrict__ part)
+{
+    __shared__ double red[256];
+    double s = 0.0;
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256)
+        s += (double)v[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void k_inv_mean(const double *__restrict__ part, int nb, long n,
+                           float *__restrict__ scal)
+{
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nb; i += 256) s += part[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double m = red[0] / (double)n;
+        const float sx = m > 0 ? (float)m : 1.0f;
+        scal[0] = 1.0f / sx;
+        scal[1] = sx;
+    }
+}
+
+// Network input from a fine volume pair (engine layout [y][x][z]).
+__global__ __launch_bounds__(256) void k_cnn_input(const float *__restrict__ x,
+                                                   const float *__restrict__ mu, int N, int Nz,
+                                                   const float *__restrict__ scal, float densFac,
+                                                   float *__restrict__ in)
+{
+    const int Y = N / 2, X = N / 2, Z = Nz / 2;
+    const long v = blockIdx.x * 256L + threadIdx.x;
+    if (v >= (long)Y * X * Z) return;
+    const int z = (int)(v % Z);
+    const long r = v / Z;
+    const int xx = (int)(r % X), y = (int)(r / X);
+    float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+    for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+        for (int dx = 0; dx < 2; ++dx) {
+            const long o = ((long)(2 * y + dy) * N + 2 * xx + dx) * Nz + 2 * z;
+            s0 += x[o] + x[o + 1];
+            if (mu) s1 += mu[o] + mu[o + 1];
+        }
+    in[2 * v] = s0 * 0.125f * scal[0];
+    in[2 * v + 1] = s1 * 0.125f * densFac;
+}
+
+// Training targets from the ESSE terms (E0, E1 volumes of one sample).
+__global__ __launch_bounds__(256) void k_cnn_target(const float *__restrict__ E0,
+                                                    const float *__restrict__ E1, int N, int Nz,
+                                                    const float *__restrict__ scal, float fMu,
+                                                    float kappa, float *__restrict__ tgt)
+{
+    const int Y = N / 2, X = N / 2, Z = Nz / 2;
+    const long v = blockIdx.x * 256L + threadIdx.x;
+    if (v >= (long)Y * X * Z) return;
+    const int z = (int)(v % Z);
+    const long r = v / Z;
+    const int xx = (int)(r % X), y = (int)(r / X);
+    float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+    for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+        for (int dx = 0; dx < 2; ++dx)
+#pragma unroll
+            for (int dz = 0; dz < 2; ++dz) {
+                const long o = ((long)(2 * y + dy) * N + 2 * xx + dx) * Nz + 2 * z + dz;
+                s0 += E0[o];
+                s1 += fMu * E0[o] - E1[o];
+            }
+    tgt[2 * v] = s0 * 0.125f * scal[0];
+    tgt[2 * v + 1] = kappa * s1 * 0.125f * scal[0];
+}
+
+// ---------------------------------------------------------------------------
+// VALU convolution: one thread per output voxel, CO accumulators.  Weights
+// [27][CI][CO] (uniform over the wave: scalar loads).  Optional bias, ReLU
+// and a mask (out *= mask > 0, the ReLU derivative of the layer below).
+// ---------------------------------------------------------------------------
+template <int CO, typename TI, typename TO>
+__global__ __launch_bounds__(256) void k_conv_valu(const TI *__restrict__ in,
+                                                   const float *__restrict__ W,
+                                                   const float *__restrict__ bias,
+                                                   const float *__restrict__ mask,
+                                                   TO *__restrict__ out, int Y, int X, int Z,
+                                                   int CI, int dil, int relu)
+{
+    const long V = (long)Y * X * Z;
+    const long v = blockIdx.x * 256L + threadIdx.x;
+    if (v >= V) return;
+    const int z = (int)(v % Z);
+    const long r = v / Z;
+    const int x = (int)(r % X), y = (int)(r / X);
+    float acc[CO];
+#pragma unroll
+    for (int co = 0; co < CO; ++co) acc[co] = bias ? bias[co] : 0.0f;
+    for (int t = 0; t < 27; ++t) {
+        int dy, dx, dz;
+        tap_off(t, dil, dy, dx, dz);
+        const int yy = y + dy, xx = x + dx, zz = z + dz;
+        const bool ok = yy >= 0 && yy < Y && xx >= 0 && xx < X && zz >= 0 && zz < Z;
+        const TI *p = in + (((long)(ok ? yy : y) * X + (ok ? xx : x)) * Z + (ok ? zz : z)) * CI;
+        const float *w = W + (long)t * CI * CO;
+        for (int ci = 0; ci < CI; ++ci) {
+            const float a = ok ? to_f(p[ci]) : 0.0f;
+#pragma unroll
+            for (int co = 0; co < CO; ++co) acc[co] = fmaf(w[ci * CO + co], a, acc[co]);
+        }
+    }
+#pragma unroll
+    for (int co = 0; co < CO; ++co) {
+        float val = acc[co];
+        if (relu) val = fmaxf(val, 0.0f);
+        if (mask && !(mask[v * CO + co] > 0.0f)) val = 0.0f;
+        from_f(out[v * CO + co], val);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Implicit-GEMM convolution 32 -> 32 channels on MFMA (inference).
+// out^T[co][voxel] = sum_tap W_tap[co][ci] in^T[ci][voxel + off]: per tap one
+// K = 32 step; A = weights (bf16 [27][32 co][32 ci] staged in LDS, lane l
+// holds co = l%16 (+16), ci = 8(l/16)..+7), B = 16 voxels' channels (lane l:
+// voxel l%16, ci = 8(l/16)..+7, one 128-bit load).  A wave owns NT tiles of
+// 16 consecutive voxels; C lane l, element i = (co 4(l/16)+i, voxel l%16).
+// ---------------------------------------------------------------------------
+constexpr int CNN_NT = 4;
+
+__global__ __launch_bounds__(256) void k_conv_mfma(const uint16_t *__restrict__ in,
+                                                   const uint4 *__restrict__ Wg,
+                                                   const float *__restrict__ bias,
+                                                   uint16_t *__restrict__ out, int Y, int X,
+                                                   int Z, int dil, int relu)
+{
+    __shared__ uint4 wl[27 * 32 * 32 * 2 / 16];
+    for (int i = threadIdx.x; i < 27 * 128; i += 256) wl[i] = Wg[i];
+    __syncthreads();
+    const long V = (long)Y * X * Z;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int n = lane & 15, g = lane >> 4;
+    const long base = ((long)blockIdx.x * 4 + w) * (16 * CNN_NT);
+    int vy[CNN_NT], vx[CNN_NT], vz[CNN_NT];
+#pragma unroll
+    for (int j = 0; j < CNN_NT; ++j) {
+        const long v = base + 16 * j + n;
+        const long vc = v < V ? v : V - 1;
+        vz[j] = (int)(vc % Z);
+        const long r = vc / Z;
+        vx[j] = (int)(r % X);
+        vy[j] = v < V ? (int)(r / X) : -(1 << 20);   // invalid voxels read nothing
+    }
+    cfloatx4_t acc[2][CNN_NT];
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int j = 0; j < CNN_NT; ++j) acc[c][j] = cfloatx4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int t = 0; t < 27; ++t) {
+        int dy, dx, dz;
+        tap_off(t, dil, dy, dx, dz);
+        const cbf16x8_t A0 = __builtin_bit_cast(cbf16x8_t, wl[t * 128 + n * 4 + g]);
+        const cbf16x8_t A1 = __builtin_bit_cast(cbf16x8_t, wl[t * 128 + (16 + n) * 4 + g]);
+        uint4 braw[CNN_NT];
+#pragma unroll
+        for (int j = 0; j < CNN_NT; ++j) {
+            const int yy = vy[j] + dy, xx = vx[j] + dx, zz = vz[j] + dz;
+            const bool ok = yy >= 0 && yy < Y && xx >= 0 && xx < X && zz >= 0 && zz < Z;
+            braw[j] = ok ? *(const uint4 *)(in + (((long)yy * X + xx) * Z + zz) * 32 + 8 * g)
+                         : make_uint4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (int j = 0; j < CNN_NT; ++j) {
+            const cbf16x8_t B = __builtin_bit_cast(cbf16x8_t, braw[j]);
+            acc[0][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A0, B, acc[0][j], 0, 0, 0);
+            acc[1][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1, B, acc[1][j], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < CNN_NT; ++j) {
+        const long v = base + 16 * j + n;
+        if (v >= V) continue;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const int co = 16 * c + 4 * g;
+            uint16_t h[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float val = acc[c][j][i] + bias[co + i];
+                if (relu) val = fmaxf(val, 0.0f);
+                from_f(h[i], val);
+            }
+            *(uint2 *)(out + v * 32 + co) =
+                make_uint2((uint32_t)h[0] | ((uint32_t)h[1] << 16),
+                           (uint32_t)h[2] | ((uint32_t)h[3] << 16));
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradients of one tap per blockIdx.y over a chunk of voxels:
+// dW[t][ci][co] += sum_v in[v + off_t][ci] dout[v][co].  Threads own IB x OB
+// blocks of (ci, co); 256/G voxel subgroups are reduced in LDS.
+// ---------------------------------------------------------------------------
+template <int CI, int CO>
+__global__ __launch_bounds__(256) void k_conv_dw(const float *__restrict__ in,
+                                                 const float *__restrict__ dout,
+                                                 float *__restrict__ dW, int Y, int X, int Z,
+                                                 int dil, int chunk)
+{
+    constexpr int IB = CI < 4 ? CI : 4, OB = CO < 4 ? CO : 4;
+    constexpr int G = (CI / IB) * (CO / OB);
+    constexpr int NSG = 256 / G;
+    __shared__ float red[NSG * CI * CO];
+    const int t = blockIdx.y;
+    int dy, dx, dz;
+    tap_off(t, dil, dy, dx, dz);
+    const int sg = threadIdx.x / G, e = threadIdx.x % G;
+    const int ci0 = (e / (CO / OB)) * IB, co0 = (e % (CO / OB)) * OB;
+    float acc[IB][OB];
+#pragma unroll
+    for (int i = 0; i < IB; ++i)
+#pragma unroll
+        for (int o = 0; o < OB; ++o) acc[i][o] = 0.0f;
+    const long V = (long)Y * X * Z;
+    const long v0 = (long)blockIdx.x * chunk;
+    const long v1 = v0 + chunk < V ? v0 + chunk : V;
+    for (long v = v0 + sg; v < v1; v += NSG) {
+        const int z = (int)(v % Z);
+        const long r = v / Z;
+        const int x = (int)(r % X), y = (int)(r / X);
+        const int yy = y + dy, xx = x + dx, zz = z + dz;
+        if (yy < 0 || yy >= Y || xx < 0 || xx >= X || zz < 0 || zz >= Z) continue;
+        const float *pi = in + (((long)yy * X + xx) * Z + zz) * CI + ci0;
+        const float *po = dout + v * CO + co0;
+        float a[IB], gg[OB];
+#pragma unroll
+        for (int i = 0; i < IB; ++i) a[i] = pi[i];
+#pragma unroll
+        for (int o = 0; o < OB; ++o) gg[o] = po[o];
+#pragma unroll
+        for (int i = 0; i < IB; ++i)
+#pragma unroll
+            for (int o = 0; o < OB; ++o) acc[i][o] = fmaf(a[i], gg[o], acc[i][o]);
+    }
+#pragma unroll
+    for (int i = 0; i < IB; ++i)
+#pragma unroll
+        for (int o = 0; o < OB; ++o) red[sg * CI * CO + (ci0 + i) * CO + co0 + o] = acc[i][o];
+    __syncthreads();
+    for (int k = threadIdx.x; k < CI * CO; k += 256) {
+        float s = 0.0f;
+        for (int q = 0; q < NSG; ++q) s += red[q * CI * CO + k];
+        atomicAdd(dW + (long)t * CI * CO + k, s);
+    }
+}
+
+// Bias gradients: db[co] += sum_v dout[v][co].
+template <int CO>
+__global__ __launch_bounds__(256) void k_bias_grad(const float *__restrict__ dout, long V,
+                                                   float *__restrict__ db, int chunk)
+{
+    constexpr int NVL = 256 / CO;
+    __shared__ float red[256];
+    const int co = threadIdx.x % CO, vl = threadIdx.x / CO;
+    const long v0 = (long)blockIdx.x * chunk;
+    const long v1 = v0 + chunk < V ? v0 + chunk : V;
+    float s = 0.0f;
+    for (long v = v0 + vl; v < v1; v += NVL) s += dout[v * CO + co];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x < CO) {
+        float t = 0.0f;
+        for (int q = 0; q < NVL; ++q) t += red[q * CO + threadIdx.x];
+        atomicAdd(db + threadIdx.x, t);
+    }
+}
+
+// Squared-error loss and its gradient gscale*(out - tgt) (gscale = 2/count).
+__global__ __launch_bounds__(256) void k_mse(const float *__restrict__ out,
+                                             const float *__restrict__ tgt, long n, float gscale,
+                                             float *__restrict__ grad, double *__restrict__ loss)
+{
+    __shared__ double red[256];
+    double s = 0.0;
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        const float d = out[i] - tgt[i];
+        grad[i] = gscale * d;
+        s += (double)d * (double)d;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(loss, red[0]);
+}
+
+// Adam (Kingma & Ba 2015) on FP32 master weights; clears the gradient.
+__global__ void k_adam(float *__restrict__ w, float *__restrict__ g, float *__restrict__ m,
+                       float *__restrict__ v, long n, float lr, float b1, float b2, float eps,
+                       float bc1, float bc2)
+{
+    const long i = blockIdx.x * 256L + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+    g[i] = 0.0f;
+}
+
+// Input-gradient weights: Wt[26 - t][co][ci] = W[t][ci][co] (layout [27][CI'][CO']
+// of the convolution with CI' = CO, CO' = CI).
+__global__ void k_flip(const float *__restrict__ W, float *__restrict__ Wt, int CI, int CO)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= 27 * CI * CO) return;
+    const int t = i / (CI * CO), r = i % (CI * CO), ci = r / CO, co = r % CO;
+    Wt[((long)(26 - t) * CO + co) * CI + ci] = W[i];
+}
+
+// BF16 copy of master weights [27][CI][CO] in MFMA order [27][CO][CI].
+__global__ void k_to_bf16_t(const float *__restrict__ W, uint16_t *__restrict__ Wb, int CI,
+                            int CO)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= 27 * CI * CO) return;
+    const int t = i / (CI * CO), r = i % (CI * CO), ci = r / CO, co = r % CO;
+    from_f(Wb[((long)t * CO + co) * CI + ci], W[i]);
+}
+
+// FP32 weights rounded to BF16 values (the VALU layers of the inference path).
+__global__ void k_round_bf16(const float *__restrict__ W, float *__restrict__ Wr, long n)
+{
+    const long i = blockIdx.x * 256L + threadIdx.x;
+    if (i >= n) return;
+    Wr[i] = (float)(__bf16)W[i];
+}
+
+// ---------------------------------------------------------------------------
+// Source volumes (engine layout [y][x][z]) from the network output: trilinear
+// upsampling by 2 (fine sample j at coarse coordinate j/2 - 1/4, clamped).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void up_coord(int j, int n, int &i0, int &i1, float &w)
+{
+    const float u = 0.5f * (float)j - 0.25f;
+    if (u < 0.0f) {
+        i0 = i1 = 0;
+        w = 0.0f;
+        return;
+    }
+    i0 = (int)u;
+    w = u - (float)i0;
+    i1 = i0 + 1 < n ? i0 + 1 : n - 1;
+    if (i0 > n - 1) i0 = n - 1;
+}
+
+__global__ __launch_bounds__(256) void k_cnn_source(const float *__restrict__ o,
+                                                    const float *__restrict__ x,
+                                                    const float *__restrict__ mu, int N, int Nz,
+                                                    const float *__restrict__ scal, float kappa,
+                                                    float fDens, float cntr, float primary,
+                                                    float *__restrict__ g0,
+                                                    float *__restrict__ g1)
+{
+    const long n = (long)N * N * Nz;
+    const long i = blockIdx.x * 256L + threadIdx.x;
+    if (i >= n) return;
+    const int z = (int)(i % Nz);
+    const long r = i / Nz;
+    const int xx = (int)(r % N), y = (int)(r / N);
+    const int Y = N / 2, X = N / 2, Z = Nz / 2;
+    int y0, y1, x0, x1, z0, z1;
+    float wy, wx, wz;
+    up_coord(y, Y, y0, y1, wy);
+    up_coord(xx, X, x0, x1, wx);
+    up_coord(z, Z, z0, z1, wz);
+    float c[2] = {0.0f, 0.0f};
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int d = 0; d < 2; ++d) {
+                const float wgt = (a ? wy : 1.0f - wy) * (b ? wx : 1.0f - wx) * (d ? wz : 1.0f - wz);
+                const long q = (((long)(a ? y1 : y0) * X + (b ? x1 : x0)) * Z + (d ? z1 : z0)) * 2;
+                c[0] = fmaf(wgt, o[q], c[0]);
+                c[1] = fmaf(wgt, o[q + 1], c[1]);
+            }
+    const float sx = scal[1];
+    const float dens = mu ? mu[i] * fDens : 0.0f;
+    const float xi = x[i];
+    g0[i] = primary * xi + (sx * c[0] + cntr * xi) * dens;
+    g1[i] = (sx / kappa) * c[1] * dens;
+}
+
+__global__ __launch_bounds__(256) void k_exact_source(const float *__restrict__ E0,
+                                                      const float *__restrict__ E1,
+                                                      const float *__restrict__ x,
+                                                      const float *__restrict__ mu, long n,
+                                                      float fMu, float fDens, float cntr,
+                                                      float primary, float *__restrict__ g0,
+                                                      float *__restrict__ g1)
+{
+    const long i = blockIdx.x * 256L + threadIdx.x;
+    if (i >= n) return;
+    const float dens = mu ? mu[i] * fDens : 0.0f;
+    const float xi = x[i], e0 = E0[i];
+    g0[i] = primary * xi + (e0 + cntr * xi) * dens;
+    g1[i] = (fMu * e0 - E1[i]) * dens;
+}
+
+void cnn_exact_sources(const float *E0, const float *E1, const float *x, const float *mu, int N,
+                       int Nz, const CnnPhys &ph, float primary, float *g0, float *g1,
+                       hipStream_t s)
+{
+    const long n = (long)N * N * Nz;
+    hipLaunchKernelGGL(k_exact_source, dim3(cdiv_u(n, 256)), dim3(256), 0, s, E0, E1, x, mu, n,
+                       ph.fMu, ph.fDens, ph.cntr, primary, g0, g1);
+    CNN_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Training phantoms: ellipsoids rotated about the slice axis.
+// ---------------------------------------------------------------------------
+struct Ellipsoid {
+    float cy, cx, cz, iay, iax, iaz, c, s, val;
+};
+
+__global__ __launch_bounds__(256) void k_phantom(const Ellipsoid *__restrict__ e, int ne, int N,
+                                                 int Nz, float *__restrict__ out)
+{
+    const long n = (long)N * N * Nz;
+    const long i = blockIdx.x * 256L + threadIdx.x;
+    if (i >= n) return;
+    const int z = (int)(i % Nz);
+    const long r = i / Nz;
+    const int x = (int)(r % N), y = (int)(r / N);
+    float v = 0.0f;
+    for (int k = 0; k < ne; ++k) {
+        const float dy = (float)y + 0.5f - e[k].cy, dx = (float)x + 0.5f - e[k].cx;
+        const float dz = (float)z + 0.5f - e[k].cz;
+        const float u = (dy * e[k].c + dx * e[k].s) * e[k].iay;
+        const float w = (-dy * e[k].s + dx * e[k].c) * e[k].iax;
+        const float q = dz * e[k].iaz;
+        if (u * u + w * w + q * q <= 1.0f) v += e[k].val;
+    }
+    out[i] = fmaxf(v, 0.0f);
+}
+
+// Gaussian blur along one axis (0 = y, 1 = x, 2 = z) of [y][x][z], radius 3 sigma.
+__global__ __launch_bounds__(256) void k_blur(const float *__restrict__ in, float *__restrict__ out,
+                                              int N, int Nz, int axis, float sigma)
+{
+    const long n = (long)N * N * Nz;
+    const long i = blockIdx.x * 256L + threadIdx.x;
+    if (i >= n) return;
+    const int z = (int)(i % Nz);
+    const long r = i / Nz;
+    const int x = (int)(r % N), y = (int)(r / N);
+    const int pos = axis == 0 ? y : axis == 1 ? x : z;
+    const int len = axis == 2 ? Nz : N;
+    const long stride = axis == 0 ? (long)N * Nz : axis == 1 ? Nz : 1;
+    const int R = (int)ceilf(3.0f * sigma);
+    float s = 0.0f, ws = 0.0f;
+    for (int k = -R; k <= R; ++k) {
+        const int p = pos + k;
+        const float wgt = expf(-0.5f * (float)(k * k) / (sigma * sigma));
+        ws += wgt;
+        if (p >= 0 && p < len) s += wgt * in[i + (long)k * stride];
+    }
+    out[i] = s / ws;
+}
+
+// Multiplicative noise x *= 1 + eps * sqrt(3) (2u - 1), u uniform from a hash.
+__global__ __launch_bounds__(256) void k_noise(float *__restrict__ x, long n, float eps,
+                                               uint32_t seed)
+{
+    const long i = blockIdx.x * 256L + threadIdx.x;
+    if (i >= n) return;
+    uint32_t h = (uint32_t)i * 0x9E3779B1u ^ seed;
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    const float u = (float)(h >> 8) * (1.0f / 16777216.0f);
+    x[i] = fmaxf(x[i] * (1.0f + eps * 1.7320508f * (2.0f * u - 1.0f)), 0.0f);
+}
+
+// ---------------------------------------------------------------------------
+// ScatterCnn
+// ---------------------------------------------------------------------------
+template <typename T>
+static T *cnn_alloc(size_t n)
+{
+    T *p = nullptr;
+    if (n) CNN_CHECK(hipMalloc((void **)&p, n * sizeof(T)));
+    return p;
+}
+
+template <typename T>
+static void cnn_free(T *&p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+static long wcount(int l) { return 27L * cnn_cin(l) * cnn_cout(l); }
+
+void ScatterCnn::alloc_state()
+{
+    for (int l = 0; l < CNN_LAYERS; ++l) {
+        const long nw = wcount(l), nb = cnn_cout(l);
+        W_[l] = cnn_alloc<float>(nw);
+        b_[l] = cnn_alloc<float>(nb);
+        gW_[l] = cnn_alloc<float>(nw);
+        gb_[l] = cnn_alloc<float>(nb);
+        mW_[l] = cnn_alloc<float>(nw);
+        vW_[l] = cnn_alloc<float>(nw);
+        mb_[l] = cnn_alloc<float>(nb);
+        vb_[l] = cnn_alloc<float>(nb);
+        Wbf_[l] = cnn_alloc<uint16_t>(nw);
+        CNN_CHECK(hipMemset(gW_[l], 0, nw * 4));
+        CNN_CHECK(hipMemset(gb_[l], 0, nb * 4));
+        CNN_CHECK(hipMemset(mW_[l], 0, nw * 4));
+        CNN_CHECK(hipMemset(vW_[l], 0, nw * 4));
+        CNN_CHECK(hipMemset(mb_[l], 0, nb * 4));
+        CNN_CHECK(hipMemset(vb_[l], 0, nb * 4));
+    }
+    Wt_ = cnn_alloc<float>(27L * 32 * 32);
+    scal_ = cnn_alloc<float>(2);
+    dsum_ = cnn_alloc<double>(1);
+}
+
+ScatterCnn::ScatterCnn(float kappa, unsigned seed) : kappa_(kappa)
+{
+    alloc_state();
+    std::mt19937 rng(seed);
+    std::vector<float> packed(cnn_num_params());
+    long o = 0;
+    for (int l = 0; l < CNN_LAYERS; ++l) {
+        std::normal_distribution<float> nd(0.0f, std::sqrt(2.0f / (27.0f * cnn_cin(l))));
+        for (long k = 0; k < wcount(l); ++k) packed[o++] = nd(rng);
+        for (int k = 0; k < cnn_cout(l); ++k) packed[o++] = 0.0f;
+    }
+    setParams(packed);
+}
+
+ScatterCnn::ScatterCnn(float kappa, const std::vector<float> &packed) : kappa_(kappa)
+{
+    alloc_state();
+    setParams(packed);
+}
+
+ScatterCnn::~ScatterCnn()
+{
+    for (int l = 0; l < CNN_LAYERS; ++l) {
+        cnn_free(W_[l]);
+        cnn_free(b_[l]);
+        cnn_free(gW_[l]);
+        cnn_free(gb_[l]);
+        cnn_free(mW_[l]);
+        cnn_free(vW_[l]);
+        cnn_free(mb_[l]);
+        cnn_free(vb_[l]);
+        cnn_free(Wbf_[l]);
+    }
+    for (auto &a : act_) cnn_free(a);
+    for (auto &a : grad_) cnn_free(a);
+    for (auto &a : actb_) cnn_free(a);
+    cnn_free(Wt_);
+    cnn_free(inbuf_);
+    cnn_free(outbuf_);
+    cnn_free(scal_);
+    cnn_free(dsum_);
+    cnn_free(psum_);
+}
+
+// packed (oracle) order per layer: W [27][Cout][Cin], b [Cout]
+void ScatterCnn::setParams(const std::vector<float> &packed)
+{
+    if ((long)packed.size() != cnn_num_params())
+        fail(ETYPE_ILLEGAL_VALUE, "scatter CNN: %ld parameters given, %ld expected",
+             (long)packed.size(), cnn_num_params());
+    long o = 0;
+    for (int l = 0; l < CNN_LAYERS; ++l) {
+        const int CI = cnn_cin(l), CO = cnn_cout(l);
+        std::vector<float> w(wcount(l));
+        for (int t = 0; t < 27; ++t)
+            for (int co = 0; co < CO; ++co)
+                for (int ci = 0; ci < CI; ++ci)
+                    w[((long)t * CI + ci) * CO + co] = packed[o + ((long)t * CO + co) * CI + ci];
+        o += wcount(l);
+        CNN_CHECK(hipMemcpy(W_[l], w.data(), w.size() * 4, hipMemcpyHostToDevice));
+        CNN_CHECK(hipMemcpy(b_[l], packed.data() + o, CO * 4, hipMemcpyHostToDevice));
+        o += CO;
+    }
+    bfDirty_ = true;
+}
+
+void ScatterCnn::getParams(std::vector<float> &packed) const
+{
+    packed.assign(cnn_num_params(), 0.0f);
+    long o = 0;
+    for (int l = 0; l < CNN_LAYERS; ++l) {
+        const int CI = cnn_cin(l), CO = cnn_cout(l);
+        std::vector<float> w(wcount(l));
+        CNN_CHECK(hipMemcpy(w.data(), W_[l], w.size() * 4, hipMemcpyDeviceToHost));
+        for (int t = 0; t < 27; ++t)
+            for (int co = 0; co < CO; ++co)
+                for (int ci = 0; ci < CI; ++ci)
+                    packed[o + ((long)t * CO + co) * CI + ci] = w[((long)t * CI + ci) * CO + co];
+        o += wcount(l);
+        CNN_CHECK(hipMemcpy(packed.data() + o, b_[l], CO * 4, hipMemcpyDeviceToHost));
+        o += CO;
+    }
+}
+
+void ScatterCnn::getGrads(std::vector<float> &packed) const
+{
+    packed.assign(cnn_num_params(), 0.0f);
+    long o = 0;
+    for (int l = 0; l < CNN_LAYERS; ++l) {
+        const int CI = cnn_cin(l), CO = cnn_cout(l);
+        std::vector<float> w(wcount(l));
+        CNN_CHECK(hipMemcpy(w.data(), gW_[l], w.size() * 4, hipMemcpyDeviceToHost));
+        for (int t = 0; t < 27; ++t)
+            for (int co = 0; co < CO; ++co)
+                for (int ci = 0; ci < CI; ++ci)
+                    packed[o + ((long)t * CO + co) * CI + ci] = w[((long)t * CI + ci) * CO + co];
+        o += wcount(l);
+        CNN_CHECK(hipMemcpy(packed.data() + o, gb_[l], CO * 4, hipMemcpyDeviceToHost));
+        o += CO;
+    }
+}
+
+void ScatterCnn::ensure_work(long V)
+{
+    if (V <= workV_) return;
+    for (auto &a : act_) cnn_free(a);
+    for (auto &a : grad_) cnn_free(a);
+    for (auto &a : actb_) cnn_free(a);
+    cnn_free(inbuf_);
+    cnn_free(outbuf_);
+    for (int l = 0; l <= CNN_LAYERS; ++l)
+        act_[l] = cnn_alloc<float>(V * (l == 0 ? 2 : l == CNN_LAYERS ? 2 : 32));
+    for (auto &a : grad_) a = cnn_alloc<float>(V * 32);
+    for (auto &a : actb_) a = cnn_alloc<uint16_t>(V * 32);
+    inbuf_ = cnn_alloc<float>(V * 2);
+    outbuf_ = cnn_alloc<float>(V * 2);
+    workV_ = V;
+}
+
+void ScatterCnn::upload_bf16_weights(hipStream_t s)
+{
+    if (!bfDirty_) return;
+    for (int l = 0; l < CNN_LAYERS; ++l) {
+        const long nw = wcount(l);
+        hipLaunchKernelGGL(k_to_bf16_t, dim3(cdiv_u(nw, 256)), dim3(256), 0, s, W_[l], Wbf_[l],
+                           cnn_cin(l), cnn_cout(l));
+    }
+    CNN_CHECK(hipGetLastError());
+    bfDirty_ = false;
+}
+
+// FP32 master weights of the VALU layers rounded to BF16 values, kept in
+// the input-gradient buffer (free outside training steps).
+void ScatterCnn::forward(const float *in, int Y, int X, int Z, float *out, bool bf16,
+                         hipStream_t s)
+{
+    const long V = (long)Y * X * Z;
+    ensure_work(V);
+    const dim3 grid(cdiv_u(V, 256)), blk(256);
+    if (!bf16) {
+        const float *a = in;
+        for (int l = 0; l < CNN_LAYERS; ++l) {
+            float *o = l == CNN_LAYERS - 1 ? out : act_[l + 1];
+            const int relu = l < CNN_LAYERS - 1;
+            if (cnn_cout(l) == 32)
+                hipLaunchKernelGGL((k_conv_valu<32, float, float>), grid, blk, 0, s, a, W_[l],
+                                   b_[l], nullptr, o, Y, X, Z, cnn_cin(l), cnn_dil(l), relu);
+            else
+                hipLaunchKernelGGL((k_conv_valu<2, float, float>), grid, blk, 0, s, a, W_[l],
+                                   b_[l], nullptr, o, Y, X, Z, cnn_cin(l), cnn_dil(l), relu);
+            a = o;
+        }
+        CNN_CHECK(hipGetLastError());
+        return;
+    }
+    upload_bf16_weights(s);
+    // BF16-rounded FP32 weights of the VALU layers: L1 in Wt_, L4 after it
+    float *w1 = Wt_, *w4 = Wt_ + wcount(0);
+    hipLaunchKernelGGL(k_round_bf16, dim3(cdiv_u(wcount(0), 256)), blk, 0, s, W_[0], w1,
+                       wcount(0));
+    hipLaunchKernelGGL(k_round_bf16, dim3(cdiv_u(wcount(3), 256)), blk, 0, s, W_[3], w4,
+                       wcount(3));
+    hipLaunchKernelGGL((k_conv_valu<32, float, uint16_t>), grid, blk, 0, s, in, w1, b_[0],
+                       nullptr, actb_[0], Y, X, Z, 2, cnn_dil(0), 1);
+    const dim3 mgrid(cdiv_u(V, 4 * 16 * CNN_NT));
+    hipLaunchKernelGGL(k_conv_mfma, mgrid, blk, 0, s, actb_[0], (const uint4 *)Wbf_[1], b_[1],
+                       actb_[1], Y, X, Z, cnn_dil(1), 1);
+    hipLaunchKernelGGL(k_conv_mfma, mgrid, blk, 0, s, actb_[1], (const uint4 *)Wbf_[2], b_[2],
+                       actb_[0], Y, X, Z, cnn_dil(2), 1);
+    hipLaunchKernelGGL((k_conv_valu<2, uint16_t, float>), grid, blk, 0, s, actb_[0], w4, b_[3],
+                       nullptr, out, Y, X, Z, 32, cnn_dil(3), 0);
+    CNN_CHECK(hipGetLastError());
+}
+
+void ScatterCnn::zeroGrads(hipStream_t s)
+{
+    for (int l = 0; l < CNN_LAYERS; ++l) {
+        CNN_CHECK(hipMemsetAsync(gW_[l], 0, wcount(l) * 4, s));
+        CNN_CHECK(hipMemsetAsync(gb_[l], 0, cnn_cout(l) * 4, s));
+    }
+}
+
+template <int CI, int CO>
+static void launch_dw(const float *in, const float *dout, float *dW, int Y, int X, int Z,
+                      int dil, hipStream_t s)
+{
+    const long V = (long)Y * X * Z;
+    const int chunk = 4096;
+    hipLaunchKernelGGL((k_conv_dw<CI, CO>), dim3(cdiv_u(V, chunk), 27), dim3(256), 0, s, in, dout,
+                       dW, Y, X, Z, dil, chunk);
+}
+
+double ScatterCnn::lossGrad(const float *in, const float *tgt, int Y, int X, int Z, float gscale,
+                            hipStream_t s)
+{
+    const long V = (long)Y * X * Z;
+    ensure_work(V);
+    const dim3 grid(cdiv_u(V, 256)), blk(256);
+    // forward, keeping the activations
+    CNN_CHECK(hipMemcpyAsync(act_[0], in, V * 2 * 4, hipMemcpyDeviceToDevice, s));
+    for (int l = 0; l < CNN_LAYERS; ++l) {
+        const int relu = l < CNN_LAYERS - 1;
+        if (cnn_cout(l) == 32)
+            hipLaunchKernelGGL((k_conv_valu<32, float, float>), grid, blk, 0, s, act_[l], W_[l],
+                               b_[l], nullptr, act_[l + 1], Y, X, Z, cnn_cin(l), cnn_dil(l), relu);
+        else
+            hipLaunchKernelGGL((k_conv_valu<2, float, float>), grid, blk, 0, s, act_[l], W_[l],
+                               b_[l], nullptr, act_[l + 1], Y, X, Z, cnn_cin(l), cnn_dil(l), relu);
+    }
+    CNN_CHECK(hipMemsetAsync(dsum_, 0, sizeof(double), s));
+    const long nout = V * 2;
+    hipLaunchKernelGGL(k_mse, dim3(std::min<long>(1024, cdiv_u(nout, 256))), blk, 0, s,
+                       act_[CNN_LAYERS], tgt, nout, gscale, grad_[0], dsum_);
+    // backward: grad_[cur] holds d(loss)/d(output of layer l) (ReLU mask applied)
+    int cur = 0;
+    for (int l = CNN_LAYERS - 1; l >= 0; --l) {
+        const int CI = cnn_cin(l), CO = cnn_cout(l), dil = cnn_dil(l);
+        const float *g = grad_[cur];
+        if (CI == 32 && CO == 32) launch_dw<32, 32>(act_[l], g, gW_[l], Y, X, Z, dil, s);
+        else if (CI == 32) launch_dw<32, 2>(act_[l], g, gW_[l], Y, X, Z, dil, s);
+        else launch_dw<2, 32>(act_[l], g, gW_[l], Y, X, Z, dil, s);
+        if (CO == 32)
+            hipLaunchKernelGGL((k_bias_grad<32>), dim3(cdiv_u(V, 4096)), blk, 0, s, g, V, gb_[l], 4096);
+        else
+            hipLaunchKernelGGL((k_bias_grad<2>), dim3(cdiv_u(V, 4096)), blk, 0, s, g, V, gb_[l], 4096);
+        if (l == 0) break;
+        // input gradient = convolution with flipped, transposed weights,
+        // masked by the ReLU of layer l-1 (its output act_[l] > 0)
+        hipLaunchKernelGGL(k_flip, dim3(cdiv_u(27L * CI * CO, 256)), blk, 0, s, W_[l], Wt_, CI, CO);
+        float *gn = grad_[1 - cur];   // CI (= 32 for l > 0) channels
+        hipLaunchKernelGGL((k_conv_valu<32, float, float>), grid, blk, 0, s, g, Wt_, nullptr,
+                           act_[l], gn, Y, X, Z, CO, dil, 0);
+        cur = 1 - cur;
+    }
+    CNN_CHECK(hipGetLastError());
+    double loss = 0;
+    CNN_CHECK(hipMemcpyAsync(&loss, dsum_, sizeof(double), hipMemcpyDeviceToHost, s));
+    CNN_CHECK(hipStreamSynchronize(s));
+    return loss / (double)nout;
+}
+
+void ScatterCnn::adamStep(float lr, hipStream_t s)
+{
+    ++step_;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float bc1 = 1.0f - std::pow(b1, (float)step_), bc2 = 1.0f - std::pow(b2, (float)step_);
+    for (int l = 0; l < CNN_LAYERS; ++l) {
+        const long nw = wcount(l), nb = cnn_cout(l);
+        hipLaunchKernelGGL(k_adam, dim3(cdiv_u(nw, 256)), dim3(256), 0, s, W_[l], gW_[l], mW_[l],
+                           vW_[l], nw, lr, b1, b2, eps, bc1, bc2);
+        hipLaunchKernelGGL(k_adam, dim3(cdiv_u(nb, 256)), dim3(256), 0, s, b_[l], gb_[l], mb_[l],
+                           vb_[l], nb, lr, b1, b2, eps, bc1, bc2);
+    }
+    CNN_CHECK(hipGetLastError());
+    bfDirty_ = true;
+}
+
+static void mean_scale(const float *x, long n, double *&psum, long &nb, float *scal, hipStream_t s)
+{
+    const long want = std::min<long>(1024, cdiv_u(n, 256));
+    if (want > nb) {
+        cnn_free(psum);
+        psum = cnn_alloc<double>(want);
+        nb = want;
+    }
+    hipLaunchKernelGGL(k_sum_partial, dim3((unsigned)want), dim3(256), 0, s, x, n, psum);
+    hipLaunchKernelGGL(k_inv_mean, dim3(1), dim3(256), 0, s, psum, (int)want, n, scal);
+}
+
+void ScatterCnn::sources(const float *x, const float *mu, int N, int Nz, const CnnPhys &ph,
+                         float primary, float *g0, float *g1, hipStream_t s)
+{
+    if (N % 2 || Nz % 2) fail(ETYPE_ILLEGAL_VALUE, "scatter CNN needs even volume sizes");
+    const int Y = N / 2, X = N / 2, Z = Nz / 2;
+    const long V = (long)Y * X * Z, n = (long)N * N * Nz;
+    ensure_work(V);
+    mean_scale(x, n, psum_, sumBlocks_, scal_, s);
+    hipLaunchKernelGGL(k_cnn_input, dim3(cdiv_u(V, 256)), dim3(256), 0, s, x, mu, N, Nz, scal_,
+                       ph.fDens, inbuf_);
+    forward(inbuf_, Y, X, Z, outbuf_, true, s);
+    hipLaunchKernelGGL(k_cnn_source, dim3(cdiv_u(n, 256)), dim3(256), 0, s, outbuf_, x, mu, N, Nz,
+                       scal_, kappa_, ph.fDens, ph.cntr, primary, g0, g1);
+    CNN_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Training on ESSE-generated pairs
+// ---------------------------------------------------------------------------
+namespace {
+
+struct PhantomGen {
+    std::mt19937 rng;
+    int N, Nz;
+    float pix;
+    PhantomGen(unsigned seed, int n, int nz, float p) : rng(seed), N(n), Nz(nz), pix(p) {}
+    float U(float a, float b) { return std::uniform_real_distribution<float>(a, b)(rng); }
+    Ellipsoid ell(float cy, float cx, float cz, float ay, float ax, float az, float ang, float v)
+    {
+        return Ellipsoid{cy, cx, cz, 1.0f / ay, 1.0f / ax, 1.0f / az, std::cos(ang), std::sin(ang), v};
+    }
+    // activity ellipsoids and attenuation ellipsoids (mu per pixel)
+    void make(std::vector<Ellipsoid> &act, std::vector<Ellipsoid> &mu)
+    {
+        act.clear();
+        mu.clear();
+        const float h = 0.5f * N, hz = 0.5f * Nz;
+        const float by = U(0.28f, 0.44f) * N, bx = U(0.28f, 0.44f) * N, bz = U(0.3f, 0.5f) * Nz;
+        const float cy = h + U(-0.05f, 0.05f) * N, cx = h + U(-0.05f, 0.05f) * N;
+        const float cz = hz + U(-0.05f, 0.05f) * Nz, ang = U(0.0f, 3.14159265f);
+        act.push_back(ell(cy, cx, cz, by, bx, bz, ang, 1.0f));
+        mu.push_back(ell(cy, cx, cz, by, bx, bz, ang, 0.15f * pix));
+        const int nin = 3 + (int)(rng() % 8);
+        for (int k = 0; k < nin; ++k) {
+            const float r = U(0.03f, 0.16f) * N;
+            const float a = U(0.0f, 0.6f), b = U(0.0f, 6.28318f);
+            act.push_back(ell(cy + a * by * std::sin(b), cx + a * bx * std::cos(b),
+                              cz + U(-0.6f, 0.6f) * bz, r * U(0.6f, 1.4f), r * U(0.6f, 1.4f),
+                              r * U(0.6f, 1.4f), U(0.0f, 3.14159f), U(-1.0f, 7.0f)));
+        }
+        // lung-like low-density region
+        const float lr = U(0.08f, 0.16f) * N;
+        mu.push_back(ell(cy + U(-0.3f, 0.3f) * by, cx + U(-0.3f, 0.3f) * bx, cz, lr, lr * U(0.6f, 1.2f),
+                         U(0.2f, 0.4f) * Nz, U(0.0f, 3.14f), -0.10f * pix));
+    }
+};
+
+}  // namespace
+
+ScatterCnn *ScatterCnn::train(const EsseModel &m, const CnnTrainParms &p, CnnTrainStats &st)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    if (!m.enabled) fail(ETYPE_ILLEGAL_VALUE, "scatter CNN: the scatter kernel has scatFac 0");
+    if (m.nterms < 2) fail(ETYPE_ILLEGAL_VALUE, "scatter CNN: the ESSE model needs 2 terms");
+    if (m.c != 1) fail(ETYPE_ILLEGAL_VALUE, "scatter CNN: collapse factor 1 only");
+    if (p.N % 2 || p.Nz % 2) fail(ETYPE_ILLEGAL_VALUE, "scatter CNN: even sizes only");
+    if (p.pairs <= p.val || p.val < 1 || p.batch < 1 || p.epochs < 0)
+        fail(ETYPE_ILLEGAL_VALUE, "scatter CNN: bad training sizes");
+    const int N = p.N, Nz = p.Nz;
+    const long n = (long)N * N * Nz;
+    const int Y = N / 2, X = N / 2, Z = Nz / 2;
+    const long V = (long)Y * X * Z;
+    hipStream_t s;
+    CNN_CHECK(hipStreamCreate(&s));
+    const int GB = 8;   // phantoms per ESSE batch
+    EsseDev esse(m, GB, s, true);
+    const float fMu = m.mu0 - m.muavg;
+    const float fDens = (1.0f / m.mu0) / m.pixsize;
+
+    float *xs = cnn_alloc<float>(n * GB), *mus = cnn_alloc<float>(n * GB), *tmp = cnn_alloc<float>(n);
+    float *terms = cnn_alloc<float>(n * GB * m.nterms);
+    float *scal = cnn_alloc<float>(2);
+    double *psum = nullptr;
+    long nb = 0;
+    float *dsIn = cnn_alloc<float>((size_t)p.pairs * V * 2);
+    float *dsTgt = cnn_alloc<float>((size_t)p.pairs * V * 2);
+    Ellipsoid *dEll = cnn_alloc<Ellipsoid>(64);
+    PhantomGen gen(p.seed, N, Nz, p.pixsize);
+    std::vector<Ellipsoid> ea, em;
+    float kappa = 0.0f;
+    const dim3 fgrid(cdiv_u(n, 256)), cgrid(cdiv_u(V, 256)), blk(256);
+    for (int p0 = 0; p0 < p.pairs; p0 += GB) {
+        const int B = std::min(GB, p.pairs - p0);
+        for (int b = 0; b < B; ++b) {
+            gen.make(ea, em);
+            float *xb = xs + b * n, *mb = mus + b * n;
+            CNN_CHECK(hipMemcpyAsync(dEll, ea.data(), ea.size() * sizeof(Ellipsoid),
+                                     hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(k_phantom, fgrid, blk, 0, s, dEll, (int)ea.size(), N, Nz, xb);
+            // estimates of early OSEM iterations are smooth, of late ones noisy
+            const float sig = gen.U(0.0f, 1.0f) < 0.7f ? gen.U(0.3f, 3.0f) : 0.0f;
+            if (sig > 0) {
+                for (int ax = 0; ax < 3; ++ax) {
+                    hipLaunchKernelGGL(k_blur, fgrid, blk, 0, s, xb, tmp, N, Nz, ax, sig);
+                    CNN_CHECK(hipMemcpyAsync(xb, tmp, n * 4, hipMemcpyDeviceToDevice, s));
+                }
+            }
+            const float eps = gen.U(0.0f, 1.0f) < 0.5f ? gen.U(0.0f, 0.5f) : 0.0f;
+            if (eps > 0) hipLaunchKernelGGL(k_noise, fgrid, blk, 0, s, xb, n, eps, (uint32_t)gen.rng());
+            // the ellipsoid list must stay valid until the kernel has read it
+            CNN_CHECK(hipStreamSynchronize(s));
+            CNN_CHECK(hipMemcpyAsync(dEll, em.data(), em.size() * sizeof(Ellipsoid),
+                                     hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(k_phantom, fgrid, blk, 0, s, dEll, (int)em.size(), N, Nz, mb);
+            CNN_CHECK(hipStreamSynchronize(s));
+        }
+        esse.convolve(xs, B, terms);
+        for (int b = 0; b < B; ++b) {
+            const long q = p0 + b;
+            mean_scale(xs + b * n, n, psum, nb, scal, s);
+            hipLaunchKernelGGL(k_cnn_input, cgrid, blk, 0, s, xs + b * n, mus + b * n, N, Nz, scal,
+                               fDens, dsIn + q * V * 2);
+            if (kappa == 0.0f) {
+                // output scale: both target channels of the first pair at unit rms
+                hipLaunchKernelGGL(k_cnn_target, cgrid, blk, 0, s, terms + (long)b * m.nterms * n,
+                                   terms + ((long)b * m.nterms + 1) * n, N, Nz, scal, fMu, 1.0f,
+                                   dsTgt + q * V * 2);
+                std::vector<float> h(V * 2);
+                CNN_CHECK(hipMemcpyAsync(h.data(), dsTgt + q * V * 2, V * 2 * 4,
+                                         hipMemcpyDeviceToHost, s));
+                CNN_CHECK(hipStreamSynchronize(s));
+                double s0 = 0, s1 = 0;
+                for (long i = 0; i < V; ++i) {
+                    s0 += (double)h[2 * i] * h[2 * i];
+                    s1 += (double)h[2 * i + 1] * h[2 * i + 1];
+                }
+                kappa = s1 > 0 ? (float)std::sqrt(s0 / s1) : 1.0f;
+            }
+            hipLaunchKernelGGL(k_cnn_target, cgrid, blk, 0, s, terms + (long)b * m.nterms * n,
+                               terms + ((long)b * m.nterms + 1) * n, N, Nz, scal, fMu, kappa,
+                               dsTgt + q * V * 2);
+        }
+        CNN_CHECK(hipGetLastError());
+    }
+    CNN_CHECK(hipStreamSynchronize(s));
+    cnn_free(xs);
+    cnn_free(mus);
+    cnn_free(tmp);
+    cnn_free(terms);
+    cnn_free(dEll);
+
+    ScatterCnn *net = new ScatterCnn(kappa, p.seed);
+    const int ntrain = p.pairs - p.val;
+    std::vector<int> order(ntrain);
+    for (int i = 0; i < ntrain; ++i) order[i] = i;
+    std::mt19937 rng(p.seed ^ 0x5bd1e995u);
+    const float gscale = 2.0f / (float)(V * 2) / (float)p.batch;
+    double epochLoss = 0;
+    const long steps = (long)p.epochs * ((ntrain + p.batch - 1) / p.batch);
+    long step = 0;
+    for (int e = 0; e < p.epochs; ++e) {
+        std::shuffle(order.begin(), order.end(), rng);
+        epochLoss = 0;
+        for (int i0 = 0; i0 < ntrain; i0 += p.batch) {
+            const int B = std::min(p.batch, ntrain - i0);
+            for (int b = 0; b < B; ++b) {
+                const long q = order[i0 + b];
+                epochLoss += net->lossGrad(dsIn + q * V * 2, dsTgt + q * V * 2, Y, X, Z, gscale, s);
+            }
+            // cosine decay of the learning rate
+            const float lr = 0.5f * p.lr * (1.0f + std::cos(3.14159265f * (float)step / (float)steps));
+            net->adamStep(lr, s);
+            ++step;
+        }
+        epochLoss /= ntrain;
+    }
+    // held-out relative L2 of the two outputs (inference path)
+    double e0 = 0, t0s = 0, e1 = 0, t1s = 0;
+    std::vector<float> ho(V * 2), ht(V * 2);
+    float *pred = cnn_alloc<float>(V * 2);
+    for (int q = ntrain; q < p.pairs; ++q) {
+        net->forward(dsIn + (long)q * V * 2, Y, X, Z, pred, true, s);
+        CNN_CHECK(hipMemcpyAsync(ho.data(), pred, V * 8, hipMemcpyDeviceToHost, s));
+        CNN_CHECK(hipMemcpyAsync(ht.data(), dsTgt + (long)q * V * 2, V * 8, hipMemcpyDeviceToHost, s));
+        CNN_CHECK(hipStreamSynchronize(s));
+        for (long i = 0; i < V; ++i) {
+            const double d0 = ho[2 * i] - ht[2 * i], d1 = ho[2 * i + 1] - ht[2 * i + 1];
+            e0 += d0 * d0;
+            e1 += d1 * d1;
+            t0s += (double)ht[2 * i] * ht[2 * i];
+            t1s += (double)ht[2 * i + 1] * ht[2 * i + 1];
+        }
+    }
+    cnn_free(pred);
+    cnn_free(dsIn);
+    cnn_free(dsTgt);
+    cnn_free(scal);
+    cnn_free(psum);
+    CNN_CHECK(hipStreamDestroy(s));
+    st.loss = (float)epochLoss;
+    st.valNrmse0 = (float)std::sqrt(e0 / std::max(t0s, 1e-300));
+    st.valNrmse1 = (float)std::sqrt(e1 / std::max(t1s, 1e-300));
+    st.kappa = kappa;
+    st.seconds = std::chrono::duration<float>(std::chrono::steady_clock::now() - t0).count();
+    return net;
+}
+
+}  // namespace irlamd
+
+// ---------------------------------------------------------------------------
+// C API (include/irl_amd.h)
+// ---------------------------------------------------------------------------
+#include "../../include/irl_amd.h"
+#include "cnn.h"
+#include "irlprj.h"
+
+namespace {
+
+int cnn_report(const irlamd::IrlError &e)
+{
+    irlamd::set_error_string(e.msg);
+    irlamd::set_last_error(e.code);
+    fprintf(stderr, "scatter CNN: %s\n", e.msg.c_str());
+    return e.code;
+}
+
+struct CnnFileHeader {
+    char magic[8];       // "IRLCNN1"
+    int32_t nparams;
+    float kappa;
+    int32_t trainN, trainNz;
+    float trainPix;
+    int32_t reserved;
+};
+
+// Device buffer of a host array for the test entry points.
+struct HostDev {
+    float *p = nullptr;
+    explicit HostDev(long n) { CNN_CHECK(hipMalloc((void **)&p, n * sizeof(float))); }
+    ~HostDev() { (void)hipFree(p); }
+};
+
+}  // namespace
+
+extern "C" {
+
+irlamd_cnn *irlamd_cnn_create(float fKappa, unsigned uSeed)
+{
+    irlamd::set_last_error(0);
+    try {
+        auto *h = new irlamd_cnn;
+        h->net.reset(new irlamd::ScatterCnn(fKappa, uSeed));
+        return h;
+    } catch (const irlamd::IrlError &e) {
+        cnn_report(e);
+        return nullptr;
+    }
+}
+
+irlamd_cnn *irlamd_cnn_train(const char *krnl, int N, int Nz, float pix, int pairs, int val,
+                             int epochs, int batch, float lr, unsigned seed, float *stats)
+{
+    irlamd::set_last_error(0);
+    try {
+        if (!krnl) irlamd::fail(irlamd::ETYPE_ILLEGAL_VALUE, "scatter CNN: no kernel file");
+        const irlamd::KrnlData k = irlamd::read_krnl_file(krnl);
+        const irlamd::EsseModel m = irlamd::make_esse_model(k, "2 2 2 3", pix, N, Nz, 1);
+        irlamd::CnnTrainParms p;
+        p.N = N;
+        p.Nz = Nz;
+        p.pixsize = pix;
+        p.pairs = pairs;
+        p.val = val;
+        p.epochs = epochs;
+        p.batch = batch;
+        p.lr = lr;
+        p.seed = seed;
+        irlamd::CnnTrainStats st;
+        auto *h = new irlamd_cnn;
+        h->net.reset(irlamd::ScatterCnn::train(m, p, st));
+        if (stats) {
+            stats[0] = st.loss;
+            stats[1] = st.valNrmse0;
+            stats[2] = st.valNrmse1;
+            stats[3] = st.seconds;
+            stats[4] = st.kappa;
+        }
+        return h;
+    } catch (const irlamd::IrlError &e) {
+        cnn_report(e);
+        return nullptr;
+    }
+}
+
+int irlamd_cnn_save(irlamd_cnn *h, const char *fname)
+{
+    irlamd::set_last_error(0);
+    try {
+        if (!h || !fname) irlamd::fail(irlamd::ETYPE_ILLEGAL_VALUE, "scatter CNN: bad argument");
+        std::vector<float> w;
+        h->net->getParams(w);
+        CnnFileHeader hd{};
+        std::memcpy(hd.magic, "IRLCNN1", 8);
+        hd.nparams = (int32_t)w.size();
+        hd.kappa = h->net->kappa();
+        FILE *f = fopen(fname, "wb");
+        if (!f) irlamd::fail(irlamd::ETYPE_IO, "scatter CNN: cannot write %s", fname);
+        const bool ok = fwrite(&hd, sizeof hd, 1, f) == 1 &&
+                        fwrite(w.data(), sizeof(float), w.size(), f) == w.size();
+        fclose(f);
+        if (!ok) irlamd::fail(irlamd::ETYPE_IO, "scatter CNN: short write to %s", fname);
+        return 0;
+    } catch (const irlamd::IrlError &e) {
+        return cnn_report(e);
+    }
+}
+
+irlamd_cnn *irlamd_cnn_load(const char *fname)
+{
+    irlamd::set_last_error(0);
+    try {
+        FILE *f = fname ? fopen(fname, "rb") : nullptr;
+        if (!f) irlamd::fail(irlamd::ETYPE_IO, "scatter CNN: cannot open %s", fname ? fname : "");
+        CnnFileHeader hd{};
+        std::vector<float> w;
+        bool ok = fread(&hd, sizeof hd, 1, f) == 1 && std::memcmp(hd.magic, "IRLCNN1", 8) == 0 &&
+                  hd.nparams == irlamd::cnn_num_params();
+        if (ok) {
+            w.resize(hd.nparams);
+            ok = fread(w.data(), sizeof(float), w.size(), f) == w.size();
+        }
+        fclose(f);
+        if (!ok) irlamd::fail(irlamd::ETYPE_IO, "scatter CNN: %s is not a network file", fname);
+        auto *h = new irlamd_cnn;
+        h->net.reset(new irlamd::ScatterCnn(hd.kappa, w));
+        return h;
+    } catch (const irlamd::IrlError &e) {
+        cnn_report(e);
+        return nullptr;
+    }
+}
+
+void irlamd_cnn_free(irlamd_cnn *h) { delete h; }
+
+int irlamd_cnn_num_params(void) { return (int)irlamd::cnn_num_params(); }
+
+int irlamd_cnn_get_params(irlamd_cnn *h, float *out, float *kappa)
+{
+    irlamd::set_last_error(0);
+    try {
+        std::vector<float> w;
+        h->net->getParams(w);
+        std::memcpy(out, w.data(), w.size() * sizeof(float));
+        if (kappa) *kappa = h->net->kappa();
+        return 0;
+    } catch (const irlamd::IrlError &e) {
+        return cnn_report(e);
+    }
+}
+
+int irlamd_cnn_set_params(irlamd_cnn *h, const float *in)
+{
+    irlamd::set_last_error(0);
+    try {
+        h->net->setParams(std::vector<float>(in, in + irlamd::cnn_num_params()));
+        return 0;
+    } catch (const irlamd::IrlError &e) {
+        return cnn_report(e);
+    }
+}
+
+int irlamd_cnn_forward(irlamd_cnn *h, const float *in, int Y, int X, int Z, int bf16, float *out)
+{
+    irlamd::set_last_error(0);
+    try {
+        if (Y < 1 || X < 1 || Z < 1) irlamd::fail(irlamd::ETYPE_ILLEGAL_VALUE, "bad CNN grid");
+        const long n = (long)Y * X * Z * 2;
+        HostDev di(n), dout(n);
+        CNN_CHECK(hipMemcpy(di.p, in, n * 4, hipMemcpyHostToDevice));
+        h->net->forward(di.p, Y, X, Z, dout.p, bf16 != 0, nullptr);
+        CNN_CHECK(hipMemcpy(out, dout.p, n * 4, hipMemcpyDeviceToHost));
+        return 0;
+    } catch (const irlamd::IrlError &e) {
+        return cnn_report(e);
+    }
+}
+
+int irlamd_cnn_loss_grad(irlamd_cnn *h, const float *in, const float *tgt, int Y, int X, int Z,
+                         double *loss, float *grads)
+{
+    irlamd::set_last_error(0);
+    try {
+        if (Y < 1 || X < 1 || Z < 1) irlamd::fail(irlamd::ETYPE_ILLEGAL_VALUE, "bad CNN grid");
+        const long n = (long)Y * X * Z * 2;
+        HostDev di(n), dt(n);
+        CNN_CHECK(hipMemcpy(di.p, in, n * 4, hipMemcpyHostToDevice));
+        CNN_CHECK(hipMemcpy(dt.p, tgt, n * 4, hipMemcpyHostToDevice));
+        h->net->zeroGrads(nullptr);
+        const double l = h->net->lossGrad(di.p, dt.p, Y, X, Z, 2.0f / (float)n, nullptr);
+        if (loss) *loss = l;
+        std::vector<float> g;
+        h->net->getGrads(g);
+        std::memcpy(grads, g.data(), g.size() * sizeof(float));
+        h->net->zeroGrads(nullptr);
+        CNN_CHECK(hipDeviceSynchronize());
+        return 0;
+    } catch (const irlamd::IrlError &e) {
+        return cnn_report(e);
+    }
+}
+
+int irlamd_cnn_sources(irlamd_cnn *h, const float *x, const float *mu, int N, int Nz,
+                       float fDens, float fCntr, float primary, float *g0, float *g1)
+{
+    irlamd::set_last_error(0);
+    try {
+        if (N < 2 || Nz < 2) irlamd::fail(irlamd::ETYPE_ILLEGAL_VALUE, "bad volume size");
+        const long n = (long)N * N * Nz;
+        HostDev dx(n), dm(n), d0(n), d1(n);
+        CNN_CHECK(hipMemcpy(dx.p, x, n * 4, hipMemcpyHostToDevice));
+        CNN_CHECK(hipMemcpy(dm.p, mu, n * 4, hipMemcpyHostToDevice));
+        irlamd::CnnPhys ph;
+        ph.fDens = fDens;
+        ph.cntr = fCntr;
+        h->net->sources(dx.p, dm.p, N, Nz, ph, primary, d0.p, d1.p, nullptr);
+        CNN_CHECK(hipMemcpy(g0, d0.p, n * 4, hipMemcpyDeviceToHost));
+        CNN_CHECK(hipMemcpy(g1, d1.p, n * 4, hipMemcpyDeviceToHost));
+        return 0;
+    } catch (const irlamd::IrlError &e) {
+        return cnn_report(e);
+    }
+}
+
+}  // extern "C"

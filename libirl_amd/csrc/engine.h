@@ -73,6 +73,7 @@ struct Geometry {
 };
 
 struct EsseModel;
+class ScatterCnn;
 
 // Scatter-model handling of one sub-iteration (osemrecon.c:652-676, 734-788).
 enum SrfMode {
@@ -107,7 +108,24 @@ public:
                                            // the estimated per-plane error exceeds 1e-3),
                                            // 2 BF16 hi+lo always, 3 BF16 plain always
         std::shared_ptr<const EsseModel> esse;  // scatter model, or null
+        // Learned scatter: the network's terms replace the per-view FFT
+        // convolutions of the scatter model while the check agrees with the
+        // model (every cnnCheckEvery-th modelled subset runs both and compares
+        // their projected sources, rel. L2 > cnnMaxNrmse switches back to the
+        // model).
+        std::shared_ptr<ScatterCnn> cnn;
+        int cnnCheckEvery = 0;             // 0: check the first modelled subset only
+        float cnnMaxNrmse = 0.01f;
+        int cnnExact = 0;                  // 1: the terms from one FFT convolution instead
     };
+
+    struct CnnStatus {
+        bool configured = false, active = false;
+        int checks = 0, used = 0;          // checked subsets, subsets run on the network
+        float lastNrmse = -1, worstNrmse = 0;   // projected source (primary + scatter)
+        float lastScatterNrmse = -1;            // its scatter part alone
+    };
+    CnnStatus cnnStatus() const;
 
     explicit Recon(const Config &cfg);
     ~Recon();

@@ -43,6 +43,7 @@
 #include <map>
 #include <vector>
 
+#include "cnn_dev.h"
 #include "engine.h"
 #include "esse_dev.h"
 
@@ -361,6 +362,99 @@ __global__ __launch_bounds__(256) void k_att(const float *__restrict__ x,
     }
     if (afac) val = val * afac[(long)v * vol + i];
     att[(long)p.b * vol + i] = val;
+}
+
+// ---------------------------------------------------------------------------
+// k_att_src: learned-scatter source of the batch's views, the first-order
+// form of the ESSE effective source (krnlprj3d.c:836-838, 903-920, 946-947):
+//   att = (rot g0 + d rot g1) afac,  d = -ln(afac) / mu0
+// where -ln(afac) is the half-pixel running attenuation sum the reference's
+// depth loops accumulate (afac = exp(-sum), prjbck.c:212-222).
+// ---------------------------------------------------------------------------
+template <bool SHEAR>
+__global__ __launch_bounds__(256) void k_att_src(const float *__restrict__ g0,
+                                                 const float *__restrict__ g1,
+                                                 const float *__restrict__ afac,
+                                                 float *__restrict__ att, int Nz, int N, int NRot,
+                                                 const int *__restrict__ bview,
+                                                 const float *__restrict__ fsin,
+                                                 const float *__restrict__ fcos,
+                                                 const int *__restrict__ head,
+                                                 const int *__restrict__ npix, ShearDev sd,
+                                                 float invMu0)
+{
+    const PixIdx p = pix_index(Nz, NRot);
+    if (!p.ok) return;
+    const long vol = (long)Nz * NRot * NRot;
+    const int i = (p.Y * NRot + p.X) * Nz + p.z;
+    const int v = bview[p.b];
+    float v0 = 0.0f, v1 = 0.0f;
+    if (SHEAR) {
+        v0 = shear_sample(sd, 2 * v, g0 + p.z, (long)N * Nz, Nz, p.X, p.Y);
+        v1 = shear_sample(sd, 2 * v, g1 + p.z, (long)N * Nz, Nz, p.X, p.Y);
+    } else if (in_circle(head, npix, p.Y, p.X)) {
+        const float NewAxis = 0.5 * (NRot - 1);
+        const float OldAxis = 0.5 * (N - 1);
+        v0 = rot_sample(g0 + p.z, (long)N * Nz, Nz, N, (float)p.X - NewAxis,
+                        (float)p.Y - NewAxis, fsin[v], fcos[v], OldAxis);
+        v1 = rot_sample(g1 + p.z, (long)N * Nz, Nz, N, (float)p.X - NewAxis,
+                        (float)p.Y - NewAxis, fsin[v], fcos[v], OldAxis);
+    }
+    const float af = afac[(long)v * vol + i];
+    const float d = af > 0.0f ? -logf(af) * invMu0 : 0.0f;
+    att[(long)p.b * vol + i] = fmaf(d, v1, v0) * af;
+}
+
+// Source check: per (view, bin, slice) column the depth sums of the model's
+// source attE (its projection before the collimator blur), of its scatter
+// part (attE - rotx afac) and of the difference to the learned source
+// (attE - attC); acc[0] += (sum diff)^2, acc[1] += (sum scatter)^2,
+// acc[2] += (sum attE)^2.
+__global__ __launch_bounds__(256) void k_scatter_check(const float *__restrict__ attE,
+                                                       const float *__restrict__ attC,
+                                                       const float *__restrict__ rotx,
+                                                       const float *__restrict__ afac,
+                                                       const int *__restrict__ bview, int B,
+                                                       int Nz, int NRot,
+                                                       double *__restrict__ acc)
+{
+    __shared__ double red[3][256];
+    const long plane = (long)NRot * Nz, vol = plane * NRot;
+    const long c = blockIdx.x * 256L + threadIdx.x;
+    double sd = 0.0, ss = 0.0, st = 0.0;
+    if (c < plane * B) {
+        const int b = (int)(c / plane);
+        const long q = c % plane;
+        const int v = bview[b];
+        float dsum = 0.0f, ssum = 0.0f, tsum = 0.0f;
+        for (int r = 0; r < NRot; ++r) {
+            const long o = (long)b * vol + r * plane + q;
+            const float e = attE[o];
+            dsum += e - attC[o];
+            ssum += e - rotx[o] * afac[(long)v * vol + r * plane + q];
+            tsum += e;
+        }
+        sd = (double)dsum * dsum;
+        ss = (double)ssum * ssum;
+        st = (double)tsum * tsum;
+    }
+    red[0][threadIdx.x] = sd;
+    red[1][threadIdx.x] = ss;
+    red[2][threadIdx.x] = st;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + w];
+            red[1][threadIdx.x] += red[1][threadIdx.x + w];
+            red[2][threadIdx.x] += red[2][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicAdd(acc, red[0][0]);
+        atomicAdd(acc + 1, red[1][0]);
+        atomicAdd(acc + 2, red[2][0]);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1033,6 +1127,13 @@ struct Recon::Impl {
     DBuf<float> rotx, rotmu, terms, ppart2, prjscat;  // scatter model (rotmu: every view)
     DBuf<int> allViews;                                // 0..V-1
     std::unique_ptr<EsseDev> esse;
+    // learned scatter
+    DBuf<float> g0, g1, att2;
+    DBuf<double> chk;
+    CnnPhys phys;
+    Recon::CnnStatus cst;
+    int cnnSubsets = 0;           // modelled subsets seen
+    bool cnnCheckNow = false;     // this subset runs both sources and compares
     DBuf<float> fsin, fcos, bsin, bcos, vt, vtsens, psf;
     DBuf<int> rowKernel, rowPad, subsets, head, npix, rowNone;
     // BF16 precision variant: Toeplitz A fragments of every table entry
@@ -1120,6 +1221,35 @@ struct Recon::Impl {
             hipLaunchKernelGGL(k_att<false>, pix_grid(g.Nz, g.NRot, g.NRot, B), dim3(256), 0,
                                stream, src ? src : x.p, afac, dst ? dst : att.p, g.Nz, g.N,
                                g.NRot, bview, fsin.p, fcos.p, head.p, npix.p, shear());
+    }
+
+    void run_att_src(const float *afacp, int B, const int *bview, float *dst)
+    {
+        const Geometry &g = c.g;
+        const float inv = 1.0f / phys.mu0;
+        if (c.shears)
+            hipLaunchKernelGGL(k_att_src<true>, pix_grid(g.Nz, g.NRot, g.NRot, B), dim3(256), 0,
+                               stream, g0.p, g1.p, afacp, dst, g.Nz, g.N, g.NRot, bview, fsin.p,
+                               fcos.p, head.p, npix.p, shear(), inv);
+        else
+            hipLaunchKernelGGL(k_att_src<false>, pix_grid(g.Nz, g.NRot, g.NRot, B), dim3(256), 0,
+                               stream, g0.p, g1.p, afacp, dst, g.Nz, g.N, g.NRot, bview, fsin.p,
+                               fcos.p, head.p, npix.p, shear(), inv);
+    }
+
+    // Learned-scatter source volumes g0, g1 from the current estimate: the
+    // network, or (cnnExact) the scatter model's FFT convolution of the
+    // unrotated estimate.
+    void run_cnn_sources()
+    {
+        const Geometry &g = c.g;
+        if (c.cnnExact) {
+            esse->convolve(x.p, 1, terms.p);
+            cnn_exact_sources(terms.p, terms.p + M, x.p, mu.p, g.N, g.Nz, phys, 1.0f, g0.p, g1.p,
+                              stream);
+        } else {
+            c.cnn->sources(x.p, mu.p, g.N, g.Nz, phys, 1.0f, g0.p, g1.p, stream);
+        }
     }
 
     // Attenuated scatter-plus-primary source of the batch's views into att
@@ -1339,6 +1469,22 @@ Recon::Recon(const Config &cfg) : impl_(new Impl), cfg_(cfg)
                 fail(ETYPE_ILLEGAL_VALUE, "scatter model prepared for another volume size");
             I.esse.reset(new EsseDev(*cfg.esse, Vs, I.stream, true));
             I.terms.alloc(I.vol * Vs * cfg.esse->nterms);
+            const EsseModel &em = *cfg.esse;
+            I.cst.configured = cfg.cnn != nullptr || cfg.cnnExact;
+            // the learned source needs the unrotated frame of the model's
+            // grid, two terms, no collapse and the attenuation factors
+            I.cst.active = I.cst.configured && g.N == g.NRot && g.N % 2 == 0 && g.Nz % 2 == 0 &&
+                           em.nterms >= 2 && em.c == 1 && (cfg.prjModel & 1);
+            if (I.cst.active) {
+                I.phys.mu0 = em.mu0;
+                I.phys.fMu = em.mu0 - em.muavg;
+                I.phys.fDens = (1.0f / em.mu0) / em.pixsize;
+                I.phys.cntr = em.cntrPixFac;
+                I.g0.alloc(I.M);
+                I.g1.alloc(I.M);
+                I.att2.alloc(I.vol * Vs);
+                I.chk.alloc(3);
+            }
         }
         I.ppart2.alloc(I.P * I.nchunk * Vs);
         I.prjscat.alloc(I.P * g.V);
@@ -1594,6 +1740,8 @@ void Recon::downloadScatterProjections(float *h) const
     prj_from_dev(d.data(), h, cfg_.g.V, cfg_.g.Nz, cfg_.g.NRot);
 }
 void Recon::synchronize() const { HIP_CHECK(hipStreamSynchronize(impl_->stream)); }
+
+Recon::CnnStatus Recon::cnnStatus() const { return impl_->cst; }
 void Recon::setTiming(bool on) { impl_->timing = on; }
 void Recon::resetTimes()
 {
@@ -1690,7 +1838,37 @@ void Recon::runViews(int s, int first, int Vs, SrfMode srf, bool accumulate, boo
     if (srf != SRF_NONE && !I.prjscat.p)
         fail(ETYPE_ILLEGAL_VALUE, "scatter model requested without a scatter kernel");
 
-    if (model && I.esse)
+    if (model && I.esse && I.cst.active) {
+        if (first == 0) {
+            // once per subset: the source volumes of the current estimate, and
+            // whether this subset also runs the scatter model as the check
+            const int k = I.cnnSubsets++;
+            I.cnnCheckNow = k == 0 || (cfg_.cnnCheckEvery > 0 && k % cfg_.cnnCheckEvery == 0);
+            I.launch("cnn_src", [&] { I.run_cnn_sources(); });
+        }
+        if (I.cnnCheckNow) {
+            I.run_srf_att(afac, Vs, bview);
+            I.launch("cnn_att", [&] { I.run_att_src(afac, Vs, bview, I.att2.p); });
+            HIP_CHECK(hipMemsetAsync(I.chk.p, 0, 3 * sizeof(double), I.stream));
+            hipLaunchKernelGGL(k_scatter_check, dim3(div_up(I.P * Vs, threads)), dim3(threads), 0,
+                               I.stream, I.att.p, I.att2.p, I.rotx.p, afac, bview, Vs, g.Nz, g.NRot,
+                               I.chk.p);
+            double h[3] = {0, 0, 0};
+            HIP_CHECK(hipMemcpyAsync(h, I.chk.p, sizeof(h), hipMemcpyDeviceToHost, I.stream));
+            HIP_CHECK(hipStreamSynchronize(I.stream));
+            // relative L2 of the learned projection (before the collimator
+            // blur) against the model's; of its scatter part for the record
+            const float e = h[2] > 0 ? (float)std::sqrt(h[0] / h[2]) : 0.0f;
+            I.cst.checks++;
+            I.cst.lastNrmse = e;
+            I.cst.lastScatterNrmse = h[1] > 0 ? (float)std::sqrt(h[0] / h[1]) : 0.0f;
+            I.cst.worstNrmse = std::max(I.cst.worstNrmse, e);
+            if (!(e <= cfg_.cnnMaxNrmse)) I.cst.active = false;   // back to the scatter model
+        } else {
+            I.launch("cnn_att", [&] { I.run_att_src(afac, Vs, bview, I.att.p); });
+            if (first == 0) I.cst.used++;
+        }
+    } else if (model && I.esse)
         I.run_srf_att(afac, Vs, bview);
     else if (model)  // scatFac == 0: no effective source, the projected image is x
         I.launch("att", [&] { I.run_att(afac, Vs, bview); });

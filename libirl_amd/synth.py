@@ -153,6 +153,26 @@ def gaussian_mixture_krnl(noffs, ndepths, pixsize, amps=(0.02, 0.006, 0.0015),
     return k.astype(np.float32), kmu.astype(np.float32)
 
 
+def krnl_full_sum(krnl):
+    """Sum of the kernel after the reference mirrors its quadrant onto
+    +-(nXYoffs-1) in both lateral axes (krnlprj3d.c:43-127): offset 0 once,
+    the others twice per axis."""
+    k = np.asarray(krnl, np.float64)
+    w = np.full(k.shape[1], 2.0)
+    w[0] = 1.0
+    return float(np.einsum("dij,i,j->", k, w, w))
+
+
+def scale_krnl_to_fraction(krnl, fraction):
+    """Scales a kernel so that a uniform activity in a uniform mu0 medium has
+    the given scatter fraction eff/(primary + eff) = S/(1 + S), S the
+    mirrored kernel sum (the effective source of such a medium is S times
+    the activity)."""
+    s = krnl_full_sum(krnl)
+    target = fraction / (1.0 - fraction)
+    return (np.asarray(krnl, np.float64) * (target / s)).astype(np.float32)
+
+
 # Workload definitions of the benchmark configurations.
 CONFIGS = {
     "c1": dict(n=64, nz=64, pix=0.4, views=60, vs=15, iters=2, orbit="circular", cfcr=25.0,
