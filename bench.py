@@ -56,10 +56,7 @@ def build_workload(tmpdir, seed=2, name="c2"):
     if "krnl_offs" in cfg:
         # ESSE kernel (33^3 support): Gaussian mixture, mu0 = 0.15/cm, scaled
         # to a 30% scatter fraction
-        k, km = synth.gaussian_mixture_krnl(cfg["krnl_offs"], cfg["krnl_depths"], pix)
-        k = synth.scale_krnl_to_fraction(k, 0.3)
-        krnl_file = os.path.join(tmpdir, name + ".krnl")
-        synth.write_krnl(krnl_file, k, km, 0.15, pix)
+        krnl_file = synth.esse_kernel_file(os.path.join(tmpdir, name + ".krnl"), cfg)
         parms.iNumSrfIterations = cfg["srf_iters"]
     prj = np.zeros((V, nz, n), np.float32)
     rc = L.irl_genprj(parms, opts, views, drf_file, krnl_file, None, act.copy(), mu.copy(), prj)
@@ -70,19 +67,23 @@ def build_workload(tmpdir, seed=2, name="c2"):
                 act=act, offs=offs, psfs=psfs, start=start, cfg=cfg, krnl_file=krnl_file)
 
 
-def time_subiterations(w, steps, warmup, precision="fp32"):
+def time_subiterations(w, steps, warmup, precision="fp32", cnn=None, check_every=0):
     """Device-resident reconstruction of the workload: 'warmup' untimed
     sub-iterations, then exactly 'steps' timed ones with per-kernel HIP events
-    on the engine's stream.  Returns (recon, seconds of the timed steps,
+    on the engine's stream.  With 'cnn' the scatter network replaces the
+    per-view ESSE convolutions (checked against ESSE every 'check_every'
+    subsets).  Returns (recon, seconds of the timed steps,
     {kernel: (ms_total, launches)}, estimate after the timed steps)."""
     import libirl_amd as L
     L.set_adv_options(0, 0)
     prev = L.set_precision(precision)
+    L.set_scatter_cnn(cnn, check_every=check_every, max_nrmse=0.01)
     try:
         rec = L.Recon(w["parms"], w["opts"], w["views"], w["drf_file"], w["krnl_file"], None,
                       w["mu"], w["prj"], None)
     finally:
         L.set_precision(prev)
+        L.set_scatter_cnn(None)
     S = rec.num_subsets
     for k in range(warmup):
         rec.subiteration(k % S)
@@ -158,12 +159,14 @@ def drf_roofline(w, rec, stages, steps, warmup):
     return dom, avg_flops / (dom_ms / dom_n * 1e-3) / 1e12
 
 
-def workload_line(w, steps, warmup, precision, ref_estimate=None):
+def workload_line(w, steps, warmup, precision, ref_estimate=None, cnn=None, check_every=0):
     """Sub-object of the JSON line for one workload and DRF arithmetic."""
-    rec, dt, stages, est = time_subiterations(w, steps, warmup, precision)
+    import libirl_amd as L
+    rec, dt, stages, est = time_subiterations(w, steps, warmup, precision, cnn, check_every)
     dom, achieved = drf_roofline(w, rec, stages, steps, warmup)
     esse = w["krnl_file"] is not None
-    name = "osem-128^3x120v-atn-drf31-%s8subsets" % ("esse33-" if esse else "")
+    name = "osem-128^3x120v-atn-drf31-%s8subsets" % (
+        ("cnn-scatter-" if cnn is not None else "esse33-") if esse else "")
     bf16 = precision != "fp32"
     peak = PEAK_BF16_TFLOPS if bf16 else PEAK_FP32_TFLOPS
     line = {"workload": name + (" (BF16 MFMA DRF, FP32 accumulation)" if bf16 else " (FP32)"),
@@ -178,6 +181,9 @@ def workload_line(w, steps, warmup, precision, ref_estimate=None):
                                 np.linalg.norm(ref_estimate))}
     else:
         line["_estimate"] = est
+    if cnn is not None:
+        line["cnn"] = L.cnn_status(rec)
+        line["cnn"]["check_every_subsets"] = check_every
     rec.close()
     return line
 
@@ -235,6 +241,26 @@ def measured_traffic(stage):
         return None
 
 
+CNN_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libirl_amd", "weights",
+                           "scatter_cnn_c3.bin")
+
+
+def load_scatter_cnn():
+    """The trained scatter network of the 128^3 ESSE workload, or None."""
+    import libirl_amd as L
+    if not os.path.exists(CNN_WEIGHTS):
+        return None
+    return L.ScatterCnn.load_file(CNN_WEIGHTS)
+
+
+def cnn_training_record():
+    try:
+        with open(CNN_WEIGHTS + ".json") as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -245,6 +271,7 @@ def main():
     ap.add_argument("--only-esse", action="store_true",
                     help="time only the ESSE workload (for profiling runs)")
     ap.add_argument("--no-bf16", action="store_true", help="skip the BF16 precision variant")
+    ap.add_argument("--no-cnn", action="store_true", help="skip the learned-scatter lines")
     ap.add_argument("--no-errors", action="store_true",
                     help="skip the FP64-oracle projection error of the BF16 variant")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -316,12 +343,20 @@ def main():
     if not args.no_esse:
         # Same geometry with ESSE scatter modelled in the projector (the
         # plain-method scatter configuration), FP32 and with the BF16 DRF.
+        # 8 + 24 sub-iterations: the configuration's 4 iterations of 8 subsets.
         we = build_workload(tmpdir, name="c3")
-        out["esse"] = workload_line(we, 16, 8, "fp32")
+        out["esse"] = workload_line(we, 24, 8, "fp32")
+        ref = out["esse"].pop("_estimate")
         if not args.no_bf16:
-            out["esse_bf16"] = workload_line(we, 16, 8, "bf16",
-                                             ref_estimate=out["esse"].pop("_estimate"))
-        out["esse"].pop("_estimate", None)
+            out["esse_bf16"] = workload_line(we, 24, 8, "bf16", ref_estimate=ref)
+        net = load_scatter_cnn()
+        if net is not None and not args.no_cnn:
+            # Learned scatter + BF16 DRF, checked against ESSE every 8 subsets
+            for prec, key in (("bf16", "cnn_bf16"), ("fp32", "cnn_fp32")):
+                line = workload_line(we, 24, 8, prec, ref_estimate=ref, cnn=net, check_every=8)
+                line["speedup_vs_esse_fp32"] = line["value"] / out["esse"]["value"]
+                line["training"] = cnn_training_record()
+                out[key] = line
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w)
     print(json.dumps(out))

@@ -72,10 +72,11 @@ int irlamd_set_precision(int iMode);
  * krnlprj3d.c:826-948) from the current estimate and the attenuation map;
  * each view's source is then their first-order combination with the view's
  * attenuation depth.  The per-view ESSE stays the fallback and the check:
- * the first scatter-modelled subset (and every iCheckEvery-th one after it)
- * runs both, and a relative L2 difference of their projected sources
- * (primary + scatter, summed over depth before the collimator blur) above
- * fMaxNrmse switches the reconstruction back to ESSE.
+ * on the first scatter-modelled subset (and every iCheckEvery-th one after
+ * it) ESSE also runs for the subset's first view, and a relative L2
+ * difference of the two projected sources (primary + scatter, summed over
+ * depth before the collimator blur) above fMaxNrmse switches the
+ * reconstruction back to ESSE, from that subset on.
  * Handles are independent of reconstructions; the network is shared (kept
  * alive) by every reconstruction created while it is set.
  * ------------------------------------------------------------------------- */

@@ -17,8 +17,10 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <random>
+#include <type_traits>
 
 #include "cnn_dev.h"
 #include "esse_dev.h"
@@ -189,90 +191,132 @@ __global__ __launch_bounds__(256) void k_conv_valu(const TI *__restrict__ in,
             for (int co = 0; co < CO; ++co) acc[co] = fmaf(w[ci * CO + co], a, acc[co]);
         }
     }
+    if constexpr (std::is_same<TO, uint16_t>::value && CO % 8 == 0) {
+        // bf16 outputs, 8 channels per 128-bit store
 #pragma unroll
-    for (int co = 0; co < CO; ++co) {
-        float val = acc[co];
-        if (relu) val = fmaxf(val, 0.0f);
-        if (mask && !(mask[v * CO + co] > 0.0f)) val = 0.0f;
-        from_f(out[v * CO + co], val);
+        for (int c8 = 0; c8 < CO; c8 += 8) {
+            uint32_t q[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                float v0 = acc[c8 + 2 * k], v1 = acc[c8 + 2 * k + 1];
+                if (relu) {
+                    v0 = fmaxf(v0, 0.0f);
+                    v1 = fmaxf(v1, 0.0f);
+                }
+                uint16_t h0, h1;
+                from_f(h0, v0);
+                from_f(h1, v1);
+                q[k] = (uint32_t)h0 | ((uint32_t)h1 << 16);
+            }
+            *(uint4 *)(out + v * CO + c8) = make_uint4(q[0], q[1], q[2], q[3]);
+        }
+    } else {
+#pragma unroll
+        for (int co = 0; co < CO; ++co) {
+            float val = acc[co];
+            if (relu) val = fmaxf(val, 0.0f);
+            if (mask && !(mask[v * CO + co] > 0.0f)) val = 0.0f;
+            from_f(out[v * CO + co], val);
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
-// Implicit-GEMM convolution 32 -> 32 channels on MFMA (inference).
+// Implicit-GEMM convolution of 32 input channels on MFMA (inference).
 // out^T[co][voxel] = sum_tap W_tap[co][ci] in^T[ci][voxel + off]: per tap one
-// K = 32 step; A = weights (bf16 [27][32 co][32 ci] staged in LDS, lane l
-// holds co = l%16 (+16), ci = 8(l/16)..+7), B = 16 voxels' channels (lane l:
-// voxel l%16, ci = 8(l/16)..+7, one 128-bit load).  A wave owns NT tiles of
-// 16 consecutive voxels; C lane l, element i = (co 4(l/16)+i, voxel l%16).
+// K = 32 step; A = weights (bf16 [27][16 NCT co][32 ci], lane l holds
+// co = l%16 (+16c), ci = 8(l/16)..+7), B = 16 voxels' channels (lane l:
+// voxel l%16, ci = 8(l/16)..+7, 128 bits); C lane l, element i = (co
+// 4(l/16)+i, voxel l%16).  A wave owns 64 consecutive slices of one (y, x)
+// row.  For each of the 9 in-plane tap offsets the wave stages the
+// neighbour row's slices [z0 - dil, z0 + 64 + dil) in LDS once (one
+// coalesced pass), and the three slice taps of that offset read their B
+// fragments from there: 9 instead of 27 global passes over the input.  A
+// fragments (identical for every wave) come from the caches.
+// NCT = 2: 32 outputs, bias + ReLU, bf16 [V][32]; NCT = 1 with COUT = 2:
+// the linear output layer, fp32 [V][2] (rows 2..15 of A are zero).
 // ---------------------------------------------------------------------------
-constexpr int CNN_NT = 4;
+constexpr int CNN_ROWZ = 64;
 
-__global__ __launch_bounds__(256) void k_conv_mfma(const uint16_t *__restrict__ in,
-                                                   const uint4 *__restrict__ Wg,
-                                                   const float *__restrict__ bias,
-                                                   uint16_t *__restrict__ out, int Y, int X,
-                                                   int Z, int dil, int relu)
+template <int NCT, int COUT>
+__global__ __launch_bounds__(256) void k_conv_mfma_rows(const uint16_t *__restrict__ in,
+                                                        const uint4 *__restrict__ Wg,
+                                                        const float *__restrict__ bias,
+                                                        void *__restrict__ outp, int Y, int X,
+                                                        int Z, int dil, int relu)
 {
-    __shared__ uint4 wl[27 * 32 * 32 * 2 / 16];
-    for (int i = threadIdx.x; i < 27 * 128; i += 256) wl[i] = Wg[i];
-    __syncthreads();
-    const long V = (long)Y * X * Z;
+    constexpr int WQ = 16 * NCT * 32 * 2 / 16;   // uint4 per tap
+    extern __shared__ uint4 seg[];               // [4 waves][ROWZ + 2 dil][4]
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int n = lane & 15, g = lane >> 4;
-    const long base = ((long)blockIdx.x * 4 + w) * (16 * CNN_NT);
-    int vy[CNN_NT], vx[CNN_NT], vz[CNN_NT];
+    const int zt = (Z + CNN_ROWZ - 1) / CNN_ROWZ;
+    const long task = (long)blockIdx.x * 4 + w;
+    const long rows = (long)Y * X;
+    const bool valid = task < rows * zt;
+    const long row = valid ? task / zt : 0;
+    const int z0 = valid ? (int)(task % zt) * CNN_ROWZ : 0;
+    const int y = (int)(row / X), x = (int)(row % X);
+    const int SL = CNN_ROWZ + 2 * dil;
+    uint4 *my = seg + (long)w * SL * 4;
+    cfloatx4_t acc[NCT][4];
 #pragma unroll
-    for (int j = 0; j < CNN_NT; ++j) {
-        const long v = base + 16 * j + n;
-        const long vc = v < V ? v : V - 1;
-        vz[j] = (int)(vc % Z);
-        const long r = vc / Z;
-        vx[j] = (int)(r % X);
-        vy[j] = v < V ? (int)(r / X) : -(1 << 20);   // invalid voxels read nothing
-    }
-    cfloatx4_t acc[2][CNN_NT];
+    for (int c = 0; c < NCT; ++c)
 #pragma unroll
-    for (int c = 0; c < 2; ++c)
-#pragma unroll
-        for (int j = 0; j < CNN_NT; ++j) acc[c][j] = cfloatx4_t{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int t = 0; t < 27; ++t) {
-        int dy, dx, dz;
-        tap_off(t, dil, dy, dx, dz);
-        const cbf16x8_t A0 = __builtin_bit_cast(cbf16x8_t, wl[t * 128 + n * 4 + g]);
-        const cbf16x8_t A1 = __builtin_bit_cast(cbf16x8_t, wl[t * 128 + (16 + n) * 4 + g]);
-        uint4 braw[CNN_NT];
-#pragma unroll
-        for (int j = 0; j < CNN_NT; ++j) {
-            const int yy = vy[j] + dy, xx = vx[j] + dx, zz = vz[j] + dz;
-            const bool ok = yy >= 0 && yy < Y && xx >= 0 && xx < X && zz >= 0 && zz < Z;
-            braw[j] = ok ? *(const uint4 *)(in + (((long)yy * X + xx) * Z + zz) * 32 + 8 * g)
-                         : make_uint4(0, 0, 0, 0);
+        for (int j = 0; j < 4; ++j) acc[c][j] = cfloatx4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int q = 0; q < 9; ++q) {
+        const int yy = y + dil * (q / 3 - 1), xx = x + dil * (q % 3 - 1);
+        const bool rok = valid && yy >= 0 && yy < Y && xx >= 0 && xx < X;
+        const uint16_t *src = in + ((long)(rok ? yy : 0) * X + (rok ? xx : 0)) * Z * 32;
+        __syncthreads();   // the previous offset's readers are done
+        for (int e = lane; e < SL * 4; e += 64) {
+            const int vz = z0 - dil + (e >> 2);
+            my[e] = (rok && vz >= 0 && vz < Z) ? *(const uint4 *)(src + (long)vz * 32 + 8 * (e & 3))
+                                               : make_uint4(0, 0, 0, 0);
         }
+        __syncthreads();
 #pragma unroll
-        for (int j = 0; j < CNN_NT; ++j) {
-            const cbf16x8_t B = __builtin_bit_cast(cbf16x8_t, braw[j]);
-            acc[0][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A0, B, acc[0][j], 0, 0, 0);
-            acc[1][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1, B, acc[1][j], 0, 0, 0);
-        }
-    }
+        for (int dzi = 0; dzi < 3; ++dzi) {
+            const int t = 3 * q + dzi;
+            cbf16x8_t A[NCT];
 #pragma unroll
-    for (int j = 0; j < CNN_NT; ++j) {
-        const long v = base + 16 * j + n;
-        if (v >= V) continue;
+            for (int c = 0; c < NCT; ++c)
+                A[c] = __builtin_bit_cast(cbf16x8_t, Wg[t * WQ + (16 * c + n) * 4 + g]);
 #pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            const int co = 16 * c + 4 * g;
-            uint16_t h[4];
+            for (int j = 0; j < 4; ++j) {
+                const cbf16x8_t B =
+                    __builtin_bit_cast(cbf16x8_t, my[(16 * j + n + dzi * dil) * 4 + g]);
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                float val = acc[c][j][i] + bias[co + i];
-                if (relu) val = fmaxf(val, 0.0f);
-                from_f(h[i], val);
+                for (int c = 0; c < NCT; ++c)
+                    acc[c][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[c], B, acc[c][j], 0, 0, 0);
             }
-            *(uint2 *)(out + v * 32 + co) =
-                make_uint2((uint32_t)h[0] | ((uint32_t)h[1] << 16),
-                           (uint32_t)h[2] | ((uint32_t)h[3] << 16));
+        }
+    }
+    if (!valid) return;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int z = z0 + 16 * j + n;
+        if (z >= Z) continue;
+        const long v = row * Z + z;
+        if constexpr (COUT == 2) {
+            if (g == 0)
+                *(float2 *)((float *)outp + v * 2) =
+                    make_float2(acc[0][j][0] + bias[0], acc[0][j][1] + bias[1]);
+        } else {
+            uint16_t *out = (uint16_t *)outp;
+#pragma unroll
+            for (int c = 0; c < NCT; ++c) {
+                const int co = 16 * c + 4 * g;
+                uint16_t h[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    float val = acc[c][j][i] + bias[co + i];
+                    if (relu) val = fmaxf(val, 0.0f);
+                    from_f(h[i], val);
+                }
+                *(uint2 *)(out + v * 32 + co) =
+                    make_uint2((uint32_t)h[0] | ((uint32_t)h[1] << 16),
+                               (uint32_t)h[2] | ((uint32_t)h[3] << 16));
+            }
         }
     }
 }
@@ -403,14 +447,16 @@ __global__ void k_flip(const float *__restrict__ W, float *__restrict__ Wt, int 
     Wt[((long)(26 - t) * CO + co) * CI + ci] = W[i];
 }
 
-// BF16 copy of master weights [27][CI][CO] in MFMA order [27][CO][CI].
+// BF16 copy of master weights [27][CI][CO] in MFMA order [27][COP][CI], rows
+// co >= CO of the padded COP zero.
 __global__ void k_to_bf16_t(const float *__restrict__ W, uint16_t *__restrict__ Wb, int CI,
-                            int CO)
+                            int CO, int COP)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= 27 * CI * CO) return;
-    const int t = i / (CI * CO), r = i % (CI * CO), ci = r / CO, co = r % CO;
-    from_f(Wb[((long)t * CO + co) * CI + ci], W[i]);
+    if (i >= 27 * CI * COP) return;
+    const int t = i / (CI * COP), r = i % (CI * COP), co = r / CI, ci = r % CI;
+    const float v = co < CO ? W[((long)t * CI + ci) * CO + co] : 0.0f;
+    from_f(Wb[i], v);
 }
 
 // FP32 weights rounded to BF16 values (the VALU layers of the inference path).
@@ -604,7 +650,7 @@ void ScatterCnn::alloc_state()
         vW_[l] = cnn_alloc<float>(nw);
         mb_[l] = cnn_alloc<float>(nb);
         vb_[l] = cnn_alloc<float>(nb);
-        Wbf_[l] = cnn_alloc<uint16_t>(nw);
+        Wbf_[l] = cnn_alloc<uint16_t>(27L * cnn_cin(l) * (cnn_cout(l) < 16 ? 16 : cnn_cout(l)));
         CNN_CHECK(hipMemset(gW_[l], 0, nw * 4));
         CNN_CHECK(hipMemset(gb_[l], 0, nb * 4));
         CNN_CHECK(hipMemset(mW_[l], 0, nw * 4));
@@ -613,6 +659,7 @@ void ScatterCnn::alloc_state()
         CNN_CHECK(hipMemset(vb_[l], 0, nb * 4));
     }
     Wt_ = cnn_alloc<float>(27L * 32 * 32);
+    Wr0_ = cnn_alloc<float>(wcount(0));
     scal_ = cnn_alloc<float>(2);
     dsum_ = cnn_alloc<double>(1);
 }
@@ -654,6 +701,7 @@ ScatterCnn::~ScatterCnn()
     for (auto &a : grad_) cnn_free(a);
     for (auto &a : actb_) cnn_free(a);
     cnn_free(Wt_);
+    cnn_free(Wr0_);
     cnn_free(inbuf_);
     cnn_free(outbuf_);
     cnn_free(scal_);
@@ -736,14 +784,19 @@ void ScatterCnn::ensure_work(long V)
     workV_ = V;
 }
 
+// Inference weights: BF16 MFMA operands of the 32-input layers (the output
+// layer padded to 16 rows) and the BF16-rounded FP32 weights of layer 1.
 void ScatterCnn::upload_bf16_weights(hipStream_t s)
 {
     if (!bfDirty_) return;
-    for (int l = 0; l < CNN_LAYERS; ++l) {
-        const long nw = wcount(l);
-        hipLaunchKernelGGL(k_to_bf16_t, dim3(cdiv_u(nw, 256)), dim3(256), 0, s, W_[l], Wbf_[l],
-                           cnn_cin(l), cnn_cout(l));
+    for (int l = 1; l < CNN_LAYERS; ++l) {
+        const int COP = cnn_cout(l) < 16 ? 16 : cnn_cout(l);
+        const long n = 27L * cnn_cin(l) * COP;
+        hipLaunchKernelGGL(k_to_bf16_t, dim3(cdiv_u(n, 256)), dim3(256), 0, s, W_[l], Wbf_[l],
+                           cnn_cin(l), cnn_cout(l), COP);
     }
+    hipLaunchKernelGGL(k_round_bf16, dim3(cdiv_u(wcount(0), 256)), dim3(256), 0, s, W_[0], Wr0_,
+                       wcount(0));
     CNN_CHECK(hipGetLastError());
     bfDirty_ = false;
 }
@@ -773,21 +826,17 @@ void ScatterCnn::forward(const float *in, int Y, int X, int Z, float *out, bool 
         return;
     }
     upload_bf16_weights(s);
-    // BF16-rounded FP32 weights of the VALU layers: L1 in Wt_, L4 after it
-    float *w1 = Wt_, *w4 = Wt_ + wcount(0);
-    hipLaunchKernelGGL(k_round_bf16, dim3(cdiv_u(wcount(0), 256)), blk, 0, s, W_[0], w1,
-                       wcount(0));
-    hipLaunchKernelGGL(k_round_bf16, dim3(cdiv_u(wcount(3), 256)), blk, 0, s, W_[3], w4,
-                       wcount(3));
-    hipLaunchKernelGGL((k_conv_valu<32, float, uint16_t>), grid, blk, 0, s, in, w1, b_[0],
+    hipLaunchKernelGGL((k_conv_valu<32, float, uint16_t>), grid, blk, 0, s, in, Wr0_, b_[0],
                        nullptr, actb_[0], Y, X, Z, 2, cnn_dil(0), 1);
-    const dim3 mgrid(cdiv_u(V, 4 * 16 * CNN_NT));
-    hipLaunchKernelGGL(k_conv_mfma, mgrid, blk, 0, s, actb_[0], (const uint4 *)Wbf_[1], b_[1],
-                       actb_[1], Y, X, Z, cnn_dil(1), 1);
-    hipLaunchKernelGGL(k_conv_mfma, mgrid, blk, 0, s, actb_[1], (const uint4 *)Wbf_[2], b_[2],
-                       actb_[0], Y, X, Z, cnn_dil(2), 1);
-    hipLaunchKernelGGL((k_conv_valu<2, uint16_t, float>), grid, blk, 0, s, actb_[0], w4, b_[3],
-                       nullptr, out, Y, X, Z, 32, cnn_dil(3), 0);
+    const long tasks = (long)Y * X * ((Z + CNN_ROWZ - 1) / CNN_ROWZ);
+    const dim3 rgrid(cdiv_u(tasks, 4));
+    auto lds = [](int dil) { return (size_t)4 * (CNN_ROWZ + 2 * dil) * 4 * sizeof(uint4); };
+    hipLaunchKernelGGL((k_conv_mfma_rows<2, 32>), rgrid, blk, lds(cnn_dil(1)), s, actb_[0],
+                       (const uint4 *)Wbf_[1], b_[1], (void *)actb_[1], Y, X, Z, cnn_dil(1), 1);
+    hipLaunchKernelGGL((k_conv_mfma_rows<2, 32>), rgrid, blk, lds(cnn_dil(2)), s, actb_[1],
+                       (const uint4 *)Wbf_[2], b_[2], (void *)actb_[0], Y, X, Z, cnn_dil(2), 1);
+    hipLaunchKernelGGL((k_conv_mfma_rows<1, 2>), rgrid, blk, lds(cnn_dil(3)), s, actb_[0],
+                       (const uint4 *)Wbf_[3], b_[3], (void *)out, Y, X, Z, cnn_dil(3), 0);
     CNN_CHECK(hipGetLastError());
 }
 
@@ -1061,6 +1110,8 @@ ScatterCnn *ScatterCnn::train(const EsseModel &m, const CnnTrainParms &p, CnnTra
             ++step;
         }
         epochLoss /= ntrain;
+        if (std::getenv("IRLAMD_CNN_VERBOSE"))
+            fprintf(stderr, "scatter CNN: epoch %d loss %.4g\n", e + 1, epochLoss);
     }
     // held-out relative L2 of the two outputs (inference path)
     double e0 = 0, t0s = 0, e1 = 0, t1s = 0;

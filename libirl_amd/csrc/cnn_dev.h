@@ -95,7 +95,8 @@ private:
     float *gW_[CNN_LAYERS] = {}, *gb_[CNN_LAYERS] = {};
     float *mW_[CNN_LAYERS] = {}, *vW_[CNN_LAYERS] = {}, *mb_[CNN_LAYERS] = {}, *vb_[CNN_LAYERS] = {};
     float *Wt_ = nullptr;            // flipped/transposed weights for the input gradient
-    uint16_t *Wbf_[CNN_LAYERS] = {}; // bf16 [27][Cout][Cin] (MFMA layers)
+    uint16_t *Wbf_[CNN_LAYERS] = {}; // bf16 [27][max(Cout,16)][Cin] (MFMA layers 2-4)
+    float *Wr0_ = nullptr;           // layer 1 weights rounded to bf16 values
     bool bfDirty_ = true;
     // work buffers for one sample
     long workV_ = 0;

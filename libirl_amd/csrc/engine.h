@@ -110,9 +110,9 @@ public:
         std::shared_ptr<const EsseModel> esse;  // scatter model, or null
         // Learned scatter: the network's terms replace the per-view FFT
         // convolutions of the scatter model while the check agrees with the
-        // model (every cnnCheckEvery-th modelled subset runs both and compares
-        // their projected sources, rel. L2 > cnnMaxNrmse switches back to the
-        // model).
+        // model (every cnnCheckEvery-th modelled subset also runs the model on
+        // its first view and compares the projected sources; rel. L2 >
+        // cnnMaxNrmse switches back to the model).
         std::shared_ptr<ScatterCnn> cnn;
         int cnnCheckEvery = 0;             // 0: check the first modelled subset only
         float cnnMaxNrmse = 0.01f;

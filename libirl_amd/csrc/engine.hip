@@ -1133,7 +1133,7 @@ struct Recon::Impl {
     CnnPhys phys;
     Recon::CnnStatus cst;
     int cnnSubsets = 0;           // modelled subsets seen
-    bool cnnCheckNow = false;     // this subset runs both sources and compares
+    bool cnnCheckNow = false;     // this subset is checked against the scatter model
     DBuf<float> fsin, fcos, bsin, bcos, vt, vtsens, psf;
     DBuf<int> rowKernel, rowPad, subsets, head, npix, rowNone;
     // BF16 precision variant: Toeplitz A fragments of every table entry
@@ -1254,12 +1254,12 @@ struct Recon::Impl {
 
     // Attenuated scatter-plus-primary source of the batch's views into att
     // (prjbck.c:129-176, 219-222): rotate x and mu, ESSE, combine.
-    void run_srf_att(const float *afacp, int B, const int *bview)
+    void run_srf_att(const float *afacp, int B, const int *bview, float *dst = nullptr)
     {
         launch("rot_x", [&] { run_att(nullptr, B, bview, x.p, rotx.p); });
         launch("esse_conv", [&] { esse->convolve(rotx.p, B, terms.p); });
         launch("esse_source", [&] {
-            esse->source(rotx.p, rotmu.p, terms.p, B, 1.0f, afacp, bview, att.p, true);
+            esse->source(rotx.p, rotmu.p, terms.p, B, 1.0f, afacp, bview, dst ? dst : att.p, true);
         });
     }
 
@@ -1846,12 +1846,13 @@ void Recon::runViews(int s, int first, int Vs, SrfMode srf, bool accumulate, boo
             I.cnnCheckNow = k == 0 || (cfg_.cnnCheckEvery > 0 && k % cfg_.cnnCheckEvery == 0);
             I.launch("cnn_src", [&] { I.run_cnn_sources(); });
         }
-        if (I.cnnCheckNow) {
-            I.run_srf_att(afac, Vs, bview);
-            I.launch("cnn_att", [&] { I.run_att_src(afac, Vs, bview, I.att2.p); });
+        I.launch("cnn_att", [&] { I.run_att_src(afac, Vs, bview, I.att.p); });
+        if (I.cnnCheckNow && first == 0) {
+            // the scatter model on the subset's first view as the check
+            I.run_srf_att(afac, 1, bview, I.att2.p);
             HIP_CHECK(hipMemsetAsync(I.chk.p, 0, 3 * sizeof(double), I.stream));
-            hipLaunchKernelGGL(k_scatter_check, dim3(div_up(I.P * Vs, threads)), dim3(threads), 0,
-                               I.stream, I.att.p, I.att2.p, I.rotx.p, afac, bview, Vs, g.Nz, g.NRot,
+            hipLaunchKernelGGL(k_scatter_check, dim3(div_up(I.P, threads)), dim3(threads), 0,
+                               I.stream, I.att2.p, I.att.p, I.rotx.p, afac, bview, 1, g.Nz, g.NRot,
                                I.chk.p);
             double h[3] = {0, 0, 0};
             HIP_CHECK(hipMemcpyAsync(h, I.chk.p, sizeof(h), hipMemcpyDeviceToHost, I.stream));
@@ -1863,11 +1864,13 @@ void Recon::runViews(int s, int first, int Vs, SrfMode srf, bool accumulate, boo
             I.cst.lastNrmse = e;
             I.cst.lastScatterNrmse = h[1] > 0 ? (float)std::sqrt(h[0] / h[1]) : 0.0f;
             I.cst.worstNrmse = std::max(I.cst.worstNrmse, e);
-            if (!(e <= cfg_.cnnMaxNrmse)) I.cst.active = false;   // back to the scatter model
-        } else {
-            I.launch("cnn_att", [&] { I.run_att_src(afac, Vs, bview, I.att.p); });
-            if (first == 0) I.cst.used++;
+            if (!(e <= cfg_.cnnMaxNrmse)) {
+                // back to the scatter model, from this subset on
+                I.cst.active = false;
+                I.run_srf_att(afac, Vs, bview);
+            }
         }
+        if (first == 0 && I.cst.active) I.cst.used++;
     } else if (model && I.esse)
         I.run_srf_att(afac, Vs, bview);
     else if (model)  // scatFac == 0: no effective source, the projected image is x

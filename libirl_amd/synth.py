@@ -173,6 +173,16 @@ def scale_krnl_to_fraction(krnl, fraction):
     return (np.asarray(krnl, np.float64) * (target / s)).astype(np.float32)
 
 
+def esse_kernel_file(fname, cfg, fraction=0.3):
+    """KRNL file of a workload configuration's ESSE kernel (Gaussian mixture,
+    mu0 = 0.15/cm, krnl_offs x krnl_offs x krnl_depths one-quadrant support)
+    scaled to the given scatter fraction."""
+    k, km = gaussian_mixture_krnl(cfg["krnl_offs"], cfg["krnl_depths"], cfg["pix"])
+    k = scale_krnl_to_fraction(k, fraction)
+    write_krnl(fname, k, km, 0.15, cfg["pix"])
+    return fname
+
+
 # Workload definitions of the benchmark configurations.
 CONFIGS = {
     "c1": dict(n=64, nz=64, pix=0.4, views=60, vs=15, iters=2, orbit="circular", cfcr=25.0,

@@ -166,7 +166,7 @@ def test_irlosem_first_order_source_with_exact_terms():
         L.set_scatter_cnn_exact(False)
     e = rel_l2(got, ref)
     print("rel L2 vs FP64 ESSE %.3g, status %s" % (e, st))
-    assert st["checks"] == 1 and st["active"] and st["used"] == 11
+    assert st["checks"] == 1 and st["active"] and st["used"] == 12
     assert st["last_nrmse"] < 0.01
     assert e < 0.02
 
@@ -194,5 +194,5 @@ def test_trained_network_gains_over_no_scatter_model():
     plain = _gpu_osem(pb, f, 0, model_srf=False)
     e_cnn, e_none = rel_l2(got, ref), rel_l2(plain, ref)
     print("rel L2 vs FP64 ESSE: CNN %.3g, no scatter model %.3g, status %s" % (e_cnn, e_none, cs))
-    assert cs["used"] == 11
+    assert cs["used"] == 12
     assert e_cnn < 0.25 * e_none
