@@ -710,7 +710,8 @@ __device__ __forceinline__ uint32_t pack_bf16(__bf16 lo, __bf16 hi)
 template <int KY, int NZT, bool BCK, bool SPLIT>
 __global__ __launch_bounds__(256) void k_drf_bf16(DrfArgs a, const uint4 *__restrict__ afr,
                                                   const uint4 *__restrict__ afrLo,
-                                                  const int *__restrict__ distSplit)
+                                                  const int *__restrict__ distSplit,
+                                                  const int4 *__restrict__ band)
 {
     extern __shared__ uint32_t ldsw[];
     constexpr int CY = (KY - 1) / 2;
@@ -760,9 +761,16 @@ __global__ __launch_bounds__(256) void k_drf_bf16(DrfArgs a, const uint4 *__rest
             const int pad = a.rowPad[(long)v * a.nrows + row];
             // hi+lo split for the planes whose estimated BF16 error is too large
             const bool sp = SPLIT && distSplit[kidx];
+            // the entry's band: PSF columns [i0, i1] and window rows from ks
+            // (multiple of 8) over nks K steps; outside it the A fragments are zero
+            const int4 bd = band[kidx];
+            const int i0 = bd.x, i1 = bd.y, ks = bd.z, nks = bd.w;
+            const int srows = 16 * (NZT - 1) + 32 * nks;   // staged rows from ks
+            const int c0 = KX - 1 - i1, nc = DRF_TX + i1 - i0;   // staged columns
             // stage pairs of consecutive slices as packed bf16 (hi, and lo)
-            for (int e = threadIdx.x; e < (THP / 2) * TW; e += 256) {
-                const int c = e / (THP / 2), q = 2 * (e - c * (THP / 2));
+            for (int e = threadIdx.x; e < (srows / 2) * nc; e += 256) {
+                const int cc = e / (srows / 2), c = c0 + cc;
+                const int q = ks + 2 * (e - cc * (srows / 2));
                 const float f0 = drf_stage_val(a, b, v, row, z0 - CY + q, x0 - CX + c, pad);
                 const float f1 = drf_stage_val(a, b, v, row, z0 - CY + q + 1, x0 - CX + c, pad);
                 const __bf16 h0 = (__bf16)f0, h1 = (__bf16)f1;
@@ -775,7 +783,7 @@ __global__ __launch_bounds__(256) void k_drf_bf16(DrfArgs a, const uint4 *__rest
             const uint4 *ap = afr + (long)kidx * KX * NKS * 64 + lane;
             const uint4 *apl = afrLo + (long)kidx * KX * NKS * 64 + lane;
             if (sp) {
-                for (int i = 0; i < KX; ++i) {
+                for (int i = i0; i <= i1; ++i) {
                     bf16x8_t A[NKS], AL[NKS];
 #pragma unroll
                     for (int s = 0; s < NKS; ++s) {
@@ -783,12 +791,13 @@ __global__ __launch_bounds__(256) void k_drf_bf16(DrfArgs a, const uint4 *__rest
                         AL[s] = __builtin_bit_cast(bf16x8_t, apl[(i * NKS + s) * 64]);
                     }
                     const int col = 16 * w + n + KX - 1 - i;
-                    const uint32_t *ch = tileH + (col * CSB + 8 * g) / 2;
-                    const uint32_t *cl = tileL + (col * CSB + 8 * g) / 2;
+                    const uint32_t *ch = tileH + (col * CSB + ks + 8 * g) / 2;
+                    const uint32_t *cl = tileL + (col * CSB + ks + 8 * g) / 2;
 #pragma unroll
                     for (int t = 0; t < NZT; ++t)
 #pragma unroll
                         for (int s = 0; s < NKS; ++s) {
+                            if (s >= nks) break;
                             const bf16x8_t B = __builtin_bit_cast(
                                 bf16x8_t, *(const uint4 *)(ch + (16 * t + 32 * s) / 2));
                             const bf16x8_t BL = __builtin_bit_cast(
@@ -798,21 +807,37 @@ __global__ __launch_bounds__(256) void k_drf_bf16(DrfArgs a, const uint4 *__rest
                             acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(AL[s], B, acc[t], 0, 0, 0);
                         }
                 }
+            } else if (nks == 1) {
+                // one K step: the NZT B fragments of a column are read once
+                bf16x8_t A = __builtin_bit_cast(bf16x8_t, ap[i0 * NKS * 64]), An = A;
+                for (int i = i0; i <= i1; ++i) {
+                    if (i < i1) An = __builtin_bit_cast(bf16x8_t, ap[(i + 1) * NKS * 64]);
+                    const int col = 16 * w + n + KX - 1 - i;
+                    const uint32_t *ch = tileH + (col * CSB + ks + 8 * g) / 2;
+                    bf16x8_t B[NZT];
+#pragma unroll
+                    for (int u = 0; u < NZT; ++u)
+                        B[u] = __builtin_bit_cast(bf16x8_t, *(const uint4 *)(ch + 8 * u));
+#pragma unroll
+                    for (int t = 0; t < NZT; ++t)
+                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[t], acc[t], 0, 0, 0);
+                    A = An;
+                }
             } else {
-                // the window of tile t, step s starts at row 16(t + 2s): the
-                // NZT + 2(NKS-1) distinct B fragments of a column are read once
+                // the window of tile t, step s starts at row ks + 16(t + 2s):
+                // the NZT + 2(NKS-1) distinct B fragments of a column are read once
                 constexpr int NB = NZT + 2 * (NKS - 1);
                 bf16x8_t A[NKS], An[NKS];
 #pragma unroll
-                for (int s = 0; s < NKS; ++s) A[s] = __builtin_bit_cast(bf16x8_t, ap[s * 64]);
-                for (int i = 0; i < KX; ++i) {
-                    if (i + 1 < KX) {
+                for (int s = 0; s < NKS; ++s) A[s] = __builtin_bit_cast(bf16x8_t, ap[(i0 * NKS + s) * 64]);
+                for (int i = i0; i <= i1; ++i) {
+                    if (i < i1) {
 #pragma unroll
                         for (int s = 0; s < NKS; ++s)
                             An[s] = __builtin_bit_cast(bf16x8_t, ap[((i + 1) * NKS + s) * 64]);
                     }
                     const int col = 16 * w + n + KX - 1 - i;
-                    const uint32_t *ch = tileH + (col * CSB + 8 * g) / 2;
+                    const uint32_t *ch = tileH + (col * CSB + ks + 8 * g) / 2;
                     bf16x8_t B[NB];
 #pragma unroll
                     for (int u = 0; u < NB; ++u)
@@ -1010,7 +1035,7 @@ static int div_up(long a, long b) { return (int)((a + b - 1) / b); }
 static void launch_drf(hipStream_t stream, bool bck, DrfArgs args, int B, int KYb);
 static void launch_drf_bf16(hipStream_t stream, bool bck, DrfArgs args, int B, int KYb,
                             bool split, const uint4 *afr, const uint4 *afrLo,
-                            const int *distSplit);
+                            const int *distSplit, const int4 *band);
 
 template <typename T>
 struct DBuf {
@@ -1139,6 +1164,7 @@ struct Recon::Impl {
     // BF16 precision variant: Toeplitz A fragments of every table entry
     DBuf<uint4> afrHi, afrLo;
     DBuf<int> distSplit;          // per table entry: hi+lo split
+    DBuf<int4> drfBand;           // per table entry: kept PSF columns, window shift, K steps
     DBuf<int> rowKernel16;        // rowKernel with -1 replaced by the delta entry nDist
     DBuf<int> rowKernel16B;       // the same for the backprojector's rows
     DBuf<int> rowKernelB, rowPadB;   // backprojector's table rows (its own table, or copies)
@@ -1358,7 +1384,7 @@ struct Recon::Impl {
     {
         if (bf16)
             launch_drf_bf16(stream, bck, args, B, KYb, bf16Split, afrHi.p, afrLo.p,
-                            distSplit.p);
+                            distSplit.p, drfBand.p);
         else
             launch_drf(stream, bck, args, B, KYb);
     }
@@ -1383,29 +1409,88 @@ static float bf2f(uint16_t h)
 }
 
 // A fragments of k_drf_bf16 for every table entry, column i and K step s:
-// lane l holds A_i[m = l%16][k = 32s + 8(l/16) + e], e < 8, with A_i[m][k] =
-// psf[m + KYb - 1 - k][i] inside the band (PSF centred in the KYb bucket),
-// as bf16 hi parts and lo = bf16(value - hi).  est[k] is the estimated
-// relative error of a plane convolved with entry k in plain BF16: 1.6e-3
-// (the std of the relative rounding of a product of two bf16-rounded
-// operands) times |psf|_2 / |psf|_1.
-static void build_bf16_fragments(const PaddedDrf &d, int KYb, std::vector<uint4> &hi,
-                                 std::vector<uint4> &lo, std::vector<double> &est)
+// lane l holds A_i[m = l%16][k = ks + 32s + 8(l/16) + e], e < 8, with
+// A_i[m][k] = psf[m + KYb - 1 - k][i] inside the band (PSF centred in the
+// KYb bucket), as bf16 hi parts and lo = bf16(value - hi).
+// Each entry is first cut to the smallest rectangle of its PSF outside which
+// the taps' absolute sum is at most 'cut' of the whole (edges with the
+// smallest sums are dropped first; cut = 0 drops only all-zero edges); band[k] = (first, last PSF column, window
+// shift ks, K steps): when the rectangle's rows fit one 32-row K step from a
+// shift ks (a multiple of 8, so 128-bit LDS reads stay aligned) the kernel
+// runs that single step, else all NKS steps from ks = 0.
+// est[k] is the estimated relative error of a plane convolved with entry k
+// in plain BF16: 1.6e-3 (the std of the relative rounding of a product of
+// two bf16-rounded operands) times |psf|_2 / |psf|_1 of the kept taps, plus
+// the dropped fraction.
+static void build_bf16_fragments(const PaddedDrf &d, int KYb, double cut, std::vector<uint4> &hi,
+                                 std::vector<uint4> &lo, std::vector<double> &est,
+                                 std::vector<int4> &band)
 {
     const int NKS = drf16_nks(KYb), oy = (KYb - d.KY) / 2;
     hi.assign((size_t)d.nDist * d.KX * NKS * 64, uint4{0, 0, 0, 0});
     lo = hi;
-    for (int k = 0; k < d.nDist; ++k)
+    est.assign(d.nDist, 0.0);
+    band.assign(d.nDist, int4{0, d.KX - 1, 0, NKS});
+    for (int k = 0; k < d.nDist; ++k) {
+        const float *P = &d.psf[(size_t)k * d.KY * d.KX];
+        double total = 0;
+        for (int t = 0; t < d.KX * d.KY; ++t) total += fabs(P[t]);
+        int r0 = 0, r1 = d.KY - 1, c0 = 0, c1 = d.KX - 1;
+        double dropped = 0;
+        for (;;) {
+            auto rsum = [&](int j) {
+                double s = 0;
+                for (int i = c0; i <= c1; ++i) s += fabs(P[j * d.KX + i]);
+                return s;
+            };
+            auto csum = [&](int i) {
+                double s = 0;
+                for (int j = r0; j <= r1; ++j) s += fabs(P[j * d.KX + i]);
+                return s;
+            };
+            double best = 1e300;
+            int which = -1;
+            if (r1 > r0) {
+                const double a = rsum(r0), b = rsum(r1);
+                if (a < best) best = a, which = 0;
+                if (b < best) best = b, which = 1;
+            }
+            if (c1 > c0) {
+                const double a = csum(c0), b = csum(c1);
+                if (a < best) best = a, which = 2;
+                if (b < best) best = b, which = 3;
+            }
+            if (which < 0 || dropped + best > cut * total) break;
+            dropped += best;
+            if (which == 0) ++r0;
+            else if (which == 1) --r1;
+            else if (which == 2) ++c0;
+            else --c1;
+        }
+        const int klo = KYb - 1 - oy - r1, khi = 15 + KYb - 1 - oy - r0;
+        int ks = (klo / 8) * 8, nks = 1;
+        if (khi - ks >= 32 || ks + 32 > 32 * NKS) {
+            ks = 0;
+            nks = NKS;
+        }
+        band[k] = int4{c0, c1, ks, nks};
+        double l1 = 0, l2 = 0;
+        for (int j = r0; j <= r1; ++j)
+            for (int i = c0; i <= c1; ++i) {
+                const double p = P[j * d.KX + i];
+                l1 += fabs(p);
+                l2 += p * p;
+            }
+        est[k] = (l1 > 0 ? 1.6e-3 * sqrt(l2) / l1 : 0.0) + (total > 0 ? dropped / total : 0.0);
         for (int i = 0; i < d.KX; ++i)
             for (int st = 0; st < NKS; ++st)
                 for (int l = 0; l < 64; ++l) {
                     uint16_t h[8], w[8];
                     for (int e = 0; e < 8; ++e) {
-                        const int m = l % 16, kk = 32 * st + 8 * (l / 16) + e;
+                        const int m = l % 16, kk = ks + 32 * st + 8 * (l / 16) + e;
                         const int j = m + KYb - 1 - kk - oy;
-                        const float val = (j >= 0 && j < d.KY)
-                                              ? d.psf[((size_t)k * d.KY + j) * d.KX + i]
-                                              : 0.0f;
+                        const float val =
+                            (j >= r0 && j <= r1 && i >= c0 && i <= c1) ? P[j * d.KX + i] : 0.0f;
                         h[e] = f2bf(val);
                         w[e] = f2bf(val - bf2f(h[e]));
                     }
@@ -1413,15 +1498,6 @@ static void build_bf16_fragments(const PaddedDrf &d, int KYb, std::vector<uint4>
                     memcpy(&hi[o], h, 16);
                     memcpy(&lo[o], w, 16);
                 }
-    est.assign(d.nDist, 0.0);
-    for (int k = 0; k < d.nDist; ++k) {
-        double l1 = 0, l2 = 0;
-        for (int t = 0; t < d.KX * d.KY; ++t) {
-            const double p = d.psf[(size_t)k * d.KX * d.KY + t];
-            l1 += fabs(p);
-            l2 += p * p;
-        }
-        est[k] = l1 > 0 ? 1.6e-3 * sqrt(l2) / l1 : 0.0;
     }
 }
 
@@ -1536,7 +1612,12 @@ Recon::Recon(const Config &cfg) : impl_(new Impl), cfg_(cfg)
             d.nDist = nd + 1;
             std::vector<uint4> hi, lo;
             std::vector<double> est;
-            build_bf16_fragments(d, KYb, hi, lo, est);
+            std::vector<int4> band;
+            // the always-split mode keeps every nonzero tap; the others drop
+            // PSF edges holding at most 1e-4 of the plane's weight
+            build_bf16_fragments(d, KYb, cfg.precision == 2 ? 0.0 : 1e-4, hi, lo, est, band);
+            I.drfBand.alloc(band.size());
+            I.drfBand.upload(band.data(), band.size(), I.stream);
             std::vector<int> split(nd + 1, 0), used(nd + 1, 0), rk16 = cfg.rowKernel,
                                                rk16B = cfg.rowKernelBck;
             if (rk16.empty()) rk16.assign((size_t)g.V * g.NRot, -1);
@@ -2397,7 +2478,7 @@ static void launch_drf(hipStream_t stream, bool bck, DrfArgs args, int B, int KY
 
 static void launch_drf_bf16(hipStream_t stream, bool bck, DrfArgs args, int B, int KYb,
                             bool split, const uint4 *afr, const uint4 *afrLo,
-                            const int *distSplit)
+                            const int *distSplit, const int4 *band)
 {
     if (args.zs != 1 || args.outZS != 1 || args.Nz % 4)
         fail(ETYPE_ILLEGAL_VALUE, "the BF16 DRF path needs slice-fastest planes, Nz %% 4 == 0");
@@ -2412,7 +2493,7 @@ static void launch_drf_bf16(hipStream_t stream, bool bck, DrfArgs args, int B, i
     dim3 grid((unsigned)nblocks), block(256);
 #define IRL_DRF16_LAUNCH(K, Z, BK, SP) \
     hipLaunchKernelGGL((k_drf_bf16<K, Z, BK, SP>), grid, block, lds, stream, args, afr, afrLo, \
-                       distSplit)
+                       distSplit, band)
 #define IRL_DRF16_CASE(K)                                                            \
     case K:                                                                          \
         if (NZT == 8) {                                                              \
