@@ -65,6 +65,37 @@ int irlamd_last_error(void);
  * NumSlices % 4 == 0.  Returns the previous mode, or -1 for an invalid one. */
 int irlamd_set_precision(int iMode);
 
+/* Reconstructions created afterwards replay each sub-iteration from a
+ * captured HIP graph from the second occurrence of its subset and scatter
+ * mode on (same kernels, arguments and order as the eager launches; subsets
+ * with a learned-scatter check and timed runs stay eager).  Returns the
+ * previous setting. */
+int irlamd_set_graphs(int bOn);
+
+/* Reconstructions created afterwards time the candidate tile shapes of the
+ * DRF kernels on their first subset at setup and keep the fastest (results
+ * do not depend on the tiles).  Returns the previous setting. */
+int irlamd_set_autotune(int bOn);
+/* Tile shapes in use (0: the size heuristic): FP32 slice groups per block or
+ * BF16 slice tiles per wave, of the projector and the backprojector. */
+int irlamd_recon_tiles(irlamd_recon *r, int *piPrj, int *piBck);
+
+/* Several reconstructions (any sizes and view counts) stepped together: each
+ * step runs the next sub-iteration of every member (subsets in order, the
+ * iteration and scatter schedule of each as in IrlOsem).  Members whose
+ * sub-iteration can be replayed are launched as child graphs of one parent
+ * graph, concurrently, in one launch per step; the others (first occurrence
+ * of a subset and mode, learned-scatter checks) run eagerly on their own
+ * streams.  The members stay owned by the caller and must outlive the
+ * batch; results equal those of stepping each member alone. */
+typedef struct irlamd_batch irlamd_batch;
+irlamd_batch *irlamd_batch_create(irlamd_recon **pRecons, int iNum);
+int irlamd_batch_step(irlamd_batch *b);
+int irlamd_batch_synchronize(irlamd_batch *b);
+/* Parent-graph launches and eagerly run sub-iterations so far. */
+int irlamd_batch_stats(irlamd_batch *b, int *piGraphLaunches, int *piEager);
+void irlamd_batch_destroy(irlamd_batch *b);
+
 /* ---------------------------------------------------------------------------
  * Learned scatter (no counterpart in the reference).  A small 3-D CNN on the
  * half-resolution grid predicts, once per sub-iteration, the two FFT

@@ -223,6 +223,16 @@ def load():
     lib.irlamd_set_scatter_cnn_exact.argtypes = [ctypes.c_int]
     lib.irlamd_recon_cnn_status.argtypes = [vp] + [ctypes.POINTER(ctypes.c_int)] * 3 + \
         [ctypes.POINTER(ctypes.c_float)] * 3
+    lib.irlamd_set_graphs.argtypes = [ctypes.c_int]
+    lib.irlamd_set_autotune.argtypes = [ctypes.c_int]
+    lib.irlamd_recon_tiles.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    lib.irlamd_batch_create.restype = vp
+    lib.irlamd_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int]
+    lib.irlamd_batch_step.argtypes = [vp]
+    lib.irlamd_batch_synchronize.argtypes = [vp]
+    lib.irlamd_batch_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    lib.irlamd_batch_destroy.argtypes = [vp]
+    lib.irlamd_batch_destroy.restype = None
     _lib = lib
     return lib
 
@@ -441,6 +451,12 @@ class Recon:
         self._check(self._lib.irlamd_recon_get_norm(self._h, s, fptr(out)))
         return out
 
+    def tiles(self):
+        """DRF tile shapes in use (projector, backprojector); 0 = size heuristic."""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        self._lib.irlamd_recon_tiles(self._h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
     def set_timing(self, on=True):
         self._lib.irlamd_recon_set_timing(self._h, 1 if on else 0)
 
@@ -599,3 +615,51 @@ def cnn_status(recon=None):
     return {"active": bool(a.value), "checks": c.value, "used": u.value,
             "last_nrmse": last.value, "worst_nrmse": worst.value,
             "last_scatter_nrmse": lsc.value}
+
+
+def set_graphs(on):
+    """Graph replay of sub-iterations for reconstructions created afterwards."""
+    return load().irlamd_set_graphs(1 if on else 0)
+
+
+def set_autotune(on):
+    """DRF tile autotuning at setup for reconstructions created afterwards."""
+    return load().irlamd_set_autotune(1 if on else 0)
+
+
+class Batch:
+    """Reconstructions stepped together (irlamd_batch_*): each step runs the
+    next sub-iteration of every member, the replayable ones in one graph launch."""
+
+    def __init__(self, recons):
+        self._lib = load()
+        self._recons = list(recons)   # members must outlive the batch
+        arr = (ctypes.c_void_p * len(self._recons))(*[r._h for r in self._recons])
+        h = self._lib.irlamd_batch_create(arr, len(self._recons))
+        if not h:
+            raise RuntimeError("irlamd_batch_create failed: " + error_string())
+        self._h = h
+
+    def step(self):
+        if self._lib.irlamd_batch_step(self._h):
+            raise RuntimeError(error_string())
+
+    def synchronize(self):
+        if self._lib.irlamd_batch_synchronize(self._h):
+            raise RuntimeError(error_string())
+
+    def stats(self):
+        g, e = ctypes.c_int(), ctypes.c_int()
+        self._lib.irlamd_batch_stats(self._h, ctypes.byref(g), ctypes.byref(e))
+        return {"graph_launches": g.value, "eager_subiterations": e.value}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.irlamd_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -117,6 +117,7 @@ public:
         int cnnCheckEvery = 0;             // 0: check the first modelled subset only
         float cnnMaxNrmse = 0.01f;
         int cnnExact = 0;                  // 1: the terms from one FFT convolution instead
+        bool autotune = false;             // autotuneTiles() at precompute
     };
 
     struct CnnStatus {
@@ -145,8 +146,29 @@ public:
     // per-view output before the 1/NumViews scaling), [view][slice][bin].
     void forwardProject(float primary, float *out_host, bool srf = false);
 
-    // One OSEM sub-iteration over subset s (osemrecon.c:687-863 body).
+    // One OSEM sub-iteration over subset s (osemrecon.c:687-863 body).  With
+    // setGraphs(true) a sub-iteration is replayed from a captured graph from
+    // the second occurrence of its (subset, mode) on.
     void subIteration(int s, bool keepUpdate, SrfMode srf = SRF_NONE);
+    void setGraphs(bool on);
+
+    // Pieces of subIteration for drivers of several reconstructions: the
+    // learned-scatter bookkeeping around a sub-iteration (beginSubset
+    // returns true when the subset must run eagerly: its check reads a
+    // result on the host), the launches themselves, and the launches
+    // captured into a graph (a hipGraph_t, not executed; owned by the caller).
+    bool beginSubset(SrfMode srf);
+    void endSubset(SrfMode srf);
+    void runSubIteration(int s, SrfMode srf) { runViews(s, 0, cfg_.Vs, srf, false, true, false); }
+    void *captureSubIteration(int s, bool keepUpdate, SrfMode srf);
+    void *streamHandle() const;       // the engine's hipStream_t
+    bool learnedScatterActive() const;
+
+    // Tile shapes of the DRF kernels chosen by timing the candidates on
+    // subset 0 (slice groups per block of the FP32 kernel, slice tiles per
+    // wave of the BF16 kernel), for the projector and the backprojector.
+    void autotuneTiles();
+    void tiles(int *prj, int *bck) const;
 
     // One view at a time (for per-angle hooks): zeroUpdate at the start of
     // the subset, then member m of subset s is projected and backprojected
@@ -186,6 +208,26 @@ private:
     struct Impl;
     Impl *impl_;
     Config cfg_;
+};
+
+// Several reconstructions stepped together: one sub-iteration of each per
+// step, the ones that can be replayed launched as child graphs of one parent
+// graph (one launch per step), the others (first occurrence of their
+// subset and mode, learned-scatter checks) eagerly on their own streams.
+class ReconBatch {
+public:
+    explicit ReconBatch(const std::vector<Recon *> &recons);
+    ~ReconBatch();
+    ReconBatch(const ReconBatch &) = delete;
+    ReconBatch &operator=(const ReconBatch &) = delete;
+    void step(const std::vector<int> &subset, const std::vector<SrfMode> &srf);
+    void synchronize();
+    int graphLaunches() const;       // parent-graph launches so far
+    int eagerSubIterations() const;  // sub-iterations run eagerly so far
+
+private:
+    struct Impl;
+    Impl *impl_;
 };
 
 // One view of the projector-level entry points.  Tables come from the host

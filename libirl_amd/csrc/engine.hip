@@ -41,6 +41,7 @@
 #include <algorithm>
 #include <type_traits>
 #include <map>
+#include <set>
 #include <vector>
 
 #include "cnn_dev.h"
@@ -1032,10 +1033,10 @@ __global__ void k_norm_input(int B, int Nz, int NRot, const int *__restrict__ bv
 // ---------------------------------------------------------------------------
 static int div_up(long a, long b) { return (int)((a + b - 1) / b); }
 
-static void launch_drf(hipStream_t stream, bool bck, DrfArgs args, int B, int KYb);
+static void launch_drf(hipStream_t stream, bool bck, DrfArgs args, int B, int KYb, int tile = 0);
 static void launch_drf_bf16(hipStream_t stream, bool bck, DrfArgs args, int B, int KYb,
                             bool split, const uint4 *afr, const uint4 *afrLo,
-                            const int *distSplit, const int4 *band);
+                            const int *distSplit, const int4 *band, int tile = 0);
 
 template <typename T>
 struct DBuf {
@@ -1143,6 +1144,17 @@ struct ShearSet {
     }
 };
 
+struct GraphKey {
+    int s, srf, keep, learned;
+    bool operator<(const GraphKey &o) const
+    {
+        if (s != o.s) return s < o.s;
+        if (srf != o.srf) return srf < o.srf;
+        if (keep != o.keep) return keep < o.keep;
+        return learned < o.learned;
+    }
+};
+
 struct Recon::Impl {
     Config c;
     hipStream_t stream = nullptr;
@@ -1159,6 +1171,16 @@ struct Recon::Impl {
     Recon::CnnStatus cst;
     int cnnSubsets = 0;           // modelled subsets seen
     bool cnnCheckNow = false;     // this subset is checked against the scatter model
+    // captured sub-iterations, by (subset, scatter mode, keep U, learned
+    // scatter in use); null until the key's second occurrence
+    bool useGraphs = false;
+    std::map<GraphKey, hipGraphExec_t> graphs;
+    void clearGraphs()
+    {
+        for (auto &kv : graphs)
+            if (kv.second) (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
+    }
     DBuf<float> fsin, fcos, bsin, bcos, vt, vtsens, psf;
     DBuf<int> rowKernel, rowPad, subsets, head, npix, rowNone;
     // BF16 precision variant: Toeplitz A fragments of every table entry
@@ -1380,13 +1402,18 @@ struct Recon::Impl {
         return std::string(base) + suf;
     }
 
-    void run_drf(bool bck, const DrfArgs &args, int B, int KYb)
+    // DRF tile shapes (0: the size heuristic): slice groups per block of the
+    // FP32 kernel, or slice tiles per wave of the BF16 kernel
+    int tilePrj = 0, tileBck = 0;
+
+    void run_drf(bool bck, const DrfArgs &args, int B, int KYb, int tile = -1)
     {
+        if (tile < 0) tile = bck ? tileBck : tilePrj;
         if (bf16)
             launch_drf_bf16(stream, bck, args, B, KYb, bf16Split, afrHi.p, afrLo.p,
-                            distSplit.p, drfBand.p);
+                            distSplit.p, drfBand.p, tile);
         else
-            launch_drf(stream, bck, args, B, KYb);
+            launch_drf(stream, bck, args, B, KYb, tile);
     }
 };
 
@@ -1700,6 +1727,7 @@ Recon::Recon(const Config &cfg) : impl_(new Impl), cfg_(cfg)
 Recon::~Recon()
 {
     if (impl_) {
+        impl_->clearGraphs();
         if (impl_->stream) {
             (void)hipStreamSynchronize(impl_->stream);
             for (auto &kv : impl_->stages)
@@ -1846,6 +1874,7 @@ void Recon::precompute(bool norms)
     Impl &I = *impl_;
     const Geometry &g = cfg_.g;
     const int threads = 256;
+    if (cfg_.autotune && !I.timing) autotuneTiles();
     if (cfg_.hasAtn) {
         const long n = (long)g.V * g.Nz * g.NRot;
         I.launch("afac", [&] {
@@ -1880,14 +1909,271 @@ void Recon::precompute(bool norms)
     synchronize();
 }
 
+// Host bookkeeping of the learned scatter at the start of a subset: counts
+// the modelled subsets and decides whether this one is checked against the
+// scatter model.  Returns true when the subset must run eagerly (a check
+// reads its result on the host).
+bool Recon::beginSubset(SrfMode srf)
+{
+    Impl &I = *impl_;
+    I.cnnCheckNow = false;
+    const bool model = srf == SRF_MODEL || srf == SRF_MODEL_STORE;
+    if (!(model && I.esse && I.cst.active)) return false;
+    const int k = I.cnnSubsets++;
+    I.cnnCheckNow = k == 0 || (cfg_.cnnCheckEvery > 0 && k % cfg_.cnnCheckEvery == 0);
+    return I.cnnCheckNow;
+}
+
+void Recon::endSubset(SrfMode srf)
+{
+    Impl &I = *impl_;
+    const bool model = srf == SRF_MODEL || srf == SRF_MODEL_STORE;
+    if (model && I.esse && I.cst.active) I.cst.used++;
+}
+
 void Recon::subIteration(int s, bool keepUpdate, SrfMode srf)
 {
+    Impl &I = *impl_;
+    const bool eager = beginSubset(srf);
+    const bool learned = I.cst.active;
+    // A captured graph of the sub-iteration replays exactly the launches of
+    // the eager path (same kernels, arguments and order); it is captured on
+    // the second occurrence of its key, when every lazily allocated buffer
+    // exists, and never for checked subsets or while kernels are timed.
+    const GraphKey key{s, (int)srf, keepUpdate ? 1 : 0, learned ? 1 : 0};
+    if (I.useGraphs && !eager && !I.timing) {
+        auto it = I.graphs.find(key);
+        if (it != I.graphs.end()) {
+            if (!it->second) {
+                hipGraph_t gr = (hipGraph_t)captureSubIteration(s, keepUpdate, srf);
+                hipGraphExec_t ex = nullptr;
+                HIP_CHECK(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
+                HIP_CHECK(hipGraphDestroy(gr));
+                it->second = ex;
+            }
+            HIP_CHECK(hipGraphLaunch(it->second, I.stream));
+            endSubset(srf);
+            return;
+        }
+        I.graphs.emplace(key, nullptr);   // first occurrence: eager
+    }
     runViews(s, 0, cfg_.Vs, srf, false, true, keepUpdate);
+    endSubset(srf);
 }
+
+// The launches of one sub-iteration captured into a graph (not executed).
+// Learned-scatter bookkeeping is the caller's (beginSubset / endSubset).
+void *Recon::captureSubIteration(int s, bool keepUpdate, SrfMode srf)
+{
+    Impl &I = *impl_;
+    if (I.timing) fail(ETYPE_ILLEGAL_VALUE, "sub-iteration capture with kernel timing on");
+    const bool check = I.cnnCheckNow;
+    I.cnnCheckNow = false;
+    HIP_CHECK(hipStreamBeginCapture(I.stream, hipStreamCaptureModeThreadLocal));
+    try {
+        runViews(s, 0, cfg_.Vs, srf, false, true, keepUpdate);
+    } catch (...) {
+        hipGraph_t dummy = nullptr;
+        (void)hipStreamEndCapture(I.stream, &dummy);
+        if (dummy) (void)hipGraphDestroy(dummy);
+        I.cnnCheckNow = check;
+        throw;
+    }
+    hipGraph_t g = nullptr;
+    HIP_CHECK(hipStreamEndCapture(I.stream, &g));
+    I.cnnCheckNow = check;
+    return g;
+}
+
+// Tile autotuner: each candidate shape of the DRF kernels is timed with HIP
+// events on the views of subset 0 (three launches after one untimed one, on
+// whatever the work buffers hold: timing does not depend on the values) and
+// the fastest is kept, separately for the projector and the backprojector.
+void Recon::autotuneTiles()
+{
+    Impl &I = *impl_;
+    if (cfg_.drf.nDist <= 0) return;
+    const int KYb = I.ky_bucket();
+    const int *bview = I.subsets.p;
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    for (int bck = 0; bck < 2; ++bck) {
+        const DrfArgs da = I.drf_args(bck != 0, bview, bck ? cfg_.bckModel : cfg_.prjModel);
+        int best = 0;
+        float bestMs = 1e30f;
+        for (int tile : {4, 8}) {
+            I.run_drf(bck != 0, da, cfg_.Vs, KYb, tile);
+            HIP_CHECK(hipEventRecord(e0, I.stream));
+            for (int r = 0; r < 3; ++r) I.run_drf(bck != 0, da, cfg_.Vs, KYb, tile);
+            HIP_CHECK(hipEventRecord(e1, I.stream));
+            HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0;
+            HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < bestMs) {
+                bestMs = ms;
+                best = tile;
+            }
+        }
+        (bck ? I.tileBck : I.tilePrj) = best;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+}
+
+void Recon::tiles(int *prj, int *bck) const
+{
+    *prj = impl_->tilePrj;
+    *bck = impl_->tileBck;
+}
+
+void *Recon::streamHandle() const { return (void *)impl_->stream; }
+
+bool Recon::learnedScatterActive() const { return impl_->cst.active; }
+
+void Recon::setGraphs(bool on)
+{
+    Impl &I = *impl_;
+    I.useGraphs = on;
+    if (!on) I.clearGraphs();
+}
+
+// ---------------------------------------------------------------------------
+// ReconBatch
+// ---------------------------------------------------------------------------
+struct ReconBatch::Impl {
+    std::vector<Recon *> r;
+    hipStream_t stream = nullptr;
+    std::vector<hipEvent_t> ev;                      // one per reconstruction
+    hipEvent_t done = nullptr;
+    // captured sub-iterations of reconstruction i by key (owned graphs)
+    std::vector<std::map<GraphKey, hipGraph_t>> child;
+    std::vector<std::set<GraphKey>> seen;            // keys run eagerly once
+    // parent graphs by topology: the members and their (mode, learned) keys
+    struct Parent {
+        hipGraphExec_t exec = nullptr;
+        std::vector<hipGraphNode_t> node;           // per member
+    };
+    std::map<std::vector<int>, Parent> parents;
+    int launches = 0, eager = 0;
+};
+
+ReconBatch::ReconBatch(const std::vector<Recon *> &recons) : impl_(new Impl)
+{
+    Impl &B = *impl_;
+    B.r = recons;
+    HIP_CHECK(hipStreamCreate(&B.stream));
+    B.ev.resize(recons.size());
+    for (auto &e : B.ev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIP_CHECK(hipEventCreateWithFlags(&B.done, hipEventDisableTiming));
+    B.child.resize(recons.size());
+    B.seen.resize(recons.size());
+}
+
+ReconBatch::~ReconBatch()
+{
+    Impl &B = *impl_;
+    (void)hipStreamSynchronize(B.stream);
+    for (auto &kv : B.parents)
+        if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+    for (auto &m : B.child)
+        for (auto &kv : m) (void)hipGraphDestroy(kv.second);
+    for (auto e : B.ev) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(B.done);
+    (void)hipStreamDestroy(B.stream);
+    delete impl_;
+}
+
+void ReconBatch::step(const std::vector<int> &subset, const std::vector<SrfMode> &srf)
+{
+    Impl &B = *impl_;
+    const int n = (int)B.r.size();
+    if ((int)subset.size() != n || (int)srf.size() != n)
+        fail(ETYPE_ILLEGAL_VALUE, "batch step: %d reconstructions, %d subsets, %d modes", n,
+             (int)subset.size(), (int)srf.size());
+    std::vector<int> members, topo;
+    std::vector<hipGraph_t> graphs;
+    for (int i = 0; i < n; ++i) {
+        Recon &R = *B.r[i];
+        const bool check = R.beginSubset(srf[i]);
+        const GraphKey key{subset[i], (int)srf[i], 0, R.learnedScatterActive() ? 1 : 0};
+        if (check || !B.seen[i].count(key)) {
+            // eager on the reconstruction's own stream, after the last batch launch
+            HIP_CHECK(hipStreamWaitEvent((hipStream_t)R.streamHandle(), B.done, 0));
+            R.runSubIteration(subset[i], srf[i]);
+            R.endSubset(srf[i]);
+            if (!check) B.seen[i].insert(key);
+            B.eager++;
+            continue;
+        }
+        auto it = B.child[i].find(key);
+        if (it == B.child[i].end())
+            it = B.child[i].emplace(key, (hipGraph_t)R.captureSubIteration(subset[i], false, srf[i]))
+                     .first;
+        members.push_back(i);
+        graphs.push_back(it->second);
+        topo.push_back(i);
+        topo.push_back(key.srf);
+        topo.push_back(key.learned);
+    }
+    if (members.empty()) return;
+    // the parent graph of this topology: one child node per member, no edges
+    auto pit = B.parents.find(topo);
+    bool fresh = pit == B.parents.end();
+    if (!fresh) {
+        for (size_t k = 0; k < members.size(); ++k)
+            if (hipGraphExecChildGraphNodeSetParams(pit->second.exec, pit->second.node[k],
+                                                    graphs[k]) != hipSuccess) {
+                (void)hipGetLastError();
+                (void)hipGraphExecDestroy(pit->second.exec);
+                B.parents.erase(pit);
+                fresh = true;
+                break;
+            }
+    }
+    if (fresh) {
+        hipGraph_t parent;
+        HIP_CHECK(hipGraphCreate(&parent, 0));
+        Impl::Parent P;
+        for (auto g : graphs) {
+            hipGraphNode_t node;
+            HIP_CHECK(hipGraphAddChildGraphNode(&node, parent, nullptr, 0, g));
+            P.node.push_back(node);
+        }
+        HIP_CHECK(hipGraphInstantiate(&P.exec, parent, nullptr, nullptr, 0));
+        HIP_CHECK(hipGraphDestroy(parent));
+        pit = B.parents.emplace(topo, P).first;
+    }
+    // the members' earlier work precedes the launch; their later work follows it
+    for (int i : members) {
+        HIP_CHECK(hipEventRecord(B.ev[i], (hipStream_t)B.r[i]->streamHandle()));
+        HIP_CHECK(hipStreamWaitEvent(B.stream, B.ev[i], 0));
+    }
+    HIP_CHECK(hipGraphLaunch(pit->second.exec, B.stream));
+    HIP_CHECK(hipEventRecord(B.done, B.stream));
+    for (int i : members) {
+        HIP_CHECK(hipStreamWaitEvent((hipStream_t)B.r[i]->streamHandle(), B.done, 0));
+        B.r[i]->endSubset(srf[i]);
+    }
+    B.launches++;
+}
+
+void ReconBatch::synchronize()
+{
+    Impl &B = *impl_;
+    HIP_CHECK(hipStreamSynchronize(B.stream));
+    for (auto *r : B.r) r->synchronize();
+}
+
+int ReconBatch::graphLaunches() const { return impl_->launches; }
+int ReconBatch::eagerSubIterations() const { return impl_->eager; }
 
 void Recon::angleStep(int s, int m, SrfMode srf)
 {
+    // per-angle hooks: the learned-scatter bookkeeping runs with the first view
+    if (m == 0) beginSubset(srf);
     runViews(s, m, 1, srf, true, false, true);
+    if (m == cfg_.Vs - 1) endSubset(srf);
 }
 
 void Recon::zeroUpdate()
@@ -1920,13 +2206,9 @@ void Recon::runViews(int s, int first, int Vs, SrfMode srf, bool accumulate, boo
         fail(ETYPE_ILLEGAL_VALUE, "scatter model requested without a scatter kernel");
 
     if (model && I.esse && I.cst.active) {
-        if (first == 0) {
-            // once per subset: the source volumes of the current estimate, and
-            // whether this subset also runs the scatter model as the check
-            const int k = I.cnnSubsets++;
-            I.cnnCheckNow = k == 0 || (cfg_.cnnCheckEvery > 0 && k % cfg_.cnnCheckEvery == 0);
-            I.launch("cnn_src", [&] { I.run_cnn_sources(); });
-        }
+        // once per subset: the source volumes of the current estimate (whether
+        // the subset is also checked was decided by beginSubset)
+        if (first == 0) I.launch("cnn_src", [&] { I.run_cnn_sources(); });
         I.launch("cnn_att", [&] { I.run_att_src(afac, Vs, bview, I.att.p); });
         if (I.cnnCheckNow && first == 0) {
             // the scatter model on the subset's first view as the check
@@ -1951,7 +2233,6 @@ void Recon::runViews(int s, int first, int Vs, SrfMode srf, bool accumulate, boo
                 I.run_srf_att(afac, Vs, bview);
             }
         }
-        if (first == 0 && I.cst.active) I.cst.used++;
     } else if (model && I.esse)
         I.run_srf_att(afac, Vs, bview);
     else if (model)  // scatFac == 0: no effective source, the projected image is x
@@ -2438,10 +2719,11 @@ static size_t drf_lds_bytes(int ZG, int KYb, int KX)
 // Launches k_drf over B batch items.  Eight slice groups (128 slices per
 // block) when the volume has more than 64 slices and two such blocks fit in
 // a CU's 160 KiB of LDS, else four.
-static void launch_drf(hipStream_t stream, bool bck, DrfArgs args, int B, int KYb)
+static void launch_drf(hipStream_t stream, bool bck, DrfArgs args, int B, int KYb, int tile)
 {
     const size_t ldsMax = 160 * 1024;
     int ZG = (args.Nz > 64 && 2 * drf_lds_bytes(8, KYb, args.KX) <= ldsMax) ? 8 : 4;
+    if ((tile == 4 || tile == 8) && drf_lds_bytes(tile, KYb, args.KX) <= ldsMax) ZG = tile;
     const size_t lds = drf_lds_bytes(ZG, KYb, args.KX);
     if (lds > ldsMax) fail(ETYPE_ILLEGAL_VALUE, "DRF PSF too wide (KX=%d)", args.KX);
     args.ztiles = div_up(args.Nz, ZG * DRF_ZT);
@@ -2478,11 +2760,11 @@ static void launch_drf(hipStream_t stream, bool bck, DrfArgs args, int B, int KY
 
 static void launch_drf_bf16(hipStream_t stream, bool bck, DrfArgs args, int B, int KYb,
                             bool split, const uint4 *afr, const uint4 *afrLo,
-                            const int *distSplit, const int4 *band)
+                            const int *distSplit, const int4 *band, int tile)
 {
     if (args.zs != 1 || args.outZS != 1 || args.Nz % 4)
         fail(ETYPE_ILLEGAL_VALUE, "the BF16 DRF path needs slice-fastest planes, Nz %% 4 == 0");
-    const int NZT = args.Nz > 64 ? 8 : 4;
+    const int NZT = tile == 4 || tile == 8 ? tile : args.Nz > 64 ? 8 : 4;
     const int TW = DRF_TX + args.KX - 1;
     const int THP = 16 * (NZT - 1) + 32 * drf16_nks(KYb);
     const size_t lds = (size_t)TW * drf16_cs(THP) * 2 * (split ? 2 : 1);

@@ -1,0 +1,122 @@
+"""Graph replay of sub-iterations, the batch of reconstructions of uneven
+sizes stepped in one parent-graph launch, and the DRF tile autotuner: each
+against the eager path of the same reconstruction, bit for bit (same kernels,
+arguments and order), with eager work and host synchronisation between the
+replays."""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from tests.cases import Problem
+
+pytestmark = pytest.mark.gpu
+MU0 = 0.15
+
+
+def _krnl_file(pix):
+    from libirl_amd import synth
+    k, km = synth.gaussian_mixture_krnl(5, 9, pix, mu0=MU0)
+    k = synth.scale_krnl_to_fraction(k, 0.3)
+    f = os.path.join(tempfile.mkdtemp(prefix="irlkrnl"), "k.krnl")
+    synth.write_krnl(f, k, km, MU0, pix)
+    return f
+
+
+def _recon(pb, krnl=None, srf_iters=0, graphs=False, autotune=False, precision="fp32"):
+    import libirl_amd as L
+    L.set_adv_options(1, 0)
+    p = pb.parms()
+    p.iNumSrfIterations = srf_iters
+    prev_g = L.set_graphs(graphs)
+    prev_a = L.set_autotune(autotune)
+    prev_p = L.set_precision(precision)
+    try:
+        return L.Recon(p, pb.options(), pb.views, pb.drf_file, krnl, None, pb.mu, pb.prj, pb.x0)
+    finally:
+        L.set_graphs(prev_g)
+        L.set_autotune(prev_a)
+        L.set_precision(prev_p)
+
+
+@pytest.mark.parametrize("scatter", [False, True])
+def test_graph_replay_equals_eager(scatter):
+    print("graph replay vs eager, scatter=%s" % scatter, flush=True)
+    pb = Problem(n=32, nz=16, nviews=8, per=2, iters=3, half=3, noise=False)
+    krnl = _krnl_file(pb.pix) if scatter else None
+    srf = 2 if scatter else 0
+    a = _recon(pb, krnl, srf, graphs=False)
+    b = _recon(pb, krnl, srf, graphs=True)
+    for it in range(3):
+        for s in range(a.num_subsets):
+            a.subiteration(s)
+            b.subiteration(s)
+            # every replay (from the second iteration on) against the eager result
+            assert np.array_equal(a.estimate(), b.estimate()), (it, s)
+
+
+def test_graph_replay_with_learned_scatter_equals_eager():
+    print("graph replay vs eager, learned scatter (exact terms)", flush=True)
+    import libirl_amd as L
+    pb = Problem(n=32, nz=16, nviews=8, per=2, iters=3, half=3, noise=False)
+    krnl = _krnl_file(pb.pix)
+    L.set_scatter_cnn_exact(True)
+    L.set_scatter_cnn(None, check_every=3, max_nrmse=1.0)
+    try:
+        a = _recon(pb, krnl, 3, graphs=False)
+        b = _recon(pb, krnl, 3, graphs=True)
+    finally:
+        L.set_scatter_cnn_exact(False)
+    for it in range(3):
+        for s in range(a.num_subsets):
+            a.subiteration(s)
+            b.subiteration(s)
+            assert np.array_equal(a.estimate(), b.estimate()), (it, s)
+    sa, sb = L.cnn_status(a), L.cnn_status(b)
+    assert sa == sb and sa["checks"] == 4 and sa["used"] == 12
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_autotuned_tiles_give_the_same_result(precision):
+    print("autotune %s" % precision, flush=True)
+    pb = Problem(n=32, nz=128, nviews=8, per=2, iters=1, half=5, noise=False)
+    a = _recon(pb, precision=precision)
+    b = _recon(pb, autotune=True, precision=precision)
+    tp, tb = b.tiles()
+    print("tiles", tp, tb)
+    assert tp in (4, 8) and tb in (4, 8) and a.tiles() == (0, 0)
+    for s in range(a.num_subsets):
+        a.subiteration(s)
+        b.subiteration(s)
+    assert np.array_equal(a.estimate(), b.estimate())
+
+
+def test_batch_of_uneven_reconstructions_equals_each_alone():
+    print("batch of 3 uneven reconstructions", flush=True)
+    import libirl_amd as L
+    probs = [Problem(n=32, nz=16, nviews=8, per=2, iters=3, half=3, noise=False, seed=1),
+             Problem(n=48, nz=24, nviews=12, per=3, iters=3, half=3, noise=False, seed=2),
+             Problem(n=40, nz=16, nviews=16, per=8, iters=3, half=3, noise=False, seed=3)]
+    krnl = [_krnl_file(p.pix) for p in probs]
+    alone = [_recon(p, k, 2) for p, k in zip(probs, krnl)]
+    batched = [_recon(p, k, 2) for p, k in zip(probs, krnl)]
+    S = [r.num_subsets for r in alone]
+    assert S == [4, 4, 2]
+    batch = L.Batch(batched)
+    steps = 3 * max(S)
+    for k in range(steps):
+        batch.step()
+        for r, n in zip(alone, S):
+            r.subiteration(k % n)
+        if k % 3 == 2:
+            batch.synchronize()
+            for r, b in zip(alone, batched):
+                assert np.array_equal(r.estimate(), b.estimate()), k
+    batch.synchronize()
+    for r, b in zip(alone, batched):
+        assert np.array_equal(r.estimate(), b.estimate())
+    st = batch.stats()
+    print("batch stats", st)
+    assert st["graph_launches"] > 0 and st["eager_subiterations"] > 0
+    batch.close()
