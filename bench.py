@@ -31,14 +31,15 @@ PEAK_HBM_GBS = 8000.0
 PEAK_BF16_TFLOPS = 2500.0
 
 
-def build_workload(tmpdir, seed=2, name="c2"):
+def build_workload(tmpdir, seed=2, name="c2", cfg=None):
     import libirl_amd as L
     from libirl_amd import synth
-    cfg = synth.CONFIGS[name]
+    if cfg is None:
+        cfg = synth.CONFIGS[name]
     n, nz, pix, V = cfg["n"], cfg["nz"], cfg["pix"], cfg["views"]
     act, mu = synth.phantom(n, nz, seed, n_inserts=cfg["inserts"])
-    views = synth.views_ellipse(V, n, seed)
-    drf_file = os.path.join(tmpdir, "c2.drft")
+    views = synth.views_ellipse(V, n, seed, cfg.get("rmin", 20.0), cfg.get("rmax", 28.0))
+    drf_file = os.path.join(tmpdir, name + ".drft")
     offs, psfs, start = synth.drf_for_orbit(drf_file, pix, n, views, cfg["half"])
     parms = L.IrlParms()
     parms.NumPixels, parms.NumSlices, parms.NumViews = n, nz, V
@@ -241,6 +242,103 @@ def measured_traffic(stage):
         return None
 
 
+def batch_configs(count=16, seed=5):
+    """Uneven reconstructions of the batched workload: N in {64, ..., 160}
+    (even multiples of 16), V a multiple of 8 in [64, 128] with 8 subsets,
+    the ESSE scatter configuration's phantom class, DRF and kernel at 0.33 cm,
+    orbits just outside each field of view."""
+    from libirl_amd import synth
+    rng = np.random.default_rng(seed)
+    base = synth.CONFIGS["c3"]
+    out = []
+    for i in range(count):
+        n = int(rng.choice(np.arange(64, 161, 16)))
+        V = int(rng.choice(np.arange(64, 129, 8)))
+        r0 = 0.5 * n * base["pix"] + 2.0
+        out.append(dict(base, n=n, nz=n, views=V, vs=V // 8, rmin=r0, rmax=r0 + 8.0,
+                        counts=2e6 * (n / 128.0) ** 2))
+    return out
+
+
+def batch_line(tmpdir, net, steps=16, warmup=16):
+    """16 concurrent reconstructions of uneven size: learned scatter + BF16
+    DRF, graph replay and autotuned tiles, stepped through one batch (one
+    parent-graph launch per step) and, for comparison, one after another;
+    then each estimate after 4 iterations against its FP32 ESSE
+    reconstruction."""
+    import libirl_amd as L
+    cfgs = batch_configs()
+    works = [build_workload(tmpdir, seed=100 + i, name="b%d" % i, cfg=c)
+             for i, c in enumerate(cfgs)]
+    L.set_adv_options(0, 0)
+    prev = (L.set_precision("bf16"), L.set_graphs(True), L.set_autotune(True))
+    L.set_scatter_cnn(net, check_every=8, max_nrmse=0.01)
+    try:
+        recs = [L.Recon(w["parms"], w["opts"], w["views"], w["drf_file"], w["krnl_file"], None,
+                        w["mu"], w["prj"], None) for w in works]
+    finally:
+        L.set_precision(prev[0])
+        L.set_graphs(prev[1])
+        L.set_autotune(prev[2])
+        L.set_scatter_cnn(None)
+    batch = L.Batch(recs, parent_graph=True)
+    for _ in range(warmup):
+        batch.step()
+    batch.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.step()
+    batch.synchronize()
+    dt_batch = time.perf_counter() - t0
+    st = batch.stats()
+    est = [r.estimate() for r in recs]          # after warmup + steps = 4 iterations
+    tiles = [r.tiles() for r in recs]
+    status = [L.cnn_status(r) for r in recs]
+    # the same steps with each member's graph launched on its own stream
+    S = [r.num_subsets for r in recs]
+    batch.set_parent_graph(False)
+    for _ in range(max(S)):
+        batch.step()                            # instantiates the members' graphs
+    batch.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.step()
+    batch.synchronize()
+    dt_streams = time.perf_counter() - t0
+    # one reconstruction at a time, each sub-iteration finished before the next
+    t0 = time.perf_counter()
+    for k in range(steps):
+        for r, n in zip(recs, S):
+            r.subiteration(k % n)
+            r.synchronize()
+    dt_seq = time.perf_counter() - t0
+    batch.close()
+    for r in recs:
+        r.close()
+    errs = []
+    for w, e in zip(works, est):
+        rec, _, _, ref = time_subiterations(w, warmup + steps, 0, "fp32")
+        rec.close()
+        errs.append(float(np.linalg.norm(e - ref) / np.linalg.norm(ref)))
+    n = len(recs)
+    return {"workload": "16 reconstructions, N in 64..160, 64..128 views, 8 subsets, ATN + DRF + "
+                        "learned scatter, BF16 DRF",
+            "value": n * steps / dt_batch, "unit": "subiters/s (all reconstructions)",
+            "ms_per_step": dt_batch * 1e3 / steps,
+            "mode": "one parent-graph launch per step (members as child graphs)",
+            "member_streams_value": n * steps / dt_streams,
+            "one_at_a_time_value": n * steps / dt_seq,
+            "batch_speedup_vs_one_at_a_time": dt_seq / dt_batch,
+            "sizes": [[c["n"], c["views"]] for c in cfgs],
+            "tiles_prj_bck": tiles,
+            "graph_launches": st["graph_launches"], "replays": st["replays"],
+            "eager_subiterations": st["eager_subiterations"],
+            "learned_scatter_active": sum(s["active"] for s in status),
+            "worst_check_nrmse": max(s["worst_nrmse"] for s in status),
+            "rel_l2_vs_fp32_esse_after_4_iterations": {"max": max(errs),
+                                                        "mean": float(np.mean(errs))}}
+
+
 CNN_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libirl_amd", "weights",
                            "scatter_cnn_c3.bin")
 
@@ -272,6 +370,10 @@ def main():
                     help="time only the ESSE workload (for profiling runs)")
     ap.add_argument("--no-bf16", action="store_true", help="skip the BF16 precision variant")
     ap.add_argument("--no-cnn", action="store_true", help="skip the learned-scatter lines")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batch of 16 uneven reconstructions")
+    ap.add_argument("--only-batch", action="store_true",
+                    help="run only the batch of 16 uneven reconstructions")
     ap.add_argument("--no-errors", action="store_true",
                     help="skip the FP64-oracle projection error of the BF16 variant")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -292,6 +394,9 @@ def main():
               "(requested --gpus %d)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     tmpdir = tempfile.mkdtemp(prefix="irlbench")
+    if args.only_batch:
+        print(json.dumps(batch_line(tmpdir, load_scatter_cnn())))
+        return
     if args.only_esse:
         we = build_workload(tmpdir, name="c3")
         line = workload_line(we, args.steps, args.warmup, "fp32")
@@ -357,6 +462,8 @@ def main():
                 line["speedup_vs_esse_fp32"] = line["value"] / out["esse"]["value"]
                 line["training"] = cnn_training_record()
                 out[key] = line
+            if not args.no_batch:
+                out["batch16"] = batch_line(tmpdir, net)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w)
     print(json.dumps(out))

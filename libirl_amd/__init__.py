@@ -230,7 +230,8 @@ def load():
     lib.irlamd_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int]
     lib.irlamd_batch_step.argtypes = [vp]
     lib.irlamd_batch_synchronize.argtypes = [vp]
-    lib.irlamd_batch_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    lib.irlamd_batch_stats.argtypes = [vp] + [ctypes.POINTER(ctypes.c_int)] * 3
+    lib.irlamd_batch_set_parent_graph.argtypes = [vp, ctypes.c_int]
     lib.irlamd_batch_destroy.argtypes = [vp]
     lib.irlamd_batch_destroy.restype = None
     _lib = lib
@@ -631,7 +632,7 @@ class Batch:
     """Reconstructions stepped together (irlamd_batch_*): each step runs the
     next sub-iteration of every member, the replayable ones in one graph launch."""
 
-    def __init__(self, recons):
+    def __init__(self, recons, parent_graph=False):
         self._lib = load()
         self._recons = list(recons)   # members must outlive the batch
         arr = (ctypes.c_void_p * len(self._recons))(*[r._h for r in self._recons])
@@ -639,6 +640,10 @@ class Batch:
         if not h:
             raise RuntimeError("irlamd_batch_create failed: " + error_string())
         self._h = h
+        self.set_parent_graph(parent_graph)
+
+    def set_parent_graph(self, on):
+        self._lib.irlamd_batch_set_parent_graph(self._h, 1 if on else 0)
 
     def step(self):
         if self._lib.irlamd_batch_step(self._h):
@@ -649,9 +654,9 @@ class Batch:
             raise RuntimeError(error_string())
 
     def stats(self):
-        g, e = ctypes.c_int(), ctypes.c_int()
-        self._lib.irlamd_batch_stats(self._h, ctypes.byref(g), ctypes.byref(e))
-        return {"graph_launches": g.value, "eager_subiterations": e.value}
+        g, r, e = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._lib.irlamd_batch_stats(self._h, ctypes.byref(g), ctypes.byref(r), ctypes.byref(e))
+        return {"graph_launches": g.value, "replays": r.value, "eager_subiterations": e.value}
 
     def close(self):
         if getattr(self, "_h", None):

@@ -660,7 +660,6 @@ void ScatterCnn::alloc_state()
     }
     Wt_ = cnn_alloc<float>(27L * 32 * 32);
     Wr0_ = cnn_alloc<float>(wcount(0));
-    scal_ = cnn_alloc<float>(2);
     dsum_ = cnn_alloc<double>(1);
 }
 
@@ -682,6 +681,35 @@ ScatterCnn::ScatterCnn(float kappa, const std::vector<float> &packed) : kappa_(k
 {
     alloc_state();
     setParams(packed);
+    prepareInference();
+}
+
+void ScatterCnn::prepareInference()
+{
+    upload_bf16_weights(nullptr);
+    CNN_CHECK(hipDeviceSynchronize());
+}
+
+CnnWork::CnnWork(int N, int Nz)
+{
+    if (N < 2 || Nz < 2 || N % 2 || Nz % 2)
+        fail(ETYPE_ILLEGAL_VALUE, "scatter CNN needs even volume sizes (%d, %d)", N, Nz);
+    V = (long)(N / 2) * (N / 2) * (Nz / 2);
+    in = cnn_alloc<float>(V * 2);
+    out = cnn_alloc<float>(V * 2);
+    for (auto &a : actb) a = cnn_alloc<uint16_t>(V * 32);
+    nb = std::min<long>(1024, cdiv_u((long)N * N * Nz, 256));
+    psum = cnn_alloc<double>(nb);
+    scal = cnn_alloc<float>(2);
+}
+
+CnnWork::~CnnWork()
+{
+    cnn_free(in);
+    cnn_free(out);
+    for (auto &a : actb) cnn_free(a);
+    cnn_free(psum);
+    cnn_free(scal);
 }
 
 ScatterCnn::~ScatterCnn()
@@ -702,11 +730,7 @@ ScatterCnn::~ScatterCnn()
     for (auto &a : actb_) cnn_free(a);
     cnn_free(Wt_);
     cnn_free(Wr0_);
-    cnn_free(inbuf_);
-    cnn_free(outbuf_);
-    cnn_free(scal_);
     cnn_free(dsum_);
-    cnn_free(psum_);
 }
 
 // packed (oracle) order per layer: W [27][Cout][Cin], b [Cout]
@@ -773,14 +797,10 @@ void ScatterCnn::ensure_work(long V)
     for (auto &a : act_) cnn_free(a);
     for (auto &a : grad_) cnn_free(a);
     for (auto &a : actb_) cnn_free(a);
-    cnn_free(inbuf_);
-    cnn_free(outbuf_);
     for (int l = 0; l <= CNN_LAYERS; ++l)
         act_[l] = cnn_alloc<float>(V * (l == 0 ? 2 : l == CNN_LAYERS ? 2 : 32));
     for (auto &a : grad_) a = cnn_alloc<float>(V * 32);
     for (auto &a : actb_) a = cnn_alloc<uint16_t>(V * 32);
-    inbuf_ = cnn_alloc<float>(V * 2);
-    outbuf_ = cnn_alloc<float>(V * 2);
     workV_ = V;
 }
 
@@ -826,16 +846,25 @@ void ScatterCnn::forward(const float *in, int Y, int X, int Z, float *out, bool 
         return;
     }
     upload_bf16_weights(s);
+    forward_bf16(in, Y, X, Z, out, actb_[0], actb_[1], s);
+}
+
+// Inference arithmetic, activations in the given buffers ([V][32] bf16 each).
+void ScatterCnn::forward_bf16(const float *in, int Y, int X, int Z, float *out, uint16_t *a0,
+                              uint16_t *a1, hipStream_t s)
+{
+    const long V = (long)Y * X * Z;
+    const dim3 grid(cdiv_u(V, 256)), blk(256);
     hipLaunchKernelGGL((k_conv_valu<32, float, uint16_t>), grid, blk, 0, s, in, Wr0_, b_[0],
-                       nullptr, actb_[0], Y, X, Z, 2, cnn_dil(0), 1);
+                       nullptr, a0, Y, X, Z, 2, cnn_dil(0), 1);
     const long tasks = (long)Y * X * ((Z + CNN_ROWZ - 1) / CNN_ROWZ);
     const dim3 rgrid(cdiv_u(tasks, 4));
     auto lds = [](int dil) { return (size_t)4 * (CNN_ROWZ + 2 * dil) * 4 * sizeof(uint4); };
-    hipLaunchKernelGGL((k_conv_mfma_rows<2, 32>), rgrid, blk, lds(cnn_dil(1)), s, actb_[0],
-                       (const uint4 *)Wbf_[1], b_[1], (void *)actb_[1], Y, X, Z, cnn_dil(1), 1);
-    hipLaunchKernelGGL((k_conv_mfma_rows<2, 32>), rgrid, blk, lds(cnn_dil(2)), s, actb_[1],
-                       (const uint4 *)Wbf_[2], b_[2], (void *)actb_[0], Y, X, Z, cnn_dil(2), 1);
-    hipLaunchKernelGGL((k_conv_mfma_rows<1, 2>), rgrid, blk, lds(cnn_dil(3)), s, actb_[0],
+    hipLaunchKernelGGL((k_conv_mfma_rows<2, 32>), rgrid, blk, lds(cnn_dil(1)), s, a0,
+                       (const uint4 *)Wbf_[1], b_[1], (void *)a1, Y, X, Z, cnn_dil(1), 1);
+    hipLaunchKernelGGL((k_conv_mfma_rows<2, 32>), rgrid, blk, lds(cnn_dil(2)), s, a1,
+                       (const uint4 *)Wbf_[2], b_[2], (void *)a0, Y, X, Z, cnn_dil(2), 1);
+    hipLaunchKernelGGL((k_conv_mfma_rows<1, 2>), rgrid, blk, lds(cnn_dil(3)), s, a0,
                        (const uint4 *)Wbf_[3], b_[3], (void *)out, Y, X, Z, cnn_dil(3), 0);
     CNN_CHECK(hipGetLastError());
 }
@@ -930,24 +959,25 @@ static void mean_scale(const float *x, long n, double *&psum, long &nb, float *s
         cnn_free(psum);
         psum = cnn_alloc<double>(want);
         nb = want;
-    }
+    }   // (a CnnWork is made with enough partial sums: no allocation here)
     hipLaunchKernelGGL(k_sum_partial, dim3((unsigned)want), dim3(256), 0, s, x, n, psum);
     hipLaunchKernelGGL(k_inv_mean, dim3(1), dim3(256), 0, s, psum, (int)want, n, scal);
 }
 
 void ScatterCnn::sources(const float *x, const float *mu, int N, int Nz, const CnnPhys &ph,
-                         float primary, float *g0, float *g1, hipStream_t s)
+                         float primary, float *g0, float *g1, CnnWork &w, hipStream_t s)
 {
-    if (N % 2 || Nz % 2) fail(ETYPE_ILLEGAL_VALUE, "scatter CNN needs even volume sizes");
     const int Y = N / 2, X = N / 2, Z = Nz / 2;
     const long V = (long)Y * X * Z, n = (long)N * N * Nz;
-    ensure_work(V);
-    mean_scale(x, n, psum_, sumBlocks_, scal_, s);
-    hipLaunchKernelGGL(k_cnn_input, dim3(cdiv_u(V, 256)), dim3(256), 0, s, x, mu, N, Nz, scal_,
-                       ph.fDens, inbuf_);
-    forward(inbuf_, Y, X, Z, outbuf_, true, s);
-    hipLaunchKernelGGL(k_cnn_source, dim3(cdiv_u(n, 256)), dim3(256), 0, s, outbuf_, x, mu, N, Nz,
-                       scal_, kappa_, ph.fDens, ph.cntr, primary, g0, g1);
+    if (N % 2 || Nz % 2 || V != w.V)
+        fail(ETYPE_ILLEGAL_VALUE, "scatter CNN work buffers made for another size");
+    if (bfDirty_) fail(ETYPE_ILLEGAL_VALUE, "scatter CNN inference weights not prepared");
+    mean_scale(x, n, w.psum, w.nb, w.scal, s);
+    hipLaunchKernelGGL(k_cnn_input, dim3(cdiv_u(V, 256)), dim3(256), 0, s, x, mu, N, Nz, w.scal,
+                       ph.fDens, w.in);
+    forward_bf16(w.in, Y, X, Z, w.out, w.actb[0], w.actb[1], s);
+    hipLaunchKernelGGL(k_cnn_source, dim3(cdiv_u(n, 256)), dim3(256), 0, s, w.out, x, mu, N, Nz,
+                       w.scal, kappa_, ph.fDens, ph.cntr, primary, g0, g1);
     CNN_CHECK(hipGetLastError());
 }
 
@@ -1086,7 +1116,7 @@ ScatterCnn *ScatterCnn::train(const EsseModel &m, const CnnTrainParms &p, CnnTra
     cnn_free(terms);
     cnn_free(dEll);
 
-    ScatterCnn *net = new ScatterCnn(kappa, p.seed);
+    ScatterCnn *net = new ScatterCnn(kappa, p.seed);   // prepared below, after training
     const int ntrain = p.pairs - p.val;
     std::vector<int> order(ntrain);
     for (int i = 0; i < ntrain; ++i) order[i] = i;
@@ -1131,6 +1161,7 @@ ScatterCnn *ScatterCnn::train(const EsseModel &m, const CnnTrainParms &p, CnnTra
         }
     }
     cnn_free(pred);
+    net->prepareInference();
     cnn_free(dsIn);
     cnn_free(dsTgt);
     cnn_free(scal);
@@ -1302,6 +1333,7 @@ int irlamd_cnn_set_params(irlamd_cnn *h, const float *in)
     irlamd::set_last_error(0);
     try {
         h->net->setParams(std::vector<float>(in, in + irlamd::cnn_num_params()));
+        h->net->prepareInference();
         return 0;
     } catch (const irlamd::IrlError &e) {
         return cnn_report(e);
@@ -1361,7 +1393,9 @@ int irlamd_cnn_sources(irlamd_cnn *h, const float *x, const float *mu, int N, in
         irlamd::CnnPhys ph;
         ph.fDens = fDens;
         ph.cntr = fCntr;
-        h->net->sources(dx.p, dm.p, N, Nz, ph, primary, d0.p, d1.p, nullptr);
+        h->net->prepareInference();
+        irlamd::CnnWork w(N, Nz);
+        h->net->sources(dx.p, dm.p, N, Nz, ph, primary, d0.p, d1.p, w, nullptr);
         CNN_CHECK(hipMemcpy(g0, d0.p, n * 4, hipMemcpyDeviceToHost));
         CNN_CHECK(hipMemcpy(g1, d1.p, n * 4, hipMemcpyDeviceToHost));
         return 0;

@@ -49,6 +49,21 @@ struct CnnTrainStats {
     float kappa = 0;
 };
 
+// Inference work buffers of one user of a network (each reconstruction has
+// its own, so reconstructions on different streams share only the weights).
+struct CnnWork {
+    long V = 0;                          // coarse voxels
+    float *in = nullptr, *out = nullptr; // [V][2]
+    uint16_t *actb[2] = {};              // bf16 activations [V][32]
+    double *psum = nullptr;              // partial sums of the mean
+    long nb = 0;
+    float *scal = nullptr;               // [0] 1/mean(x), [1] mean(x)
+    CnnWork(int N, int Nz);
+    ~CnnWork();
+    CnnWork(const CnnWork &) = delete;
+    CnnWork &operator=(const CnnWork &) = delete;
+};
+
 class ScatterCnn {
 public:
     // Untrained network (He-normal weights from 'seed'), output scale kappa.
@@ -76,9 +91,15 @@ public:
     void adamStep(float lr, hipStream_t s);
 
     // Source volumes of a reconstruction (engine layout [y][x][z], N x N x Nz):
-    // g0, g1 from the current estimate x and attenuation map mu (per pixel).
+    // g0, g1 from the current estimate x and attenuation map mu (per pixel),
+    // in the caller's work buffers (made for the same N, Nz).  Reads only the
+    // weights, so users on different streams may run it concurrently.
     void sources(const float *x, const float *mu, int N, int Nz, const CnnPhys &ph,
-                 float primary, float *g0, float *g1, hipStream_t s);
+                 float primary, float *g0, float *g1, CnnWork &w, hipStream_t s);
+
+    // BF16 inference weights from the master weights (after loading or
+    // training; synchronous).
+    void prepareInference();
 
     // Train from ESSE-generated pairs (the model's kernel at the given size).
     static ScatterCnn *train(const EsseModel &m, const CnnTrainParms &p, CnnTrainStats &st);
@@ -87,6 +108,8 @@ private:
     void alloc_state();
     void ensure_work(long V);
     void upload_bf16_weights(hipStream_t s);
+    void forward_bf16(const float *in, int Y, int X, int Z, float *out, uint16_t *a0,
+                      uint16_t *a1, hipStream_t s);
 
     float kappa_ = 1.0f;
     int step_ = 0;
@@ -103,11 +126,7 @@ private:
     float *act_[CNN_LAYERS + 1] = {};   // fp32 activations (training / FP32 forward)
     float *grad_[2] = {};
     uint16_t *actb_[2] = {};             // bf16 activations (inference)
-    float *inbuf_ = nullptr, *outbuf_ = nullptr;
     double *dsum_ = nullptr;
-    float *scal_ = nullptr;              // [0] 1/sx, [1] sx
-    long sumBlocks_ = 0;
-    double *psum_ = nullptr;
 };
 
 // g0, g1 from the scatter model's own terms E0, E1 of the unrotated

@@ -211,9 +211,11 @@ private:
 };
 
 // Several reconstructions stepped together: one sub-iteration of each per
-// step, the ones that can be replayed launched as child graphs of one parent
-// graph (one launch per step), the others (first occurrence of their
-// subset and mode, learned-scatter checks) eagerly on their own streams.
+// step.  The ones that can be replayed run from captured graphs, each on its
+// own stream (concurrently), or with setParentGraph(true) as child graphs
+// of one parent graph (one launch per step); the others (first occurrence
+// of their subset and mode, learned-scatter checks) run eagerly on their
+// own streams after them.
 class ReconBatch {
 public:
     explicit ReconBatch(const std::vector<Recon *> &recons);
@@ -221,8 +223,10 @@ public:
     ReconBatch(const ReconBatch &) = delete;
     ReconBatch &operator=(const ReconBatch &) = delete;
     void step(const std::vector<int> &subset, const std::vector<SrfMode> &srf);
+    void setParentGraph(bool on);
     void synchronize();
     int graphLaunches() const;       // parent-graph launches so far
+    int replays() const;             // sub-iterations replayed from graphs so far
     int eagerSubIterations() const;  // sub-iterations run eagerly so far
 
 private:

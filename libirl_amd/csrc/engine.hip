@@ -1166,6 +1166,7 @@ struct Recon::Impl {
     std::unique_ptr<EsseDev> esse;
     // learned scatter
     DBuf<float> g0, g1, att2;
+    std::unique_ptr<CnnWork> cnnWork;   // this reconstruction's inference buffers
     DBuf<double> chk;
     CnnPhys phys;
     Recon::CnnStatus cst;
@@ -1296,7 +1297,7 @@ struct Recon::Impl {
             cnn_exact_sources(terms.p, terms.p + M, x.p, mu.p, g.N, g.Nz, phys, 1.0f, g0.p, g1.p,
                               stream);
         } else {
-            c.cnn->sources(x.p, mu.p, g.N, g.Nz, phys, 1.0f, g0.p, g1.p, stream);
+            c.cnn->sources(x.p, mu.p, g.N, g.Nz, phys, 1.0f, g0.p, g1.p, *cnnWork, stream);
         }
     }
 
@@ -1585,6 +1586,7 @@ Recon::Recon(const Config &cfg) : impl_(new Impl), cfg_(cfg)
                 I.phys.cntr = em.cntrPixFac;
                 I.g0.alloc(I.M);
                 I.g1.alloc(I.M);
+                if (cfg.cnn) I.cnnWork.reset(new CnnWork(g.N, g.Nz));
                 I.att2.alloc(I.vol * Vs);
                 I.chk.alloc(3);
             }
@@ -2055,7 +2057,10 @@ struct ReconBatch::Impl {
         std::vector<hipGraphNode_t> node;           // per member
     };
     std::map<std::vector<int>, Parent> parents;
-    int launches = 0, eager = 0;
+    // own-stream mode: each member's instantiated graphs by key
+    std::vector<std::map<GraphKey, hipGraphExec_t>> exec;
+    bool parent = false;
+    int launches = 0, eager = 0, replays = 0;
 };
 
 ReconBatch::ReconBatch(const std::vector<Recon *> &recons) : impl_(new Impl)
@@ -2068,6 +2073,7 @@ ReconBatch::ReconBatch(const std::vector<Recon *> &recons) : impl_(new Impl)
     HIP_CHECK(hipEventCreateWithFlags(&B.done, hipEventDisableTiming));
     B.child.resize(recons.size());
     B.seen.resize(recons.size());
+    B.exec.resize(recons.size());
 }
 
 ReconBatch::~ReconBatch()
@@ -2076,8 +2082,11 @@ ReconBatch::~ReconBatch()
     (void)hipStreamSynchronize(B.stream);
     for (auto &kv : B.parents)
         if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+    for (auto *r : B.r) r->synchronize();
     for (auto &m : B.child)
         for (auto &kv : m) (void)hipGraphDestroy(kv.second);
+    for (auto &m : B.exec)
+        for (auto &kv : m) (void)hipGraphExecDestroy(kv.second);
     for (auto e : B.ev) (void)hipEventDestroy(e);
     (void)hipEventDestroy(B.done);
     (void)hipStreamDestroy(B.stream);
@@ -2091,19 +2100,31 @@ void ReconBatch::step(const std::vector<int> &subset, const std::vector<SrfMode>
     if ((int)subset.size() != n || (int)srf.size() != n)
         fail(ETYPE_ILLEGAL_VALUE, "batch step: %d reconstructions, %d subsets, %d modes", n,
              (int)subset.size(), (int)srf.size());
-    std::vector<int> members, topo;
+    std::vector<int> members, eager, topo;
     std::vector<hipGraph_t> graphs;
+    std::vector<char> check(n, 0);
     for (int i = 0; i < n; ++i) {
         Recon &R = *B.r[i];
-        const bool check = R.beginSubset(srf[i]);
+        check[i] = R.beginSubset(srf[i]);
         const GraphKey key{subset[i], (int)srf[i], 0, R.learnedScatterActive() ? 1 : 0};
-        if (check || !B.seen[i].count(key)) {
-            // eager on the reconstruction's own stream, after the last batch launch
-            HIP_CHECK(hipStreamWaitEvent((hipStream_t)R.streamHandle(), B.done, 0));
-            R.runSubIteration(subset[i], srf[i]);
+        if (check[i] || !B.seen[i].count(key)) {
+            if (!check[i]) B.seen[i].insert(key);
+            eager.push_back(i);
+            continue;
+        }
+        if (!B.parent) {
+            // own stream: the member's cached graph of this key
+            auto it = B.exec[i].find(key);
+            if (it == B.exec[i].end()) {
+                hipGraph_t g = (hipGraph_t)R.captureSubIteration(subset[i], false, srf[i]);
+                hipGraphExec_t ex = nullptr;
+                HIP_CHECK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+                HIP_CHECK(hipGraphDestroy(g));
+                it = B.exec[i].emplace(key, ex).first;
+            }
+            HIP_CHECK(hipGraphLaunch(it->second, (hipStream_t)R.streamHandle()));
             R.endSubset(srf[i]);
-            if (!check) B.seen[i].insert(key);
-            B.eager++;
+            B.replays++;
             continue;
         }
         auto it = B.child[i].find(key);
@@ -2116,47 +2137,60 @@ void ReconBatch::step(const std::vector<int> &subset, const std::vector<SrfMode>
         topo.push_back(key.srf);
         topo.push_back(key.learned);
     }
-    if (members.empty()) return;
-    // the parent graph of this topology: one child node per member, no edges
-    auto pit = B.parents.find(topo);
-    bool fresh = pit == B.parents.end();
-    if (!fresh) {
-        for (size_t k = 0; k < members.size(); ++k)
-            if (hipGraphExecChildGraphNodeSetParams(pit->second.exec, pit->second.node[k],
-                                                    graphs[k]) != hipSuccess) {
-                (void)hipGetLastError();
-                (void)hipGraphExecDestroy(pit->second.exec);
-                B.parents.erase(pit);
-                fresh = true;
-                break;
-            }
-    }
-    if (fresh) {
-        hipGraph_t parent;
-        HIP_CHECK(hipGraphCreate(&parent, 0));
-        Impl::Parent P;
-        for (auto g : graphs) {
-            hipGraphNode_t node;
-            HIP_CHECK(hipGraphAddChildGraphNode(&node, parent, nullptr, 0, g));
-            P.node.push_back(node);
+    if (!members.empty()) {
+        // the parent graph of this topology: one child node per member, no edges
+        auto pit = B.parents.find(topo);
+        bool fresh = pit == B.parents.end();
+        if (!fresh) {
+            for (size_t k = 0; k < members.size(); ++k)
+                if (hipGraphExecChildGraphNodeSetParams(pit->second.exec, pit->second.node[k],
+                                                        graphs[k]) != hipSuccess) {
+                    (void)hipGetLastError();
+                    (void)hipGraphExecDestroy(pit->second.exec);
+                    B.parents.erase(pit);
+                    fresh = true;
+                    break;
+                }
         }
-        HIP_CHECK(hipGraphInstantiate(&P.exec, parent, nullptr, nullptr, 0));
-        HIP_CHECK(hipGraphDestroy(parent));
-        pit = B.parents.emplace(topo, P).first;
+        if (fresh) {
+            hipGraph_t parent;
+            HIP_CHECK(hipGraphCreate(&parent, 0));
+            Impl::Parent P;
+            for (auto g : graphs) {
+                hipGraphNode_t node;
+                HIP_CHECK(hipGraphAddChildGraphNode(&node, parent, nullptr, 0, g));
+                P.node.push_back(node);
+            }
+            HIP_CHECK(hipGraphInstantiate(&P.exec, parent, nullptr, nullptr, 0));
+            HIP_CHECK(hipGraphDestroy(parent));
+            pit = B.parents.emplace(topo, P).first;
+        }
+        // the members' earlier work precedes the launch; their later work follows it
+        for (int i : members) {
+            HIP_CHECK(hipEventRecord(B.ev[i], (hipStream_t)B.r[i]->streamHandle()));
+            HIP_CHECK(hipStreamWaitEvent(B.stream, B.ev[i], 0));
+        }
+        HIP_CHECK(hipGraphLaunch(pit->second.exec, B.stream));
+        HIP_CHECK(hipEventRecord(B.done, B.stream));
+        for (int i : members) {
+            HIP_CHECK(hipStreamWaitEvent((hipStream_t)B.r[i]->streamHandle(), B.done, 0));
+            B.r[i]->endSubset(srf[i]);
+        }
+        B.launches++;
+        B.replays += (int)members.size();
     }
-    // the members' earlier work precedes the launch; their later work follows it
-    for (int i : members) {
-        HIP_CHECK(hipEventRecord(B.ev[i], (hipStream_t)B.r[i]->streamHandle()));
-        HIP_CHECK(hipStreamWaitEvent(B.stream, B.ev[i], 0));
+    // eager members last: a checked subset reads its check on the host, which
+    // waits only for its own stream
+    for (int i : eager) {
+        Recon &R = *B.r[i];
+        R.runSubIteration(subset[i], srf[i]);
+        R.endSubset(srf[i]);
+        B.eager++;
     }
-    HIP_CHECK(hipGraphLaunch(pit->second.exec, B.stream));
-    HIP_CHECK(hipEventRecord(B.done, B.stream));
-    for (int i : members) {
-        HIP_CHECK(hipStreamWaitEvent((hipStream_t)B.r[i]->streamHandle(), B.done, 0));
-        B.r[i]->endSubset(srf[i]);
-    }
-    B.launches++;
 }
+
+void ReconBatch::setParentGraph(bool on) { impl_->parent = on; }
+int ReconBatch::replays() const { return impl_->replays; }
 
 void ReconBatch::synchronize()
 {

@@ -92,8 +92,9 @@ def test_autotuned_tiles_give_the_same_result(precision):
     assert np.array_equal(a.estimate(), b.estimate())
 
 
-def test_batch_of_uneven_reconstructions_equals_each_alone():
-    print("batch of 3 uneven reconstructions", flush=True)
+@pytest.mark.parametrize("parent", [False, True])
+def test_batch_of_uneven_reconstructions_equals_each_alone(parent):
+    print("batch of 3 uneven reconstructions, parent graph %s" % parent, flush=True)
     import libirl_amd as L
     probs = [Problem(n=32, nz=16, nviews=8, per=2, iters=3, half=3, noise=False, seed=1),
              Problem(n=48, nz=24, nviews=12, per=3, iters=3, half=3, noise=False, seed=2),
@@ -103,7 +104,7 @@ def test_batch_of_uneven_reconstructions_equals_each_alone():
     batched = [_recon(p, k, 2) for p, k in zip(probs, krnl)]
     S = [r.num_subsets for r in alone]
     assert S == [4, 4, 2]
-    batch = L.Batch(batched)
+    batch = L.Batch(batched, parent_graph=parent)
     steps = 3 * max(S)
     for k in range(steps):
         batch.step()
@@ -118,5 +119,40 @@ def test_batch_of_uneven_reconstructions_equals_each_alone():
         assert np.array_equal(r.estimate(), b.estimate())
     st = batch.stats()
     print("batch stats", st)
-    assert st["graph_launches"] > 0 and st["eager_subiterations"] > 0
+    assert st["replays"] > 0 and st["eager_subiterations"] > 0
+    assert (st["graph_launches"] > 0) == parent
+    batch.close()
+
+
+def test_batch_with_a_shared_network_equals_each_alone():
+    # every member shares one network's weights; each has its own inference
+    # buffers, so members overlapping on their streams do not interfere
+    print("batch of 3 with a shared scatter network", flush=True)
+    import libirl_amd as L
+    net = L.ScatterCnn.create(kappa=1.0, seed=3)
+    p, _ = net.params()
+    layers = L.unpack_cnn_params(p)
+    layers[-1] = (layers[-1][0] * 0.01, layers[-1][1])
+    net.set_params(L.pack_cnn_params(layers))
+    probs = [Problem(n=32, nz=16, nviews=8, per=2, iters=3, half=3, noise=False, seed=1),
+             Problem(n=48, nz=24, nviews=12, per=3, iters=3, half=3, noise=False, seed=2),
+             Problem(n=40, nz=16, nviews=16, per=8, iters=3, half=3, noise=False, seed=3)]
+    krnl = [_krnl_file(p.pix) for p in probs]
+    L.set_scatter_cnn(net, check_every=5, max_nrmse=1e9)
+    try:
+        alone = [_recon(p, k, 3) for p, k in zip(probs, krnl)]
+        batched = [_recon(p, k, 3) for p, k in zip(probs, krnl)]
+    finally:
+        L.set_scatter_cnn(None)
+    S = [r.num_subsets for r in alone]
+    batch = L.Batch(batched)
+    for k in range(3 * max(S)):
+        batch.step()
+        for r, n in zip(alone, S):
+            r.subiteration(k % n)
+            r.synchronize()      # alone: one at a time
+    batch.synchronize()
+    for r, b in zip(alone, batched):
+        assert np.array_equal(r.estimate(), b.estimate())
+        assert L.cnn_status(r) == L.cnn_status(b) and L.cnn_status(b)["active"]
     batch.close()
