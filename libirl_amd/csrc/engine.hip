@@ -503,6 +503,8 @@ struct DrfArgs {
     int vec4;                     // outZS == 1 and 16-byte aligned rows: float4 stores
     int rchunk, nchunk;           // PRJ row chunking
     int xtiles, ztiles;
+    const int2 *colRange;         // per table entry: first and last PSF column holding a
+                                  // nonzero tap (null: all KX columns)
 };
 
 __device__ __forceinline__ float drf_in(const DrfArgs &a, int b, int v, int row, int g, int gx)
@@ -614,23 +616,27 @@ __global__ __launch_bounds__(64 * ZG) void k_drf(DrfArgs a)
             }
         } else {
             const int pad = a.rowPad[(long)v * a.nrows + row];
+            // PSF columns that are zero throughout add exact zeros: skipped,
+            // with the tile columns only they would read
+            const int2 cr = a.colRange ? a.colRange[kidx] : make_int2(0, KX - 1);
+            const int cb = KX - 1 - cr.y, twc = DRF_TX + cr.y - cr.x;
             // stage: consecutive threads take consecutive elements of the
             // input's fastest axis (coalesced loads)
             if (zfast) {
-                for (int e = threadIdx.x; e < TH * TW; e += NT) {
-                    const int c = e / TH, q = e - c * TH;
+                for (int e = threadIdx.x; e < TH * twc; e += NT) {
+                    const int c = cb + e / TH, q = e % TH;
                     tile[c * CS + q] = drf_stage_val(a, b, v, row, z0 - CY + q, x0 - CX + c, pad);
                 }
             } else {
-                for (int e = threadIdx.x; e < TH * TW; e += NT) {
-                    const int q = e / TW, c = e - q * TW;
+                for (int e = threadIdx.x; e < TH * twc; e += NT) {
+                    const int q = e / twc, c = cb + e % twc;
                     tile[c * CS + q] = drf_stage_val(a, b, v, row, z0 - CY + q, x0 - CX + c, pad);
                 }
             }
             const float *kp = a.psf + (long)kidx * KX * KYP;
             for (int e = threadIdx.x; e < KX * KYP; e += NT) psfT[e] = kp[e];
             __syncthreads();
-            for (int i = 0; i < KX; ++i) {
+            for (int i = cr.x; i <= cr.y; ++i) {
                 float pc[KYP];
                 const float4 *pp = (const float4 *)(psfT + i * KYP);
 #pragma unroll
@@ -1188,6 +1194,7 @@ struct Recon::Impl {
     DBuf<uint4> afrHi, afrLo;
     DBuf<int> distSplit;          // per table entry: hi+lo split
     DBuf<int4> drfBand;           // per table entry: kept PSF columns, window shift, K steps
+    DBuf<int2> colRange;          // per table entry: nonzero PSF columns (FP32 kernel)
     DBuf<int> rowKernel16;        // rowKernel with -1 replaced by the delta entry nDist
     DBuf<int> rowKernel16B;       // the same for the backprojector's rows
     DBuf<int> rowKernelB, rowPadB;   // backprojector's table rows (its own table, or copies)
@@ -1357,6 +1364,7 @@ struct Recon::Impl {
         d.nrows = g.NRot;
         d.KX = c.drf.nDist > 0 ? c.drf.KX : 1;
         d.psf = psf.p;
+        d.colRange = colRange.p;
         if (bck)
             d.rowKernel = (model & 2) ? (bf16 ? rowKernel16B.p : rowKernelB.p) : rowNone.p;
         else
@@ -1630,6 +1638,21 @@ Recon::Recon(const Config &cfg) : impl_(new Impl), cfg_(cfg)
                     padded[((size_t)d * KX + i) * KYP + j + oy] =
                         cfg.drf.psf[((size_t)d * KY + j) * KX + i];
         up(I.psf, padded);
+        // first and last column of each entry holding a nonzero tap
+        std::vector<int2> cr(cfg.drf.nDist);
+        for (int d = 0; d < cfg.drf.nDist; ++d) {
+            int lo = KX, hi = -1;
+            for (int i = 0; i < KX; ++i)
+                for (int j = 0; j < KY; ++j)
+                    if (cfg.drf.psf[((size_t)d * KY + j) * KX + i] != 0.0f) {
+                        lo = std::min(lo, i);
+                        hi = std::max(hi, i);
+                    }
+            cr[d] = hi < 0 ? make_int2(0, -1) : make_int2(lo, hi);
+        }
+        I.colRange.alloc(cr.size());
+        I.colRange.upload(cr.data(), cr.size(), I.stream);
+        HIP_CHECK(hipStreamSynchronize(I.stream));   // cr dies with this block
         if (cfg.precision > 0) {
             if (g.Nz % 4)
                 fail(ETYPE_ILLEGAL_VALUE, "the BF16 DRF path needs NumSlices %% 4 == 0 (%d)", g.Nz);
@@ -2134,14 +2157,34 @@ void ReconBatch::step(const std::vector<int> &subset, const std::vector<SrfMode>
         members.push_back(i);
         graphs.push_back(it->second);
         topo.push_back(i);
+        topo.push_back(key.s);
         topo.push_back(key.srf);
         topo.push_back(key.learned);
     }
     if (!members.empty()) {
-        // the parent graph of this topology: one child node per member, no edges
+        // the parent graph of this combination of members and subsets (the
+        // combinations recur with the subset cycles): one child node per
+        // member, no edges.  Past a bound on the cache, an executable of the
+        // same topology gets its child graphs swapped instead.
         auto pit = B.parents.find(topo);
+        bool swap = false;
+        if (pit == B.parents.end() && B.parents.size() >= 64) {
+            for (auto q = B.parents.begin(); q != B.parents.end(); ++q) {
+                const std::vector<int> &k = q->first;
+                bool same = k.size() == topo.size();
+                for (size_t j = 0; same && j < k.size(); j += 4)
+                    same = k[j] == topo[j] && k[j + 2] == topo[j + 2] && k[j + 3] == topo[j + 3];
+                if (same) {
+                    Impl::Parent P = q->second;
+                    B.parents.erase(q);
+                    pit = B.parents.emplace(topo, P).first;
+                    swap = true;
+                    break;
+                }
+            }
+        }
         bool fresh = pit == B.parents.end();
-        if (!fresh) {
+        if (swap) {
             for (size_t k = 0; k < members.size(); ++k)
                 if (hipGraphExecChildGraphNodeSetParams(pit->second.exec, pit->second.node[k],
                                                         graphs[k]) != hipSuccess) {

@@ -663,7 +663,7 @@ static int lines_per_block(int P) { return 256 / fft_threads(P); }
 static size_t fft_lds(int P) { return (size_t)lines_per_block(P) * (P + P / 16 + 1) * sizeof(float2); }
 
 // Half-spectrum width along the fastest axis: kf <= P/2, padded to 16.
-static int half_width(int P) { return std::min(P, ((P / 2 + 1 + 15) / 16) * 16); }
+static int half_width(int P) { return std::min(P, P / 2 + 1); }
 
 #define ESSE_P_DISPATCH(P, LAUNCH)                                                           \
     do {                                                                                      \
