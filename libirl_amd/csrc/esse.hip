@@ -118,7 +118,7 @@ __device__ __forceinline__ void dft(float2 *x)
     if constexpr (r >= 16) dft_stage<r, 8, INV>(x);
 }
 
-// cos and sin of 2*pi*n/60 (the radix-10 and radix-12 DFTs use n = 6k, 5k).
+// cos and sin of 2*pi*n/60 (the radix-10 and radix-12 twiddles are n = 6k, 5k).
 __device__ constexpr float kCos60[60] = {1.000000000f, 0.994521895f, 0.978147601f, 0.951056516f, 0.913545458f, 0.866025404f, 0.809016994f, 0.743144825f, 0.669130606f, 0.587785252f, 0.500000000f, 0.406736643f, 0.309016994f, 0.207911691f, 0.104528463f, 0.000000000f, -0.104528463f, -0.207911691f, -0.309016994f, -0.406736643f, -0.500000000f, -0.587785252f, -0.669130606f, -0.743144825f, -0.809016994f, -0.866025404f, -0.913545458f, -0.951056516f, -0.978147601f, -0.994521895f, -1.000000000f, -0.994521895f, -0.978147601f, -0.951056516f, -0.913545458f, -0.866025404f, -0.809016994f, -0.743144825f, -0.669130606f, -0.587785252f, -0.500000000f, -0.406736643f, -0.309016994f, -0.207911691f, -0.104528463f, -0.000000000f, 0.104528463f, 0.207911691f, 0.309016994f, 0.406736643f, 0.500000000f, 0.587785252f, 0.669130606f, 0.743144825f, 0.809016994f, 0.866025404f, 0.913545458f, 0.951056516f, 0.978147601f, 0.994521895f};
 __device__ constexpr float kSin60[60] = {0.000000000f, 0.104528463f, 0.207911691f, 0.309016994f, 0.406736643f, 0.500000000f, 0.587785252f, 0.669130606f, 0.743144825f, 0.809016994f, 0.866025404f, 0.913545458f, 0.951056516f, 0.978147601f, 0.994521895f, 1.000000000f, 0.994521895f, 0.978147601f, 0.951056516f, 0.913545458f, 0.866025404f, 0.809016994f, 0.743144825f, 0.669130606f, 0.587785252f, 0.500000000f, 0.406736643f, 0.309016994f, 0.207911691f, 0.104528463f, 0.000000000f, -0.104528463f, -0.207911691f, -0.309016994f, -0.406736643f, -0.500000000f, -0.587785252f, -0.669130606f, -0.743144825f, -0.809016994f, -0.866025404f, -0.913545458f, -0.951056516f, -0.978147601f, -0.994521895f, -1.000000000f, -0.994521895f, -0.978147601f, -0.951056516f, -0.913545458f, -0.866025404f, -0.809016994f, -0.743144825f, -0.669130606f, -0.587785252f, -0.500000000f, -0.406736643f, -0.309016994f, -0.207911691f, -0.104528463f};
 
@@ -157,11 +157,103 @@ __device__ __forceinline__ void dft_direct_out(const float2 *x, float2 *y)
     }
 }
 
+// x - i*y and x + i*y
+__device__ __forceinline__ float2 c_mi(float2 a, float2 b) { return make_float2(a.x + b.y, a.y - b.x); }
+__device__ __forceinline__ float2 c_pi(float2 a, float2 b) { return make_float2(a.x - b.y, a.y + b.x); }
+
+// 3- and 5-point DFTs in place, natural order (y_k = sum_n x_n W^{nk},
+// W = exp(-+2 pi i / r)), from the symmetric sums and differences of the
+// input pairs (n, r - n).
+template <bool INV>
+__device__ __forceinline__ void dft3(float2 &x0, float2 &x1, float2 &x2)
+{
+    constexpr float h = 0.866025404f;   // sin(2 pi/3)
+    const float2 t = cadd(x1, x2), d = csub(x1, x2);
+    const float2 m = make_float2(x0.x - 0.5f * t.x, x0.y - 0.5f * t.y);
+    const float2 b = make_float2(h * d.x, h * d.y);
+    x0 = cadd(x0, t);
+    x1 = INV ? c_pi(m, b) : c_mi(m, b);
+    x2 = INV ? c_mi(m, b) : c_pi(m, b);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft5(float2 &x0, float2 &x1, float2 &x2, float2 &x3, float2 &x4)
+{
+    constexpr float c1 = 0.309016994f, c2 = -0.809016994f;   // cos(2 pi/5), cos(4 pi/5)
+    constexpr float s1 = 0.951056516f, s2 = 0.587785252f;    // sin(2 pi/5), sin(4 pi/5)
+    const float2 t1 = cadd(x1, x4), t2 = cadd(x2, x3), t3 = csub(x1, x4), t4 = csub(x2, x3);
+    const float2 a1 = make_float2(x0.x + c1 * t1.x + c2 * t2.x, x0.y + c1 * t1.y + c2 * t2.y);
+    const float2 a2 = make_float2(x0.x + c2 * t1.x + c1 * t2.x, x0.y + c2 * t1.y + c1 * t2.y);
+    const float2 b1 = make_float2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
+    const float2 b2 = make_float2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
+    x0 = cadd(x0, cadd(t1, t2));
+    x1 = INV ? c_pi(a1, b1) : c_mi(a1, b1);
+    x4 = INV ? c_mi(a1, b1) : c_pi(a1, b1);
+    x2 = INV ? c_pi(a2, b2) : c_mi(a2, b2);
+    x3 = INV ? c_mi(a2, b2) : c_pi(a2, b2);
+}
+
+// 10 = 2 x 5: DFT5 of the even and odd samples, then one radix-2 stage with
+// the twiddles W10^k (= W60^{6k}).
+template <bool INV>
+__device__ __forceinline__ void dft10(float2 *x)
+{
+    float2 e0 = x[0], e1 = x[2], e2 = x[4], e3 = x[6], e4 = x[8];
+    float2 o0 = x[1], o1 = x[3], o2 = x[5], o3 = x[7], o4 = x[9];
+    dft5<INV>(e0, e1, e2, e3, e4);
+    dft5<INV>(o0, o1, o2, o3, o4);
+    o1 = rot60<6, INV>(o1);
+    o2 = rot60<12, INV>(o2);
+    o3 = rot60<18, INV>(o3);
+    o4 = rot60<24, INV>(o4);
+    x[0] = cadd(e0, o0);
+    x[5] = csub(e0, o0);
+    x[1] = cadd(e1, o1);
+    x[6] = csub(e1, o1);
+    x[2] = cadd(e2, o2);
+    x[7] = csub(e2, o2);
+    x[3] = cadd(e3, o3);
+    x[8] = csub(e3, o3);
+    x[4] = cadd(e4, o4);
+    x[9] = csub(e4, o4);
+}
+
+// 12 = 3 x 4: X[k1 + 3 k2] = sum_j W4^{j k2} W12^{j k1} DFT3(x[j], x[j+4], x[j+8])[k1].
+template <bool INV>
+__device__ __forceinline__ void dft12(float2 *x)
+{
+    float2 d[4][3];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        d[j][0] = x[j];
+        d[j][1] = x[j + 4];
+        d[j][2] = x[j + 8];
+        dft3<INV>(d[j][0], d[j][1], d[j][2]);
+    }
+    d[1][1] = rot60<5, INV>(d[1][1]);
+    d[1][2] = rot60<10, INV>(d[1][2]);
+    d[2][1] = rot60<10, INV>(d[2][1]);
+    d[2][2] = rot60<20, INV>(d[2][2]);
+    d[3][1] = rot60<15, INV>(d[3][1]);
+    d[3][2] = rot60<30, INV>(d[3][2]);
+#pragma unroll
+    for (int k1 = 0; k1 < 3; ++k1) {
+        float2 t[4] = {d[0][k1], d[1][k1], d[2][k1], d[3][k1]};
+        dft<4, INV>(t);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) x[k1 + 3 * k2] = t[k2];
+    }
+}
+
 template <int r, bool INV>
 __device__ __forceinline__ void dft_any(float2 *x)
 {
     if constexpr ((r & (r - 1)) == 0) {
         dft<r, INV>(x);
+    } else if constexpr (r == 10) {
+        dft10<INV>(x);
+    } else if constexpr (r == 12) {
+        dft12<INV>(x);
     } else {
         float2 y[r];
         dft_direct_out<r, INV, 0>(x, y);
