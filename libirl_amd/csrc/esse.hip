@@ -267,14 +267,14 @@ __device__ __forceinline__ void dft_any(float2 *x)
 // every thread owns R/r whole units in every pass.
 template <int P>
 struct FftPlan;
-template <> struct FftPlan<16> { static constexpr int T = 1, R = 16, r0 = 16, r1 = 0, r2 = 0; };
-template <> struct FftPlan<32> { static constexpr int T = 2, R = 16, r0 = 16, r1 = 2, r2 = 0; };
-template <> struct FftPlan<64> { static constexpr int T = 4, R = 16, r0 = 16, r1 = 4, r2 = 0; };
-template <> struct FftPlan<128> { static constexpr int T = 8, R = 16, r0 = 16, r1 = 8, r2 = 0; };
-template <> struct FftPlan<160> { static constexpr int T = 8, R = 20, r0 = 4, r1 = 4, r2 = 10; };
-template <> struct FftPlan<192> { static constexpr int T = 16, R = 12, r0 = 4, r1 = 4, r2 = 12; };
-template <> struct FftPlan<256> { static constexpr int T = 16, R = 16, r0 = 16, r1 = 16, r2 = 0; };
-template <> struct FftPlan<512> { static constexpr int T = 32, R = 16, r0 = 16, r1 = 16, r2 = 2; };
+template <> struct FftPlan<16> { static constexpr int T = 1, R = 16, r0 = 16, r1 = 0, r2 = 0, r3 = 0, r4 = 0; };
+template <> struct FftPlan<32> { static constexpr int T = 2, R = 16, r0 = 16, r1 = 2, r2 = 0, r3 = 0, r4 = 0; };
+template <> struct FftPlan<64> { static constexpr int T = 4, R = 16, r0 = 16, r1 = 4, r2 = 0, r3 = 0, r4 = 0; };
+template <> struct FftPlan<128> { static constexpr int T = 8, R = 16, r0 = 16, r1 = 8, r2 = 0, r3 = 0, r4 = 0; };
+template <> struct FftPlan<160> { static constexpr int T = 8, R = 20, r0 = 4, r1 = 4, r2 = 10, r3 = 0, r4 = 0; };
+template <> struct FftPlan<192> { static constexpr int T = 16, R = 12, r0 = 4, r1 = 4, r2 = 12, r3 = 0, r4 = 0; };
+template <> struct FftPlan<256> { static constexpr int T = 16, R = 16, r0 = 16, r1 = 16, r2 = 0, r3 = 0, r4 = 0; };
+template <> struct FftPlan<512> { static constexpr int T = 32, R = 16, r0 = 16, r1 = 16, r2 = 2, r3 = 0, r4 = 0; };
 
 // Line FFT of length P: thread tt holds x[tt + T*w] in a[w] (w < R) on
 // entry and X[tt + T*w] on exit.  Pass with radix r and stride S: unit
@@ -295,7 +295,12 @@ __device__ __forceinline__ int fft_skew(int i) { return i + (i >> 4); }
 template <int P, int IDX>
 __host__ __device__ constexpr int fft_radix()
 {
-    return IDX == 0 ? FftPlan<P>::r0 : IDX == 1 ? FftPlan<P>::r1 : IDX == 2 ? FftPlan<P>::r2 : 0;
+    return IDX == 0   ? FftPlan<P>::r0
+           : IDX == 1 ? FftPlan<P>::r1
+           : IDX == 2 ? FftPlan<P>::r2
+           : IDX == 3 ? FftPlan<P>::r3
+           : IDX == 4 ? FftPlan<P>::r4
+                      : 0;
 }
 
 template <int P, int IDX, int S, bool INV>
