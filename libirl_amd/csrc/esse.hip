@@ -524,16 +524,13 @@ __global__ __launch_bounds__(256) void k_fft_y_conv(const float2 *__restrict__ b
     constexpr int LB = 256 / F::T;
     extern __shared__ float2 sm[];
     const int li = threadIdx.x % LB, tt = threadIdx.x / LB;
-    const int ngroups = (Wc + LB - 1) / LB;
+    // columns (m, kc) flattened: a block's LB lanes take LB consecutive
+    // elements of a depth row, across m boundaries, none idle but the last
+    const long ystride = (long)Pm * Wc;
     long bid = xcd_block();
     const int b = (int)(bid % B);
-    bid /= B;
-    const int g = (int)(bid % ngroups);
-    const int m = (int)(bid / ngroups);
-    const int c = g * LB + li;
-    const bool act = c < Wc;
-    const long ystride = (long)Pm * Wc;
-    const long col = (long)m * Wc + c;
+    const long col = (bid / B) * LB + li;
+    const bool act = col < ystride;
     const float2 *src = buf + (long)b * ny * ystride + col;
     float2 a[F::R];
 #pragma unroll
@@ -893,7 +890,7 @@ void EsseDev::convolve(const float *act, int B, float *terms)
     fft_f_fwd(src, buf_, (long)B * ny * nm, nm, nf);
     fft_strided(Pm, false, buf_, buf_, (long)B * ny, plane, Wc, Wc, nm, Pm, twm_);
     for (int t = 0; t < T; ++t) {
-        const dim3 grid((unsigned)((long)B * cdiv(Wc, lines_per_block(Py)) * Pm));
+        const dim3 grid((unsigned)((long)B * cdiv((long)Pm * Wc, lines_per_block(Py))));
         const float2 *kh = khat_ + (long)Py * plane * t;
         float2 *ok = outk_ + (long)t * ny * plane;
         const long obs = (long)T * ny * plane;
