@@ -464,20 +464,22 @@ __global__ __launch_bounds__(256) void k_fft_f_inv(const float2 *__restrict__ bu
 
 // Strided lines: element n of line (bi, c), c < W, is at bi*bstride +
 // n*nstride + c.  Input n >= nIn is zero; n < nOut is written to dst (may
-// alias src).  Blocks take LB consecutive columns of one base.
+// alias src).  Blocks take LB consecutive lines of the flattened (bi, c)
+// order, so only the very last block has idle lanes.
 template <int P, bool INV>
 __global__ __launch_bounds__(256) void k_fft_strided(const float2 *src, float2 *dst, long bstride,
                                                      long nstride, int W, int nIn, int nOut,
-                                                     const float2 *__restrict__ tw)
+                                                     long nlines, const float2 *__restrict__ tw)
 {
     using F = FftLine<P>;
     constexpr int LB = 256 / F::T;
     extern __shared__ float2 sm[];
     const int li = threadIdx.x % LB, tt = threadIdx.x / LB;
-    const int ngroups = (W + LB - 1) / LB;
-    const long bi = blockIdx.x / ngroups;
-    const int c = (int)(blockIdx.x % ngroups) * LB + li;
-    const bool act = c < W;
+    // lines (base, column) flattened, LB consecutive ones per block
+    const long line = (long)blockIdx.x * LB + li;
+    const bool act = line < nlines;
+    const long bi = act ? line / W : 0;
+    const int c = act ? (int)(line - bi * W) : 0;
     const long off = bi * bstride + c;
     float2 a[F::R];
 #pragma unroll
@@ -796,15 +798,16 @@ void EsseDev::fft_f_inv(const float2 *in, float *out, long rows, int nm, int nf,
 void EsseDev::fft_strided(int P, bool inv, const float2 *src, float2 *dst, long nbases,
                           long bstride, long nstride, int W, int nIn, int nOut, const float2 *tw)
 {
-    const dim3 grid((unsigned)(nbases * cdiv(W, lines_per_block(P))));
+    const long nlines = nbases * W;
+    const dim3 grid((unsigned)cdiv(nlines, lines_per_block(P)));
 #define L_(PP)                                                                                  \
     do {                                                                                        \
         if (inv)                                                                                \
             hipLaunchKernelGGL((k_fft_strided<PP, true>), grid, dim3(256), fft_lds(PP), stream_, \
-                               src, dst, bstride, nstride, W, nIn, nOut, tw);                   \
+                               src, dst, bstride, nstride, W, nIn, nOut, nlines, tw);           \
         else                                                                                    \
             hipLaunchKernelGGL((k_fft_strided<PP, false>), grid, dim3(256), fft_lds(PP),        \
-                               stream_, src, dst, bstride, nstride, W, nIn, nOut, tw);          \
+                               stream_, src, dst, bstride, nstride, W, nIn, nOut, nlines, tw);  \
     } while (0)
     ESSE_P_DISPATCH(P, L_);
 #undef L_
