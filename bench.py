@@ -282,30 +282,35 @@ def batch_line(tmpdir, net, steps=16, warmup=16):
         L.set_autotune(prev[2])
         L.set_scatter_cnn(None)
     batch = L.Batch(recs)
+
+    def timed(parent):
+        batch.set_parent_graph(parent)
+        batch.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            batch.step()
+        batch.synchronize()
+        return time.perf_counter() - t0
+
     for _ in range(warmup):
         batch.step()
-    batch.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        batch.step()
-    batch.synchronize()
-    dt_batch = time.perf_counter() - t0
+    dts = [timed(False)]
     st = batch.stats()
     est = [r.estimate() for r in recs]          # after warmup + steps = 4 iterations
     tiles = [r.tiles() for r in recs]
     status = [L.cnn_status(r) for r in recs]
-    # the same steps as one parent-graph launch per step
+    # the same steps as one parent-graph launch per step (its executables
+    # instantiated first), each mode timed twice, alternating
     S = [r.num_subsets for r in recs]
     batch.set_parent_graph(True)
     for _ in range(max(S)):
-        batch.step()                            # instantiates the parent graphs
-    batch.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
         batch.step()
-    batch.synchronize()
-    dt_parent = time.perf_counter() - t0
+    dtp = [timed(True)]
+    dts.append(timed(False))
+    dtp.append(timed(True))
+    dt_batch, dt_parent = min(dts), min(dtp)
     pst = batch.stats()
+    batch.set_parent_graph(False)
     # one reconstruction at a time, each sub-iteration finished before the next
     t0 = time.perf_counter()
     for k in range(steps):
@@ -326,8 +331,10 @@ def batch_line(tmpdir, net, steps=16, warmup=16):
                         "learned scatter, BF16 DRF",
             "value": n * steps / dt_batch, "unit": "subiters/s (all reconstructions)",
             "ms_per_step": dt_batch * 1e3 / steps,
-            "mode": "each member's sub-iteration graph launched on its own stream",
+            "mode": "each member's sub-iteration graph launched on its own stream (best of 2 timed runs)",
+            "member_streams_runs": [n * steps / t for t in dts],
             "parent_graph_value": n * steps / dt_parent,
+            "parent_graph_runs": [n * steps / t for t in dtp],
             "parent_graph_launches": pst["graph_launches"],
             "one_at_a_time_value": n * steps / dt_seq,
             "batch_speedup_vs_one_at_a_time": dt_seq / dt_batch,

@@ -129,6 +129,42 @@ __global__ __launch_bounds__(256) void k_cnn_input(const float *__restrict__ x,
     in[2 * v + 1] = s1 * 0.125f * densFac;
 }
 
+// Inference input: 2x2x2 means of x (not yet divided by mean(x): layer 1
+// applies 1/mean(x) when it stages its input) and of mu * densFac, plus the
+// block sums of x (FP64) for that mean.
+__global__ __launch_bounds__(256) void k_cnn_pool(const float *__restrict__ x,
+                                                  const float *__restrict__ mu, int N, int Nz,
+                                                  float densFac, float *__restrict__ in,
+                                                  double *__restrict__ part)
+{
+    __shared__ double red[256];
+    const int Y = N / 2, X = N / 2, Z = Nz / 2;
+    const long v = blockIdx.x * 256L + threadIdx.x;
+    float s0 = 0.0f, s1 = 0.0f;
+    if (v < (long)Y * X * Z) {
+        const int z = (int)(v % Z);
+        const long r = v / Z;
+        const int xx = (int)(r % X), y = (int)(r / X);
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 2; ++dx) {
+                const long o = ((long)(2 * y + dy) * N + 2 * xx + dx) * Nz + 2 * z;
+                s0 += x[o] + x[o + 1];
+                if (mu) s1 += mu[o] + mu[o + 1];
+            }
+        in[2 * v] = s0 * 0.125f;
+        in[2 * v + 1] = s1 * 0.125f * densFac;
+    }
+    red[threadIdx.x] = (double)s0;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
 // Training targets from the ESSE terms (E0, E1 volumes of one sample).
 __global__ __launch_bounds__(256) void k_cnn_target(const float *__restrict__ E0,
                                                     const float *__restrict__ E1, int N, int Nz,
@@ -218,6 +254,69 @@ __global__ __launch_bounds__(256) void k_conv_valu(const TI *__restrict__ in,
             if (mask && !(mask[v * CO + co] > 0.0f)) val = 0.0f;
             from_f(out[v * CO + co], val);
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Layer 1 (2 -> 32 channels, dilation 1) of the inference path on the VALU:
+// a wave owns 64 slices of one (y, x) row and stages the 9 neighbour rows'
+// slices [z0 - 1, z0 + 65) of both input channels in LDS (channel 0 times
+// scal[0]), then each lane accumulates its voxel's 32 outputs over the 27
+// taps from LDS in the order of k_conv_valu (same arithmetic), bf16 out.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_conv_l1_rows(const float2 *__restrict__ in,
+                                                      const float *__restrict__ W,
+                                                      const float *__restrict__ bias,
+                                                      const float *__restrict__ scal,
+                                                      uint16_t *__restrict__ out, int Y, int X,
+                                                      int Z)
+{
+    constexpr int SL = 64 + 2;
+    __shared__ float2 seg[4][9 * SL];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int zt = (Z + 63) / 64;
+    const long task = (long)blockIdx.x * 4 + w;
+    const bool valid = task < (long)Y * X * zt;
+    const long row = valid ? task / zt : 0;
+    const int z0 = valid ? (int)(task % zt) * 64 : 0;
+    const int y = (int)(row / X), x = (int)(row % X);
+    const float s0 = scal[0];
+    for (int e = lane; e < 9 * SL; e += 64) {
+        const int q = e / SL, k = e - q * SL;
+        const int yy = y + q / 3 - 1, xx = x + q % 3 - 1, zz = z0 - 1 + k;
+        const bool ok = valid && yy >= 0 && yy < Y && xx >= 0 && xx < X && zz >= 0 && zz < Z;
+        float2 v = ok ? in[((long)yy * X + xx) * Z + zz] : make_float2(0.0f, 0.0f);
+        v.x = v.x * s0;
+        seg[w][e] = v;
+    }
+    __syncthreads();
+    const int z = z0 + lane;
+    if (!valid || z >= Z) return;
+    float acc[32];
+#pragma unroll
+    for (int co = 0; co < 32; ++co) acc[co] = bias[co];
+    for (int q = 0; q < 9; ++q)
+#pragma unroll
+        for (int dz = 0; dz < 3; ++dz) {
+            const float2 v = seg[w][q * SL + lane + dz];
+            const float *wt = W + (3 * q + dz) * 64;
+#pragma unroll
+            for (int co = 0; co < 32; ++co) acc[co] = fmaf(wt[co], v.x, acc[co]);
+#pragma unroll
+            for (int co = 0; co < 32; ++co) acc[co] = fmaf(wt[32 + co], v.y, acc[co]);
+        }
+    uint16_t *o = out + (((long)y * X + x) * Z + z) * 32;
+#pragma unroll
+    for (int c8 = 0; c8 < 32; c8 += 8) {
+        uint32_t qv[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            uint16_t h0, h1;
+            from_f(h0, fmaxf(acc[c8 + 2 * k], 0.0f));
+            from_f(h1, fmaxf(acc[c8 + 2 * k + 1], 0.0f));
+            qv[k] = (uint32_t)h0 | ((uint32_t)h1 << 16);
+        }
+        *(uint4 *)(o + c8) = make_uint4(qv[0], qv[1], qv[2], qv[3]);
     }
 }
 
@@ -661,6 +760,9 @@ void ScatterCnn::alloc_state()
     Wt_ = cnn_alloc<float>(27L * 32 * 32);
     Wr0_ = cnn_alloc<float>(wcount(0));
     dsum_ = cnn_alloc<double>(1);
+    one_ = cnn_alloc<float>(2);
+    const float ones[2] = {1.0f, 1.0f};
+    CNN_CHECK(hipMemcpy(one_, ones, sizeof ones, hipMemcpyHostToDevice));
 }
 
 ScatterCnn::ScatterCnn(float kappa, unsigned seed) : kappa_(kappa)
@@ -698,7 +800,7 @@ CnnWork::CnnWork(int N, int Nz)
     in = cnn_alloc<float>(V * 2);
     out = cnn_alloc<float>(V * 2);
     for (auto &a : actb) a = cnn_alloc<uint16_t>(V * 32);
-    nb = std::min<long>(1024, cdiv_u((long)N * N * Nz, 256));
+    nb = cdiv_u(V, 256);
     psum = cnn_alloc<double>(nb);
     scal = cnn_alloc<float>(2);
 }
@@ -731,6 +833,7 @@ ScatterCnn::~ScatterCnn()
     cnn_free(Wt_);
     cnn_free(Wr0_);
     cnn_free(dsum_);
+    cnn_free(one_);
 }
 
 // packed (oracle) order per layer: W [27][Cout][Cin], b [Cout]
@@ -846,18 +949,18 @@ void ScatterCnn::forward(const float *in, int Y, int X, int Z, float *out, bool 
         return;
     }
     upload_bf16_weights(s);
-    forward_bf16(in, Y, X, Z, out, actb_[0], actb_[1], s);
+    forward_bf16(in, Y, X, Z, out, actb_[0], actb_[1], nullptr, s);
 }
 
-// Inference arithmetic, activations in the given buffers ([V][32] bf16 each).
+// Inference arithmetic, activations in the given buffers ([V][32] bf16 each);
+// channel 0 of the input is multiplied by scal[0] (1 for normalised input).
 void ScatterCnn::forward_bf16(const float *in, int Y, int X, int Z, float *out, uint16_t *a0,
-                              uint16_t *a1, hipStream_t s)
+                              uint16_t *a1, const float *scal, hipStream_t s)
 {
-    const long V = (long)Y * X * Z;
-    const dim3 grid(cdiv_u(V, 256)), blk(256);
-    hipLaunchKernelGGL((k_conv_valu<32, float, uint16_t>), grid, blk, 0, s, in, Wr0_, b_[0],
-                       nullptr, a0, Y, X, Z, 2, cnn_dil(0), 1);
+    const dim3 blk(256);
     const long tasks = (long)Y * X * ((Z + CNN_ROWZ - 1) / CNN_ROWZ);
+    hipLaunchKernelGGL(k_conv_l1_rows, dim3(cdiv_u(tasks, 4)), blk, 0, s, (const float2 *)in,
+                       Wr0_, b_[0], scal ? scal : one_, a0, Y, X, Z);
     const dim3 rgrid(cdiv_u(tasks, 4));
     auto lds = [](int dil) { return (size_t)4 * (CNN_ROWZ + 2 * dil) * 4 * sizeof(uint4); };
     hipLaunchKernelGGL((k_conv_mfma_rows<2, 32>), rgrid, blk, lds(cnn_dil(1)), s, a0,
@@ -972,10 +1075,11 @@ void ScatterCnn::sources(const float *x, const float *mu, int N, int Nz, const C
     if (N % 2 || Nz % 2 || V != w.V)
         fail(ETYPE_ILLEGAL_VALUE, "scatter CNN work buffers made for another size");
     if (bfDirty_) fail(ETYPE_ILLEGAL_VALUE, "scatter CNN inference weights not prepared");
-    mean_scale(x, n, w.psum, w.nb, w.scal, s);
-    hipLaunchKernelGGL(k_cnn_input, dim3(cdiv_u(V, 256)), dim3(256), 0, s, x, mu, N, Nz, w.scal,
-                       ph.fDens, w.in);
-    forward_bf16(w.in, Y, X, Z, w.out, w.actb[0], w.actb[1], s);
+    const unsigned pb = cdiv_u(V, 256);
+    if (pb > w.nb) fail(ETYPE_ILLEGAL_VALUE, "scatter CNN work buffers too small");
+    hipLaunchKernelGGL(k_cnn_pool, dim3(pb), dim3(256), 0, s, x, mu, N, Nz, ph.fDens, w.in, w.psum);
+    hipLaunchKernelGGL(k_inv_mean, dim3(1), dim3(256), 0, s, w.psum, (int)pb, n, w.scal);
+    forward_bf16(w.in, Y, X, Z, w.out, w.actb[0], w.actb[1], w.scal, s);
     hipLaunchKernelGGL(k_cnn_source, dim3(cdiv_u(n, 256)), dim3(256), 0, s, w.out, x, mu, N, Nz,
                        w.scal, kappa_, ph.fDens, ph.cntr, primary, g0, g1);
     CNN_CHECK(hipGetLastError());

@@ -55,7 +55,7 @@ struct CnnWork {
     long V = 0;                          // coarse voxels
     float *in = nullptr, *out = nullptr; // [V][2]
     uint16_t *actb[2] = {};              // bf16 activations [V][32]
-    double *psum = nullptr;              // partial sums of the mean
+    double *psum = nullptr;              // block sums of x (one per 256 coarse voxels)
     long nb = 0;
     float *scal = nullptr;               // [0] 1/mean(x), [1] mean(x)
     CnnWork(int N, int Nz);
@@ -109,7 +109,7 @@ private:
     void ensure_work(long V);
     void upload_bf16_weights(hipStream_t s);
     void forward_bf16(const float *in, int Y, int X, int Z, float *out, uint16_t *a0,
-                      uint16_t *a1, hipStream_t s);
+                      uint16_t *a1, const float *scal, hipStream_t s);
 
     float kappa_ = 1.0f;
     int step_ = 0;
@@ -127,6 +127,7 @@ private:
     float *grad_[2] = {};
     uint16_t *actb_[2] = {};             // bf16 activations (inference)
     double *dsum_ = nullptr;
+    float *one_ = nullptr;               // {1, 1}: the scale of normalised input
 };
 
 // g0, g1 from the scatter model's own terms E0, E1 of the unrotated
