@@ -335,8 +335,18 @@ static dim3 pix_grid(int Nz, int width, int rows, int B)
 
 // ---------------------------------------------------------------------------
 // k_att: rotated estimate times attenuation factor for the batch's views.
-// One thread per output (view, row, bin, slice), slices fastest.
+// One thread per (view, bin, slice) and ATT_ROWS consecutive rows, slices
+// fastest: the rows' independent gathers and factor loads are in flight
+// together.
 // ---------------------------------------------------------------------------
+constexpr int ATT_ROWS = 4;
+
+static dim3 att_grid(int Nz, int width, int rows, int B)
+{
+    return dim3(((Nz + 63) / 64) * ((width + PIX_Q - 1) / PIX_Q),
+                (rows + ATT_ROWS - 1) / ATT_ROWS, B);
+}
+
 template <bool SHEAR>
 __global__ __launch_bounds__(256) void k_att(const float *__restrict__ x,
                                              const float *__restrict__ afac,
@@ -350,19 +360,24 @@ __global__ __launch_bounds__(256) void k_att(const float *__restrict__ x,
     const PixIdx p = pix_index(Nz, NRot);
     if (!p.ok) return;
     const long vol = (long)Nz * NRot * NRot;
-    const int i = (p.Y * NRot + p.X) * Nz + p.z;
     const int v = bview[p.b];
-    float val = 0.0f;
-    if (SHEAR) {
-        val = shear_sample(sd, 2 * v, x + p.z, (long)N * Nz, Nz, p.X, p.Y);
-    } else if (in_circle(head, npix, p.Y, p.X)) {
-        const float NewAxis = 0.5 * (NRot - 1);
-        const float OldAxis = 0.5 * (N - 1);
-        val = rot_sample(x + p.z, (long)N * Nz, Nz, N, (float)p.X - NewAxis,
-                         (float)p.Y - NewAxis, fsin[v], fcos[v], OldAxis);
+#pragma unroll
+    for (int r = 0; r < ATT_ROWS; ++r) {
+        const int Y = p.Y * ATT_ROWS + r;
+        if (Y >= NRot) break;
+        const int i = (Y * NRot + p.X) * Nz + p.z;
+        float val = 0.0f;
+        if (SHEAR) {
+            val = shear_sample(sd, 2 * v, x + p.z, (long)N * Nz, Nz, p.X, Y);
+        } else if (in_circle(head, npix, Y, p.X)) {
+            const float NewAxis = 0.5 * (NRot - 1);
+            const float OldAxis = 0.5 * (N - 1);
+            val = rot_sample(x + p.z, (long)N * Nz, Nz, N, (float)p.X - NewAxis,
+                             (float)Y - NewAxis, fsin[v], fcos[v], OldAxis);
+        }
+        if (afac) val = val * afac[(long)v * vol + i];
+        att[(long)p.b * vol + i] = val;
     }
-    if (afac) val = val * afac[(long)v * vol + i];
-    att[(long)p.b * vol + i] = val;
 }
 
 // ---------------------------------------------------------------------------
@@ -387,23 +402,28 @@ __global__ __launch_bounds__(256) void k_att_src(const float *__restrict__ g0,
     const PixIdx p = pix_index(Nz, NRot);
     if (!p.ok) return;
     const long vol = (long)Nz * NRot * NRot;
-    const int i = (p.Y * NRot + p.X) * Nz + p.z;
     const int v = bview[p.b];
-    float v0 = 0.0f, v1 = 0.0f;
-    if (SHEAR) {
-        v0 = shear_sample(sd, 2 * v, g0 + p.z, (long)N * Nz, Nz, p.X, p.Y);
-        v1 = shear_sample(sd, 2 * v, g1 + p.z, (long)N * Nz, Nz, p.X, p.Y);
-    } else if (in_circle(head, npix, p.Y, p.X)) {
-        const float NewAxis = 0.5 * (NRot - 1);
-        const float OldAxis = 0.5 * (N - 1);
-        v0 = rot_sample(g0 + p.z, (long)N * Nz, Nz, N, (float)p.X - NewAxis,
-                        (float)p.Y - NewAxis, fsin[v], fcos[v], OldAxis);
-        v1 = rot_sample(g1 + p.z, (long)N * Nz, Nz, N, (float)p.X - NewAxis,
-                        (float)p.Y - NewAxis, fsin[v], fcos[v], OldAxis);
+#pragma unroll
+    for (int r = 0; r < ATT_ROWS; ++r) {
+        const int Y = p.Y * ATT_ROWS + r;
+        if (Y >= NRot) break;
+        const int i = (Y * NRot + p.X) * Nz + p.z;
+        float v0 = 0.0f, v1 = 0.0f;
+        if (SHEAR) {
+            v0 = shear_sample(sd, 2 * v, g0 + p.z, (long)N * Nz, Nz, p.X, Y);
+            v1 = shear_sample(sd, 2 * v, g1 + p.z, (long)N * Nz, Nz, p.X, Y);
+        } else if (in_circle(head, npix, Y, p.X)) {
+            const float NewAxis = 0.5 * (NRot - 1);
+            const float OldAxis = 0.5 * (N - 1);
+            v0 = rot_sample(g0 + p.z, (long)N * Nz, Nz, N, (float)p.X - NewAxis,
+                            (float)Y - NewAxis, fsin[v], fcos[v], OldAxis);
+            v1 = rot_sample(g1 + p.z, (long)N * Nz, Nz, N, (float)p.X - NewAxis,
+                            (float)Y - NewAxis, fsin[v], fcos[v], OldAxis);
+        }
+        const float af = afac[(long)v * vol + i];
+        const float d = af > 0.0f ? -logf(af) * invMu0 : 0.0f;
+        att[(long)p.b * vol + i] = fmaf(d, v1, v0) * af;
     }
-    const float af = afac[(long)v * vol + i];
-    const float d = af > 0.0f ? -logf(af) * invMu0 : 0.0f;
-    att[(long)p.b * vol + i] = fmaf(d, v1, v0) * af;
 }
 
 // Source check: per (view, bin, slice) column the depth sums of the model's
@@ -1270,11 +1290,11 @@ struct Recon::Impl {
     {
         const Geometry &g = c.g;
         if (c.shears)
-            hipLaunchKernelGGL(k_att<true>, pix_grid(g.Nz, g.NRot, g.NRot, B), dim3(256), 0,
+            hipLaunchKernelGGL(k_att<true>, att_grid(g.Nz, g.NRot, g.NRot, B), dim3(256), 0,
                                stream, src ? src : x.p, afac, dst ? dst : att.p, g.Nz, g.N,
                                g.NRot, bview, fsin.p, fcos.p, head.p, npix.p, shear());
         else
-            hipLaunchKernelGGL(k_att<false>, pix_grid(g.Nz, g.NRot, g.NRot, B), dim3(256), 0,
+            hipLaunchKernelGGL(k_att<false>, att_grid(g.Nz, g.NRot, g.NRot, B), dim3(256), 0,
                                stream, src ? src : x.p, afac, dst ? dst : att.p, g.Nz, g.N,
                                g.NRot, bview, fsin.p, fcos.p, head.p, npix.p, shear());
     }
@@ -1284,11 +1304,11 @@ struct Recon::Impl {
         const Geometry &g = c.g;
         const float inv = 1.0f / phys.mu0;
         if (c.shears)
-            hipLaunchKernelGGL(k_att_src<true>, pix_grid(g.Nz, g.NRot, g.NRot, B), dim3(256), 0,
+            hipLaunchKernelGGL(k_att_src<true>, att_grid(g.Nz, g.NRot, g.NRot, B), dim3(256), 0,
                                stream, g0.p, g1.p, afacp, dst, g.Nz, g.N, g.NRot, bview, fsin.p,
                                fcos.p, head.p, npix.p, shear(), inv);
         else
-            hipLaunchKernelGGL(k_att_src<false>, pix_grid(g.Nz, g.NRot, g.NRot, B), dim3(256), 0,
+            hipLaunchKernelGGL(k_att_src<false>, att_grid(g.Nz, g.NRot, g.NRot, B), dim3(256), 0,
                                stream, g0.p, g1.p, afacp, dst, g.Nz, g.N, g.NRot, bview, fsin.p,
                                fcos.p, head.p, npix.p, shear(), inv);
     }
