@@ -80,6 +80,21 @@ int irlamd_set_autotune(int bOn);
  * BF16 slice tiles per wave, of the projector and the backprojector. */
 int irlamd_recon_tiles(irlamd_recon *r, int *piPrj, int *piBck);
 
+/* FP32 DRF convolution algorithm of reconstructions created afterwards:
+ * 0 = the reference's choice (the default): the 2-D FFT kernel for tables
+ * flagged bFFTconvolve (convolve_ypad.c:62-94, conv2dPsfFFT.c:53-118), the
+ * spatial kernel otherwise; 1 = the FFT kernel for every table; 2 = the
+ * faster of the two per direction, timed at setup on the first subset;
+ * -1 = the spatial kernel always.  The FFT kernel needs slice-fastest
+ * planes with NumRotPixs + KX - 1 and NumSlices + KY - 1 at most 160 (the
+ * spatial kernel runs otherwise); its results equal the spatial kernel's to
+ * FP32 rounding of the transforms.  BF16 modes ignore the setting.
+ * Returns the previous mode, or -2 for an invalid one. */
+int irlamd_set_drf_fft(int iMode);
+/* DRF algorithm in use per direction (0 spatial, 1 FFT) and the FFT length
+ * (0: none built). */
+int irlamd_recon_drf_algorithms(irlamd_recon *r, int *piPrj, int *piBck, int *piLength);
+
 /* Several reconstructions (any sizes and view counts) stepped together: each
  * step runs the next sub-iteration of every member (subsets in order, the
  * iteration and scatter schedule of each as in IrlOsem).  Members whose

@@ -226,6 +226,8 @@ def load():
     lib.irlamd_set_graphs.argtypes = [ctypes.c_int]
     lib.irlamd_set_autotune.argtypes = [ctypes.c_int]
     lib.irlamd_recon_tiles.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    lib.irlamd_set_drf_fft.argtypes = [ctypes.c_int]
+    lib.irlamd_recon_drf_algorithms.argtypes = [vp] + [ctypes.POINTER(ctypes.c_int)] * 3
     lib.irlamd_batch_create.restype = vp
     lib.irlamd_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int]
     lib.irlamd_batch_step.argtypes = [vp]
@@ -458,6 +460,14 @@ class Recon:
         self._lib.irlamd_recon_tiles(self._h, ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
 
+    def drf_algorithms(self):
+        """FP32 DRF algorithm per direction ("spatial" or "fft") and the FFT length."""
+        a, b, n = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._lib.irlamd_recon_drf_algorithms(self._h, ctypes.byref(a), ctypes.byref(b),
+                                              ctypes.byref(n))
+        name = ("spatial", "fft")
+        return name[a.value], name[b.value], n.value
+
     def set_timing(self, on=True):
         self._lib.irlamd_recon_set_timing(self._h, 1 if on else 0)
 
@@ -626,6 +636,20 @@ def set_graphs(on):
 def set_autotune(on):
     """DRF tile autotuning at setup for reconstructions created afterwards."""
     return load().irlamd_set_autotune(1 if on else 0)
+
+
+_DRF_FFT_MODES = {"reference": 0, "fft": 1, "auto": 2, "spatial": -1}
+
+
+def set_drf_fft(mode):
+    """FP32 DRF convolution algorithm for reconstructions created afterwards:
+    "reference" (the FFT kernel for tables flagged bFFTconvolve, the default),
+    "fft" (every table), "auto" (the faster per direction, timed at setup) or
+    "spatial" (never the FFT kernel).  Returns the previous mode's name."""
+    if mode not in _DRF_FFT_MODES:
+        raise ValueError("DRF algorithm mode %r" % (mode,))
+    old = load().irlamd_set_drf_fft(_DRF_FFT_MODES[mode])
+    return {v: k for k, v in _DRF_FFT_MODES.items()}[old]
 
 
 class Batch:

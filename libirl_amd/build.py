@@ -14,7 +14,7 @@ LIB = os.path.join(HERE, "libirl_amd.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
-HIP_SOURCES = ["engine.hip", "esse.hip", "cnn.hip"]
+HIP_SOURCES = ["engine.hip", "esse.hip", "cnn.hip", "drf_fft.hip"]
 CPP_SOURCES = ["host_util.cpp", "irlosem.cpp", "prjbck_api.cpp", "abi_check.cpp", "esse_host.cpp",
                "orbit.cpp", "grf.cpp"]
 

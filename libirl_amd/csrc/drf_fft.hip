@@ -1,0 +1,436 @@
+// drf_fft.hip -- the per-plane DRF convolution of k_drf by 2-D FFT on
+// gfx950: the device form of the reference's Fourier-domain branch
+// (convolve_ypad.c:60-94, conv2dPsfFFT.c:53-118, PSF wrapped to the origin
+// as PadSwap does, padswap.c:21-119), on a P x P grid just large enough for
+// the circular convolution to equal the zero-padded linear one on the kept
+// outputs (the reference pads to 2N).  The axial padding of the spatial
+// branch (pad2d.c:21-141) is applied to the input as k_drf does, so the
+// results equal k_drf's to FP32 rounding of the transforms.
+//
+// One block transforms whole planes in LDS.  Forward: the real slice
+// lines of two bins at a time are one complex line (real and imaginary
+// parts), split into their half spectra (k <= P/2) afterwards; then the
+// P/2 + 1 complex lines along the bins.  Thread (line, tt) then holds
+// S[kz][kx = tt + T w] in registers.  PRJ multiplies every row's spectrum
+// by its PSF spectrum and accumulates in registers, so one inverse serves
+// a chunk of rows; BCK multiplies and inverts per row.  Inverse: complex
+// lines along the bins, then the half spectra of two bins completed by
+// Hermitian symmetry as one complex line, whose real and imaginary outputs
+// are the two bins' slices.  Line transforms are fft_line (fft_dev.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <vector>
+
+#include "drf_fft.h"
+#include "engine.h"
+#include "fft_dev.h"
+
+namespace irlamd {
+
+#define DFFT_CHECK(expr)                                                                 \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            ::irlamd::fail(ETYPE_DEVICE, "HIP error '%s' in %s (%s:%d)",                 \
+                           hipGetErrorString(e_), #expr, __FILE__, __LINE__);            \
+    } while (0)
+
+template <int P>
+struct DrfFftShape {
+    static constexpr int T = FftPlan<P>::T, R = FftPlan<P>::R;
+    static constexpr int LS = FftLine<P>::LS;          // LDS line stride (float2)
+    static constexpr int WC = P / 2 + 1;               // half spectrum along the slices
+    static constexpr int NL = (WC * T + 63) / 64 * 64 / T;  // line slots per block
+    static constexpr int NT = NL * T;                  // threads per block
+    static constexpr size_t LDS = (size_t)NL * LS * sizeof(float2);
+    static constexpr int KH = (P / 2) / T + 1;         // registers that can hold k <= P/2
+    static_assert(NT <= 1024, "block size");
+};
+
+// Forward 2-D transform of plane (b, row) of the DRF input (with axial pad
+// 'pad').  On exit thread (line li, tt), li < P/2 + 1, holds
+// S[kz = li][kx = tt + T w] in s[w].  Every thread of the block calls it.
+template <int P>
+__device__ __forceinline__ void drf_fft_forward(const DrfArgs &a, int b, int v, int row, int pad,
+                                                float2 *s, float2 *sm, int li, int tt,
+                                                const float2 *__restrict__ tw)
+{
+    using D = DrfFftShape<P>;
+    float2 *slot = sm + li * D::LS;
+    const int c0 = 2 * li, c1 = c0 + 1;
+    const bool zact = c0 < a.NRot, has1 = c1 < a.NRot;
+    // slice lines of bins c0 (real part) and c1 (imaginary part), slices
+    // contiguous (zs == 1): FFT index n < Nz holds slice n
+    {
+        const long e0 = (long)c0 * a.xs;
+        const float *i0 = a.in + (long)b * a.inB + (long)row * a.inRow + e0;
+        const float *f0 = a.afac ? a.afac + (long)v * a.afB + (long)row * a.afRow + e0 : nullptr;
+#pragma unroll
+        for (int w = 0; w < D::R; ++w) {
+            const int n = tt + D::T * w;
+            float re = 0.0f, im = 0.0f;
+            if (zact && n < a.Nz) {
+                re = i0[n];
+                if (has1) im = i0[n + a.xs];
+                if (f0) {
+                    re *= f0[n];
+                    if (has1) im *= f0[n + a.xs];
+                }
+            }
+            s[w] = make_float2(re, im);
+        }
+    }
+    if (pad > 0 && a.avg > 0 && zact) {
+        // axial pad: FFT index n in [Nz, Nz + pad) holds the mean of the
+        // last avg slices, n in [P - pad, P) (slices -pad..-1) that of the
+        // first avg (pad2d.c:21-141, as drf_stage_val)
+        const float lo0 = drf_stage_val(a, b, v, row, -1, c0, pad);
+        const float lo1 = drf_stage_val(a, b, v, row, -1, c1, pad);
+        const float hi0 = drf_stage_val(a, b, v, row, a.Nz, c0, pad);
+        const float hi1 = drf_stage_val(a, b, v, row, a.Nz, c1, pad);
+#pragma unroll
+        for (int w = 0; w < D::R; ++w) {
+            const int n = tt + D::T * w;
+            if (n >= a.Nz && n < a.Nz + pad) s[w] = make_float2(hi0, hi1);
+            if (n >= P - pad) s[w] = make_float2(lo0, lo1);
+        }
+    }
+    fft_line<P, false>(s, slot, tt, tw);
+    // Between phases the lane indices pass through an empty asm, so each
+    // phase computes its own LDS addresses instead of the compiler keeping
+    // those of every phase live at once.
+    asm volatile("" : "+v"(tt), "+v"(li));
+    slot = sm + li * D::LS;
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < D::R; ++w) slot[fft_skew(tt + D::T * w)] = s[w];
+    __syncthreads();
+    // Z = A + iB: A[k] = (Z[k] + conj Z[P-k]) / 2, B[k] = (Z[k] - conj Z[P-k]) / 2i
+    float2 A[D::KH], Bq[D::KH];
+#pragma unroll
+    for (int w = 0; w < D::KH; ++w) {
+        const int k = tt + D::T * w;
+        const float2 zk = s[w];
+        const float2 zm = slot[fft_skew(k == 0 ? 0 : P - k)];
+        A[w] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+        Bq[w] = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
+    }
+    __syncthreads();
+    if (zact) {
+#pragma unroll
+        for (int w = 0; w < D::KH; ++w) {
+            const int k = tt + D::T * w;
+            if (k <= P / 2) {
+                sm[k * D::LS + fft_skew(c0)] = A[w];
+                if (c1 < a.NRot) sm[k * D::LS + fft_skew(c1)] = Bq[w];
+            }
+        }
+    }
+    __syncthreads();
+    asm volatile("" : "+v"(tt), "+v"(li));
+    slot = sm + li * D::LS;
+    // bin lines: line kz = li, bins >= NRot are zero
+    const bool xact = li < D::WC;
+#pragma unroll
+    for (int w = 0; w < D::R; ++w) {
+        const int x = tt + D::T * w;
+        s[w] = (xact && x < a.NRot) ? slot[fft_skew(x)] : make_float2(0.0f, 0.0f);
+    }
+    fft_line<P, false>(s, slot, tt, tw);
+}
+
+// Inverse 2-D transform of the half spectrum held as drf_fft_forward leaves
+// it (already scaled by 1/P^2).  On exit thread (line li, tt) holds slice
+// n = tt + T w of bin 2 li in g[w].x and of bin 2 li + 1 in g[w].y.
+template <int P>
+__device__ __forceinline__ void drf_fft_inverse(const DrfArgs &a, float2 *g, float2 *sm, int li,
+                                                int tt, const float2 *__restrict__ tw)
+{
+    using D = DrfFftShape<P>;
+    float2 *slot = sm + li * D::LS;
+    fft_line<P, true>(g, slot, tt, tw);
+    asm volatile("" : "+v"(tt), "+v"(li));
+    slot = sm + li * D::LS;
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < D::R; ++w) {
+        const int x = tt + D::T * w;
+        if (x < a.NRot) slot[fft_skew(x)] = g[w];
+    }
+    __syncthreads();
+    asm volatile("" : "+v"(tt), "+v"(li));
+    slot = sm + li * D::LS;
+    const int c0 = 2 * li, c1 = c0 + 1;
+    const bool has0 = c0 < a.NRot, has1 = c1 < a.NRot;
+#pragma unroll
+    for (int w = 0; w < D::R; ++w) {
+        const int n = tt + D::T * w;
+        const bool upper = n > P / 2;
+        const int kk = upper ? P - n : n;
+        float2 A = has0 ? sm[kk * D::LS + fft_skew(c0)] : make_float2(0.0f, 0.0f);
+        float2 B = has1 ? sm[kk * D::LS + fft_skew(c1)] : make_float2(0.0f, 0.0f);
+        if (upper) {
+            A.y = -A.y;
+            B.y = -B.y;
+        }
+        g[w] = make_float2(A.x - B.y, A.y + B.x);
+    }
+    __syncthreads();
+    fft_line<P, true>(g, slot, tt, tw);
+}
+
+// PRJ: block = (batch item b, chunk ch) sums rows [ch rchunk, ...) of
+// DRF_row (*) in[b][row] and writes the chunk's partial projection.
+template <int P>
+__global__ __launch_bounds__(DrfFftShape<P>::NT) void k_drf_fft_prj(DrfArgs a,
+                                                                    const float2 *__restrict__ H,
+                                                                    const float2 *__restrict__ tw)
+{
+    using D = DrfFftShape<P>;
+    extern __shared__ float2 sm[];
+    const int li = threadIdx.x / D::T, tt = threadIdx.x % D::T;
+    const int b = blockIdx.x / a.nchunk, ch = blockIdx.x % a.nchunk;
+    const int r0 = ch * a.rchunk, r1 = min(r0 + a.rchunk, a.nrows);
+    const int v = a.bview[b];
+    const bool xact = li < D::WC;
+    const float unit = 1.0f / ((float)P * (float)P);
+    float2 acc[D::R], s[D::R];
+#pragma unroll
+    for (int w = 0; w < D::R; ++w) acc[w] = make_float2(0.0f, 0.0f);
+    for (int row = r0; row < r1; ++row) {
+        const int kidx = a.rowKernel[(long)v * a.nrows + row];
+        const int pad = (kidx < 0 || a.avg <= 0) ? 0 : a.rowPad[(long)v * a.nrows + row];
+        // the lane indices pass through an empty asm each row, so the
+        // compiler recomputes the transforms' LDS addresses instead of
+        // keeping hundreds of them live across the loop
+        int lq = li, tq = tt;
+        asm volatile("" : "+v"(lq), "+v"(tq));
+        drf_fft_forward<P>(a, b, v, row, pad, s, sm, lq, tq, tw);
+        if (xact) {
+            if (kidx >= 0) {
+                const float2 *h = H + (long)kidx * P * D::WC + li * D::T + tt;
+#pragma unroll
+                for (int w = 0; w < D::R; ++w) {
+                    const float2 hv = h[w * D::WC * D::T];
+                    acc[w].x = fmaf(s[w].x, hv.x, fmaf(-s[w].y, hv.y, acc[w].x));
+                    acc[w].y = fmaf(s[w].x, hv.y, fmaf(s[w].y, hv.x, acc[w].y));
+                }
+            } else {
+                // distance <= 0: added unconvolved
+#pragma unroll
+                for (int w = 0; w < D::R; ++w) {
+                    acc[w].x = fmaf(s[w].x, unit, acc[w].x);
+                    acc[w].y = fmaf(s[w].y, unit, acc[w].y);
+                }
+            }
+        }
+        __syncthreads();   // the next row rewrites the line slots
+    }
+    drf_fft_inverse<P>(a, acc, sm, li, tt, tw);
+    const int c0 = 2 * li, c1 = c0 + 1;
+    float *o = a.out + (long)b * a.outB + (long)ch * a.outRow;
+#pragma unroll
+    for (int w = 0; w < D::R; ++w) {
+        const int n = tt + D::T * w;
+        if (n < a.Nz) {
+            if (c0 < a.NRot) o[(long)c0 * a.outXS + n] = acc[w].x;
+            if (c1 < a.NRot) o[(long)c1 * a.outXS + n] = acc[w].y;
+        }
+    }
+}
+
+// BCK: block = (batch item b, rows [r0, r0 + rows)): planes[row] =
+// scale * DRF_row (*) in[b][row].  The input spectrum is reused across the
+// block's rows while the input does not depend on the row (no factors)
+// and the axial pad stays the same.
+template <int P>
+__global__ __launch_bounds__(DrfFftShape<P>::NT) void k_drf_fft_bck(DrfArgs a,
+                                                                    const float2 *__restrict__ H,
+                                                                    const float2 *__restrict__ tw,
+                                                                    int rows, int nblk)
+{
+    using D = DrfFftShape<P>;
+    extern __shared__ float2 sm[];
+    const int li = threadIdx.x / D::T, tt = threadIdx.x % D::T;
+    const int b = blockIdx.x / nblk;
+    const int r0 = (blockIdx.x % nblk) * rows, r1 = min(r0 + rows, a.nrows);
+    const int v = a.bview[b];
+    const bool xact = li < D::WC;
+    const bool reuse = a.afac == nullptr && a.inRow == 0;
+    const float unit = 1.0f / ((float)P * (float)P);
+    const int c0 = 2 * li, c1 = c0 + 1;
+    float2 spec[D::R], g[D::R];
+    int have = -1;
+    for (int row = r0; row < r1; ++row) {
+        const int kidx = a.rowKernel[(long)v * a.nrows + row];
+        const int pad = (kidx < 0 || a.avg <= 0) ? 0 : a.rowPad[(long)v * a.nrows + row];
+        int lq = li, tq = tt;
+        asm volatile("" : "+v"(lq), "+v"(tq));
+        if (!reuse || pad != have) {
+            drf_fft_forward<P>(a, b, v, row, pad, spec, sm, lq, tq, tw);
+            have = pad;
+            __syncthreads();   // the inverse rewrites the line slots
+        }
+        if (xact) {
+            if (kidx >= 0) {
+                const float2 *h = H + (long)kidx * P * D::WC + li * D::T + tt;
+#pragma unroll
+                for (int w = 0; w < D::R; ++w) {
+                    const float2 hv = h[w * D::WC * D::T];
+                    g[w].x = fmaf(spec[w].x, hv.x, -spec[w].y * hv.y);
+                    g[w].y = fmaf(spec[w].x, hv.y, spec[w].y * hv.x);
+                }
+            } else {
+#pragma unroll
+                for (int w = 0; w < D::R; ++w) g[w] = make_float2(spec[w].x * unit, spec[w].y * unit);
+            }
+        } else {
+#pragma unroll
+            for (int w = 0; w < D::R; ++w) g[w] = make_float2(0.0f, 0.0f);
+        }
+        drf_fft_inverse<P>(a, g, sm, lq, tq, tw);
+        float *o = a.out + (long)b * a.outB + (long)row * a.outRow;
+#pragma unroll
+        for (int w = 0; w < D::R; ++w) {
+            const int n = tt + D::T * w;
+            if (n < a.Nz) {
+                if (c0 < a.NRot) o[(long)c0 * a.outXS + n] = g[w].x * a.scale;
+                if (c1 < a.NRot) o[(long)c1 * a.outXS + n] = g[w].y * a.scale;
+            }
+        }
+        __syncthreads();   // the next row rewrites the line slots
+    }
+}
+
+// H[e][w][kz][tt] = (1/P^2) sum_{i,j} psf_e[j][i] W^(kz (j - CY) + kx (i - CX)),
+// kx = tt + T w, W = exp(-2 pi i / P): the transform of the PSF with its
+// centre at the origin, scaled for the unnormalised inverse.  FP64 sums;
+// root[m] = W^m in double.
+template <int P>
+__global__ void k_drf_spectra(const float *__restrict__ psf, int nE, int KX, int KY, int KYP,
+                              const double2 *__restrict__ root, float2 *__restrict__ H)
+{
+    using D = DrfFftShape<P>;
+    const long per = (long)P * D::WC;
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long)nE * per) return;
+    const int e = (int)(id / per);
+    const int rem = (int)(id % per);
+    const int tt = rem % D::T, kz = (rem / D::T) % D::WC, w = rem / (D::T * D::WC);
+    const int kx = tt + D::T * w;
+    const int CX = (KX - 1) / 2, CY = (KY - 1) / 2;
+    const float *p = psf + (long)e * KX * KYP;
+    double re = 0.0, im = 0.0;
+    for (int i = 0; i < KX; ++i) {
+        double cr = 0.0, ci = 0.0;
+        for (int j = 0; j < KY; ++j) {
+            const double val = p[i * KYP + j];
+            if (val == 0.0) continue;
+            const double2 r = root[((kz * (j - CY)) % P + P) % P];
+            cr += val * r.x;
+            ci += val * r.y;
+        }
+        const double2 r = root[((kx * (i - CX)) % P + P) % P];
+        re += cr * r.x - ci * r.y;
+        im += cr * r.y + ci * r.x;
+    }
+    const double sc = 1.0 / ((double)P * P);
+    H[id] = make_float2((float)(re * sc), (float)(im * sc));
+}
+
+// supported transform lengths, ascending
+static const int kDrfFftSizes[] = {32, 64, 128, 160};
+
+int drf_fft_size(int NRot, int Nz, int KX, int KY)
+{
+    const int need = std::max(NRot + KX - 1, Nz + KY - 1);
+    for (int P : kDrfFftSizes)
+        if (P >= need) return P;
+    return 0;
+}
+
+void DrfFftTables::release()
+{
+    if (H) (void)hipFree(H);
+    if (tw) (void)hipFree(tw);
+    H = tw = nullptr;
+    P = nE = 0;
+}
+
+void DrfFftTables::build(hipStream_t s, int P_, const float *psf, int nE_, int KX, int KY, int KYP)
+{
+    release();
+    if (nE_ <= 0) fail(ETYPE_ILLEGAL_VALUE, "FFT DRF: empty PSF table");
+    const long per = (long)P_ * (P_ / 2 + 1);
+    std::vector<double2> hroot(P_);
+    for (int m = 0; m < P_; ++m) {
+        const double ang = -2.0 * M_PI * m / P_;
+        hroot[m] = make_double2(cos(ang), sin(ang));
+    }
+    const std::vector<float2> htw = twiddles(P_);
+    double2 *root = nullptr;
+    DFFT_CHECK(hipMalloc((void **)&root, P_ * sizeof(double2)));
+    DFFT_CHECK(hipMalloc((void **)&H, (size_t)nE_ * per * sizeof(float2)));
+    DFFT_CHECK(hipMalloc((void **)&tw, P_ * sizeof(float2)));
+    P = P_;
+    nE = nE_;
+    DFFT_CHECK(hipMemcpyAsync(root, hroot.data(), P_ * sizeof(double2), hipMemcpyHostToDevice, s));
+    DFFT_CHECK(hipMemcpyAsync(tw, htw.data(), P_ * sizeof(float2), hipMemcpyHostToDevice, s));
+    const long n = (long)nE_ * per;
+    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    switch (P_) {
+    case 32: hipLaunchKernelGGL(k_drf_spectra<32>, grid, block, 0, s, psf, nE_, KX, KY, KYP, root, H); break;
+    case 64: hipLaunchKernelGGL(k_drf_spectra<64>, grid, block, 0, s, psf, nE_, KX, KY, KYP, root, H); break;
+    case 128: hipLaunchKernelGGL(k_drf_spectra<128>, grid, block, 0, s, psf, nE_, KX, KY, KYP, root, H); break;
+    case 160: hipLaunchKernelGGL(k_drf_spectra<160>, grid, block, 0, s, psf, nE_, KX, KY, KYP, root, H); break;
+    default: fail(ETYPE_ILLEGAL_VALUE, "FFT DRF: unsupported length %d", P_);
+    }
+    DFFT_CHECK(hipGetLastError());
+    DFFT_CHECK(hipStreamSynchronize(s));
+    DFFT_CHECK(hipFree(root));
+}
+
+template <int P>
+static void launch_p(hipStream_t s, bool bck, const DrfArgs &a, int B, const DrfFftTables &t)
+{
+    using D = DrfFftShape<P>;
+    // every line pair, every half-spectrum line and every kept output fit the block
+    if (a.NRot + 1 > 2 * D::WC || a.NRot > P || a.Nz > P)
+        fail(ETYPE_ILLEGAL_VALUE, "FFT DRF: %d x %d planes do not fit length %d", a.NRot, a.Nz, P);
+    if (a.zs != 1 || a.outZS != 1)
+        fail(ETYPE_ILLEGAL_VALUE, "FFT DRF: slice-fastest planes only");
+    if (bck) {
+        // without per-row factors a block reuses one input spectrum for 8 rows
+        const bool reuse = a.afac == nullptr && a.inRow == 0;
+        const int rows = reuse ? std::min(8, a.nrows) : 1;
+        const int nblk = (a.nrows + rows - 1) / rows;
+        const long nb = (long)B * nblk;
+        if (nb <= 0 || nb > INT_MAX) fail(ETYPE_ILLEGAL_VALUE, "bad FFT DRF launch size");
+        hipLaunchKernelGGL(k_drf_fft_bck<P>, dim3((unsigned)nb), dim3(D::NT), D::LDS, s, a, t.H,
+                           t.tw, rows, nblk);
+    } else {
+        const long nb = (long)B * a.nchunk;
+        if (nb <= 0 || nb > INT_MAX || a.rchunk <= 0)
+            fail(ETYPE_ILLEGAL_VALUE, "bad FFT DRF launch size");
+        hipLaunchKernelGGL(k_drf_fft_prj<P>, dim3((unsigned)nb), dim3(D::NT), D::LDS, s, a, t.H,
+                           t.tw);
+    }
+    DFFT_CHECK(hipGetLastError());
+}
+
+void launch_drf_fft(hipStream_t s, bool bck, const DrfArgs &a, int B, const DrfFftTables &t)
+{
+    if (!t.H || !t.tw) fail(ETYPE_ILLEGAL_VALUE, "FFT DRF: tables not built");
+    switch (t.P) {
+    case 32: launch_p<32>(s, bck, a, B, t); break;
+    case 64: launch_p<64>(s, bck, a, B, t); break;
+    case 128: launch_p<128>(s, bck, a, B, t); break;
+    case 160: launch_p<160>(s, bck, a, B, t); break;
+    default: fail(ETYPE_ILLEGAL_VALUE, "FFT DRF: unsupported length %d", t.P);
+    }
+}
+
+}  // namespace irlamd

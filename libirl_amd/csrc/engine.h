@@ -118,6 +118,10 @@ public:
         float cnnMaxNrmse = 0.01f;
         int cnnExact = 0;                  // 1: the terms from one FFT convolution instead
         bool autotune = false;             // autotuneTiles() at precompute
+        // FP32 DRF convolution: 0 spatial kernel, 1 2-D FFT kernel (where its
+        // sizes fit), 2 the faster of the two per direction, timed at
+        // precompute
+        int drfFft = 0;
     };
 
     struct CnnStatus {
@@ -169,6 +173,8 @@ public:
     // wave of the BF16 kernel), for the projector and the backprojector.
     void autotuneTiles();
     void tiles(int *prj, int *bck) const;
+    // FP32 DRF algorithm in use per direction: 0 spatial, 1 FFT (length P)
+    void drfAlgorithms(int *prj, int *bck, int *P) const;
 
     // One view at a time (for per-angle hooks): zeroUpdate at the start of
     // the subset, then member m of subset s is projected and backprojected

@@ -45,6 +45,8 @@
 #include <vector>
 
 #include "cnn_dev.h"
+#include "drf_dev.h"
+#include "drf_fft.h"
 #include "engine.h"
 #include "esse_dev.h"
 
@@ -501,54 +503,6 @@ __global__ __launch_bounds__(256) void k_scatter_check(const float *__restrict__
 // ---------------------------------------------------------------------------
 constexpr int DRF_TX = 64;
 constexpr int DRF_ZT = 16;
-
-struct DrfArgs {
-    int NRot, Nz;                 // plane width (bins) and height (slices)
-    int nrows;                    // planes (rows) per batch item
-    int KX;                       // padded PSF width (odd)
-    const float *psf;             // [nDist][KX][KYP] transposed, KYP = bucket padded to 4
-    const int *rowKernel;         // [V][NRot]
-    const int *rowPad;            // [V][NRot]
-    int avg;                      // axial average rows (clamped to [0, Nz])
-    const int *bview;             // batch slot -> view
-    const float *in;              // PRJ: rotated, attenuated planes; BCK: ratio plane
-    long inB, inRow;              // BCK: inRow = 0 (one ratio plane per view)
-    const float *afac;            // factors multiplied into the input, or null
-    long afB, afRow;
-    int zs, xs;                   // in/afac slice and bin strides
-    float scale;                  // BCK: output scale (fScaleFac)
-    float *out;                   // PRJ: partial sums per chunk; BCK: planes
-    long outB, outRow;
-    int outZS, outXS;
-    int vec4;                     // outZS == 1 and 16-byte aligned rows: float4 stores
-    int rchunk, nchunk;           // PRJ row chunking
-    int xtiles, ztiles;
-    const int2 *colRange;         // per table entry: first and last PSF column holding a
-                                  // nonzero tap (null: all KX columns)
-};
-
-__device__ __forceinline__ float drf_in(const DrfArgs &a, int b, int v, int row, int g, int gx)
-{
-    const long off = (long)g * a.zs + (long)gx * a.xs;
-    float r = a.in[(long)b * a.inB + (long)row * a.inRow + off];
-    if (a.afac) r = r * a.afac[(long)v * a.afB + (long)row * a.afRow + off];
-    return r;
-}
-
-// Value of padded-plane row g (global slice index, may be outside [0,Nz)) at
-// bin gx, following pfPad2d/pfAvgRows/pfCopyRows (pad2d.c:21-141).
-__device__ float drf_stage_val(const DrfArgs &a, int b, int v, int row, int g, int gx, int pad)
-{
-    if (gx < 0 || gx >= a.NRot) return 0.0f;
-    if (g >= 0 && g < a.Nz) return drf_in(a, b, v, row, g, gx);
-    if (pad <= 0 || a.avg <= 0) return 0.0f;
-    if (g < -pad || g >= a.Nz + pad) return 0.0f;
-    const int base = g < 0 ? 0 : a.Nz - a.avg;
-    float s = 0.0f;
-    for (int k = 0; k < a.avg; ++k) s += drf_in(a, b, v, row, base + k, gx);
-    if (a.avg > 1) s /= (float)a.avg;
-    return s;
-}
 
 __host__ __device__ constexpr int drf_col_stride(int TH)
 {
@@ -1424,9 +1378,9 @@ struct Recon::Impl {
 
     // Stage name of a DRF launch: the precision variant is suffixed _bf16
     // (plain BF16 operands) or _bf16x3 (hi+lo split).
-    std::string drf_name(const char *base) const
+    std::string drf_name(const char *base, bool bck) const
     {
-        if (!bf16) return base;
+        if (!bf16) return (bck ? fftBck : fftPrj) ? std::string(base) + "_fft" : base;
         const char *suf = nSplit == 0 ? "_bf16" : nSplit < nUsed ? "_bf16mix" : "_bf16x3";
         return std::string(base) + suf;
     }
@@ -1435,9 +1389,62 @@ struct Recon::Impl {
     // FP32 kernel, or slice tiles per wave of the BF16 kernel
     int tilePrj = 0, tileBck = 0;
 
+    // FP32 DRF by 2-D FFT (drf_fft.hip): tables of the PSF spectra, the
+    // directions using it, and the projector's row chunking for each
+    // algorithm (the FFT kernel keeps one CU per block, so fewer, larger
+    // chunks)
+    DrfFftTables fft;
+    bool fftPrj = false, fftBck = false;
+    int rchunkD = 1, nchunkD = 1, rchunkF = 1, nchunkF = 1;
+
+    void set_fft(bool prjOn, bool bckOn)
+    {
+        fftPrj = prjOn;
+        fftBck = bckOn;
+        rchunk = prjOn ? rchunkF : rchunkD;
+        nchunk = prjOn ? nchunkF : nchunkD;
+    }
+
+    // Times the spatial and the FFT kernel per direction on the first
+    // subset's views (one launch to warm up, three timed) and keeps the
+    // faster.  Inputs are whatever the buffers hold; only the time counts.
+    void choose_drf(int Vs, int prjModel, int bckModel)
+    {
+        const int KYb = ky_bucket();
+        hipEvent_t e0, e1;
+        HIP_CHECK(hipEventCreate(&e0));
+        HIP_CHECK(hipEventCreate(&e1));
+        for (int bck = 0; bck < 2; ++bck) {
+            float ms[2] = {0.0f, 0.0f};
+            for (int f = 0; f < 2; ++f) {
+                if (bck)
+                    set_fft(fftPrj, f != 0);
+                else
+                    set_fft(f != 0, fftBck);
+                const DrfArgs da = drf_args(bck != 0, subsets.p, bck ? bckModel : prjModel);
+                run_drf(bck != 0, da, Vs, KYb);
+                HIP_CHECK(hipEventRecord(e0, stream));
+                for (int r = 0; r < 3; ++r) run_drf(bck != 0, da, Vs, KYb);
+                HIP_CHECK(hipEventRecord(e1, stream));
+                HIP_CHECK(hipEventSynchronize(e1));
+                HIP_CHECK(hipEventElapsedTime(&ms[f], e0, e1));
+            }
+            if (bck)
+                set_fft(fftPrj, ms[1] < ms[0]);
+            else
+                set_fft(ms[1] < ms[0], fftBck);
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+
     void run_drf(bool bck, const DrfArgs &args, int B, int KYb, int tile = -1)
     {
         if (tile < 0) tile = bck ? tileBck : tilePrj;
+        if (!bf16 && (bck ? fftBck : fftPrj)) {
+            launch_drf_fft(stream, bck, args, B, fft);
+            return;
+        }
         if (bf16)
             launch_drf_bf16(stream, bck, args, B, KYb, bf16Split, afrHi.p, afrLo.p,
                             distSplit.p, drfBand.p, tile);
@@ -1578,8 +1585,10 @@ Recon::Recon(const Config &cfg) : impl_(new Impl), cfg_(cfg)
     I.vol = (long)g.NRot * g.Nz * g.NRot;
     // Rows per projection chunk: enough chunks that a subset's projector
     // launch fills the 256 CUs several times over.
-    I.rchunk = std::max(1, g.NRot / 32);
-    I.nchunk = div_up(g.NRot, I.rchunk);
+    I.rchunk = I.rchunkD = std::max(1, g.NRot / 32);
+    I.nchunk = I.nchunkD = div_up(g.NRot, I.rchunk);
+    I.rchunkF = std::max(I.rchunkD, g.NRot / 16);
+    I.nchunkF = div_up(g.NRot, I.rchunkF);
     const int Vs = cfg.Vs;
     I.x.alloc(I.M);
     I.u.alloc(I.M);
@@ -1673,6 +1682,13 @@ Recon::Recon(const Config &cfg) : impl_(new Impl), cfg_(cfg)
         I.colRange.alloc(cr.size());
         I.colRange.upload(cr.data(), cr.size(), I.stream);
         HIP_CHECK(hipStreamSynchronize(I.stream));   // cr dies with this block
+        if (cfg.precision == 0 && cfg.drfFft > 0) {
+            const int P = drf_fft_size(g.NRot, g.Nz, KX, KYb);
+            if (P > 0) {
+                I.fft.build(I.stream, P, I.psf.p, cfg.drf.nDist, KX, KYb, KYP);
+                if (cfg.drfFft == 1) I.set_fft(true, true);
+            }
+        }
         if (cfg.precision > 0) {
             if (g.Nz % 4)
                 fail(ETYPE_ILLEGAL_VALUE, "the BF16 DRF path needs NumSlices %% 4 == 0 (%d)", g.Nz);
@@ -1920,6 +1936,7 @@ void Recon::precompute(bool norms)
     const Geometry &g = cfg_.g;
     const int threads = 256;
     if (cfg_.autotune && !I.timing) autotuneTiles();
+    if (cfg_.drfFft == 2 && I.fft.P > 0) I.choose_drf(cfg_.Vs, cfg_.prjModel, cfg_.bckModel);
     if (cfg_.hasAtn) {
         const long n = (long)g.V * g.Nz * g.NRot;
         I.launch("afac", [&] {
@@ -1946,7 +1963,7 @@ void Recon::precompute(bool norms)
                                I.vt.p, I.ratio.p);
         });
         const DrfArgs da = I.drf_args(true, bview, cfg_.bckModel);
-        I.launch(I.drf_name("drf_bck_norm"), [&] { I.run_drf(true, da, Vs, KYb); });
+        I.launch(I.drf_name("drf_bck_norm", true), [&] { I.run_drf(true, da, Vs, KYb); });
         BckArgs ba = I.bck_args(bview);
         ba.ninvOut = I.ninv.p + I.M * s;
         I.launch("norm", [&] { I.run_bck(true, ba); });
@@ -2070,6 +2087,13 @@ void Recon::tiles(int *prj, int *bck) const
 {
     *prj = impl_->tilePrj;
     *bck = impl_->tileBck;
+}
+
+void Recon::drfAlgorithms(int *prj, int *bck, int *P) const
+{
+    *prj = impl_->fftPrj ? 1 : 0;
+    *bck = impl_->fftBck ? 1 : 0;
+    *P = impl_->fft.P;
 }
 
 void *Recon::streamHandle() const { return (void *)impl_->stream; }
@@ -2337,14 +2361,14 @@ void Recon::runViews(int s, int first, int Vs, SrfMode srf, bool accumulate, boo
     else
         I.launch("att", [&] { I.run_att(afac, Vs, bview); });
     const DrfArgs da = I.drf_args(false, bview, cfg_.prjModel);
-    I.launch(I.drf_name("drf_prj"), [&] { I.run_drf(false, da, Vs, KYb); });
+    I.launch(I.drf_name("drf_prj", false), [&] { I.run_drf(false, da, Vs, KYb); });
     if (srf == SRF_MODEL_STORE) {
         // projection without the scatter model for the stored estimate
         // (osemrecon.c:757-786), reusing the rotated estimate
         I.launch("att", [&] { I.run_att(afac, Vs, bview); });
         DrfArgs d2 = da;
         d2.out = I.ppart2.p;
-        I.launch(I.drf_name("drf_prj"), [&] { I.run_drf(false, d2, Vs, KYb); });
+        I.launch(I.drf_name("drf_prj", false), [&] { I.run_drf(false, d2, Vs, KYb); });
     }
     RatioArgs ra{};
     ra.srfMode = srf == SRF_MODEL_STORE ? 2 : srf == SRF_ADD_STORED ? 3 : 0;
@@ -2368,7 +2392,7 @@ void Recon::runViews(int s, int first, int Vs, SrfMode srf, bool accumulate, boo
                            I.stream, ra);
     });
     const DrfArgs db = I.drf_args(true, bview, cfg_.bckModel);
-    I.launch(I.drf_name("drf_bck"), [&] { I.run_drf(true, db, Vs, KYb); });
+    I.launch(I.drf_name("drf_bck", true), [&] { I.run_drf(true, db, Vs, KYb); });
     BckArgs ba = I.bck_args(bview);
     ba.B = Vs;
     ba.x = update ? I.x.p : nullptr;
@@ -2409,7 +2433,7 @@ void Recon::forwardProject(float primary, float *out_host, bool srf)
             I.launch("att", [&] { I.run_att(afac, B, bview); });
         }
         const DrfArgs da = I.drf_args(false, bview, cfg_.prjModel);
-        I.launch(I.drf_name("drf_prj"), [&] { I.run_drf(false, da, B, KYb); });
+        I.launch(I.drf_name("drf_prj", false), [&] { I.run_drf(false, da, B, KYb); });
         RatioArgs ra{};
         ra.B = B;
         ra.Nz = g.Nz;
