@@ -20,7 +20,7 @@ int drf_fft_size(int NRot, int Nz, int KX, int KY);
 // Spectra of one PSF table and the line-transform twiddles (device).
 struct DrfFftTables {
     int P = 0, nE = 0;
-    float2 *H = nullptr;    // [nE][P (P/2+1)] in the kernels' register order
+    float2 *H = nullptr;    // [nE][P/2+1][P] in the kernels' register order
     float2 *tw = nullptr;   // [P]
     // psf: device [nE][KX][KYP] (transposed, PSFs centred in KX x KY)
     void build(hipStream_t s, int P, const float *psf, int nE, int KX, int KY, int KYP);

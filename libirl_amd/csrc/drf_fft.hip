@@ -45,10 +45,40 @@ struct DrfFftShape {
     static constexpr int WC = P / 2 + 1;               // half spectrum along the slices
     static constexpr int NL = (WC * T + 63) / 64 * 64 / T;  // line slots per block
     static constexpr int NT = NL * T;                  // threads per block
-    static constexpr size_t LDS = (size_t)NL * LS * sizeof(float2);
+    // line slots, then the twiddles
+    static constexpr size_t LDS = ((size_t)NL * LS + P) * sizeof(float2);
     static constexpr int KH = (P / 2) / T + 1;         // registers that can hold k <= P/2
-    static_assert(NT <= 1024, "block size");
+    static_assert(NT <= 1024 && R % 2 == 0, "block size, 128-bit spectrum loads");
 };
+
+// First line slot of the calling thread's wave (uniform per wave).
+template <int P>
+__device__ __forceinline__ int drf_fft_wave_first_line()
+{
+    const int first = __builtin_amdgcn_readfirstlane((int)threadIdx.x);
+    return (first / 64) * (64 / DrfFftShape<P>::T);
+}
+
+// Axial pad of a bin pair's slice line: FFT index n in [Nz, Nz + pad) holds
+// the mean of the last avg slices, n in [P - pad, P) (slices -pad..-1) that
+// of the first avg (pad2d.c:21-141, as drf_stage_val).
+template <int P>
+__device__ __forceinline__ void drf_fft_pad(const DrfArgs &a, int b, int v, int row, int pad,
+                                            float2 *s, int c0, int c1, bool zact, int tt)
+{
+    using D = DrfFftShape<P>;
+    if (pad <= 0 || a.avg <= 0 || !zact) return;
+    const float lo0 = drf_stage_val(a, b, v, row, -1, c0, pad);
+    const float lo1 = drf_stage_val(a, b, v, row, -1, c1, pad);
+    const float hi0 = drf_stage_val(a, b, v, row, a.Nz, c0, pad);
+    const float hi1 = drf_stage_val(a, b, v, row, a.Nz, c1, pad);
+#pragma unroll
+    for (int w = 0; w < D::R; ++w) {
+        const int n = tt + D::T * w;
+        if (n >= a.Nz && n < a.Nz + pad) s[w] = make_float2(hi0, hi1);
+        if (n >= P - pad) s[w] = make_float2(lo0, lo1);
+    }
+}
 
 // Forward 2-D transform of plane (b, row) of the DRF input (with axial pad
 // 'pad').  On exit thread (line li, tt), li < P/2 + 1, holds
@@ -62,43 +92,46 @@ __device__ __forceinline__ void drf_fft_forward(const DrfArgs &a, int b, int v, 
     float2 *slot = sm + li * D::LS;
     const int c0 = 2 * li, c1 = c0 + 1;
     const bool zact = c0 < a.NRot, has1 = c1 < a.NRot;
+    // waves beyond the last bin pair skip the slice transform and keep its
+    // barriers (the condition is uniform per wave)
+    const bool zIdle = drf_fft_wave_first_line<P>() >= (a.NRot + 1) / 2;
     // slice lines of bins c0 (real part) and c1 (imaginary part), slices
     // contiguous (zs == 1): FFT index n < Nz holds slice n
-    {
-        const long e0 = (long)c0 * a.xs;
-        const float *i0 = a.in + (long)b * a.inB + (long)row * a.inRow + e0;
-        const float *f0 = a.afac ? a.afac + (long)v * a.afB + (long)row * a.afRow + e0 : nullptr;
+    if (zIdle) {
+#pragma unroll
+        for (int w = 0; w < D::R; ++w) s[w] = make_float2(0.0f, 0.0f);
+        fft_line_barriers<P, false>();
+    } else {
+        // loads at clamped positions (always inside the plane), then selects
+        const int q0 = min(c0, a.NRot - 1), q1 = min(c1, a.NRot - 1);
+        const float *base = a.in + (long)b * a.inB + (long)row * a.inRow;
+        const float *i0 = base + (long)q0 * a.xs, *i1 = base + (long)q1 * a.xs;
+        const float *fb = a.afac ? a.afac + (long)v * a.afB + (long)row * a.afRow : nullptr;
+        const float *g0 = fb ? fb + (long)q0 * a.xs : nullptr, *g1 = fb ? fb + (long)q1 * a.xs : nullptr;
 #pragma unroll
         for (int w = 0; w < D::R; ++w) {
-            const int n = tt + D::T * w;
-            float re = 0.0f, im = 0.0f;
-            if (zact && n < a.Nz) {
-                re = i0[n];
-                if (has1) im = i0[n + a.xs];
-                if (f0) {
-                    re *= f0[n];
-                    if (has1) im *= f0[n + a.xs];
-                }
+            // uniform per register: wholly inside, straddling or beyond the slices
+            const int n0 = D::T * w;
+            if (n0 >= a.Nz) {
+                s[w] = make_float2(0.0f, 0.0f);
+                continue;
             }
-            s[w] = make_float2(re, im);
+            const bool whole = n0 + D::T <= a.Nz;
+            const int n = tt + n0;
+            const int nc = whole ? n : min(n, a.Nz - 1);
+            float re = i0[nc], im = i1[nc];
+            if (fb) {
+                re *= g0[nc];
+                im *= g1[nc];
+            }
+            const bool in = whole || n < a.Nz;
+            s[w] = make_float2((zact && in) ? re : 0.0f, (has1 && in) ? im : 0.0f);
         }
+        drf_fft_pad<P>(a, b, v, row, pad, s, c0, c1, zact, tt);
+        fft_line<P, false>(s, slot, tt, tw);
     }
-    if (pad > 0 && a.avg > 0 && zact) {
-        // axial pad: FFT index n in [Nz, Nz + pad) holds the mean of the
-        // last avg slices, n in [P - pad, P) (slices -pad..-1) that of the
-        // first avg (pad2d.c:21-141, as drf_stage_val)
-        const float lo0 = drf_stage_val(a, b, v, row, -1, c0, pad);
-        const float lo1 = drf_stage_val(a, b, v, row, -1, c1, pad);
-        const float hi0 = drf_stage_val(a, b, v, row, a.Nz, c0, pad);
-        const float hi1 = drf_stage_val(a, b, v, row, a.Nz, c1, pad);
-#pragma unroll
-        for (int w = 0; w < D::R; ++w) {
-            const int n = tt + D::T * w;
-            if (n >= a.Nz && n < a.Nz + pad) s[w] = make_float2(hi0, hi1);
-            if (n >= P - pad) s[w] = make_float2(lo0, lo1);
-        }
-    }
-    fft_line<P, false>(s, slot, tt, tw);
+    // (idle waves write their slots below; the bin pairs' values overwrite
+    // every position the bin lines read)
     // Between phases the lane indices pass through an empty asm, so each
     // phase computes its own LDS addresses instead of the compiler keeping
     // those of every phase live at once.
@@ -137,7 +170,8 @@ __device__ __forceinline__ void drf_fft_forward(const DrfArgs &a, int b, int v, 
 #pragma unroll
     for (int w = 0; w < D::R; ++w) {
         const int x = tt + D::T * w;
-        s[w] = (xact && x < a.NRot) ? slot[fft_skew(x)] : make_float2(0.0f, 0.0f);
+        const float2 val = slot[fft_skew(x)];   // inside the slot for every x < P
+        s[w] = (xact && x < a.NRot) ? val : make_float2(0.0f, 0.0f);
     }
     fft_line<P, false>(s, slot, tt, tw);
 }
@@ -165,18 +199,26 @@ __device__ __forceinline__ void drf_fft_inverse(const DrfArgs &a, float2 *g, flo
     slot = sm + li * D::LS;
     const int c0 = 2 * li, c1 = c0 + 1;
     const bool has0 = c0 < a.NRot, has1 = c1 < a.NRot;
+    if (drf_fft_wave_first_line<P>() >= (a.NRot + 1) / 2) {
+        // no bin pair in this wave: the transform's barriers only
+#pragma unroll
+        for (int w = 0; w < D::R; ++w) g[w] = make_float2(0.0f, 0.0f);
+        __syncthreads();
+        fft_line_barriers<P, true>();
+        return;
+    }
+    // reads at clamped bins (inside line slot kk), then selects
+    const int s0 = fft_skew(min(c0, P - 1)), s1 = fft_skew(min(c1, P - 1));
 #pragma unroll
     for (int w = 0; w < D::R; ++w) {
         const int n = tt + D::T * w;
         const bool upper = n > P / 2;
         const int kk = upper ? P - n : n;
-        float2 A = has0 ? sm[kk * D::LS + fft_skew(c0)] : make_float2(0.0f, 0.0f);
-        float2 B = has1 ? sm[kk * D::LS + fft_skew(c1)] : make_float2(0.0f, 0.0f);
-        if (upper) {
-            A.y = -A.y;
-            B.y = -B.y;
-        }
-        g[w] = make_float2(A.x - B.y, A.y + B.x);
+        const float2 A0 = sm[kk * D::LS + s0], B0 = sm[kk * D::LS + s1];
+        const float2 A = has0 ? A0 : make_float2(0.0f, 0.0f);
+        const float2 B = has1 ? B0 : make_float2(0.0f, 0.0f);
+        const float sg = upper ? -1.0f : 1.0f;   // conj of the mirrored half
+        g[w] = make_float2(A.x - sg * B.y, sg * A.y + B.x);
     }
     __syncthreads();
     fft_line<P, true>(g, slot, tt, tw);
@@ -187,11 +229,14 @@ __device__ __forceinline__ void drf_fft_inverse(const DrfArgs &a, float2 *g, flo
 template <int P>
 __global__ __launch_bounds__(DrfFftShape<P>::NT) void k_drf_fft_prj(DrfArgs a,
                                                                     const float2 *__restrict__ H,
-                                                                    const float2 *__restrict__ tw)
+                                                                    const float2 *__restrict__ twg)
 {
     using D = DrfFftShape<P>;
     extern __shared__ float2 sm[];
     const int li = threadIdx.x / D::T, tt = threadIdx.x % D::T;
+    float2 *tw = sm + D::NL * D::LS;
+    for (int i = threadIdx.x; i < P; i += D::NT) tw[i] = twg[i];
+    __syncthreads();
     const int b = blockIdx.x / a.nchunk, ch = blockIdx.x % a.nchunk;
     const int r0 = ch * a.rchunk, r1 = min(r0 + a.rchunk, a.nrows);
     const int v = a.bview[b];
@@ -211,12 +256,18 @@ __global__ __launch_bounds__(DrfFftShape<P>::NT) void k_drf_fft_prj(DrfArgs a,
         drf_fft_forward<P>(a, b, v, row, pad, s, sm, lq, tq, tw);
         if (xact) {
             if (kidx >= 0) {
-                const float2 *h = H + (long)kidx * P * D::WC + li * D::T + tt;
+                // this thread's R values are contiguous: R/2 128-bit loads
+                const float4 *h = (const float4 *)(H + ((long)kidx * D::WC + li) * P + tt * D::R);
 #pragma unroll
-                for (int w = 0; w < D::R; ++w) {
-                    const float2 hv = h[w * D::WC * D::T];
-                    acc[w].x = fmaf(s[w].x, hv.x, fmaf(-s[w].y, hv.y, acc[w].x));
-                    acc[w].y = fmaf(s[w].x, hv.y, fmaf(s[w].y, hv.x, acc[w].y));
+                for (int w2 = 0; w2 < D::R / 2; ++w2) {
+                    const float4 hq = h[w2];
+                    const float2 hv[2] = {make_float2(hq.x, hq.y), make_float2(hq.z, hq.w)};
+#pragma unroll
+                    for (int e = 0; e < 2; ++e) {
+                        const int w = 2 * w2 + e;
+                        acc[w].x = fmaf(s[w].x, hv[e].x, fmaf(-s[w].y, hv[e].y, acc[w].x));
+                        acc[w].y = fmaf(s[w].x, hv[e].y, fmaf(s[w].y, hv[e].x, acc[w].y));
+                    }
                 }
             } else {
                 // distance <= 0: added unconvolved
@@ -249,12 +300,15 @@ __global__ __launch_bounds__(DrfFftShape<P>::NT) void k_drf_fft_prj(DrfArgs a,
 template <int P>
 __global__ __launch_bounds__(DrfFftShape<P>::NT) void k_drf_fft_bck(DrfArgs a,
                                                                     const float2 *__restrict__ H,
-                                                                    const float2 *__restrict__ tw,
+                                                                    const float2 *__restrict__ twg,
                                                                     int rows, int nblk)
 {
     using D = DrfFftShape<P>;
     extern __shared__ float2 sm[];
     const int li = threadIdx.x / D::T, tt = threadIdx.x % D::T;
+    float2 *tw = sm + D::NL * D::LS;
+    for (int i = threadIdx.x; i < P; i += D::NT) tw[i] = twg[i];
+    __syncthreads();
     const int b = blockIdx.x / nblk;
     const int r0 = (blockIdx.x % nblk) * rows, r1 = min(r0 + rows, a.nrows);
     const int v = a.bview[b];
@@ -276,12 +330,17 @@ __global__ __launch_bounds__(DrfFftShape<P>::NT) void k_drf_fft_bck(DrfArgs a,
         }
         if (xact) {
             if (kidx >= 0) {
-                const float2 *h = H + (long)kidx * P * D::WC + li * D::T + tt;
+                const float4 *h = (const float4 *)(H + ((long)kidx * D::WC + li) * P + tt * D::R);
 #pragma unroll
-                for (int w = 0; w < D::R; ++w) {
-                    const float2 hv = h[w * D::WC * D::T];
-                    g[w].x = fmaf(spec[w].x, hv.x, -spec[w].y * hv.y);
-                    g[w].y = fmaf(spec[w].x, hv.y, spec[w].y * hv.x);
+                for (int w2 = 0; w2 < D::R / 2; ++w2) {
+                    const float4 hq = h[w2];
+                    const float2 hv[2] = {make_float2(hq.x, hq.y), make_float2(hq.z, hq.w)};
+#pragma unroll
+                    for (int e = 0; e < 2; ++e) {
+                        const int w = 2 * w2 + e;
+                        g[w].x = fmaf(spec[w].x, hv[e].x, -spec[w].y * hv[e].y);
+                        g[w].y = fmaf(spec[w].x, hv[e].y, spec[w].y * hv[e].x);
+                    }
                 }
             } else {
 #pragma unroll
@@ -305,7 +364,7 @@ __global__ __launch_bounds__(DrfFftShape<P>::NT) void k_drf_fft_bck(DrfArgs a,
     }
 }
 
-// H[e][w][kz][tt] = (1/P^2) sum_{i,j} psf_e[j][i] W^(kz (j - CY) + kx (i - CX)),
+// H[e][kz][tt][w] = (1/P^2) sum_{i,j} psf_e[j][i] W^(kz (j - CY) + kx (i - CX)),
 // kx = tt + T w, W = exp(-2 pi i / P): the transform of the PSF with its
 // centre at the origin, scaled for the unnormalised inverse.  FP64 sums;
 // root[m] = W^m in double.
@@ -319,7 +378,7 @@ __global__ void k_drf_spectra(const float *__restrict__ psf, int nE, int KX, int
     if (id >= (long)nE * per) return;
     const int e = (int)(id / per);
     const int rem = (int)(id % per);
-    const int tt = rem % D::T, kz = (rem / D::T) % D::WC, w = rem / (D::T * D::WC);
+    const int w = rem % D::R, tt = (rem / D::R) % D::T, kz = rem / P;
     const int kx = tt + D::T * w;
     const int CX = (KX - 1) / 2, CY = (KY - 1) / 2;
     const float *p = psf + (long)e * KX * KYP;

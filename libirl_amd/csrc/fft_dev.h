@@ -260,7 +260,7 @@ struct FftLine {
     static_assert(T * R == P, "plan");
 };
 
-__device__ __forceinline__ int fft_skew(int i) { return i + (i >> 4); }
+__device__ __forceinline__ int fft_skew(int i) { return (int)((unsigned)i + ((unsigned)i >> 4)); }
 
 template <int P, int IDX>
 __host__ __device__ constexpr int fft_radix()
@@ -281,15 +281,17 @@ struct FftPass {
     static constexpr int U = R / r;           // units per thread
     static constexpr int T = FftPlan<P>::T;
     static_assert(U * r == R && (P / r) % T == 0, "plan");
+    // unsigned index arithmetic: S is a power of two in every plan, so the
+    // divisions are shifts
     __device__ static int in_pos(int tt, int k, int u)
     {
-        const int j = tt + T * k, p = j / S, q = j % S;
-        return q + S * (p + m * u);
+        const unsigned j = (unsigned)tt + T * k, p = j / S, q = j % S;
+        return (int)(q + S * (p + m * u));
     }
     __device__ static int out_pos(int tt, int k, int v)
     {
-        const int j = tt + T * k, p = j / S, q = j % S;
-        return q + S * (r * p + v);
+        const unsigned j = (unsigned)tt + T * k, p = j / S, q = j % S;
+        return (int)(q + S * (r * p + v));
     }
     __device__ static void compute(float2 *a, int tt, const float2 *__restrict__ tw)
     {
@@ -297,10 +299,11 @@ struct FftPass {
         for (int k = 0; k < U; ++k) {
             dft_any<r, INV>(a + k * r);
             if (m > 1) {
-                const int p = (tt + T * k) / S;
+                const unsigned p = ((unsigned)tt + T * k) / S;
 #pragma unroll
                 for (int v = 1; v < r; ++v) {
-                    float2 w = tw[(v * p * S) % P];
+                    // v p S < r m S = P: no reduction needed
+                    float2 w = tw[v * p * S];
                     if (INV) w.y = -w.y;
                     a[k * r + v] = cmul(a[k * r + v], w);
                 }
@@ -358,6 +361,27 @@ __device__ __forceinline__ void fft_line(float2 *a, float2 *ldsl, int tt,
                                          const float2 *__restrict__ tw)
 {
     fft_passes<P, 0, 1, INV>(a, ldsl, tt, tw);
+}
+
+// The block barriers of fft_passes<P, IDX, S, INV> without its work: the
+// same recursion, two barriers per pass that exchanges through LDS.  A wave
+// whose every line is idle calls this instead of fft_line, so the block's
+// waves still meet at every barrier.
+template <int P, int IDX, int S, bool INV>
+__device__ __forceinline__ void fft_passes_barriers()
+{
+    using PS = FftPass<P, IDX, S, INV>;
+    if constexpr (PS::m != 1) {
+        __syncthreads();
+        __syncthreads();
+        fft_passes_barriers<P, IDX + 1, S * PS::r, INV>();
+    }
+}
+
+template <int P, bool INV>
+__device__ __forceinline__ void fft_line_barriers()
+{
+    fft_passes_barriers<P, 0, 1, INV>();
 }
 
 // W_P^k = exp(-2 pi i k / P), k < P, rounded from double: the twiddles the
