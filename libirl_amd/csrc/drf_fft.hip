@@ -92,15 +92,14 @@ __device__ __forceinline__ void drf_fft_forward(const DrfArgs &a, int b, int v, 
     float2 *slot = sm + li * D::LS;
     const int c0 = 2 * li, c1 = c0 + 1;
     const bool zact = c0 < a.NRot, has1 = c1 < a.NRot;
-    // waves beyond the last bin pair skip the slice transform and keep its
-    // barriers (the condition is uniform per wave)
+    // waves beyond the last bin pair skip the slice transform (the condition
+    // is uniform per wave; the line transforms synchronise within a wave)
     const bool zIdle = drf_fft_wave_first_line<P>() >= (a.NRot + 1) / 2;
     // slice lines of bins c0 (real part) and c1 (imaginary part), slices
     // contiguous (zs == 1): FFT index n < Nz holds slice n
     if (zIdle) {
 #pragma unroll
         for (int w = 0; w < D::R; ++w) s[w] = make_float2(0.0f, 0.0f);
-        fft_line_barriers<P, false>();
     } else {
         // loads at clamped positions (always inside the plane), then selects
         const int q0 = min(c0, a.NRot - 1), q1 = min(c1, a.NRot - 1);
@@ -128,7 +127,7 @@ __device__ __forceinline__ void drf_fft_forward(const DrfArgs &a, int b, int v, 
             s[w] = make_float2((zact && in) ? re : 0.0f, (has1 && in) ? im : 0.0f);
         }
         drf_fft_pad<P>(a, b, v, row, pad, s, c0, c1, zact, tt);
-        fft_line<P, false>(s, slot, tt, tw);
+        fft_line<P, false, true>(s, slot, tt, tw);
     }
     // (idle waves write their slots below; the bin pairs' values overwrite
     // every position the bin lines read)
@@ -173,7 +172,7 @@ __device__ __forceinline__ void drf_fft_forward(const DrfArgs &a, int b, int v, 
         const float2 val = slot[fft_skew(x)];   // inside the slot for every x < P
         s[w] = (xact && x < a.NRot) ? val : make_float2(0.0f, 0.0f);
     }
-    fft_line<P, false>(s, slot, tt, tw);
+    fft_line<P, false, true>(s, slot, tt, tw);
 }
 
 // Inverse 2-D transform of the half spectrum held as drf_fft_forward leaves
@@ -185,7 +184,7 @@ __device__ __forceinline__ void drf_fft_inverse(const DrfArgs &a, float2 *g, flo
 {
     using D = DrfFftShape<P>;
     float2 *slot = sm + li * D::LS;
-    fft_line<P, true>(g, slot, tt, tw);
+    fft_line<P, true, true>(g, slot, tt, tw);
     asm volatile("" : "+v"(tt), "+v"(li));
     slot = sm + li * D::LS;
     __syncthreads();
@@ -199,29 +198,29 @@ __device__ __forceinline__ void drf_fft_inverse(const DrfArgs &a, float2 *g, flo
     slot = sm + li * D::LS;
     const int c0 = 2 * li, c1 = c0 + 1;
     const bool has0 = c0 < a.NRot, has1 = c1 < a.NRot;
-    if (drf_fft_wave_first_line<P>() >= (a.NRot + 1) / 2) {
-        // no bin pair in this wave: the transform's barriers only
+    // waves with no bin pair skip the slice transform; every wave meets the
+    // block barrier below (the line transform synchronises within a wave)
+    const bool idle = drf_fft_wave_first_line<P>() >= (a.NRot + 1) / 2;
+    if (idle) {
 #pragma unroll
         for (int w = 0; w < D::R; ++w) g[w] = make_float2(0.0f, 0.0f);
-        __syncthreads();
-        fft_line_barriers<P, true>();
-        return;
-    }
-    // reads at clamped bins (inside line slot kk), then selects
-    const int s0 = fft_skew(min(c0, P - 1)), s1 = fft_skew(min(c1, P - 1));
+    } else {
+        // reads at clamped bins (inside line slot kk), then selects
+        const int s0 = fft_skew(min(c0, P - 1)), s1 = fft_skew(min(c1, P - 1));
 #pragma unroll
-    for (int w = 0; w < D::R; ++w) {
-        const int n = tt + D::T * w;
-        const bool upper = n > P / 2;
-        const int kk = upper ? P - n : n;
-        const float2 A0 = sm[kk * D::LS + s0], B0 = sm[kk * D::LS + s1];
-        const float2 A = has0 ? A0 : make_float2(0.0f, 0.0f);
-        const float2 B = has1 ? B0 : make_float2(0.0f, 0.0f);
-        const float sg = upper ? -1.0f : 1.0f;   // conj of the mirrored half
-        g[w] = make_float2(A.x - sg * B.y, sg * A.y + B.x);
+        for (int w = 0; w < D::R; ++w) {
+            const int n = tt + D::T * w;
+            const bool upper = n > P / 2;
+            const int kk = upper ? P - n : n;
+            const float2 A0 = sm[kk * D::LS + s0], B0 = sm[kk * D::LS + s1];
+            const float2 A = has0 ? A0 : make_float2(0.0f, 0.0f);
+            const float2 B = has1 ? B0 : make_float2(0.0f, 0.0f);
+            const float sg = upper ? -1.0f : 1.0f;   // conj of the mirrored half
+            g[w] = make_float2(A.x - sg * B.y, sg * A.y + B.x);
+        }
     }
-    __syncthreads();
-    fft_line<P, true>(g, slot, tt, tw);
+    __syncthreads();   // every read of the line slots is done
+    if (!idle) fft_line<P, true, true>(g, slot, tt, tw);
 }
 
 // PRJ: block = (batch item b, chunk ch) sums rows [ch rchunk, ...) of

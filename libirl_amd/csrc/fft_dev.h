@@ -312,7 +312,24 @@ struct FftPass {
     }
 };
 
-template <int P, int IDX, int S, bool INV>
+// Synchronisation between the passes' LDS writes and reads.  Block-wide by
+// default.  With WAVE, a line's threads are consecutive lanes of one wave
+// (T divides 64, lines start at multiples of T): a wave's LDS instructions
+// are performed in issue order, so only the compiler must keep the accesses
+// in program order.
+template <bool WAVE>
+__device__ __forceinline__ void fft_sync()
+{
+    if constexpr (WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
+template <int P, int IDX, int S, bool INV, bool WAVE = false>
 __device__ __forceinline__ void fft_passes(float2 *a, float2 *ldsl, int tt,
                                            const float2 *__restrict__ tw)
 {
@@ -346,42 +363,22 @@ __device__ __forceinline__ void fft_passes(float2 *a, float2 *ldsl, int tt,
 #pragma unroll
         for (int w = 0; w < R; ++w) a[w] = b[w];
     } else {
-        __syncthreads();   // earlier readers of the LDS line are done
+        fft_sync<WAVE>();   // earlier readers of the LDS line are done
 #pragma unroll
         for (int k = 0; k < PS::U; ++k)
 #pragma unroll
             for (int v = 0; v < PS::r; ++v) ldsl[fft_skew(PS::out_pos(tt, k, v))] = a[k * PS::r + v];
-        __syncthreads();
-        fft_passes<P, IDX + 1, S * PS::r, INV>(a, ldsl, tt, tw);
+        fft_sync<WAVE>();
+        fft_passes<P, IDX + 1, S * PS::r, INV, WAVE>(a, ldsl, tt, tw);
     }
 }
 
-template <int P, bool INV>
+template <int P, bool INV, bool WAVE = false>
 __device__ __forceinline__ void fft_line(float2 *a, float2 *ldsl, int tt,
                                          const float2 *__restrict__ tw)
 {
-    fft_passes<P, 0, 1, INV>(a, ldsl, tt, tw);
-}
-
-// The block barriers of fft_passes<P, IDX, S, INV> without its work: the
-// same recursion, two barriers per pass that exchanges through LDS.  A wave
-// whose every line is idle calls this instead of fft_line, so the block's
-// waves still meet at every barrier.
-template <int P, int IDX, int S, bool INV>
-__device__ __forceinline__ void fft_passes_barriers()
-{
-    using PS = FftPass<P, IDX, S, INV>;
-    if constexpr (PS::m != 1) {
-        __syncthreads();
-        __syncthreads();
-        fft_passes_barriers<P, IDX + 1, S * PS::r, INV>();
-    }
-}
-
-template <int P, bool INV>
-__device__ __forceinline__ void fft_line_barriers()
-{
-    fft_passes_barriers<P, 0, 1, INV>();
+    static_assert(!WAVE || 64 % FftPlan<P>::T == 0, "a line within one wave");
+    fft_passes<P, 0, 1, INV, WAVE>(a, ldsl, tt, tw);
 }
 
 // W_P^k = exp(-2 pi i k / P), k < P, rounded from double: the twiddles the
