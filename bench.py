@@ -9,7 +9,11 @@ sampled at 2M counts, 8 subsets x 15 views.
 
 A step is one OSEM sub-iteration (15 views of projection, ratio,
 backprojection, then the multiplicative update) with every input resident in
-HBM.  The timed region cycles through the subsets in order.
+HBM.  The timed region cycles through the subsets in order.  The headline is
+the FP32 path with its DRF convolution algorithm chosen by timing at setup
+(the reference's spatial convolution or its 2-D FFT form); the spatial FP32
+path is reported beside it (fp32_spatial) and is the FP32 reference of the
+other lines' errors.
 
 Prints one JSON line (see the keys below).
 """
@@ -68,22 +72,27 @@ def build_workload(tmpdir, seed=2, name="c2", cfg=None):
                 act=act, offs=offs, psfs=psfs, start=start, cfg=cfg, krnl_file=krnl_file)
 
 
-def time_subiterations(w, steps, warmup, precision="fp32", cnn=None, check_every=0):
+def time_subiterations(w, steps, warmup, precision="fp32", cnn=None, check_every=0,
+                       drf="spatial"):
     """Device-resident reconstruction of the workload: 'warmup' untimed
     sub-iterations, then exactly 'steps' timed ones with per-kernel HIP events
     on the engine's stream.  With 'cnn' the scatter network replaces the
     per-view ESSE convolutions (checked against ESSE every 'check_every'
-    subsets).  Returns (recon, seconds of the timed steps,
+    subsets).  'drf' is the FP32 DRF algorithm mode (set_drf_fft: "spatial"
+    the reference's spatial convolution, "auto" the faster of it and the 2-D
+    FFT, timed at setup).  Returns (recon, seconds of the timed steps,
     {kernel: (ms_total, launches)}, estimate after the timed steps)."""
     import libirl_amd as L
     L.set_adv_options(0, 0)
     prev = L.set_precision(precision)
+    prev_drf = L.set_drf_fft(drf)
     L.set_scatter_cnn(cnn, check_every=check_every, max_nrmse=0.01)
     try:
         rec = L.Recon(w["parms"], w["opts"], w["views"], w["drf_file"], w["krnl_file"], None,
                       w["mu"], w["prj"], None)
     finally:
         L.set_precision(prev)
+        L.set_drf_fft(prev_drf)
         L.set_scatter_cnn(None)
     S = rec.num_subsets
     for k in range(warmup):
@@ -148,33 +157,76 @@ def cpu_baseline(w, nviews=15):
                        "(%.1f s), scaled x%.2f" % (nviews, dt, p.NumAngPerSubset / nviews))
 
 
+def fft_drf_bytes(w, rec, stage):
+    """Algorithmic HBM bytes of one launch of the FFT DRF kernel of a stage:
+    PRJ reads the subset's attenuated rotated planes and writes one partial
+    projection per chunk of rows (the engine's FFT chunking: max(NRot/32,
+    NRot/16) rows); BCK reads the ratio planes and, with attenuation, every
+    row's factors, and writes every row's plane.  The PSF spectra are read
+    from the on-chip caches (a few MB per table) and not counted."""
+    p = w["parms"]
+    n, nz, vs = p.NumPixels, p.NumSlices, rec.views_per_subset
+    plane = 4 * n * nz
+    if stage.startswith("drf_prj"):
+        rchunk = max(max(1, n // 32), n // 16)
+        nchunk = -(-n // rchunk)
+        return vs * n * plane + vs * nchunk * plane
+    atn = w["mu"] is not None
+    return vs * plane + (vs * n * plane if atn else 0) + vs * n * plane
+
+
 def drf_roofline(w, rec, stages, steps, warmup):
-    """Dominant DRF stage and its achieved algorithmic TFLOP/s (flops of the
-    timed subsets / average launch time from the HIP events)."""
+    """Dominant DRF stage and its roofline entry.  Spatial kernels: achieved
+    algorithmic TFLOP/s (flops of the timed subsets / average launch time from
+    the HIP events) against the FP32 or BF16 peak; FFT kernels: achieved
+    algorithmic HBM GB/s (fft_drf_bytes / average launch time) against the
+    HBM peak.  Returns (stage, roofline dict without peak/frac for spatial
+    kernels' precision, which the caller fills)."""
     S = rec.num_subsets
-    flops = drf_flops_per_subset(w, rec)
-    avg_flops = float(np.mean([flops[(warmup + k) % S] for k in range(steps)]))
     cands = [k for k in stages if k.startswith("drf_prj") or k.startswith("drf_bck")]
     dom = max(cands, key=lambda k: stages[k][0])
     dom_ms, dom_n = stages[dom]
-    return dom, avg_flops / (dom_ms / dom_n * 1e-3) / 1e12
+    avg_s = dom_ms / dom_n * 1e-3
+    if dom.endswith("_fft"):
+        gbs = fft_drf_bytes(w, rec, dom) / avg_s / 1e9
+        return dom, {"kernel": dom, "bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": gbs / PEAK_HBM_GBS}
+    flops = drf_flops_per_subset(w, rec)
+    avg_flops = float(np.mean([flops[(warmup + k) % S] for k in range(steps)]))
+    return dom, {"kernel": dom, "bound": "mfma", "achieved": avg_flops / avg_s / 1e12,
+                 "unit": "TFLOP/s"}
 
 
-def workload_line(w, steps, warmup, precision, ref_estimate=None, cnn=None, check_every=0):
+DRF_NAMES = {"spatial": "spatial FP32 DRF", "auto": "FP32 DRF, faster of spatial and 2-D FFT"}
+
+
+def roofline_line(w, rec, stages, steps, warmup, precision):
+    dom, roof = drf_roofline(w, rec, stages, steps, warmup)
+    if roof["bound"] == "mfma":
+        roof["peak"] = PEAK_BF16_TFLOPS if precision != "fp32" else PEAK_FP32_TFLOPS
+        roof["frac"] = roof["achieved"] / roof["peak"]
+    return dom, roof
+
+
+def workload_line(w, steps, warmup, precision, ref_estimate=None, cnn=None, check_every=0,
+                  drf="spatial"):
     """Sub-object of the JSON line for one workload and DRF arithmetic."""
     import libirl_amd as L
-    rec, dt, stages, est = time_subiterations(w, steps, warmup, precision, cnn, check_every)
-    dom, achieved = drf_roofline(w, rec, stages, steps, warmup)
+    rec, dt, stages, est = time_subiterations(w, steps, warmup, precision, cnn, check_every, drf)
+    dom, roof = roofline_line(w, rec, stages, steps, warmup, precision)
     esse = w["krnl_file"] is not None
     name = "osem-128^3x120v-atn-drf31-%s8subsets" % (
         ("cnn-scatter-" if cnn is not None else "esse33-") if esse else "")
     bf16 = precision != "fp32"
-    peak = PEAK_BF16_TFLOPS if bf16 else PEAK_FP32_TFLOPS
-    line = {"workload": name + (" (BF16 MFMA DRF, FP32 accumulation)" if bf16 else " (FP32)"),
+    algo = rec.drf_algorithms()
+    line = {"workload": name + (" (BF16 MFMA DRF, FP32 accumulation)" if bf16 else
+                                " (%s)" % DRF_NAMES[drf]),
             "value": steps / dt, "unit": "subiters/s", "ms_per_step": dt * 1e3 / steps,
-            "drf_arithmetic": dom.split("_", 2)[2] if dom.count("_") >= 2 else "fp32",
-            "roofline": {"kernel": dom, "bound": "mfma", "achieved": achieved, "peak": peak,
-                         "unit": "TFLOP/s", "frac": achieved / peak},
+            "drf_arithmetic": (dom.split("_", 2)[2] if dom.count("_") >= 2 else "fp32")
+            if bf16 else "fp32",
+            "drf_algorithm": {"prj": algo[0], "bck": algo[1], "fft_length": algo[2]}
+            if not bf16 else "bf16 mfma",
+            "roofline": roof,
             "stages_ms_per_launch": {k: v[0] / max(v[1], 1) for k, v in stages.items()}}
     if ref_estimate is not None:
         line["errors"] = {"osem_rel_l2_vs_fp32_gpu_after_%d_subiterations" % (warmup + steps):
@@ -385,6 +437,8 @@ def main():
                     help="run only the batch of 16 uneven reconstructions")
     ap.add_argument("--no-errors", action="store_true",
                     help="skip the FP64-oracle projection error of the BF16 variant")
+    ap.add_argument("--drf", default="auto", choices=["auto", "spatial", "fft"],
+                    help="FP32 DRF algorithm of the --only-esse line")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -408,16 +462,23 @@ def main():
         return
     if args.only_esse:
         we = build_workload(tmpdir, name="c3")
-        line = workload_line(we, args.steps, args.warmup, "fp32")
+        line = workload_line(we, args.steps, args.warmup, "fp32", drf=args.drf)
         line.pop("_estimate", None)
         print(json.dumps(line))
         return
     w = build_workload(tmpdir)
-    rec, dt, stages, est32 = time_subiterations(w, args.steps, args.warmup)
+    # the reference's spatial DRF convolution first: its estimate is the
+    # FP32 reference of the other lines
+    spatial = workload_line(w, args.steps, args.warmup, "fp32", drf="spatial")
+    est32 = spatial.pop("_estimate")
+    spatial["roofline"]["traffic"] = measured_traffic(spatial["roofline"]["kernel"])
+    # headline: the library's FP32 path with the DRF algorithm timed at setup
+    rec, dt, stages, est = time_subiterations(w, args.steps, args.warmup, drf="auto")
     S = rec.num_subsets
     ms_step = dt * 1e3 / args.steps
-    dom, achieved = drf_roofline(w, rec, stages, args.steps, args.warmup)
-    traffic = measured_traffic(dom)
+    dom, roof = roofline_line(w, rec, stages, args.steps, args.warmup, "fp32")
+    roof["traffic"] = measured_traffic(dom)
+    algo = rec.drf_algorithms()
     out = {
         "metric": METRIC,
         "value": args.steps / dt,
@@ -434,11 +495,13 @@ def main():
         "config": {"workload": "osem-128^3x120v-atn-drf31-8subsets (plain FP32, no scatter)",
                    "N": 128, "Nz": 128, "views": 120, "subsets": S,
                    "views_per_subset": rec.views_per_subset, "drf": "31x31 Gaussian",
+                   "drf_algorithm": {"prj": algo[0], "bck": algo[1], "fft_length": algo[2]},
                    "parallelism": "single GPU"},
-        "roofline": {"kernel": dom, "bound": "mfma", "achieved": achieved,
-                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic},
+        "roofline": roof,
         "stages_ms_per_launch": {k: v[0] / max(v[1], 1) for k, v in stages.items()},
+        "errors": {"osem_rel_l2_vs_spatial_fp32_after_%d_subiterations" % (
+            args.warmup + args.steps): float(np.linalg.norm(est - est32) / np.linalg.norm(est32))},
+        "fp32_spatial": spatial,
     }
     rec.close()
     if not args.no_bf16:
@@ -459,16 +522,23 @@ def main():
         # plain-method scatter configuration), FP32 and with the BF16 DRF.
         # 8 + 24 sub-iterations: the configuration's 4 iterations of 8 subsets.
         we = build_workload(tmpdir, name="c3")
-        out["esse"] = workload_line(we, 24, 8, "fp32")
-        ref = out["esse"].pop("_estimate")
+        # the plain method with the reference's spatial DRF (the FP32
+        # reference of the other lines), then with the timed DRF choice
+        out["esse_spatial"] = workload_line(we, 24, 8, "fp32", drf="spatial")
+        ref = out["esse_spatial"].pop("_estimate")
+        out["esse"] = workload_line(we, 24, 8, "fp32", ref_estimate=ref, drf="auto")
         if not args.no_bf16:
             out["esse_bf16"] = workload_line(we, 24, 8, "bf16", ref_estimate=ref)
         net = load_scatter_cnn()
         if net is not None and not args.no_cnn:
-            # Learned scatter + BF16 DRF, checked against ESSE every 8 subsets
+            # Learned scatter + BF16 DRF (and + FP32 DRF), checked against
+            # ESSE every 8 subsets
             for prec, key in (("bf16", "cnn_bf16"), ("fp32", "cnn_fp32")):
-                line = workload_line(we, 24, 8, prec, ref_estimate=ref, cnn=net, check_every=8)
+                line = workload_line(we, 24, 8, prec, ref_estimate=ref, cnn=net, check_every=8,
+                                     drf="auto")
                 line["speedup_vs_esse_fp32"] = line["value"] / out["esse"]["value"]
+                line["speedup_vs_esse_fp32_spatial"] = (line["value"] /
+                                                        out["esse_spatial"]["value"])
                 line["training"] = cnn_training_record()
                 out[key] = line
             if not args.no_batch:
