@@ -21,9 +21,14 @@ int drf_fft_size(int NRot, int Nz, int KX, int KY);
 struct DrfFftTables {
     int P = 0, nE = 0;
     float2 *H = nullptr;    // [nE][P/2+1][P] in the kernels' register order
+    // PSFs symmetric in both axes have real spectra, even in both
+    // frequencies: [nE][P/2+1][P/2+1] real (a quarter of the bytes)
+    float *Hr = nullptr;
     float2 *tw = nullptr;   // [P]
-    // psf: device [nE][KX][KYP] (transposed, PSFs centred in KX x KY)
-    void build(hipStream_t s, int P, const float *psf, int nE, int KX, int KY, int KYP);
+    // psf: device [nE][KX][KYP] (transposed, PSFs centred in KX x KY);
+    // symmetric: every PSF equals its mirror images in x and in y
+    void build(hipStream_t s, int P, const float *psf, int nE, int KX, int KY, int KYP,
+               bool symmetric);
     void release();
     DrfFftTables() = default;
     DrfFftTables(const DrfFftTables &) = delete;

@@ -129,27 +129,27 @@ __device__ __forceinline__ void drf_fft_forward(const DrfArgs &a, int b, int v, 
         drf_fft_pad<P>(a, b, v, row, pad, s, c0, c1, zact, tt);
         fft_line<P, false, true>(s, slot, tt, tw);
     }
-    // (idle waves write their slots below; the bin pairs' values overwrite
-    // every position the bin lines read)
     // Between phases the lane indices pass through an empty asm, so each
     // phase computes its own LDS addresses instead of the compiler keeping
     // those of every phase live at once.
     asm volatile("" : "+v"(tt), "+v"(li));
     slot = sm + li * D::LS;
-    __syncthreads();
-#pragma unroll
-    for (int w = 0; w < D::R; ++w) slot[fft_skew(tt + D::T * w)] = s[w];
-    __syncthreads();
-    // Z = A + iB: A[k] = (Z[k] + conj Z[P-k]) / 2, B[k] = (Z[k] - conj Z[P-k]) / 2i
+    // Z = A + iB: A[k] = (Z[k] + conj Z[P-k]) / 2, B[k] = (Z[k] - conj Z[P-k]) / 2i.
+    // Z[P - k], k = tt + T w, is in lane T - tt of the line, register R-1-w
+    // (tt > 0), or in this thread, register R - w (tt = 0): a lane shuffle
+    // within the wave, no LDS.
+    const int lane = (int)(threadIdx.x & 63);
+    const int src = (lane & ~(D::T - 1)) | ((D::T - tt) & (D::T - 1));
     float2 A[D::KH], Bq[D::KH];
 #pragma unroll
     for (int w = 0; w < D::KH; ++w) {
-        const int k = tt + D::T * w;
         const float2 zk = s[w];
-        const float2 zm = slot[fft_skew(k == 0 ? 0 : P - k)];
+        float2 zm = make_float2(__shfl(s[D::R - 1 - w].x, src, 64), __shfl(s[D::R - 1 - w].y, src, 64));
+        if (tt == 0) zm = s[w == 0 ? 0 : D::R - w];
         A[w] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
         Bq[w] = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
     }
+    // every wave's slice transforms are done with their line slots
     __syncthreads();
     if (zact) {
 #pragma unroll
@@ -228,6 +228,7 @@ __device__ __forceinline__ void drf_fft_inverse(const DrfArgs &a, float2 *g, flo
 template <int P>
 __global__ __launch_bounds__(DrfFftShape<P>::NT) void k_drf_fft_prj(DrfArgs a,
                                                                     const float2 *__restrict__ H,
+                                                                    const float *__restrict__ Hr,
                                                                     const float2 *__restrict__ twg)
 {
     using D = DrfFftShape<P>;
@@ -254,7 +255,17 @@ __global__ __launch_bounds__(DrfFftShape<P>::NT) void k_drf_fft_prj(DrfArgs a,
         asm volatile("" : "+v"(lq), "+v"(tq));
         drf_fft_forward<P>(a, b, v, row, pad, s, sm, lq, tq, tw);
         if (xact) {
-            if (kidx >= 0) {
+            if (kidx >= 0 && Hr) {
+                // real spectrum, even in kx: value at min(kx, P - kx)
+                const float *h = Hr + ((long)kidx * D::WC + li) * D::WC;
+#pragma unroll
+                for (int w = 0; w < D::R; ++w) {
+                    const int kx = tt + D::T * w;
+                    const float hv = h[kx <= P / 2 ? kx : P - kx];
+                    acc[w].x = fmaf(s[w].x, hv, acc[w].x);
+                    acc[w].y = fmaf(s[w].y, hv, acc[w].y);
+                }
+            } else if (kidx >= 0) {
                 // this thread's R values are contiguous: R/2 128-bit loads
                 const float4 *h = (const float4 *)(H + ((long)kidx * D::WC + li) * P + tt * D::R);
 #pragma unroll
@@ -299,6 +310,7 @@ __global__ __launch_bounds__(DrfFftShape<P>::NT) void k_drf_fft_prj(DrfArgs a,
 template <int P>
 __global__ __launch_bounds__(DrfFftShape<P>::NT) void k_drf_fft_bck(DrfArgs a,
                                                                     const float2 *__restrict__ H,
+                                                                    const float *__restrict__ Hr,
                                                                     const float2 *__restrict__ twg,
                                                                     int rows, int nblk)
 {
@@ -328,7 +340,15 @@ __global__ __launch_bounds__(DrfFftShape<P>::NT) void k_drf_fft_bck(DrfArgs a,
             __syncthreads();   // the inverse rewrites the line slots
         }
         if (xact) {
-            if (kidx >= 0) {
+            if (kidx >= 0 && Hr) {
+                const float *h = Hr + ((long)kidx * D::WC + li) * D::WC;
+#pragma unroll
+                for (int w = 0; w < D::R; ++w) {
+                    const int kx = tt + D::T * w;
+                    const float hv = h[kx <= P / 2 ? kx : P - kx];
+                    g[w] = make_float2(spec[w].x * hv, spec[w].y * hv);
+                }
+            } else if (kidx >= 0) {
                 const float4 *h = (const float4 *)(H + ((long)kidx * D::WC + li) * P + tt * D::R);
 #pragma unroll
                 for (int w2 = 0; w2 < D::R / 2; ++w2) {
@@ -399,6 +419,33 @@ __global__ void k_drf_spectra(const float *__restrict__ psf, int nE, int KX, int
     H[id] = make_float2((float)(re * sc), (float)(im * sc));
 }
 
+// Hr[e][kz][kx] = (1/P^2) sum_{i,j} psf_e[j][i] cos(2 pi kz (j - CY)/P)
+// cos(2 pi kx (i - CX)/P), kz, kx <= P/2: the spectrum above for a PSF
+// symmetric in both axes (the sine terms cancel in pairs).
+template <int P>
+__global__ void k_drf_spectra_real(const float *__restrict__ psf, int nE, int KX, int KY,
+                                   int KYP, const double2 *__restrict__ root,
+                                   float *__restrict__ Hr)
+{
+    constexpr int WC = P / 2 + 1;
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long)nE * WC * WC) return;
+    const int e = (int)(id / (WC * WC));
+    const int kz = (int)(id / WC % WC), kx = (int)(id % WC);
+    const int CX = (KX - 1) / 2, CY = (KY - 1) / 2;
+    const float *p = psf + (long)e * KX * KYP;
+    double acc = 0.0;
+    for (int i = 0; i < KX; ++i) {
+        double c = 0.0;
+        for (int j = 0; j < KY; ++j) {
+            const double val = p[i * KYP + j];
+            if (val != 0.0) c += val * root[((kz * (j - CY)) % P + P) % P].x;
+        }
+        acc += c * root[((kx * (i - CX)) % P + P) % P].x;
+    }
+    Hr[id] = (float)(acc / ((double)P * P));
+}
+
 // supported transform lengths, ascending
 static const int kDrfFftSizes[] = {32, 64, 128, 160};
 
@@ -413,12 +460,15 @@ int drf_fft_size(int NRot, int Nz, int KX, int KY)
 void DrfFftTables::release()
 {
     if (H) (void)hipFree(H);
+    if (Hr) (void)hipFree(Hr);
     if (tw) (void)hipFree(tw);
     H = tw = nullptr;
+    Hr = nullptr;
     P = nE = 0;
 }
 
-void DrfFftTables::build(hipStream_t s, int P_, const float *psf, int nE_, int KX, int KY, int KYP)
+void DrfFftTables::build(hipStream_t s, int P_, const float *psf, int nE_, int KX, int KY, int KYP,
+                         bool symmetric)
 {
     release();
     if (nE_ <= 0) fail(ETYPE_ILLEGAL_VALUE, "FFT DRF: empty PSF table");
@@ -431,21 +481,35 @@ void DrfFftTables::build(hipStream_t s, int P_, const float *psf, int nE_, int K
     const std::vector<float2> htw = twiddles(P_);
     double2 *root = nullptr;
     DFFT_CHECK(hipMalloc((void **)&root, P_ * sizeof(double2)));
-    DFFT_CHECK(hipMalloc((void **)&H, (size_t)nE_ * per * sizeof(float2)));
+    const long wc = P_ / 2 + 1;
+    if (symmetric)
+        DFFT_CHECK(hipMalloc((void **)&Hr, (size_t)nE_ * wc * wc * sizeof(float)));
+    else
+        DFFT_CHECK(hipMalloc((void **)&H, (size_t)nE_ * per * sizeof(float2)));
     DFFT_CHECK(hipMalloc((void **)&tw, P_ * sizeof(float2)));
     P = P_;
     nE = nE_;
     DFFT_CHECK(hipMemcpyAsync(root, hroot.data(), P_ * sizeof(double2), hipMemcpyHostToDevice, s));
     DFFT_CHECK(hipMemcpyAsync(tw, htw.data(), P_ * sizeof(float2), hipMemcpyHostToDevice, s));
-    const long n = (long)nE_ * per;
+    const long n = symmetric ? (long)nE_ * wc * wc : (long)nE_ * per;
     const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+#define DFFT_SPECTRA(PP)                                                                       \
+    case PP:                                                                                   \
+        if (symmetric)                                                                         \
+            hipLaunchKernelGGL(k_drf_spectra_real<PP>, grid, block, 0, s, psf, nE_, KX, KY,    \
+                               KYP, root, Hr);                                                 \
+        else                                                                                   \
+            hipLaunchKernelGGL(k_drf_spectra<PP>, grid, block, 0, s, psf, nE_, KX, KY, KYP,    \
+                               root, H);                                                       \
+        break;
     switch (P_) {
-    case 32: hipLaunchKernelGGL(k_drf_spectra<32>, grid, block, 0, s, psf, nE_, KX, KY, KYP, root, H); break;
-    case 64: hipLaunchKernelGGL(k_drf_spectra<64>, grid, block, 0, s, psf, nE_, KX, KY, KYP, root, H); break;
-    case 128: hipLaunchKernelGGL(k_drf_spectra<128>, grid, block, 0, s, psf, nE_, KX, KY, KYP, root, H); break;
-    case 160: hipLaunchKernelGGL(k_drf_spectra<160>, grid, block, 0, s, psf, nE_, KX, KY, KYP, root, H); break;
+        DFFT_SPECTRA(32)
+        DFFT_SPECTRA(64)
+        DFFT_SPECTRA(128)
+        DFFT_SPECTRA(160)
     default: fail(ETYPE_ILLEGAL_VALUE, "FFT DRF: unsupported length %d", P_);
     }
+#undef DFFT_SPECTRA
     DFFT_CHECK(hipGetLastError());
     DFFT_CHECK(hipStreamSynchronize(s));
     DFFT_CHECK(hipFree(root));
@@ -468,20 +532,20 @@ static void launch_p(hipStream_t s, bool bck, const DrfArgs &a, int B, const Drf
         const long nb = (long)B * nblk;
         if (nb <= 0 || nb > INT_MAX) fail(ETYPE_ILLEGAL_VALUE, "bad FFT DRF launch size");
         hipLaunchKernelGGL(k_drf_fft_bck<P>, dim3((unsigned)nb), dim3(D::NT), D::LDS, s, a, t.H,
-                           t.tw, rows, nblk);
+                           t.Hr, t.tw, rows, nblk);
     } else {
         const long nb = (long)B * a.nchunk;
         if (nb <= 0 || nb > INT_MAX || a.rchunk <= 0)
             fail(ETYPE_ILLEGAL_VALUE, "bad FFT DRF launch size");
         hipLaunchKernelGGL(k_drf_fft_prj<P>, dim3((unsigned)nb), dim3(D::NT), D::LDS, s, a, t.H,
-                           t.tw);
+                           t.Hr, t.tw);
     }
     DFFT_CHECK(hipGetLastError());
 }
 
 void launch_drf_fft(hipStream_t s, bool bck, const DrfArgs &a, int B, const DrfFftTables &t)
 {
-    if (!t.H || !t.tw) fail(ETYPE_ILLEGAL_VALUE, "FFT DRF: tables not built");
+    if ((!t.H && !t.Hr) || !t.tw) fail(ETYPE_ILLEGAL_VALUE, "FFT DRF: tables not built");
     switch (t.P) {
     case 32: launch_p<32>(s, bck, a, B, t); break;
     case 64: launch_p<64>(s, bck, a, B, t); break;

@@ -1685,7 +1685,18 @@ Recon::Recon(const Config &cfg) : impl_(new Impl), cfg_(cfg)
         if (cfg.precision == 0 && cfg.drfFft > 0) {
             const int P = drf_fft_size(g.NRot, g.Nz, KX, KYb);
             if (P > 0) {
-                I.fft.build(I.stream, P, I.psf.p, cfg.drf.nDist, KX, KYb, KYP);
+                // real, even spectra when every PSF is its own mirror image in
+                // both axes (exact comparison: the products are then equal)
+                bool sym = true;
+                for (int d = 0; d < cfg.drf.nDist && sym; ++d)
+                    for (int j = 0; j < KY && sym; ++j)
+                        for (int i = 0; i < KX && sym; ++i) {
+                            const float *q = &cfg.drf.psf[(size_t)d * KY * KX];
+                            const float v0 = q[(size_t)j * KX + i];
+                            sym = v0 == q[(size_t)j * KX + (KX - 1 - i)] &&
+                                  v0 == q[(size_t)(KY - 1 - j) * KX + i];
+                        }
+                I.fft.build(I.stream, P, I.psf.p, cfg.drf.nDist, KX, KYb, KYP, sym);
                 if (cfg.drfFft == 1) I.set_fft(true, true);
             }
         }

@@ -96,6 +96,43 @@ def test_fft_kernel_with_rows_beyond_the_detector():
     b.close()
 
 
+def _asymmetric_table(pb, half, ramp_x, ramp_y):
+    """Replaces the problem's DRF table by its Gaussians times linear ramps
+    (the spectra of the FFT kernel are then complex)."""
+    import os
+    from libirl_amd import synth
+    from oracle import oracle as O
+    offs, psfs, start = synth.drf_for_orbit(pb.drf_file, pb.pix, pb.n, pb.views, half)
+    r = np.arange(2 * half + 1) - half
+    ramp = np.outer(1.0 + ramp_y * r / half, 1.0 + ramp_x * r / half)
+    psfs = [(p * ramp / (p * ramp).sum()).astype(np.float32) for p in psfs]
+    pb.drf_file = os.path.join(pb.tmpdir, "asym.drft")
+    synth.write_drft(pb.drf_file, pb.pix, pb.pix, start, offs, psfs)
+    pb.drf = O.DrfTable(start, pb.pix, offs, psfs)
+
+
+# an x ramp only (symmetric in y) and both ramps: complex spectra either way
+@pytest.mark.parametrize("n,nz,half,ramps", [(32, 16, 5, (0.4, 0.0)), (128, 32, 15, (0.3, 0.5))])
+def test_asymmetric_psfs(n, nz, half, ramps):
+    print("FFT DRF, asymmetric PSFs N=%d ramps %s" % (n, ramps), flush=True)
+    pb = Problem(n=n, nz=nz, nviews=8, per=2, iters=2, half=half, noise=False,
+                 pix=0.5 if n <= 64 else 0.33)
+    _asymmetric_table(pb, half, *ramps)
+    a = _recon(pb, "spatial")
+    b = _recon(pb, "fft")
+    ga, gb = _run(a, 2), _run(b, 2)
+    e = rel_l2(gb, ga)
+    print("rel L2 FFT vs spatial %.3g" % e)
+    assert e <= 1e-5
+    if n <= 32:
+        ref = pb.oracle_osem("f64")
+        e_ref = rel_l2(gb, ref)
+        print("rel L2 vs FP64 oracle %.3g" % e_ref)
+        assert e_ref <= 1e-4
+    a.close()
+    b.close()
+
+
 def test_timed_choice_runs_one_of_the_two():
     print("FFT DRF timed choice", flush=True)
     pb = Problem(n=64, nz=32, nviews=8, per=2, iters=1, half=15, noise=False)
