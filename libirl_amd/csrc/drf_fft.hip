@@ -80,13 +80,34 @@ __device__ __forceinline__ void drf_fft_pad(const DrfArgs &a, int b, int v, int 
     }
 }
 
+// One 4-byte load per 128-byte line of plane (b, row) of the input and of
+// its factors (nothing for row < 0): pulls the block's next plane toward its
+// L2 while the current one is transformed.  The caller folds the returned
+// value in only after the row's later loads, so these stay in flight.
+__device__ __forceinline__ float drf_fft_touch(const DrfArgs &a, int b, int v, int row)
+{
+    float r = 0.0f;
+    if (row < 0) return r;
+    const long span = (long)(a.NRot - 1) * a.xs + a.Nz;
+    const float *p = a.in + (long)b * a.inB + (long)row * a.inRow;
+    const float *f = a.afac ? a.afac + (long)v * a.afB + (long)row * a.afRow : nullptr;
+    for (long e = (long)threadIdx.x * 32; e < span; e += (long)blockDim.x * 32) {
+        r += p[e];
+        if (f) r += f[e];
+    }
+    return r;
+}
+
 // Forward 2-D transform of plane (b, row) of the DRF input (with axial pad
 // 'pad').  On exit thread (line li, tt), li < P/2 + 1, holds
 // S[kz = li][kx = tt + T w] in s[w].  Every thread of the block calls it.
+// After the plane's loads it issues drf_fft_touch of row 'next' into
+// 'pending'.
 template <int P>
 __device__ __forceinline__ void drf_fft_forward(const DrfArgs &a, int b, int v, int row, int pad,
                                                 float2 *s, float2 *sm, int li, int tt,
-                                                const float2 *__restrict__ tw)
+                                                const float2 *__restrict__ tw, int next,
+                                                float &pending)
 {
     using D = DrfFftShape<P>;
     float2 *slot = sm + li * D::LS;
@@ -127,8 +148,10 @@ __device__ __forceinline__ void drf_fft_forward(const DrfArgs &a, int b, int v, 
             s[w] = make_float2((zact && in) ? re : 0.0f, (has1 && in) ? im : 0.0f);
         }
         drf_fft_pad<P>(a, b, v, row, pad, s, c0, c1, zact, tt);
-        fft_line<P, false, true>(s, slot, tt, tw);
     }
+    __builtin_amdgcn_sched_barrier(0);   // the next plane's loads after this one's
+    pending = drf_fft_touch(a, b, v, next);
+    if (!zIdle) fft_line<P, false, true>(s, slot, tt, tw);
     // Between phases the lane indices pass through an empty asm, so each
     // phase computes its own LDS addresses instead of the compiler keeping
     // those of every phase live at once.
@@ -243,6 +266,7 @@ __global__ __launch_bounds__(DrfFftShape<P>::NT) void k_drf_fft_prj(DrfArgs a,
     const bool xact = li < D::WC;
     const float unit = 1.0f / ((float)P * (float)P);
     float2 acc[D::R], s[D::R];
+    float keep = 0.0f;
 #pragma unroll
     for (int w = 0; w < D::R; ++w) acc[w] = make_float2(0.0f, 0.0f);
     for (int row = r0; row < r1; ++row) {
@@ -253,7 +277,9 @@ __global__ __launch_bounds__(DrfFftShape<P>::NT) void k_drf_fft_prj(DrfArgs a,
         // keeping hundreds of them live across the loop
         int lq = li, tq = tt;
         asm volatile("" : "+v"(lq), "+v"(tq));
-        drf_fft_forward<P>(a, b, v, row, pad, s, sm, lq, tq, tw);
+        float pending;
+        drf_fft_forward<P>(a, b, v, row, pad, s, sm, lq, tq, tw, row + 1 < r1 ? row + 1 : -1,
+                           pending);
         if (xact) {
             if (kidx >= 0 && Hr) {
                 // real spectrum, even in kx: value at min(kx, P - kx)
@@ -288,8 +314,10 @@ __global__ __launch_bounds__(DrfFftShape<P>::NT) void k_drf_fft_prj(DrfArgs a,
                 }
             }
         }
+        keep += pending;
         __syncthreads();   // the next row rewrites the line slots
     }
+    if (a.nrows < 0) a.out[0] = keep;   // never: keeps the touch loads
     drf_fft_inverse<P>(a, acc, sm, li, tt, tw);
     const int c0 = 2 * li, c1 = c0 + 1;
     float *o = a.out + (long)b * a.outB + (long)ch * a.outRow;
@@ -328,14 +356,17 @@ __global__ __launch_bounds__(DrfFftShape<P>::NT) void k_drf_fft_bck(DrfArgs a,
     const float unit = 1.0f / ((float)P * (float)P);
     const int c0 = 2 * li, c1 = c0 + 1;
     float2 spec[D::R], g[D::R];
+    float keep = 0.0f;
     int have = -1;
     for (int row = r0; row < r1; ++row) {
         const int kidx = a.rowKernel[(long)v * a.nrows + row];
         const int pad = (kidx < 0 || a.avg <= 0) ? 0 : a.rowPad[(long)v * a.nrows + row];
         int lq = li, tq = tt;
         asm volatile("" : "+v"(lq), "+v"(tq));
+        float pending = 0.0f;
         if (!reuse || pad != have) {
-            drf_fft_forward<P>(a, b, v, row, pad, spec, sm, lq, tq, tw);
+            drf_fft_forward<P>(a, b, v, row, pad, spec, sm, lq, tq, tw,
+                               (!reuse && row + 1 < r1) ? row + 1 : -1, pending);
             have = pad;
             __syncthreads();   // the inverse rewrites the line slots
         }
@@ -379,8 +410,10 @@ __global__ __launch_bounds__(DrfFftShape<P>::NT) void k_drf_fft_bck(DrfArgs a,
                 if (c1 < a.NRot) o[(long)c1 * a.outXS + n] = g[w].y * a.scale;
             }
         }
+        keep += pending;
         __syncthreads();   // the next row rewrites the line slots
     }
+    if (a.nrows < 0) a.out[0] = keep;   // never: keeps the touch loads
 }
 
 // H[e][kz][tt][w] = (1/P^2) sum_{i,j} psf_e[j][i] W^(kz (j - CY) + kx (i - CX)),
@@ -525,9 +558,9 @@ static void launch_p(hipStream_t s, bool bck, const DrfArgs &a, int B, const Drf
     if (a.zs != 1 || a.outZS != 1)
         fail(ETYPE_ILLEGAL_VALUE, "FFT DRF: slice-fastest planes only");
     if (bck) {
-        // without per-row factors a block reuses one input spectrum for 8 rows
-        const bool reuse = a.afac == nullptr && a.inRow == 0;
-        const int rows = reuse ? std::min(8, a.nrows) : 1;
+        // 8 rows per block: without per-row factors one input spectrum
+        // serves them all, with factors the next row's plane is prefetched
+        const int rows = std::min(8, a.nrows);
         const int nblk = (a.nrows + rows - 1) / rows;
         const long nb = (long)B * nblk;
         if (nb <= 0 || nb > INT_MAX) fail(ETYPE_ILLEGAL_VALUE, "bad FFT DRF launch size");
