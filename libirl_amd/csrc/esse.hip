@@ -160,24 +160,26 @@ __device__ __forceinline__ long xcd_block()
     return (xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per) + k;
 }
 
-// Y conv of one kernel term: lines along depth at (b, m, column c).
-// Forward FFT (rows y >= ny are zero), multiply by the term's spectrum,
-// inverse FFT, keep y < ny.  One launch per term keeps the kernel at one
-// line's worth of registers (the forward transform is recomputed per term
-// from the buffer instead of being held).  The batch index runs fastest over
+// Y conv of the kernel terms: lines along depth at (b, m, column c).
+// Forward FFT (rows y >= ny are zero) once, then per term: multiply by the
+// term's spectrum, inverse FFT, keep y < ny.  Between the phases the lane
+// indices pass through an empty asm, so each phase computes its own LDS
+// addresses instead of the compiler keeping those of every phase live (that
+// kept the kernel at two waves per SIMD).  The batch index runs fastest over
 // the logical blocks, so the B blocks that read the same spectrum columns
 // run together on one XCD and share them through its L2.
-template <int P>
+template <int P, int NT>
 __global__ __launch_bounds__(256) void k_fft_y_conv(const float2 *__restrict__ buf,
                                                     const float2 *__restrict__ khat,
-                                                    float2 *__restrict__ outk, long outBStride,
-                                                    int B, int ny, int Pm, int Wc,
+                                                    long khatStride, float2 *__restrict__ outk,
+                                                    long outTStride, long outBStride, int B,
+                                                    int ny, int Pm, int Wc,
                                                     const float2 *__restrict__ tw)
 {
     using F = FftLine<P>;
     constexpr int LB = 256 / F::T;
     extern __shared__ float2 sm[];
-    const int li = threadIdx.x % LB, tt = threadIdx.x / LB;
+    int li = threadIdx.x % LB, tt = threadIdx.x / LB;
     // columns (m, kc) flattened: a block's LB lanes take LB consecutive
     // elements of a depth row, across m boundaries, none idle but the last
     const long ystride = (long)Pm * Wc;
@@ -193,19 +195,25 @@ __global__ __launch_bounds__(256) void k_fft_y_conv(const float2 *__restrict__ b
         a[w] = (act && n < ny) ? src[n * ystride] : make_float2(0.0f, 0.0f);
     }
     fft_line<P, false>(a, sm + li * F::LS, tt, tw);
-    const float2 *kt = khat + col;
+#pragma unroll 1
+    for (int t = 0; t < NT; ++t) {
+        asm volatile("" : "+v"(tt), "+v"(li));
+        const float2 *kt = khat + t * khatStride + col;
+        float2 c[F::R];
 #pragma unroll
-    for (int w = 0; w < F::R; ++w) {
-        const int n = tt + F::T * w;
-        a[w] = act ? cmul(a[w], kt[n * ystride]) : make_float2(0.0f, 0.0f);
-    }
-    fft_line<P, true>(a, sm + li * F::LS, tt, tw);
-    if (!act) return;
-    float2 *dst = outk + (long)b * outBStride + col;
+        for (int w = 0; w < F::R; ++w) {
+            const int n = tt + F::T * w;
+            c[w] = act ? cmul(a[w], kt[n * ystride]) : make_float2(0.0f, 0.0f);
+        }
+        fft_line<P, true>(c, sm + li * F::LS, tt, tw);
+        if (act) {
+            float2 *dst = outk + (long)b * outBStride + t * outTStride + col;
 #pragma unroll
-    for (int w = 0; w < F::R; ++w) {
-        const int n = tt + F::T * w;
-        if (n < ny) dst[n * ystride] = a[w];
+            for (int w = 0; w < F::R; ++w) {
+                const int n = tt + F::T * w;
+                if (n < ny) dst[n * ystride] = c[w];
+            }
+        }
     }
 }
 
@@ -534,13 +542,19 @@ void EsseDev::convolve(const float *act, int B, float *terms)
     const long plane = (long)Pm * Wc;
     fft_f_fwd(src, buf_, (long)B * ny * nm, nm, nf);
     fft_strided(Pm, false, buf_, buf_, (long)B * ny, plane, Wc, Wc, nm, Pm, twm_);
-    for (int t = 0; t < T; ++t) {
+    {
         const dim3 grid((unsigned)((long)B * cdiv((long)Pm * Wc, lines_per_block(Py))));
-        const float2 *kh = khat_ + (long)Py * plane * t;
-        float2 *ok = outk_ + (long)t * ny * plane;
-        const long obs = (long)T * ny * plane;
-#define L_(PP) hipLaunchKernelGGL(k_fft_y_conv<PP>, grid, dim3(256), fft_lds(PP), stream_, buf_, \
-                                  kh, ok, obs, B, ny, Pm, Wc, twy_)
+        const long khs = (long)Py * plane, ots = (long)ny * plane, obs = (long)T * ny * plane;
+#define L_(PP)                                                                                 \
+    if (T == 1)                                                                                \
+        hipLaunchKernelGGL((k_fft_y_conv<PP, 1>), grid, dim3(256), fft_lds(PP), stream_, buf_, \
+                           khat_, khs, outk_, ots, obs, B, ny, Pm, Wc, twy_);                  \
+    else if (T == 2)                                                                           \
+        hipLaunchKernelGGL((k_fft_y_conv<PP, 2>), grid, dim3(256), fft_lds(PP), stream_, buf_, \
+                           khat_, khs, outk_, ots, obs, B, ny, Pm, Wc, twy_);                  \
+    else                                                                                       \
+        hipLaunchKernelGGL((k_fft_y_conv<PP, 3>), grid, dim3(256), fft_lds(PP), stream_, buf_, \
+                           khat_, khs, outk_, ots, obs, B, ny, Pm, Wc, twy_)
         ESSE_P_DISPATCH(Py, L_);
 #undef L_
     }

@@ -12,9 +12,9 @@ from tests.cases import Problem, rel_l2
 pytestmark = pytest.mark.gpu
 
 
-def _recon(pb, mode):
+def _recon(pb, mode, fast_rotate=0):
     import libirl_amd as L
-    L.set_adv_options(0, 0)
+    L.set_adv_options(fast_rotate, 0)
     prev = L.set_drf_fft(mode)
     try:
         return L.Recon(pb.parms(), pb.options(), pb.views, pb.drf_file, None, None, pb.mu, pb.prj,
@@ -129,6 +129,21 @@ def test_asymmetric_psfs(n, nz, half, ramps):
         e_ref = rel_l2(gb, ref)
         print("rel L2 vs FP64 oracle %.3g" % e_ref)
         assert e_ref <= 1e-4
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("fast_rotate", [0, 1])
+def test_fft_projector_with_both_rotations(fast_rotate):
+    # the bilinear (rot3dpar) and the three-shear rotation feed the same
+    # FFT kernels
+    print("FFT DRF, fast rotate %d" % fast_rotate, flush=True)
+    pb = Problem(n=64, nz=24, nviews=8, per=2, iters=2, half=15, noise=False)
+    a = _recon(pb, "spatial", fast_rotate)
+    b = _recon(pb, "fft", fast_rotate)
+    e = rel_l2(_run(b, 2), _run(a, 2))
+    print("rel L2 FFT vs spatial %.3g" % e)
+    assert e <= 1e-5
     a.close()
     b.close()
 
