@@ -312,12 +312,12 @@ def batch_configs(count=16, seed=5):
     return out
 
 
-def batch_line(tmpdir, net, steps=16, warmup=16):
+def batch_line(tmpdir, net, steps=16, warmup=24):
     """16 concurrent reconstructions of uneven size: learned scatter + BF16
-    DRF, graph replay and autotuned tiles, stepped through one batch (one
-    parent-graph launch per step) and, for comparison, one after another;
-    then each estimate after 4 iterations against its FP32 ESSE
-    reconstruction."""
+    DRF, graph replay and autotuned tiles, stepped through one batch (each
+    member's graph on its own stream, and one parent-graph launch per step)
+    and, for comparison, one after another; then each estimate after 5
+    iterations against its FP32 ESSE reconstruction."""
     import libirl_amd as L
     cfgs = batch_configs()
     works = [build_workload(tmpdir, seed=100 + i, name="b%d" % i, cfg=c)
@@ -348,7 +348,9 @@ def batch_line(tmpdir, net, steps=16, warmup=16):
         batch.step()
     dts = [timed(False)]
     st = batch.stats()
-    est = [r.estimate() for r in recs]          # after warmup + steps = 4 iterations
+    # warmup: every member's subsets run eagerly once, then are captured and
+    # replayed once, so the timed steps replay warm graph executables
+    est = [r.estimate() for r in recs]          # after warmup + steps = 5 iterations
     tiles = [r.tiles() for r in recs]
     status = [L.cnn_status(r) for r in recs]
     # the same steps as one parent-graph launch per step (its executables
@@ -396,7 +398,7 @@ def batch_line(tmpdir, net, steps=16, warmup=16):
             "eager_subiterations": st["eager_subiterations"],
             "learned_scatter_active": sum(s["active"] for s in status),
             "worst_check_nrmse": max(s["worst_nrmse"] for s in status),
-            "rel_l2_vs_fp32_esse_after_4_iterations": {"max": max(errs),
+            "rel_l2_vs_fp32_esse_after_5_iterations": {"max": max(errs),
                                                         "mean": float(np.mean(errs))}}
 
 
