@@ -1287,6 +1287,13 @@ struct Recon::Impl {
     void run_srf_att(const float *afacp, int B, const int *bview, float *dst = nullptr)
     {
         launch("rot_x", [&] { run_att(nullptr, B, bview, x.p, rotx.p); });
+        if (esse->canConvolveSource(rotmu.p, afacp, bview)) {
+            // the source formed from the convolution's last pass (no terms stored)
+            launch("esse_conv_source", [&] {
+                esse->convolveSource(rotx.p, rotmu.p, B, 1.0f, afacp, bview, dst ? dst : att.p);
+            });
+            return;
+        }
         launch("esse_conv", [&] { esse->convolve(rotx.p, B, terms.p); });
         launch("esse_source", [&] {
             esse->source(rotx.p, rotmu.p, terms.p, B, 1.0f, afacp, bview, dst ? dst : att.p, true);

@@ -114,6 +114,68 @@ __global__ __launch_bounds__(256) void k_fft_f_inv(const float2 *__restrict__ bu
     }
 }
 
+// F inverse of terms 0 and 1 of line (b, y, m) as one complex line (the
+// two half spectra completed by Hermitian symmetry as its real and
+// imaginary parts), then the effective source of CalcEffScatterSource for
+// two Taylor terms (krnlprj3d.c:903-947, as k_esse_source) on the line's
+// voxels, slice-fastest layout, no collapse:
+//   eff = (E0 - d E1) exp(d fMu) (+ cntr act) * map fDens, clamped at 0,
+//   out = (act primary + eff) afac,
+// with the depth d = S / mu0 from the factors afac = exp(-S).
+template <int P>
+__global__ __launch_bounds__(256) void k_fft_f_inv_src(
+    const float2 *__restrict__ buf, long rows, int nm, int nf, int ny, int Pm, int Wc,
+    long tstride, const float2 *__restrict__ tw, float scale, const float *__restrict__ act,
+    const float *__restrict__ map, const float *__restrict__ afac, const int *__restrict__ bview,
+    float fAtnCnvFac, float fMu, float cntr, float fDensFac, float primary,
+    float *__restrict__ out)
+{
+    using F = FftLine<P>;
+    constexpr int LB = 256 / F::T;
+    extern __shared__ float2 sm[];
+    const int li = threadIdx.x / F::T, tt = threadIdx.x % F::T;
+    const long r = (long)blockIdx.x * LB + li;
+    const bool act_ = r < rows;
+    const long rr = act_ ? r : 0;
+    const long b = rr / ((long)ny * nm), yl = rr - b * ny * nm;   // yl = y nm + m
+    // term t of line (b, y, m): ((b T + t) ny + y) Pm + m, T = 2
+    const long y = yl / nm, m = yl - y * nm;
+    const float2 *s0 = buf + (((b * 2) * ny + y) * Pm + m) * Wc;
+    const float2 *s1 = s0 + tstride;
+    float2 a[F::R];
+#pragma unroll
+    for (int w = 0; w < F::R; ++w) {
+        const int n = tt + F::T * w;
+        const bool up = n > P / 2;
+        const int k = up ? P - n : n;
+        float2 A = s0[k], B = s1[k];
+        if (up) {
+            A.y = -A.y;
+            B.y = -B.y;
+        }
+        a[w] = make_float2(A.x - B.y, A.y + B.x);
+    }
+    fft_line<P, true, true>(a, sm + li * F::LS, tt, tw);
+    if (!act_) return;
+    const long vol = (long)ny * nm * nf;
+    const long vo = (long)bview[b] * vol + yl * nf;   // this line in the view's volumes
+    const long bo = b * vol + yl * nf;                // and in the batch's
+#pragma unroll
+    for (int w = 0; w < F::R; ++w) {
+        const int n = tt + F::T * w;
+        if (n >= nf) continue;
+        const float e0 = a[w].x * scale, e1 = a[w].y * scale;
+        const float f = afac[vo + n], mp = map[vo + n], av = act[bo + n];
+        const float d = (float)(-log((double)f)) * fAtnCnvFac;
+        float eff = e0 - d * e1;
+        eff = (float)((double)eff * exp((double)(d * fMu)));
+        if (cntr > 0) eff = eff + av * cntr;
+        eff = eff * (mp * fDensFac);
+        if (eff < 0) eff = 0.0f;
+        out[bo + n] = (av * primary + eff) * f;
+    }
+}
+
 // Strided lines: element n of line (bi, c), c < W, is at bi*bstride +
 // n*nstride + c.  Input n >= nIn is zero; n < nOut is written to dst (may
 // alias src).  Blocks take LB consecutive lines of the flattened (bi, c)
@@ -575,6 +637,41 @@ void EsseDev::convolve(const float *act, int B, float *terms)
         }
         ESSE_CHECK(hipGetLastError());
     }
+}
+
+bool EsseDev::convolveSource(const float *act, const float *map, int B, float primary,
+                             const float *afac, const int *bview, float *out)
+{
+    if (!canConvolveSource(map, afac, bview)) return false;
+    if (B > maxB_) fail(ETYPE_ILLEGAL_VALUE, "ESSE batch %d exceeds %d", B, maxB_);
+    const int Py = m_.Py, Pm = Pm_, Wc = Wc_, T = 2;
+    const int ny = m_.nyc, nf = m_.nzc, nm = m_.nxc;
+    const long plane = (long)Pm * Wc;
+    fft_f_fwd(act, buf_, (long)B * ny * nm, nm, nf);
+    fft_strided(Pm, false, buf_, buf_, (long)B * ny, plane, Wc, Wc, nm, Pm, twm_);
+    {
+        const dim3 grid((unsigned)((long)B * cdiv((long)Pm * Wc, lines_per_block(Py))));
+        const long khs = (long)Py * plane, ots = (long)ny * plane, obs = (long)T * ny * plane;
+#define L_(PP)                                                                                 \
+    hipLaunchKernelGGL((k_fft_y_conv<PP, 2>), grid, dim3(256), fft_lds(PP), stream_, buf_,     \
+                       khat_, khs, outk_, ots, obs, B, ny, Pm, Wc, twy_)
+        ESSE_P_DISPATCH(Py, L_);
+#undef L_
+    }
+    fft_strided(Pm, true, outk_, outk_, (long)B * T * ny, plane, Wc, Wc, Pm, nm, twm_);
+    const float scale = (float)(1.0 / ((double)m_.Px * Py * m_.Pz));
+    const float fAtnCnvFac = 1.0 / (m_.mu0);
+    const float fMu = m_.mu0 - m_.muavg;
+    const float fDens = fAtnCnvFac / m_.pixsize;
+    const long rows = (long)B * ny * nm;
+    const dim3 grid((unsigned)cdiv(rows, lines_per_block(Pf_)));
+#define L_(PP)                                                                                 \
+    hipLaunchKernelGGL(k_fft_f_inv_src<PP>, grid, dim3(256), fft_lds(PP), stream_, outk_, rows, \
+                       nm, nf, ny, Pm, Wc, (long)ny * plane, twf_, scale, act, map, afac,     \
+                       bview, fAtnCnvFac, fMu, m_.cntrPixFac, fDens, primary, out)
+    ESSE_P_DISPATCH(Pf_, L_);
+#undef L_
+    return true;
 }
 
 void EsseDev::source(const float *act, const float *map, const float *terms, int B,

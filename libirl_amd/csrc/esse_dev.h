@@ -31,6 +31,22 @@ public:
     void source(const float *act, const float *map, const float *terms, int B, float primary,
                 const float *afac, const int *bview, float *out, bool mapByView = false);
 
+    // convolve + source in one pass over the FFT output, where the model and
+    // the buffers allow it: T = 2 terms, no collapse, slice-fastest layout,
+    // maps and factors of every view (indexed by bview) with afac = exp(-S),
+    // S the running half-pixel sum of the map along depth (what the engine's
+    // factors are for its rotated mu).  The two terms' inverse transforms
+    // along the slices run as one complex transform and the source is formed
+    // from them in registers, with the depth taken from the factors
+    // (-ln afac); the terms are not stored.  Returns false, having done
+    // nothing, when the conditions do not hold.
+    bool convolveSource(const float *act, const float *map, int B, float primary,
+                        const float *afac, const int *bview, float *out);
+    bool canConvolveSource(const float *map, const float *afac, const int *bview) const
+    {
+        return m_.enabled && m_.nterms == 2 && m_.c == 1 && zfast_ && afac && bview && map;
+    }
+
     const EsseModel &model() const { return m_; }
     int terms() const { return m_.nterms; }
 
