@@ -337,6 +337,15 @@ __global__ __launch_bounds__(256) void k_conv_l1_rows(const float2 *__restrict__
 // ---------------------------------------------------------------------------
 constexpr int CNN_ROWZ = 64;
 
+// Orders a wave's own LDS writes and reads (a wave's LDS instructions are
+// performed in issue order; this keeps the compiler from reordering them).
+__device__ __forceinline__ void cnn_wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int NCT, int COUT>
 __global__ __launch_bounds__(256) void k_conv_mfma_rows(const uint16_t *__restrict__ in,
                                                         const uint4 *__restrict__ Wg,
@@ -366,13 +375,16 @@ __global__ __launch_bounds__(256) void k_conv_mfma_rows(const uint16_t *__restri
         const int yy = y + dil * (q / 3 - 1), xx = x + dil * (q % 3 - 1);
         const bool rok = valid && yy >= 0 && yy < Y && xx >= 0 && xx < X;
         const uint16_t *src = in + ((long)(rok ? yy : 0) * X + (rok ? xx : 0)) * Z * 32;
-        __syncthreads();   // the previous offset's readers are done
+        // each wave stages and reads only its own segment: the wave's LDS
+        // accesses are performed in order, so only the compiler must keep
+        // them in program order (no block barrier)
+        cnn_wave_sync();   // the previous offset's readers are done
         for (int e = lane; e < SL * 4; e += 64) {
             const int vz = z0 - dil + (e >> 2);
             my[e] = (rok && vz >= 0 && vz < Z) ? *(const uint4 *)(src + (long)vz * 32 + 8 * (e & 3))
                                                : make_uint4(0, 0, 0, 0);
         }
-        __syncthreads();
+        cnn_wave_sync();
 #pragma unroll
         for (int dzi = 0; dzi < 3; ++dzi) {
             const int t = 3 * q + dzi;
