@@ -371,20 +371,35 @@ __global__ __launch_bounds__(256) void k_conv_mfma_rows(const uint16_t *__restri
     for (int c = 0; c < NCT; ++c)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[c][j] = cfloatx4_t{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int q = 0; q < 9; ++q) {
+    // the neighbour row segment of offset q, SL*4 <= 5*64 uint4 per wave:
+    // loaded into registers one offset ahead, so its latency overlaps the
+    // previous offset's products
+    constexpr int PRE = 5;
+    uint4 pre[PRE];
+    auto fetch = [&](int q) {
         const int yy = y + dil * (q / 3 - 1), xx = x + dil * (q % 3 - 1);
         const bool rok = valid && yy >= 0 && yy < Y && xx >= 0 && xx < X;
         const uint16_t *src = in + ((long)(rok ? yy : 0) * X + (rok ? xx : 0)) * Z * 32;
+#pragma unroll
+        for (int k = 0; k < PRE; ++k) {
+            const int e = lane + 64 * k;
+            const int vz = z0 - dil + (e >> 2);
+            pre[k] = (e < SL * 4 && rok && vz >= 0 && vz < Z)
+                         ? *(const uint4 *)(src + (long)vz * 32 + 8 * (e & 3))
+                         : make_uint4(0, 0, 0, 0);
+        }
+    };
+    fetch(0);
+    for (int q = 0; q < 9; ++q) {
         // each wave stages and reads only its own segment: the wave's LDS
         // accesses are performed in order, so only the compiler must keep
         // them in program order (no block barrier)
         cnn_wave_sync();   // the previous offset's readers are done
-        for (int e = lane; e < SL * 4; e += 64) {
-            const int vz = z0 - dil + (e >> 2);
-            my[e] = (rok && vz >= 0 && vz < Z) ? *(const uint4 *)(src + (long)vz * 32 + 8 * (e & 3))
-                                               : make_uint4(0, 0, 0, 0);
-        }
+#pragma unroll
+        for (int k = 0; k < PRE; ++k)
+            if (lane + 64 * k < SL * 4) my[lane + 64 * k] = pre[k];
         cnn_wave_sync();
+        if (q + 1 < 9) fetch(q + 1);
 #pragma unroll
         for (int dzi = 0; dzi < 3; ++dzi) {
             const int t = 3 * q + dzi;
@@ -974,6 +989,8 @@ void ScatterCnn::forward_bf16(const float *in, int Y, int X, int Z, float *out, 
     hipLaunchKernelGGL(k_conv_l1_rows, dim3(cdiv_u(tasks, 4)), blk, 0, s, (const float2 *)in,
                        Wr0_, b_[0], scal ? scal : one_, a0, Y, X, Z);
     const dim3 rgrid(cdiv_u(tasks, 4));
+    // k_conv_mfma_rows prefetches a segment of at most 5 x 64 uint4 per wave
+    static_assert(4 * (CNN_ROWZ + 2 * cnn_dil(CNN_LAYERS - 1)) <= 5 * 64, "segment size");
     auto lds = [](int dil) { return (size_t)4 * (CNN_ROWZ + 2 * dil) * 4 * sizeof(uint4); };
     hipLaunchKernelGGL((k_conv_mfma_rows<2, 32>), rgrid, blk, lds(cnn_dil(1)), s, a0,
                        (const uint4 *)Wbf_[1], b_[1], (void *)a1, Y, X, Z, cnn_dil(1), 1);
