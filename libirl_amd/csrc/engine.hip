@@ -748,17 +748,49 @@ __global__ __launch_bounds__(256) void k_drf_bf16(DrfArgs a, const uint4 *__rest
             const int i0 = bd.x, i1 = bd.y, ks = bd.z, nks = bd.w;
             const int srows = 16 * (NZT - 1) + 32 * nks;   // staged rows from ks
             const int c0 = KX - 1 - i1, nc = DRF_TX + i1 - i0;   // staged columns
-            // stage pairs of consecutive slices as packed bf16 (hi, and lo)
-            for (int e = threadIdx.x; e < (srows / 2) * nc; e += 256) {
-                const int cc = e / (srows / 2), c = c0 + cc;
-                const int q = ks + 2 * (e - cc * (srows / 2));
-                const float f0 = drf_stage_val(a, b, v, row, z0 - CY + q, x0 - CX + c, pad);
-                const float f1 = drf_stage_val(a, b, v, row, z0 - CY + q + 1, x0 - CX + c, pad);
+            // stage pairs of consecutive slices as packed bf16 (hi, and lo);
+            // element e = (column cc, pair qi) advances by 256 per step with
+            // (cc, qi) updated incrementally (no per-element division)
+            const int sp2 = srows / 2, total = sp2 * nc;
+            const int dcc = 256 / sp2, dqi = 256 - dcc * sp2;
+            int cc = (int)threadIdx.x / sp2, qi = (int)threadIdx.x - cc * sp2;
+            // without an axial pad (and in-range bins) a value is the input
+            // times the factor, or zero outside the slices
+            const bool plain = pad <= 0 || a.avg <= 0;
+            const long ibase = (long)b * a.inB + (long)row * a.inRow;
+            const long fbase = (long)v * a.afB + (long)row * a.afRow;
+            for (int e = threadIdx.x; e < total; e += 256) {
+                const int c = c0 + cc, q = ks + 2 * qi;
+                const int gx = x0 - CX + c, g0 = z0 - CY + q;
+                float f0, f1;
+                if (plain) {
+                    f0 = f1 = 0.0f;
+                    if (gx >= 0 && gx < a.NRot) {
+                        const long o = (long)gx * a.xs;
+                        if (g0 >= 0 && g0 < a.Nz) {
+                            f0 = a.in[ibase + o + (long)g0 * a.zs];
+                            if (a.afac) f0 = f0 * a.afac[fbase + o + (long)g0 * a.zs];
+                        }
+                        if (g0 + 1 >= 0 && g0 + 1 < a.Nz) {
+                            f1 = a.in[ibase + o + (long)(g0 + 1) * a.zs];
+                            if (a.afac) f1 = f1 * a.afac[fbase + o + (long)(g0 + 1) * a.zs];
+                        }
+                    }
+                } else {
+                    f0 = drf_stage_val(a, b, v, row, g0, gx, pad);
+                    f1 = drf_stage_val(a, b, v, row, g0 + 1, gx, pad);
+                }
                 const __bf16 h0 = (__bf16)f0, h1 = (__bf16)f1;
                 tileH[(c * CSB + q) / 2] = pack_bf16(h0, h1);
                 if (sp)
                     tileL[(c * CSB + q) / 2] =
                         pack_bf16((__bf16)(f0 - (float)h0), (__bf16)(f1 - (float)h1));
+                cc += dcc;
+                qi += dqi;
+                if (qi >= sp2) {
+                    qi -= sp2;
+                    ++cc;
+                }
             }
             __syncthreads();
             const uint4 *ap = afr + (long)kidx * KX * NKS * 64 + lane;
