@@ -41,4 +41,16 @@ struct DrfFftTables {
 // BCK writes every row's plane.
 void launch_drf_fft(hipStream_t s, bool bck, const DrfArgs &a, int B, const DrfFftTables &t);
 
+// The scatter model's slice and bin passes per plane (both lengths P):
+// forward act [planes][nm][nf] -> buf [planes][P][P/2+1]; inverse of terms
+// 0 and 1 of outk [B][2][ny][P][P/2+1] with the effective source formed from
+// them (d from the attenuation factors, as EsseDev::convolveSource).
+bool esse_fft2_supported(int Pf, int Pm);
+void launch_esse_fwd2d(hipStream_t s, int P, const float *act, float2 *buf, long planes, int nm,
+                       int nf, const float2 *tw);
+void launch_esse_inv2d_src(hipStream_t s, int P, const float2 *outk, int B, int ny, int nm, int nf,
+                           float scale, const float *act, const float *map, const float *afac,
+                           const int *bview, float fAtnCnvFac, float fMu, float cntr,
+                           float fDensFac, float primary, float *out, const float2 *tw);
+
 }  // namespace irlamd

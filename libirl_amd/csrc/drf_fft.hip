@@ -98,6 +98,11 @@ __device__ __forceinline__ float drf_fft_touch(const DrfArgs &a, int b, int v, i
     return r;
 }
 
+template <int P>
+__device__ __forceinline__ void fft2_forward_rest(float2 *s, float2 *sm, int li, int tt,
+                                                  const float2 *__restrict__ tw, bool zIdle,
+                                                  int nbins);
+
 // Forward 2-D transform of plane (b, row) of the DRF input (with axial pad
 // 'pad').  On exit thread (line li, tt), li < P/2 + 1, holds
 // S[kz = li][kx = tt + T w] in s[w].  Every thread of the block calls it.
@@ -151,6 +156,22 @@ __device__ __forceinline__ void drf_fft_forward(const DrfArgs &a, int b, int v, 
     }
     __builtin_amdgcn_sched_barrier(0);   // the next plane's loads after this one's
     pending = drf_fft_touch(a, b, v, next);
+    fft2_forward_rest<P>(s, sm, li, tt, tw, zIdle, a.NRot);
+}
+
+// The forward 2-D transform from the loaded slice lines on: thread (line
+// li, tt) holds slices tt + T w of bins 2 li (real part) and 2 li + 1
+// (imaginary part) in s[w], zero where there is no such bin or slice (zIdle:
+// the wave has no bin pair).  On exit it holds S[kz = li][kx = tt + T w].
+template <int P>
+__device__ __forceinline__ void fft2_forward_rest(float2 *s, float2 *sm, int li, int tt,
+                                                  const float2 *__restrict__ tw, bool zIdle,
+                                                  int nbins)
+{
+    using D = DrfFftShape<P>;
+    float2 *slot = sm + li * D::LS;
+    const int c0 = 2 * li, c1 = c0 + 1;
+    const bool zact = c0 < nbins;
     if (!zIdle) fft_line<P, false, true>(s, slot, tt, tw);
     // Between phases the lane indices pass through an empty asm, so each
     // phase computes its own LDS addresses instead of the compiler keeping
@@ -180,7 +201,7 @@ __device__ __forceinline__ void drf_fft_forward(const DrfArgs &a, int b, int v, 
             const int k = tt + D::T * w;
             if (k <= P / 2) {
                 sm[k * D::LS + fft_skew(c0)] = A[w];
-                if (c1 < a.NRot) sm[k * D::LS + fft_skew(c1)] = Bq[w];
+                if (c1 < nbins) sm[k * D::LS + fft_skew(c1)] = Bq[w];
             }
         }
     }
@@ -193,7 +214,7 @@ __device__ __forceinline__ void drf_fft_forward(const DrfArgs &a, int b, int v, 
     for (int w = 0; w < D::R; ++w) {
         const int x = tt + D::T * w;
         const float2 val = slot[fft_skew(x)];   // inside the slot for every x < P
-        s[w] = (xact && x < a.NRot) ? val : make_float2(0.0f, 0.0f);
+        s[w] = (xact && x < nbins) ? val : make_float2(0.0f, 0.0f);
     }
     fft_line<P, false, true>(s, slot, tt, tw);
 }
@@ -202,8 +223,20 @@ __device__ __forceinline__ void drf_fft_forward(const DrfArgs &a, int b, int v, 
 // it (already scaled by 1/P^2).  On exit thread (line li, tt) holds slice
 // n = tt + T w of bin 2 li in g[w].x and of bin 2 li + 1 in g[w].y.
 template <int P>
+__device__ __forceinline__ void fft2_inverse(int nbins, float2 *g, float2 *sm, int li, int tt,
+                                             const float2 *__restrict__ tw);
+
+template <int P>
 __device__ __forceinline__ void drf_fft_inverse(const DrfArgs &a, float2 *g, float2 *sm, int li,
                                                 int tt, const float2 *__restrict__ tw)
+{
+    fft2_inverse<P>(a.NRot, g, sm, li, tt, tw);
+}
+
+// The inverse transform for planes of nbins bins (see drf_fft_inverse).
+template <int P>
+__device__ __forceinline__ void fft2_inverse(int nbins, float2 *g, float2 *sm, int li, int tt,
+                                             const float2 *__restrict__ tw)
 {
     using D = DrfFftShape<P>;
     float2 *slot = sm + li * D::LS;
@@ -214,16 +247,16 @@ __device__ __forceinline__ void drf_fft_inverse(const DrfArgs &a, float2 *g, flo
 #pragma unroll
     for (int w = 0; w < D::R; ++w) {
         const int x = tt + D::T * w;
-        if (x < a.NRot) slot[fft_skew(x)] = g[w];
+        if (x < nbins) slot[fft_skew(x)] = g[w];
     }
     __syncthreads();
     asm volatile("" : "+v"(tt), "+v"(li));
     slot = sm + li * D::LS;
     const int c0 = 2 * li, c1 = c0 + 1;
-    const bool has0 = c0 < a.NRot, has1 = c1 < a.NRot;
+    const bool has0 = c0 < nbins, has1 = c1 < nbins;
     // waves with no bin pair skip the slice transform; every wave meets the
     // block barrier below (the line transform synchronises within a wave)
-    const bool idle = drf_fft_wave_first_line<P>() >= (a.NRot + 1) / 2;
+    const bool idle = drf_fft_wave_first_line<P>() >= (nbins + 1) / 2;
     if (idle) {
 #pragma unroll
         for (int w = 0; w < D::R; ++w) g[w] = make_float2(0.0f, 0.0f);
@@ -586,6 +619,192 @@ void launch_drf_fft(hipStream_t s, bool bck, const DrfArgs &a, int B, const DrfF
     case 160: launch_p<160>(s, bck, a, B, t); break;
     default: fail(ETYPE_ILLEGAL_VALUE, "FFT DRF: unsupported length %d", t.P);
     }
+}
+
+}  // namespace irlamd
+
+namespace irlamd {
+
+// ---------------------------------------------------------------------------
+// Scatter model (esse.hip): the slice and bin passes of its 3-D FFT per
+// (batch item, depth) plane in LDS with the machinery above, where both
+// in-plane lengths are P.  The depth pass between them stays in esse.hip.
+// ---------------------------------------------------------------------------
+
+// Forward: act [planes][nm][nf] real (slices fastest) -> buf
+// [planes][P][P/2+1] (bin frequency, slice frequency).  A block takes rpb
+// consecutive planes and pulls the next one toward its L2 while transforming.
+template <int P>
+__global__ __launch_bounds__(DrfFftShape<P>::NT) void k_esse_fwd2d(
+    const float *__restrict__ act, float2 *__restrict__ buf, long planes, int nm, int nf, int rpb,
+    const float2 *__restrict__ twg)
+{
+    using D = DrfFftShape<P>;
+    extern __shared__ float2 sm[];
+    const int li = threadIdx.x / D::T, tt = threadIdx.x % D::T;
+    float2 *tw = sm + D::NL * D::LS;
+    for (int i = threadIdx.x; i < P; i += D::NT) tw[i] = twg[i];
+    __syncthreads();
+    const long p0 = (long)blockIdx.x * rpb, p1 = min(p0 + rpb, planes);
+    const long pl = (long)nm * nf;
+    const bool zIdle = drf_fft_wave_first_line<P>() >= (nm + 1) / 2;
+    float keep = 0.0f;
+    for (long p = p0; p < p1; ++p) {
+        int lq = li, tq = tt;
+        asm volatile("" : "+v"(lq), "+v"(tq));
+        float2 s[D::R];
+        if (zIdle) {
+#pragma unroll
+            for (int w = 0; w < D::R; ++w) s[w] = make_float2(0.0f, 0.0f);
+        } else {
+            const int c0 = 2 * lq, c1 = c0 + 1;
+            const bool zact = c0 < nm, has1 = c1 < nm;
+            const float *i0 = act + p * pl + (long)min(c0, nm - 1) * nf;
+            const float *i1 = act + p * pl + (long)min(c1, nm - 1) * nf;
+#pragma unroll
+            for (int w = 0; w < D::R; ++w) {
+                const int n0 = D::T * w;
+                if (n0 >= nf) {
+                    s[w] = make_float2(0.0f, 0.0f);
+                    continue;
+                }
+                const bool whole = n0 + D::T <= nf;
+                const int n = tq + n0;
+                const int nc = whole ? n : min(n, nf - 1);
+                const float re = i0[nc], im = i1[nc];
+                const bool in = whole || n < nf;
+                s[w] = make_float2((zact && in) ? re : 0.0f, (has1 && in) ? im : 0.0f);
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);   // the next plane's loads after this one's
+        float pending = 0.0f;
+        if (p + 1 < p1)
+            for (long e = (long)threadIdx.x * 32; e < pl; e += (long)D::NT * 32)
+                pending += act[(p + 1) * pl + e];
+        fft2_forward_rest<P>(s, sm, lq, tq, tw, zIdle, nm);
+        if (lq < D::WC) {
+            float2 *o = buf + p * (long)P * D::WC + lq;
+#pragma unroll
+            for (int w = 0; w < D::R; ++w) o[(long)(tq + D::T * w) * D::WC] = s[w];
+        }
+        keep += pending;
+        __syncthreads();   // the next plane rewrites the line slots
+    }
+    if (planes < 0) buf[0].x = keep;   // never: keeps the touch loads
+}
+
+// Inverse of terms 0 and 1 of every (batch, depth) plane of outk
+// [B][2][ny][P][P/2+1] and the effective source on the plane's voxels (the
+// epilogue of k_fft_f_inv_src: eff = (E0 - d E1) exp(d fMu) (+ cntr act) *
+// map fDens, clamped at 0, out = (act primary + eff) afac, d = -ln(afac) /
+// mu0), out [B][ny][nm][nf].  act, map and afac as in k_fft_f_inv_src.
+template <int P>
+__global__ __launch_bounds__(DrfFftShape<P>::NT) void k_esse_inv2d_src(
+    const float2 *__restrict__ outk, int B, int ny, int nm, int nf, int rpb, float scale,
+    const float *__restrict__ act, const float *__restrict__ map, const float *__restrict__ afac,
+    const int *__restrict__ bview, float fAtnCnvFac, float fMu, float cntr, float fDensFac,
+    float primary, float *__restrict__ out, const float2 *__restrict__ twg)
+{
+    using D = DrfFftShape<P>;
+    extern __shared__ float2 sm[];
+    const int li = threadIdx.x / D::T, tt = threadIdx.x % D::T;
+    float2 *tw = sm + D::NL * D::LS;
+    for (int i = threadIdx.x; i < P; i += D::NT) tw[i] = twg[i];
+    __syncthreads();
+    const long planes = (long)B * ny;
+    const long p0 = (long)blockIdx.x * rpb, p1 = min(p0 + rpb, planes);
+    const long pl = (long)nm * nf, vol = (long)ny * pl;
+    const long spec = (long)P * D::WC;   // one term's plane
+    for (long p = p0; p < p1; ++p) {
+        const int b = (int)(p / ny), y = (int)(p % ny);
+        float2 e0[D::R], g[D::R];
+#pragma unroll 1
+        for (int t = 0; t < 2; ++t) {
+            int lq = li, tq = tt;
+            asm volatile("" : "+v"(lq), "+v"(tq));
+            const float2 *src = outk + (((long)b * 2 + t) * ny + y) * spec + lq;
+            if (lq < D::WC) {
+#pragma unroll
+                for (int w = 0; w < D::R; ++w) g[w] = src[(long)(tq + D::T * w) * D::WC];
+            } else {
+#pragma unroll
+                for (int w = 0; w < D::R; ++w) g[w] = make_float2(0.0f, 0.0f);
+            }
+            fft2_inverse<P>(nm, g, sm, lq, tq, tw);
+            if (t == 0) {
+#pragma unroll
+                for (int w = 0; w < D::R; ++w) e0[w] = g[w];
+            }
+        }
+        // voxels (b, y, bin 2 li (.x) and 2 li + 1 (.y), slice tt + T w)
+        const int c0 = 2 * li;
+        const long vo = (long)bview[b] * vol + (long)y * pl, bo = p * pl;
+#pragma unroll
+        for (int w = 0; w < D::R; ++w) {
+            const int n = tt + D::T * w;
+            if (n >= nf) continue;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int c = c0 + h;
+                if (c >= nm) continue;
+                const long o = (long)c * nf + n;
+                const float e0v = (h ? e0[w].y : e0[w].x) * scale;
+                const float e1v = (h ? g[w].y : g[w].x) * scale;
+                const float f = afac[vo + o], mp = map[vo + o], av = act[bo + o];
+                const float d = (float)(-log((double)f)) * fAtnCnvFac;
+                float eff = e0v - d * e1v;
+                eff = (float)((double)eff * exp((double)(d * fMu)));
+                if (cntr > 0) eff = eff + av * cntr;
+                eff = eff * (mp * fDensFac);
+                if (eff < 0) eff = 0.0f;
+                out[bo + o] = (av * primary + eff) * f;
+            }
+        }
+        __syncthreads();   // the next plane rewrites the line slots
+    }
+}
+
+bool esse_fft2_supported(int Pf, int Pm) { return Pf == Pm && (Pf == 128 || Pf == 160); }
+
+void launch_esse_fwd2d(hipStream_t s, int P, const float *act, float2 *buf, long planes, int nm,
+                       int nf, const float2 *tw)
+{
+    const int rpb = 8;
+    const long nb = (planes + rpb - 1) / rpb;
+    if (nb <= 0 || nb > INT_MAX || nm > P || nf > P) fail(ETYPE_ILLEGAL_VALUE, "bad ESSE 2-D FFT size");
+#define EFWD(PP)                                                                                \
+    hipLaunchKernelGGL(k_esse_fwd2d<PP>, dim3((unsigned)nb), dim3(DrfFftShape<PP>::NT),         \
+                       DrfFftShape<PP>::LDS, s, act, buf, planes, nm, nf, rpb, tw)
+    if (P == 128)
+        EFWD(128);
+    else if (P == 160)
+        EFWD(160);
+    else
+        fail(ETYPE_ILLEGAL_VALUE, "ESSE 2-D FFT length %d", P);
+#undef EFWD
+    DFFT_CHECK(hipGetLastError());
+}
+
+void launch_esse_inv2d_src(hipStream_t s, int P, const float2 *outk, int B, int ny, int nm, int nf,
+                           float scale, const float *act, const float *map, const float *afac,
+                           const int *bview, float fAtnCnvFac, float fMu, float cntr,
+                           float fDensFac, float primary, float *out, const float2 *tw)
+{
+    const int rpb = 8;
+    const long nb = ((long)B * ny + rpb - 1) / rpb;
+    if (nb <= 0 || nb > INT_MAX || nm > P || nf > P) fail(ETYPE_ILLEGAL_VALUE, "bad ESSE 2-D FFT size");
+#define EINV(PP)                                                                                \
+    hipLaunchKernelGGL(k_esse_inv2d_src<PP>, dim3((unsigned)nb), dim3(DrfFftShape<PP>::NT),     \
+                       DrfFftShape<PP>::LDS, s, outk, B, ny, nm, nf, rpb, scale, act, map, afac, \
+                       bview, fAtnCnvFac, fMu, cntr, fDensFac, primary, out, tw)
+    if (P == 128)
+        EINV(128);
+    else if (P == 160)
+        EINV(160);
+    else
+        fail(ETYPE_ILLEGAL_VALUE, "ESSE 2-D FFT length %d", P);
+#undef EINV
+    DFFT_CHECK(hipGetLastError());
 }
 
 }  // namespace irlamd

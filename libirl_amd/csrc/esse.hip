@@ -27,6 +27,7 @@
 #include <memory>
 #include <vector>
 
+#include "drf_fft.h"
 #include "esse_dev.h"
 #include "fft_dev.h"
 
@@ -586,6 +587,19 @@ EsseDev::~EsseDev()
         if (p) (void)hipFree(p);
 }
 
+// Slice and bin forward passes of planes [planes][nm][nf] into buf_: one
+// 2-D transform per plane in LDS where both lengths allow it (drf_fft.hip),
+// else the two line passes.
+void EsseDev::forward2(const float *in, long planes, int nm, int nf)
+{
+    if (esse_fft2_supported(Pf_, Pm_)) {
+        launch_esse_fwd2d(stream_, Pf_, in, buf_, planes, nm, nf, twf_);
+        return;
+    }
+    fft_f_fwd(in, buf_, planes * nm, nm, nf);
+    fft_strided(Pm_, false, buf_, buf_, planes, (long)Pm_ * Wc_, Wc_, Wc_, nm, Pm_, twm_);
+}
+
 void EsseDev::convolve(const float *act, int B, float *terms)
 {
     if (!m_.enabled) return;
@@ -602,8 +616,7 @@ void EsseDev::convolve(const float *act, int B, float *terms)
         src = coll_;
     }
     const long plane = (long)Pm * Wc;
-    fft_f_fwd(src, buf_, (long)B * ny * nm, nm, nf);
-    fft_strided(Pm, false, buf_, buf_, (long)B * ny, plane, Wc, Wc, nm, Pm, twm_);
+    forward2(src, B * ny, nm, nf);
     {
         const dim3 grid((unsigned)((long)B * cdiv((long)Pm * Wc, lines_per_block(Py))));
         const long khs = (long)Py * plane, ots = (long)ny * plane, obs = (long)T * ny * plane;
@@ -647,8 +660,7 @@ bool EsseDev::convolveSource(const float *act, const float *map, int B, float pr
     const int Py = m_.Py, Pm = Pm_, Wc = Wc_, T = 2;
     const int ny = m_.nyc, nf = m_.nzc, nm = m_.nxc;
     const long plane = (long)Pm * Wc;
-    fft_f_fwd(act, buf_, (long)B * ny * nm, nm, nf);
-    fft_strided(Pm, false, buf_, buf_, (long)B * ny, plane, Wc, Wc, nm, Pm, twm_);
+    forward2(act, B * ny, nm, nf);
     {
         const dim3 grid((unsigned)((long)B * cdiv((long)Pm * Wc, lines_per_block(Py))));
         const long khs = (long)Py * plane, ots = (long)ny * plane, obs = (long)T * ny * plane;
@@ -658,11 +670,17 @@ bool EsseDev::convolveSource(const float *act, const float *map, int B, float pr
         ESSE_P_DISPATCH(Py, L_);
 #undef L_
     }
-    fft_strided(Pm, true, outk_, outk_, (long)B * T * ny, plane, Wc, Wc, Pm, nm, twm_);
     const float scale = (float)(1.0 / ((double)m_.Px * Py * m_.Pz));
     const float fAtnCnvFac = 1.0 / (m_.mu0);
     const float fMu = m_.mu0 - m_.muavg;
     const float fDens = fAtnCnvFac / m_.pixsize;
+    if (esse_fft2_supported(Pf_, Pm_)) {
+        // bin and slice inverses of both terms per plane, source in the epilogue
+        launch_esse_inv2d_src(stream_, Pf_, outk_, B, ny, nm, nf, scale, act, map, afac, bview,
+                              fAtnCnvFac, fMu, m_.cntrPixFac, fDens, primary, out, twf_);
+        return true;
+    }
+    fft_strided(Pm, true, outk_, outk_, (long)B * T * ny, plane, Wc, Wc, Pm, nm, twm_);
     const long rows = (long)B * ny * nm;
     const dim3 grid((unsigned)cdiv(rows, lines_per_block(Pf_)));
 #define L_(PP)                                                                                 \

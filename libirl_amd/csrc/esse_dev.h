@@ -51,6 +51,7 @@ public:
     int terms() const { return m_.nterms; }
 
 private:
+    void forward2(const float *in, long planes, int nm, int nf);
     void fft_f_fwd(const float *in, float2 *out, long rows, int nm, int nf);
     void fft_f_inv(const float2 *in, float *out, long rows, int nm, int nf, float scale);
     void fft_strided(int P, bool inv, const float2 *src, float2 *dst, long nbases, long bstride,
