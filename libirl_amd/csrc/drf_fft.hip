@@ -697,7 +697,8 @@ __global__ __launch_bounds__(DrfFftShape<P>::NT) void k_esse_fwd2d(
 // [B][2][ny][P][P/2+1] and the effective source on the plane's voxels (the
 // epilogue of k_fft_f_inv_src: eff = (E0 - d E1) exp(d fMu) (+ cntr act) *
 // map fDens, clamped at 0, out = (act primary + eff) afac, d = -ln(afac) /
-// mu0), out [B][ny][nm][nf].  act, map and afac as in k_fft_f_inv_src.
+// mu0, single-precision logarithm and exponential), out [B][ny][nm][nf].
+// act, map and afac as in k_fft_f_inv_src.
 template <int P>
 __global__ __launch_bounds__(DrfFftShape<P>::NT) void k_esse_inv2d_src(
     const float2 *__restrict__ outk, int B, int ny, int nm, int nf, int rpb, float scale,
@@ -751,9 +752,9 @@ __global__ __launch_bounds__(DrfFftShape<P>::NT) void k_esse_inv2d_src(
                 const float e0v = (h ? e0[w].y : e0[w].x) * scale;
                 const float e1v = (h ? g[w].y : g[w].x) * scale;
                 const float f = afac[vo + o], mp = map[vo + o], av = act[bo + o];
-                const float d = (float)(-log((double)f)) * fAtnCnvFac;
+                const float d = -logf(f) * fAtnCnvFac;
                 float eff = e0v - d * e1v;
-                eff = (float)((double)eff * exp((double)(d * fMu)));
+                eff = eff * expf(d * fMu);
                 if (cntr > 0) eff = eff + av * cntr;
                 eff = eff * (mp * fDensFac);
                 if (eff < 0) eff = 0.0f;

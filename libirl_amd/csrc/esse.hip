@@ -122,7 +122,9 @@ __global__ __launch_bounds__(256) void k_fft_f_inv(const float2 *__restrict__ bu
 // voxels, slice-fastest layout, no collapse:
 //   eff = (E0 - d E1) exp(d fMu) (+ cntr act) * map fDens, clamped at 0,
 //   out = (act primary + eff) afac,
-// with the depth d = S / mu0 from the factors afac = exp(-S).
+// with the depth d = S / mu0 from the factors afac = exp(-S), logarithm and
+// exponential in single precision (k_esse_source takes the exponential in
+// double as the reference; the two differ by rounding).
 template <int P>
 __global__ __launch_bounds__(256) void k_fft_f_inv_src(
     const float2 *__restrict__ buf, long rows, int nm, int nf, int ny, int Pm, int Wc,
@@ -167,9 +169,9 @@ __global__ __launch_bounds__(256) void k_fft_f_inv_src(
         if (n >= nf) continue;
         const float e0 = a[w].x * scale, e1 = a[w].y * scale;
         const float f = afac[vo + n], mp = map[vo + n], av = act[bo + n];
-        const float d = (float)(-log((double)f)) * fAtnCnvFac;
+        const float d = -logf(f) * fAtnCnvFac;
         float eff = e0 - d * e1;
-        eff = (float)((double)eff * exp((double)(d * fMu)));
+        eff = eff * expf(d * fMu);
         if (cntr > 0) eff = eff + av * cntr;
         eff = eff * (mp * fDensFac);
         if (eff < 0) eff = 0.0f;
