@@ -767,10 +767,17 @@ __global__ __launch_bounds__(DrfFftShape<P>::NT) void k_esse_inv2d_src(
 
 bool esse_fft2_supported(int Pf, int Pm) { return Pf == Pm && (Pf == 128 || Pf == 160); }
 
+// planes per block: enough that the blocks fit one pass over the 256 CUs
+// (one block per CU), at most 8
+static int esse_planes_per_block(long planes)
+{
+    return (int)std::max(1L, std::min(8L, (planes + 255) / 256));
+}
+
 void launch_esse_fwd2d(hipStream_t s, int P, const float *act, float2 *buf, long planes, int nm,
                        int nf, const float2 *tw)
 {
-    const int rpb = 8;
+    const int rpb = esse_planes_per_block(planes);
     const long nb = (planes + rpb - 1) / rpb;
     if (nb <= 0 || nb > INT_MAX || nm > P || nf > P) fail(ETYPE_ILLEGAL_VALUE, "bad ESSE 2-D FFT size");
 #define EFWD(PP)                                                                                \
@@ -791,7 +798,7 @@ void launch_esse_inv2d_src(hipStream_t s, int P, const float2 *outk, int B, int 
                            const int *bview, float fAtnCnvFac, float fMu, float cntr,
                            float fDensFac, float primary, float *out, const float2 *tw)
 {
-    const int rpb = 8;
+    const int rpb = esse_planes_per_block((long)B * ny);
     const long nb = ((long)B * ny + rpb - 1) / rpb;
     if (nb <= 0 || nb > INT_MAX || nm > P || nf > P) fail(ETYPE_ILLEGAL_VALUE, "bad ESSE 2-D FFT size");
 #define EINV(PP)                                                                                \

@@ -592,9 +592,16 @@ EsseDev::~EsseDev()
 // Slice and bin forward passes of planes [planes][nm][nf] into buf_: one
 // 2-D transform per plane in LDS where both lengths allow it (drf_fft.hip),
 // else the two line passes.
+// (the plane kernels take one CU per block: for fewer than 1024 planes, as
+// a single view's check, the line passes keep more of the chip busy)
+static bool plane_kernels(int Pf, int Pm, long planes)
+{
+    return esse_fft2_supported(Pf, Pm) && planes >= 1024;
+}
+
 void EsseDev::forward2(const float *in, long planes, int nm, int nf)
 {
-    if (esse_fft2_supported(Pf_, Pm_)) {
+    if (plane_kernels(Pf_, Pm_, planes)) {
         launch_esse_fwd2d(stream_, Pf_, in, buf_, planes, nm, nf, twf_);
         return;
     }
@@ -676,7 +683,7 @@ bool EsseDev::convolveSource(const float *act, const float *map, int B, float pr
     const float fAtnCnvFac = 1.0 / (m_.mu0);
     const float fMu = m_.mu0 - m_.muavg;
     const float fDens = fAtnCnvFac / m_.pixsize;
-    if (esse_fft2_supported(Pf_, Pm_)) {
+    if (plane_kernels(Pf_, Pm_, (long)B * ny)) {
         // bin and slice inverses of both terms per plane, source in the epilogue
         launch_esse_inv2d_src(stream_, Pf_, outk_, B, ny, nm, nf, scale, act, map, afac, bview,
                               fAtnCnvFac, fMu, m_.cntrPixFac, fDens, primary, out, twf_);
