@@ -80,6 +80,16 @@ int irlamd_set_autotune(int bOn);
  * BF16 slice tiles per wave, of the projector and the backprojector. */
 int irlamd_recon_tiles(irlamd_recon *r, int *piPrj, int *piBck);
 
+/* Scatter model of reconstructions created afterwards: 1 (the default) =
+ * where the grid allows (equal in-plane FFT lengths of 128 or 160, two
+ * Taylor terms, no collapse, at least 1024 planes per batch), the slice and
+ * bin passes run as 2-D transforms per plane and the effective source is
+ * formed in the last pass (depth from the attenuation factors, single-
+ * precision logarithm and exponential); 0 = the line passes and the separate
+ * source kernel (running depth sum, FP64 exponential, as krnlprj3d.c:903-947).
+ * The two agree to rounding.  Returns the previous setting. */
+int irlamd_set_esse_fused(int bOn);
+
 /* FP32 DRF convolution algorithm of reconstructions created afterwards:
  * 0 = the reference's choice (the default): the 2-D FFT kernel for tables
  * flagged bFFTconvolve (convolve_ypad.c:62-94, conv2dPsfFFT.c:53-118), the

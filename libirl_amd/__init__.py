@@ -227,6 +227,7 @@ def load():
     lib.irlamd_set_autotune.argtypes = [ctypes.c_int]
     lib.irlamd_recon_tiles.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     lib.irlamd_set_drf_fft.argtypes = [ctypes.c_int]
+    lib.irlamd_set_esse_fused.argtypes = [ctypes.c_int]
     lib.irlamd_recon_drf_algorithms.argtypes = [vp] + [ctypes.POINTER(ctypes.c_int)] * 3
     lib.irlamd_batch_create.restype = vp
     lib.irlamd_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int]
@@ -636,6 +637,13 @@ def set_graphs(on):
 def set_autotune(on):
     """DRF tile autotuning at setup for reconstructions created afterwards."""
     return load().irlamd_set_autotune(1 if on else 0)
+
+
+def set_esse_fused(on):
+    """Scatter model's plane transforms and fused source (the default) or the
+    line passes and separate source kernel, for reconstructions created
+    afterwards.  Returns the previous setting."""
+    return bool(load().irlamd_set_esse_fused(1 if on else 0))
 
 
 _DRF_FFT_MODES = {"reference": 0, "fft": 1, "auto": 2, "spatial": -1}

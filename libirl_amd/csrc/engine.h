@@ -122,6 +122,10 @@ public:
         // sizes fit), 2 the faster of the two per direction, timed at
         // precompute
         int drfFft = 0;
+        // scatter model: 2-D plane transforms and the source formed in the
+        // last FFT pass where the sizes allow (else the line passes and the
+        // separate source kernel with the reference's FP64 exponential)
+        bool esseFused = true;
     };
 
     struct CnnStatus {

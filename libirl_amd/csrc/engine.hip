@@ -1648,6 +1648,7 @@ Recon::Recon(const Config &cfg) : impl_(new Impl), cfg_(cfg)
             if (cfg.esse->nx != g.NRot || cfg.esse->ny != g.NRot || cfg.esse->nz != g.Nz)
                 fail(ETYPE_ILLEGAL_VALUE, "scatter model prepared for another volume size");
             I.esse.reset(new EsseDev(*cfg.esse, Vs, I.stream, true));
+            I.esse->setFused(cfg.esseFused);
             I.terms.alloc(I.vol * Vs * cfg.esse->nterms);
             const EsseModel &em = *cfg.esse;
             I.cst.configured = cfg.cnn != nullptr || cfg.cnnExact;

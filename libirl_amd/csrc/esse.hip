@@ -601,7 +601,7 @@ static bool plane_kernels(int Pf, int Pm, long planes)
 
 void EsseDev::forward2(const float *in, long planes, int nm, int nf)
 {
-    if (plane_kernels(Pf_, Pm_, planes)) {
+    if (fused_ && plane_kernels(Pf_, Pm_, planes)) {
         launch_esse_fwd2d(stream_, Pf_, in, buf_, planes, nm, nf, twf_);
         return;
     }

@@ -44,8 +44,11 @@ public:
                         const float *afac, const int *bview, float *out);
     bool canConvolveSource(const float *map, const float *afac, const int *bview) const
     {
-        return m_.enabled && m_.nterms == 2 && m_.c == 1 && zfast_ && afac && bview && map;
+        return fused_ && m_.enabled && m_.nterms == 2 && m_.c == 1 && zfast_ && afac && bview &&
+               map;
     }
+    // off: the line passes everywhere and no fused source
+    void setFused(bool on) { fused_ = on; }
 
     const EsseModel &model() const { return m_; }
     int terms() const { return m_.nterms; }
@@ -69,6 +72,7 @@ private:
     float *coll_ = nullptr;    // collapsed activity [B][nyc][nzc][nxc]
     float *res_ = nullptr;     // collapsed terms [B][T][nyc][nzc][nxc] (c > 1)
     float2 *twf_ = nullptr, *twy_ = nullptr, *twm_ = nullptr;
+    bool fused_ = true;
 };
 
 }  // namespace irlamd

@@ -146,3 +146,37 @@ def test_project_with_srf_matches_oracle():
         print("model %d rel L2 %.3g" % (model, e))
         assert e < 2e-5
     L.free_prj()
+
+
+def _esse_recon(pb, f, fused):
+    import libirl_amd as L
+    L.set_adv_options(0, 0)
+    prev = L.set_esse_fused(fused)
+    try:
+        parms = pb.parms()
+        parms.iNumSrfIterations = 1
+        r = L.Recon(parms, pb.options(), pb.views, pb.drf_file, f, None, pb.mu, pb.prj, pb.x0)
+    finally:
+        L.set_esse_fused(prev)
+    for s in range(r.num_subsets):
+        r.subiteration(s)
+    x = r.estimate()
+    r.close()
+    return x
+
+
+# In-plane FFT lengths 128 (96 + 32) and 160 (128 + 32) with at least 1024
+# depth planes per batch (12 x 96, 8 x 128): the 2-D plane transforms and the
+# source formed in their last pass, against the line passes and the separate
+# source kernel.
+@pytest.mark.parametrize("n,nviews,per", [(96, 24, 12), (128, 16, 8)])
+def test_esse_plane_transforms_match_line_passes(n, nviews, per):
+    print("ESSE plane transforms vs line passes N=Nz=%d views/subset %d" % (n, per), flush=True)
+    pb = Problem(n=n, nz=n, nviews=nviews, per=per, iters=1, drf=False, noise=False,
+                 pix=0.5 if n <= 96 else 0.4)
+    f, k, km = _krnl_file(33, 17, pb.pix)
+    a = _esse_recon(pb, f, False)
+    b = _esse_recon(pb, f, True)
+    e = rel_l2(b, a)
+    print("rel L2 plane/fused vs line passes %.3g" % e)
+    assert e <= 1e-5
