@@ -72,43 +72,69 @@ def build_workload(tmpdir, seed=2, name="c2", cfg=None):
                 act=act, offs=offs, psfs=psfs, start=start, cfg=cfg, krnl_file=krnl_file)
 
 
+def run_parms(w, subiterations):
+    """The workload's parameters with enough iterations for 'subiterations'
+    sub-iterations, the scatter modelled in all of them where the workload
+    models it in all of its own (each sub-iteration does the work of the
+    workload's own iterations)."""
+    parms = type(w["parms"]).from_buffer_copy(w["parms"])
+    S0 = max(1, parms.NumViews // parms.NumAngPerSubset)
+    iters = max(parms.NumIterations, -(-subiterations // S0))
+    if parms.iNumSrfIterations >= parms.NumIterations:
+        parms.iNumSrfIterations = iters
+    parms.NumIterations = iters
+    return parms
+
+
 def time_subiterations(w, steps, warmup, precision="fp32", cnn=None, check_every=0,
-                       drf="spatial"):
+                       drf="spatial", graphs=True, stage_steps=None):
     """Device-resident reconstruction of the workload: 'warmup' untimed
-    sub-iterations, then exactly 'steps' timed ones with per-kernel HIP events
-    on the engine's stream.  With 'cnn' the scatter network replaces the
-    per-view ESSE convolutions (checked against ESSE every 'check_every'
-    subsets).  'drf' is the FP32 DRF algorithm mode (set_drf_fft: "spatial"
-    the reference's spatial convolution, "auto" the faster of it and the 2-D
-    FFT, timed at setup).  Returns (recon, seconds of the timed steps,
-    {kernel: (ms_total, launches)}, estimate after the timed steps)."""
+    sub-iterations, then exactly 'steps' timed ones, replayed from the
+    library's captured sub-iteration graphs when 'graphs' (each subset's graph
+    is captured at its second occurrence, so a warmup of two passes over the
+    subsets keeps capture out of the timed steps), with no kernel timing.
+    Then 'stage_steps' (default 'steps') more sub-iterations run eagerly with
+    per-kernel HIP events on the engine's stream for the per-kernel times.
+    With 'cnn' the scatter network replaces the per-view ESSE convolutions
+    (checked against ESSE every 'check_every' subsets).  'drf' is the FP32
+    DRF algorithm mode (set_drf_fft: "spatial" the reference's spatial
+    convolution, "auto" the faster of it and the 2-D FFT, timed at setup).
+    Returns (recon, seconds of the timed steps, {kernel: (ms_total,
+    launches)} of the event pass, estimate after warmup + steps)."""
     import libirl_amd as L
     L.set_adv_options(0, 0)
+    n = steps if stage_steps is None else stage_steps
+    parms = run_parms(w, warmup + steps + n)
     prev = L.set_precision(precision)
     prev_drf = L.set_drf_fft(drf)
+    prev_g = L.set_graphs(graphs)
     L.set_scatter_cnn(cnn, check_every=check_every, max_nrmse=0.01)
     try:
-        rec = L.Recon(w["parms"], w["opts"], w["views"], w["drf_file"], w["krnl_file"], None,
+        rec = L.Recon(parms, w["opts"], w["views"], w["drf_file"], w["krnl_file"], None,
                       w["mu"], w["prj"], None)
     finally:
         L.set_precision(prev)
         L.set_drf_fft(prev_drf)
+        L.set_graphs(prev_g)
         L.set_scatter_cnn(None)
     S = rec.num_subsets
     for k in range(warmup):
         rec.subiteration(k % S)
-    rec.synchronize()
-    rec.reset_timing()
-    rec.set_timing(True)
     rec.synchronize()
     t0 = time.perf_counter()
     for k in range(steps):
         rec.subiteration((warmup + k) % S)
     rec.synchronize()
     dt = time.perf_counter() - t0
+    est = rec.estimate()
+    rec.reset_timing()
+    rec.set_timing(True)
+    for k in range(n):
+        rec.subiteration((warmup + steps + k) % S)
+    rec.synchronize()
     stages = rec.stage_times()
     rec.set_timing(False)
-    return rec, dt, stages, rec.estimate()
+    return rec, dt, stages, est
 
 
 def drf_flops_per_subset(w, recon):
@@ -175,7 +201,7 @@ def fft_drf_bytes(w, rec, stage):
     return vs * plane + (vs * n * plane if atn else 0) + vs * n * plane
 
 
-def drf_roofline(w, rec, stages, steps, warmup):
+def drf_roofline(w, rec, stages, steps, first):
     """Dominant DRF stage and its roofline entry.  Spatial kernels: achieved
     algorithmic TFLOP/s (flops of the timed subsets / average launch time from
     the HIP events) against the FP32 or BF16 peak; FFT kernels: achieved
@@ -192,7 +218,7 @@ def drf_roofline(w, rec, stages, steps, warmup):
         return dom, {"kernel": dom, "bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": gbs / PEAK_HBM_GBS}
     flops = drf_flops_per_subset(w, rec)
-    avg_flops = float(np.mean([flops[(warmup + k) % S] for k in range(steps)]))
+    avg_flops = float(np.mean([flops[(first + k) % S] for k in range(steps)]))
     return dom, {"kernel": dom, "bound": "mfma", "achieved": avg_flops / avg_s / 1e12,
                  "unit": "TFLOP/s"}
 
@@ -200,8 +226,10 @@ def drf_roofline(w, rec, stages, steps, warmup):
 DRF_NAMES = {"spatial": "spatial FP32 DRF", "auto": "FP32 DRF, faster of spatial and 2-D FFT"}
 
 
-def roofline_line(w, rec, stages, steps, warmup, precision):
-    dom, roof = drf_roofline(w, rec, stages, steps, warmup)
+def roofline_line(w, rec, stages, steps, first, precision):
+    """Roofline entry of the dominant DRF stage; the event pass covered
+    'steps' sub-iterations from subset index 'first'."""
+    dom, roof = drf_roofline(w, rec, stages, steps, first)
     if roof["bound"] == "mfma":
         roof["peak"] = PEAK_BF16_TFLOPS if precision != "fp32" else PEAK_FP32_TFLOPS
         roof["frac"] = roof["achieved"] / roof["peak"]
@@ -213,7 +241,7 @@ def workload_line(w, steps, warmup, precision, ref_estimate=None, cnn=None, chec
     """Sub-object of the JSON line for one workload and DRF arithmetic."""
     import libirl_amd as L
     rec, dt, stages, est = time_subiterations(w, steps, warmup, precision, cnn, check_every, drf)
-    dom, roof = roofline_line(w, rec, stages, steps, warmup, precision)
+    dom, roof = roofline_line(w, rec, stages, steps, warmup + steps, precision)
     esse = w["krnl_file"] is not None
     name = "osem-128^3x120v-atn-drf31-%s8subsets" % (
         ("cnn-scatter-" if cnn is not None else "esse33-") if esse else "")
@@ -326,8 +354,11 @@ def batch_line(tmpdir, net, steps=16, warmup=24):
     prev = (L.set_precision("bf16"), L.set_graphs(True), L.set_autotune(True))
     L.set_scatter_cnn(net, check_every=8, max_nrmse=0.01)
     try:
-        recs = [L.Recon(w["parms"], w["opts"], w["views"], w["drf_file"], w["krnl_file"], None,
-                        w["mu"], w["prj"], None) for w in works]
+        # iterations for every step below (warmup, the timed runs, the
+        # parent-graph warmup, one at a time), the scatter modelled in all
+        total = warmup + 5 * steps + max(c["views"] // c["vs"] for c in cfgs)
+        recs = [L.Recon(run_parms(w, total), w["opts"], w["views"], w["drf_file"],
+                        w["krnl_file"], None, w["mu"], w["prj"], None) for w in works]
     finally:
         L.set_precision(prev[0])
         L.set_graphs(prev[1])
@@ -377,7 +408,7 @@ def batch_line(tmpdir, net, steps=16, warmup=24):
         r.close()
     errs = []
     for w, e in zip(works, est):
-        rec, _, _, ref = time_subiterations(w, warmup + steps, 0, "fp32")
+        rec, _, _, ref = time_subiterations(w, warmup + steps, 0, "fp32", stage_steps=0)
         rec.close()
         errs.append(float(np.linalg.norm(e - ref) / np.linalg.norm(ref)))
     n = len(recs)
@@ -426,7 +457,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=24)
-    ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-esse", action="store_true", help="skip the ESSE workload line")
     ap.add_argument("--only-esse", action="store_true",
@@ -478,7 +509,7 @@ def main():
     rec, dt, stages, est = time_subiterations(w, args.steps, args.warmup, drf="auto")
     S = rec.num_subsets
     ms_step = dt * 1e3 / args.steps
-    dom, roof = roofline_line(w, rec, stages, args.steps, args.warmup, "fp32")
+    dom, roof = roofline_line(w, rec, stages, args.steps, args.warmup + args.steps, "fp32")
     roof["traffic"] = measured_traffic(dom)
     algo = rec.drf_algorithms()
     out = {
@@ -522,21 +553,21 @@ def main():
     if not args.no_esse:
         # Same geometry with ESSE scatter modelled in the projector (the
         # plain-method scatter configuration), FP32 and with the BF16 DRF.
-        # 8 + 24 sub-iterations: the configuration's 4 iterations of 8 subsets.
         we = build_workload(tmpdir, name="c3")
+        K, W = args.steps, args.warmup
         # the plain method with the reference's spatial DRF (the FP32
         # reference of the other lines), then with the timed DRF choice
-        out["esse_spatial"] = workload_line(we, 24, 8, "fp32", drf="spatial")
+        out["esse_spatial"] = workload_line(we, K, W, "fp32", drf="spatial")
         ref = out["esse_spatial"].pop("_estimate")
-        out["esse"] = workload_line(we, 24, 8, "fp32", ref_estimate=ref, drf="auto")
+        out["esse"] = workload_line(we, K, W, "fp32", ref_estimate=ref, drf="auto")
         if not args.no_bf16:
-            out["esse_bf16"] = workload_line(we, 24, 8, "bf16", ref_estimate=ref)
+            out["esse_bf16"] = workload_line(we, K, W, "bf16", ref_estimate=ref)
         net = load_scatter_cnn()
         if net is not None and not args.no_cnn:
             # Learned scatter + BF16 DRF (and + FP32 DRF), checked against
             # ESSE every 8 subsets
             for prec, key in (("bf16", "cnn_bf16"), ("fp32", "cnn_fp32")):
-                line = workload_line(we, 24, 8, prec, ref_estimate=ref, cnn=net, check_every=8,
+                line = workload_line(we, K, W, prec, ref_estimate=ref, cnn=net, check_every=8,
                                      drf="auto")
                 line["speedup_vs_esse_fp32"] = line["value"] / out["esse"]["value"]
                 line["speedup_vs_esse_fp32_spatial"] = (line["value"] /

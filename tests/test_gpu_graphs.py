@@ -24,7 +24,8 @@ def _krnl_file(pix):
     return f
 
 
-def _recon(pb, krnl=None, srf_iters=0, graphs=False, autotune=False, precision="fp32"):
+def _recon(pb, krnl=None, srf_iters=0, graphs=False, autotune=False, precision="fp32",
+           drf="reference"):
     import libirl_amd as L
     L.set_adv_options(1, 0)
     p = pb.parms()
@@ -32,22 +33,28 @@ def _recon(pb, krnl=None, srf_iters=0, graphs=False, autotune=False, precision="
     prev_g = L.set_graphs(graphs)
     prev_a = L.set_autotune(autotune)
     prev_p = L.set_precision(precision)
+    prev_d = L.set_drf_fft(drf)
     try:
         return L.Recon(p, pb.options(), pb.views, pb.drf_file, krnl, None, pb.mu, pb.prj, pb.x0)
     finally:
         L.set_graphs(prev_g)
         L.set_autotune(prev_a)
         L.set_precision(prev_p)
+        L.set_drf_fft(prev_d)
 
 
-@pytest.mark.parametrize("scatter", [False, True])
-def test_graph_replay_equals_eager(scatter):
-    print("graph replay vs eager, scatter=%s" % scatter, flush=True)
+# the spatial DRF kernel, and the 2-D FFT kernels (transform length 64)
+@pytest.mark.parametrize("scatter,drf", [(False, "reference"), (True, "reference"),
+                                         (False, "fft"), (True, "fft")])
+def test_graph_replay_equals_eager(scatter, drf):
+    print("graph replay vs eager, scatter=%s, drf=%s" % (scatter, drf), flush=True)
     pb = Problem(n=32, nz=16, nviews=8, per=2, iters=3, half=3, noise=False)
     krnl = _krnl_file(pb.pix) if scatter else None
     srf = 2 if scatter else 0
-    a = _recon(pb, krnl, srf, graphs=False)
-    b = _recon(pb, krnl, srf, graphs=True)
+    a = _recon(pb, krnl, srf, graphs=False, drf=drf)
+    b = _recon(pb, krnl, srf, graphs=True, drf=drf)
+    if drf == "fft":
+        assert a.drf_algorithms()[:2] == ("fft", "fft") == b.drf_algorithms()[:2]
     for it in range(3):
         for s in range(a.num_subsets):
             a.subiteration(s)
