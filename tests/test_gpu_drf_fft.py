@@ -48,6 +48,30 @@ def test_fft_equals_spatial_kernel(n, nz, half, P):
     b.close()
 
 
+# ragged sizes: slice counts that are not multiples of a line's 8 threads
+# (a register straddles the last slice; 7 slices: fewer than one register),
+# bin-pair counts that leave the last wave's line slots partly idle (15, 23,
+# 31 pairs; NumPixels is even, as the library requires)
+@pytest.mark.parametrize("n,nz,half", [(30, 13, 3), (46, 21, 5), (62, 7, 4)])
+def test_fft_equals_spatial_kernel_ragged_sizes(n, nz, half):
+    print("FFT DRF vs spatial, ragged N=%d Nz=%d half=%d" % (n, nz, half), flush=True)
+    pb = Problem(n=n, nz=nz, nviews=8, per=2, iters=2, half=half, noise=False)
+    a = _recon(pb, "spatial")
+    b = _recon(pb, "fft")
+    assert b.drf_algorithms()[:2] == ("fft", "fft")
+    ga, gb = _run(a, 2), _run(b, 2)
+    e = rel_l2(gb, ga)
+    print("rel L2 FFT vs spatial %.3g" % e)
+    assert e <= 1e-5
+    if n <= 32:
+        ref = pb.oracle_osem("f64")
+        e_ref = rel_l2(gb, ref)
+        print("rel L2 vs FP64 oracle %.3g" % e_ref)
+        assert e_ref <= 1e-4
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("atn", [True, False])
 def test_irlosem_with_fft_tables_matches_oracle(atn):
     # bFFTconvolve tables: the reference's Fourier branch (no axial pad)
