@@ -101,12 +101,14 @@ def test_compute_orbit_matches_oracle(otype):
     assert np.array_equal(g_img, o_img)
 
 
-@pytest.mark.parametrize("grf_in_bck", [0, 1])
-def test_irlosem_with_computed_grf(grf_in_bck):
+# drf "fft": the 2-D FFT kernels, the backprojector's with the spectra of
+# its own table
+@pytest.mark.parametrize("grf_in_bck,drf", [(0, "reference"), (1, "reference"), (1, "fft")])
+def test_irlosem_with_computed_grf(grf_in_bck, drf):
     """IrlOsem without a DRF table file (computed GRF, IrlSetup.c:243-334),
     or with a file plus the GRF in the backprojector (bUseGrfInBck), against
     the oracle run with the same tables."""
-    print("IrlOsem computed GRF, GRF in backprojector=%d" % grf_in_bck, flush=True)
+    print("IrlOsem computed GRF, GRF in backprojector=%d, drf=%s" % (grf_in_bck, drf), flush=True)
     import libirl_amd as L
     pb = Problem(n=32, nz=16, nviews=16, per=4, iters=2, pix=0.4)
     parms = pb.parms()
@@ -130,8 +132,19 @@ def test_irlosem_with_computed_grf(grf_in_bck):
     opts.bUseGrfInBck = grf_in_bck
     atn, prj, recon = pb.mu.copy(), pb.prj.copy(), pb.x0.copy()
     L.set_adv_options(0, 0)
-    rc = L.irl_osem(parms, opts, pb.views, pb.drf_file if grf_in_bck else None, None, None, atn,
-                    prj, recon)
+    prev = L.set_drf_fft(drf)
+    try:
+        if drf == "fft":
+            # the same setup as a device-resident reconstruction: both
+            # directions on the FFT kernels
+            r = L.Recon(parms, opts, pb.views, pb.drf_file if grf_in_bck else None, None, None,
+                        pb.mu, pb.prj, pb.x0)
+            assert r.drf_algorithms()[:2] == ("fft", "fft")
+            r.close()
+        rc = L.irl_osem(parms, opts, pb.views, pb.drf_file if grf_in_bck else None, None, None,
+                        atn, prj, recon)
+    finally:
+        L.set_drf_fft(prev)
     assert rc == 0, L.error_string()
     e = rel_l2(recon, ref)
     print("rel L2 vs f32 oracle with the same tables: %.3g" % e)
