@@ -63,18 +63,21 @@ def test_graph_replay_equals_eager(scatter, drf):
             assert np.array_equal(a.estimate(), b.estimate()), (it, s)
 
 
-def test_graph_replay_with_learned_scatter_equals_eager():
-    print("graph replay vs eager, learned scatter (exact terms)", flush=True)
+@pytest.mark.parametrize("drf", ["reference", "fft"])
+def test_graph_replay_with_learned_scatter_equals_eager(drf):
+    print("graph replay vs eager, learned scatter (exact terms), drf=%s" % drf, flush=True)
     import libirl_amd as L
     pb = Problem(n=32, nz=16, nviews=8, per=2, iters=3, half=3, noise=False)
     krnl = _krnl_file(pb.pix)
     L.set_scatter_cnn_exact(True)
     L.set_scatter_cnn(None, check_every=3, max_nrmse=1.0)
     try:
-        a = _recon(pb, krnl, 3, graphs=False)
-        b = _recon(pb, krnl, 3, graphs=True)
+        a = _recon(pb, krnl, 3, graphs=False, drf=drf)
+        b = _recon(pb, krnl, 3, graphs=True, drf=drf)
     finally:
         L.set_scatter_cnn_exact(False)
+    if drf == "fft":
+        assert a.drf_algorithms()[:2] == ("fft", "fft") == b.drf_algorithms()[:2]
     for it in range(3):
         for s in range(a.num_subsets):
             a.subiteration(s)
