@@ -88,3 +88,21 @@ def test_drft_file_round_trip_through_the_library(tmp_path):
     d = t2.psDrfs[2]
     a = np.ctypeslib.as_array(d.Psf, shape=(2 * d.MaxOffsY - 1, 2 * d.MaxOffsX - 1))
     assert np.array_equal(a, psfs[2])
+
+
+def test_mode_setters_return_the_previous_setting():
+    # host-side switches for reconstructions created afterwards: no GPU work
+    import libirl_amd as L
+    if not os.path.exists(L.LIB_PATH):
+        pytest.skip("library not built")
+    lib = L.load()
+    fused = L.set_esse_fused(False)
+    assert L.set_esse_fused(fused) is False
+    prev = L.set_drf_fft("fft")
+    assert prev in ("reference", "fft", "auto", "spatial")
+    assert L.set_drf_fft("auto") == "fft"
+    # an unknown mode is refused and leaves the setting as it was
+    assert lib.irlamd_set_drf_fft(5) == -2
+    assert L.set_drf_fft(prev) == "auto"
+    with pytest.raises(ValueError):
+        L.set_drf_fft("fastest")
