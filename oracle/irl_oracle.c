@@ -34,8 +34,15 @@
  *                                orc_osem
  *   krnlprj3d.c:43-127, 403-547, 549-673, 676-954
  *                                orc_esse_init / orc_esse_effsrc
+ *   osemrecon.c:92-184           orc_measured_to_one / orc_nonmeasured_to_zero
+ *   initial_est.c:29-180         orc_initial_estimate
  *
- * Pinning status of every exported function is stated in oracle/README.md.
+ * Every exported function is checked by tests/test_oracle*.py against
+ * closed forms, known answers or independent library routines (scipy
+ * convolution, brute-force sums).  Parity unpinned: the rounding of
+ * libmip's real FFT (realft3d / multrealft3d), which the reference's ESSE
+ * and FFT-DRF products use and which is absent here; those products are
+ * restated as the circular convolutions they compute.
  */
 #include <math.h>
 #include <stdint.h>
@@ -1131,3 +1138,89 @@ void orc_reorder(int d0, int d1, int d2, const REAL *in, REAL *out)
 }
 
 int orc_real_size(void) { return (int)sizeof(REAL); }
+
+/* ------------------------------------------------------------------ */
+/* Measured-bin masks of one view, exported for the known-answer tests */
+/* (osemrecon.c:92-138 and 140-184); PWid = PixelWidth / BinWidth.     */
+/* ------------------------------------------------------------------ */
+void orc_measured_to_one(REAL *out, const REAL *meas, int nRot, int nBins, int nSlices,
+                         float PWid, float Left, float Right)
+{
+    set_measured_to_one(out, meas, nRot, nBins, nSlices, PWid, Left, Right);
+}
+
+void orc_nonmeasured_to_zero(REAL *prj, const REAL *meas, int nRot, int nBins, int nSlices,
+                             float PWid, float Left)
+{
+    set_nonmeasured_to_zero(prj, meas, nRot, nBins, nSlices, PWid, Left);
+}
+
+/* ------------------------------------------------------------------ */
+/* pfComputeInitialEstimate, initial_est.c:29-158, with vZeroBehind    */
+/* (initial_est.c:160-180) and Circular = TRUE (IrlSetup.c:59).        */
+/* prj is [V][Nz][NRot]; atn is the reordered (internal [y][z][x]) map */
+/* before fAtnScaleFac, as IrlOsem passes it (irlosem.c:171-186), or   */
+/* NULL; out is internal order.  sn/cs are the float sin and cos of    */
+/* the view angles computed in double (IrlSetup.c:119-122).  libmip's  */
+/* sum_float is absent; the projection sum is accumulated in double.   */
+/* Returns 1 when the threshold was negative (the reference's warning, */
+/* the map support is then ignored), else 0.                           */
+/* ------------------------------------------------------------------ */
+int orc_initial_estimate(int N, int Nz, int NRot, int V, float fPixelWidth, int bUseContour,
+                         float fAtnMapThresh, const float *sn, const float *cs,
+                         const float *cfcr, const float *prj, const float *atn, float *out)
+{
+    float fNumReconPix, fAvg, fMapThresh, fRadius, fCent, fX, fY, fR, fPixelRadRot;
+    double sum = 0.0;
+    long i, M = (long)N * N * Nz;
+    int a, ix, iy, s, r, c, bUseMapSupport = 0, warned = 0;
+    float *img;
+
+    fNumReconPix = (float)(N * N * Nz);
+    fNumReconPix *= M_PI / 4.0;
+    for (i = 0; i < (long)NRot * Nz * V; ++i) sum += prj[i];
+    fAvg = (float)sum / (fNumReconPix * V);
+    if (atn) {
+        fMapThresh = fAtnMapThresh;
+        fMapThresh *= fPixelWidth;
+        if (fMapThresh < 0.0) warned = 1;
+        else bUseMapSupport = fMapThresh > 0.0;
+    }
+    if (!bUseContour) {
+        for (i = 0; i < M; ++i) out[i] = fAvg;
+    } else {
+        img = (float *)malloc(sizeof(float) * N * N);
+        for (i = 0; i < (long)N * N; ++i) img[i] = fAvg;
+        for (a = 0; a < V; ++a) {
+            fPixelRadRot = cfcr[a] / fPixelWidth;
+            for (iy = 0; iy < N; ++iy) {
+                fY = iy - (float)N / 2.0;
+                for (ix = 0; ix < N; ++ix) {
+                    fX = ix - (float)N / 2.0;
+                    fR = floor(fX * cs[a] + fY * sn[a] + 0.5);
+                    if (fR > fPixelRadRot) img[ix + iy * N] = 0.0f;
+                }
+            }
+        }
+        for (s = 0; s < Nz; ++s)
+            for (r = 0; r < N; ++r)
+                for (c = 0; c < N; ++c) out[(long)r * Nz * N + (long)s * N + c] = img[c + r * N];
+        free(img);
+    }
+    if (bUseMapSupport)
+        for (i = 0; i < M; ++i)
+            if (atn[i] < fMapThresh) out[i] = 0.0f;
+    fRadius = (float)N * 0.5;
+    fCent = fRadius;
+    for (r = 0; r < N; ++r) {
+        fY = (float)r - fCent;
+        if (fY > 0) fY += 1;
+        for (c = 0; c < N; ++c) {
+            fX = (float)c - fCent;
+            if (fX > 0) fX += 1;
+            if (hypot(fX, fY) > fRadius)
+                for (s = 0; s < Nz; ++s) out[(long)r * Nz * N + (long)s * N + c] = 0.0f;
+        }
+    }
+    return warned;
+}

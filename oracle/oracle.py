@@ -61,6 +61,11 @@ def lib(real="f64"):
                                      ctypes.c_float, vp, ctypes.c_float, ctypes.c_int, vp, vp])
     L.orc_compute_orbit.restype = ctypes.c_int
     L.orc_real_size.restype = ctypes.c_int
+    L.orc_measured_to_one.argtypes = [vp, vp] + [ctypes.c_int] * 3 + [ctypes.c_float] * 3
+    L.orc_nonmeasured_to_zero.argtypes = [vp, vp] + [ctypes.c_int] * 3 + [ctypes.c_float] * 2
+    L.orc_initial_estimate.argtypes = ([ctypes.c_int] * 4 + [ctypes.c_float, ctypes.c_int,
+                                        ctypes.c_float] + [vp] * 6)
+    L.orc_initial_estimate.restype = ctypes.c_int
     _libs[real] = L
     return L
 
@@ -193,6 +198,48 @@ def reorder(d0, d1, d2, a, real="f64"):
     out = np.empty((d1, d0, d2), _dt(real))
     lib(real).orc_reorder(d0, d1, d2, a.ctypes.data, out.ctypes.data)
     return out
+
+
+def measured_to_one(meas, left, right, pwid=1.0, nbins=None, real="f64"):
+    """SetMeasuredPrjBinsToOne (osemrecon.c:92-138) of one view's [Nz][NRot]
+    measured projection: the norm image's bin mask."""
+    meas = _arr(meas, real)
+    nz, nrot = meas.shape
+    out = np.empty_like(meas)
+    lib(real).orc_measured_to_one(out.ctypes.data, meas.ctypes.data, nrot,
+                                  nrot if nbins is None else nbins, nz, pwid, left, right)
+    return out
+
+
+def nonmeasured_to_zero(prj, meas, left, pwid=1.0, nbins=None, real="f64"):
+    """SetNonMeasuredBinsToZero (osemrecon.c:140-184): a copy of prj with the
+    bins outside the measured range, or whose measured value is negative,
+    set to zero."""
+    out = _arr(prj, real).copy()
+    meas = _arr(meas, real)
+    nz, nrot = out.shape
+    lib(real).orc_nonmeasured_to_zero(out.ctypes.data, meas.ctypes.data, nrot,
+                                      nrot if nbins is None else nbins, nz, pwid, left)
+    return out
+
+
+def initial_estimate(n, nz, pw, views, prj, atn=None, contour=False, thresh=0.0):
+    """pfComputeInitialEstimate (initial_est.c:29-180) with IrlOsem's inputs:
+    prj [V][Nz][N] float32, atn the reordered but unscaled map (internal
+    [y][z][x]) or None, views with .Angle and .CFCR.  Returns (internal-order
+    estimate, warned)."""
+    prj = np.ascontiguousarray(prj, np.float32)
+    V = prj.shape[0]
+    ang = np.array([v.Angle for v in views], np.float64)
+    sn = np.sin(ang).astype(np.float32)
+    cs = np.cos(ang).astype(np.float32)
+    cf = np.array([v.CFCR for v in views], np.float32)
+    a = None if atn is None else np.ascontiguousarray(atn, np.float32)
+    out = np.empty((n, nz, n), np.float32)
+    w = lib("f32").orc_initial_estimate(n, nz, prj.shape[2], V, pw, int(contour), thresh,
+                                        sn.ctypes.data, cs.ctypes.data, cf.ctypes.data,
+                                        prj.ctypes.data, _p(a), out.ctypes.data)
+    return out, bool(w)
 
 
 class Esse:

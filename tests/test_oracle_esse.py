@@ -69,3 +69,115 @@ def test_zero_scatter_factor_disables_the_model():
     assert not e.h
     out = e.effsrc(np.ones((16, 8, 16)), np.ones((16, 8, 16)))
     assert np.all(out == 0)
+
+
+def _two_tap_kernel(a, b, s, m1, m2, ndepths=9, noffs=3):
+    """K = a at the kernel centre + b at depth offset s (same bin and slice);
+    krnlMu = m1 at the centre, m2 at the offset tap and mu0 elsewhere (where
+    K is zero, so those values only test that they do not enter)."""
+    k = np.zeros((ndepths, noffs, noffs), np.float32)
+    km = np.full_like(k, MU0)
+    c = ndepths // 2 - 1
+    k[c, 0, 0], km[c, 0, 0] = a, m1
+    k[c - s, 0, 0], km[c - s, 0, 0] = b, m2
+    return k, km
+
+
+def _depth_shift(act, s):
+    out = np.zeros_like(act)
+    out[s:] = act[:-s]
+    return out
+
+
+def _taylor_case(nterms):
+    rng = np.random.default_rng(7)
+    n, nz, s, mu = 16, 4, 2, 0.06
+    a, b, m1, m2 = 1.0, 0.5, 0.10, 0.22
+    k, km = _two_tap_kernel(a, b, s, m1, m2)
+    e = O.Esse(k, km, MU0, PIX, n, nz, nterms=nterms)
+    act = 0.5 + rng.random((n, nz, n))
+    mp = np.full((n, nz, n), mu)
+    # kernel-weighted mean of krnlMu (krnlprj3d.c:500-515) and the
+    # half-pixel depth d(y) = (y + 1/2) mu / mu0 of a uniform map
+    muavg = (a * m1 + b * m2) / (a + b)
+    d = ((np.arange(n) + 0.5) * mu / MU0)[:, None, None]
+    sh = _depth_shift(act, s)
+    e0 = a * act + b * sh
+    e1 = a * (m1 - muavg) * act + b * (m2 - muavg) * sh
+    e2 = a * (m1 - muavg) ** 2 * act + b * (m2 - muavg) ** 2 * sh
+    return e, act, mp, d, muavg, e0, e1, e2, mu
+
+
+def test_first_order_term_with_a_nonuniform_mu_kernel():
+    # krnlMu differs over the kernel's support, so K (krnlMu - muavg) does
+    # not vanish: eff = (E0 - d E1) exp(d (mu0 - muavg)) mu / (mu0 pix)
+    # (krnlprj3d.c:826-854, 903-950).  A sign flip of the first-order term
+    # moves the result by 2 d E1 (up to 30 % here).
+    e, act, mp, d, muavg, e0, e1, _, mu = _taylor_case(2)
+    want = np.maximum((e0 - d * e1) * np.exp(d * (MU0 - muavg)), 0) * mu / (MU0 * PIX)
+    got = e.effsrc(act, mp)
+    assert np.abs(d * e1).max() > 0.1 * np.abs(e0).max()
+    np.testing.assert_allclose(got, want, rtol=2e-6)
+
+
+def test_second_order_term_with_three_taylor_terms():
+    # T = 3 (reachable through pvInitESSEScatterModel's parameter string):
+    # + d^2/2 * (K (krnlMu - muavg)^2 (*) act) (krnlprj3d.c:856-899)
+    e, act, mp, d, muavg, e0, e1, e2, mu = _taylor_case(3)
+    want = (np.maximum((e0 - d * e1 + 0.5 * d * d * e2) * np.exp(d * (MU0 - muavg)), 0) *
+            mu / (MU0 * PIX))
+    got = e.effsrc(act, mp)
+    np.testing.assert_allclose(got, want, rtol=2e-6)
+    # and the second-order term is large enough to be seen at that tolerance
+    two = O.Esse(*_two_tap_kernel(1.0, 0.5, 2, 0.10, 0.22), MU0, PIX, 16, 4, nterms=2)
+    assert np.abs(got - two.effsrc(act, mp)).max() > 1e-3 * np.abs(got).max()
+
+
+# ExpandByFac's smoothing runs in place (krnlprj3d.c:632-670): each sample
+# of a (2 (c/2) + 1)-point window average along an axis reads the already
+# smoothed samples before it, and the first and last c/2 samples keep their
+# value.  For an image that varies only along depth, with one coarse depth
+# row of 1s (c^3 fine voxels of 1 collapse to c^3, expand back to 1), the
+# in-place average of the depth profile was worked out by hand:
+#   c = 2, 3-point window, x = [0 0 1 1 0 0 0 0]
+#   c = 3, 3-point window, x = [0 0 0 1 1 1 0 0 0 0 0 0]
+#   c = 4, 5-point window, x = [0 0 0 0 1 1 1 1 0 ...] (first 9 samples)
+# (a window computed out of place would give a profile symmetric about the
+# row, e.g. [0 1/3 2/3 2/3 1/3 0 0 0] for c = 2).
+_EXPAND_PROFILE = {
+    2: (8, [0, 1 / 3, 7 / 9, 16 / 27, 16 / 81, 16 / 243, 16 / 729, 0]),
+    3: (12, [0, 0, 1 / 3, 7 / 9, 25 / 27, 52 / 81, 52 / 243, 52 / 729, 52 / 2187, 52 / 6561,
+             52 / 19683, 0]),
+    4: (16, [0, 0, 1 / 5, 11 / 25, 91 / 125, 521 / 625, 2226 / 3125, 7956 / 15625,
+             19086 / 78125]),
+}
+
+
+def test_expand_smooths_in_place_along_each_axis():
+    for c, (n, prof) in _EXPAND_PROFILE.items():
+        nz = 2 * c
+        act = np.zeros((n, nz, n))
+        act[c:2 * c] = 1.0                     # coarse depth row 1
+        mp = np.full((n, nz, n), 0.05)
+        e = O.Esse(*_delta_kernel(), MU0, PIX, n, nz, collapse=c)
+        got = e.effsrc(act, mp) * (MU0 * PIX) / 0.05
+        # constant along slices and bins, so the other two axes' passes
+        # keep every value
+        for y, want in enumerate(prof):
+            np.testing.assert_allclose(got[y], want, rtol=1e-6, atol=1e-12,
+                                       err_msg="c=%d depth %d" % (c, y))
+
+
+def test_expand_smooths_bins_and_slices_in_place():
+    # the same profile along the bin axis and along the slice axis
+    c, (n, prof) = 2, _EXPAND_PROFILE[2]
+    mp = np.full((n, n, n), 0.05)
+    e = O.Esse(*_delta_kernel(), MU0, PIX, n, n, collapse=c)
+    for axis in (1, 2):
+        act = np.zeros((n, n, n))
+        idx = [slice(None)] * 3
+        idx[axis] = slice(c, 2 * c)
+        act[tuple(idx)] = 1.0
+        got = e.effsrc(act, mp) * (MU0 * PIX) / 0.05
+        np.testing.assert_allclose(np.moveaxis(got, axis, 0)[:, 3, 5], prof, rtol=1e-6,
+                                   atol=1e-12, err_msg="axis %d" % axis)
