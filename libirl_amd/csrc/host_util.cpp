@@ -322,8 +322,8 @@ void vWriteDrfTab(DrfTab_t *t, char *pchOutFname, char *pchTitle, char *pchColNa
 {
     FILE *fp = fopen(pchOutFname, "wb");
     if (!fp) {
-        irlamd::set_error_string(std::string("Error opening DRF table for writing: ") +
-                                 pchOutFname);
+        // drftabio.c:73-77
+        irlamd::set_error_string(std::string("Error opening DRF Table file: ") + pchOutFname);
         return;
     }
     char tbuf[80] = {0}, cbuf[20] = {0};
