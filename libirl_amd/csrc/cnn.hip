@@ -1353,6 +1353,8 @@ irlamd_cnn *irlamd_cnn_create(float fKappa, unsigned uSeed)
     try {
         auto *h = new irlamd_cnn;
         h->net.reset(new irlamd::ScatterCnn(fKappa, uSeed));
+        // usable at once by reconstructions (BF16 inference form of the weights)
+        h->net->prepareInference();
         return h;
     } catch (const irlamd::IrlError &e) {
         cnn_report(e);

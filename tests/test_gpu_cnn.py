@@ -164,6 +164,7 @@ def test_irlosem_first_order_source_with_exact_terms():
         st = L.cnn_status()
     finally:
         L.set_scatter_cnn_exact(False)
+        L.set_scatter_cnn(None, check_every=0, max_nrmse=0.01)   # the library defaults
     e = rel_l2(got, ref)
     print("rel L2 vs FP64 ESSE %.3g, status %s" % (e, st))
     assert st["checks"] == 1 and st["active"] and st["used"] == 12

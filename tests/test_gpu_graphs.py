@@ -76,6 +76,7 @@ def test_graph_replay_with_learned_scatter_equals_eager(drf):
         b = _recon(pb, krnl, 3, graphs=True, drf=drf)
     finally:
         L.set_scatter_cnn_exact(False)
+        L.set_scatter_cnn(None, check_every=0, max_nrmse=0.01)   # the library defaults
     if drf == "fft":
         assert a.drf_algorithms()[:2] == ("fft", "fft") == b.drf_algorithms()[:2]
     for it in range(3):
