@@ -752,7 +752,8 @@ __global__ __launch_bounds__(DrfFftShape<P>::NT) void k_esse_inv2d_src(
                 const float e0v = (h ? e0[w].y : e0[w].x) * scale;
                 const float e1v = (h ? g[w].y : g[w].x) * scale;
                 const float f = afac[vo + o], mp = map[vo + o], av = act[bo + o];
-                const float d = -logf(f) * fAtnCnvFac;
+                // f == 0 (depth past float range): d = 0 keeps eff finite, out is 0
+                const float d = f > 0.0f ? -logf(f) * fAtnCnvFac : 0.0f;
                 float eff = e0v - d * e1v;
                 eff = eff * expf(d * fMu);
                 if (cntr > 0) eff = eff + av * cntr;

@@ -1444,11 +1444,20 @@ struct Recon::Impl {
         nchunk = prjOn ? nchunkF : nchunkD;
     }
 
+    // Device table of Vs view indices for the set-up timings (tile autotuning,
+    // DRF algorithm choice): subset 0's views, or the first Vs views when the
+    // reconstruction has no subset table (IrlGenprj's forward projection).
+    const int *timing_views() const
+    {
+        return c.subsets.size() >= (size_t)c.Vs ? subsets.p : allViews.p;
+    }
+
     // Times the spatial and the FFT kernel per direction on the first
     // subset's views (one launch to warm up, three timed) and keeps the
     // faster.  Inputs are whatever the buffers hold; only the time counts.
     void choose_drf(int Vs, int prjModel, int bckModel)
     {
+        const int *tv = timing_views();
         const int KYb = ky_bucket();
         hipEvent_t e0, e1;
         HIP_CHECK(hipEventCreate(&e0));
@@ -1460,7 +1469,7 @@ struct Recon::Impl {
                     set_fft(fftPrj, f != 0);
                 else
                     set_fft(f != 0, fftBck);
-                const DrfArgs da = drf_args(bck != 0, subsets.p, bck ? bckModel : prjModel);
+                const DrfArgs da = drf_args(bck != 0, tv, bck ? bckModel : prjModel);
                 run_drf(bck != 0, da, Vs, KYb);
                 HIP_CHECK(hipEventRecord(e0, stream));
                 for (int r = 0; r < 3; ++r) run_drf(bck != 0, da, Vs, KYb);
@@ -2107,7 +2116,7 @@ void Recon::autotuneTiles()
     Impl &I = *impl_;
     if (cfg_.drf.nDist <= 0) return;
     const int KYb = I.ky_bucket();
-    const int *bview = I.subsets.p;
+    const int *bview = I.timing_views();
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
@@ -2370,6 +2379,10 @@ void Recon::runViews(int s, int first, int Vs, SrfMode srf, bool accumulate, boo
     Impl &I = *impl_;
     const Geometry &g = cfg_.g;
     const int threads = 256;
+    if (s < 0 || first < 0 || first + Vs > cfg_.Vs ||
+        cfg_.subsets.size() < (size_t)(s + 1) * cfg_.Vs)
+        fail(ETYPE_ILLEGAL_VALUE, "sub-iteration %d (views %d..%d) outside the subset table", s,
+             first, first + Vs);
     const int *bview = I.subsets.p + (long)s * cfg_.Vs + first;
     const int KYb = cfg_.drf.nDist > 0 ? I.ky_bucket() : 7;
     const float *afac = (cfg_.prjModel & 1) ? I.afac.p : nullptr;
@@ -2474,12 +2487,19 @@ void Recon::forwardProject(float primary, float *out_host, bool srf)
         const int B = std::min(Vs, g.V - v0);
         const int *bview = order.p + v0;
         if (srf && I.esse) {
+            // the reconstruction's scatter path (run_srf_att) with fPrimaryFac
             I.launch("rot_x", [&] { I.run_att(nullptr, B, bview, I.x.p, I.rotx.p); });
-            I.launch("esse_conv", [&] { I.esse->convolve(I.rotx.p, B, I.terms.p); });
-            I.launch("esse_source", [&] {
-                I.esse->source(I.rotx.p, I.rotmu.p, I.terms.p, B, primary, afac, bview, I.att.p,
-                               true);
-            });
+            if (I.esse->canConvolveSource(I.rotmu.p, afac, bview)) {
+                I.launch("esse_conv_source", [&] {
+                    I.esse->convolveSource(I.rotx.p, I.rotmu.p, B, primary, afac, bview, I.att.p);
+                });
+            } else {
+                I.launch("esse_conv", [&] { I.esse->convolve(I.rotx.p, B, I.terms.p); });
+                I.launch("esse_source", [&] {
+                    I.esse->source(I.rotx.p, I.rotmu.p, I.terms.p, B, primary, afac, bview,
+                                   I.att.p, true);
+                });
+            }
         } else {
             I.launch("att", [&] { I.run_att(afac, B, bview); });
         }

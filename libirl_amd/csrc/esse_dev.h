@@ -38,8 +38,11 @@ public:
     // factors are for its rotated mu).  The two terms' inverse transforms
     // along the slices run as one complex transform and the source is formed
     // from them in registers, with the depth taken from the factors
-    // (-ln afac); the terms are not stored.  Returns false, having done
-    // nothing, when the conditions do not hold.
+    // (-ln afac); the terms are not stored.  The factors must have been
+    // built with an attenuation factor of 1 (the engine's precomputed ones
+    // are); a factor of 0 (a depth past the float range) gives a zero
+    // source.  Returns false, having done nothing, when the conditions do
+    // not hold.
     bool convolveSource(const float *act, const float *map, int B, float primary,
                         const float *afac, const int *bview, float *out);
     bool canConvolveSource(const float *map, const float *afac, const int *bview) const

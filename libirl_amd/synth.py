@@ -72,7 +72,10 @@ def views_ellipse(nviews, n, seed, rmin=20.0, rmax=28.0):
 
 def gaussian_drf_table(pixsize, start, ndist, spacing, half, fwhm0=0.3, slope=0.02):
     """Gaussian DRF with FWHM linear in distance, (2*half+1)^2 support, each
-    plane normalised to sum 1.  Returns (offs, psfs) with offs=(half+1)."""
+    plane normalised to sum 1.  Taps below the normal float range (< 2^-126,
+    far Gaussian tails of the narrow planes) are stored as 0: as subnormal
+    operands they would only slow CPU float arithmetic 10-100x.  Returns
+    (offs, psfs) with offs=(half+1)."""
     psfs = []
     ax = (np.arange(2 * half + 1) - half) * pixsize
     for d in range(ndist):
@@ -81,7 +84,9 @@ def gaussian_drf_table(pixsize, start, ndist, spacing, half, fwhm0=0.3, slope=0.
         sig = fwhm / (2.0 * np.sqrt(2.0 * np.log(2.0)))
         g = np.exp(-(ax ** 2) / (2 * sig * sig))
         p = np.outer(g, g)
-        psfs.append((p / p.sum()).astype(np.float32))
+        p = (p / p.sum()).astype(np.float32)
+        p[np.abs(p) < np.finfo(np.float32).tiny] = 0.0
+        psfs.append(p)
     return [(half + 1, half + 1)] * ndist, psfs
 
 

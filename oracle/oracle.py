@@ -66,6 +66,8 @@ def lib(real="f64"):
     L.orc_initial_estimate.argtypes = ([ctypes.c_int] * 4 + [ctypes.c_float, ctypes.c_int,
                                         ctypes.c_float] + [vp] * 6)
     L.orc_initial_estimate.restype = ctypes.c_int
+    L.orc_esse_effsrc_cols.argtypes = [vp, vp, vp] + [ctypes.c_int] * 4 + [vp, vp]
+    L.orc_esse_effsrc_cols.restype = ctypes.c_int
     _libs[real] = L
     return L
 
@@ -200,6 +202,28 @@ def reorder(d0, d1, d2, a, real="f64"):
     return out
 
 
+def project_srf_columns(n, nz, angle, esse, atn, recon, cols, primary=1.0, real="f64", fast=0):
+    """Project (prjbck.c:68-252) with attenuation and ESSE scatter, no DRF,
+    of one view, on the detector columns cols = [(slice, bin), ...] only:
+    rotation by 0.5 pi + angle (rot3dpar, or with 'fast' the three-shear
+    rotation vShearsRotate), the effective source on those
+    depth columns (Esse.effsrc_columns; esse None: attenuation only) and the
+    half-pixel attenuated depth sum.  atn (per pixel) and recon are internal [y][z][x] volumes."""
+    ang = np.float32(0.5 * np.pi + np.float64(np.float32(angle)))
+    if fast:
+        ra = shears_rotate(float(ang), atn, nz, n, real=real)
+        rx = shears_rotate(float(ang), recon, nz, n, real=real)
+    else:
+        ra = rot3dpar(float(ang), atn, 0, nz, n, n, real=real)
+        rx = rot3dpar(float(ang), recon, 0, nz, n, n, real=real)
+    c = np.asarray(cols, np.int64).reshape(-1, 2)
+    eff = 0.0 if esse is None else esse.effsrc_columns(rx, ra, c)   # [col][row]
+    mu = ra[:, c[:, 0], c[:, 1]].T                           # [col][row]
+    src = rx[:, c[:, 0], c[:, 1]].T * primary + eff
+    S = np.cumsum(mu, axis=1) - 0.5 * mu                     # half-pixel running sum
+    return (src * np.exp(-S)).sum(axis=1)
+
+
 def measured_to_one(meas, left, right, pwid=1.0, nbins=None, real="f64"):
     """SetMeasuredPrjBinsToOne (osemrecon.c:92-138) of one view's [Nz][NRot]
     measured projection: the norm image's bin mask."""
@@ -262,6 +286,20 @@ class Esse:
         out = np.zeros_like(act)
         lib(self.real).orc_esse_effsrc(self.h, act.ctypes.data, mp.ctypes.data, nx, ny, nz,
                                        out.ctypes.data)
+        return out
+
+    def effsrc_columns(self, act, mp, cols):
+        """The effective source on the depth columns cols = [(slice, bin), ...]
+        of [depth][slice][bin] volumes: [len(cols)][depth]."""
+        act = _arr(act, self.real)
+        mp = _arr(mp, self.real)
+        ny, nz, nx = act.shape
+        c = np.ascontiguousarray(cols, np.int32).reshape(-1, 2)
+        out = np.zeros((len(c), ny), _dt(self.real))
+        rc = lib(self.real).orc_esse_effsrc_cols(self.h, act.ctypes.data, mp.ctypes.data, nx, ny,
+                                                 nz, len(c), c.ctypes.data, out.ctypes.data)
+        if rc:
+            raise ValueError("column evaluation needs an uncollapsed model")
         return out
 
     def __del__(self):

@@ -13,18 +13,21 @@ class Problem:
 
     def __init__(self, n=32, nz=16, nviews=16, per=4, iters=2, pix=0.5, seed=1, half=5,
                  atn=True, drf=True, cfcr=None, counts=2e5, noise=True, orbit="circular",
-                 pad=0, avg=0, fft=False):
+                 pad=0, avg=0, fft=False, inserts=(2, 3), rmax=None, views=None):
         self.n, self.nz, self.nviews, self.per, self.iters, self.pix = n, nz, nviews, per, iters, pix
         self.pad, self.avg = pad, avg
-        act, mu = synth.phantom(n, nz, seed, n_inserts=(2, 3))
+        act, mu = synth.phantom(n, nz, seed, n_inserts=inserts)
         self.act = act
         self.mu = mu if atn else None
         if cfcr is None:
             cfcr = n * pix / 2 + 2.0
-        if orbit == "circular":
+        if views is not None:
+            self.views = views
+        elif orbit == "circular":
             self.views = synth.views_circular(nviews, cfcr, n)
         else:
-            self.views = synth.views_ellipse(nviews, n, seed, cfcr, cfcr + 4.0)
+            self.views = synth.views_ellipse(nviews, n, seed, cfcr,
+                                             cfcr + 4.0 if rmax is None else rmax)
         self.tmpdir = tempfile.mkdtemp(prefix="irltest")
         self.drf_file = None
         self.drf = None
@@ -79,6 +82,22 @@ class Problem:
                    self.pix, self.drf, prj, self.atn_int, x0, pad=self.pad, avg=self.avg,
                    real=real, iter_cb=iter_cb, fast=fast)
         return O.reorder(self.n, self.nz, self.n, x.reshape(self.n, self.nz, self.n))
+
+
+def config_problem(name, seed=1, **kw):
+    """Problem of one of the benchmark configurations (libirl_amd.synth.CONFIGS:
+    sizes, pixel, views and subsets, orbit, phantom class, DRF support,
+    counts); keyword arguments override them."""
+    c = dict(synth.CONFIGS[name])
+    args = dict(n=c["n"], nz=c["nz"], nviews=c["views"], per=c["vs"], iters=c["iters"],
+                pix=c["pix"], seed=seed, half=c["half"], counts=c["counts"],
+                inserts=c["inserts"], orbit=c["orbit"])
+    if c["orbit"] == "circular":
+        args["cfcr"] = c["cfcr"]
+    else:
+        args["cfcr"], args["rmax"] = c.get("rmin", 20.0), c.get("rmax", 28.0)
+    args.update(kw)
+    return Problem(**args)
 
 
 def rel_l2(a, b):

@@ -1,0 +1,269 @@
+"""Parity at the benchmark configurations (libirl_amd.synth.CONFIGS), each
+within the GPU suite's time: C1 in full against the oracle; the 128^3 x
+120-view geometry of C2 (one sub-iteration, spatial and FFT DRF) against the
+oracle; the C3 scatter model at 128^3 with its 33^3 kernel on the fused
+plane-transform path against the oracle on sampled detector columns; the
+uneven batch of C5 at a noise-free fixed point, with a small member against
+the oracle."""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+import libirl_amd as L
+from libirl_amd import synth
+from oracle import oracle as O
+from tests.cases import Problem, config_problem, rel_l2
+
+pytestmark = pytest.mark.gpu
+MU0 = 0.15
+
+
+@pytest.mark.timeout(600)
+def test_c1_full_reconstruction_matches_oracle():
+    # 64^3 at 0.4 cm, 60 views on a 25 cm circle, 15x15 Gaussian DRF,
+    # attenuation, 500k Poisson counts, 4 subsets x 2 iterations from a
+    # uniform estimate
+    print("C1: 64^3, 60 views, 4x2", flush=True)
+    pb = config_problem("c1", seed=3)
+    assert (pb.n, pb.nviews, pb.per, pb.iters) == (64, 60, 15, 2)
+    atn, prj, recon = pb.mu.copy(), pb.prj.copy(), pb.x0.copy()
+    L.set_adv_options(0, 0)
+    rc = L.irl_osem(pb.parms(), pb.options(), pb.views, pb.drf_file, None, None, atn, prj, recon)
+    assert rc == 0, L.error_string()
+    print("counts %.0f" % pb.prj.sum(), flush=True)
+    ref32, ref64 = pb.oracle_osem("f32"), pb.oracle_osem("f64")
+    e32, e64 = rel_l2(recon, ref32), rel_l2(recon, ref64)
+    print("rel L2 vs f32 %.3g vs f64 %.3g" % (e32, e64))
+    assert e32 <= 1e-4 and e64 <= 1e-4
+
+
+@pytest.fixture(scope="module")
+def c2_subset0():
+    """The C2 geometry (128^3 at 0.33 cm, 120 views on a seeded 20-28 cm
+    ellipse, 31x31 DRF, attenuation, 2M Poisson counts, 8 subsets); data
+    from the GPU projector (IrlGenprj, rescaled by V), the same input on
+    both sides.  The oracle side is the 15-view problem of subset 0
+    {0, 8, ..., 112}, whose one iteration (the subset's sensitivity, then
+    one update) is the 120-view reconstruction's first sub-iteration."""
+    c = synth.CONFIGS["c2"]
+    n, nz, pix, V, seed = c["n"], c["nz"], c["pix"], c["views"], 4
+    act, mu = synth.phantom(n, nz, seed, n_inserts=c["inserts"])
+    views = synth.views_ellipse(V, n, seed, 20.0, 28.0)
+    drf_file = os.path.join(tempfile.mkdtemp(prefix="irlc2"), "c2.drft")
+    offs, psfs, start = synth.drf_for_orbit(drf_file, pix, n, views, c["half"])
+    p = L.IrlParms()
+    p.NumPixels, p.NumSlices, p.NumViews = n, nz, V
+    p.BinWidth, p.fAtnScaleFac, p.fScatEstFac = pix, pix, 1.0
+    p.NumAngPerSubset, p.NumIterations, p.SrfCollapseFac = c["vs"], 1, 1
+    o = L.Options()
+    o.bModelDrf, o.bReconIsInitEst, o.iMsgLevel = 1, 1, 1
+    prj = np.zeros((V, nz, n), np.float32)
+    L.set_adv_options(0, 0)
+    rc = L.irl_genprj(p, o, views, drf_file, None, None, act.copy(), mu.copy(), prj)
+    assert rc == 0, L.error_string()
+    prj = synth.poisson(prj * V, c["counts"], seed)
+    x0 = np.full((nz, n, n), float(prj.sum()) / (n * n * nz * np.pi / 4 * V), np.float32)
+    sub = list(range(0, V, 8))
+    drf = O.DrfTable(start, pix, offs, psfs)
+    ov = [dict(angle=views[k].Angle, left=views[k].Left, right=views[k].Right,
+               cfcr=views[k].CFCR) for k in sub]
+    ref = O.osem(nz, n, ov, 15, 1, 3, 3, pix, drf, prj[sub], O.reorder(nz, n, n, mu * pix),
+                 O.reorder(nz, n, n, x0), real="f32")
+    ref = O.reorder(n, nz, n, ref.reshape(n, nz, n))
+    return p, o, views, drf_file, mu, prj, x0, ref
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("drf", ["reference", "fft"])
+def test_c2_first_subiteration_matches_oracle(c2_subset0, drf):
+    print("C2: 128^3, 120 views, subset 0, DRF %s" % drf, flush=True)
+    p, o, views, drf_file, mu, prj, x0, ref = c2_subset0
+    L.set_adv_options(0, 0)
+    prev = L.set_drf_fft(drf)
+    try:
+        rec = L.Recon(p, o, views, drf_file, None, None, mu, prj, x0)
+    finally:
+        L.set_drf_fft(prev)
+    try:
+        assert rec.num_subsets == 8
+        if drf == "fft":
+            assert rec.drf_algorithms()[:2] == ("fft", "fft")
+        rec.subiteration(0)
+        got = rec.estimate()
+    finally:
+        rec.close()
+    e = rel_l2(got, ref)
+    print("rel L2 vs f32 oracle %.3g" % e)
+    assert e <= 1e-4
+
+
+def _c3_inputs(nviews, seed=5):
+    c = synth.CONFIGS["c3"]
+    n, nz, pix = c["n"], c["nz"], c["pix"]
+    act, mu = synth.phantom(n, nz, seed, n_inserts=c["inserts"])
+    views = synth.views_ellipse(c["views"], n, seed, 20.0, 28.0)[::c["views"] // nviews]
+    tmp = tempfile.mkdtemp(prefix="irlc3")
+    k, km = synth.gaussian_mixture_krnl(c["krnl_offs"], c["krnl_depths"], pix)
+    k = synth.scale_krnl_to_fraction(k, 0.3)
+    krnl = os.path.join(tmp, "c3.krnl")
+    synth.write_krnl(krnl, k, km, MU0, pix)
+    return c, act, mu, views, krnl, k, km
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("fused", [True, False])
+def test_c3_scatter_projection_matches_oracle_on_columns(fused):
+    # 128^3 with the 17 x 33 (33^3 mirrored) ESSE kernel, attenuation and
+    # scatter, no DRF: a projection bin is a depth sum of the attenuated
+    # primary + effective source, so the oracle evaluates the scatter model
+    # only on sampled columns (the full direct sum would take hours).  With
+    # 8 views of 128 depth planes the fused path runs the 2-D plane
+    # transforms with the source in the last pass (160-point FFTs).
+    # IrlGenprj rotates with the three-shear rotation (irlgenprj.c:19-21).
+    V = 8
+    print("C3: 128^3 ESSE 33^3, %d views, fused=%s" % (V, fused), flush=True)
+    c, act, mu, views, krnl, k, km = _c3_inputs(V)
+    n, nz, pix = c["n"], c["nz"], c["pix"]
+    p = L.IrlParms()
+    p.NumPixels, p.NumSlices, p.NumViews = n, nz, V
+    p.BinWidth, p.fAtnScaleFac, p.fScatEstFac = pix, pix, 1.0
+    p.NumAngPerSubset, p.NumIterations, p.SrfCollapseFac, p.iNumSrfIterations = V, 1, 1, 1
+    o = L.Options()
+    o.bModelDrf, o.bReconIsInitEst, o.iMsgLevel = 0, 1, 1
+    prj = np.zeros((V, nz, n), np.float32)
+    prev = L.set_esse_fused(fused)
+    try:
+        rc = L.irl_genprj(p, o, views, None, krnl, None, act.copy(), mu.copy(), prj)
+    finally:
+        L.set_esse_fused(prev)
+    assert rc == 0, L.error_string()
+    esse = O.Esse(k, km, MU0, pix, n, nz, real="f64")
+    atn = O.reorder(nz, n, n, mu * pix)
+    x = O.reorder(nz, n, n, act)
+    rng = np.random.default_rng(9)
+    errs, scat = [], []
+    for v, view in enumerate(views):
+        cols = np.stack([rng.integers(0, nz, 256), rng.integers(n // 4, 3 * n // 4, 256)], 1)
+        want = O.project_srf_columns(n, nz, view.Angle, esse, atn, x, cols, fast=1)
+        prim = O.project_srf_columns(n, nz, view.Angle, None, atn, x, cols, fast=1)
+        got = prj[v][cols[:, 0], cols[:, 1]].astype(np.float64) * V
+        errs.append(rel_l2(got, want))
+        scat.append(float(np.linalg.norm(want - prim) / np.linalg.norm(want)))
+    print("per-view rel L2 %s; scatter share of the projection %.2f" %
+          (["%.2g" % e for e in errs], float(np.mean(scat))))
+    assert max(errs) <= 1e-4
+    assert np.mean(scat) > 0.1     # the scatter part is a real share of the projection
+
+
+def _member(c, seed, tmp):
+    """One uneven reconstruction of the batched configuration: phantom,
+    ellipse orbit just outside the field of view, DRF and ESSE kernel."""
+    n, nz, pix, V = c["n"], c["nz"], c["pix"], c["views"]
+    act, mu = synth.phantom(n, nz, seed, n_inserts=c["inserts"])
+    views = synth.views_ellipse(V, n, seed, c["rmin"], c["rmax"])
+    drf = os.path.join(tmp, "m%d.drft" % seed)
+    synth.drf_for_orbit(drf, pix, n, views, c["half"])
+    krnl = synth.esse_kernel_file(os.path.join(tmp, "m%d.krnl" % seed), c)
+    p = L.IrlParms()
+    p.NumPixels, p.NumSlices, p.NumViews = n, nz, V
+    p.BinWidth, p.fAtnScaleFac, p.fScatEstFac = pix, pix, 1.0
+    p.NumAngPerSubset, p.NumIterations, p.SrfCollapseFac = c["vs"], 2, 1
+    p.iNumSrfIterations = 2
+    o = L.Options()
+    o.bModelDrf, o.bReconIsInitEst, o.iMsgLevel = 1, 1, 1
+    return act, mu, views, drf, krnl, p, o
+
+
+def test_c5_batch_holds_the_noise_free_fixed_point():
+    # 16 uneven reconstructions (N 64..160, 64..128 views, 8 subsets, ATN +
+    # DRF + ESSE in every step), graphs and tile autotuning on, stepped as
+    # one batch.  Data are their own model's projections of x* (IrlGenprj
+    # on the GPU, which scales by 1/V and rotates with the three-shear
+    # rotation, as the reconstructions do here) and x0 = x*/V: every update
+    # ratio is 1 where the object projects, so each estimate must stay at x0.
+    rng = np.random.default_rng(5)
+    base = synth.CONFIGS["c3"]
+    tmp = tempfile.mkdtemp(prefix="irlc5")
+    recs, x0s = [], []
+    L.set_adv_options(1, 0)
+    prev = (L.set_graphs(True), L.set_autotune(True))
+    try:
+        for i in range(16):
+            n = int(rng.choice(np.arange(64, 161, 16)))
+            V = int(rng.choice(np.arange(64, 129, 8)))
+            r0 = 0.5 * n * base["pix"] + 2.0
+            c = dict(base, n=n, nz=n, views=V, vs=V // 8, rmin=r0, rmax=r0 + 8.0)
+            act, mu, views, drf, krnl, p, o = _member(c, 200 + i, tmp)
+            print("member %d: N=%d V=%d" % (i, n, V), flush=True)
+            prj = np.zeros((V, n, n), np.float32)
+            rc = L.irl_genprj(p, o, views, drf, krnl, None, act.copy(), mu.copy(), prj)
+            assert rc == 0, L.error_string()
+            x0 = (act / V).astype(np.float32)
+            recs.append(L.Recon(p, o, views, drf, krnl, None, mu, prj, x0))
+            x0s.append(x0)
+    finally:
+        L.set_graphs(prev[0])
+        L.set_autotune(prev[1])
+        L.set_adv_options(0, 0)
+    batch = L.Batch(recs)
+    try:
+        for _ in range(16):            # two iterations of every member
+            batch.step()
+        batch.synchronize()
+        st = batch.stats()
+    finally:
+        batch.close()
+    errs = []
+    for r, x0 in zip(recs, x0s):
+        errs.append(rel_l2(r.estimate(), x0))
+        r.close()
+    print("rel L2 to x0 after 2 iterations: max %.3g mean %.3g, stats %s" %
+          (max(errs), float(np.mean(errs)), st))
+    assert st["replays"] > 0
+    assert max(errs) <= 1e-4
+
+
+def test_c5_small_member_in_an_uneven_batch_matches_oracle():
+    # a 32^3 member with ESSE beside two larger ones in one batch: after its
+    # three iterations (scatter modelled in two) it equals the FP64 oracle
+    print("C5: small member of an uneven batch vs oracle", flush=True)
+    pb = Problem(n=32, nz=16, nviews=8, per=2, iters=3, half=3, noise=False, seed=6)
+    k, km = synth.gaussian_mixture_krnl(5, 9, pb.pix, mu0=MU0)
+    k = synth.scale_krnl_to_fraction(k, 0.3)
+    tmp = tempfile.mkdtemp(prefix="irlc5s")
+    krnl = os.path.join(tmp, "k.krnl")
+    synth.write_krnl(krnl, k, km, MU0, pb.pix)
+    esse = O.Esse(k, km, MU0, pb.pix, pb.n, pb.nz, real="f64")
+    x0 = O.reorder(pb.nz, pb.n, pb.n, pb.x0)
+    ref = O.osem(pb.nz, pb.n, pb.oracle_views(), pb.per, pb.iters, pb.prj_model | 4,
+                 pb.prj_model, pb.pix, pb.drf, np.clip(pb.prj, 0, None), pb.atn_int, x0,
+                 esse=esse.h, srf_iterations=2)
+    ref = O.reorder(pb.n, pb.nz, pb.n, ref.reshape(pb.n, pb.nz, pb.n))
+    others = [Problem(n=48, nz=24, nviews=12, per=3, iters=3, half=3, noise=False, seed=7),
+              Problem(n=64, nz=32, nviews=16, per=2, iters=3, half=3, noise=False, seed=8)]
+    L.set_adv_options(0, 0)
+    prev = (L.set_graphs(True), L.set_autotune(True))
+    try:
+        p = pb.parms()
+        p.iNumSrfIterations = 2
+        small = L.Recon(p, pb.options(), pb.views, pb.drf_file, krnl, None, pb.mu, pb.prj, pb.x0)
+        big = [L.Recon(q.parms(), q.options(), q.views, q.drf_file, None, None, q.mu, q.prj, q.x0)
+               for q in others]
+    finally:
+        L.set_graphs(prev[0])
+        L.set_autotune(prev[1])
+    batch = L.Batch([small] + big)
+    try:
+        for _ in range(3 * small.num_subsets):
+            batch.step()
+        batch.synchronize()
+    finally:
+        batch.close()
+    got = small.estimate()
+    for r in [small] + big:
+        r.close()
+    e = rel_l2(got, ref)
+    print("rel L2 vs f64 oracle %.3g" % e)
+    assert e <= 1e-4

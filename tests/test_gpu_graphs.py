@@ -167,3 +167,28 @@ def test_batch_with_a_shared_network_equals_each_alone():
         assert np.array_equal(r.estimate(), b.estimate())
         assert L.cnn_status(r) == L.cnn_status(b) and L.cnn_status(b)["active"]
     batch.close()
+
+
+def test_genprj_with_set_up_timings_equals_plain():
+    # IrlGenprj's forward projection has no subset table; the set-up timings
+    # (tile autotuning, the timed FP32 DRF algorithm choice) run on its first
+    # views and leave the projections as without them (up to the FFT
+    # kernel's rounding when the timing picks it)
+    print("IrlGenprj with autotune and the timed DRF choice", flush=True)
+    import libirl_amd as L
+    pb = Problem(n=32, nz=16, nviews=8, per=2, iters=1, half=3, noise=False)
+
+    def genprj():
+        out = np.zeros((pb.nviews, pb.nz, pb.n), np.float32)
+        rc = L.irl_genprj(pb.parms(), pb.options(), pb.views, pb.drf_file, None, None,
+                          pb.act.copy(), pb.mu.copy(), out)
+        assert rc == 0, L.error_string()
+        return out
+    plain = genprj()
+    prev = (L.set_autotune(True), L.set_drf_fft("auto"))
+    try:
+        timed = genprj()
+    finally:
+        L.set_autotune(prev[0])
+        L.set_drf_fft(prev[1])
+    assert np.abs(timed - plain).max() <= 1e-5 * np.abs(plain).max()

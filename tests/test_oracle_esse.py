@@ -181,3 +181,41 @@ def test_expand_smooths_bins_and_slices_in_place():
         got = e.effsrc(act, mp) * (MU0 * PIX) / 0.05
         np.testing.assert_allclose(np.moveaxis(got, axis, 0)[:, 3, 5], prof, rtol=1e-6,
                                    atol=1e-12, err_msg="axis %d" % axis)
+
+
+def test_column_evaluation_equals_the_full_model():
+    # the sampled-column form (used where the full model is too slow) gives
+    # the full model's values on those columns, for two and three terms
+    from libirl_amd import synth
+    rng = np.random.default_rng(3)
+    n, nz = 24, 12
+    k, km = synth.gaussian_mixture_krnl(5, 9, PIX)
+    act = rng.random((n, nz, n))
+    mp = rng.random((n, nz, n)) * 0.08
+    cols = [(int(z), int(x)) for z, x in zip(rng.integers(0, nz, 40), rng.integers(0, n, 40))]
+    for nterms in (2, 3):
+        e = O.Esse(k, km, MU0, PIX, n, nz, nterms=nterms, cntrpixfac=0.2)
+        full = e.effsrc(act, mp)
+        got = e.effsrc_columns(act, mp, cols)
+        want = np.stack([full[:, z, x] for z, x in cols])
+        np.testing.assert_allclose(got, want, rtol=1e-11, atol=1e-14)
+
+
+def test_column_projection_equals_project():
+    # Project with attenuation and ESSE (no DRF) on sampled columns equals
+    # the oracle's whole Project of the view
+    from libirl_amd import synth
+    rng = np.random.default_rng(4)
+    n, nz = 32, 8
+    k, km = synth.gaussian_mixture_krnl(5, 9, PIX)
+    e = O.Esse(k, km, MU0, PIX, n, nz)
+    act = rng.random((n, nz, n))
+    atn = rng.random((n, nz, n)) * 0.07
+    O.init_prj(nz, n, n, 7)
+    cols = [(int(z), int(x)) for z, x in zip(rng.integers(0, nz, 30), rng.integers(0, n, 30))]
+    for fast in (0, 1):
+        for angle in (-0.3, 2.1):
+            full = O.project(nz, n, n, 5, PIX, angle, 30.0, None, atn, act, esse=e.h, fast=fast)
+            got = O.project_srf_columns(n, nz, angle, e, atn, act, cols, fast=fast)
+            want = np.array([full[z, x] for z, x in cols])
+            np.testing.assert_allclose(got, want, rtol=1e-12)
