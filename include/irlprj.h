@@ -105,6 +105,20 @@ void gendrf(int Type, float Rfan, float fColThick, float fHoleDiam, float fGap, 
 void ConvolveDrfYpad(float *pfIn, int inX, int inY, Drf_t *psDrf, int bFFTConvolve,
                      int iYpad, int iYavg, float *pfOut);
 
+/* No counterpart in the reference.  ConvolveDrfYpad for callers that cannot
+ * allocate device memory (tight serving loops, harnesses that forbid
+ * hipMalloc, processes with HIP graph capture in progress elsewhere): the
+ * same kernel and result, with the PSF, the planes and the row tables staged
+ * in a file-scope device array of the library (one per GPU, 540 KB, held
+ * until the process ends) instead of allocated buffers.  Limits: inX * inY
+ * <= 65536 values, PSF at most 63 x 63 (MaxOffsX, MaxOffsY <= 32).  Runs on
+ * the current device's null stream and returns once pfOut holds the result.
+ * Calls that overlap on one GPU share its array and race.  Returns 0, or
+ * nonzero with pchIrlErrorString() set (sizes beyond the limits, NULL or
+ * invalid DRF). */
+int ConvolveDrfYpadNoAlloc(float *pfIn, int inX, int inY, Drf_t *psDrf, int bFFTConvolve,
+                           int iYpad, int iYavg, float *pfOut);
+
 DrfTab_t sReadDrfTab(char *pchInFname, char **ppchInTitle, char **ppchInColName);
 
 void vWriteDrfTab(DrfTab_t *psOutDrfTab, char *pchOutFname, char *pchTitle,

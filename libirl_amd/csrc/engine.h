@@ -266,6 +266,10 @@ void prjws_init(int Nz, int N, int NRot, int model);
 void prjws_free();
 void prjws_project(const PrjCall &c, const float *atn, const float *recon, float *calcprj);
 void prjws_backproject(const PrjCall &c, const float *atn, const float *ratio, float *image);
+// The same with no device allocation (a file-scope device array holds the
+// staging); psf is KY x KX, x fastest.
+void dev_convolve_plane_noalloc(const float *in, int inX, int inY, const float *psf, int KX, int KY,
+                                int pad, int avg, float *out);
 void dev_convolve_plane(const float *in, int inX, int inY, const PaddedDrf &d, int pad, int avg,
                         float *out);
 

@@ -41,6 +41,7 @@
 #include <algorithm>
 #include <type_traits>
 #include <map>
+#include <mutex>
 #include <set>
 #include <vector>
 
@@ -2898,6 +2899,87 @@ void dev_convolve_plane(const float *in, int inX, int inY, const PaddedDrf &d, i
     dout.download(out, pl, st);
     HIP_CHECK(hipStreamSynchronize(st));
     HIP_CHECK(hipStreamDestroy(st));
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free plane convolution (ConvolveDrfYpadNoAlloc).  The PSF (in
+// the kernel's padded column-major form), the input and output planes and
+// the row tables are staged in one file-scope device array; each GPU holds
+// its own copy until the process ends.  Host code never uses the array's
+// name as an address: the device address is looked up once per device with
+// hipGetSymbolAddress and kept.  Every call on a device shares that device's
+// array, so calls that overlap on one device race; nothing in it has to
+// start at zero (the kernel writes every output value).
+// ---------------------------------------------------------------------------
+constexpr int NOALLOC_MAX_PLANE = 256 * 256;   // inX * inY
+constexpr int NOALLOC_PSF = 63 * 64;           // KX <= 63 columns of KYP <= 64 taps
+constexpr int NOALLOC_TAB = 64;                // row tables (ints)
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) float scratch[NOALLOC_PSF + 2 * NOALLOC_MAX_PLANE + NOALLOC_TAB];
+
+static float *noalloc_scratch()
+{
+    static float *ptr[NOALLOC_MAX_DEVICES] = {};
+    static std::mutex m;
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+        fail(ETYPE_ILLEGAL_VALUE, "device %d beyond the %d the allocation-free path serves", dev,
+             NOALLOC_MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(m);
+    if (!ptr[dev]) {
+        void *p = nullptr;
+        HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        ptr[dev] = (float *)p;
+    }
+    return ptr[dev];
+}
+
+void dev_convolve_plane_noalloc(const float *in, int inX, int inY, const float *psf, int KX, int KY,
+                                int pad, int avg, float *out)
+{
+    if (inX <= 0 || inY <= 0 || (long)inX * inY > NOALLOC_MAX_PLANE)
+        fail(ETYPE_ILLEGAL_VALUE,
+             "plane of %d x %d values: the allocation-free convolution takes at most %d", inX,
+             inY, NOALLOC_MAX_PLANE);
+    if (KX < 1 || KY < 1 || KX > 63 || KY > 63)
+        fail(ETYPE_ILLEGAL_VALUE, "PSF of %d x %d: the allocation-free convolution takes at most "
+                                  "63 x 63", KX, KY);
+    const int KYb = PrjWs::bucket(KY), KYP = (KYb + 3) & ~3, oy = (KYb - KY) / 2;
+    float h[NOALLOC_PSF];
+    std::fill(h, h + (size_t)KX * KYP, 0.0f);
+    for (int j = 0; j < KY; ++j)
+        for (int i = 0; i < KX; ++i) h[(size_t)i * KYP + j + oy] = psf[(size_t)j * KX + i];
+    float *dpsf = noalloc_scratch();
+    float *din = dpsf + NOALLOC_PSF, *dout = din + NOALLOC_MAX_PLANE;
+    int *tab = (int *)(dout + NOALLOC_MAX_PLANE);
+    const int htab[3] = {0, pad, 0};   // DRF index of the row, its axial pad, view 0
+    const long pl = (long)inX * inY;
+    hipStream_t st = nullptr;          // the current device's null stream
+    HIP_CHECK(hipMemcpyAsync(dpsf, h, sizeof(float) * KX * KYP, hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipMemcpyAsync(din, in, sizeof(float) * pl, hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipMemcpyAsync(tab, htab, sizeof htab, hipMemcpyHostToDevice, st));
+    DrfArgs da{};
+    da.NRot = inX;
+    da.nrows = 1;
+    da.Nz = inY;
+    da.KX = KX;
+    da.psf = dpsf;
+    da.rowKernel = tab;
+    da.rowPad = tab + 1;
+    da.avg = std::max(0, std::min(avg, inY));
+    da.bview = tab + 2;
+    da.in = din;
+    da.zs = inX;
+    da.xs = 1;
+    da.out = dout;
+    da.outZS = inX;
+    da.outXS = 1;
+    da.scale = 1.0f;
+    da.xtiles = div_up(inX, DRF_TX);
+    launch_drf(st, true, da, 1, KYb);
+    HIP_CHECK(hipMemcpyAsync(out, dout, sizeof(float) * pl, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
 }
 
 static size_t drf_lds_bytes(int ZG, int KYb, int KX)

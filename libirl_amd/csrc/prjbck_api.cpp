@@ -214,6 +214,33 @@ void ConvolveDrfYpad(float *pfIn, int inX, int inY, Drf_t *psDrf, int bFFTConvol
     }
 }
 
+// ConvolveDrfYpad with no device allocation (no counterpart in the
+// reference): the same kernel and arguments, staged in the library's
+// file-scope device array.
+int ConvolveDrfYpadNoAlloc(float *pfIn, int inX, int inY, Drf_t *psDrf, int bFFTConvolve,
+                           int iYpad, int iYavg, float *pfOut)
+{
+    set_last_error(0);
+    try {
+        if (!psDrf) fail(ETYPE_ILLEGAL_VALUE, "can't convolve using NULL DRF");
+        if (psDrf->bFFTdone) fail(ETYPE_ILLEGAL_VALUE, "DRF is already Fourier transformed");
+        if (psDrf->MaxOffsX < 1 || psDrf->MaxOffsY < 1)
+            fail(ETYPE_ILLEGAL_VALUE, "DRF has illegal offsets %d x %d", psDrf->MaxOffsX,
+                 psDrf->MaxOffsY);
+        int pad = std::min(std::max(iYpad, 0), psDrf->MaxOffsY - 1);
+        if (bFFTConvolve) {
+            pad = 0;
+            iYavg = 0;
+        }
+        dev_convolve_plane_noalloc(pfIn, inX, inY, psDrf->Psf, 2 * psDrf->MaxOffsX - 1,
+                                   2 * psDrf->MaxOffsY - 1, pad, iYavg, pfOut);
+        return 0;
+    } catch (const IrlError &e) {
+        report(e, "ConvolveDrfYpadNoAlloc");
+        return e.code;
+    }
+}
+
 }  // extern "C"
 
 extern "C" {
