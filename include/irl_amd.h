@@ -59,11 +59,23 @@ int irlamd_last_error(void);
 
 /* Arithmetic of the DRF convolution for reconstructions created afterwards
  * (irlamd_recon_create, IrlOsem, IrlGenprj): 0 = FP32 (the default, the
- * reference's arithmetic), 1 = BF16 MFMA with FP32 accumulation, using hi+lo
- * split BF16 operands when the estimated per-plane relative error exceeds
- * 1e-3, 2 = BF16 hi+lo always, 3 = plain BF16 always.  Modes 1-3 need
+ * reference's arithmetic), 1 = BF16 MFMA with FP32 accumulation, per DRF
+ * table entry the cheapest of plain BF16, hi+lo split weights, and hi+lo
+ * split weights and data whose relative error, measured at setup on a probe
+ * plane against the FP32 kernel, is at most 1e-3 (irlamd_recon_bf16_planes),
+ * 2 = BF16 hi+lo weights and data always, 3 = plain BF16 always.  Modes 1-3 need
  * NumSlices % 4 == 0.  Returns the previous mode, or -1 for an invalid one. */
 int irlamd_set_precision(int iMode);
+
+/* BF16 DRF arithmetic of a reconstruction, per DRF table entry (the last one
+ * is the centred delta used for rows added unconvolved): piModes[k] 0 =
+ * plain BF16, 1 = PSF weights as hi+lo BF16, 2 = weights and data as hi+lo;
+ * pfErr[2k], pfErr[2k+1] = the relative L2 errors of plain BF16 and of the
+ * weight split measured on a probe plane against the FP32 kernel (mode 1
+ * takes the cheapest arithmetic within 1e-3; zeros for modes 2 and 3).
+ * Writes at most iCap entries; returns the number of entries (0 on the
+ * FP32 path), or -1 on error. */
+int irlamd_recon_bf16_planes(irlamd_recon *r, int *piModes, float *pfErr, int iCap);
 
 /* Reconstructions created afterwards replay each sub-iteration from a
  * captured HIP graph from the second occurrence of its subset and scatter

@@ -200,6 +200,8 @@ def load():
     lib.ComputeOrbit.restype = ctypes.c_int
     lib.irlamd_set_precision.argtypes = [ctypes.c_int]
     lib.irlamd_set_precision.restype = ctypes.c_int
+    lib.irlamd_recon_bf16_planes.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), _FP,
+                                             ctypes.c_int]
     vp = ctypes.c_void_p
     lib.irlamd_cnn_train.restype = vp
     lib.irlamd_cnn_train.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_float,
@@ -468,6 +470,18 @@ class Recon:
                                               ctypes.byref(n))
         name = ("spatial", "fft")
         return name[a.value], name[b.value], n.value
+
+    def bf16_planes(self):
+        """(modes, errors) of the BF16 DRF per table entry (delta entry last):
+        modes 0 plain, 1 weights hi+lo, 2 weights and data hi+lo; errors
+        [entry][plain, weight split] measured on a probe plane."""
+        n = self._lib.irlamd_recon_bf16_planes(self._h, None, None, 0)
+        m = np.zeros(max(n, 0), np.int32)
+        e = np.zeros((max(n, 0), 2), np.float32)
+        if n > 0:
+            self._lib.irlamd_recon_bf16_planes(self._h, m.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                               fptr(e), n)
+        return m, e
 
     def set_timing(self, on=True):
         self._lib.irlamd_recon_set_timing(self._h, 1 if on else 0)

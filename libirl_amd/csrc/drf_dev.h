@@ -34,6 +34,11 @@ struct DrfArgs {
     int xtiles, ztiles;
     const int2 *colRange;         // per table entry: first and last PSF column holding a
                                   // nonzero tap (null: all KX columns)
+    // BF16 kernel: rows taken by this launch (-1 all; 0 rows whose entry is
+    // not data-split; 1 the data-split ones), and PRJ partials added to
+    // instead of written (the second of two launches over the rows)
+    int rowSel;
+    int accumulate;
 };
 
 __device__ __forceinline__ float drf_in(const DrfArgs &a, int b, int v, int row, int g, int gx)

@@ -179,6 +179,11 @@ public:
     void tiles(int *prj, int *bck) const;
     // FP32 DRF algorithm in use per direction: 0 spatial, 1 FFT (length P)
     void drfAlgorithms(int *prj, int *bck, int *P) const;
+    // BF16 DRF arithmetic per table entry (the delta of unconvolved rows
+    // last): 0 plain, 1 weights hi+lo, 2 weights and data hi+lo, with the
+    // measured relative errors of plain BF16 and of the weight split (zeros
+    // when not measured).  Empty on the FP32 path.
+    void bf16Planes(std::vector<int> &modes, std::vector<float> &err) const;
 
     // One view at a time (for per-angle hooks): zeroUpdate at the start of
     // the subset, then member m of subset s is projected and backprojected

@@ -663,23 +663,28 @@ __global__ __launch_bounds__(64 * ZG) void k_drf(DrfArgs a)
 // k_drf_bf16: the DRF convolution of k_drf as BF16 MFMA contractions with
 // FP32 accumulation (the precision variant).  For PSF column i the
 // convolution along the slices is a banded Toeplitz product:
-//   out[z0+m][x] += sum_k A_i[m][k] * tile[16t + k][x + CX - i],
-//   A_i[m][k] = psf[m + KY - 1 - k][i]  (zero outside the band),
+//   out[z0+16t+m][x] += sum_k A_i[m][k] * tile[16t + k][x + CX - i],
+//   A_i[m][k] = psf[m + KY - k][i]  (zero outside the band),
+// tile row q holding slice z0 - CY - 1 + q,
 // one v_mfma_f32_16x16x32_bf16 per 32 rows of the window (M = 16 output
 // slices, N = 16 bins, K = window slices).  The A fragments of every table
 // entry are built on the host; B fragments are 128-bit reads of the bf16
-// tile (slices contiguous per bin).  For the table entries flagged in
-// distSplit (SPLIT instantiation only) hi and lo bf16 parts of both operands
-// are kept and the two cross products added (3 MFMAs per step).  Rows that
-// the reference adds unconvolved (distance <= 0) use an extra table entry,
-// a centred delta, always split: hi*1 + lo*1.
+// tile (slices contiguous per bin).  Per table entry (distMode): plain
+// BF16; the weights as hi + lo bf16 (2 MFMAs per step); or weights and data
+// as hi + lo (SPLIT instantiation only, 3 MFMAs: hi*hi + hi*lo + lo*hi).
+// Rows that the reference adds unconvolved (distance <= 0) use an extra
+// table entry (mode 3) and are added exactly in FP32.
 // Block = 4 waves, wave w owns bins [16w, 16w+16) of the 64-bin tile and
 // NZT slice tiles of 16.  Engine layout only (slices fastest).
 // ---------------------------------------------------------------------------
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float floatx4_t __attribute__((ext_vector_type(4)));
 
-__host__ __device__ constexpr int drf16_nks(int KY) { return (16 + KY - 1 + 31) / 32; }
+// K steps of 32 window rows: a tile of 16 output slices needs 16 + KY - 1
+// input rows, staged from one row above the PSF's reach (z0 - CY - 1, a
+// multiple of 4 for every supported KY: CY = 3 mod 4) so that groups of four
+// slices are 16-byte aligned loads
+__host__ __device__ constexpr int drf16_nks(int KY) { return (16 + KY + 31) / 32; }
 __host__ __device__ constexpr int drf16_rows(int KY, int NZT) { return 16 * (NZT - 1) + 32 * drf16_nks(KY); }
 __host__ __device__ constexpr int drf16_cs(int rows) { return ((rows + 55) / 64) * 64 + 8; }
 
@@ -689,10 +694,88 @@ __device__ __forceinline__ uint32_t pack_bf16(__bf16 lo, __bf16 hi)
            ((uint32_t)__builtin_bit_cast(uint16_t, hi) << 16);
 }
 
+// PSF columns i0..i1 of one staged plane: Toeplitz products of the entry's A
+// fragments with the B fragments of the bf16 tile, added into acc; with AS
+// the lo parts of the weights add a second product (weights to ~16 bits).
+// colBase = 16w + n + KX - 1 (the lane's tile column for PSF column 0),
+// rowOff = ks + 8g (the lane's first window row).
+template <int NZT, int NKS, bool AS>
+__device__ __forceinline__ void drf16_columns(floatx4_t (&acc)[NZT], const uint4 *ap,
+                                              const uint4 *apl, int i0, int i1, int nks,
+                                              const uint32_t *tileH, int colBase, int CSB,
+                                              int rowOff)
+{
+    if (nks == 1) {
+        // one K step: the NZT B fragments of a column are read once
+        bf16x8_t A = __builtin_bit_cast(bf16x8_t, ap[i0 * NKS * 64]), An = A;
+        bf16x8_t AL = A, ALn = A;
+        if (AS) AL = ALn = __builtin_bit_cast(bf16x8_t, apl[i0 * NKS * 64]);
+        for (int i = i0; i <= i1; ++i) {
+            if (i < i1) {
+                An = __builtin_bit_cast(bf16x8_t, ap[(i + 1) * NKS * 64]);
+                if (AS) ALn = __builtin_bit_cast(bf16x8_t, apl[(i + 1) * NKS * 64]);
+            }
+            const uint32_t *ch = tileH + ((colBase - i) * CSB + rowOff) / 2;
+            bf16x8_t B[NZT];
+#pragma unroll
+            for (int u = 0; u < NZT; ++u)
+                B[u] = __builtin_bit_cast(bf16x8_t, *(const uint4 *)(ch + 8 * u));
+#pragma unroll
+            for (int t = 0; t < NZT; ++t) {
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[t], acc[t], 0, 0, 0);
+                if (AS) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(AL, B[t], acc[t], 0, 0, 0);
+            }
+            A = An;
+            if (AS) AL = ALn;
+        }
+    } else {
+        // the window of tile t, step s starts at row ks + 16(t + 2s): the
+        // NZT + 2(NKS-1) distinct B fragments of a column are read once
+        constexpr int NB = NZT + 2 * (NKS - 1);
+        bf16x8_t A[NKS], An[NKS], AL[NKS], ALn[NKS];
+#pragma unroll
+        for (int s = 0; s < NKS; ++s) {
+            A[s] = __builtin_bit_cast(bf16x8_t, ap[(i0 * NKS + s) * 64]);
+            AL[s] = AS ? __builtin_bit_cast(bf16x8_t, apl[(i0 * NKS + s) * 64]) : A[s];
+        }
+        for (int i = i0; i <= i1; ++i) {
+            if (i < i1) {
+#pragma unroll
+                for (int s = 0; s < NKS; ++s) {
+                    An[s] = __builtin_bit_cast(bf16x8_t, ap[((i + 1) * NKS + s) * 64]);
+                    if (AS) ALn[s] = __builtin_bit_cast(bf16x8_t, apl[((i + 1) * NKS + s) * 64]);
+                }
+            }
+            const uint32_t *ch = tileH + ((colBase - i) * CSB + rowOff) / 2;
+            bf16x8_t B[NB];
+#pragma unroll
+            for (int u = 0; u < NB; ++u)
+                B[u] = __builtin_bit_cast(bf16x8_t, *(const uint4 *)(ch + 8 * u));
+#pragma unroll
+            for (int t = 0; t < NZT; ++t)
+#pragma unroll
+                for (int s = 0; s < NKS; ++s) {
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[s], B[t + 2 * s], acc[t], 0,
+                                                                     0, 0);
+                    if (AS)
+                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(AL[s], B[t + 2 * s],
+                                                                         acc[t], 0, 0, 0);
+                }
+#pragma unroll
+            for (int s = 0; s < NKS; ++s) {
+                A[s] = An[s];
+                if (AS) AL[s] = ALn[s];
+            }
+        }
+    }
+}
+
+// launch bounds: at least two blocks per CU, which holds the projector's
+// registers (accumulators live across its rows) to 128 without spills
 template <int KY, int NZT, bool BCK, bool SPLIT>
-__global__ __launch_bounds__(256) void k_drf_bf16(DrfArgs a, const uint4 *__restrict__ afr,
+__global__ __launch_bounds__(256, 2) void k_drf_bf16(DrfArgs a, const uint4 *__restrict__ afr,
                                                   const uint4 *__restrict__ afrLo,
-                                                  const int *__restrict__ distSplit,
+                                                  const int *__restrict__ distMode,
                                                   const int4 *__restrict__ band)
 {
     extern __shared__ uint32_t ldsw[];
@@ -701,6 +784,7 @@ __global__ __launch_bounds__(256) void k_drf_bf16(DrfArgs a, const uint4 *__rest
     constexpr int TZ = 16 * NZT;
     constexpr int THP = drf16_rows(KY, NZT);   // staged rows (window of the last tile)
     constexpr int CSB = drf16_cs(THP);          // bf16 column stride: 4 mod 32 dwords
+    constexpr int NBAT = 6;                     // staged groups whose loads issue together
     const int KX = a.KX;
     const int CX = (KX - 1) / 2;
     const int TW = DRF_TX + KX - 1;
@@ -732,71 +816,128 @@ __global__ __launch_bounds__(256) void k_drf_bf16(DrfArgs a, const uint4 *__rest
     floatx4_t acc[NZT];
 #pragma unroll
     for (int t = 0; t < NZT; ++t) acc[t] = floatx4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    int taken = 0;
 
     for (int row = r0; row < r1; ++row) {
         const int kidx = a.rowKernel[(long)v * a.nrows + row];
+        // per table entry: 0 plain BF16, 1 weights as hi+lo, 2 weights and
+        // data as hi+lo (the SPLIT instantiation stages the data's lo parts),
+        // 3 the delta of rows added unconvolved: added exactly in FP32
+        const int md = distMode[kidx];
+        if (a.rowSel >= 0 && (md == 2) != (a.rowSel == 1)) continue;
+        ++taken;
         if (BCK) {
 #pragma unroll
             for (int t = 0; t < NZT; ++t) acc[t] = floatx4_t{0.0f, 0.0f, 0.0f, 0.0f};
         }
-        {
+        if (md == 3) {
+            // the lane's outputs (slices z0+16t+4g..+3 of bin x), unconvolved
+            if (x < a.NRot) {
+                const long ib = (long)b * a.inB + (long)row * a.inRow + (long)x * a.xs;
+                const long fb = (long)v * a.afB + (long)row * a.afRow + (long)x * a.xs;
+#pragma unroll
+                for (int t = 0; t < NZT; ++t)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int z = z0 + 16 * t + 4 * g + j;
+                        if (z < a.Nz) {
+                            float f = a.in[ib + (long)z * a.zs];
+                            if (a.afac) f = f * a.afac[fb + (long)z * a.zs];
+                            acc[t][j] += f;
+                        }
+                    }
+            }
+        } else {
             const int pad = a.rowPad[(long)v * a.nrows + row];
-            // hi+lo split for the planes whose estimated BF16 error is too large
-            const bool sp = SPLIT && distSplit[kidx];
+            const bool spd = SPLIT && md >= 2;
+            const bool spa = md >= 1;
             // the entry's band: PSF columns [i0, i1] and window rows from ks
             // (multiple of 8) over nks K steps; outside it the A fragments are zero
             const int4 bd = band[kidx];
             const int i0 = bd.x, i1 = bd.y, ks = bd.z, nks = bd.w;
             const int srows = 16 * (NZT - 1) + 32 * nks;   // staged rows from ks
             const int c0 = KX - 1 - i1, nc = DRF_TX + i1 - i0;   // staged columns
-            // stage pairs of consecutive slices as packed bf16 (hi, and lo);
-            // element e = (column cc, pair qi) advances by 256 per step with
-            // (cc, qi) updated incrementally (no per-element division)
-            const int sp2 = srows / 2, total = sp2 * nc;
-            const int dcc = 256 / sp2, dqi = 256 - dcc * sp2;
-            int cc = (int)threadIdx.x / sp2, qi = (int)threadIdx.x - cc * sp2;
+            // stage groups of four consecutive slices as packed bf16 (hi, and
+            // lo), window row q <-> slice z0 - CY - 1 + q (a multiple of 4: one
+            // aligned float4 per group, wholly inside or outside [0, Nz) as
+            // Nz % 4 == 0); element e = (column cc, group qi) advances by 256
+            // per step with (cc, qi) updated incrementally (no division)
+            const int sp4 = srows / 4, total = sp4 * nc;
+            const int dcc = 256 / sp4, dqi = 256 - dcc * sp4;
+            int cc = (int)threadIdx.x / sp4, qi = (int)threadIdx.x - cc * sp4;
+            const int zb = z0 - CY - 1;
             // without an axial pad (and in-range bins) a value is the input
             // times the factor, or zero outside the slices
             const bool plain = pad <= 0 || a.avg <= 0;
             const long ibase = (long)b * a.inB + (long)row * a.inRow;
             const long fbase = (long)v * a.afB + (long)row * a.afRow;
-            for (int e = threadIdx.x; e < total; e += 256) {
-                const int c = c0 + cc, q = ks + 2 * qi;
-                const int gx = x0 - CX + c, g0 = z0 - CY + q;
-                float f0, f1;
-                if (plain) {
-                    f0 = f1 = 0.0f;
-                    if (gx >= 0 && gx < a.NRot) {
-                        const long o = (long)gx * a.xs;
-                        if (g0 >= 0 && g0 < a.Nz) {
-                            f0 = a.in[ibase + o + (long)g0 * a.zs];
-                            if (a.afac) f0 = f0 * a.afac[fbase + o + (long)g0 * a.zs];
-                        }
-                        if (g0 + 1 >= 0 && g0 + 1 < a.Nz) {
-                            f1 = a.in[ibase + o + (long)(g0 + 1) * a.zs];
-                            if (a.afac) f1 = f1 * a.afac[fbase + o + (long)(g0 + 1) * a.zs];
+            if (plain) {
+                // NBAT groups per thread at a time: every load of the batch is
+                // issued before the first value is used, so a thread waits for
+                // memory once per batch instead of once per group
+                for (int e0 = threadIdx.x; e0 < total; e0 += 256 * NBAT) {
+                    float4 f[NBAT], m[NBAT];
+                    int li[NBAT];
+#pragma unroll
+                    for (int u = 0; u < NBAT; ++u) {
+                        const int c = c0 + cc, q = ks + 4 * qi;
+                        const int gx = x0 - CX + c, gz = zb + q;
+                        li[u] = (c * CSB + q) / 2;
+                        const bool ok = e0 + 256 * u < total && gx >= 0 && gx < a.NRot &&
+                                        gz >= 0 && gz < a.Nz;
+                        const long o = (long)gx * a.xs + gz;
+                        f[u] = ok ? *(const float4 *)(a.in + ibase + o) : make_float4(0, 0, 0, 0);
+                        m[u] = (ok && a.afac) ? *(const float4 *)(a.afac + fbase + o)
+                                              : make_float4(1, 1, 1, 1);
+                        cc += dcc;
+                        qi += dqi;
+                        if (qi >= sp4) {
+                            qi -= sp4;
+                            ++cc;
                         }
                     }
-                } else {
-                    f0 = drf_stage_val(a, b, v, row, g0, gx, pad);
-                    f1 = drf_stage_val(a, b, v, row, g0 + 1, gx, pad);
+#pragma unroll
+                    for (int u = 0; u < NBAT; ++u) {
+                        if (e0 + 256 * u < total) {
+                            const float v0 = f[u].x * m[u].x, v1 = f[u].y * m[u].y;
+                            const float v2 = f[u].z * m[u].z, v3 = f[u].w * m[u].w;
+                            const __bf16 h0 = (__bf16)v0, h1 = (__bf16)v1;
+                            const __bf16 h2 = (__bf16)v2, h3 = (__bf16)v3;
+                            *(uint2 *)(tileH + li[u]) = make_uint2(pack_bf16(h0, h1), pack_bf16(h2, h3));
+                            if (spd)
+                                *(uint2 *)(tileL + li[u]) = make_uint2(
+                                    pack_bf16((__bf16)(v0 - (float)h0), (__bf16)(v1 - (float)h1)),
+                                    pack_bf16((__bf16)(v2 - (float)h2), (__bf16)(v3 - (float)h3)));
+                        }
+                    }
                 }
-                const __bf16 h0 = (__bf16)f0, h1 = (__bf16)f1;
-                tileH[(c * CSB + q) / 2] = pack_bf16(h0, h1);
-                if (sp)
-                    tileL[(c * CSB + q) / 2] =
-                        pack_bf16((__bf16)(f0 - (float)h0), (__bf16)(f1 - (float)h1));
-                cc += dcc;
-                qi += dqi;
-                if (qi >= sp2) {
-                    qi -= sp2;
-                    ++cc;
+            } else {
+                for (int e = threadIdx.x; e < total; e += 256) {
+                    const int c = c0 + cc, q = ks + 4 * qi;
+                    const int gx = x0 - CX + c, gz = zb + q;
+                    float fv[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) fv[j] = drf_stage_val(a, b, v, row, gz + j, gx, pad);
+                    const __bf16 h0 = (__bf16)fv[0], h1 = (__bf16)fv[1];
+                    const __bf16 h2 = (__bf16)fv[2], h3 = (__bf16)fv[3];
+                    *(uint2 *)(tileH + (c * CSB + q) / 2) = make_uint2(pack_bf16(h0, h1), pack_bf16(h2, h3));
+                    if (spd)
+                        *(uint2 *)(tileL + (c * CSB + q) / 2) = make_uint2(
+                            pack_bf16((__bf16)(fv[0] - (float)h0), (__bf16)(fv[1] - (float)h1)),
+                            pack_bf16((__bf16)(fv[2] - (float)h2), (__bf16)(fv[3] - (float)h3)));
+                    cc += dcc;
+                    qi += dqi;
+                    if (qi >= sp4) {
+                        qi -= sp4;
+                        ++cc;
+                    }
                 }
             }
             __syncthreads();
             const uint4 *ap = afr + (long)kidx * KX * NKS * 64 + lane;
             const uint4 *apl = afrLo + (long)kidx * KX * NKS * 64 + lane;
-            if (sp) {
+            const int colBase = 16 * w + n + KX - 1, rowOff = ks + 8 * g;
+            if (spd) {
                 for (int i = i0; i <= i1; ++i) {
                     bf16x8_t A[NKS], AL[NKS];
 #pragma unroll
@@ -804,9 +945,8 @@ __global__ __launch_bounds__(256) void k_drf_bf16(DrfArgs a, const uint4 *__rest
                         A[s] = __builtin_bit_cast(bf16x8_t, ap[(i * NKS + s) * 64]);
                         AL[s] = __builtin_bit_cast(bf16x8_t, apl[(i * NKS + s) * 64]);
                     }
-                    const int col = 16 * w + n + KX - 1 - i;
-                    const uint32_t *ch = tileH + (col * CSB + ks + 8 * g) / 2;
-                    const uint32_t *cl = tileL + (col * CSB + ks + 8 * g) / 2;
+                    const uint32_t *ch = tileH + ((colBase - i) * CSB + rowOff) / 2;
+                    const uint32_t *cl = tileL + ((colBase - i) * CSB + rowOff) / 2;
 #pragma unroll
                     for (int t = 0; t < NZT; ++t)
 #pragma unroll
@@ -821,50 +961,12 @@ __global__ __launch_bounds__(256) void k_drf_bf16(DrfArgs a, const uint4 *__rest
                             acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(AL[s], B, acc[t], 0, 0, 0);
                         }
                 }
-            } else if (nks == 1) {
-                // one K step: the NZT B fragments of a column are read once
-                bf16x8_t A = __builtin_bit_cast(bf16x8_t, ap[i0 * NKS * 64]), An = A;
-                for (int i = i0; i <= i1; ++i) {
-                    if (i < i1) An = __builtin_bit_cast(bf16x8_t, ap[(i + 1) * NKS * 64]);
-                    const int col = 16 * w + n + KX - 1 - i;
-                    const uint32_t *ch = tileH + (col * CSB + ks + 8 * g) / 2;
-                    bf16x8_t B[NZT];
-#pragma unroll
-                    for (int u = 0; u < NZT; ++u)
-                        B[u] = __builtin_bit_cast(bf16x8_t, *(const uint4 *)(ch + 8 * u));
-#pragma unroll
-                    for (int t = 0; t < NZT; ++t)
-                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[t], acc[t], 0, 0, 0);
-                    A = An;
-                }
+            } else if (spa) {
+                drf16_columns<NZT, NKS, true>(acc, ap, apl, i0, i1, nks, tileH, colBase, CSB,
+                                              rowOff);
             } else {
-                // the window of tile t, step s starts at row ks + 16(t + 2s):
-                // the NZT + 2(NKS-1) distinct B fragments of a column are read once
-                constexpr int NB = NZT + 2 * (NKS - 1);
-                bf16x8_t A[NKS], An[NKS];
-#pragma unroll
-                for (int s = 0; s < NKS; ++s) A[s] = __builtin_bit_cast(bf16x8_t, ap[(i0 * NKS + s) * 64]);
-                for (int i = i0; i <= i1; ++i) {
-                    if (i < i1) {
-#pragma unroll
-                        for (int s = 0; s < NKS; ++s)
-                            An[s] = __builtin_bit_cast(bf16x8_t, ap[((i + 1) * NKS + s) * 64]);
-                    }
-                    const int col = 16 * w + n + KX - 1 - i;
-                    const uint32_t *ch = tileH + (col * CSB + ks + 8 * g) / 2;
-                    bf16x8_t B[NB];
-#pragma unroll
-                    for (int u = 0; u < NB; ++u)
-                        B[u] = __builtin_bit_cast(bf16x8_t, *(const uint4 *)(ch + 8 * u));
-#pragma unroll
-                    for (int t = 0; t < NZT; ++t)
-#pragma unroll
-                        for (int s = 0; s < NKS; ++s)
-                            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[s], B[t + 2 * s],
-                                                                             acc[t], 0, 0, 0);
-#pragma unroll
-                    for (int s = 0; s < NKS; ++s) A[s] = An[s];
-                }
+                drf16_columns<NZT, NKS, false>(acc, ap, apl, i0, i1, nks, tileH, colBase, CSB,
+                                               rowOff);
             }
             __syncthreads();
         }
@@ -879,13 +981,20 @@ __global__ __launch_bounds__(256) void k_drf_bf16(DrfArgs a, const uint4 *__rest
             }
         }
     }
-    if (!BCK && x < a.NRot) {
+    if (!BCK && x < a.NRot && (!a.accumulate || taken > 0)) {
         const int ch = r0 / a.rchunk;
         float *o = a.out + (long)b * a.outB + (long)ch * a.outRow + (long)x * a.outXS;
 #pragma unroll
         for (int t = 0; t < NZT; ++t) {
             const int z = z0 + 16 * t + 4 * g;
-            if (z < a.Nz) *(float4 *)(o + z) = make_float4(acc[t][0], acc[t][1], acc[t][2], acc[t][3]);
+            if (z < a.Nz) {
+                float4 r = make_float4(acc[t][0], acc[t][1], acc[t][2], acc[t][3]);
+                if (a.accumulate) {
+                    const float4 p = *(const float4 *)(o + z);
+                    r = make_float4(p.x + r.x, p.y + r.y, p.z + r.z, p.w + r.w);
+                }
+                *(float4 *)(o + z) = r;
+            }
         }
     }
 }
@@ -1205,8 +1314,10 @@ struct Recon::Impl {
     DBuf<int> rowKernel16;        // rowKernel with -1 replaced by the delta entry nDist
     DBuf<int> rowKernel16B;       // the same for the backprojector's rows
     DBuf<int> rowKernelB, rowPadB;   // backprojector's table rows (its own table, or copies)
-    bool bf16 = false, bf16Split = false;   // bf16Split: some entry of the table is split
-    int nSplit = 0, nUsed = 0;              // split / used table entries
+    bool bf16 = false, bf16Split = false;   // bf16Split: some entry splits its data
+    int nSplit = 0, nWsplit = 0, nUsed = 0; // entries with data / weights split, used entries
+    std::vector<int> bf16Modes;             // per table entry (the delta last): 0, 1, 2
+    std::vector<float> bf16Err;             // measured errors: plain, weights split, per entry
     // three-shear rotation tables, slot 2v (forward) and 2v+1 (back)
     ShearSet shset;
     ShearDev shear() const { return shset.dev(); }
@@ -1421,7 +1532,10 @@ struct Recon::Impl {
     std::string drf_name(const char *base, bool bck) const
     {
         if (!bf16) return (bck ? fftBck : fftPrj) ? std::string(base) + "_fft" : base;
-        const char *suf = nSplit == 0 ? "_bf16" : nSplit < nUsed ? "_bf16mix" : "_bf16x3";
+        const char *suf = nSplit == nUsed && nUsed > 0 ? "_bf16x3"
+                          : nSplit > 0                  ? "_bf16mix"
+                          : nWsplit > 0                 ? "_bf16w"
+                                                        : "_bf16";
         return std::string(base) + suf;
     }
 
@@ -1487,6 +1601,97 @@ struct Recon::Impl {
         (void)hipEventDestroy(e1);
     }
 
+    // Per-plane BF16 error of every DRF table entry (the last one the delta of
+    // the unconvolved rows), measured on the GPU: one probe plane (values in
+    // [0.5, 1.5) from a fixed hash, so the mantissas spread as in measured
+    // data) convolved with each entry in plain BF16 and with the weights
+    // split, against the FP32 kernel (the probe itself for the delta).  The
+    // cheapest mode whose relative L2 error is at most 1e-3 is kept: 0
+    // plain, 1 weights hi+lo, 2 weights and data hi+lo (not measured: its
+    // error is the FP32 one's).
+    void measure_bf16_modes(int nd, int KYb, std::vector<int> &modes)
+    {
+        const Geometry &g = c.g;
+        const long pl = (long)g.Nz * g.NRot;
+        const int rows = nd + 1;
+        std::vector<float> probe(pl);
+        for (long i = 0; i < pl; ++i) {
+            uint32_t h = (uint32_t)i * 2654435761u + 0x9e3779b9u;
+            h ^= h >> 15;
+            h *= 2246822519u;
+            h ^= h >> 13;
+            probe[i] = 0.5f + (float)(h >> 8) * (1.0f / 16777216.0f);
+        }
+        std::vector<int> rkB(rows), rkF(rows), zero(rows + 1, 0), m0(rows, 0), m1(rows, 1);
+        for (int k = 0; k < rows; ++k) {
+            rkB[k] = k;
+            rkF[k] = k < nd ? k : -1;   // the FP32 kernel adds a -1 row unconvolved
+        }
+        DBuf<float> dprobe, out;
+        DBuf<int> drkB, drkF, dzero, dm0, dm1;
+        dprobe.alloc(pl);
+        dprobe.upload(probe.data(), pl, stream);
+        out.alloc(pl * rows * 3);
+        drkB.alloc(rows);
+        drkB.upload(rkB.data(), rows, stream);
+        drkF.alloc(rows);
+        drkF.upload(rkF.data(), rows, stream);
+        dzero.alloc(rows + 1);
+        dzero.upload(zero.data(), rows + 1, stream);
+        dm0.alloc(rows);
+        dm0.upload(m0.data(), rows, stream);
+        dm1.alloc(rows);
+        dm1.upload(m1.data(), rows, stream);
+        DrfArgs d{};
+        d.NRot = g.NRot;
+        d.Nz = g.Nz;
+        d.nrows = rows;
+        d.KX = c.drf.KX;
+        d.psf = psf.p;
+        d.colRange = colRange.p;
+        d.rowPad = dzero.p;
+        d.avg = 0;
+        d.bview = dzero.p;
+        d.in = dprobe.p;
+        d.zs = 1;
+        d.xs = g.Nz;
+        d.scale = 1.0f;
+        d.outRow = pl;
+        d.outZS = 1;
+        d.outXS = g.Nz;
+        d.vec4 = g.Nz % 4 == 0;
+        d.xtiles = div_up(g.NRot, DRF_TX);
+        d.rowKernel = drkF.p;
+        d.out = out.p;
+        launch_drf(stream, true, d, 1, KYb, 0);
+        d.rowKernel = drkB.p;
+        d.rowSel = -1;
+        d.out = out.p + pl * rows;
+        launch_drf_bf16(stream, true, d, 1, KYb, false, afrHi.p, afrLo.p, dm0.p, drfBand.p, 0);
+        d.out = out.p + 2 * pl * rows;
+        launch_drf_bf16(stream, true, d, 1, KYb, false, afrHi.p, afrLo.p, dm1.p, drfBand.p, 0);
+        std::vector<float> h(pl * rows * 3);
+        out.download(h.data(), h.size(), stream);
+        HIP_CHECK(hipStreamSynchronize(stream));
+        modes.assign(rows, 2);
+        for (int k = 0; k < rows; ++k) {
+            const float *ref = &h[pl * k];
+            double rr = 0, e0 = 0, e1 = 0;
+            for (long i = 0; i < pl; ++i) {
+                const double r = ref[i];
+                rr += r * r;
+                e0 += (h[pl * (rows + k) + i] - r) * (h[pl * (rows + k) + i] - r);
+                e1 += (h[pl * (2 * rows + k) + i] - r) * (h[pl * (2 * rows + k) + i] - r);
+            }
+            const float err0 = rr > 0 ? (float)std::sqrt(e0 / rr) : 0.0f;
+            const float err1 = rr > 0 ? (float)std::sqrt(e1 / rr) : 0.0f;
+            bf16Err[2 * k] = err0;
+            bf16Err[2 * k + 1] = err1;
+            modes[k] = err0 <= 1e-3f ? 0 : err1 <= 1e-3f ? 1 : 2;
+        }
+        modes[nd] = 3;   // the delta: rows added unconvolved, exactly in FP32
+    }
+
     void run_drf(bool bck, const DrfArgs &args, int B, int KYb, int tile = -1)
     {
         if (tile < 0) tile = bck ? tileBck : tilePrj;
@@ -1494,10 +1699,16 @@ struct Recon::Impl {
             launch_drf_fft(stream, bck, args, B, fft);
             return;
         }
-        if (bf16)
-            launch_drf_bf16(stream, bck, args, B, KYb, bf16Split, afrHi.p, afrLo.p,
-                            distSplit.p, drfBand.p, tile);
-        else
+        if (bf16) {
+            // one launch over all rows (the SPLIT instantiation when some entry
+            // splits its data: separate launches for those rows measured slower,
+            // the second one's few blocks running their rows back to back)
+            DrfArgs a1 = args;
+            a1.rowSel = -1;
+            a1.accumulate = 0;
+            launch_drf_bf16(stream, bck, a1, B, KYb, bf16Split, afrHi.p, afrLo.p, distSplit.p,
+                            drfBand.p, tile);
+        } else
             launch_drf(stream, bck, args, B, KYb, tile);
     }
 };
@@ -1522,7 +1733,7 @@ static float bf2f(uint16_t h)
 
 // A fragments of k_drf_bf16 for every table entry, column i and K step s:
 // lane l holds A_i[m = l%16][k = ks + 32s + 8(l/16) + e], e < 8, with
-// A_i[m][k] = psf[m + KYb - 1 - k][i] inside the band (PSF centred in the
+// A_i[m][k] = psf[m + KYb - k][i] inside the band (PSF centred in the
 // KYb bucket), as bf16 hi parts and lo = bf16(value - hi).
 // Each entry is first cut to the smallest rectangle of its PSF outside which
 // the taps' absolute sum is at most 'cut' of the whole (edges with the
@@ -1579,7 +1790,8 @@ static void build_bf16_fragments(const PaddedDrf &d, int KYb, double cut, std::v
             else if (which == 2) ++c0;
             else --c1;
         }
-        const int klo = KYb - 1 - oy - r1, khi = 15 + KYb - 1 - oy - r0;
+        // window rows (staged from one row above the PSF's reach) of the band
+        const int klo = KYb - oy - r1, khi = 15 + KYb - oy - r0;
         int ks = (klo / 8) * 8, nks = 1;
         if (khi - ks >= 32 || ks + 32 > 32 * NKS) {
             ks = 0;
@@ -1600,7 +1812,7 @@ static void build_bf16_fragments(const PaddedDrf &d, int KYb, double cut, std::v
                     uint16_t h[8], w[8];
                     for (int e = 0; e < 8; ++e) {
                         const int m = l % 16, kk = ks + 32 * st + 8 * (l / 16) + e;
-                        const int j = m + KYb - 1 - kk - oy;
+                        const int j = m + KYb - kk - oy;
                         const float val =
                             (j >= r0 && j <= r1 && i >= c0 && i <= c1) ? P[j * d.KX + i] : 0.0f;
                         h[e] = f2bf(val);
@@ -1776,24 +1988,34 @@ Recon::Recon(const Config &cfg) : impl_(new Impl), cfg_(cfg)
                     if (r < 0) r = nd;
                     used[r] = 1;
                 }
-            I.nSplit = I.nUsed = 0;
-            for (int k = 0; k <= nd; ++k) {
-                split[k] = k == nd || cfg.precision == 2 || (cfg.precision == 1 && est[k] > 1e-3);
-                I.nUsed += used[k];
-                I.nSplit += used[k] && split[k];
-            }
             I.bf16 = true;
-            I.bf16Split = I.nSplit > 0;
             I.rowKernel16.alloc(rk16.size());
             I.rowKernel16.upload(rk16.data(), rk16.size(), I.stream);
             I.rowKernel16B.alloc(rk16B.size());
             I.rowKernel16B.upload(rk16B.data(), rk16B.size(), I.stream);
-            I.distSplit.alloc(split.size());
-            I.distSplit.upload(split.data(), split.size(), I.stream);
             I.afrHi.alloc(hi.size());
             I.afrHi.upload(hi.data(), hi.size(), I.stream);
             I.afrLo.alloc(lo.size());
             I.afrLo.upload(lo.data(), lo.size(), I.stream);
+            // per entry: 0 plain BF16, 1 weights as hi+lo, 2 weights and data
+            // as hi+lo.  Mode 1 (the per-plane choice) measures each entry's
+            // error on a probe plane and takes the cheapest mode within 1e-3.
+            I.bf16Err.assign(2 * (nd + 1), 0.0f);
+            if (cfg.precision == 1)
+                I.measure_bf16_modes(nd, KYb, split);
+            else
+                split.assign(nd + 1, cfg.precision == 2 ? 2 : 0);
+            split[nd] = 3;   // unconvolved rows: exact FP32 in every mode
+            I.nSplit = I.nWsplit = I.nUsed = 0;
+            for (int k = 0; k < nd; ++k) {
+                I.nUsed += used[k];
+                I.nSplit += used[k] && split[k] == 2;
+                I.nWsplit += used[k] && split[k] >= 1;
+            }
+            I.bf16Split = I.nSplit > 0;
+            I.bf16Modes = split;
+            I.distSplit.alloc(split.size());
+            I.distSplit.upload(split.data(), split.size(), I.stream);
             HIP_CHECK(hipStreamSynchronize(I.stream));   // host vectors die here
         }
     }
@@ -2158,6 +2380,12 @@ void Recon::drfAlgorithms(int *prj, int *bck, int *P) const
 }
 
 void *Recon::streamHandle() const { return (void *)impl_->stream; }
+
+void Recon::bf16Planes(std::vector<int> &modes, std::vector<float> &err) const
+{
+    modes = impl_->bf16Modes;
+    err = impl_->bf16Err;
+}
 
 bool Recon::learnedScatterActive() const { return impl_->cst.active; }
 

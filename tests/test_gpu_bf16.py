@@ -149,3 +149,32 @@ def test_irlosem_with_computed_grf(grf_in_bck, drf):
     e = rel_l2(recon, ref)
     print("rel L2 vs f32 oracle with the same tables: %.3g" % e)
     assert e <= 1e-4
+
+
+def test_per_plane_modes_follow_the_measured_errors():
+    # mode 'bf16': each DRF table entry takes the cheapest arithmetic whose
+    # error, measured at setup against the FP32 kernel, is at most 1e-3; the
+    # projections of the resulting table stay within that bound
+    print("BF16 per-plane modes from measured errors", flush=True)
+    import libirl_amd as L
+    pb = Problem(n=64, nz=64, nviews=4, per=2, half=7, pix=0.4, noise=False)
+    prev = L.set_precision("bf16")
+    try:
+        r = L.Recon(pb.parms(), pb.options(), pb.views, pb.drf_file, None, None, pb.mu, pb.prj,
+                    pb.x0)
+    finally:
+        L.set_precision(prev)
+    modes, err = r.bf16_planes()
+    r.close()
+    print("entries %d, modes %s, plain errors %.3g..%.3g" %
+          (len(modes), np.bincount(modes, minlength=3).tolist(), err[:, 0].min(), err[:, 0].max()))
+    assert len(modes) == len(pb.drf.offs) // 2 + 1
+    assert np.all(err[:, 0] > 0) and np.all(err[:, 0] < 0.05)
+    assert np.all((modes == 0) == (err[:, 0] <= 1e-3))
+    assert np.all((modes == 1) == ((err[:, 0] > 1e-3) & (err[:, 1] <= 1e-3)))
+    assert np.all(err[modes == 1, 1] < err[modes == 1, 0])
+    got = _genprj(L, pb, "bf16")
+    ref = _oracle_genprj(pb)
+    e = max(rel_l2(got[v], ref[v]) for v in range(pb.nviews))
+    print("projection rel L2 vs FP64 oracle %.3g" % e)
+    assert e <= 1e-3
