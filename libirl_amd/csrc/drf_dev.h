@@ -39,6 +39,10 @@ struct DrfArgs {
     // instead of written (the second of two launches over the rows)
     int rowSel;
     int accumulate;
+    // BF16 PRJ: the input planes of rows whose entry does not split its data
+    // hold bf16 values (the first half of the plane, same indexing), the
+    // others FP32
+    int inBf16;
 };
 
 __device__ __forceinline__ float drf_in(const DrfArgs &a, int b, int v, int row, int g, int gx)

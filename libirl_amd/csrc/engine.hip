@@ -350,6 +350,19 @@ static dim3 att_grid(int Nz, int width, int rows, int B)
                 (rows + ATT_ROWS - 1) / ATT_ROWS, B);
 }
 
+// Output of a plane of the BF16 projector's input: bf16 (the projector's own
+// rounding of the value, stored in the first half of the plane) where the
+// row's DRF entry is plain or splits only its weights, else FP32 (split
+// data, rows added unconvolved).  rk16/dmode null: FP32.
+__device__ __forceinline__ void att_store(float *att, long plane, long idx, float val, int v,
+                                          int Y, int NRot, const int *rk16, const int *dmode)
+{
+    if (rk16 && dmode[rk16[(long)v * NRot + Y]] <= 1)
+        ((__bf16 *)(att + plane))[idx] = (__bf16)val;
+    else
+        att[plane + idx] = val;
+}
+
 template <bool SHEAR>
 __global__ __launch_bounds__(256) void k_att(const float *__restrict__ x,
                                              const float *__restrict__ afac,
@@ -358,7 +371,9 @@ __global__ __launch_bounds__(256) void k_att(const float *__restrict__ x,
                                              const float *__restrict__ fsin,
                                              const float *__restrict__ fcos,
                                              const int *__restrict__ head,
-                                             const int *__restrict__ npix, ShearDev sd)
+                                             const int *__restrict__ npix, ShearDev sd,
+                                             const int *__restrict__ rk16 = nullptr,
+                                             const int *__restrict__ dmode = nullptr)
 {
     const PixIdx p = pix_index(Nz, NRot);
     if (!p.ok) return;
@@ -379,7 +394,8 @@ __global__ __launch_bounds__(256) void k_att(const float *__restrict__ x,
                              (float)Y - NewAxis, fsin[v], fcos[v], OldAxis);
         }
         if (afac) val = val * afac[(long)v * vol + i];
-        att[(long)p.b * vol + i] = val;
+        const long plane = (long)p.b * vol + (long)Y * NRot * Nz;
+        att_store(att, plane, i - (long)Y * NRot * Nz, val, v, Y, NRot, rk16, dmode);
     }
 }
 
@@ -400,7 +416,9 @@ __global__ __launch_bounds__(256) void k_att_src(const float *__restrict__ g0,
                                                  const float *__restrict__ fcos,
                                                  const int *__restrict__ head,
                                                  const int *__restrict__ npix, ShearDev sd,
-                                                 float invMu0)
+                                                 float invMu0,
+                                                 const int *__restrict__ rk16 = nullptr,
+                                                 const int *__restrict__ dmode = nullptr)
 {
     const PixIdx p = pix_index(Nz, NRot);
     if (!p.ok) return;
@@ -425,7 +443,9 @@ __global__ __launch_bounds__(256) void k_att_src(const float *__restrict__ g0,
         }
         const float af = afac[(long)v * vol + i];
         const float d = af > 0.0f ? -logf(af) * invMu0 : 0.0f;
-        att[(long)p.b * vol + i] = fmaf(d, v1, v0) * af;
+        const long plane = (long)p.b * vol + (long)Y * NRot * Nz;
+        att_store(att, plane, i - (long)Y * NRot * Nz, fmaf(d, v1, v0) * af, v, Y, NRot, rk16,
+                  dmode);
     }
 }
 
@@ -671,7 +691,8 @@ __global__ __launch_bounds__(64 * ZG) void k_drf(DrfArgs a)
 // entry are built on the host; B fragments are 128-bit reads of the bf16
 // tile (slices contiguous per bin).  Per table entry (distMode): plain
 // BF16; the weights as hi + lo bf16 (2 MFMAs per step); or weights and data
-// as hi + lo (SPLIT instantiation only, 3 MFMAs: hi*hi + hi*lo + lo*hi).
+// as hi + lo (3 MFMAs per step: hi*hi + lo*hi weights on the values' hi
+// parts, then a second staging pass of the lo parts times the hi weights).
 // Rows that the reference adds unconvolved (distance <= 0) use an extra
 // table entry (mode 3) and are added exactly in FP32.
 // Block = 4 waves, wave w owns bins [16w, 16w+16) of the 64-bin tile and
@@ -692,6 +713,20 @@ __device__ __forceinline__ uint32_t pack_bf16(__bf16 lo, __bf16 hi)
 {
     return (uint32_t)__builtin_bit_cast(uint16_t, lo) |
            ((uint32_t)__builtin_bit_cast(uint16_t, hi) << 16);
+}
+
+// Four consecutive slices' values as two dwords of packed bf16: the values
+// rounded to bf16 (pass 0) or the bf16 rounding of their remainders v - hi
+// (pass 1, the lo parts of a split).
+__device__ __forceinline__ uint2 drf16_pack4(const float (&v)[4], int pass)
+{
+    __bf16 h[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        h[j] = (__bf16)v[j];
+        if (pass) h[j] = (__bf16)(v[j] - (float)h[j]);
+    }
+    return make_uint2(pack_bf16(h[0], h[1]), pack_bf16(h[2], h[3]));
 }
 
 // PSF columns i0..i1 of one staged plane: Toeplitz products of the entry's A
@@ -770,9 +805,11 @@ __device__ __forceinline__ void drf16_columns(floatx4_t (&acc)[NZT], const uint4
     }
 }
 
-// launch bounds: at least two blocks per CU, which holds the projector's
-// registers (accumulators live across its rows) to 128 without spills
-template <int KY, int NZT, bool BCK, bool SPLIT>
+// Launch bounds: two blocks per CU at least.  The backprojector (one plane
+// per block) compiles to about 125 registers (4 waves per SIMD); the
+// projector's loop over its chunk's rows to about 230 (2 waves per SIMD),
+// which tighter bounds only turn into spills.
+template <int KY, int NZT, bool BCK>
 __global__ __launch_bounds__(256, 2) void k_drf_bf16(DrfArgs a, const uint4 *__restrict__ afr,
                                                   const uint4 *__restrict__ afrLo,
                                                   const int *__restrict__ distMode,
@@ -789,7 +826,6 @@ __global__ __launch_bounds__(256, 2) void k_drf_bf16(DrfArgs a, const uint4 *__r
     const int CX = (KX - 1) / 2;
     const int TW = DRF_TX + KX - 1;
     uint32_t *tileH = ldsw;                     // [TW][CSB] bf16, as dword pairs
-    uint32_t *tileL = ldsw + TW * CSB / 2;
 
     int bid = blockIdx.x;
     const int xt = bid % a.xtiles;
@@ -821,8 +857,8 @@ __global__ __launch_bounds__(256, 2) void k_drf_bf16(DrfArgs a, const uint4 *__r
     for (int row = r0; row < r1; ++row) {
         const int kidx = a.rowKernel[(long)v * a.nrows + row];
         // per table entry: 0 plain BF16, 1 weights as hi+lo, 2 weights and
-        // data as hi+lo (the SPLIT instantiation stages the data's lo parts),
-        // 3 the delta of rows added unconvolved: added exactly in FP32
+        // data as hi+lo (two passes), 3 the delta of rows added unconvolved:
+        // added exactly in FP32
         const int md = distMode[kidx];
         if (a.rowSel >= 0 && (md == 2) != (a.rowSel == 1)) continue;
         ++taken;
@@ -849,7 +885,6 @@ __global__ __launch_bounds__(256, 2) void k_drf_bf16(DrfArgs a, const uint4 *__r
             }
         } else {
             const int pad = a.rowPad[(long)v * a.nrows + row];
-            const bool spd = SPLIT && md >= 2;
             const bool spa = md >= 1;
             // the entry's band: PSF columns [i0, i1] and window rows from ks
             // (multiple of 8) over nks K steps; outside it the A fragments are zero
@@ -857,38 +892,101 @@ __global__ __launch_bounds__(256, 2) void k_drf_bf16(DrfArgs a, const uint4 *__r
             const int i0 = bd.x, i1 = bd.y, ks = bd.z, nks = bd.w;
             const int srows = 16 * (NZT - 1) + 32 * nks;   // staged rows from ks
             const int c0 = KX - 1 - i1, nc = DRF_TX + i1 - i0;   // staged columns
-            // stage groups of four consecutive slices as packed bf16 (hi, and
-            // lo), window row q <-> slice z0 - CY - 1 + q (a multiple of 4: one
-            // aligned float4 per group, wholly inside or outside [0, Nz) as
-            // Nz % 4 == 0); element e = (column cc, group qi) advances by 256
-            // per step with (cc, qi) updated incrementally (no division)
             const int sp4 = srows / 4, total = sp4 * nc;
             const int dcc = 256 / sp4, dqi = 256 - dcc * sp4;
-            int cc = (int)threadIdx.x / sp4, qi = (int)threadIdx.x - cc * sp4;
             const int zb = z0 - CY - 1;
             // without an axial pad (and in-range bins) a value is the input
             // times the factor, or zero outside the slices
             const bool plain = pad <= 0 || a.avg <= 0;
             const long ibase = (long)b * a.inB + (long)row * a.inRow;
             const long fbase = (long)v * a.afB + (long)row * a.afRow;
-            if (plain) {
-                // NBAT groups per thread at a time: every load of the batch is
-                // issued before the first value is used, so a thread waits for
-                // memory once per batch instead of once per group
-                for (int e0 = threadIdx.x; e0 < total; e0 += 256 * NBAT) {
-                    float4 f[NBAT], m[NBAT];
-                    int li[NBAT];
+            const uint4 *ap = afr + (long)kidx * KX * NKS * 64 + lane;
+            const uint4 *apl = afrLo + (long)kidx * KX * NKS * 64 + lane;
+            const int colBase = 16 * w + n + KX - 1, rowOff = ks + 8 * g;
+            // data split (mode 2): a second pass stages the lo parts of the
+            // values in the same tile and adds their product with the hi
+            // weights (hi*hi + lo*hi weights in the first pass): one tile of
+            // LDS per block in every mode
+            const int passes = md == 2 ? 2 : 1;
+            for (int ph = 0; ph < passes; ++ph) {
+                // stage groups of four consecutive slices as packed bf16,
+                // window row q <-> slice zb + q (a multiple of 4: one aligned
+                // float4 per group, wholly inside or outside [0, Nz) as
+                // Nz % 4 == 0); element e = (column cc, group qi) advances by
+                // 256 per step with (cc, qi) updated incrementally
+                int cc = (int)threadIdx.x / sp4, qi = (int)threadIdx.x - cc * sp4;
+                if (!BCK && a.inBf16 && md <= 1 && plain) {
+                    // input already in bf16 (the producer's rounding of the
+                    // same value): four slices per 8-byte load
+                    const uint16_t *inh = (const uint16_t *)(a.in + ibase);
+                    for (int e0 = threadIdx.x; e0 < total; e0 += 256 * NBAT) {
+                        uint2 hv[NBAT];
+                        int li[NBAT];
 #pragma unroll
-                    for (int u = 0; u < NBAT; ++u) {
+                        for (int u = 0; u < NBAT; ++u) {
+                            const int c = c0 + cc, q = ks + 4 * qi;
+                            const int gx = x0 - CX + c, gz = zb + q;
+                            li[u] = (c * CSB + q) / 2;
+                            const bool ok = e0 + 256 * u < total && gx >= 0 && gx < a.NRot &&
+                                            gz >= 0 && gz < a.Nz;
+                            hv[u] = ok ? *(const uint2 *)(inh + (long)gx * a.xs + gz)
+                                       : make_uint2(0, 0);
+                            cc += dcc;
+                            qi += dqi;
+                            if (qi >= sp4) {
+                                qi -= sp4;
+                                ++cc;
+                            }
+                        }
+#pragma unroll
+                        for (int u = 0; u < NBAT; ++u)
+                            if (e0 + 256 * u < total) *(uint2 *)(tileH + li[u]) = hv[u];
+                    }
+                } else if (plain) {
+                    // NBAT groups per thread at a time: every load of the batch
+                    // is issued before the first value is used, so a thread
+                    // waits for memory once per batch instead of once per group
+                    for (int e0 = threadIdx.x; e0 < total; e0 += 256 * NBAT) {
+                        float4 f[NBAT], m[NBAT];
+                        int li[NBAT];
+#pragma unroll
+                        for (int u = 0; u < NBAT; ++u) {
+                            const int c = c0 + cc, q = ks + 4 * qi;
+                            const int gx = x0 - CX + c, gz = zb + q;
+                            li[u] = (c * CSB + q) / 2;
+                            const bool ok = e0 + 256 * u < total && gx >= 0 && gx < a.NRot &&
+                                            gz >= 0 && gz < a.Nz;
+                            const long o = (long)gx * a.xs + gz;
+                            f[u] = ok ? *(const float4 *)(a.in + ibase + o)
+                                      : make_float4(0, 0, 0, 0);
+                            m[u] = (ok && a.afac) ? *(const float4 *)(a.afac + fbase + o)
+                                                  : make_float4(1, 1, 1, 1);
+                            cc += dcc;
+                            qi += dqi;
+                            if (qi >= sp4) {
+                                qi -= sp4;
+                                ++cc;
+                            }
+                        }
+#pragma unroll
+                        for (int u = 0; u < NBAT; ++u) {
+                            if (e0 + 256 * u < total) {
+                                const float vv[4] = {f[u].x * m[u].x, f[u].y * m[u].y,
+                                                     f[u].z * m[u].z, f[u].w * m[u].w};
+                                *(uint2 *)(tileH + li[u]) = drf16_pack4(vv, ph);
+                            }
+                        }
+                    }
+                } else {
+                    // axial padding: element by element through the pad rules
+                    for (int e = threadIdx.x; e < total; e += 256) {
                         const int c = c0 + cc, q = ks + 4 * qi;
                         const int gx = x0 - CX + c, gz = zb + q;
-                        li[u] = (c * CSB + q) / 2;
-                        const bool ok = e0 + 256 * u < total && gx >= 0 && gx < a.NRot &&
-                                        gz >= 0 && gz < a.Nz;
-                        const long o = (long)gx * a.xs + gz;
-                        f[u] = ok ? *(const float4 *)(a.in + ibase + o) : make_float4(0, 0, 0, 0);
-                        m[u] = (ok && a.afac) ? *(const float4 *)(a.afac + fbase + o)
-                                              : make_float4(1, 1, 1, 1);
+                        float fv[4];
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            fv[j] = drf_stage_val(a, b, v, row, gz + j, gx, pad);
+                        *(uint2 *)(tileH + (c * CSB + q) / 2) = drf16_pack4(fv, ph);
                         cc += dcc;
                         qi += dqi;
                         if (qi >= sp4) {
@@ -896,79 +994,16 @@ __global__ __launch_bounds__(256, 2) void k_drf_bf16(DrfArgs a, const uint4 *__r
                             ++cc;
                         }
                     }
-#pragma unroll
-                    for (int u = 0; u < NBAT; ++u) {
-                        if (e0 + 256 * u < total) {
-                            const float v0 = f[u].x * m[u].x, v1 = f[u].y * m[u].y;
-                            const float v2 = f[u].z * m[u].z, v3 = f[u].w * m[u].w;
-                            const __bf16 h0 = (__bf16)v0, h1 = (__bf16)v1;
-                            const __bf16 h2 = (__bf16)v2, h3 = (__bf16)v3;
-                            *(uint2 *)(tileH + li[u]) = make_uint2(pack_bf16(h0, h1), pack_bf16(h2, h3));
-                            if (spd)
-                                *(uint2 *)(tileL + li[u]) = make_uint2(
-                                    pack_bf16((__bf16)(v0 - (float)h0), (__bf16)(v1 - (float)h1)),
-                                    pack_bf16((__bf16)(v2 - (float)h2), (__bf16)(v3 - (float)h3)));
-                        }
-                    }
                 }
-            } else {
-                for (int e = threadIdx.x; e < total; e += 256) {
-                    const int c = c0 + cc, q = ks + 4 * qi;
-                    const int gx = x0 - CX + c, gz = zb + q;
-                    float fv[4];
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) fv[j] = drf_stage_val(a, b, v, row, gz + j, gx, pad);
-                    const __bf16 h0 = (__bf16)fv[0], h1 = (__bf16)fv[1];
-                    const __bf16 h2 = (__bf16)fv[2], h3 = (__bf16)fv[3];
-                    *(uint2 *)(tileH + (c * CSB + q) / 2) = make_uint2(pack_bf16(h0, h1), pack_bf16(h2, h3));
-                    if (spd)
-                        *(uint2 *)(tileL + (c * CSB + q) / 2) = make_uint2(
-                            pack_bf16((__bf16)(fv[0] - (float)h0), (__bf16)(fv[1] - (float)h1)),
-                            pack_bf16((__bf16)(fv[2] - (float)h2), (__bf16)(fv[3] - (float)h3)));
-                    cc += dcc;
-                    qi += dqi;
-                    if (qi >= sp4) {
-                        qi -= sp4;
-                        ++cc;
-                    }
-                }
+                __syncthreads();
+                if (spa && ph == 0)
+                    drf16_columns<NZT, NKS, true>(acc, ap, apl, i0, i1, nks, tileH, colBase, CSB,
+                                                  rowOff);
+                else
+                    drf16_columns<NZT, NKS, false>(acc, ap, apl, i0, i1, nks, tileH, colBase,
+                                                   CSB, rowOff);
+                __syncthreads();
             }
-            __syncthreads();
-            const uint4 *ap = afr + (long)kidx * KX * NKS * 64 + lane;
-            const uint4 *apl = afrLo + (long)kidx * KX * NKS * 64 + lane;
-            const int colBase = 16 * w + n + KX - 1, rowOff = ks + 8 * g;
-            if (spd) {
-                for (int i = i0; i <= i1; ++i) {
-                    bf16x8_t A[NKS], AL[NKS];
-#pragma unroll
-                    for (int s = 0; s < NKS; ++s) {
-                        A[s] = __builtin_bit_cast(bf16x8_t, ap[(i * NKS + s) * 64]);
-                        AL[s] = __builtin_bit_cast(bf16x8_t, apl[(i * NKS + s) * 64]);
-                    }
-                    const uint32_t *ch = tileH + ((colBase - i) * CSB + rowOff) / 2;
-                    const uint32_t *cl = tileL + ((colBase - i) * CSB + rowOff) / 2;
-#pragma unroll
-                    for (int t = 0; t < NZT; ++t)
-#pragma unroll
-                        for (int s = 0; s < NKS; ++s) {
-                            if (s >= nks) break;
-                            const bf16x8_t B = __builtin_bit_cast(
-                                bf16x8_t, *(const uint4 *)(ch + (16 * t + 32 * s) / 2));
-                            const bf16x8_t BL = __builtin_bit_cast(
-                                bf16x8_t, *(const uint4 *)(cl + (16 * t + 32 * s) / 2));
-                            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[s], B, acc[t], 0, 0, 0);
-                            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[s], BL, acc[t], 0, 0, 0);
-                            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(AL[s], B, acc[t], 0, 0, 0);
-                        }
-                }
-            } else if (spa) {
-                drf16_columns<NZT, NKS, true>(acc, ap, apl, i0, i1, nks, tileH, colBase, CSB,
-                                              rowOff);
-            } else {
-                drf16_columns<NZT, NKS, false>(acc, ap, apl, i0, i1, nks, tileH, colBase, CSB,
-                                               rowOff);
-            }
-            __syncthreads();
         }
         if (BCK && x < a.NRot) {
             float *o = a.out + (long)b * a.outB + (long)row * a.outRow + (long)x * a.outXS;
@@ -1384,31 +1419,35 @@ struct Recon::Impl {
     }
 
     void run_att(const float *afac, int B, const int *bview, const float *src = nullptr,
-                 float *dst = nullptr)
+                 float *dst = nullptr, bool half = false)
     {
         const Geometry &g = c.g;
+        const int *rk = half ? rowKernel16.p : nullptr;
+        const int *dm = half ? distSplit.p : nullptr;
         if (c.shears)
             hipLaunchKernelGGL(k_att<true>, att_grid(g.Nz, g.NRot, g.NRot, B), dim3(256), 0,
                                stream, src ? src : x.p, afac, dst ? dst : att.p, g.Nz, g.N,
-                               g.NRot, bview, fsin.p, fcos.p, head.p, npix.p, shear());
+                               g.NRot, bview, fsin.p, fcos.p, head.p, npix.p, shear(), rk, dm);
         else
             hipLaunchKernelGGL(k_att<false>, att_grid(g.Nz, g.NRot, g.NRot, B), dim3(256), 0,
                                stream, src ? src : x.p, afac, dst ? dst : att.p, g.Nz, g.N,
-                               g.NRot, bview, fsin.p, fcos.p, head.p, npix.p, shear());
+                               g.NRot, bview, fsin.p, fcos.p, head.p, npix.p, shear(), rk, dm);
     }
 
-    void run_att_src(const float *afacp, int B, const int *bview, float *dst)
+    void run_att_src(const float *afacp, int B, const int *bview, float *dst, bool half = false)
     {
         const Geometry &g = c.g;
         const float inv = 1.0f / phys.mu0;
+        const int *rk = half ? rowKernel16.p : nullptr;
+        const int *dm = half ? distSplit.p : nullptr;
         if (c.shears)
             hipLaunchKernelGGL(k_att_src<true>, att_grid(g.Nz, g.NRot, g.NRot, B), dim3(256), 0,
                                stream, g0.p, g1.p, afacp, dst, g.Nz, g.N, g.NRot, bview, fsin.p,
-                               fcos.p, head.p, npix.p, shear(), inv);
+                               fcos.p, head.p, npix.p, shear(), inv, rk, dm);
         else
             hipLaunchKernelGGL(k_att_src<false>, att_grid(g.Nz, g.NRot, g.NRot, B), dim3(256), 0,
                                stream, g0.p, g1.p, afacp, dst, g.Nz, g.N, g.NRot, bview, fsin.p,
-                               fcos.p, head.p, npix.p, shear(), inv);
+                               fcos.p, head.p, npix.p, shear(), inv, rk, dm);
     }
 
     // Learned-scatter source volumes g0, g1 from the current estimate: the
@@ -1700,9 +1739,10 @@ struct Recon::Impl {
             return;
         }
         if (bf16) {
-            // one launch over all rows (the SPLIT instantiation when some entry
-            // splits its data: separate launches for those rows measured slower,
-            // the second one's few blocks running their rows back to back)
+            // one launch over all rows (data-split rows run two staging
+            // passes in the same tile; separate launches for them measured
+            // slower, the second one's few blocks running their rows back to
+            // back)
             DrfArgs a1 = args;
             a1.rowSel = -1;
             a1.accumulate = 0;
@@ -2619,11 +2659,17 @@ void Recon::runViews(int s, int first, int Vs, SrfMode srf, bool accumulate, boo
     if (srf != SRF_NONE && !I.prjscat.p)
         fail(ETYPE_ILLEGAL_VALUE, "scatter model requested without a scatter kernel");
 
+    // The BF16 projector's input planes may be written in bf16 (rows whose
+    // DRF entry does not split its data; lossless: the projector rounds the
+    // same values to bf16) by the rotation kernels; the scatter model's
+    // source kernels and a checked subset's learned source stay FP32.
+    bool attHalf = false;
     if (model && I.esse && I.cst.active) {
         // once per subset: the source volumes of the current estimate (whether
         // the subset is also checked was decided by beginSubset)
         if (first == 0) I.launch("cnn_src", [&] { I.run_cnn_sources(); });
-        I.launch("cnn_att", [&] { I.run_att_src(afac, Vs, bview, I.att.p); });
+        attHalf = I.bf16 && !I.cnnCheckNow;
+        I.launch("cnn_att", [&] { I.run_att_src(afac, Vs, bview, I.att.p, attHalf); });
         if (I.cnnCheckNow && first == 0) {
             // the scatter model on the subset's first view as the check
             I.run_srf_att(afac, 1, bview, I.att2.p);
@@ -2649,17 +2695,19 @@ void Recon::runViews(int s, int first, int Vs, SrfMode srf, bool accumulate, boo
         }
     } else if (model && I.esse)
         I.run_srf_att(afac, Vs, bview);
-    else if (model)  // scatFac == 0: no effective source, the projected image is x
-        I.launch("att", [&] { I.run_att(afac, Vs, bview); });
-    else
-        I.launch("att", [&] { I.run_att(afac, Vs, bview); });
-    const DrfArgs da = I.drf_args(false, bview, cfg_.prjModel);
+    else {   // no scatter model (or scatFac == 0: the projected image is x)
+        attHalf = I.bf16;
+        I.launch("att", [&] { I.run_att(afac, Vs, bview, nullptr, nullptr, attHalf); });
+    }
+    DrfArgs da = I.drf_args(false, bview, cfg_.prjModel);
+    da.inBf16 = attHalf ? 1 : 0;
     I.launch(I.drf_name("drf_prj", false), [&] { I.run_drf(false, da, Vs, KYb); });
     if (srf == SRF_MODEL_STORE) {
         // projection without the scatter model for the stored estimate
         // (osemrecon.c:757-786), reusing the rotated estimate
-        I.launch("att", [&] { I.run_att(afac, Vs, bview); });
+        I.launch("att", [&] { I.run_att(afac, Vs, bview, nullptr, nullptr, I.bf16); });
         DrfArgs d2 = da;
+        d2.inBf16 = I.bf16 ? 1 : 0;
         d2.out = I.ppart2.p;
         I.launch(I.drf_name("drf_prj", false), [&] { I.run_drf(false, d2, Vs, KYb); });
     }
@@ -3269,23 +3317,21 @@ static void launch_drf_bf16(hipStream_t stream, bool bck, DrfArgs args, int B, i
     const int NZT = tile == 4 || tile == 8 ? tile : args.Nz > 64 ? 8 : 4;
     const int TW = DRF_TX + args.KX - 1;
     const int THP = 16 * (NZT - 1) + 32 * drf16_nks(KYb);
-    const size_t lds = (size_t)TW * drf16_cs(THP) * 2 * (split ? 2 : 1);
+    const size_t lds = (size_t)TW * drf16_cs(THP) * 2;   // one bf16 tile (splits run two passes)
     if (lds > 160 * 1024) fail(ETYPE_ILLEGAL_VALUE, "DRF PSF too wide for the BF16 path (KX=%d)", args.KX);
     args.ztiles = div_up(args.Nz, 16 * NZT);
     const long nblocks = (long)args.xtiles * args.ztiles * B * (bck ? args.nrows : args.nchunk);
     if (nblocks <= 0 || nblocks > INT_MAX) fail(ETYPE_ILLEGAL_VALUE, "bad DRF launch size");
     dim3 grid((unsigned)nblocks), block(256);
-#define IRL_DRF16_LAUNCH(K, Z, BK, SP) \
-    hipLaunchKernelGGL((k_drf_bf16<K, Z, BK, SP>), grid, block, lds, stream, args, afr, afrLo, \
+#define IRL_DRF16_LAUNCH(K, Z, BK) \
+    hipLaunchKernelGGL((k_drf_bf16<K, Z, BK>), grid, block, lds, stream, args, afr, afrLo, \
                        distSplit, band)
 #define IRL_DRF16_CASE(K)                                                            \
     case K:                                                                          \
         if (NZT == 8) {                                                              \
-            if (bck) { if (split) IRL_DRF16_LAUNCH(K, 8, true, true); else IRL_DRF16_LAUNCH(K, 8, true, false); } \
-            else { if (split) IRL_DRF16_LAUNCH(K, 8, false, true); else IRL_DRF16_LAUNCH(K, 8, false, false); } \
+            if (bck) IRL_DRF16_LAUNCH(K, 8, true); else IRL_DRF16_LAUNCH(K, 8, false); \
         } else {                                                                     \
-            if (bck) { if (split) IRL_DRF16_LAUNCH(K, 4, true, true); else IRL_DRF16_LAUNCH(K, 4, true, false); } \
-            else { if (split) IRL_DRF16_LAUNCH(K, 4, false, true); else IRL_DRF16_LAUNCH(K, 4, false, false); } \
+            if (bck) IRL_DRF16_LAUNCH(K, 4, true); else IRL_DRF16_LAUNCH(K, 4, false); \
         }                                                                            \
         break;
     switch (KYb) {
