@@ -36,6 +36,12 @@ int irlamd_recon_subsets(const irlamd_recon *r, int *piNumSubsets, int *piViewsP
 /* One sub-iteration (all views of subset iSubset, then the update). */
 int irlamd_recon_subiteration(irlamd_recon *r, int iSubset);
 
+/* With graph replay on (irlamd_set_graphs): captures and instantiates every
+ * subset's sub-iteration for the scatter modes of the remaining iterations
+ * now, so that later sub-iterations replay from their first occurrence
+ * (learned-scatter checks still run eagerly).  Executes no GPU work. */
+int irlamd_recon_prepare_graphs(irlamd_recon *r);
+
 /* Waits for queued GPU work. */
 int irlamd_recon_synchronize(irlamd_recon *r);
 

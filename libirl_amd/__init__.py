@@ -169,6 +169,7 @@ def load():
                                          ctypes.POINTER(ctypes.c_int)]
     lib.irlamd_recon_subiteration.argtypes = [ctypes.c_void_p, ctypes.c_int]
     lib.irlamd_recon_synchronize.argtypes = [ctypes.c_void_p]
+    lib.irlamd_recon_prepare_graphs.argtypes = [ctypes.c_void_p]
     lib.irlamd_recon_get_estimate.argtypes = [ctypes.c_void_p, _FP]
     lib.irlamd_recon_get_norm.argtypes = [ctypes.c_void_p, ctypes.c_int, _FP]
     lib.irlamd_recon_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
@@ -444,6 +445,10 @@ class Recon:
 
     def synchronize(self):
         self._check(self._lib.irlamd_recon_synchronize(self._h))
+
+    def prepare_graphs(self):
+        """Captures every subset's sub-iteration graph now (graphs on)."""
+        self._check(self._lib.irlamd_recon_prepare_graphs(self._h))
 
     def estimate(self):
         p = self.parms

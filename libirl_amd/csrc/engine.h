@@ -159,6 +159,10 @@ public:
     // the second occurrence of its (subset, mode) on.
     void subIteration(int s, bool keepUpdate, SrfMode srf = SRF_NONE);
     void setGraphs(bool on);
+    // With graphs on: captures and instantiates subset s's sub-iteration for
+    // the mode now, so that its first occurrence already replays (subsets
+    // with a learned-scatter check still run eagerly).  Executes nothing.
+    void prepareGraph(int s, bool keepUpdate, SrfMode srf);
 
     // Pieces of subIteration for drivers of several reconstructions: the
     // learned-scatter bookkeeping around a sub-iteration (beginSubset

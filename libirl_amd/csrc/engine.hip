@@ -2346,6 +2346,20 @@ void Recon::subIteration(int s, bool keepUpdate, SrfMode srf)
     endSubset(srf);
 }
 
+void Recon::prepareGraph(int s, bool keepUpdate, SrfMode srf)
+{
+    Impl &I = *impl_;
+    if (!I.useGraphs || I.timing) return;
+    const GraphKey key{s, (int)srf, keepUpdate ? 1 : 0, I.cst.active ? 1 : 0};
+    auto it = I.graphs.find(key);
+    if (it != I.graphs.end() && it->second) return;
+    hipGraph_t gr = (hipGraph_t)captureSubIteration(s, keepUpdate, srf);
+    hipGraphExec_t ex = nullptr;
+    HIP_CHECK(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
+    HIP_CHECK(hipGraphDestroy(gr));
+    I.graphs[key] = ex;
+}
+
 // The launches of one sub-iteration captured into a graph (not executed).
 // Learned-scatter bookkeeping is the caller's (beginSubset / endSubset).
 void *Recon::captureSubIteration(int s, bool keepUpdate, SrfMode srf)

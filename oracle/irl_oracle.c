@@ -1295,3 +1295,35 @@ int orc_esse_effsrc_cols(const void *pv, const REAL *pfAct, const REAL *pfMap, i
     free(actT);
     return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* The ESSE model's grids and constants, for the FFT restatement of    */
+/* the CPU baseline (oracle/esse_fft.py): dims = (Ky depth, Kz slices, */
+/* Kx offsets), params = (mu0, muavg, pixsize, cntrpixfac, nterms,     */
+/* collapse, scatfac), grid k = dense [Ky][Kz][Kx] kernel term k.      */
+/* ------------------------------------------------------------------ */
+void orc_esse_info(const void *pv, int *dims, double *params)
+{
+    const orc_esse *e = (const orc_esse *)pv;
+    dims[0] = e->Ky;
+    dims[1] = e->Kz;
+    dims[2] = e->Kx;
+    params[0] = e->mu0;
+    params[1] = e->muavg;
+    params[2] = e->pixsize;
+    params[3] = e->cntrpixfac;
+    params[4] = e->nterms;
+    params[5] = e->collapse;
+    params[6] = e->scatfac;
+}
+
+void orc_esse_grid(const void *pv, int k, float *grid)
+{
+    const orc_esse *e = (const orc_esse *)pv;
+    long n = (long)e->Kx * e->Ky * e->Kz, t;
+    for (t = 0; t < n; ++t) grid[t] = 0.0f;
+    for (t = 0; t < e->nnz[k]; ++t) {
+        int ky = e->idx[k][3 * t], kz = e->idx[k][3 * t + 1], kx = e->idx[k][3 * t + 2];
+        grid[kx + (long)e->Kx * (kz + (long)e->Kz * ky)] = e->val[k][t];
+    }
+}

@@ -68,6 +68,8 @@ def lib(real="f64"):
     L.orc_initial_estimate.restype = ctypes.c_int
     L.orc_esse_effsrc_cols.argtypes = [vp, vp, vp] + [ctypes.c_int] * 4 + [vp, vp]
     L.orc_esse_effsrc_cols.restype = ctypes.c_int
+    L.orc_esse_info.argtypes = [vp, vp, vp]
+    L.orc_esse_grid.argtypes = [vp, ctypes.c_int, vp]
     _libs[real] = L
     return L
 
@@ -287,6 +289,22 @@ class Esse:
         lib(self.real).orc_esse_effsrc(self.h, act.ctypes.data, mp.ctypes.data, nx, ny, nz,
                                        out.ctypes.data)
         return out
+
+    def info(self):
+        """(grid dims (Ky, Kz, Kx), dict of the model's constants)."""
+        d = np.zeros(3, np.int32)
+        q = np.zeros(7, np.float64)
+        lib(self.real).orc_esse_info(self.h, d.ctypes.data, q.ctypes.data)
+        return tuple(int(v) for v in d), dict(mu0=q[0], muavg=q[1], pixsize=q[2], cntr=q[3],
+                                               nterms=int(q[4]), collapse=int(q[5]),
+                                               scatfac=q[6])
+
+    def grid(self, k):
+        """Dense [Ky][Kz][Kx] kernel term k (0: K, 1: K(mu - muavg), 2: K(mu - muavg)^2)."""
+        dims, _ = self.info()
+        g = np.zeros(dims, np.float32)
+        lib(self.real).orc_esse_grid(self.h, k, g.ctypes.data)
+        return g
 
     def effsrc_columns(self, act, mp, cols):
         """The effective source on the depth columns cols = [(slice, bin), ...]
