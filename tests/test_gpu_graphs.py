@@ -192,3 +192,31 @@ def test_genprj_with_set_up_timings_equals_plain():
         L.set_autotune(prev[0])
         L.set_drf_fft(prev[1])
     assert np.abs(timed - plain).max() <= 1e-5 * np.abs(plain).max()
+
+
+@pytest.mark.parametrize("learned", [False, True])
+def test_graphs_prepared_at_setup_equal_eager(learned):
+    """Every subset's graph captured before the first sub-iteration (for
+    every scatter mode of the remaining iterations: modelled, modelled and
+    stored, stored added), then replayed from the first occurrence on."""
+    print("graphs prepared at setup vs eager, learned=%s" % learned, flush=True)
+    import libirl_amd as L
+    pb = Problem(n=32, nz=16, nviews=8, per=2, iters=3, half=3, noise=False)
+    krnl = _krnl_file(pb.pix)
+    if learned:
+        L.set_scatter_cnn_exact(True)
+        L.set_scatter_cnn(None, check_every=3, max_nrmse=1.0)
+    try:
+        a = _recon(pb, krnl, 2, graphs=False)
+        b = _recon(pb, krnl, 2, graphs=True)
+    finally:
+        L.set_scatter_cnn_exact(False)
+        L.set_scatter_cnn(None, check_every=0, max_nrmse=0.01)   # the library defaults
+    b.prepare_graphs()
+    for it in range(3):
+        for s in range(a.num_subsets):
+            a.subiteration(s)
+            b.subiteration(s)
+            assert np.array_equal(a.estimate(), b.estimate()), (it, s)
+    if learned:
+        assert L.cnn_status(a) == L.cnn_status(b)
