@@ -180,6 +180,11 @@ int irlamd_cnn_num_params(void);
 int irlamd_cnn_get_params(irlamd_cnn *h, float *pfParams, float *pfKappa);
 int irlamd_cnn_set_params(irlamd_cnn *h, const float *pfParams);
 
+/* Slices per wave of the inference path's MFMA layers (64, 32 or 16; the
+ * output does not depend on it), for launches made or captured afterwards.
+ * No counterpart in the reference (tuning of this library's kernels). */
+int irlamd_cnn_set_row_length(irlamd_cnn *h, int iRowLength);
+
 /* Network on a host input [Y][X][Z][2] -> output [Y][X][Z][2]; bBf16 selects
  * the inference arithmetic (BF16 MFMA layers), else FP32 throughout. */
 int irlamd_cnn_forward(irlamd_cnn *h, const float *pfIn, int iY, int iX, int iZ, int bBf16,

@@ -218,6 +218,7 @@ def load():
     lib.irlamd_cnn_get_params.argtypes = [vp, _FP, _FP]
     lib.irlamd_cnn_set_params.argtypes = [vp, _FP]
     lib.irlamd_cnn_forward.argtypes = [vp, _FP] + [ctypes.c_int] * 4 + [_FP]
+    lib.irlamd_cnn_set_row_length.argtypes = [vp, ctypes.c_int]
     lib.irlamd_cnn_loss_grad.argtypes = [vp, _FP, _FP] + [ctypes.c_int] * 3 + \
         [ctypes.POINTER(ctypes.c_double), _FP]
     lib.irlamd_cnn_sources.argtypes = [vp, _FP, _FP, ctypes.c_int, ctypes.c_int, ctypes.c_float,
@@ -563,6 +564,11 @@ class ScatterCnn:
     def set_params(self, p):
         p = np.ascontiguousarray(p, np.float32)
         if self._lib.irlamd_cnn_set_params(self._h, fptr(p)):
+            raise RuntimeError(error_string())
+
+    def set_row_length(self, rowz):
+        """Slices per wave of the inference path's MFMA layers (64, 32, 16)."""
+        if self._lib.irlamd_cnn_set_row_length(self._h, rowz):
             raise RuntimeError(error_string())
 
     def forward(self, inp, bf16=False):

@@ -14,6 +14,7 @@
 // transposed weights, weight gradients reduced per tap in LDS and added to
 // global memory with atomics, Adam on FP32 master weights.
 #include <hip/hip_runtime.h>
+#include <climits>
 
 #include <chrono>
 #include <cmath>
@@ -139,17 +140,17 @@ __global__ __launch_bounds__(256) void k_cnn_pool(const float *__restrict__ x,
 {
     __shared__ double red[256];
     const int Y = N / 2, X = N / 2, Z = Nz / 2;
-    const long v = blockIdx.x * 256L + threadIdx.x;
+    const int v = blockIdx.x * 256 + threadIdx.x;
     float s0 = 0.0f, s1 = 0.0f;
-    if (v < (long)Y * X * Z) {
-        const int z = (int)(v % Z);
-        const long r = v / Z;
-        const int xx = (int)(r % X), y = (int)(r / X);
+    if (v < Y * X * Z) {
+        const int z = v % Z;
+        const int r = v / Z;
+        const int xx = r % X, y = r / X;
 #pragma unroll
         for (int dy = 0; dy < 2; ++dy)
 #pragma unroll
             for (int dx = 0; dx < 2; ++dx) {
-                const long o = ((long)(2 * y + dy) * N + 2 * xx + dx) * Nz + 2 * z;
+                const int o = ((2 * y + dy) * N + 2 * xx + dx) * Nz + 2 * z;
                 s0 += x[o] + x[o + 1];
                 if (mu) s1 += mu[o] + mu[o + 1];
             }
@@ -333,9 +334,12 @@ __global__ __launch_bounds__(256) void k_conv_l1_rows(const float2 *__restrict__
 // fragments from there: 9 instead of 27 global passes over the input.  A
 // fragments (identical for every wave) come from the caches.
 // NCT = 2: 32 outputs, bias + ReLU, bf16 [V][32]; NCT = 1 with COUT = 2:
-// the linear output layer, fp32 [V][2] (rows 2..15 of A are zero).
+// the linear output layer, fp32 [V][2] (rows 2..15 of A are zero).  ROWZ
+// (64, 32 or 16) is the slices per wave: shorter rows give more waves for
+// the same volume, longer rows less halo per slice.
 // ---------------------------------------------------------------------------
-constexpr int CNN_ROWZ = 64;
+constexpr int CNN_ROWZ = 64;   // the layer-1 kernel's rows
+constexpr int CNN_MAXDIL = 1 << (CNN_LAYERS - 1);
 
 // Orders a wave's own LDS writes and reads (a wave's LDS instructions are
 // performed in issue order; this keeps the compiler from reordering them).
@@ -346,7 +350,7 @@ __device__ __forceinline__ void cnn_wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <int NCT, int COUT>
+template <int NCT, int COUT, int ROWZ>
 __global__ __launch_bounds__(256) void k_conv_mfma_rows(const uint16_t *__restrict__ in,
                                                         const uint4 *__restrict__ Wg,
                                                         const float *__restrict__ bias,
@@ -357,24 +361,25 @@ __global__ __launch_bounds__(256) void k_conv_mfma_rows(const uint16_t *__restri
     extern __shared__ uint4 seg[];               // [4 waves][ROWZ + 2 dil][4]
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int n = lane & 15, g = lane >> 4;
-    const int zt = (Z + CNN_ROWZ - 1) / CNN_ROWZ;
+    constexpr int NJ = ROWZ / 16;
+    const int zt = (Z + ROWZ - 1) / ROWZ;
     const long task = (long)blockIdx.x * 4 + w;
     const long rows = (long)Y * X;
     const bool valid = task < rows * zt;
     const long row = valid ? task / zt : 0;
-    const int z0 = valid ? (int)(task % zt) * CNN_ROWZ : 0;
+    const int z0 = valid ? (int)(task % zt) * ROWZ : 0;
     const int y = (int)(row / X), x = (int)(row % X);
-    const int SL = CNN_ROWZ + 2 * dil;
+    const int SL = ROWZ + 2 * dil;
     uint4 *my = seg + (long)w * SL * 4;
-    cfloatx4_t acc[NCT][4];
+    cfloatx4_t acc[NCT][NJ];
 #pragma unroll
     for (int c = 0; c < NCT; ++c)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[c][j] = cfloatx4_t{0.0f, 0.0f, 0.0f, 0.0f};
-    // the neighbour row segment of offset q, SL*4 <= 5*64 uint4 per wave:
-    // loaded into registers one offset ahead, so its latency overlaps the
-    // previous offset's products
-    constexpr int PRE = 5;
+        for (int j = 0; j < NJ; ++j) acc[c][j] = cfloatx4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    // the neighbour row segment of offset q, SL*4 <= PRE*64 uint4 per wave
+    // (dil <= CNN_MAXDIL): loaded into registers one offset ahead, so its
+    // latency overlaps the previous offset's products
+    constexpr int PRE = ((ROWZ + 2 * CNN_MAXDIL) * 4 + 63) / 64;
     uint4 pre[PRE];
     auto fetch = [&](int q) {
         const int yy = y + dil * (q / 3 - 1), xx = x + dil * (q % 3 - 1);
@@ -389,6 +394,10 @@ __global__ __launch_bounds__(256) void k_conv_mfma_rows(const uint16_t *__restri
                          : make_uint4(0, 0, 0, 0);
         }
     };
+    // LDS slot of voxel r's channel quarter c (8 channels, 16 bytes): the
+    // quarters rotate by r/4, so the 16 lanes of a B-fragment read (16
+    // consecutive voxels, one quarter) cover all 64 banks once
+    auto slot = [](int r, int c) { return r * 4 + ((c + (r >> 2)) & 3); };
     fetch(0);
     for (int q = 0; q < 9; ++q) {
         // each wave stages and reads only its own segment: the wave's LDS
@@ -396,8 +405,10 @@ __global__ __launch_bounds__(256) void k_conv_mfma_rows(const uint16_t *__restri
         // them in program order (no block barrier)
         cnn_wave_sync();   // the previous offset's readers are done
 #pragma unroll
-        for (int k = 0; k < PRE; ++k)
-            if (lane + 64 * k < SL * 4) my[lane + 64 * k] = pre[k];
+        for (int k = 0; k < PRE; ++k) {
+            const int e = lane + 64 * k;
+            if (e < SL * 4) my[slot(e >> 2, e & 3)] = pre[k];
+        }
         cnn_wave_sync();
         if (q + 1 < 9) fetch(q + 1);
 #pragma unroll
@@ -408,9 +419,9 @@ __global__ __launch_bounds__(256) void k_conv_mfma_rows(const uint16_t *__restri
             for (int c = 0; c < NCT; ++c)
                 A[c] = __builtin_bit_cast(cbf16x8_t, Wg[t * WQ + (16 * c + n) * 4 + g]);
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
+            for (int j = 0; j < NJ; ++j) {
                 const cbf16x8_t B =
-                    __builtin_bit_cast(cbf16x8_t, my[(16 * j + n + dzi * dil) * 4 + g]);
+                    __builtin_bit_cast(cbf16x8_t, my[slot(16 * j + n + dzi * dil, g)]);
 #pragma unroll
                 for (int c = 0; c < NCT; ++c)
                     acc[c][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[c], B, acc[c][j], 0, 0, 0);
@@ -419,7 +430,7 @@ __global__ __launch_bounds__(256) void k_conv_mfma_rows(const uint16_t *__restri
     }
     if (!valid) return;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < NJ; ++j) {
         const int z = z0 + 16 * j + n;
         if (z >= Z) continue;
         const long v = row * Z + z;
@@ -619,12 +630,12 @@ __global__ __launch_bounds__(256) void k_cnn_source(const float *__restrict__ o,
                                                     float *__restrict__ g0,
                                                     float *__restrict__ g1)
 {
-    const long n = (long)N * N * Nz;
-    const long i = blockIdx.x * 256L + threadIdx.x;
+    const int n = N * N * Nz;
+    const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    const int z = (int)(i % Nz);
-    const long r = i / Nz;
-    const int xx = (int)(r % N), y = (int)(r / N);
+    const int z = i % Nz;
+    const int r = i / Nz;
+    const int xx = r % N, y = r / N;
     const int Y = N / 2, X = N / 2, Z = Nz / 2;
     int y0, y1, x0, x1, z0, z1;
     float wy, wx, wz;
@@ -639,7 +650,7 @@ __global__ __launch_bounds__(256) void k_cnn_source(const float *__restrict__ o,
 #pragma unroll
             for (int d = 0; d < 2; ++d) {
                 const float wgt = (a ? wy : 1.0f - wy) * (b ? wx : 1.0f - wx) * (d ? wz : 1.0f - wz);
-                const long q = (((long)(a ? y1 : y0) * X + (b ? x1 : x0)) * Z + (d ? z1 : z0)) * 2;
+                const int q = (((a ? y1 : y0) * X + (b ? x1 : x0)) * Z + (d ? z1 : z0)) * 2;
                 c[0] = fmaf(wgt, o[q], c[0]);
                 c[1] = fmaf(wgt, o[q + 1], c[1]);
             }
@@ -988,17 +999,38 @@ void ScatterCnn::forward_bf16(const float *in, int Y, int X, int Z, float *out, 
     const long tasks = (long)Y * X * ((Z + CNN_ROWZ - 1) / CNN_ROWZ);
     hipLaunchKernelGGL(k_conv_l1_rows, dim3(cdiv_u(tasks, 4)), blk, 0, s, (const float2 *)in,
                        Wr0_, b_[0], scal ? scal : one_, a0, Y, X, Z);
-    const dim3 rgrid(cdiv_u(tasks, 4));
-    // k_conv_mfma_rows prefetches a segment of at most 5 x 64 uint4 per wave
-    static_assert(4 * (CNN_ROWZ + 2 * cnn_dil(CNN_LAYERS - 1)) <= 5 * 64, "segment size");
-    auto lds = [](int dil) { return (size_t)4 * (CNN_ROWZ + 2 * dil) * 4 * sizeof(uint4); };
-    hipLaunchKernelGGL((k_conv_mfma_rows<2, 32>), rgrid, blk, lds(cnn_dil(1)), s, a0,
-                       (const uint4 *)Wbf_[1], b_[1], (void *)a1, Y, X, Z, cnn_dil(1), 1);
-    hipLaunchKernelGGL((k_conv_mfma_rows<2, 32>), rgrid, blk, lds(cnn_dil(2)), s, a1,
-                       (const uint4 *)Wbf_[2], b_[2], (void *)a0, Y, X, Z, cnn_dil(2), 1);
-    hipLaunchKernelGGL((k_conv_mfma_rows<1, 2>), rgrid, blk, lds(cnn_dil(3)), s, a0,
-                       (const uint4 *)Wbf_[3], b_[3], (void *)out, Y, X, Z, cnn_dil(3), 0);
+    static_assert(cnn_dil(CNN_LAYERS - 1) == CNN_MAXDIL, "largest dilation");
+    switch (rowz_) {
+    case 64: mfma_layers<64>(Y, X, Z, out, a0, a1, s); break;
+    case 32: mfma_layers<32>(Y, X, Z, out, a0, a1, s); break;
+    case 16: mfma_layers<16>(Y, X, Z, out, a0, a1, s); break;
+    default: fail(ETYPE_ILLEGAL_VALUE, "scatter CNN row length %d (64, 32 or 16)", rowz_);
+    }
     CNN_CHECK(hipGetLastError());
+}
+
+// Layers 2-4 with rows of ROWZ slices per wave (a0 holds layer 1's output).
+template <int ROWZ>
+void ScatterCnn::mfma_layers(int Y, int X, int Z, float *out, uint16_t *a0, uint16_t *a1,
+                             hipStream_t s)
+{
+    const dim3 blk(256);
+    const long tasks = (long)Y * X * ((Z + ROWZ - 1) / ROWZ);
+    const dim3 rgrid(cdiv_u(tasks, 4));
+    auto lds = [](int dil) { return (size_t)4 * (ROWZ + 2 * dil) * 4 * sizeof(uint4); };
+    hipLaunchKernelGGL((k_conv_mfma_rows<2, 32, ROWZ>), rgrid, blk, lds(cnn_dil(1)), s, a0,
+                       (const uint4 *)Wbf_[1], b_[1], (void *)a1, Y, X, Z, cnn_dil(1), 1);
+    hipLaunchKernelGGL((k_conv_mfma_rows<2, 32, ROWZ>), rgrid, blk, lds(cnn_dil(2)), s, a1,
+                       (const uint4 *)Wbf_[2], b_[2], (void *)a0, Y, X, Z, cnn_dil(2), 1);
+    hipLaunchKernelGGL((k_conv_mfma_rows<1, 2, ROWZ>), rgrid, blk, lds(cnn_dil(3)), s, a0,
+                       (const uint4 *)Wbf_[3], b_[3], (void *)out, Y, X, Z, cnn_dil(3), 0);
+}
+
+void ScatterCnn::setRowLength(int rowz)
+{
+    if (rowz != 64 && rowz != 32 && rowz != 16)
+        fail(ETYPE_ILLEGAL_VALUE, "scatter CNN row length %d (64, 32 or 16)", rowz);
+    rowz_ = rowz;
 }
 
 void ScatterCnn::zeroGrads(hipStream_t s)
@@ -1103,6 +1135,8 @@ void ScatterCnn::sources(const float *x, const float *mu, int N, int Nz, const C
     const long V = (long)Y * X * Z, n = (long)N * N * Nz;
     if (N % 2 || Nz % 2 || V != w.V)
         fail(ETYPE_ILLEGAL_VALUE, "scatter CNN work buffers made for another size");
+    if (n > INT_MAX)   // the source kernels index in 32 bits
+        fail(ETYPE_ILLEGAL_VALUE, "scatter CNN volume too large (%ld voxels)", n);
     if (bfDirty_) fail(ETYPE_ILLEGAL_VALUE, "scatter CNN inference weights not prepared");
     const unsigned pb = cdiv_u(V, 256);
     if (pb > w.nb) fail(ETYPE_ILLEGAL_VALUE, "scatter CNN work buffers too small");
@@ -1469,6 +1503,17 @@ int irlamd_cnn_set_params(irlamd_cnn *h, const float *in)
     try {
         h->net->setParams(std::vector<float>(in, in + irlamd::cnn_num_params()));
         h->net->prepareInference();
+        return 0;
+    } catch (const irlamd::IrlError &e) {
+        return cnn_report(e);
+    }
+}
+
+int irlamd_cnn_set_row_length(irlamd_cnn *h, int rowz)
+{
+    irlamd::set_last_error(0);
+    try {
+        h->net->setRowLength(rowz);
         return 0;
     } catch (const irlamd::IrlError &e) {
         return cnn_report(e);

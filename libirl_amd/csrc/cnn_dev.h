@@ -101,6 +101,11 @@ public:
     // training; synchronous).
     void prepareInference();
 
+    // Slices per wave of the inference path's MFMA layers (64, 32 or 16;
+    // the result does not depend on it).
+    void setRowLength(int rowz);
+    int rowLength() const { return rowz_; }
+
     // Train from ESSE-generated pairs (the model's kernel at the given size).
     static ScatterCnn *train(const EsseModel &m, const CnnTrainParms &p, CnnTrainStats &st);
 
@@ -110,6 +115,10 @@ private:
     void upload_bf16_weights(hipStream_t s);
     void forward_bf16(const float *in, int Y, int X, int Z, float *out, uint16_t *a0,
                       uint16_t *a1, const float *scal, hipStream_t s);
+    template <int ROWZ>
+    void mfma_layers(int Y, int X, int Z, float *out, uint16_t *a0, uint16_t *a1, hipStream_t s);
+
+    int rowz_ = 64;
 
     float kappa_ = 1.0f;
     int step_ = 0;

@@ -1054,14 +1054,15 @@ struct RatioArgs {
     int srfMode;                  // 0 none, 2 store estimate, 3 add stored estimate
 };
 
+// grid (plane blocks, batch): 32-bit indices within a plane
 __global__ void k_ratio(RatioArgs a)
 {
-    const long pl = (long)a.Nz * a.NRot;
-    const long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (gid >= pl * a.B) return;
-    const int b = (int)(gid / pl);
-    const long i = gid % pl;
-    const int xb = (int)(i / a.Nz);   // projections are [bin][slice]
+    const int pl = a.Nz * a.NRot;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= pl) return;
+    const int b = blockIdx.y;
+    const long gid = (long)b * pl + i;
+    const int xb = i / a.Nz;   // projections are [bin][slice]
     const int v = a.bview[b];
     float p = 0.0f;
     for (int c = 0; c < a.nchunk; ++c) p += a.ppart[((long)b * a.nchunk + c) * pl + i];
@@ -2743,7 +2744,7 @@ void Recon::runViews(int s, int first, int Vs, SrfMode srf, bool accumulate, boo
     ra.scale = 1.0f;
     ra.ratio = I.ratio.p;
     I.launch("ratio", [&] {
-        hipLaunchKernelGGL(k_ratio, dim3(div_up(I.P * Vs, threads)), dim3(threads), 0,
+        hipLaunchKernelGGL(k_ratio, dim3(div_up(I.P, threads), Vs), dim3(threads), 0,
                            I.stream, ra);
     });
     const DrfArgs db = I.drf_args(true, bview, cfg_.bckModel);

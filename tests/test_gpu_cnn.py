@@ -56,6 +56,24 @@ def test_forward_bf16_matches_oracle(shape):
     assert e < 2e-2
 
 
+# Z = 40: rows of 64 slices cover it with one partial row, of 32 and 16
+# with a partial last row (16: none)
+def test_row_lengths_give_identical_outputs():
+    print("CNN forward BF16 MFMA, rows of 64, 32 and 16 slices", flush=True)
+    net, _ = _net()
+    inp = _grid_input((12, 10, 40), 5)
+    outs = []
+    try:
+        for rz in (64, 32, 16):
+            net.set_row_length(rz)
+            outs.append(net.forward(inp, bf16=True))
+    finally:
+        net.set_row_length(64)
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+    with pytest.raises(RuntimeError):
+        net.set_row_length(48)
+
+
 def test_loss_gradients_match_oracle():
     print("CNN loss gradients", flush=True)
     import libirl_amd as L
