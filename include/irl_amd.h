@@ -207,10 +207,12 @@ int irlamd_set_scatter_cnn(irlamd_cnn *h, int iCheckEvery, float fMaxNrmse);
 int irlamd_set_scatter_cnn_exact(int bExact);
 /* State of the learned scatter of a reconstruction (or, with r = NULL, of the
  * last IrlOsem call): whether it is still in use, checks run, subsets run on
- * the network, last and worst checked relative L2 of the projected source,
- * and the last one of its scatter part alone. */
+ * the network, last and worst checked relative L2 of the projected source
+ * (each check: the larger of two views, the subset's first and middle one),
+ * and the last and worst of its scatter part alone. */
 int irlamd_recon_cnn_status(irlamd_recon *r, int *piActive, int *piChecks, int *piUsed,
-                            float *pfLastNrmse, float *pfWorstNrmse, float *pfLastScatterNrmse);
+                            float *pfLastNrmse, float *pfWorstNrmse, float *pfLastScatterNrmse,
+                            float *pfWorstScatterNrmse);
 
 #ifdef __cplusplus
 }

@@ -226,7 +226,7 @@ def load():
     lib.irlamd_set_scatter_cnn.argtypes = [vp, ctypes.c_int, ctypes.c_float]
     lib.irlamd_set_scatter_cnn_exact.argtypes = [ctypes.c_int]
     lib.irlamd_recon_cnn_status.argtypes = [vp] + [ctypes.POINTER(ctypes.c_int)] * 3 + \
-        [ctypes.POINTER(ctypes.c_float)] * 3
+        [ctypes.POINTER(ctypes.c_float)] * 4
     lib.irlamd_set_graphs.argtypes = [ctypes.c_int]
     lib.irlamd_set_autotune.argtypes = [ctypes.c_int]
     lib.irlamd_recon_tiles.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
@@ -645,13 +645,14 @@ def set_scatter_cnn_exact(on):
 def cnn_status(recon=None):
     """{active, checks, used, last_nrmse, worst_nrmse} of a Recon, or of the last IrlOsem."""
     a, c, u = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-    last, worst, lsc = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+    last, worst = ctypes.c_float(), ctypes.c_float()
+    lsc, wsc = ctypes.c_float(), ctypes.c_float()
     load().irlamd_recon_cnn_status(recon._h if recon is not None else None, ctypes.byref(a),
                                    ctypes.byref(c), ctypes.byref(u), ctypes.byref(last),
-                                   ctypes.byref(worst), ctypes.byref(lsc))
+                                   ctypes.byref(worst), ctypes.byref(lsc), ctypes.byref(wsc))
     return {"active": bool(a.value), "checks": c.value, "used": u.value,
             "last_nrmse": last.value, "worst_nrmse": worst.value,
-            "last_scatter_nrmse": lsc.value}
+            "last_scatter_nrmse": lsc.value, "worst_scatter_nrmse": wsc.value}
 
 
 def set_graphs(on):

@@ -132,7 +132,7 @@ public:
         bool configured = false, active = false;
         int checks = 0, used = 0;          // checked subsets, subsets run on the network
         float lastNrmse = -1, worstNrmse = 0;   // projected source (primary + scatter)
-        float lastScatterNrmse = -1;            // its scatter part alone
+        float lastScatterNrmse = -1, worstScatterNrmse = 0;   // its scatter part alone
     };
     CnnStatus cnnStatus() const;
 

@@ -1928,7 +1928,7 @@ Recon::Recon(const Config &cfg) : impl_(new Impl), cfg_(cfg)
                 I.g1.alloc(I.M);
                 if (cfg.cnn) I.cnnWork.reset(new CnnWork(g.N, g.Nz));
                 I.att2.alloc(I.vol * Vs);
-                I.chk.alloc(3);
+                I.chk.alloc(6);
             }
         }
         I.ppart2.alloc(I.P * I.nchunk * Vs);
@@ -2686,22 +2686,35 @@ void Recon::runViews(int s, int first, int Vs, SrfMode srf, bool accumulate, boo
         attHalf = I.bf16 && !I.cnnCheckNow;
         I.launch("cnn_att", [&] { I.run_att_src(afac, Vs, bview, I.att.p, attHalf); });
         if (I.cnnCheckNow && first == 0) {
-            // the scatter model on the subset's first view as the check
-            I.run_srf_att(afac, 1, bview, I.att2.p);
-            HIP_CHECK(hipMemsetAsync(I.chk.p, 0, 3 * sizeof(double), I.stream));
-            hipLaunchKernelGGL(k_scatter_check, dim3(div_up(I.P, threads)), dim3(threads), 0,
-                               I.stream, I.att2.p, I.att.p, I.rotx.p, afac, bview, 1, g.Nz, g.NRot,
-                               I.chk.p);
-            double h[3] = {0, 0, 0};
-            HIP_CHECK(hipMemcpyAsync(h, I.chk.p, sizeof(h), hipMemcpyDeviceToHost, I.stream));
+            // the scatter model on two of the subset's views (its first and
+            // middle one) as the check; one view where the batch has one
+            const int nchk = Vs >= 2 ? 2 : 1;
+            const int chkb[2] = {0, Vs / 2};
+            HIP_CHECK(hipMemsetAsync(I.chk.p, 0, 3 * nchk * sizeof(double), I.stream));
+            for (int j = 0; j < nchk; ++j) {
+                I.run_srf_att(afac, 1, bview + chkb[j], I.att2.p);
+                hipLaunchKernelGGL(k_scatter_check, dim3(div_up(I.P, threads)), dim3(threads), 0,
+                                   I.stream, I.att2.p, I.att.p + (long)chkb[j] * I.vol, I.rotx.p,
+                                   afac, bview + chkb[j], 1, g.Nz, g.NRot, I.chk.p + 3 * j);
+            }
+            double h[6] = {0, 0, 0, 0, 0, 0};
+            HIP_CHECK(hipMemcpyAsync(h, I.chk.p, 3 * nchk * sizeof(double), hipMemcpyDeviceToHost,
+                                     I.stream));
             HIP_CHECK(hipStreamSynchronize(I.stream));
             // relative L2 of the learned projection (before the collimator
-            // blur) against the model's; of its scatter part for the record
-            const float e = h[2] > 0 ? (float)std::sqrt(h[0] / h[2]) : 0.0f;
+            // blur) against the model's, the larger of the views; of its
+            // scatter part for the record
+            float e = 0.0f, es = 0.0f;
+            for (int j = 0; j < nchk; ++j) {
+                const double *q = h + 3 * j;
+                e = std::max(e, q[2] > 0 ? (float)std::sqrt(q[0] / q[2]) : 0.0f);
+                es = std::max(es, q[1] > 0 ? (float)std::sqrt(q[0] / q[1]) : 0.0f);
+            }
             I.cst.checks++;
             I.cst.lastNrmse = e;
-            I.cst.lastScatterNrmse = h[1] > 0 ? (float)std::sqrt(h[0] / h[1]) : 0.0f;
+            I.cst.lastScatterNrmse = es;
             I.cst.worstNrmse = std::max(I.cst.worstNrmse, e);
+            I.cst.worstScatterNrmse = std::max(I.cst.worstScatterNrmse, es);
             if (!(e <= cfg_.cnnMaxNrmse)) {
                 // back to the scatter model, from this subset on
                 I.cst.active = false;
