@@ -456,26 +456,29 @@ __global__ __launch_bounds__(256) void k_att_src(const float *__restrict__ g0,
 // acc[2] += (sum attE)^2.
 __global__ __launch_bounds__(256) void k_scatter_check(const float *__restrict__ attE,
                                                        const float *__restrict__ attC,
+                                                       const int *__restrict__ cplane,
                                                        const float *__restrict__ rotx,
                                                        const float *__restrict__ afac,
-                                                       const int *__restrict__ bview, int B,
-                                                       int Nz, int NRot,
-                                                       double *__restrict__ acc)
+                                                       const int *__restrict__ bview, int Nz,
+                                                       int NRot, double *__restrict__ acc)
 {
+    // grid (plane blocks, checked views): view b's model source attE plane b,
+    // its learned source attC plane cplane[b], sums into acc[3b..3b+2]
     __shared__ double red[3][256];
-    const long plane = (long)NRot * Nz, vol = plane * NRot;
-    const long c = blockIdx.x * 256L + threadIdx.x;
+    const int plane = NRot * Nz;
+    const long vol = (long)plane * NRot;
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    const int b = blockIdx.y;
     double sd = 0.0, ss = 0.0, st = 0.0;
-    if (c < plane * B) {
-        const int b = (int)(c / plane);
-        const long q = c % plane;
+    if (q < plane) {
         const int v = bview[b];
+        const long ce = (long)b * vol, cc = (long)cplane[b] * vol;
         float dsum = 0.0f, ssum = 0.0f, tsum = 0.0f;
         for (int r = 0; r < NRot; ++r) {
-            const long o = (long)b * vol + r * plane + q;
-            const float e = attE[o];
-            dsum += e - attC[o];
-            ssum += e - rotx[o] * afac[(long)v * vol + r * plane + q];
+            const long o = (long)r * plane + q;
+            const float e = attE[ce + o];
+            dsum += e - attC[cc + o];
+            ssum += e - rotx[ce + o] * afac[(long)v * vol + o];
             tsum += e;
         }
         sd = (double)dsum * dsum;
@@ -495,9 +498,9 @@ __global__ __launch_bounds__(256) void k_scatter_check(const float *__restrict__
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        atomicAdd(acc, red[0][0]);
-        atomicAdd(acc + 1, red[1][0]);
-        atomicAdd(acc + 2, red[2][0]);
+        atomicAdd(acc + 3 * b, red[0][0]);
+        atomicAdd(acc + 3 * b + 1, red[1][0]);
+        atomicAdd(acc + 3 * b + 2, red[2][0]);
     }
 }
 
@@ -1330,6 +1333,8 @@ struct Recon::Impl {
     Recon::CnnStatus cst;
     int cnnSubsets = 0;           // modelled subsets seen
     bool cnnCheckNow = false;     // this subset is checked against the scatter model
+    DBuf<int> chkViews, chkPlanes;   // [S][2] checked views, their planes in the batch
+    std::vector<int> chkViewsHost;
     // captured sub-iterations, by (subset, scatter mode, keep U, learned
     // scatter in use); null until the key's second occurrence
     bool useGraphs = false;
@@ -2081,6 +2086,22 @@ Recon::Recon(const Config &cfg) : impl_(new Impl), cfg_(cfg)
     I.subsets.alloc(cfg.subsets.size());
     I.subsets.upload(cfg.subsets.data(), cfg.subsets.size(), I.stream);
     {
+        // learned-scatter check views of every subset (its view 0 and Vs/2)
+        // and their planes in the subset's batch
+        const int S = cfg.Vs > 0 ? (int)(cfg.subsets.size() / cfg.Vs) : 0;
+        I.chkViewsHost.resize(2 * (size_t)std::max(S, 1), 0);
+        for (int k = 0; k < S; ++k) {
+            I.chkViewsHost[2 * k] = cfg.subsets[(size_t)k * cfg.Vs];
+            I.chkViewsHost[2 * k + 1] = cfg.subsets[(size_t)k * cfg.Vs + cfg.Vs / 2];
+        }
+        I.chkViews.alloc(I.chkViewsHost.size());
+        I.chkViews.upload(I.chkViewsHost.data(), I.chkViewsHost.size(), I.stream);
+        const int planes[2] = {0, cfg.Vs / 2};
+        I.chkPlanes.alloc(2);
+        I.chkPlanes.upload(planes, 2, I.stream);
+        HIP_CHECK(hipStreamSynchronize(I.stream));   // 'planes' dies here
+    }
+    {
         std::vector<int> iota(g.V);
         for (int v = 0; v < g.V; ++v) iota[v] = v;
         I.allViews.alloc(g.V);
@@ -2688,15 +2709,15 @@ void Recon::runViews(int s, int first, int Vs, SrfMode srf, bool accumulate, boo
         if (I.cnnCheckNow && first == 0) {
             // the scatter model on two of the subset's views (its first and
             // middle one) as the check; one view where the batch has one
+            // (views chkViews[s] = subset view 0 and Vs/2, planes chkPlanes),
+            // both through one batched scatter-model pass
             const int nchk = Vs >= 2 ? 2 : 1;
-            const int chkb[2] = {0, Vs / 2};
+            const int *cv = nchk == 2 ? I.chkViews.p + 2 * s : bview;
             HIP_CHECK(hipMemsetAsync(I.chk.p, 0, 3 * nchk * sizeof(double), I.stream));
-            for (int j = 0; j < nchk; ++j) {
-                I.run_srf_att(afac, 1, bview + chkb[j], I.att2.p);
-                hipLaunchKernelGGL(k_scatter_check, dim3(div_up(I.P, threads)), dim3(threads), 0,
-                                   I.stream, I.att2.p, I.att.p + (long)chkb[j] * I.vol, I.rotx.p,
-                                   afac, bview + chkb[j], 1, g.Nz, g.NRot, I.chk.p + 3 * j);
-            }
+            I.run_srf_att(afac, nchk, cv, I.att2.p);
+            hipLaunchKernelGGL(k_scatter_check, dim3(div_up(I.P, threads), nchk), dim3(threads),
+                               0, I.stream, I.att2.p, I.att.p, I.chkPlanes.p, I.rotx.p, afac, cv,
+                               g.Nz, g.NRot, I.chk.p);
             double h[6] = {0, 0, 0, 0, 0, 0};
             HIP_CHECK(hipMemcpyAsync(h, I.chk.p, 3 * nchk * sizeof(double), hipMemcpyDeviceToHost,
                                      I.stream));
