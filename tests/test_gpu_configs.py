@@ -3,6 +3,8 @@ within the GPU suite's time: C1 in full against the oracle; the 128^3 x
 120-view geometry of C2 (one sub-iteration, spatial and FFT DRF) against the
 oracle; the C3 scatter model at 128^3 with its 33^3 kernel on the fused
 plane-transform path against the oracle on sampled detector columns; the
+C4 stage errors (projection, sensitivity, update) of the FP32 and BF16 DRF
+arithmetic against the FP64 oracle at the same geometry; the
 uneven batch of C5 at a noise-free fixed point, with a small member against
 the oracle."""
 import os
@@ -97,6 +99,100 @@ def test_c2_first_subiteration_matches_oracle(c2_subset0, drf):
     e = rel_l2(got, ref)
     print("rel L2 vs f32 oracle %.3g" % e)
     assert e <= 1e-4
+
+
+@pytest.fixture(scope="module")
+def c4_fp64():
+    """The 128^3 x 120-view geometry with attenuation and the 31x31 DRF
+    (the scatter workload's geometry without its scatter model, whose direct
+    FP64 sum is out of reach at this size) and its FP64 oracle stages:
+    forward projections of the phantom at views 0 and 40, the subset-0
+    sensitivity image (sum over the subset's 15 views of BackProject of
+    ones), and one update on subset 0 from a uniform estimate."""
+    c = synth.CONFIGS["c3"]
+    n, nz, pix, V, seed = c["n"], c["nz"], c["pix"], c["views"], 6
+    act, mu = synth.phantom(n, nz, seed, n_inserts=c["inserts"])
+    views = synth.views_ellipse(V, n, seed, 20.0, 28.0)
+    drf_file = os.path.join(tempfile.mkdtemp(prefix="irlc4"), "c4.drft")
+    offs, psfs, start = synth.drf_for_orbit(drf_file, pix, n, views, c["half"])
+    p = L.IrlParms()
+    p.NumPixels, p.NumSlices, p.NumViews = n, nz, V
+    p.BinWidth, p.fAtnScaleFac, p.fScatEstFac = pix, pix, 1.0
+    p.NumAngPerSubset, p.NumIterations, p.SrfCollapseFac = c["vs"], 1, 1
+    o = L.Options()
+    o.bModelDrf, o.bReconIsInitEst, o.iMsgLevel = 1, 1, 1
+    prj = np.zeros((V, nz, n), np.float32)
+    L.set_adv_options(0, 0)
+    rc = L.irl_genprj(p, o, views, drf_file, None, None, act.copy(), mu.copy(), prj)
+    assert rc == 0, L.error_string()
+    prj = synth.poisson(prj * V, c["counts"], seed)
+    x0 = np.full((nz, n, n), float(prj.sum()) / (n * n * nz * np.pi / 4 * V), np.float32)
+    drf = O.DrfTable(start, pix, offs, psfs)
+    atn = O.reorder(nz, n, n, mu * pix)
+    O.init_prj(nz, n, n, 3)
+    pviews = [0, 40]
+    xa = O.reorder(nz, n, n, act)
+    proj = [O.project(nz, n, n, 3, pix, views[k].Angle, views[k].CFCR, drf, atn, xa, fast=1)
+            for k in pviews]
+    sub = list(range(0, V, 8))
+    sens = np.zeros((n, nz, n))
+    ones = np.ones((nz, n))
+    for k in sub:
+        O.backproject(nz, n, n, 3, pix, views[k].Angle, views[k].CFCR, drf, atn, ones, sens)
+    ov = [dict(angle=views[k].Angle, left=views[k].Left, right=views[k].Right,
+               cfcr=views[k].CFCR) for k in sub]
+    upd = O.osem(nz, n, ov, 15, 1, 3, 3, pix, drf, prj[sub], atn, O.reorder(nz, n, n, x0))
+    upd = O.reorder(n, nz, n, upd.reshape(n, nz, n))
+    return dict(p=p, o=o, views=views, drf_file=drf_file, act=act, mu=mu, prj=prj, x0=x0,
+                pviews=pviews, proj=proj, sens=sens, upd=upd)
+
+
+# Stage errors of the FP32 path and of the BF16 DRF (per-plane modes from the
+# measured errors, and the always-split mode) against FP64; the scatter
+# model stays FP32 (its stage error is the column test above).
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("precision,tol", [("fp32", (1e-6, 1e-6, 1e-6)),
+                                           ("bf16", (1e-3, 1e-3, 1e-3)),
+                                           ("bf16x3", (2e-6, 2e-6, 4e-6))])
+def test_c4_per_stage_errors_vs_fp64(c4_fp64, precision, tol):
+    print("C4: 128^3, 120 views, ATN + DRF 31, stage errors, %s" % precision, flush=True)
+    d = c4_fp64
+    p, o, views = d["p"], d["o"], d["views"]
+    n, nz = p.NumPixels, p.NumSlices
+    L.set_adv_options(0, 0)
+    prev = L.set_precision(precision)
+    try:
+        # projection: IrlGenprj of the phantom at the two views (three-shear
+        # rotation, irlgenprj.c:19-21), one subset of both
+        q = L.IrlParms()
+        for f, _ in p._fields_:
+            setattr(q, f, getattr(p, f))
+        q.NumViews = q.NumAngPerSubset = 2
+        pv = [views[k] for k in d["pviews"]]
+        gp = np.zeros((2, nz, n), np.float32)
+        rc = L.irl_genprj(q, o, pv, d["drf_file"], None, None, d["act"].copy(), d["mu"].copy(),
+                          gp)
+        assert rc == 0, L.error_string()
+        e_prj = max(rel_l2(gp[k].astype(np.float64) * 2, d["proj"][k]) for k in range(2))
+        # sensitivity and one update (bilinear rotation, osemrecon.c:302)
+        L.set_adv_options(0, 0)
+        rec = L.Recon(p, o, views, d["drf_file"], None, None, d["mu"], d["prj"], d["x0"])
+    finally:
+        L.set_precision(prev)
+    try:
+        ninv = rec.norm(0)
+        rec.subiteration(0)
+        got = rec.estimate()
+    finally:
+        rec.close()
+    sens = d["sens"]
+    m = ninv > 0
+    e_sens = rel_l2(1.0 / ninv[m].astype(np.float64), sens[m])
+    e_upd = rel_l2(got, d["upd"])
+    print("rel L2 vs FP64: projection %.3g, sensitivity %.3g, update %.3g" %
+          (e_prj, e_sens, e_upd))
+    assert np.count_nonzero(m) > 0.5 * m.size
+    assert e_prj <= tol[0] and e_sens <= tol[1] and e_upd <= tol[2]
 
 
 def _c3_inputs(nviews, seed=5):
