@@ -660,6 +660,43 @@ def traffic_summary(fetch_csv, write_csv, probe_fetch_csv, probe_write_csv, prob
             "kernels": kernels}
 
 
+def kernel_summary(stats_csv, top=25):
+    """Markdown table of a rocprofv3 --stats kernel_stats.csv (kernel,
+    calls, total and average ms, share)."""
+    rows = []
+    with open(stats_csv) as f:
+        for r in csv.DictReader(f):
+            name = r["Name"].split("(")[0].replace("void ", "").replace("irlamd::", "")
+            rows.append((name, int(r["Calls"]), float(r["TotalDurationNs"]) * 1e-6,
+                         float(r["AverageNs"]) * 1e-6, float(r["Percentage"])))
+    rows.sort(key=lambda t: -t[2])
+    out = ["| kernel | calls | total ms | avg ms | % |", "|---|---|---|---|---|"]
+    for name, n, tot, avg, pct in rows[:top]:
+        out.append("| %s | %d | %.3f | %.4f | %.1f |" % (name, n, tot, avg, pct))
+    return "\n".join(out)
+
+
+def counter_summary(counter_csv, top=25):
+    """Markdown table of the per-dispatch means of every counter of a
+    rocprofv3 counter_collection.csv, per kernel."""
+    vals, names = {}, []
+    with open(counter_csv) as f:
+        for r in csv.DictReader(f):
+            k = r["Kernel_Name"].split("(")[0].replace("void ", "").replace("irlamd::", "")
+            c = r["Counter_Name"]
+            if c not in names:
+                names.append(c)
+            vals.setdefault(k, {}).setdefault(c, []).append(float(r["Counter_Value"]))
+    out = ["| kernel | dispatches | " + " | ".join(names) + " |",
+           "|---|---|" + "---|" * len(names)]
+    ks = sorted(vals, key=lambda k: -len(next(iter(vals[k].values()))))[:top]
+    for k in ks:
+        n = len(next(iter(vals[k].values())))
+        out.append("| %s | %d | " % (k, n) + " | ".join(
+            "%.4g" % np.mean(vals[k][c]) if c in vals[k] else "" for c in names) + " |")
+    return "\n".join(out)
+
+
 # ---------------------------------------------------------------------------
 
 def main():
@@ -688,7 +725,17 @@ def main():
     ap.add_argument("--traffic-summary", nargs=4,
                     metavar=("FETCH_CSV", "WRITE_CSV", "PROBE_FETCH_CSV", "PROBE_WRITE_CSV"),
                     help="print the HBM traffic summary of rocprofv3 counter runs")
+    ap.add_argument("--kernel-summary", metavar="KERNEL_STATS_CSV",
+                    help="print a markdown table of a rocprofv3 kernel_stats.csv")
+    ap.add_argument("--counter-summary", metavar="COUNTER_COLLECTION_CSV",
+                    help="print a markdown table of a rocprofv3 counter_collection.csv")
     args = ap.parse_args()
+    if args.kernel_summary:
+        print(kernel_summary(args.kernel_summary))
+        return
+    if args.counter_summary:
+        print(counter_summary(args.counter_summary))
+        return
     if args.traffic_summary:
         print(json.dumps(traffic_summary(*args.traffic_summary,
                                          probe_bytes=args.probe_mib * (1 << 20)), indent=1))

@@ -66,6 +66,16 @@ __device__ __forceinline__ void tap_off(int t, int dil, int &dy, int &dx, int &d
     dz = dil * (t % 3 - 1);
 }
 
+// Block index in an XCD-contiguous order: workgroups are dispatched to the
+// 8 XCDs round-robin (workgroup i on XCD i % 8), so block i takes logical
+// block (i % 8) * (n / 8) + i / 8 and each XCD works through one contiguous
+// range of rows, whose neighbour rows then stay in that XCD's L2.
+__device__ __forceinline__ unsigned xcd_contiguous_block()
+{
+    const unsigned n = gridDim.x, i = blockIdx.x;
+    return n % 8 == 0 ? (i % 8) * (n / 8) + i / 8 : i;
+}
+
 // ---------------------------------------------------------------------------
 // Mean of a volume (FP64 accumulation): partial sums per block, then 1/mean.
 // ---------------------------------------------------------------------------
@@ -276,7 +286,7 @@ __global__ __launch_bounds__(256) void k_conv_l1_rows(const float2 *__restrict__
     __shared__ float2 seg[4][9 * SL];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int zt = (Z + 63) / 64;
-    const long task = (long)blockIdx.x * 4 + w;
+    const long task = (long)xcd_contiguous_block() * 4 + w;
     const bool valid = task < (long)Y * X * zt;
     const long row = valid ? task / zt : 0;
     const int z0 = valid ? (int)(task % zt) * 64 : 0;
@@ -363,7 +373,7 @@ __global__ __launch_bounds__(256) void k_conv_mfma_rows(const uint16_t *__restri
     const int n = lane & 15, g = lane >> 4;
     constexpr int NJ = ROWZ / 16;
     const int zt = (Z + ROWZ - 1) / ROWZ;
-    const long task = (long)blockIdx.x * 4 + w;
+    const long task = (long)xcd_contiguous_block() * 4 + w;
     const long rows = (long)Y * X;
     const bool valid = task < rows * zt;
     const long row = valid ? task / zt : 0;
