@@ -269,14 +269,18 @@ __global__ __launch_bounds__(256) void k_conv_valu(const TI *__restrict__ in,
 }
 
 // ---------------------------------------------------------------------------
-// Layer 1 (2 -> 32 channels, dilation 1) of the inference path on the VALU:
-// a wave owns 64 slices of one (y, x) row and stages the 9 neighbour rows'
-// slices [z0 - 1, z0 + 65) of both input channels in LDS (channel 0 times
-// scal[0]), then each lane accumulates its voxel's 32 outputs over the 27
-// taps from LDS in the order of k_conv_valu (same arithmetic), bf16 out.
+// Layer 1 (2 -> 32 channels, dilation 1) of the inference path on MFMA: the
+// 27 taps x 2 channels form K = 54 (padded to 64, two K = 32 steps) of
+// out^T[co][voxel] = sum_k W1[co][k] in[k][voxel], k = 2 tap + channel.  A
+// wave owns 64 slices of one (y, x) row, stages the 9 neighbour rows' slices
+// [z0 - 1, z0 + 65) of both channels in LDS once (channel 0 times scal[0]),
+// and builds each lane's B fragment (voxel n, k = 32 s + 8 (l/16) .. +7,
+// i.e. 4 taps x 2 channels) from there, rounded to bf16.  A fragments
+// (bf16 [2 s][2 co tiles][64 lanes][8], k >= 54 zero) come from the caches.
+// Bias + ReLU, bf16 [V][32] out.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_conv_l1_rows(const float2 *__restrict__ in,
-                                                      const float *__restrict__ W,
+__global__ __launch_bounds__(256) void k_conv_l1_mfma(const float2 *__restrict__ in,
+                                                      const uint4 *__restrict__ Wf,
                                                       const float *__restrict__ bias,
                                                       const float *__restrict__ scal,
                                                       uint16_t *__restrict__ out, int Y, int X,
@@ -285,6 +289,7 @@ __global__ __launch_bounds__(256) void k_conv_l1_rows(const float2 *__restrict__
     constexpr int SL = 64 + 2;
     __shared__ float2 seg[4][9 * SL];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int n = lane & 15, g = lane >> 4;
     const int zt = (Z + 63) / 64;
     const long task = (long)xcd_contiguous_block() * 4 + w;
     const bool valid = task < (long)Y * X * zt;
@@ -301,33 +306,48 @@ __global__ __launch_bounds__(256) void k_conv_l1_rows(const float2 *__restrict__
         seg[w][e] = v;
     }
     __syncthreads();
-    const int z = z0 + lane;
-    if (!valid || z >= Z) return;
-    float acc[32];
+    cfloatx4_t acc[2][4];
 #pragma unroll
-    for (int co = 0; co < 32; ++co) acc[co] = bias[co];
-    for (int q = 0; q < 9; ++q)
+    for (int c = 0; c < 2; ++c)
 #pragma unroll
-        for (int dz = 0; dz < 3; ++dz) {
-            const float2 v = seg[w][q * SL + lane + dz];
-            const float *wt = W + (3 * q + dz) * 64;
+        for (int j = 0; j < 4; ++j) acc[c][j] = cfloatx4_t{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-            for (int co = 0; co < 32; ++co) acc[co] = fmaf(wt[co], v.x, acc[co]);
+    for (int st = 0; st < 2; ++st) {
+        cbf16x8_t A[2];
 #pragma unroll
-            for (int co = 0; co < 32; ++co) acc[co] = fmaf(wt[32 + co], v.y, acc[co]);
+        for (int c = 0; c < 2; ++c)
+            A[c] = __builtin_bit_cast(cbf16x8_t, Wf[(st * 2 + c) * 64 + lane]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            cbf16x8_t B;
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                const int t = 16 * st + 4 * g + h;   // tap (27 and up: padding)
+                float2 v = make_float2(0.0f, 0.0f);
+                if (t < 27) v = seg[w][(t / 3) * SL + 16 * j + n + t % 3];
+                B[2 * h] = (__bf16)v.x;
+                B[2 * h + 1] = (__bf16)v.y;
+            }
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+                acc[c][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[c], B, acc[c][j], 0, 0, 0);
         }
-    uint16_t *o = out + (((long)y * X + x) * Z + z) * 32;
+    }
+    if (!valid) return;
 #pragma unroll
-    for (int c8 = 0; c8 < 32; c8 += 8) {
-        uint32_t qv[4];
+    for (int j = 0; j < 4; ++j) {
+        const int z = z0 + 16 * j + n;
+        if (z >= Z) continue;
+        const long v = row * Z + z;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            uint16_t h0, h1;
-            from_f(h0, fmaxf(acc[c8 + 2 * k], 0.0f));
-            from_f(h1, fmaxf(acc[c8 + 2 * k + 1], 0.0f));
-            qv[k] = (uint32_t)h0 | ((uint32_t)h1 << 16);
+        for (int c = 0; c < 2; ++c) {
+            const int co = 16 * c + 4 * g;
+            uint16_t h[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) from_f(h[i], fmaxf(acc[c][j][i] + bias[co + i], 0.0f));
+            *(uint2 *)(out + v * 32 + co) = make_uint2((uint32_t)h[0] | ((uint32_t)h[1] << 16),
+                                                       (uint32_t)h[2] | ((uint32_t)h[3] << 16));
         }
-        *(uint4 *)(o + c8) = make_uint4(qv[0], qv[1], qv[2], qv[3]);
     }
 }
 
@@ -606,12 +626,17 @@ __global__ void k_to_bf16_t(const float *__restrict__ W, uint16_t *__restrict__ 
     from_f(Wb[i], v);
 }
 
-// FP32 weights rounded to BF16 values (the VALU layers of the inference path).
-__global__ void k_round_bf16(const float *__restrict__ W, float *__restrict__ Wr, long n)
+// Layer 1's A fragments from its master weights [27][2][32]: entry
+// ((s * 2 + c) * 64 + l) * 8 + i is W[tap][ch][co] for co = 16 c + l % 16,
+// k = 32 s + 8 (l / 16) + i = 2 tap + ch (zero for tap >= 27), in bf16.
+__global__ void k_l1_fragments(const float *__restrict__ W, uint16_t *__restrict__ Wf)
 {
-    const long i = blockIdx.x * 256L + threadIdx.x;
-    if (i >= n) return;
-    Wr[i] = (float)(__bf16)W[i];
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= 2 * 2 * 64 * 8) return;
+    const int i = e % 8, l = (e / 8) % 64, c = (e / 512) % 2, st = e / 1024;
+    const int co = 16 * c + l % 16, k = 32 * st + 8 * (l / 16) + i;
+    const int t = k / 2, ch = k % 2;
+    from_f(Wf[e], t < 27 ? W[(t * 2 + ch) * 32 + co] : 0.0f);
 }
 
 // ---------------------------------------------------------------------------
@@ -806,7 +831,7 @@ void ScatterCnn::alloc_state()
         CNN_CHECK(hipMemset(vb_[l], 0, nb * 4));
     }
     Wt_ = cnn_alloc<float>(27L * 32 * 32);
-    Wr0_ = cnn_alloc<float>(wcount(0));
+    Wl1_ = cnn_alloc<uint16_t>(2 * 2 * 64 * 8);
     dsum_ = cnn_alloc<double>(1);
     one_ = cnn_alloc<float>(2);
     const float ones[2] = {1.0f, 1.0f};
@@ -879,7 +904,7 @@ ScatterCnn::~ScatterCnn()
     for (auto &a : grad_) cnn_free(a);
     for (auto &a : actb_) cnn_free(a);
     cnn_free(Wt_);
-    cnn_free(Wr0_);
+    cnn_free(Wl1_);
     cnn_free(dsum_);
     cnn_free(one_);
 }
@@ -966,14 +991,14 @@ void ScatterCnn::upload_bf16_weights(hipStream_t s)
         hipLaunchKernelGGL(k_to_bf16_t, dim3(cdiv_u(n, 256)), dim3(256), 0, s, W_[l], Wbf_[l],
                            cnn_cin(l), cnn_cout(l), COP);
     }
-    hipLaunchKernelGGL(k_round_bf16, dim3(cdiv_u(wcount(0), 256)), dim3(256), 0, s, W_[0], Wr0_,
-                       wcount(0));
+    static_assert(cnn_cin(0) == 2 && cnn_cout(0) == 32, "layer 1 fragments");
+    hipLaunchKernelGGL(k_l1_fragments, dim3(8), dim3(256), 0, s, W_[0], Wl1_);
     CNN_CHECK(hipGetLastError());
     bfDirty_ = false;
 }
 
-// FP32 master weights of the VALU layers rounded to BF16 values, kept in
-// the input-gradient buffer (free outside training steps).
+// Network forward: FP32 throughout (training arithmetic), or the inference
+// arithmetic (bf16 weights and activations on MFMA, FP32 accumulation).
 void ScatterCnn::forward(const float *in, int Y, int X, int Z, float *out, bool bf16,
                          hipStream_t s)
 {
@@ -1007,8 +1032,8 @@ void ScatterCnn::forward_bf16(const float *in, int Y, int X, int Z, float *out, 
 {
     const dim3 blk(256);
     const long tasks = (long)Y * X * ((Z + CNN_ROWZ - 1) / CNN_ROWZ);
-    hipLaunchKernelGGL(k_conv_l1_rows, dim3(cdiv_u(tasks, 4)), blk, 0, s, (const float2 *)in,
-                       Wr0_, b_[0], scal ? scal : one_, a0, Y, X, Z);
+    hipLaunchKernelGGL(k_conv_l1_mfma, dim3(cdiv_u(tasks, 4)), blk, 0, s, (const float2 *)in,
+                       (const uint4 *)Wl1_, b_[0], scal ? scal : one_, a0, Y, X, Z);
     static_assert(cnn_dil(CNN_LAYERS - 1) == CNN_MAXDIL, "largest dilation");
     switch (rowz_) {
     case 64: mfma_layers<64>(Y, X, Z, out, a0, a1, s); break;

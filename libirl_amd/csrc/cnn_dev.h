@@ -128,7 +128,7 @@ private:
     float *mW_[CNN_LAYERS] = {}, *vW_[CNN_LAYERS] = {}, *mb_[CNN_LAYERS] = {}, *vb_[CNN_LAYERS] = {};
     float *Wt_ = nullptr;            // flipped/transposed weights for the input gradient
     uint16_t *Wbf_[CNN_LAYERS] = {}; // bf16 [27][max(Cout,16)][Cin] (MFMA layers 2-4)
-    float *Wr0_ = nullptr;           // layer 1 weights rounded to bf16 values
+    uint16_t *Wl1_ = nullptr;        // layer 1 A fragments (bf16 [2][2][64][8])
     bool bfDirty_ = true;
     // work buffers for one sample
     long workV_ = 0;
