@@ -381,7 +381,7 @@ __device__ __forceinline__ void cnn_wave_sync()
 }
 
 template <int NCT, int COUT, int ROWZ>
-__global__ __launch_bounds__(256) void k_conv_mfma_rows(const uint16_t *__restrict__ in,
+__global__ __launch_bounds__(256, 4) void k_conv_mfma_rows(const uint16_t *__restrict__ in,
                                                         const uint4 *__restrict__ Wg,
                                                         const float *__restrict__ bias,
                                                         void *__restrict__ outp, int Y, int X,
@@ -440,21 +440,27 @@ __global__ __launch_bounds__(256) void k_conv_mfma_rows(const uint16_t *__restri
             if (e < SL * 4) my[slot(e >> 2, e & 3)] = pre[k];
         }
         cnn_wave_sync();
+        // the offset's A fragments before the next segment's loads: vector
+        // loads retire in order, so waiting for A does not wait for the
+        // prefetch
+        cbf16x8_t A[3][NCT];
+#pragma unroll
+        for (int dzi = 0; dzi < 3; ++dzi)
+#pragma unroll
+            for (int c = 0; c < NCT; ++c)
+                A[dzi][c] = __builtin_bit_cast(cbf16x8_t,
+                                               Wg[(3 * q + dzi) * WQ + (16 * c + n) * 4 + g]);
         if (q + 1 < 9) fetch(q + 1);
 #pragma unroll
         for (int dzi = 0; dzi < 3; ++dzi) {
-            const int t = 3 * q + dzi;
-            cbf16x8_t A[NCT];
-#pragma unroll
-            for (int c = 0; c < NCT; ++c)
-                A[c] = __builtin_bit_cast(cbf16x8_t, Wg[t * WQ + (16 * c + n) * 4 + g]);
 #pragma unroll
             for (int j = 0; j < NJ; ++j) {
                 const cbf16x8_t B =
                     __builtin_bit_cast(cbf16x8_t, my[slot(16 * j + n + dzi * dil, g)]);
 #pragma unroll
                 for (int c = 0; c < NCT; ++c)
-                    acc[c][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[c], B, acc[c][j], 0, 0, 0);
+                    acc[c][j] =
+                        __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[dzi][c], B, acc[c][j], 0, 0, 0);
             }
         }
     }
