@@ -34,7 +34,7 @@ typedef struct _OsemData {
 } sOsemData, *psOsemData;
 
 typedef struct _AdvOptions {
-    int bFastRotate;      /* accepted; the GPU path always rotates bilinearly */
+    int bFastRotate;      /* 1 (default): three-shear rotation (fastrotate.c), 0: bilinear (rot3dpar.c) */
     int iStartIteration;
 } sAdvOptions, *psAdvOptions;
 
