@@ -529,7 +529,7 @@ def batch_line(tmpdir, net, steps=16, warmup=24):
     # warmup: every member's subsets run eagerly once, then are captured and
     # replayed once, so the timed steps replay warm graph executables
     est = [r.estimate() for r in recs]          # after warmup + steps = 5 iterations
-    tiles = [r.tiles() for r in recs]
+    tiles = [r.tiles() + (r.cnn_row_length(),) for r in recs]
     status = [L.cnn_status(r) for r in recs]
     # the same steps as one parent-graph launch per step (its executables
     # instantiated first), each mode timed twice, alternating
@@ -571,7 +571,7 @@ def batch_line(tmpdir, net, steps=16, warmup=24):
             "one_at_a_time_value": n * steps / dt_seq,
             "batch_speedup_vs_one_at_a_time": dt_seq / dt_batch,
             "sizes": [[c["n"], c["views"]] for c in cfgs],
-            "tiles_prj_bck": tiles,
+            "tiles_prj_bck_cnnrow": tiles,
             "replays": st["replays"],
             "eager_subiterations": st["eager_subiterations"],
             "learned_scatter_active": sum(s["active"] for s in status),

@@ -205,6 +205,11 @@ int irlamd_set_scatter_cnn(irlamd_cnn *h, int iCheckEvery, float fMaxNrmse);
 /* Diagnostic: the same first-order source with the terms from one FFT
  * convolution of the unrotated estimate instead of the network. */
 int irlamd_set_scatter_cnn_exact(int bExact);
+/* Slices per wave of the scatter network's MFMA layers that a reconstruction
+ * uses (its own choice: timed at setup with irlamd_set_autotune(1), else 64;
+ * 0 without the learned scatter). */
+int irlamd_recon_cnn_row_length(irlamd_recon *r);
+
 /* State of the learned scatter of a reconstruction (or, with r = NULL, of the
  * last IrlOsem call): whether it is still in use, checks run, subsets run on
  * the network, last and worst checked relative L2 of the projected source

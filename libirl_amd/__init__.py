@@ -228,6 +228,7 @@ def load():
     lib.irlamd_recon_cnn_status.argtypes = [vp] + [ctypes.POINTER(ctypes.c_int)] * 3 + \
         [ctypes.POINTER(ctypes.c_float)] * 4
     lib.irlamd_set_graphs.argtypes = [ctypes.c_int]
+    lib.irlamd_recon_cnn_row_length.argtypes = [vp]
     lib.irlamd_set_autotune.argtypes = [ctypes.c_int]
     lib.irlamd_recon_tiles.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     lib.irlamd_set_drf_fft.argtypes = [ctypes.c_int]
@@ -468,6 +469,10 @@ class Recon:
         a, b = ctypes.c_int(), ctypes.c_int()
         self._lib.irlamd_recon_tiles(self._h, ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
+
+    def cnn_row_length(self):
+        """Slices per wave of the scatter network's MFMA layers in use (0: none)."""
+        return self._lib.irlamd_recon_cnn_row_length(self._h)
 
     def drf_algorithms(self):
         """FP32 DRF algorithm per direction ("spatial" or "fft") and the FFT length."""

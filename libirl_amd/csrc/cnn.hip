@@ -1028,24 +1028,24 @@ void ScatterCnn::forward(const float *in, int Y, int X, int Z, float *out, bool 
         return;
     }
     upload_bf16_weights(s);
-    forward_bf16(in, Y, X, Z, out, actb_[0], actb_[1], nullptr, s);
+    forward_bf16(in, Y, X, Z, out, actb_[0], actb_[1], nullptr, rowz_, s);
 }
 
 // Inference arithmetic, activations in the given buffers ([V][32] bf16 each);
 // channel 0 of the input is multiplied by scal[0] (1 for normalised input).
 void ScatterCnn::forward_bf16(const float *in, int Y, int X, int Z, float *out, uint16_t *a0,
-                              uint16_t *a1, const float *scal, hipStream_t s)
+                              uint16_t *a1, const float *scal, int rowz, hipStream_t s)
 {
     const dim3 blk(256);
     const long tasks = (long)Y * X * ((Z + CNN_ROWZ - 1) / CNN_ROWZ);
     hipLaunchKernelGGL(k_conv_l1_mfma, dim3(cdiv_u(tasks, 4)), blk, 0, s, (const float2 *)in,
                        (const uint4 *)Wl1_, b_[0], scal ? scal : one_, a0, Y, X, Z);
     static_assert(cnn_dil(CNN_LAYERS - 1) == CNN_MAXDIL, "largest dilation");
-    switch (rowz_) {
+    switch (rowz) {
     case 64: mfma_layers<64>(Y, X, Z, out, a0, a1, s); break;
     case 32: mfma_layers<32>(Y, X, Z, out, a0, a1, s); break;
     case 16: mfma_layers<16>(Y, X, Z, out, a0, a1, s); break;
-    default: fail(ETYPE_ILLEGAL_VALUE, "scatter CNN row length %d (64, 32 or 16)", rowz_);
+    default: fail(ETYPE_ILLEGAL_VALUE, "scatter CNN row length %d (64, 32 or 16)", rowz);
     }
     CNN_CHECK(hipGetLastError());
 }
@@ -1183,7 +1183,7 @@ void ScatterCnn::sources(const float *x, const float *mu, int N, int Nz, const C
     if (pb > w.nb) fail(ETYPE_ILLEGAL_VALUE, "scatter CNN work buffers too small");
     hipLaunchKernelGGL(k_cnn_pool, dim3(pb), dim3(256), 0, s, x, mu, N, Nz, ph.fDens, w.in, w.psum);
     hipLaunchKernelGGL(k_inv_mean, dim3(1), dim3(256), 0, s, w.psum, (int)pb, n, w.scal);
-    forward_bf16(w.in, Y, X, Z, w.out, w.actb[0], w.actb[1], w.scal, s);
+    forward_bf16(w.in, Y, X, Z, w.out, w.actb[0], w.actb[1], w.scal, w.rowz, s);
     hipLaunchKernelGGL(k_cnn_source, dim3(cdiv_u(n, 256)), dim3(256), 0, s, w.out, x, mu, N, Nz,
                        w.scal, kappa_, ph.fDens, ph.cntr, primary, g0, g1);
     CNN_CHECK(hipGetLastError());

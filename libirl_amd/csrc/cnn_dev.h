@@ -58,6 +58,7 @@ struct CnnWork {
     double *psum = nullptr;              // block sums of x (one per 256 coarse voxels)
     long nb = 0;
     float *scal = nullptr;               // [0] 1/mean(x), [1] mean(x)
+    int rowz = 64;                       // slices per wave of the MFMA layers (64, 32, 16)
     CnnWork(int N, int Nz);
     ~CnnWork();
     CnnWork(const CnnWork &) = delete;
@@ -102,7 +103,8 @@ public:
     void prepareInference();
 
     // Slices per wave of the inference path's MFMA layers (64, 32 or 16;
-    // the result does not depend on it).
+    // the result does not depend on it) for forward(); sources() takes the
+    // work buffers' own choice (CnnWork::rowz).
     void setRowLength(int rowz);
     int rowLength() const { return rowz_; }
 
@@ -114,7 +116,7 @@ private:
     void ensure_work(long V);
     void upload_bf16_weights(hipStream_t s);
     void forward_bf16(const float *in, int Y, int X, int Z, float *out, uint16_t *a0,
-                      uint16_t *a1, const float *scal, hipStream_t s);
+                      uint16_t *a1, const float *scal, int rowz, hipStream_t s);
     template <int ROWZ>
     void mfma_layers(int Y, int X, int Z, float *out, uint16_t *a0, uint16_t *a1, hipStream_t s);
 

@@ -111,7 +111,7 @@ public:
         // Learned scatter: the network's terms replace the per-view FFT
         // convolutions of the scatter model while the check agrees with the
         // model (every cnnCheckEvery-th modelled subset also runs the model on
-        // its first view and compares the projected sources; rel. L2 >
+        // two of its views and compares the projected sources; rel. L2 >
         // cnnMaxNrmse switches back to the model).
         std::shared_ptr<ScatterCnn> cnn;
         int cnnCheckEvery = 0;             // 0: check the first modelled subset only
@@ -180,6 +180,7 @@ public:
     // subset 0 (slice groups per block of the FP32 kernel, slice tiles per
     // wave of the BF16 kernel), for the projector and the backprojector.
     void autotuneTiles();
+    int cnnRowLength() const;   // the scatter network's row length in use (0: none)
     void tiles(int *prj, int *bck) const;
     // FP32 DRF algorithm in use per direction: 0 spatial, 1 FFT (length P)
     void drfAlgorithms(int *prj, int *bck, int *P) const;

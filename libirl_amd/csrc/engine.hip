@@ -2413,13 +2413,33 @@ void *Recon::captureSubIteration(int s, bool keepUpdate, SrfMode srf)
 void Recon::autotuneTiles()
 {
     Impl &I = *impl_;
-    if (cfg_.drf.nDist <= 0) return;
-    const int KYb = I.ky_bucket();
-    const int *bview = I.timing_views();
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
-    for (int bck = 0; bck < 2; ++bck) {
+    // the scatter network's row length (slices per wave of its MFMA layers):
+    // the source build timed for each, on the current estimate
+    if (I.cnnWork && !cfg_.cnnExact && cfg_.cnn) {
+        int best = 64;
+        float bestMs = 1e30f;
+        for (int rz : {64, 32, 16}) {
+            I.cnnWork->rowz = rz;
+            I.run_cnn_sources();
+            HIP_CHECK(hipEventRecord(e0, I.stream));
+            for (int r = 0; r < 3; ++r) I.run_cnn_sources();
+            HIP_CHECK(hipEventRecord(e1, I.stream));
+            HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0;
+            HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < bestMs) {
+                bestMs = ms;
+                best = rz;
+            }
+        }
+        I.cnnWork->rowz = best;
+    }
+    const int KYb = cfg_.drf.nDist > 0 ? I.ky_bucket() : 0;
+    const int *bview = I.timing_views();
+    for (int bck = 0; bck < 2 && cfg_.drf.nDist > 0; ++bck) {
         const DrfArgs da = I.drf_args(bck != 0, bview, bck ? cfg_.bckModel : cfg_.prjModel);
         int best = 0;
         float bestMs = 1e30f;
@@ -2447,6 +2467,8 @@ void Recon::tiles(int *prj, int *bck) const
     *prj = impl_->tilePrj;
     *bck = impl_->tileBck;
 }
+
+int Recon::cnnRowLength() const { return impl_->cnnWork ? impl_->cnnWork->rowz : 0; }
 
 void Recon::drfAlgorithms(int *prj, int *bck, int *P) const
 {

@@ -220,3 +220,28 @@ def test_graphs_prepared_at_setup_equal_eager(learned):
             assert np.array_equal(a.estimate(), b.estimate()), (it, s)
     if learned:
         assert L.cnn_status(a) == L.cnn_status(b)
+
+
+def test_autotuned_network_row_length_gives_the_same_result():
+    """The learned scatter's row length (slices per wave of the network's
+    MFMA layers) timed at setup: any choice gives the same sources, so the
+    reconstruction equals the untuned one bit for bit."""
+    print("autotune with learned scatter", flush=True)
+    import libirl_amd as L
+    pb = Problem(n=32, nz=16, nviews=8, per=2, iters=2, half=3, noise=False)
+    krnl = _krnl_file(pb.pix)
+    net = L.ScatterCnn.create(kappa=1.0, seed=4)
+    L.set_scatter_cnn(net, check_every=0, max_nrmse=1e9)
+    try:
+        a = _recon(pb, krnl, 2)
+        b = _recon(pb, krnl, 2, autotune=True)
+    finally:
+        L.set_scatter_cnn(None, check_every=0, max_nrmse=0.01)   # the library defaults
+    assert a.cnn_row_length() == 64 and b.cnn_row_length() in (64, 32, 16)
+    print("row length", b.cnn_row_length())
+    for it in range(2):
+        for s in range(a.num_subsets):
+            a.subiteration(s)
+            b.subiteration(s)
+    assert np.array_equal(a.estimate(), b.estimate())
+    assert L.cnn_status(a)["active"] and L.cnn_status(b)["active"]
