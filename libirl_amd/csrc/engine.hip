@@ -462,34 +462,42 @@ __global__ __launch_bounds__(256) void k_scatter_check(const float *__restrict__
                                                        const int *__restrict__ bview, int Nz,
                                                        int NRot, double *__restrict__ acc)
 {
-    // grid (plane blocks, checked views): view b's model source attE plane b,
-    // its learned source attC plane cplane[b], sums into acc[3b..3b+2]
-    __shared__ double red[3][256];
+    // grid (blocks of 64 columns, checked views): view b's model source
+    // attE plane b, its learned source attC plane cplane[b], sums into
+    // acc[3b..3b+2].  The block's four waves take every fourth row of the
+    // same 64 columns; their partial depth sums are added in row-group order.
+    __shared__ float part[3][4][64];
+    __shared__ double red[3][64];
     const int plane = NRot * Nz;
     const long vol = (long)plane * NRot;
-    const int q = blockIdx.x * 256 + threadIdx.x;
+    const int col = threadIdx.x & 63, rg = threadIdx.x >> 6;
+    const int q = blockIdx.x * 64 + col;
     const int b = blockIdx.y;
-    double sd = 0.0, ss = 0.0, st = 0.0;
+    float dsum = 0.0f, ssum = 0.0f, tsum = 0.0f;
     if (q < plane) {
         const int v = bview[b];
         const long ce = (long)b * vol, cc = (long)cplane[b] * vol;
-        float dsum = 0.0f, ssum = 0.0f, tsum = 0.0f;
-        for (int r = 0; r < NRot; ++r) {
+        for (int r = rg; r < NRot; r += 4) {
             const long o = (long)r * plane + q;
             const float e = attE[ce + o];
             dsum += e - attC[cc + o];
             ssum += e - rotx[ce + o] * afac[(long)v * vol + o];
             tsum += e;
         }
-        sd = (double)dsum * dsum;
-        ss = (double)ssum * ssum;
-        st = (double)tsum * tsum;
     }
-    red[0][threadIdx.x] = sd;
-    red[1][threadIdx.x] = ss;
-    red[2][threadIdx.x] = st;
+    part[0][rg][col] = dsum;
+    part[1][rg][col] = ssum;
+    part[2][rg][col] = tsum;
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
+    if (rg == 0) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float t = ((part[k][0][col] + part[k][1][col]) + part[k][2][col]) + part[k][3][col];
+            red[k][col] = (double)t * t;
+        }
+    }
+    __syncthreads();
+    for (int w = 32; w > 0; w >>= 1) {
         if (threadIdx.x < w) {
             red[0][threadIdx.x] += red[0][threadIdx.x + w];
             red[1][threadIdx.x] += red[1][threadIdx.x + w];
@@ -2737,7 +2745,7 @@ void Recon::runViews(int s, int first, int Vs, SrfMode srf, bool accumulate, boo
             const int *cv = nchk == 2 ? I.chkViews.p + 2 * s : bview;
             HIP_CHECK(hipMemsetAsync(I.chk.p, 0, 3 * nchk * sizeof(double), I.stream));
             I.run_srf_att(afac, nchk, cv, I.att2.p);
-            hipLaunchKernelGGL(k_scatter_check, dim3(div_up(I.P, threads), nchk), dim3(threads),
+            hipLaunchKernelGGL(k_scatter_check, dim3(div_up(I.P, 64), nchk), dim3(threads),
                                0, I.stream, I.att2.p, I.att.p, I.chkPlanes.p, I.rotx.p, afac, cv,
                                g.Nz, g.NRot, I.chk.p);
             double h[6] = {0, 0, 0, 0, 0, 0};
