@@ -100,8 +100,10 @@ def run_parms(w, subiterations):
 
 def time_subiterations(w, steps, warmup, precision="fp32", cnn=None, check_every=0,
                        drf="spatial", graphs=True, stage_steps=None):
-    """Device-resident reconstruction of the workload: every subset's
-    sub-iteration graph captured at setup (graphs on), 'warmup' untimed
+    """Device-resident reconstruction of the workload (tile shapes of the
+    DRF kernels and the network's row length timed at setup by the library's
+    autotuner): every subset's sub-iteration graph captured at setup (graphs
+    on), 'warmup' untimed
     sub-iterations, then exactly 'steps' timed ones replayed from the graphs,
     with no kernel timing.  Then 'stage_steps' (default 'steps') more
     sub-iterations run eagerly with per-kernel HIP events on the engine's
@@ -119,6 +121,7 @@ def time_subiterations(w, steps, warmup, precision="fp32", cnn=None, check_every
     prev = L.set_precision(precision)
     prev_drf = L.set_drf_fft(drf)
     prev_g = L.set_graphs(graphs)
+    prev_t = L.set_autotune(True)
     L.set_scatter_cnn(cnn, check_every=check_every, max_nrmse=0.01)
     try:
         rec = L.Recon(parms, w["opts"], w["views"], w["drf_file"], w["krnl_file"], None,
@@ -127,6 +130,7 @@ def time_subiterations(w, steps, warmup, precision="fp32", cnn=None, check_every
         L.set_precision(prev)
         L.set_drf_fft(prev_drf)
         L.set_graphs(prev_g)
+        L.set_autotune(prev_t)
         L.set_scatter_cnn(None)
     if graphs:
         rec.prepare_graphs()
@@ -340,6 +344,7 @@ def workload_line(w, steps, warmup, precision, ref_estimate=None, cnn=None, chec
     algo = rec.drf_algorithms()
     line = {"workload": name + (" (BF16 MFMA DRF, FP32 accumulation)" if bf16 else
                                 " (%s)" % DRF_NAMES[drf]),
+            "tiles_prj_bck_cnnrow": list(rec.tiles()) + [rec.cnn_row_length()],
             "value": steps / dt, "unit": "subiters/s", "ms_per_step": dt * 1e3 / steps,
             "drf_algorithm": {"prj": algo[0], "bck": algo[1], "fft_length": algo[2]}
             if not bf16 else "bf16 mfma",

@@ -97,6 +97,11 @@ int irlamd_set_autotune(int bOn);
 /* Tile shapes in use (0: the size heuristic): FP32 slice groups per block or
  * BF16 slice tiles per wave, of the projector and the backprojector. */
 int irlamd_recon_tiles(irlamd_recon *r, int *piPrj, int *piBck);
+/* Sets the DRF tile shapes (0: size heuristic; BF16: 2, 4 or 8 slice tiles
+ * per wave, FP32: 4 or 8 slice groups per block); captured sub-iteration
+ * graphs are dropped.  Results do not depend on the tiles.  No counterpart
+ * in the reference (tuning of this library's kernels). */
+int irlamd_recon_set_tiles(irlamd_recon *r, int iPrj, int iBck);
 
 /* Scatter model of reconstructions created afterwards: 1 (the default) =
  * where the grid allows (equal in-plane FFT lengths of 128 or 160, two

@@ -231,6 +231,7 @@ def load():
     lib.irlamd_recon_cnn_row_length.argtypes = [vp]
     lib.irlamd_set_autotune.argtypes = [ctypes.c_int]
     lib.irlamd_recon_tiles.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    lib.irlamd_recon_set_tiles.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     lib.irlamd_set_drf_fft.argtypes = [ctypes.c_int]
     lib.irlamd_set_esse_fused.argtypes = [ctypes.c_int]
     lib.irlamd_recon_drf_algorithms.argtypes = [vp] + [ctypes.POINTER(ctypes.c_int)] * 3
@@ -469,6 +470,10 @@ class Recon:
         a, b = ctypes.c_int(), ctypes.c_int()
         self._lib.irlamd_recon_tiles(self._h, ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
+
+    def set_tiles(self, prj, bck):
+        """DRF tile shapes (0: size heuristic; BF16 2/4/8, FP32 4/8)."""
+        self._check(self._lib.irlamd_recon_set_tiles(self._h, prj, bck))
 
     def cnn_row_length(self):
         """Slices per wave of the scatter network's MFMA layers in use (0: none)."""

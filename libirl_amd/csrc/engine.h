@@ -182,6 +182,7 @@ public:
     void autotuneTiles();
     int cnnRowLength() const;   // the scatter network's row length in use (0: none)
     void tiles(int *prj, int *bck) const;
+    void setTiles(int prj, int bck);   // 0: size heuristic; drops captured graphs
     // FP32 DRF algorithm in use per direction: 0 spatial, 1 FFT (length P)
     void drfAlgorithms(int *prj, int *bck, int *P) const;
     // BF16 DRF arithmetic per table entry (the delta of unconvolved rows

@@ -2451,7 +2451,10 @@ void Recon::autotuneTiles()
         const DrfArgs da = I.drf_args(bck != 0, bview, bck ? cfg_.bckModel : cfg_.prjModel);
         int best = 0;
         float bestMs = 1e30f;
-        for (int tile : {4, 8}) {
+        // BF16: 2, 4 or 8 slice tiles of 16 per wave; FP32: 4 or 8 slice
+        // groups per block
+        const std::vector<int> cand = I.bf16 ? std::vector<int>{2, 4, 8} : std::vector<int>{4, 8};
+        for (int tile : cand) {
             I.run_drf(bck != 0, da, cfg_.Vs, KYb, tile);
             HIP_CHECK(hipEventRecord(e0, I.stream));
             for (int r = 0; r < 3; ++r) I.run_drf(bck != 0, da, cfg_.Vs, KYb, tile);
@@ -2474,6 +2477,17 @@ void Recon::tiles(int *prj, int *bck) const
 {
     *prj = impl_->tilePrj;
     *bck = impl_->tileBck;
+}
+
+void Recon::setTiles(int prj, int bck)
+{
+    Impl &I = *impl_;
+    auto ok = [&](int t) { return t == 0 || t == 4 || t == 8 || (I.bf16 && t == 2); };
+    if (!ok(prj) || !ok(bck))
+        fail(ETYPE_ILLEGAL_VALUE, "DRF tiles %d, %d (0, 4, 8%s)", prj, bck, I.bf16 ? ", 2" : "");
+    I.tilePrj = prj;
+    I.tileBck = bck;
+    I.clearGraphs();
 }
 
 int Recon::cnnRowLength() const { return impl_->cnnWork ? impl_->cnnWork->rowz : 0; }
@@ -3393,7 +3407,7 @@ static void launch_drf_bf16(hipStream_t stream, bool bck, DrfArgs args, int B, i
 {
     if (args.zs != 1 || args.outZS != 1 || args.Nz % 4)
         fail(ETYPE_ILLEGAL_VALUE, "the BF16 DRF path needs slice-fastest planes, Nz %% 4 == 0");
-    const int NZT = tile == 4 || tile == 8 ? tile : args.Nz > 64 ? 8 : 4;
+    const int NZT = tile == 2 || tile == 4 || tile == 8 ? tile : args.Nz > 64 ? 8 : 4;
     const int TW = DRF_TX + args.KX - 1;
     const int THP = 16 * (NZT - 1) + 32 * drf16_nks(KYb);
     const size_t lds = (size_t)TW * drf16_cs(THP) * 2;   // one bf16 tile (splits run two passes)
@@ -3409,8 +3423,10 @@ static void launch_drf_bf16(hipStream_t stream, bool bck, DrfArgs args, int B, i
     case K:                                                                          \
         if (NZT == 8) {                                                              \
             if (bck) IRL_DRF16_LAUNCH(K, 8, true); else IRL_DRF16_LAUNCH(K, 8, false); \
-        } else {                                                                     \
+        } else if (NZT == 4) {                                                       \
             if (bck) IRL_DRF16_LAUNCH(K, 4, true); else IRL_DRF16_LAUNCH(K, 4, false); \
+        } else {                                                                     \
+            if (bck) IRL_DRF16_LAUNCH(K, 2, true); else IRL_DRF16_LAUNCH(K, 2, false); \
         }                                                                            \
         break;
     switch (KYb) {

@@ -96,11 +96,39 @@ def test_autotuned_tiles_give_the_same_result(precision):
     b = _recon(pb, autotune=True, precision=precision)
     tp, tb = b.tiles()
     print("tiles", tp, tb)
-    assert tp in (4, 8) and tb in (4, 8) and a.tiles() == (0, 0)
+    allowed = (2, 4, 8) if precision == "bf16" else (4, 8)
+    assert tp in allowed and tb in allowed and a.tiles() == (0, 0)
     for s in range(a.num_subsets):
         a.subiteration(s)
         b.subiteration(s)
     assert np.array_equal(a.estimate(), b.estimate())
+
+
+@pytest.mark.parametrize("precision,tiles", [("fp32", (4, 8)), ("bf16", (2, 4, 8))])
+def test_every_drf_tile_gives_the_same_result(precision, tiles):
+    """Each DRF tile shape set by hand (the autotuner's candidates) against the
+    size heuristic, bit for bit, with graphs on (set_tiles drops the captured
+    graphs)."""
+    print("DRF tiles %s %s" % (precision, tiles), flush=True)
+    pb = Problem(n=32, nz=128, nviews=8, per=2, iters=1, half=5, noise=False)
+    ref = _recon(pb, precision=precision)
+    for s in range(ref.num_subsets):
+        ref.subiteration(s)
+    want = ref.estimate()
+    ref.close()
+    for tp in tiles:
+        for tb in tiles:
+            r = _recon(pb, precision=precision, graphs=True)
+            r.set_tiles(tp, tb)
+            assert r.tiles() == (tp, tb)
+            for s in range(r.num_subsets):
+                r.subiteration(s)
+            assert np.array_equal(r.estimate(), want), (tp, tb)
+            r.close()
+    r = _recon(pb, precision=precision)
+    with pytest.raises(RuntimeError):
+        r.set_tiles(3, 4)
+    r.close()
 
 
 @pytest.mark.parametrize("parent", [False, True])
