@@ -129,6 +129,49 @@ __device__ __forceinline__ float rot_sample(const float *__restrict__ plane, lon
     return z0123;
 }
 
+// rot_sample for the engine's slice-fastest volumes, where the 64 lanes of a
+// wave are 64 slices of one pixel: the caller passes a wave-uniform position
+// (X, Y, sn, cs), so the sample coordinates, the tap validity and the tap
+// offsets are scalar values and a tap costs one vector load; offsets within
+// a volume fit 32 bits (checked when a reconstruction is created).  The
+// float operations are those of rot_sample.
+__device__ __forceinline__ float rot_sample_w(const float *__restrict__ plane, int ystride,
+                                              int xstride, int OldDim, float X, float Y,
+                                              float sn, float cs, float OldAxis)
+{
+    const float FloatX = X * cs - Y * sn + OldAxis;
+    const float FloatY = X * sn + Y * cs + OldAxis;
+    const int OldX = (int)FloatX;
+    const int OldY = (int)FloatY;
+    const bool x0 = OldX >= 0 && OldX < OldDim;
+    const bool y0 = OldY >= 0 && OldY < OldDim;
+    const bool x1 = OldX + 1 >= 0 && OldX + 1 < OldDim;
+    const bool y1 = OldY + 1 >= 0 && OldY + 1 < OldDim;
+    const float *p0 = plane + (OldY * ystride + OldX * xstride);
+    const float z0 = (x0 && y0) ? p0[0] : 0.0f;
+    const float z1 = (x0 && y1) ? p0[ystride] : 0.0f;
+    const float z2 = (x1 && y0) ? p0[xstride] : 0.0f;
+    const float z3 = (x1 && y1) ? p0[ystride + xstride] : 0.0f;
+    const float wy = FloatY - (float)OldY;
+    float z01 = z0 + wy * (z1 - z0);
+    float z23 = z2 + wy * (z3 - z2);
+    if (FloatY < 0) {
+        z01 = z0;
+        z23 = z2;
+    }
+    const float wx = FloatX - (float)OldX;
+    float z0123 = z01 + wx * (z23 - z01);
+    if (FloatX < 0) z0123 = z01;
+    return z0123;
+}
+
+// Value of lane k of a wave (k wave-uniform), as a scalar.
+__device__ __forceinline__ int lane_int(int v, int k) { return __builtin_amdgcn_readlane(v, k); }
+__device__ __forceinline__ float lane_float(float v, int k)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), k));
+}
+
 __device__ __forceinline__ bool in_circle(const int *__restrict__ head,
                                           const int *__restrict__ npix, int NewY, int NewX)
 {
@@ -324,7 +367,8 @@ __device__ __forceinline__ PixIdx pix_index(int Nz, int width)
     const int zb = blockIdx.x % nzb, xq = blockIdx.x / nzb;
     PixIdx p;
     p.z = zb * 64 + (threadIdx.x & 63);
-    p.X = xq * PIX_Q + (threadIdx.x >> 6);
+    // one pixel per wave: held as a scalar
+    p.X = __builtin_amdgcn_readfirstlane(xq * PIX_Q + (int)(threadIdx.x >> 6));
     p.Y = blockIdx.y;
     p.b = blockIdx.z;
     p.ok = p.z < Nz && p.X < width;
@@ -340,27 +384,48 @@ static dim3 pix_grid(int Nz, int width, int rows, int B)
 // k_att: rotated estimate times attenuation factor for the batch's views.
 // One thread per (view, bin, slice) and ATT_ROWS consecutive rows, slices
 // fastest: the rows' independent gathers and factor loads are in flight
-// together.
+// together.  The rows' circle extents are read once per wave into lanes
+// 0..ATT_ROWS-1 and taken from there as scalars.
 // ---------------------------------------------------------------------------
-constexpr int ATT_ROWS = 4;
+constexpr int ATT_ROWS = 8;
 
-static dim3 att_grid(int Nz, int width, int rows, int B)
+// Per row Y0 + lane (lanes < ATT_ROWS): circle extents [head, end] and
+// whether the row's projector input is stored in bf16 (att_store's rule);
+// every lane of the wave must be active.
+struct RowExt {
+    int h, e, half;
+};
+__device__ __forceinline__ RowExt row_extents(const int *__restrict__ head,
+                                              const int *__restrict__ npix, int Y0, int NRot,
+                                              int v, const int *__restrict__ rk16,
+                                              const int *__restrict__ dmode)
 {
-    return dim3(((Nz + 63) / 64) * ((width + PIX_Q - 1) / PIX_Q),
-                (rows + ATT_ROWS - 1) / ATT_ROWS, B);
+    const int lane = threadIdx.x & 63;
+    RowExt r{0, -1, 0};
+    if (lane < ATT_ROWS && Y0 + lane < NRot) {
+        r.h = head[Y0 + lane];
+        r.e = r.h + npix[Y0 + lane] - 1;
+        r.half = rk16 && dmode[rk16[(long)v * NRot + Y0 + lane]] <= 1;
+    }
+    return r;
 }
 
 // Output of a plane of the BF16 projector's input: bf16 (the projector's own
 // rounding of the value, stored in the first half of the plane) where the
 // row's DRF entry is plain or splits only its weights, else FP32 (split
-// data, rows added unconvolved).  rk16/dmode null: FP32.
-__device__ __forceinline__ void att_store(float *att, long plane, long idx, float val, int v,
-                                          int Y, int NRot, const int *rk16, const int *dmode)
+// data, rows added unconvolved); 'half' from row_extents.
+__device__ __forceinline__ void att_store_h(float *att, long plane, long idx, float val, bool half)
 {
-    if (rk16 && dmode[rk16[(long)v * NRot + Y]] <= 1)
+    if (half)
         ((__bf16 *)(att + plane))[idx] = (__bf16)val;
     else
         att[plane + idx] = val;
+}
+
+static dim3 att_grid(int Nz, int width, int rows, int B)
+{
+    return dim3(((Nz + 63) / 64) * ((width + PIX_Q - 1) / PIX_Q),
+                (rows + ATT_ROWS - 1) / ATT_ROWS, B);
 }
 
 template <bool SHEAR>
@@ -376,26 +441,31 @@ __global__ __launch_bounds__(256) void k_att(const float *__restrict__ x,
                                              const int *__restrict__ dmode = nullptr)
 {
     const PixIdx p = pix_index(Nz, NRot);
-    if (!p.ok) return;
+    if (p.X >= NRot) return;   // whole waves
     const long vol = (long)Nz * NRot * NRot;
     const int v = bview[p.b];
+    const int Y0 = p.Y * ATT_ROWS;
+    const RowExt ext = row_extents(head, npix, Y0, NRot, v, rk16, dmode);
+    // lanes past the last slice sample slice Nz-1 and store nothing
+    const int z = min(p.z, Nz - 1);
+    const float NewAxis = 0.5f * (float)(NRot - 1);
+    const float OldAxis = 0.5f * (float)(N - 1);
+    const float sn = fsin[v], cs = fcos[v];
 #pragma unroll
     for (int r = 0; r < ATT_ROWS; ++r) {
-        const int Y = p.Y * ATT_ROWS + r;
+        const int Y = Y0 + r;
         if (Y >= NRot) break;
-        const int i = (Y * NRot + p.X) * Nz + p.z;
+        const int i = (Y * NRot + p.X) * Nz + z;
         float val = 0.0f;
         if (SHEAR) {
-            val = shear_sample(sd, 2 * v, x + p.z, (long)N * Nz, Nz, p.X, Y);
-        } else if (in_circle(head, npix, Y, p.X)) {
-            const float NewAxis = 0.5 * (NRot - 1);
-            const float OldAxis = 0.5 * (N - 1);
-            val = rot_sample(x + p.z, (long)N * Nz, Nz, N, (float)p.X - NewAxis,
-                             (float)Y - NewAxis, fsin[v], fcos[v], OldAxis);
+            val = shear_sample(sd, 2 * v, x + z, (long)N * Nz, Nz, p.X, Y);
+        } else if (p.X >= lane_int(ext.h, r) && p.X <= lane_int(ext.e, r)) {
+            val = rot_sample_w(x + z, N * Nz, Nz, N, (float)p.X - NewAxis, (float)Y - NewAxis,
+                               sn, cs, OldAxis);
         }
         if (afac) val = val * afac[(long)v * vol + i];
         const long plane = (long)p.b * vol + (long)Y * NRot * Nz;
-        att_store(att, plane, i - (long)Y * NRot * Nz, val, v, Y, NRot, rk16, dmode);
+        if (p.ok) att_store_h(att, plane, i - (long)Y * NRot * Nz, val, lane_int(ext.half, r));
     }
 }
 
@@ -421,31 +491,36 @@ __global__ __launch_bounds__(256) void k_att_src(const float *__restrict__ g0,
                                                  const int *__restrict__ dmode = nullptr)
 {
     const PixIdx p = pix_index(Nz, NRot);
-    if (!p.ok) return;
+    if (p.X >= NRot) return;   // whole waves
     const long vol = (long)Nz * NRot * NRot;
     const int v = bview[p.b];
+    const int Y0 = p.Y * ATT_ROWS;
+    const RowExt ext = row_extents(head, npix, Y0, NRot, v, rk16, dmode);
+    const int z = min(p.z, Nz - 1);   // lanes past the last slice store nothing
+    const float NewAxis = 0.5f * (float)(NRot - 1);
+    const float OldAxis = 0.5f * (float)(N - 1);
+    const float sn = fsin[v], cs = fcos[v];
 #pragma unroll
     for (int r = 0; r < ATT_ROWS; ++r) {
-        const int Y = p.Y * ATT_ROWS + r;
+        const int Y = Y0 + r;
         if (Y >= NRot) break;
-        const int i = (Y * NRot + p.X) * Nz + p.z;
+        const int i = (Y * NRot + p.X) * Nz + z;
         float v0 = 0.0f, v1 = 0.0f;
         if (SHEAR) {
-            v0 = shear_sample(sd, 2 * v, g0 + p.z, (long)N * Nz, Nz, p.X, Y);
-            v1 = shear_sample(sd, 2 * v, g1 + p.z, (long)N * Nz, Nz, p.X, Y);
-        } else if (in_circle(head, npix, Y, p.X)) {
-            const float NewAxis = 0.5 * (NRot - 1);
-            const float OldAxis = 0.5 * (N - 1);
-            v0 = rot_sample(g0 + p.z, (long)N * Nz, Nz, N, (float)p.X - NewAxis,
-                            (float)Y - NewAxis, fsin[v], fcos[v], OldAxis);
-            v1 = rot_sample(g1 + p.z, (long)N * Nz, Nz, N, (float)p.X - NewAxis,
-                            (float)Y - NewAxis, fsin[v], fcos[v], OldAxis);
+            v0 = shear_sample(sd, 2 * v, g0 + z, (long)N * Nz, Nz, p.X, Y);
+            v1 = shear_sample(sd, 2 * v, g1 + z, (long)N * Nz, Nz, p.X, Y);
+        } else if (p.X >= lane_int(ext.h, r) && p.X <= lane_int(ext.e, r)) {
+            v0 = rot_sample_w(g0 + z, N * Nz, Nz, N, (float)p.X - NewAxis, (float)Y - NewAxis,
+                              sn, cs, OldAxis);
+            v1 = rot_sample_w(g1 + z, N * Nz, Nz, N, (float)p.X - NewAxis, (float)Y - NewAxis,
+                              sn, cs, OldAxis);
         }
         const float af = afac[(long)v * vol + i];
         const float d = af > 0.0f ? -logf(af) * invMu0 : 0.0f;
         const long plane = (long)p.b * vol + (long)Y * NRot * Nz;
-        att_store(att, plane, i - (long)Y * NRot * Nz, fmaf(d, v1, v0) * af, v, Y, NRot, rk16,
-                  dmode);
+        if (p.ok)
+            att_store_h(att, plane, i - (long)Y * NRot * Nz, fmaf(d, v1, v0) * af,
+                        lane_int(ext.half, r));
     }
 }
 
@@ -1066,6 +1141,23 @@ struct RatioArgs {
 };
 
 // grid (plane blocks, batch): 32-bit indices within a plane
+// Sum of a column's chunk partials in chunk order; the loads of eight chunks
+// are issued before their adds.
+__device__ __forceinline__ float chunk_sum(const float *__restrict__ pp, int nchunk, int pl)
+{
+    float p = 0.0f;
+    int c = 0;
+    for (; c + 8 <= nchunk; c += 8) {
+        float t[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) t[u] = pp[(c + u) * pl];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) p += t[u];
+    }
+    for (; c < nchunk; ++c) p += pp[c * pl];
+    return p;
+}
+
 __global__ void k_ratio(RatioArgs a)
 {
     const int pl = a.Nz * a.NRot;
@@ -1075,15 +1167,13 @@ __global__ void k_ratio(RatioArgs a)
     const long gid = (long)b * pl + i;
     const int xb = i / a.Nz;   // projections are [bin][slice]
     const int v = a.bview[b];
-    float p = 0.0f;
-    for (int c = 0; c < a.nchunk; ++c) p += a.ppart[((long)b * a.nchunk + c) * pl + i];
+    const float p = chunk_sum(a.ppart + (long)b * a.nchunk * pl + i, a.nchunk, pl);
     float calc = p * a.scale;
     calc = calc * a.vtsens[v];
     if (a.srfMode == 3) {
         calc = calc + a.prjscat[(long)v * pl + i];
     } else if (a.srfMode == 2) {
-        float p2 = 0.0f;
-        for (int c = 0; c < a.nchunk; ++c) p2 += a.ppart2[((long)b * a.nchunk + c) * pl + i];
+        float p2 = chunk_sum(a.ppart2 + (long)b * a.nchunk * pl + i, a.nchunk, pl);
         p2 = (p2 * a.scale) * a.vt[v];   // scaled by ViewTime only (osemrecon.c:772-773)
         a.prjscat[(long)v * pl + i] = calc - p2;
     }
@@ -1142,28 +1232,42 @@ __global__ __launch_bounds__(256) void k_bck(BckArgs a)
 {
     typedef typename std::conditional<NORM, double, float>::type acc_t;
     const PixIdx p = pix_index(a.Nz, a.N);
-    if (!p.ok) return;
-    const int z = p.z;
+    if (p.X >= a.N) return;   // whole waves
+    // lanes past the last slice sample slice Nz-1 and store nothing, so
+    // every lane of the wave takes part in the per-view lane broadcasts
+    const int z = min(p.z, a.Nz - 1);
     const long gid = ((long)p.Y * a.N + p.X) * a.Nz + z;
     const int skip = (a.NRot - a.N) / 2;
     const int Y = p.Y + skip, X = p.X + skip;
-    const float Axis = 0.5 * (a.NRot - 1);
+    const float Axis = 0.5f * (float)(a.NRot - 1);
     const bool inside = in_circle(a.head, a.npix, Y, X);
     const float Xf = (float)X - Axis, Yf = (float)Y - Axis;
     const long vol = (long)a.NRot * a.NRot * a.Nz;
+    const int lane = threadIdx.x & 63;
     acc_t U = (!NORM && a.accumulate) ? (acc_t)a.u[gid] : (acc_t)0;
-    for (int b = 0; b < a.B; ++b) {
-        const int v = a.bview[b];
-        float val = 0.0f;
-        if (SHEAR)
-            val = shear_sample(a.sd, 2 * v + 1, a.planes + (long)b * vol + z,
-                               (long)a.NRot * a.Nz, a.Nz, X, Y);
-        else if (inside)
-            val = rot_sample(a.planes + (long)b * vol + z, (long)a.NRot * a.Nz, a.Nz, a.NRot,
-                             Xf, Yf, a.bsin[v], a.bcos[v], Axis);
-        U = U + (acc_t)val;
-        if ((double)U < 1.0e-5) U = 0;
+    // views in groups of 64: lane k holds view b0+k's sine and cosine
+    for (int b0 = 0; b0 < a.B; b0 += 64) {
+        float lsn = 0.0f, lcs = 0.0f;
+        if (!SHEAR && b0 + lane < a.B) {
+            const int v = a.bview[b0 + lane];
+            lsn = a.bsin[v];
+            lcs = a.bcos[v];
+        }
+        const int nb = min(64, a.B - b0);
+        for (int k = 0; k < nb; ++k) {
+            const int b = b0 + k;
+            float val = 0.0f;
+            if (SHEAR)
+                val = shear_sample(a.sd, 2 * a.bview[b] + 1, a.planes + (long)b * vol + z,
+                                   (long)a.NRot * a.Nz, a.Nz, X, Y);
+            else if (inside)
+                val = rot_sample_w(a.planes + (long)b * vol + z, a.NRot * a.Nz, a.Nz, a.NRot, Xf,
+                                   Yf, lane_float(lsn, k), lane_float(lcs, k), Axis);
+            U = U + (acc_t)val;
+            if ((double)U < 1.0e-5) U = 0;
+        }
     }
+    if (!p.ok) return;
     if (NORM) {
         a.ninvOut[gid] = U < 1e-6 ? 0.0f : (float)(1.0 / (double)U);
     } else {
