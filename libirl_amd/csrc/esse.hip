@@ -170,7 +170,7 @@ __global__ __launch_bounds__(256) void k_fft_f_inv_src(
         const float e0 = a[w].x * scale, e1 = a[w].y * scale;
         const float f = afac[vo + n], mp = map[vo + n], av = act[bo + n];
         // f == 0 (depth past float range): d = 0 keeps eff finite, out is 0
-                const float d = f > 0.0f ? -logf(f) * fAtnCnvFac : 0.0f;
+        const float d = f > 0.0f ? -logf(f) * fAtnCnvFac : 0.0f;
         float eff = e0 - d * e1;
         eff = eff * expf(d * fMu);
         if (cntr > 0) eff = eff + av * cntr;

@@ -40,3 +40,16 @@ def test_irlosem_tall_volume_uses_wide_tiles():
     e = rel_l2(got, ref32)
     print("rel L2 vs f32 %.3g" % e)
     assert e <= 1e-4
+
+
+def test_irlosem_more_views_per_subset_than_a_wave_and_ragged_slices():
+    # 72 views in one subset (the back rotation takes the views' angles in
+    # groups of 64 lanes) and 70 slices (the last wave of slices is partly idle)
+    print("case: N=16 Nz=70 V=72 one subset atn+drf", flush=True)
+    pb = Problem(n=16, nz=70, nviews=72, per=72, iters=2, half=3, pix=0.5)
+    ref32 = pb.oracle_osem("f32")
+    ref64 = pb.oracle_osem("f64")
+    got = _gpu_osem(pb)
+    e32, e64 = rel_l2(got, ref32), rel_l2(got, ref64)
+    print("rel L2 vs f32 %.3g vs f64 %.3g" % (e32, e64))
+    assert e32 <= 1e-4 and e64 <= 1e-4
