@@ -21,9 +21,9 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--config", default="c3")
     ap.add_argument("--out", required=True)
-    ap.add_argument("--pairs", type=int, default=2000)
+    ap.add_argument("--pairs", type=int, default=3200)
     ap.add_argument("--val", type=int, default=100)
-    ap.add_argument("--epochs", type=int, default=16)
+    ap.add_argument("--epochs", type=int, default=36)
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=1)
