@@ -495,7 +495,8 @@ def batch_configs(count=16, seed=5):
 def batch_line(tmpdir, net, steps=16, warmup=24):
     """16 concurrent reconstructions of uneven size: learned scatter + BF16
     DRF, graph replay and autotuned tiles, stepped through one batch (each
-    member's graph on its own stream, and one parent-graph launch per step)
+    member's graph on its own stream, one parent-graph launch per step, and
+    one launch per step of a graph with the members' launches as branches)
     and, for comparison, one after another; then each estimate after 5
     iterations against its FP32 ESSE reconstruction."""
     import libirl_amd as L
@@ -508,7 +509,7 @@ def batch_line(tmpdir, net, steps=16, warmup=24):
     try:
         # iterations for every step below (warmup, the timed runs, the
         # parent-graph warmup, one at a time), the scatter modelled in all
-        total = warmup + 5 * steps + max(c["views"] // c["vs"] for c in cfgs)
+        total = warmup + 7 * steps + 2 * max(c["views"] // c["vs"] for c in cfgs)
         recs = [L.Recon(run_parms(w, total), w["opts"], w["views"], w["drf_file"],
                         w["krnl_file"], None, w["mu"], w["prj"], None) for w in works]
     finally:
@@ -516,7 +517,9 @@ def batch_line(tmpdir, net, steps=16, warmup=24):
         L.set_graphs(prev[1])
         L.set_autotune(prev[2])
         L.set_scatter_cnn(None)
-    batch = L.Batch(recs)
+    # warmup and the first timed run on member streams (executables of each
+    # mode are made before that mode is timed)
+    batch = L.Batch(recs, parent_graph=0)
 
     def timed(parent):
         batch.set_parent_graph(parent)
@@ -529,7 +532,7 @@ def batch_line(tmpdir, net, steps=16, warmup=24):
 
     for _ in range(warmup):
         batch.step()
-    dts = [timed(False)]
+    dts = [timed(0)]
     st = batch.stats()
     # warmup: every member's subsets run eagerly once, then are captured and
     # replayed once, so the timed steps replay warm graph executables
@@ -539,15 +542,21 @@ def batch_line(tmpdir, net, steps=16, warmup=24):
     # the same steps as one parent-graph launch per step (its executables
     # instantiated first), each mode timed twice, alternating
     S = [r.num_subsets for r in recs]
-    batch.set_parent_graph(True)
+    batch.set_parent_graph(1)
     for _ in range(max(S)):
         batch.step()
-    dtp = [timed(True)]
-    dts.append(timed(False))
-    dtp.append(timed(True))
-    dt_batch, dt_parent = min(dts), min(dtp)
+    dtp = [timed(1)]
+    dts.append(timed(0))
+    dtp.append(timed(1))
     pst = batch.stats()
-    batch.set_parent_graph(False)
+    # one graph per step with the members' launches as parallel branches
+    batch.set_parent_graph(2)
+    for _ in range(max(S)):
+        batch.step()
+    dtf = [timed(2), timed(2)]
+    batch_launches = batch.stats()["graph_launches"]
+    dt_streams, dt_parent, dt_flat = min(dts), min(dtp), min(dtf)
+    batch.set_parent_graph(0)
     # one reconstruction at a time, each sub-iteration finished before the next
     t0 = time.perf_counter()
     for k in range(steps):
@@ -566,15 +575,18 @@ def batch_line(tmpdir, net, steps=16, warmup=24):
     n = len(recs)
     return {"workload": "16 reconstructions, N in 64..160, 64..128 views, 8 subsets, ATN + DRF + "
                         "learned scatter, BF16 DRF",
-            "value": n * steps / dt_batch, "unit": "subiters/s (all reconstructions)",
-            "ms_per_step": dt_batch * 1e3 / steps,
-            "mode": "each member's sub-iteration graph launched on its own stream (best of 2 timed runs)",
+            "value": n * steps / dt_flat, "unit": "subiters/s (all reconstructions)",
+            "ms_per_step": dt_flat * 1e3 / steps,
+            "mode": "one graph launch per step, the members' launches as parallel branches "
+                    "(the library's default; best of 2 timed runs)",
+            "one_graph_runs": [n * steps / t for t in dtf],
+            "member_streams_value": n * steps / dt_streams,
             "member_streams_runs": [n * steps / t for t in dts],
             "parent_graph_value": n * steps / dt_parent,
             "parent_graph_runs": [n * steps / t for t in dtp],
-            "parent_graph_launches": pst["graph_launches"],
+            "graph_launches": batch_launches,
             "one_at_a_time_value": n * steps / dt_seq,
-            "batch_speedup_vs_one_at_a_time": dt_seq / dt_batch,
+            "batch_speedup_vs_one_at_a_time": dt_seq / dt_flat,
             "sizes": [[c["n"], c["views"]] for c in cfgs],
             "tiles_prj_bck_cnnrow": tiles,
             "replays": st["replays"],

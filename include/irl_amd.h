@@ -131,19 +131,21 @@ int irlamd_recon_drf_algorithms(irlamd_recon *r, int *piPrj, int *piBck, int *pi
 /* Several reconstructions (any sizes and view counts) stepped together: each
  * step runs the next sub-iteration of every member (subsets in order, the
  * iteration and scatter schedule of each as in IrlOsem).  Members whose
- * sub-iteration can be replayed run from captured graphs, each on its own
- * stream so that they overlap on the GPU, or (irlamd_batch_set_parent_graph)
- * as child graphs of one parent graph, one launch per step; the others
+ * sub-iteration can be replayed run from captured graphs: by default
+ * (irlamd_batch_set_parent_graph mode 2) as parallel branches of one graph
+ * captured with every member's launches, one launch per step; mode 1: the
+ * members' graphs as child graphs of one parent graph; mode 0: each member's
+ * graph on its own stream (they overlap on the GPU).  The others
  * (first occurrence of a subset and mode, learned-scatter checks) run
  * eagerly on their own streams after them.  The members stay owned by the
  * caller and must outlive the batch; results equal those of stepping each
  * member alone. */
 typedef struct irlamd_batch irlamd_batch;
 irlamd_batch *irlamd_batch_create(irlamd_recon **pRecons, int iNum);
-int irlamd_batch_set_parent_graph(irlamd_batch *b, int bOn);
+int irlamd_batch_set_parent_graph(irlamd_batch *b, int iMode);
 int irlamd_batch_step(irlamd_batch *b);
 int irlamd_batch_synchronize(irlamd_batch *b);
-/* Parent-graph launches, sub-iterations replayed from graphs and
+/* One-launch-per-step graph launches, sub-iterations replayed from graphs and
  * sub-iterations run eagerly so far. */
 int irlamd_batch_stats(irlamd_batch *b, int *piGraphLaunches, int *piReplays, int *piEager);
 void irlamd_batch_destroy(irlamd_batch *b);

@@ -700,7 +700,7 @@ class Batch:
     """Reconstructions stepped together (irlamd_batch_*): each step runs the
     next sub-iteration of every member, the replayable ones in one graph launch."""
 
-    def __init__(self, recons, parent_graph=False):
+    def __init__(self, recons, parent_graph=2):
         self._lib = load()
         self._recons = list(recons)   # members must outlive the batch
         arr = (ctypes.c_void_p * len(self._recons))(*[r._h for r in self._recons])
@@ -710,8 +710,11 @@ class Batch:
         self._h = h
         self.set_parent_graph(parent_graph)
 
-    def set_parent_graph(self, on):
-        self._lib.irlamd_batch_set_parent_graph(self._h, 1 if on else 0)
+    def set_parent_graph(self, mode):
+        """0: member graphs on their own streams; 1: child graphs of one parent
+        graph; 2: one graph with the members' launches as parallel branches."""
+        if self._lib.irlamd_batch_set_parent_graph(self._h, int(mode)):
+            raise RuntimeError(error_string())
 
     def step(self):
         if self._lib.irlamd_batch_step(self._h):

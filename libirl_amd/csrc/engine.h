@@ -173,6 +173,9 @@ public:
     void endSubset(SrfMode srf);
     void runSubIteration(int s, SrfMode srf) { runViews(s, 0, cfg_.Vs, srf, false, true, false); }
     void *captureSubIteration(int s, bool keepUpdate, SrfMode srf);
+    // The sub-iteration's launches on the engine's stream while that stream
+    // takes part in a capture begun elsewhere (joined by an event wait).
+    void recordSubIteration(int s, SrfMode srf);
     void *streamHandle() const;       // the engine's hipStream_t
     bool learnedScatterActive() const;
 
@@ -244,9 +247,12 @@ public:
     ReconBatch(const ReconBatch &) = delete;
     ReconBatch &operator=(const ReconBatch &) = delete;
     void step(const std::vector<int> &subset, const std::vector<SrfMode> &srf);
-    void setParentGraph(bool on);
+    // 0: each member's graph on its own stream; 1: the members' graphs as
+    // child graphs of one parent graph; 2: one graph captured with the
+    // members' launches as parallel branches.  One launch per step in 1, 2.
+    void setParentGraph(int mode);
     void synchronize();
-    int graphLaunches() const;       // parent-graph launches so far
+    int graphLaunches() const;       // one-launch-per-step launches so far
     int replays() const;             // sub-iterations replayed from graphs so far
     int eagerSubIterations() const;  // sub-iterations run eagerly so far
 

@@ -2518,6 +2518,21 @@ void *Recon::captureSubIteration(int s, bool keepUpdate, SrfMode srf)
     return g;
 }
 
+void Recon::recordSubIteration(int s, SrfMode srf)
+{
+    Impl &I = *impl_;
+    if (I.timing) fail(ETYPE_ILLEGAL_VALUE, "sub-iteration capture with kernel timing on");
+    const bool check = I.cnnCheckNow;
+    I.cnnCheckNow = false;
+    try {
+        runViews(s, 0, cfg_.Vs, srf, false, true, false);
+    } catch (...) {
+        I.cnnCheckNow = check;
+        throw;
+    }
+    I.cnnCheckNow = check;
+}
+
 // Tile autotuner: each candidate shape of the DRF kernels is timed with HIP
 // events on the views of subset 0 (three launches after one untimed one, on
 // whatever the work buffers hold: timing does not depend on the values) and
@@ -2637,9 +2652,12 @@ struct ReconBatch::Impl {
         std::vector<hipGraphNode_t> node;           // per member
     };
     std::map<std::vector<int>, Parent> parents;
+    // one-graph mode: executables by topology (members and their keys)
+    std::map<std::vector<int>, hipGraphExec_t> flat;
+    hipEvent_t fork = nullptr;
     // own-stream mode: each member's instantiated graphs by key
     std::vector<std::map<GraphKey, hipGraphExec_t>> exec;
-    bool parent = false;
+    int parent = 2;
     int launches = 0, eager = 0, replays = 0;
 };
 
@@ -2651,6 +2669,7 @@ ReconBatch::ReconBatch(const std::vector<Recon *> &recons) : impl_(new Impl)
     B.ev.resize(recons.size());
     for (auto &e : B.ev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIP_CHECK(hipEventCreateWithFlags(&B.done, hipEventDisableTiming));
+    HIP_CHECK(hipEventCreateWithFlags(&B.fork, hipEventDisableTiming));
     B.child.resize(recons.size());
     B.seen.resize(recons.size());
     B.exec.resize(recons.size());
@@ -2662,6 +2681,8 @@ ReconBatch::~ReconBatch()
     (void)hipStreamSynchronize(B.stream);
     for (auto &kv : B.parents)
         if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+    for (auto &kv : B.flat)
+        if (kv.second) (void)hipGraphExecDestroy(kv.second);
     for (auto *r : B.r) r->synchronize();
     for (auto &m : B.child)
         for (auto &kv : m) (void)hipGraphDestroy(kv.second);
@@ -2669,6 +2690,7 @@ ReconBatch::~ReconBatch()
         for (auto &kv : m) (void)hipGraphExecDestroy(kv.second);
     for (auto e : B.ev) (void)hipEventDestroy(e);
     (void)hipEventDestroy(B.done);
+    (void)hipEventDestroy(B.fork);
     (void)hipStreamDestroy(B.stream);
     delete impl_;
 }
@@ -2707,18 +2729,69 @@ void ReconBatch::step(const std::vector<int> &subset, const std::vector<SrfMode>
             B.replays++;
             continue;
         }
-        auto it = B.child[i].find(key);
-        if (it == B.child[i].end())
-            it = B.child[i].emplace(key, (hipGraph_t)R.captureSubIteration(subset[i], false, srf[i]))
-                     .first;
+        if (B.parent == 1) {
+            auto it = B.child[i].find(key);
+            if (it == B.child[i].end())
+                it = B.child[i]
+                         .emplace(key, (hipGraph_t)R.captureSubIteration(subset[i], false, srf[i]))
+                         .first;
+            graphs.push_back(it->second);
+        }
         members.push_back(i);
-        graphs.push_back(it->second);
         topo.push_back(i);
         topo.push_back(key.s);
         topo.push_back(key.srf);
         topo.push_back(key.learned);
     }
-    if (!members.empty()) {
+    if (!members.empty() && B.parent == 2) {
+        // one graph of this combination of members and subsets: the members'
+        // launches captured on their own streams as branches forked from and
+        // joined to the batch's stream (the combinations recur with the
+        // subset cycles; past 128 of them the cache starts again)
+        auto fit = B.flat.find(topo);
+        if (fit == B.flat.end()) {
+            if (B.flat.size() >= 128) {
+                HIP_CHECK(hipStreamSynchronize(B.stream));
+                for (auto &kv : B.flat) (void)hipGraphExecDestroy(kv.second);
+                B.flat.clear();
+            }
+            HIP_CHECK(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
+            hipGraph_t g = nullptr;
+            try {
+                HIP_CHECK(hipEventRecord(B.fork, B.stream));
+                for (int i : members) {
+                    hipStream_t st = (hipStream_t)B.r[i]->streamHandle();
+                    HIP_CHECK(hipStreamWaitEvent(st, B.fork, 0));
+                    B.r[i]->recordSubIteration(subset[i], srf[i]);
+                    HIP_CHECK(hipEventRecord(B.ev[i], st));
+                    HIP_CHECK(hipStreamWaitEvent(B.stream, B.ev[i], 0));
+                }
+            } catch (...) {
+                (void)hipStreamEndCapture(B.stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            HIP_CHECK(hipStreamEndCapture(B.stream, &g));
+            hipGraphExec_t ex = nullptr;
+            const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HIP_CHECK(ie);
+            fit = B.flat.emplace(topo, ex).first;
+        }
+        // the members' earlier work precedes the launch; their later work follows it
+        for (int i : members) {
+            HIP_CHECK(hipEventRecord(B.ev[i], (hipStream_t)B.r[i]->streamHandle()));
+            HIP_CHECK(hipStreamWaitEvent(B.stream, B.ev[i], 0));
+        }
+        HIP_CHECK(hipGraphLaunch(fit->second, B.stream));
+        HIP_CHECK(hipEventRecord(B.done, B.stream));
+        for (int i : members) {
+            HIP_CHECK(hipStreamWaitEvent((hipStream_t)B.r[i]->streamHandle(), B.done, 0));
+            B.r[i]->endSubset(srf[i]);
+        }
+        B.launches++;
+        B.replays += (int)members.size();
+    } else if (!members.empty()) {
         // the parent graph of this combination of members and subsets (the
         // combinations recur with the subset cycles): one child node per
         // member, no edges.  Past a bound on the cache, an executable of the
@@ -2789,7 +2862,11 @@ void ReconBatch::step(const std::vector<int> &subset, const std::vector<SrfMode>
     }
 }
 
-void ReconBatch::setParentGraph(bool on) { impl_->parent = on; }
+void ReconBatch::setParentGraph(int mode)
+{
+    if (mode < 0 || mode > 2) fail(ETYPE_ILLEGAL_VALUE, "batch launch mode %d (0, 1 or 2)", mode);
+    impl_->parent = mode;
+}
 int ReconBatch::replays() const { return impl_->replays; }
 
 void ReconBatch::synchronize()

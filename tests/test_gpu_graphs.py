@@ -131,9 +131,9 @@ def test_every_drf_tile_gives_the_same_result(precision, tiles):
     r.close()
 
 
-@pytest.mark.parametrize("parent", [False, True])
+@pytest.mark.parametrize("parent", [0, 1, 2])
 def test_batch_of_uneven_reconstructions_equals_each_alone(parent):
-    print("batch of 3 uneven reconstructions, parent graph %s" % parent, flush=True)
+    print("batch of 3 uneven reconstructions, launch mode %d" % parent, flush=True)
     import libirl_amd as L
     probs = [Problem(n=32, nz=16, nviews=8, per=2, iters=3, half=3, noise=False, seed=1),
              Problem(n=48, nz=24, nviews=12, per=3, iters=3, half=3, noise=False, seed=2),
@@ -159,7 +159,9 @@ def test_batch_of_uneven_reconstructions_equals_each_alone(parent):
     st = batch.stats()
     print("batch stats", st)
     assert st["replays"] > 0 and st["eager_subiterations"] > 0
-    assert (st["graph_launches"] > 0) == parent
+    assert (st["graph_launches"] > 0) == (parent > 0)
+    with pytest.raises(RuntimeError):
+        batch.set_parent_graph(3)
     batch.close()
 
 
