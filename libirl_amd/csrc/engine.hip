@@ -457,13 +457,15 @@ __global__ __launch_bounds__(256) void k_att(const float *__restrict__ x,
         if (Y >= NRot) break;
         const int i = (Y * NRot + p.X) * Nz + z;
         float val = 0.0f;
+        // outside the circle the value is +0 whatever the factor: not read
+        const bool inside = SHEAR || (p.X >= lane_int(ext.h, r) && p.X <= lane_int(ext.e, r));
         if (SHEAR) {
             val = shear_sample(sd, 2 * v, x + z, (long)N * Nz, Nz, p.X, Y);
-        } else if (p.X >= lane_int(ext.h, r) && p.X <= lane_int(ext.e, r)) {
+        } else if (inside) {
             val = rot_sample_w(x + z, N * Nz, Nz, N, (float)p.X - NewAxis, (float)Y - NewAxis,
                                sn, cs, OldAxis);
         }
-        if (afac) val = val * afac[(long)v * vol + i];
+        if (afac && inside) val = val * afac[(long)v * vol + i];
         const long plane = (long)p.b * vol + (long)Y * NRot * Nz;
         if (p.ok) att_store_h(att, plane, i - (long)Y * NRot * Nz, val, lane_int(ext.half, r));
     }
@@ -506,21 +508,25 @@ __global__ __launch_bounds__(256) void k_att_src(const float *__restrict__ g0,
         if (Y >= NRot) break;
         const int i = (Y * NRot + p.X) * Nz + z;
         float v0 = 0.0f, v1 = 0.0f;
+        // outside the circle the source is +0 whatever the factor: not read
+        const bool inside = SHEAR || (p.X >= lane_int(ext.h, r) && p.X <= lane_int(ext.e, r));
         if (SHEAR) {
             v0 = shear_sample(sd, 2 * v, g0 + z, (long)N * Nz, Nz, p.X, Y);
             v1 = shear_sample(sd, 2 * v, g1 + z, (long)N * Nz, Nz, p.X, Y);
-        } else if (p.X >= lane_int(ext.h, r) && p.X <= lane_int(ext.e, r)) {
+        } else if (inside) {
             v0 = rot_sample_w(g0 + z, N * Nz, Nz, N, (float)p.X - NewAxis, (float)Y - NewAxis,
                               sn, cs, OldAxis);
             v1 = rot_sample_w(g1 + z, N * Nz, Nz, N, (float)p.X - NewAxis, (float)Y - NewAxis,
                               sn, cs, OldAxis);
         }
-        const float af = afac[(long)v * vol + i];
-        const float d = af > 0.0f ? -logf(af) * invMu0 : 0.0f;
+        float out = 0.0f;
+        if (inside) {
+            const float af = afac[(long)v * vol + i];
+            const float d = af > 0.0f ? -logf(af) * invMu0 : 0.0f;
+            out = fmaf(d, v1, v0) * af;
+        }
         const long plane = (long)p.b * vol + (long)Y * NRot * Nz;
-        if (p.ok)
-            att_store_h(att, plane, i - (long)Y * NRot * Nz, fmaf(d, v1, v0) * af,
-                        lane_int(ext.half, r));
+        if (p.ok) att_store_h(att, plane, i - (long)Y * NRot * Nz, out, lane_int(ext.half, r));
     }
 }
 
