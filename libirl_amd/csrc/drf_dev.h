@@ -43,6 +43,10 @@ struct DrfArgs {
     // hold bf16 values (the first half of the plane, same indexing), the
     // others FP32
     int inBf16;
+    // BF16 PRJ: per row the bins [head, head + npix) of the inscribed
+    // circle, outside which every producer of the input planes writes +0
+    // (the input is then staged as zeros without being read); null: all bins
+    const int *circHead, *circNpix;
 };
 
 __device__ __forceinline__ float drf_in(const DrfArgs &a, int b, int v, int row, int g, int gx)

@@ -91,7 +91,13 @@ __device__ __forceinline__ float drf_fft_touch(const DrfArgs &a, int b, int v, i
     const long span = (long)(a.NRot - 1) * a.xs + a.Nz;
     const float *p = a.in + (long)b * a.inB + (long)row * a.inRow;
     const float *f = a.afac ? a.afac + (long)v * a.afB + (long)row * a.afRow : nullptr;
-    for (long e = (long)threadIdx.x * 32; e < span; e += (long)blockDim.x * 32) {
+    // projector: only the bins of the row's circle are read
+    long e0 = 0, e1 = span;
+    if (a.circHead) {
+        e0 = (long)a.circHead[row] * a.xs;
+        e1 = min(span, (long)(a.circHead[row] + a.circNpix[row]) * a.xs);
+    }
+    for (long e = e0 + (long)threadIdx.x * 32; e < e1; e += (long)blockDim.x * 32) {
         r += p[e];
         if (f) r += f[e];
     }
@@ -118,6 +124,14 @@ __device__ __forceinline__ void drf_fft_forward(const DrfArgs &a, int b, int v, 
     float2 *slot = sm + li * D::LS;
     const int c0 = 2 * li, c1 = c0 + 1;
     const bool zact = c0 < a.NRot, has1 = c1 < a.NRot;
+    // projector: the input is +0 outside the row's circle, so bins there are
+    // not read (their values are the zeros the selects below give)
+    bool rd0 = zact, rd1 = has1;
+    if (a.circHead) {
+        const int h = a.circHead[row], e = h + a.circNpix[row];
+        rd0 = rd0 && c0 >= h && c0 < e;
+        rd1 = rd1 && c1 >= h && c1 < e;
+    }
     // waves beyond the last bin pair skip the slice transform (the condition
     // is uniform per wave; the line transforms synchronise within a wave)
     const bool zIdle = drf_fft_wave_first_line<P>() >= (a.NRot + 1) / 2;
@@ -144,13 +158,15 @@ __device__ __forceinline__ void drf_fft_forward(const DrfArgs &a, int b, int v, 
             const bool whole = n0 + D::T <= a.Nz;
             const int n = tt + n0;
             const int nc = whole ? n : min(n, a.Nz - 1);
-            float re = i0[nc], im = i1[nc];
+            float re = 0.0f, im = 0.0f;
+            if (rd0) re = i0[nc];
+            if (rd1) im = i1[nc];
             if (fb) {
                 re *= g0[nc];
                 im *= g1[nc];
             }
             const bool in = whole || n < a.Nz;
-            s[w] = make_float2((zact && in) ? re : 0.0f, (has1 && in) ? im : 0.0f);
+            s[w] = make_float2((rd0 && in) ? re : 0.0f, (rd1 && in) ? im : 0.0f);
         }
         drf_fft_pad<P>(a, b, v, row, pad, s, c0, c1, zact, tt);
     }

@@ -990,6 +990,12 @@ __global__ __launch_bounds__(256, 2) void k_drf_bf16(DrfArgs a, const uint4 *__r
             // without an axial pad (and in-range bins) a value is the input
             // times the factor, or zero outside the slices
             const bool plain = pad <= 0 || a.avg <= 0;
+            // the projector's input is +0 outside the row's circle: not read
+            int gx0 = 0, gx1 = a.NRot;
+            if (!BCK && a.circHead) {
+                gx0 = a.circHead[row];
+                gx1 = gx0 + a.circNpix[row];
+            }
             const long ibase = (long)b * a.inB + (long)row * a.inRow;
             const long fbase = (long)v * a.afB + (long)row * a.afRow;
             const uint4 *ap = afr + (long)kidx * KX * NKS * 64 + lane;
@@ -1019,7 +1025,7 @@ __global__ __launch_bounds__(256, 2) void k_drf_bf16(DrfArgs a, const uint4 *__r
                             const int c = c0 + cc, q = ks + 4 * qi;
                             const int gx = x0 - CX + c, gz = zb + q;
                             li[u] = (c * CSB + q) / 2;
-                            const bool ok = e0 + 256 * u < total && gx >= 0 && gx < a.NRot &&
+                            const bool ok = e0 + 256 * u < total && gx >= gx0 && gx < gx1 &&
                                             gz >= 0 && gz < a.Nz;
                             hv[u] = ok ? *(const uint2 *)(inh + (long)gx * a.xs + gz)
                                        : make_uint2(0, 0);
@@ -1046,7 +1052,7 @@ __global__ __launch_bounds__(256, 2) void k_drf_bf16(DrfArgs a, const uint4 *__r
                             const int c = c0 + cc, q = ks + 4 * qi;
                             const int gx = x0 - CX + c, gz = zb + q;
                             li[u] = (c * CSB + q) / 2;
-                            const bool ok = e0 + 256 * u < total && gx >= 0 && gx < a.NRot &&
+                            const bool ok = e0 + 256 * u < total && gx >= gx0 && gx < gx1 &&
                                             gz >= 0 && gz < a.Nz;
                             const long o = (long)gx * a.xs + gz;
                             f[u] = ok ? *(const float4 *)(a.in + ibase + o)
@@ -1681,6 +1687,8 @@ struct Recon::Impl {
             d.in = att.p;
             d.inB = vol;
             d.inRow = pl;
+            d.circHead = head.p;
+            d.circNpix = npix.p;
             d.out = ppart.p;
             d.outB = pl * nchunk;
             d.outRow = pl;
