@@ -663,13 +663,49 @@ __device__ __forceinline__ void up_coord(int j, int n, int &i0, int &i1, float &
     if (i0 > n - 1) i0 = n - 1;
 }
 
+// Source volumes out: g0[i], g1[i] in FP32, or (gp) both as bf16 in one
+// 32-bit word, g0 in the low half.  The bf16 values are rounded
+// stochastically with a fixed per-voxel dither (the 16 bits below the bf16
+// mantissa plus a hash of the voxel index, truncated): the rounding errors
+// have zero mean and do not repeat across voxels of equal value (an early,
+// nearly uniform estimate), so the depth sums of the projector average them
+// out instead of adding them up.  The hash is the lowbias32 integer mixer.
+__device__ __forceinline__ uint32_t dither_hash(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+__device__ __forceinline__ uint32_t bf16_dithered(float v, uint32_t r16)
+{
+    const uint32_t u = __builtin_bit_cast(uint32_t, v);
+    // finite values only (|v| below the largest bf16): the carry into the
+    // exponent is the correct rounding up to the next binade
+    return (u + r16) >> 16;
+}
+__device__ __forceinline__ void store_sources(float *g0, float *g1, uint32_t *gp, long i, float a,
+                                              float b)
+{
+    if (gp) {
+        const uint32_t h = dither_hash((uint32_t)i);
+        gp[i] = bf16_dithered(a, h & 0xffffu) | (bf16_dithered(b, h >> 16) << 16);
+    } else {
+        g0[i] = a;
+        g1[i] = b;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_cnn_source(const float *__restrict__ o,
                                                     const float *__restrict__ x,
                                                     const float *__restrict__ mu, int N, int Nz,
                                                     const float *__restrict__ scal, float kappa,
                                                     float fDens, float cntr, float primary,
                                                     float *__restrict__ g0,
-                                                    float *__restrict__ g1)
+                                                    float *__restrict__ g1,
+                                                    uint32_t *__restrict__ gp)
 {
     const int n = N * N * Nz;
     const int i = blockIdx.x * 256 + threadIdx.x;
@@ -698,8 +734,8 @@ __global__ __launch_bounds__(256) void k_cnn_source(const float *__restrict__ o,
     const float sx = scal[1];
     const float dens = mu ? mu[i] * fDens : 0.0f;
     const float xi = x[i];
-    g0[i] = primary * xi + (sx * c[0] + cntr * xi) * dens;
-    g1[i] = (sx / kappa) * c[1] * dens;
+    store_sources(g0, g1, gp, i, primary * xi + (sx * c[0] + cntr * xi) * dens,
+                  (sx / kappa) * c[1] * dens);
 }
 
 __global__ __launch_bounds__(256) void k_exact_source(const float *__restrict__ E0,
@@ -708,23 +744,24 @@ __global__ __launch_bounds__(256) void k_exact_source(const float *__restrict__ 
                                                       const float *__restrict__ mu, long n,
                                                       float fMu, float fDens, float cntr,
                                                       float primary, float *__restrict__ g0,
-                                                      float *__restrict__ g1)
+                                                      float *__restrict__ g1,
+                                                      uint32_t *__restrict__ gp)
 {
     const long i = blockIdx.x * 256L + threadIdx.x;
     if (i >= n) return;
     const float dens = mu ? mu[i] * fDens : 0.0f;
     const float xi = x[i], e0 = E0[i];
-    g0[i] = primary * xi + (e0 + cntr * xi) * dens;
-    g1[i] = (fMu * e0 - E1[i]) * dens;
+    store_sources(g0, g1, gp, i, primary * xi + (e0 + cntr * xi) * dens,
+                  (fMu * e0 - E1[i]) * dens);
 }
 
 void cnn_exact_sources(const float *E0, const float *E1, const float *x, const float *mu, int N,
                        int Nz, const CnnPhys &ph, float primary, float *g0, float *g1,
-                       hipStream_t s)
+                       hipStream_t s, uint32_t *packed)
 {
     const long n = (long)N * N * Nz;
     hipLaunchKernelGGL(k_exact_source, dim3(cdiv_u(n, 256)), dim3(256), 0, s, E0, E1, x, mu, n,
-                       ph.fMu, ph.fDens, ph.cntr, primary, g0, g1);
+                       ph.fMu, ph.fDens, ph.cntr, primary, g0, g1, packed);
     CNN_CHECK(hipGetLastError());
 }
 
@@ -1170,7 +1207,8 @@ static void mean_scale(const float *x, long n, double *&psum, long &nb, float *s
 }
 
 void ScatterCnn::sources(const float *x, const float *mu, int N, int Nz, const CnnPhys &ph,
-                         float primary, float *g0, float *g1, CnnWork &w, hipStream_t s)
+                         float primary, float *g0, float *g1, CnnWork &w, hipStream_t s,
+                         uint32_t *packed)
 {
     const int Y = N / 2, X = N / 2, Z = Nz / 2;
     const long V = (long)Y * X * Z, n = (long)N * N * Nz;
@@ -1185,7 +1223,7 @@ void ScatterCnn::sources(const float *x, const float *mu, int N, int Nz, const C
     hipLaunchKernelGGL(k_inv_mean, dim3(1), dim3(256), 0, s, w.psum, (int)pb, n, w.scal);
     forward_bf16(w.in, Y, X, Z, w.out, w.actb[0], w.actb[1], w.scal, w.rowz, s);
     hipLaunchKernelGGL(k_cnn_source, dim3(cdiv_u(n, 256)), dim3(256), 0, s, w.out, x, mu, N, Nz,
-                       w.scal, kappa_, ph.fDens, ph.cntr, primary, g0, g1);
+                       w.scal, kappa_, ph.fDens, ph.cntr, primary, g0, g1, packed);
     CNN_CHECK(hipGetLastError());
 }
 

@@ -94,9 +94,12 @@ public:
     // Source volumes of a reconstruction (engine layout [y][x][z], N x N x Nz):
     // g0, g1 from the current estimate x and attenuation map mu (per pixel),
     // in the caller's work buffers (made for the same N, Nz).  Reads only the
-    // weights, so users on different streams may run it concurrently.
+    // weights, so users on different streams may run it concurrently.  With
+    // 'packed' the volumes are written there instead, rounded to bf16 (g0 in
+    // the low, g1 in the high half of each 32-bit word).
     void sources(const float *x, const float *mu, int N, int Nz, const CnnPhys &ph,
-                 float primary, float *g0, float *g1, CnnWork &w, hipStream_t s);
+                 float primary, float *g0, float *g1, CnnWork &w, hipStream_t s,
+                 uint32_t *packed = nullptr);
 
     // BF16 inference weights from the master weights (after loading or
     // training; synchronous).
@@ -142,9 +145,10 @@ private:
 };
 
 // g0, g1 from the scatter model's own terms E0, E1 of the unrotated
-// estimate (engine layout): the learned source with exact terms.
+// estimate (engine layout): the learned source with exact terms ('packed'
+// as in ScatterCnn::sources).
 void cnn_exact_sources(const float *E0, const float *E1, const float *x, const float *mu, int N,
                        int Nz, const CnnPhys &ph, float primary, float *g0, float *g1,
-                       hipStream_t s);
+                       hipStream_t s, uint32_t *packed = nullptr);
 
 }  // namespace irlamd

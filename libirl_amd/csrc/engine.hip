@@ -135,34 +135,78 @@ __device__ __forceinline__ float rot_sample(const float *__restrict__ plane, lon
 // offsets are scalar values and a tap costs one vector load; offsets within
 // a volume fit 32 bits (checked when a reconstruction is created).  The
 // float operations are those of rot_sample.
+struct RotPos {
+    float FloatX, FloatY;
+    int OldX, OldY, off;          // off: element offset of tap (OldY, OldX)
+    bool t0, t1, t2, t3;          // taps (0,0), (1,0), (0,1), (1,1) (y, x) inside
+};
+__device__ __forceinline__ RotPos rot_pos(int ystride, int xstride, int OldDim, float X, float Y,
+                                          float sn, float cs, float OldAxis)
+{
+    RotPos q;
+    q.FloatX = X * cs - Y * sn + OldAxis;
+    q.FloatY = X * sn + Y * cs + OldAxis;
+    q.OldX = (int)q.FloatX;
+    q.OldY = (int)q.FloatY;
+    const bool x0 = q.OldX >= 0 && q.OldX < OldDim;
+    const bool y0 = q.OldY >= 0 && q.OldY < OldDim;
+    const bool x1 = q.OldX + 1 >= 0 && q.OldX + 1 < OldDim;
+    const bool y1 = q.OldY + 1 >= 0 && q.OldY + 1 < OldDim;
+    q.t0 = x0 && y0;
+    q.t1 = x0 && y1;
+    q.t2 = x1 && y0;
+    q.t3 = x1 && y1;
+    q.off = q.OldY * ystride + q.OldX * xstride;
+    return q;
+}
+__device__ __forceinline__ float rot_lerp(const RotPos &q, float z0, float z1, float z2, float z3)
+{
+    const float wy = q.FloatY - (float)q.OldY;
+    float z01 = z0 + wy * (z1 - z0);
+    float z23 = z2 + wy * (z3 - z2);
+    if (q.FloatY < 0) {
+        z01 = z0;
+        z23 = z2;
+    }
+    const float wx = q.FloatX - (float)q.OldX;
+    float z0123 = z01 + wx * (z23 - z01);
+    if (q.FloatX < 0) z0123 = z01;
+    return z0123;
+}
 __device__ __forceinline__ float rot_sample_w(const float *__restrict__ plane, int ystride,
                                               int xstride, int OldDim, float X, float Y,
                                               float sn, float cs, float OldAxis)
 {
-    const float FloatX = X * cs - Y * sn + OldAxis;
-    const float FloatY = X * sn + Y * cs + OldAxis;
-    const int OldX = (int)FloatX;
-    const int OldY = (int)FloatY;
-    const bool x0 = OldX >= 0 && OldX < OldDim;
-    const bool y0 = OldY >= 0 && OldY < OldDim;
-    const bool x1 = OldX + 1 >= 0 && OldX + 1 < OldDim;
-    const bool y1 = OldY + 1 >= 0 && OldY + 1 < OldDim;
-    const float *p0 = plane + (OldY * ystride + OldX * xstride);
-    const float z0 = (x0 && y0) ? p0[0] : 0.0f;
-    const float z1 = (x0 && y1) ? p0[ystride] : 0.0f;
-    const float z2 = (x1 && y0) ? p0[xstride] : 0.0f;
-    const float z3 = (x1 && y1) ? p0[ystride + xstride] : 0.0f;
-    const float wy = FloatY - (float)OldY;
-    float z01 = z0 + wy * (z1 - z0);
-    float z23 = z2 + wy * (z3 - z2);
-    if (FloatY < 0) {
-        z01 = z0;
-        z23 = z2;
-    }
-    const float wx = FloatX - (float)OldX;
-    float z0123 = z01 + wx * (z23 - z01);
-    if (FloatX < 0) z0123 = z01;
-    return z0123;
+    const RotPos q = rot_pos(ystride, xstride, OldDim, X, Y, sn, cs, OldAxis);
+    const float *p0 = plane + q.off;
+    const float z0 = q.t0 ? p0[0] : 0.0f;
+    const float z1 = q.t1 ? p0[ystride] : 0.0f;
+    const float z2 = q.t2 ? p0[xstride] : 0.0f;
+    const float z3 = q.t3 ? p0[ystride + xstride] : 0.0f;
+    return rot_lerp(q, z0, z1, z2, z3);
+}
+
+// Two bf16 values packed in 32 bits (the first in the low half), as floats.
+__device__ __forceinline__ float bf_lo(uint32_t w) { return __builtin_bit_cast(float, w << 16); }
+__device__ __forceinline__ float bf_hi(uint32_t w)
+{
+    return __builtin_bit_cast(float, w & 0xffff0000u);
+}
+
+// rot_sample_w of two volumes held as bf16 pairs (one 4-byte tap for both):
+// the samples of the low and of the high halves.
+__device__ __forceinline__ float2 rot_sample_pair(const uint32_t *__restrict__ plane, int ystride,
+                                                  int xstride, int OldDim, float X, float Y,
+                                                  float sn, float cs, float OldAxis)
+{
+    const RotPos q = rot_pos(ystride, xstride, OldDim, X, Y, sn, cs, OldAxis);
+    const uint32_t *p0 = plane + q.off;
+    const uint32_t w0 = q.t0 ? p0[0] : 0u;
+    const uint32_t w1 = q.t1 ? p0[ystride] : 0u;
+    const uint32_t w2 = q.t2 ? p0[xstride] : 0u;
+    const uint32_t w3 = q.t3 ? p0[ystride + xstride] : 0u;
+    return make_float2(rot_lerp(q, bf_lo(w0), bf_lo(w1), bf_lo(w2), bf_lo(w3)),
+                       rot_lerp(q, bf_hi(w0), bf_hi(w1), bf_hi(w2), bf_hi(w3)));
 }
 
 // Value of lane k of a wave (k wave-uniform), as a scalar.
@@ -476,11 +520,13 @@ __global__ __launch_bounds__(256) void k_att(const float *__restrict__ x,
 // form of the ESSE effective source (krnlprj3d.c:836-838, 903-920, 946-947):
 //   att = (rot g0 + d rot g1) afac,  d = -ln(afac) / mu0
 // where -ln(afac) is the half-pixel running attenuation sum the reference's
-// depth loops accumulate (afac = exp(-sum), prjbck.c:212-222).
+// depth loops accumulate (afac = exp(-sum), prjbck.c:212-222).  PK: g0 and
+// g1 held as bf16 pairs in gp (bilinear rotation only).
 // ---------------------------------------------------------------------------
-template <bool SHEAR>
+template <bool SHEAR, bool PK>
 __global__ __launch_bounds__(256) void k_att_src(const float *__restrict__ g0,
                                                  const float *__restrict__ g1,
+                                                 const uint32_t *__restrict__ gp,
                                                  const float *__restrict__ afac,
                                                  float *__restrict__ att, int Nz, int N, int NRot,
                                                  const int *__restrict__ bview,
@@ -513,6 +559,11 @@ __global__ __launch_bounds__(256) void k_att_src(const float *__restrict__ g0,
         if (SHEAR) {
             v0 = shear_sample(sd, 2 * v, g0 + z, (long)N * Nz, Nz, p.X, Y);
             v1 = shear_sample(sd, 2 * v, g1 + z, (long)N * Nz, Nz, p.X, Y);
+        } else if (inside && PK) {
+            const float2 v = rot_sample_pair(gp + z, N * Nz, Nz, N, (float)p.X - NewAxis,
+                                             (float)Y - NewAxis, sn, cs, OldAxis);
+            v0 = v.x;
+            v1 = v.y;
         } else if (inside) {
             v0 = rot_sample_w(g0 + z, N * Nz, Nz, N, (float)p.X - NewAxis, (float)Y - NewAxis,
                               sn, cs, OldAxis);
@@ -1451,6 +1502,10 @@ struct Recon::Impl {
     std::unique_ptr<EsseDev> esse;
     // learned scatter
     DBuf<float> g0, g1, att2;
+    // BF16 precision modes other than the always-split one (bilinear
+    // rotation): g0 and g1 as bf16 pairs instead, one 4-byte gather tap for
+    // both (the projector takes bf16 values of the same magnitude)
+    DBuf<uint32_t> gpk;
     std::unique_ptr<CnnWork> cnnWork;   // this reconstruction's inference buffers
     DBuf<double> chk;
     CnnPhys phys;
@@ -1570,14 +1625,19 @@ struct Recon::Impl {
         const float inv = 1.0f / phys.mu0;
         const int *rk = half ? rowKernel16.p : nullptr;
         const int *dm = half ? distSplit.p : nullptr;
+        const dim3 grid = att_grid(g.Nz, g.NRot, g.NRot, B);
         if (c.shears)
-            hipLaunchKernelGGL(k_att_src<true>, att_grid(g.Nz, g.NRot, g.NRot, B), dim3(256), 0,
-                               stream, g0.p, g1.p, afacp, dst, g.Nz, g.N, g.NRot, bview, fsin.p,
+            hipLaunchKernelGGL((k_att_src<true, false>), grid, dim3(256), 0, stream, g0.p, g1.p,
+                               nullptr, afacp, dst, g.Nz, g.N, g.NRot, bview, fsin.p, fcos.p,
+                               head.p, npix.p, shear(), inv, rk, dm);
+        else if (gpk.p)
+            hipLaunchKernelGGL((k_att_src<false, true>), grid, dim3(256), 0, stream, nullptr,
+                               nullptr, gpk.p, afacp, dst, g.Nz, g.N, g.NRot, bview, fsin.p,
                                fcos.p, head.p, npix.p, shear(), inv, rk, dm);
         else
-            hipLaunchKernelGGL(k_att_src<false>, att_grid(g.Nz, g.NRot, g.NRot, B), dim3(256), 0,
-                               stream, g0.p, g1.p, afacp, dst, g.Nz, g.N, g.NRot, bview, fsin.p,
-                               fcos.p, head.p, npix.p, shear(), inv, rk, dm);
+            hipLaunchKernelGGL((k_att_src<false, false>), grid, dim3(256), 0, stream, g0.p, g1.p,
+                               nullptr, afacp, dst, g.Nz, g.N, g.NRot, bview, fsin.p, fcos.p,
+                               head.p, npix.p, shear(), inv, rk, dm);
     }
 
     // Learned-scatter source volumes g0, g1 from the current estimate: the
@@ -1589,9 +1649,9 @@ struct Recon::Impl {
         if (c.cnnExact) {
             esse->convolve(x.p, 1, terms.p);
             cnn_exact_sources(terms.p, terms.p + M, x.p, mu.p, g.N, g.Nz, phys, 1.0f, g0.p, g1.p,
-                              stream);
+                              stream, gpk.p);
         } else {
-            c.cnn->sources(x.p, mu.p, g.N, g.Nz, phys, 1.0f, g0.p, g1.p, *cnnWork, stream);
+            c.cnn->sources(x.p, mu.p, g.N, g.Nz, phys, 1.0f, g0.p, g1.p, *cnnWork, stream, gpk.p);
         }
     }
 
@@ -2055,8 +2115,12 @@ Recon::Recon(const Config &cfg) : impl_(new Impl), cfg_(cfg)
                 I.phys.fMu = em.mu0 - em.muavg;
                 I.phys.fDens = (1.0f / em.mu0) / em.pixsize;
                 I.phys.cntr = em.cntrPixFac;
-                I.g0.alloc(I.M);
-                I.g1.alloc(I.M);
+                if (cfg.precision > 0 && cfg.precision != 2 && !cfg.shears) {
+                    I.gpk.alloc(I.M);
+                } else {
+                    I.g0.alloc(I.M);
+                    I.g1.alloc(I.M);
+                }
                 if (cfg.cnn) I.cnnWork.reset(new CnnWork(g.N, g.Nz));
                 I.att2.alloc(I.vol * Vs);
                 I.chk.alloc(6);
