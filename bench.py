@@ -345,6 +345,7 @@ def workload_line(w, steps, warmup, precision, ref_estimate=None, cnn=None, chec
     line = {"workload": name + (" (BF16 MFMA DRF, FP32 accumulation)" if bf16 else
                                 " (%s)" % DRF_NAMES[drf]),
             "tiles_prj_bck_cnnrow": list(rec.tiles()) + [rec.cnn_row_length()],
+            "prj_rows_per_chunk": rec.prj_rows_per_chunk(),
             "value": steps / dt, "unit": "subiters/s", "ms_per_step": dt * 1e3 / steps,
             "drf_algorithm": {"prj": algo[0], "bck": algo[1], "fft_length": algo[2]}
             if not bf16 else "bf16 mfma",
@@ -538,6 +539,7 @@ def batch_line(tmpdir, net, steps=16, warmup=24):
     # replayed once, so the timed steps replay warm graph executables
     est = [r.estimate() for r in recs]          # after warmup + steps = 5 iterations
     tiles = [r.tiles() + (r.cnn_row_length(),) for r in recs]
+    rows_per_chunk = [r.prj_rows_per_chunk() for r in recs]
     status = [L.cnn_status(r) for r in recs]
     # the same steps as one parent-graph launch per step (its executables
     # instantiated first), each mode timed twice, alternating
@@ -589,6 +591,7 @@ def batch_line(tmpdir, net, steps=16, warmup=24):
             "batch_speedup_vs_one_at_a_time": dt_seq / dt_flat,
             "sizes": [[c["n"], c["views"]] for c in cfgs],
             "tiles_prj_bck_cnnrow": tiles,
+            "prj_rows_per_chunk": rows_per_chunk,
             "replays": st["replays"],
             "eager_subiterations": st["eager_subiterations"],
             "learned_scatter_active": sum(s["active"] for s in status),

@@ -102,6 +102,13 @@ int irlamd_recon_tiles(irlamd_recon *r, int *piPrj, int *piBck);
  * graphs are dropped.  Results do not depend on the tiles.  No counterpart
  * in the reference (tuning of this library's kernels). */
 int irlamd_recon_set_tiles(irlamd_recon *r, int iPrj, int iBck);
+/* Rows per chunk of the spatial projector (each chunk's rows summed into one
+ * partial plane, the chunks added in order by the ratio step): NumRotPixs/32,
+ * or with the BF16 DRF twice that where the projector's launch at its
+ * largest tile still has a block for every CU.  Fixed by the geometry (it
+ * groups the float sums of the projection).  No counterpart in the
+ * reference (tuning of this library's kernels). */
+int irlamd_recon_prj_rows_per_chunk(irlamd_recon *r);
 
 /* Scatter model of reconstructions created afterwards: 1 (the default) =
  * where the grid allows (equal in-plane FFT lengths of 128 or 160, two

@@ -229,6 +229,7 @@ def load():
         [ctypes.POINTER(ctypes.c_float)] * 4
     lib.irlamd_set_graphs.argtypes = [ctypes.c_int]
     lib.irlamd_recon_cnn_row_length.argtypes = [vp]
+    lib.irlamd_recon_prj_rows_per_chunk.argtypes = [vp]
     lib.irlamd_set_autotune.argtypes = [ctypes.c_int]
     lib.irlamd_recon_tiles.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     lib.irlamd_recon_set_tiles.argtypes = [vp, ctypes.c_int, ctypes.c_int]
@@ -478,6 +479,10 @@ class Recon:
     def cnn_row_length(self):
         """Slices per wave of the scatter network's MFMA layers in use (0: none)."""
         return self._lib.irlamd_recon_cnn_row_length(self._h)
+
+    def prj_rows_per_chunk(self):
+        """Rows per partial plane of the spatial projector in use."""
+        return self._lib.irlamd_recon_prj_rows_per_chunk(self._h)
 
     def drf_algorithms(self):
         """FP32 DRF algorithm per direction ("spatial" or "fft") and the FFT length."""

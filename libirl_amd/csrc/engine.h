@@ -184,6 +184,7 @@ public:
     // wave of the BF16 kernel), for the projector and the backprojector.
     void autotuneTiles();
     int cnnRowLength() const;   // the scatter network's row length in use (0: none)
+    int prjRowsPerChunk() const;   // the spatial projector's rows per partial plane
     void tiles(int *prj, int *bck) const;
     void setTiles(int prj, int bck);   // 0: size heuristic; drops captured graphs
     // FP32 DRF algorithm in use per direction: 0 spatial, 1 FFT (length P)

@@ -1708,6 +1708,28 @@ struct Recon::Impl {
     // DRF arguments of the engine's slice-fastest planes.  PRJ: input att
     // [B][row][bin][z], output ppart [B][chunk][bin][z]; BCK: input ratio
     // [B][bin][z] times afac [V][row][bin][z], output planes [B][row][bin][z].
+    // k_ratio's arguments for a batch of B views (no scatter-model mode)
+    RatioArgs ratio_args(const int *bview, int B) const
+    {
+        RatioArgs ra{};
+        ra.ppart2 = ppart2.p;
+        ra.prjscat = prjscat.p;
+        ra.B = B;
+        ra.Nz = c.g.Nz;
+        ra.NRot = c.g.NRot;
+        ra.nchunk = nchunk;
+        ra.bview = bview;
+        ra.ppart = ppart.p;
+        ra.prj = prj.p;
+        ra.scat = scat.p;
+        ra.vtsens = vtsens.p;
+        ra.vt = vt.p;
+        ra.colKeep = colKeep.p;
+        ra.scale = 1.0f;
+        ra.ratio = ratio.p;
+        return ra;
+    }
+
     DrfArgs drf_args(bool bck, const int *bview, int model) const
     {
         const Geometry &g = c.g;
@@ -2077,8 +2099,20 @@ Recon::Recon(const Config &cfg) : impl_(new Impl), cfg_(cfg)
     I.P = (long)g.NRot * g.Nz;
     I.vol = (long)g.NRot * g.Nz * g.NRot;
     // Rows per projection chunk: enough chunks that a subset's projector
-    // launch fills the 256 CUs several times over.
+    // launch fills the CUs several times over.  The BF16 kernel takes twice
+    // the rows (half the partial planes for the ratio step to add) where its
+    // launch at the largest tile (128 slices per block) still has a block
+    // for every CU.  A rule of the geometry, not a timing: the chunking
+    // groups the projector's float sums, so results stay reproducible.
     I.rchunk = I.rchunkD = std::max(1, g.NRot / 32);
+    if (cfg.precision > 0 && 2 * I.rchunkD < g.NRot) {
+        int dev = 0, cus = 0;
+        HIP_CHECK(hipGetDevice(&dev));
+        HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        const long blocks = (long)div_up(g.NRot, DRF_TX) * div_up(g.Nz, 128) * cfg.Vs *
+                            div_up(g.NRot, 2 * I.rchunkD);
+        if (blocks >= cus) I.rchunk = I.rchunkD = 2 * I.rchunkD;
+    }
     I.nchunk = I.nchunkD = div_up(g.NRot, I.rchunk);
     I.rchunkF = std::max(I.rchunkD, g.NRot / 16);
     I.nchunkF = div_up(g.NRot, I.rchunkF);
@@ -2645,7 +2679,10 @@ void Recon::autotuneTiles()
     const int KYb = cfg_.drf.nDist > 0 ? I.ky_bucket() : 0;
     const int *bview = I.timing_views();
     for (int bck = 0; bck < 2 && cfg_.drf.nDist > 0; ++bck) {
-        const DrfArgs da = I.drf_args(bck != 0, bview, bck ? cfg_.bckModel : cfg_.prjModel);
+        DrfArgs da = I.drf_args(bck != 0, bview, bck ? cfg_.bckModel : cfg_.prjModel);
+        // the projector's input as the sub-iterations store it (the values
+        // are not used)
+        if (!bck) da.inBf16 = I.bf16 ? 1 : 0;
         int best = 0;
         float bestMs = 1e30f;
         // BF16: 2, 4 or 8 slice tiles of 16 per wave; FP32: 4 or 8 slice
@@ -2688,6 +2725,8 @@ void Recon::setTiles(int prj, int bck)
 }
 
 int Recon::cnnRowLength() const { return impl_->cnnWork ? impl_->cnnWork->rowz : 0; }
+
+int Recon::prjRowsPerChunk() const { return impl_->rchunkD; }
 
 void Recon::drfAlgorithms(int *prj, int *bck, int *P) const
 {
@@ -3063,23 +3102,8 @@ void Recon::runViews(int s, int first, int Vs, SrfMode srf, bool accumulate, boo
         d2.out = I.ppart2.p;
         I.launch(I.drf_name("drf_prj", false), [&] { I.run_drf(false, d2, Vs, KYb); });
     }
-    RatioArgs ra{};
+    RatioArgs ra = I.ratio_args(bview, Vs);
     ra.srfMode = srf == SRF_MODEL_STORE ? 2 : srf == SRF_ADD_STORED ? 3 : 0;
-    ra.ppart2 = I.ppart2.p;
-    ra.prjscat = I.prjscat.p;
-    ra.B = Vs;
-    ra.Nz = g.Nz;
-    ra.NRot = g.NRot;
-    ra.nchunk = I.nchunk;
-    ra.bview = bview;
-    ra.ppart = I.ppart.p;
-    ra.prj = I.prj.p;
-    ra.scat = I.scat.p;
-    ra.vtsens = I.vtsens.p;
-    ra.vt = I.vt.p;
-    ra.colKeep = I.colKeep.p;
-    ra.scale = 1.0f;
-    ra.ratio = I.ratio.p;
     I.launch("ratio", [&] {
         hipLaunchKernelGGL(k_ratio, dim3(div_up(I.P, threads), Vs), dim3(threads), 0,
                            I.stream, ra);
