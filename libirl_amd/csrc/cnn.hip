@@ -95,10 +95,12 @@ __global__ __launch_bounds__(256) void k_sum_partial(const float *__restrict__ v
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-__global__ void k_inv_mean(const double *__restrict__ part, int nb, long n,
-                           float *__restrict__ scal)
+// 1/mean and mean (1 where the mean is not positive) of n values from nb
+// FP64 block sums, by a block of 256 threads with 256 doubles of LDS: every
+// caller adds the sums in the same order, so all get the same floats.
+__device__ __forceinline__ void inv_mean_block(const double *__restrict__ part, int nb, long n,
+                                               double *red, float &inv, float &mean)
 {
-    __shared__ double red[256];
     double s = 0.0;
     for (int i = threadIdx.x; i < nb; i += 256) s += part[i];
     red[threadIdx.x] = s;
@@ -107,11 +109,20 @@ __global__ void k_inv_mean(const double *__restrict__ part, int nb, long n,
         if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
+    const double m = red[0] / (double)n;
+    mean = m > 0 ? (float)m : 1.0f;
+    inv = 1.0f / mean;
+}
+
+__global__ __launch_bounds__(256) void k_inv_mean(const double *__restrict__ part, int nb, long n,
+                                                  float *__restrict__ scal)
+{
+    __shared__ double red[256];
+    float inv, mean;
+    inv_mean_block(part, nb, n, red, inv, mean);
     if (threadIdx.x == 0) {
-        const double m = red[0] / (double)n;
-        const float sx = m > 0 ? (float)m : 1.0f;
-        scal[0] = 1.0f / sx;
-        scal[1] = sx;
+        scal[0] = inv;
+        scal[1] = mean;
     }
 }
 
@@ -282,9 +293,10 @@ __global__ __launch_bounds__(256) void k_conv_valu(const TI *__restrict__ in,
 __global__ __launch_bounds__(256) void k_conv_l1_mfma(const float2 *__restrict__ in,
                                                       const uint4 *__restrict__ Wf,
                                                       const float *__restrict__ bias,
-                                                      const float *__restrict__ scal,
+                                                      float *__restrict__ scal,
                                                       uint16_t *__restrict__ out, int Y, int X,
-                                                      int Z)
+                                                      int Z, const double *__restrict__ part,
+                                                      int nb, long nvals)
 {
     constexpr int SL = 64 + 2;
     __shared__ float2 seg[4][9 * SL];
@@ -296,7 +308,21 @@ __global__ __launch_bounds__(256) void k_conv_l1_mfma(const float2 *__restrict__
     const long row = valid ? task / zt : 0;
     const int z0 = valid ? (int)(task % zt) * 64 : 0;
     const int y = (int)(row / X), x = (int)(row % X);
-    const float s0 = scal[0];
+    // the input's scale 1/mean(x): from scal, or (part) from the block sums
+    // of x, every block adding them as k_inv_mean does and block 0 storing
+    // both factors in scal for the later kernels
+    float s0;
+    if (part) {
+        __shared__ double red[256];
+        float mean;
+        inv_mean_block(part, nb, nvals, red, s0, mean);
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            scal[0] = s0;
+            scal[1] = mean;
+        }
+    } else {
+        s0 = scal[0];
+    }
     for (int e = lane; e < 9 * SL; e += 64) {
         const int q = e / SL, k = e - q * SL;
         const int yy = y + q / 3 - 1, xx = x + q % 3 - 1, zz = z0 - 1 + k;
@@ -1071,12 +1097,13 @@ void ScatterCnn::forward(const float *in, int Y, int X, int Z, float *out, bool 
 // Inference arithmetic, activations in the given buffers ([V][32] bf16 each);
 // channel 0 of the input is multiplied by scal[0] (1 for normalised input).
 void ScatterCnn::forward_bf16(const float *in, int Y, int X, int Z, float *out, uint16_t *a0,
-                              uint16_t *a1, const float *scal, int rowz, hipStream_t s)
+                              uint16_t *a1, float *scal, int rowz, hipStream_t s,
+                              const double *part, int nb, long n)
 {
     const dim3 blk(256);
     const long tasks = (long)Y * X * ((Z + CNN_ROWZ - 1) / CNN_ROWZ);
     hipLaunchKernelGGL(k_conv_l1_mfma, dim3(cdiv_u(tasks, 4)), blk, 0, s, (const float2 *)in,
-                       (const uint4 *)Wl1_, b_[0], scal ? scal : one_, a0, Y, X, Z);
+                       (const uint4 *)Wl1_, b_[0], scal ? scal : one_, a0, Y, X, Z, part, nb, n);
     static_assert(cnn_dil(CNN_LAYERS - 1) == CNN_MAXDIL, "largest dilation");
     switch (rowz) {
     case 64: mfma_layers<64>(Y, X, Z, out, a0, a1, s); break;
@@ -1220,8 +1247,9 @@ void ScatterCnn::sources(const float *x, const float *mu, int N, int Nz, const C
     const unsigned pb = cdiv_u(V, 256);
     if (pb > w.nb) fail(ETYPE_ILLEGAL_VALUE, "scatter CNN work buffers too small");
     hipLaunchKernelGGL(k_cnn_pool, dim3(pb), dim3(256), 0, s, x, mu, N, Nz, ph.fDens, w.in, w.psum);
-    hipLaunchKernelGGL(k_inv_mean, dim3(1), dim3(256), 0, s, w.psum, (int)pb, n, w.scal);
-    forward_bf16(w.in, Y, X, Z, w.out, w.actb[0], w.actb[1], w.scal, w.rowz, s);
+    // layer 1 takes mean(x) from the pooling's block sums and stores it
+    forward_bf16(w.in, Y, X, Z, w.out, w.actb[0], w.actb[1], w.scal, w.rowz, s, w.psum,
+                 (int)pb, n);
     hipLaunchKernelGGL(k_cnn_source, dim3(cdiv_u(n, 256)), dim3(256), 0, s, w.out, x, mu, N, Nz,
                        w.scal, kappa_, ph.fDens, ph.cntr, primary, g0, g1, packed);
     CNN_CHECK(hipGetLastError());

@@ -118,8 +118,11 @@ private:
     void alloc_state();
     void ensure_work(long V);
     void upload_bf16_weights(hipStream_t s);
+    // scal: the input's {1/mean, mean}; with 'part' (nb FP64 block sums of
+    // the n values) layer 1 computes them and stores them in scal
     void forward_bf16(const float *in, int Y, int X, int Z, float *out, uint16_t *a0,
-                      uint16_t *a1, const float *scal, int rowz, hipStream_t s);
+                      uint16_t *a1, float *scal, int rowz, hipStream_t s,
+                      const double *part = nullptr, int nb = 0, long n = 0);
     template <int ROWZ>
     void mfma_layers(int Y, int X, int Z, float *out, uint16_t *a0, uint16_t *a1, hipStream_t s);
 

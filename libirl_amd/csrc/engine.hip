@@ -584,37 +584,66 @@ __global__ __launch_bounds__(256) void k_att_src(const float *__restrict__ g0,
 // Source check: per (view, bin, slice) column the depth sums of the model's
 // source attE (its projection before the collimator blur), of its scatter
 // part (attE - rotx afac) and of the difference to the learned source
-// (attE - attC); acc[0] += (sum diff)^2, acc[1] += (sum scatter)^2,
-// acc[2] += (sum attE)^2.
-__global__ __launch_bounds__(256) void k_scatter_check(const float *__restrict__ attE,
+// (attE - attC); per block the sums over its columns of (sum diff)^2,
+// (sum scatter)^2 and (sum attE)^2 into part3[(b * blocks + block) * 3 ..].
+// attC holds the projector's input as stored: with rk16, the rows
+// att_store_h wrote in bf16 are read as bf16.
+constexpr int CHK_MAXROWS = 2048;   // rows of a checked plane (NumRotPixs)
+constexpr int CHK_RG = 8;           // row groups (waves) per block
+__global__ __launch_bounds__(64 * CHK_RG) void k_scatter_check(const float *__restrict__ attE,
                                                        const float *__restrict__ attC,
                                                        const int *__restrict__ cplane,
                                                        const float *__restrict__ rotx,
                                                        const float *__restrict__ afac,
                                                        const int *__restrict__ bview, int Nz,
-                                                       int NRot, double *__restrict__ acc)
+                                                       int NRot, double *__restrict__ part3,
+                                                       const int *__restrict__ rk16 = nullptr,
+                                                       const int *__restrict__ dmode = nullptr)
 {
     // grid (blocks of 64 columns, checked views): view b's model source
     // attE plane b, its learned source attC plane cplane[b], sums into
-    // acc[3b..3b+2].  The block's four waves take every fourth row of the
-    // same 64 columns; their partial depth sums are added in row-group order.
-    __shared__ float part[3][4][64];
+    // part3.  The block's CHK_RG waves take every CHK_RG-th row of
+    // the same 64 columns; their partial depth sums are added in row-group
+    // order.
+    __shared__ float part[3][CHK_RG][64];
     __shared__ double red[3][64];
+    __shared__ unsigned char half[CHK_MAXROWS];   // per row: attC stored in bf16
     const int plane = NRot * Nz;
     const long vol = (long)plane * NRot;
     const int col = threadIdx.x & 63, rg = threadIdx.x >> 6;
     const int q = blockIdx.x * 64 + col;
     const int b = blockIdx.y;
+    const int v = bview[b];
+    for (int r = threadIdx.x; r < NRot; r += 64 * CHK_RG)
+        half[r] = rk16 && dmode[rk16[(long)v * NRot + r]] <= 1;
+    __syncthreads();
     float dsum = 0.0f, ssum = 0.0f, tsum = 0.0f;
     if (q < plane) {
-        const int v = bview[b];
-        const long ce = (long)b * vol, cc = (long)cplane[b] * vol;
-        for (int r = rg; r < NRot; r += 4) {
-            const long o = (long)r * plane + q;
-            const float e = attE[ce + o];
-            dsum += e - attC[cc + o];
-            ssum += e - rotx[ce + o] * afac[(long)v * vol + o];
-            tsum += e;
+        const long ce = (long)b * vol, cc = (long)cplane[b] * vol, cf = (long)v * vol;
+        // CB rows of the thread's row group at a time: every load of the
+        // batch is issued before the sums take them, in row order
+        constexpr int CB = 8;
+        for (int r0 = rg; r0 < NRot; r0 += CHK_RG * CB) {
+            float e[CB], c[CB], m[CB];
+#pragma unroll
+            for (int k = 0; k < CB; ++k) {
+                const int r = r0 + CHK_RG * k;
+                e[k] = c[k] = m[k] = 0.0f;
+                if (r < NRot) {
+                    const long o = (long)r * plane + q;
+                    e[k] = attE[ce + o];
+                    c[k] = half[r] ? (float)((const __bf16 *)(attC + cc + (long)r * plane))[q]
+                                   : attC[cc + o];
+                    m[k] = rotx[ce + o] * afac[cf + o];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < CB; ++k)
+                if (r0 + CHK_RG * k < NRot) {
+                    dsum += e[k] - c[k];
+                    ssum += e[k] - m[k];
+                    tsum += e[k];
+                }
         }
     }
     part[0][rg][col] = dsum;
@@ -624,7 +653,9 @@ __global__ __launch_bounds__(256) void k_scatter_check(const float *__restrict__
     if (rg == 0) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            const float t = ((part[k][0][col] + part[k][1][col]) + part[k][2][col]) + part[k][3][col];
+            float t = part[k][0][col];
+#pragma unroll
+            for (int j = 1; j < CHK_RG; ++j) t += part[k][j][col];
             red[k][col] = (double)t * t;
         }
     }
@@ -637,11 +668,35 @@ __global__ __launch_bounds__(256) void k_scatter_check(const float *__restrict__
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        atomicAdd(acc + 3 * b, red[0][0]);
-        atomicAdd(acc + 3 * b + 1, red[1][0]);
-        atomicAdd(acc + 3 * b + 2, red[2][0]);
+    if (threadIdx.x < 3)
+        part3[((long)b * gridDim.x + blockIdx.x) * 3 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// Outcome of a check: the block sums of k_scatter_check added in block
+// order into acc[3j..3j+2] for each of the nchk views (nb blocks per view),
+// and fail = 1 when the larger relative L2 error of the learned projection
+// (sqrt of acc[3j] / acc[3j+2]) exceeds maxNrmse or is not a number.
+__global__ __launch_bounds__(64) void k_check_outcome(const double *__restrict__ part3, int nb,
+                                                      int nchk, float maxNrmse,
+                                                      double *__restrict__ acc,
+                                                      int *__restrict__ fail)
+{
+    const int t = threadIdx.x;
+    if (t < 3 * nchk) {
+        const int j = t / 3, k = t % 3;
+        double s = 0.0;
+        for (int i = 0; i < nb; ++i) s += part3[((long)j * nb + i) * 3 + k];
+        acc[t] = s;
     }
+    __syncthreads();
+    if (t != 0) return;
+    float e = 0.0f;
+    for (int j = 0; j < nchk; ++j) {
+        const double *q = acc + 3 * j;
+        const float r = q[2] > 0 ? (float)sqrt(q[0] / q[2]) : 0.0f;
+        if (r > e || r != r) e = r;
+    }
+    fail[0] = !(e <= maxNrmse);
 }
 
 // ---------------------------------------------------------------------------
@@ -1288,12 +1343,14 @@ struct BckArgs {
     int writeU;
     float *ninvOut;               // NORM: output
     ShearDev sd;                  // shear tables (back rotation in slot 2v+1)
+    const int *skip;              // when *skip != 0 nothing is written (a failed check)
 };
 
 template <bool NORM, bool SHEAR>
 __global__ __launch_bounds__(256) void k_bck(BckArgs a)
 {
     typedef typename std::conditional<NORM, double, float>::type acc_t;
+    if (a.skip && *a.skip) return;
     const PixIdx p = pix_index(a.Nz, a.N);
     if (p.X >= a.N) return;   // whole waves
     // lanes past the last slice sample slice Nz-1 and store nothing, so
@@ -1508,6 +1565,19 @@ struct Recon::Impl {
     DBuf<uint32_t> gpk;
     std::unique_ptr<CnnWork> cnnWork;   // this reconstruction's inference buffers
     DBuf<double> chk;
+    // A check's scatter-model pass runs on the side stream beside the
+    // projection of the learned source; its outcome (chkFail, 1 = failed)
+    // gates the update on the device, and the sums and outcome come back
+    // through pinned host memory (chkDone marks their arrival).
+    DBuf<int> chkFail;
+    DBuf<double> chkPart;         // per checked view and block of k_scatter_check: 3 sums
+    hipStream_t side = nullptr;
+    hipEvent_t chkFork = nullptr, chkDone = nullptr;
+    struct CheckHost {
+        double q[6];
+        int fail;
+    };
+    CheckHost *chkHost = nullptr;
     CnnPhys phys;
     Recon::CnnStatus cst;
     int cnnSubsets = 0;           // modelled subsets seen
@@ -1671,6 +1741,58 @@ struct Recon::Impl {
         launch("esse_source", [&] {
             esse->source(rotx.p, rotmu.p, terms.p, B, 1.0f, afacp, bview, dst ? dst : att.p, true);
         });
+    }
+
+    // Check of the learned source against the scatter model on two of the
+    // subset's views (its first and middle one; one view where the batch has
+    // one), on the side stream (the engine's while kernels are timed): the
+    // model's source (att2 planes 0, 1), the depth-sum errors against the
+    // learned source as the projector takes it (att planes chkPlanes, bf16
+    // rows with 'half'), the outcome chkFail, the copies to chkHost and
+    // chkDone.  The caller makes the engine's stream wait for chkDone before
+    // the update gated by chkFail and before att is written again.
+    void start_check(const float *afacp, int s, int Vs, const int *bview, float maxNrmse,
+                     bool half)
+    {
+        const Geometry &g = c.g;
+        const int nchk = Vs >= 2 ? 2 : 1;
+        const int *cv = nchk == 2 ? chkViews.p + 2 * s : bview;
+        if (g.NRot > CHK_MAXROWS)
+            fail(ETYPE_ILLEGAL_VALUE, "learned-scatter check: NumRotPixs %d above %d", g.NRot,
+                 CHK_MAXROWS);
+        const hipStream_t own = stream;
+        const bool fork = !timing;
+        if (fork) {
+            HIP_CHECK(hipEventRecord(chkFork, own));
+            HIP_CHECK(hipStreamWaitEvent(side, chkFork, 0));
+            stream = side;
+            esse->setStream(side);
+        }
+        auto restore = [&] {
+            stream = own;
+            esse->setStream(own);
+        };
+        try {
+            run_srf_att(afacp, nchk, cv, att2.p);
+            const int nb = (int)div_up(P, 64);
+            launch("cnn_check", [&] {
+                hipLaunchKernelGGL(k_scatter_check, dim3(nb, nchk), dim3(64 * CHK_RG), 0, stream,
+                                   att2.p, att.p, chkPlanes.p, rotx.p, afacp, cv, g.Nz, g.NRot,
+                                   chkPart.p, half ? rowKernel16.p : nullptr,
+                                   half ? distSplit.p : nullptr);
+                hipLaunchKernelGGL(k_check_outcome, dim3(1), dim3(64), 0, stream, chkPart.p, nb,
+                                   nchk, maxNrmse, chk.p, chkFail.p);
+            });
+            HIP_CHECK(hipMemcpyAsync(chkHost->q, chk.p, 3 * nchk * sizeof(double),
+                                     hipMemcpyDeviceToHost, stream));
+            HIP_CHECK(hipMemcpyAsync(&chkHost->fail, chkFail.p, sizeof(int),
+                                     hipMemcpyDeviceToHost, stream));
+            HIP_CHECK(hipEventRecord(chkDone, stream));
+        } catch (...) {
+            restore();
+            throw;
+        }
+        restore();
     }
 
     BckArgs bck_args(const int *bview)
@@ -2156,8 +2278,15 @@ Recon::Recon(const Config &cfg) : impl_(new Impl), cfg_(cfg)
                     I.g1.alloc(I.M);
                 }
                 if (cfg.cnn) I.cnnWork.reset(new CnnWork(g.N, g.Nz));
-                I.att2.alloc(I.vol * Vs);
+                I.att2.alloc(I.vol * Vs);   // the model's source of the checked views
                 I.chk.alloc(6);
+                I.chkFail.alloc(1);
+                I.chkPart.alloc(2 * 3 * div_up(I.P, 64));
+                HIP_CHECK(hipStreamCreate(&I.side));
+                HIP_CHECK(hipEventCreateWithFlags(&I.chkFork, hipEventDisableTiming));
+                HIP_CHECK(hipEventCreateWithFlags(&I.chkDone, hipEventDisableTiming));
+                HIP_CHECK(hipHostMalloc((void **)&I.chkHost, sizeof(Impl::CheckHost),
+                                        hipHostMallocDefault));
             }
         }
         I.ppart2.alloc(I.P * I.nchunk * Vs);
@@ -2364,6 +2493,13 @@ Recon::~Recon()
                 for (auto e : kv.second.ev) (void)hipEventDestroy(e);
             (void)hipStreamDestroy(impl_->stream);
         }
+        if (impl_->side) {
+            (void)hipStreamSynchronize(impl_->side);
+            (void)hipStreamDestroy(impl_->side);
+        }
+        if (impl_->chkFork) (void)hipEventDestroy(impl_->chkFork);
+        if (impl_->chkDone) (void)hipEventDestroy(impl_->chkDone);
+        if (impl_->chkHost) (void)hipHostFree(impl_->chkHost);
         delete impl_;
     }
 }
@@ -3040,49 +3176,20 @@ void Recon::runViews(int s, int first, int Vs, SrfMode srf, bool accumulate, boo
     // The BF16 projector's input planes may be written in bf16 (rows whose
     // DRF entry does not split its data; lossless: the projector rounds the
     // same values to bf16) by the rotation kernels; the scatter model's
-    // source kernels and a checked subset's learned source stay FP32.
-    bool attHalf = false;
+    // source kernels stay FP32.
+    bool attHalf = false, checking = false;
     if (model && I.esse && I.cst.active) {
         // once per subset: the source volumes of the current estimate (whether
         // the subset is also checked was decided by beginSubset)
         if (first == 0) I.launch("cnn_src", [&] { I.run_cnn_sources(); });
-        attHalf = I.bf16 && !I.cnnCheckNow;
+        attHalf = I.bf16;
         I.launch("cnn_att", [&] { I.run_att_src(afac, Vs, bview, I.att.p, attHalf); });
         if (I.cnnCheckNow && first == 0) {
-            // the scatter model on two of the subset's views (its first and
-            // middle one) as the check; one view where the batch has one
-            // (views chkViews[s] = subset view 0 and Vs/2, planes chkPlanes),
-            // both through one batched scatter-model pass
-            const int nchk = Vs >= 2 ? 2 : 1;
-            const int *cv = nchk == 2 ? I.chkViews.p + 2 * s : bview;
-            HIP_CHECK(hipMemsetAsync(I.chk.p, 0, 3 * nchk * sizeof(double), I.stream));
-            I.run_srf_att(afac, nchk, cv, I.att2.p);
-            hipLaunchKernelGGL(k_scatter_check, dim3(div_up(I.P, 64), nchk), dim3(threads),
-                               0, I.stream, I.att2.p, I.att.p, I.chkPlanes.p, I.rotx.p, afac, cv,
-                               g.Nz, g.NRot, I.chk.p);
-            double h[6] = {0, 0, 0, 0, 0, 0};
-            HIP_CHECK(hipMemcpyAsync(h, I.chk.p, 3 * nchk * sizeof(double), hipMemcpyDeviceToHost,
-                                     I.stream));
-            HIP_CHECK(hipStreamSynchronize(I.stream));
-            // relative L2 of the learned projection (before the collimator
-            // blur) against the model's, the larger of the views; of its
-            // scatter part for the record
-            float e = 0.0f, es = 0.0f;
-            for (int j = 0; j < nchk; ++j) {
-                const double *q = h + 3 * j;
-                e = std::max(e, q[2] > 0 ? (float)std::sqrt(q[0] / q[2]) : 0.0f);
-                es = std::max(es, q[1] > 0 ? (float)std::sqrt(q[0] / q[1]) : 0.0f);
-            }
-            I.cst.checks++;
-            I.cst.lastNrmse = e;
-            I.cst.lastScatterNrmse = es;
-            I.cst.worstNrmse = std::max(I.cst.worstNrmse, e);
-            I.cst.worstScatterNrmse = std::max(I.cst.worstScatterNrmse, es);
-            if (!(e <= cfg_.cnnMaxNrmse)) {
-                // back to the scatter model, from this subset on
-                I.cst.active = false;
-                I.run_srf_att(afac, Vs, bview);
-            }
+            // the check runs beside the projection of the learned source and
+            // gates the update: a failed check leaves x and U as they were,
+            // and the subset runs again on the scatter model below
+            checking = true;
+            I.start_check(afac, s, Vs, bview, cfg_.cnnMaxNrmse, attHalf);
         }
     } else if (model && I.esse)
         I.run_srf_att(afac, Vs, bview);
@@ -3095,7 +3202,9 @@ void Recon::runViews(int s, int first, int Vs, SrfMode srf, bool accumulate, boo
     I.launch(I.drf_name("drf_prj", false), [&] { I.run_drf(false, da, Vs, KYb); });
     if (srf == SRF_MODEL_STORE) {
         // projection without the scatter model for the stored estimate
-        // (osemrecon.c:757-786), reusing the rotated estimate
+        // (osemrecon.c:757-786), reusing the rotated estimate; a check reads
+        // att until chkDone
+        if (checking) HIP_CHECK(hipStreamWaitEvent(I.stream, I.chkDone, 0));
         I.launch("att", [&] { I.run_att(afac, Vs, bview, nullptr, nullptr, I.bf16); });
         DrfArgs d2 = da;
         d2.inBf16 = I.bf16 ? 1 : 0;
@@ -3117,7 +3226,33 @@ void Recon::runViews(int s, int first, int Vs, SrfMode srf, bool accumulate, boo
     ba.u = I.u.p;
     ba.accumulate = accumulate ? 1 : 0;
     ba.writeU = keepUpdate ? 1 : 0;
+    if (checking) {
+        HIP_CHECK(hipStreamWaitEvent(I.stream, I.chkDone, 0));
+        ba.skip = I.chkFail.p;
+    }
     I.launch("bck_update", [&] { I.run_bck(false, ba); });
+    if (!checking) return;
+    // the check's outcome (its copies arrive while the backprojection runs)
+    HIP_CHECK(hipEventSynchronize(I.chkDone));
+    const int nchk = Vs >= 2 ? 2 : 1;
+    float e = 0.0f, es = 0.0f;   // the larger of the views; a NaN stays
+    for (int j = 0; j < nchk; ++j) {
+        const double *q = I.chkHost->q + 3 * j;
+        const float t = q[2] > 0 ? (float)std::sqrt(q[0] / q[2]) : 0.0f;
+        const float ts = q[1] > 0 ? (float)std::sqrt(q[0] / q[1]) : 0.0f;
+        if (t > e || t != t) e = t;
+        if (ts > es || ts != ts) es = ts;
+    }
+    I.cst.checks++;
+    I.cst.lastNrmse = e;
+    I.cst.lastScatterNrmse = es;
+    if (e > I.cst.worstNrmse || e != e) I.cst.worstNrmse = e;
+    if (es > I.cst.worstScatterNrmse || es != es) I.cst.worstScatterNrmse = es;
+    if (I.chkHost->fail) {
+        // back to the scatter model, from this subset on
+        I.cst.active = false;
+        runViews(s, first, Vs, srf, accumulate, update, keepUpdate);
+    }
 }
 
 // Forward projection of the current estimate for every view (IrlGenprj).

@@ -52,6 +52,9 @@ public:
     }
     // off: the line passes everywhere and no fused source
     void setFused(bool on) { fused_ = on; }
+    // the stream later calls launch on (the work buffers are shared: calls
+    // on different streams must not overlap)
+    void setStream(hipStream_t s) { stream_ = s; }
 
     const EsseModel &model() const { return m_; }
     int terms() const { return m_.nterms; }
