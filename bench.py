@@ -253,8 +253,10 @@ def stage_work(w, rec, stage, subsets, bf16):
     P = p.NumPixels * p.NumSlices
     mean = lambda per: float(np.mean([per[s] for s in subsets]))
     if stage == "cnn_att":
-        # afac of the subset's views, the two source terms once, the planes
-        return "hbm", 4 * Vs * M + 2 * 4 * M + mean(att_write_bytes(w, rec, bf16)), "B"
+        # afac of the subset's views, the two source terms once (FP32, or
+        # with the BF16 DRF one bf16 pair per voxel), the planes
+        src = 4 * M if bf16 else 2 * 4 * M
+        return "hbm", 4 * Vs * M + src + mean(att_write_bytes(w, rec, bf16)), "B"
     if stage == "att":
         return "hbm", 4 * Vs * M + 4 * M + mean(att_write_bytes(w, rec, bf16)), "B"
     if stage == "rot_x":
@@ -292,21 +294,27 @@ def roofline(w, rec, stages, steps, first, precision, traffic=None):
     out = {"kernel": stage_kernel(dom), "stage": dom, "bound": bound, "achieved": ach,
            "peak": peak, "unit": u, "frac": ach / peak, "avg_launch_ms": ms / nl,
            "algorithmic_per_launch": amount, "algorithmic_unit": unit,
-           "traffic": traffic_for(traffic, dom)}
+           "traffic": traffic_for(traffic, dom, rec.tiles())}
     return out
 
 
-def traffic_for(traffic, stage):
+def traffic_for(traffic, stage, tiles=None):
     """Measured HBM bytes per dispatch of a stage's kernel from a traffic
     summary (kernel names as rocprofv3 prints them: k_drf_bf16<KY, NZT, BCK>,
-    k_bck<NORM, SHEAR>, k_att_src<SHEAR>), or None."""
+    k_bck<NORM, SHEAR>, k_att_src<SHEAR, PK>), or None.  For the DRF kernels
+    the instantiation of the tile in use ('tiles' = (projector,
+    backprojector)); the others in the summary are the autotuner's timings."""
     if traffic is None:
         return None
     kern = stage_kernel(stage)
     names = [k for k in traffic["kernels"] if k == kern or k.startswith(kern + "<")]
     if kern in ("k_drf_bf16", "k_drf"):
-        want = "true" if stage.startswith("drf_bck") else "false"
+        bck = stage.startswith("drf_bck")
+        want = "true" if bck else "false"
         names = [k for k in names if k.split("<")[1].split(",")[2].strip(" >") == want]
+        if tiles is not None and len(names) > 1:
+            t = str(tiles[1] if bck else tiles[0])
+            names = [k for k in names if k.split("<")[1].split(",")[1].strip() == t]
     elif kern == "k_bck":
         names = [k for k in names if k.startswith("k_bck<false")]
     if len(names) != 1:
