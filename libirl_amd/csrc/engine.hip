@@ -672,28 +672,36 @@ __global__ __launch_bounds__(64 * CHK_RG) void k_scatter_check(const float *__re
         part3[((long)b * gridDim.x + blockIdx.x) * 3 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// Outcome of a check: the block sums of k_scatter_check added in block
-// order into acc[3j..3j+2] for each of the nchk views (nb blocks per view),
+// Outcome of a check: the block sums of k_scatter_check added into
+// acc[3j..3j+2] for each of the nchk views (nb blocks per view; thread t
+// adds blocks t, t + 256, ... in order, then a fixed tree over the threads),
 // and fail = 1 when the larger relative L2 error of the learned projection
 // (sqrt of acc[3j] / acc[3j+2]) exceeds maxNrmse or is not a number.
-__global__ __launch_bounds__(64) void k_check_outcome(const double *__restrict__ part3, int nb,
-                                                      int nchk, float maxNrmse,
-                                                      double *__restrict__ acc,
-                                                      int *__restrict__ fail)
+__global__ __launch_bounds__(256) void k_check_outcome(const double *__restrict__ part3, int nb,
+                                                       int nchk, float maxNrmse,
+                                                       double *__restrict__ acc,
+                                                       int *__restrict__ fail)
 {
+    __shared__ double red[6][256];
     const int t = threadIdx.x;
-    if (t < 3 * nchk) {
-        const int j = t / 3, k = t % 3;
+    for (int q = 0; q < 3 * nchk; ++q) {
+        const int j = q / 3, k = q % 3;
         double s = 0.0;
-        for (int i = 0; i < nb; ++i) s += part3[((long)j * nb + i) * 3 + k];
-        acc[t] = s;
+        for (int i = t; i < nb; i += 256) s += part3[((long)j * nb + i) * 3 + k];
+        red[q][t] = s;
     }
     __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w)
+            for (int q = 0; q < 3 * nchk; ++q) red[q][t] += red[q][t + w];
+        __syncthreads();
+    }
+    if (t < 3 * nchk) acc[t] = red[t][0];
     if (t != 0) return;
     float e = 0.0f;
     for (int j = 0; j < nchk; ++j) {
-        const double *q = acc + 3 * j;
-        const float r = q[2] > 0 ? (float)sqrt(q[0] / q[2]) : 0.0f;
+        const double q0 = red[3 * j][0], q2 = red[3 * j + 2][0];
+        const float r = q2 > 0 ? (float)sqrt(q0 / q2) : 0.0f;
         if (r > e || r != r) e = r;
     }
     fail[0] = !(e <= maxNrmse);
@@ -1780,7 +1788,7 @@ struct Recon::Impl {
                                    att2.p, att.p, chkPlanes.p, rotx.p, afacp, cv, g.Nz, g.NRot,
                                    chkPart.p, half ? rowKernel16.p : nullptr,
                                    half ? distSplit.p : nullptr);
-                hipLaunchKernelGGL(k_check_outcome, dim3(1), dim3(64), 0, stream, chkPart.p, nb,
+                hipLaunchKernelGGL(k_check_outcome, dim3(1), dim3(256), 0, stream, chkPart.p, nb,
                                    nchk, maxNrmse, chk.p, chkFail.p);
             });
             HIP_CHECK(hipMemcpyAsync(chkHost->q, chk.p, 3 * nchk * sizeof(double),
