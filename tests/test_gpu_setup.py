@@ -158,12 +158,14 @@ def test_resume_at_start_iteration_matches_oracle(scatter):
         assert d2 < 1e-6        # without scatter the iteration number plays no part
 
 
-@pytest.mark.parametrize("graphs", [False, True])
-def test_learned_scatter_falls_back_to_esse(graphs):
+@pytest.mark.parametrize("graphs,precision", [(False, "fp32"), (True, "fp32"), (True, "bf16")])
+def test_learned_scatter_falls_back_to_esse(graphs, precision):
     # a network that is far from ESSE (random weights, large output scale):
     # the first check exceeds 1 % and the reconstruction continues on ESSE,
-    # with the same floats as a reconstruction that never had a network
-    print("learned-scatter fallback, graphs=%s" % graphs, flush=True)
+    # with the same floats as a reconstruction that never had a network.
+    # The check runs beside the learned projection and gates the update on
+    # the device; with the BF16 DRF it reads the projector's bf16 rows.
+    print("learned-scatter fallback, graphs=%s precision=%s" % (graphs, precision), flush=True)
     import libirl_amd as L
     pb = Problem(n=32, nz=16, nviews=8, per=2, iters=3, half=3, noise=False)
     krnl, _, _ = _krnl(pb.pix)
@@ -172,6 +174,7 @@ def test_learned_scatter_falls_back_to_esse(graphs):
     net = L.ScatterCnn.create(kappa=50.0, seed=11)
     L.set_adv_options(0, 0)
     prev = L.set_graphs(graphs)
+    prev_p = L.set_precision(precision)
     try:
         plain = _irlosem(pb, p, krnl=krnl)
         L.set_scatter_cnn(net, check_every=2, max_nrmse=0.01)
@@ -180,6 +183,7 @@ def test_learned_scatter_falls_back_to_esse(graphs):
     finally:
         L.set_scatter_cnn(None, check_every=0, max_nrmse=0.01)
         L.set_graphs(prev)
+        L.set_precision(prev_p)
     print("status", st)
     assert st["checks"] == 1 and not st["active"] and st["used"] == 0
     assert st["last_nrmse"] > 0.01
