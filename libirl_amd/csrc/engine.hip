@@ -431,7 +431,7 @@ static dim3 pix_grid(int Nz, int width, int rows, int B)
 // together.  The rows' circle extents are read once per wave into lanes
 // 0..ATT_ROWS-1 and taken from there as scalars.
 // ---------------------------------------------------------------------------
-constexpr int ATT_ROWS = 8;
+constexpr int ATT_ROWS = 16;
 
 // Per row Y0 + lane (lanes < ATT_ROWS): circle extents [head, end] and
 // whether the row's projector input is stored in bf16 (att_store's rule);
